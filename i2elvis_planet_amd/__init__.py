@@ -1,0 +1,184 @@
+"""MI355X-native I2ELVIS marker-in-cell time step.
+
+Python mirror of the reference's step interface (headmart.c:321-366, driven by
+i2mart.c:60-153): a `Context` owns one device copy of the model state and
+exposes maxvelstep / viterate / titerate / movemark / ronurecalc / step with
+the reference's semantics, calling libi2e.so through its C-ABI
+(include/i2e.h).  There is no CPU fallback: without the built library or a
+visible GPU every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import Params, SolverOpts, Stats
+from .state import HostState
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libi2e.so")
+
+_EXPORTS = [
+    "i2e_create", "i2e_destroy", "i2e_set_solver", "i2e_default_solver", "i2e_upload", "i2e_download",
+    "i2e_maxvelstep", "i2e_viterate", "i2e_titerate", "i2e_movemark", "i2e_ronurecalc", "i2e_step",
+    "i2e_get_scalars", "i2e_set_scalars", "i2e_get_stats", "i2e_sync", "i2e_last_error",
+    "i2e_stokes_apply_dev", "i2e_gravity_apply_dev", "i2e_heat_apply_dev", "i2e_device_ptr", "i2e_time_kernels",
+]
+
+_lib = None
+
+
+class I2EError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """dlopen libi2e.so and declare its signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise I2EError("libi2e.so not built: run `python -m i2elvis_planet_amd.build`")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    P = C.POINTER(Params)
+    S = C.POINTER(abi.HostStateC)
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "i2e_create": ([P, C.c_int64, C.POINTER(vp)], C.c_int),
+        "i2e_destroy": ([vp], C.c_int),
+        "i2e_set_solver": ([vp, C.POINTER(SolverOpts)], C.c_int),
+        "i2e_default_solver": ([C.POINTER(SolverOpts)], C.c_int),
+        "i2e_upload": ([vp, S], C.c_int),
+        "i2e_download": ([vp, S], C.c_int),
+        "i2e_maxvelstep": ([vp, dp], C.c_int),
+        "i2e_viterate": ([vp, C.c_int], C.c_int),
+        "i2e_titerate": ([vp, C.c_int], C.c_int),
+        "i2e_movemark": ([vp], C.c_int),
+        "i2e_ronurecalc": ([vp], C.c_int),
+        "i2e_step": ([vp], C.c_int),
+        "i2e_get_scalars": ([vp, dp, dp, dp], C.c_int),
+        "i2e_set_scalars": ([vp, C.c_double, C.c_double], C.c_int),
+        "i2e_get_stats": ([vp, C.POINTER(Stats)], C.c_int),
+        "i2e_sync": ([vp], C.c_int),
+        "i2e_last_error": ([], C.c_char_p),
+        "i2e_stokes_apply_dev": ([vp, vp, vp], C.c_int),
+        "i2e_gravity_apply_dev": ([vp, vp, vp], C.c_int),
+        "i2e_heat_apply_dev": ([vp, vp, vp], C.c_int),
+        "i2e_device_ptr": ([vp, C.c_char_p], vp),
+        "i2e_time_kernels": ([vp, C.c_int, dp, dp, dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _chk(st, what):
+    if st != 0:
+        msg = _lib.i2e_last_error().decode(errors="replace")
+        raise I2EError("%s failed (status %d): %s" % (what, st, msg))
+
+
+def default_solver():
+    o = SolverOpts()
+    _chk(load_library().i2e_default_solver(C.byref(o)), "i2e_default_solver")
+    return o
+
+
+class Context:
+    """Device-resident model state plus the reference's step functions."""
+
+    def __init__(self, params, hs, max_markers=None, solver=None):
+        L = load_library()
+        self.p = params
+        self._h = C.c_void_p()
+        M = hs.marknum if max_markers is None else int(max_markers)
+        _chk(L.i2e_create(C.byref(params), M, C.byref(self._h)), "i2e_create")
+        if solver is not None:
+            self.set_solver(solver)
+        self.upload(hs)
+
+    def close(self):
+        if self._h:
+            _lib.i2e_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_solver(self, o):
+        _chk(_lib.i2e_set_solver(self._h, C.byref(o)), "i2e_set_solver")
+
+    def upload(self, hs):
+        self._cs = hs.cstruct()
+        _chk(_lib.i2e_upload(self._h, C.byref(self._cs)), "i2e_upload")
+
+    def download(self, hs):
+        cs = hs.cstruct()
+        _chk(_lib.i2e_download(self._h, C.byref(cs)), "i2e_download")
+        hs.pull_scalars(cs)
+        return hs
+
+    # the reference's step functions ------------------------------------------------
+    def maxvelstep(self):
+        ts = C.c_double(0)
+        _chk(_lib.i2e_maxvelstep(self._h, C.byref(ts)), "maxvelstep")
+        return ts.value
+
+    def viterate(self, n0=0):
+        _chk(_lib.i2e_viterate(self._h, n0), "viterate")
+
+    def titerate(self, n0=0):
+        _chk(_lib.i2e_titerate(self._h, n0), "titerate")
+
+    def movemark(self):
+        _chk(_lib.i2e_movemark(self._h), "movemark")
+
+    def ronurecalc(self):
+        _chk(_lib.i2e_ronurecalc(self._h), "ronurecalc")
+
+    def step(self):
+        _chk(_lib.i2e_step(self._h), "step")
+
+    def sync(self):
+        _chk(_lib.i2e_sync(self._h), "sync")
+
+    # scalars / diagnostics -----------------------------------------------------------
+    def scalars(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _chk(_lib.i2e_get_scalars(self._h, C.byref(a), C.byref(b), C.byref(c)), "get_scalars")
+        return a.value, b.value, c.value
+
+    def set_scalars(self, timestep, timesum):
+        _chk(_lib.i2e_set_scalars(self._h, timestep, timesum), "set_scalars")
+
+    def stats(self):
+        s = Stats()
+        _chk(_lib.i2e_get_stats(self._h, C.byref(s)), "get_stats")
+        return s
+
+    def device_ptr(self, name):
+        p = _lib.i2e_device_ptr(self._h, name.encode())
+        if not p:
+            raise I2EError("unknown device array %r" % name)
+        return p
+
+    def apply(self, which, x_dev, y_dev):
+        f = {"stokes": _lib.i2e_stokes_apply_dev, "gravity": _lib.i2e_gravity_apply_dev,
+             "heat": _lib.i2e_heat_apply_dev}[which]
+        _chk(f(self._h, C.c_void_p(x_dev), C.c_void_p(y_dev)), which + "_apply")
+
+    def time_kernels(self, reps=5):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _chk(_lib.i2e_time_kernels(self._h, reps, C.byref(a), C.byref(b), C.byref(c)), "time_kernels")
+        return a.value, b.value, c.value
+
+
+__all__ = ["Context", "HostState", "Params", "SolverOpts", "Stats", "default_solver", "load_library", "I2EError"]

@@ -1,0 +1,95 @@
+"""ctypes mirrors of the structs declared in include/i2e.h.
+
+The layouts must match the C header field for field; tests/test_abi.py checks
+the sizes against the compiled library.
+"""
+import ctypes as C
+
+MAXTYPES = 64
+
+I2E_OK = 0
+I2E_ERR_ARG = 1
+I2E_ERR_CAPACITY = 2
+I2E_ERR_HIP = 3
+I2E_ERR_NOCONV = 4
+I2E_ERR_UNSUPPORTED = 5
+I2E_ERR_NODEVICE = 6
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_i8p = C.POINTER(C.c_int8)
+_i64p = C.POINTER(C.c_int64)
+
+
+class Rock(C.Structure):
+    """One row of the reference's ROCKS table (init.t3c, headmart.c:127-131)."""
+    _fields_ = [(n, C.c_double) for n in
+                ("nu0", "dh", "dv", "ss", "mm", "n0", "n1", "ro", "aa", "bb", "cp", "kt", "kf", "kp")]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("xnumx", C.c_int64), ("ynumy", C.c_int64),
+        ("mnumx", C.c_int64), ("mnumy", C.c_int64),
+        ("xsize", C.c_double), ("ysize", C.c_double),
+        ("maxxystep", C.c_double), ("maxtkstep", C.c_double), ("maxtmstep", C.c_double),
+        ("nubeg", C.c_double), ("nuend", C.c_double), ("nucontr", C.c_double), ("strmin", C.c_double),
+        ("heatdif", C.c_double), ("frictyn", C.c_double), ("adiabyn", C.c_double),
+        ("heatmod", C.c_int32), ("densimod", C.c_int32),
+        ("outgrid", C.c_int32), ("markmod", C.c_int32),
+        ("gykoef", C.c_double), ("tmp_ambient", C.c_double),
+        ("al2627_init", C.c_double), ("fe6056_init", C.c_double),
+        ("h0_scale", C.c_double), ("longlived", C.c_double),
+        ("por_init", C.c_double), ("k_cutoff", C.c_double),
+        ("radmax_frac", C.c_double), ("eta_outer", C.c_double),
+        ("melt_ts", C.c_double), ("melt_tl", C.c_double), ("melt_switch", C.c_double),
+        ("melt_eta", C.c_double), ("melt_hlat", C.c_double),
+        ("melt_on", C.c_int32), ("porosity_on", C.c_int32),
+        ("ntypes", C.c_int32), ("pad0", C.c_int32),
+        ("rock", Rock * MAXTYPES),
+        ("is_silicate", C.c_uint8 * MAXTYPES),
+        ("is_iron", C.c_uint8 * MAXTYPES),
+        ("porous", C.c_uint8 * MAXTYPES),
+        ("meltable", C.c_uint8 * MAXTYPES),
+    ]
+
+
+class HostStateC(C.Structure):
+    _fields_ = [
+        ("gx", _dp), ("gy", _dp),
+        ("vx", _dp), ("vy", _dp), ("pr", _dp), ("gp", _dp), ("ga", _dp),
+        ("exx", _dp), ("eyy", _dp), ("exy", _dp), ("sxx", _dp), ("syy", _dp), ("sxy", _dp),
+        ("nu", _dp), ("nd", _dp), ("ro", _dp), ("tk", _dp), ("cp", _dp), ("kt", _dp), ("ht", _dp), ("et", _dp),
+        ("marknum", C.c_int64),
+        ("markx", _fp), ("marky", _fp), ("markk", _fp), ("markv", _fp), ("markp", _fp),
+        ("markpor", _fp), ("marktmax", _fp),
+        ("markt", _i8p),
+        ("bondm", _i64p), ("bondn", _i64p), ("bondv", _dp), ("bondnum", C.c_int64),
+        ("timestep", C.c_double), ("timesum", C.c_double), ("gxkoef", C.c_double),
+    ]
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("restart", C.c_int32), ("maxit", C.c_int32), ("nsmooth", C.c_int32),
+        ("stokes_rtol", C.c_double), ("heat_rtol", C.c_double), ("grav_rtol", C.c_double),
+        ("schur_scale", C.c_double), ("omega", C.c_double),
+        ("coarse_max", C.c_int32), ("pad1", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("stokes_iters", C.c_int32), ("heat_iters", C.c_int32), ("grav_iters", C.c_int32), ("pad", C.c_int32),
+        ("stokes_relres", C.c_double), ("heat_relres", C.c_double), ("grav_relres", C.c_double),
+        ("numin", C.c_double), ("numax", C.c_double),
+        ("ms_grav", C.c_double), ("ms_stokes", C.c_double), ("ms_heat", C.c_double), ("ms_markers", C.c_double),
+    ]
+
+
+NODE_FIELDS = ("vx", "vy", "pr", "gp", "ga", "exx", "eyy", "exy", "sxx", "syy", "sxy",
+               "nu", "nd", "ro", "tk", "cp", "kt", "ht", "et")
+MARKER_FIELDS_F32 = ("markx", "marky", "markk", "markv", "markp", "markpor", "marktmax")

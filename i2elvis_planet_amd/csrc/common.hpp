@@ -1,0 +1,136 @@
+// common.hpp -- shared device helpers of the I2ELVIS step library (gfx950).
+//
+// Grid geometry, the reference's cell search and two-point Fornberg weights
+// (markmart.c:1051-1088, movemart.c:1269-1319) with the operation order that
+// makes marker arithmetic bit-identical to the serial C code when both sides
+// are compiled with -ffp-contract=off.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/i2e.h"
+
+#define I2E_HD __host__ __device__ __forceinline__
+
+namespace i2e {
+
+void set_error(const char *fmt, ...);
+
+#define HIP_OK(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      ::i2e::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return I2E_ERR_HIP;                                                             \
+    }                                                                                 \
+  } while (0)
+
+#define ST_OK(call)              \
+  do {                           \
+    int s_ = (call);             \
+    if (s_ != I2E_OK) return s_; \
+  } while (0)
+
+// Grid lines of one level.  Node m = m1*Y + m2.
+struct Grid {
+  int X, Y;
+  double xsize, ysize;
+  const double *gx, *gy;  // device
+};
+
+// m1serch / m2serch (markmart.c:1051-1088): bisection over grid lines,
+// clamped to n-2.
+I2E_HD int serch(const double *g, int n, double x) {
+  int m, m0 = 0, m1 = n - 1;
+  do {
+    m = (m0 + m1) / 2;
+    if (g[m] > x)
+      m1 = m;
+    else
+      m0 = m;
+  } while ((m1 - m0) > 1);
+  if (m0 > n - 2) m0 = n - 2;
+  return m0;
+}
+
+// fdweight(n=1, m=0): interpolation weights at xi between x0 and x1.
+I2E_HD void fd2_interp(double x0, double x1, double xi, double &w0, double &w1) {
+  double c5 = x0 - xi, c4 = x1 - xi, c3 = x1 - x0;
+  w1 = -1.0 * c5 * 1.0 / c3;
+  w0 = c4 * 1.0 / c3;
+}
+
+I2E_HD double subline(const double *g, int m, int sh) {
+  return sh < 0 ? (g[m] + g[m - 1]) / 2.0 : (sh > 0 ? (g[m] + g[m + 1]) / 2.0 : g[m]);
+}
+
+// Bilinear stencil (origin i0,j0 + weights) on one staggered sub-grid.
+struct W4 {
+  int i0, j0;
+  double wx0, wx1, wy0, wy1;
+};
+
+I2E_HD void nodewt2(const double *gx, const double *gy, int i0, int j0, double x, double y, int ynx, int yny, W4 &w) {
+  w.i0 = i0;
+  w.j0 = j0;
+  fd2_interp(subline(gx, i0, ynx), subline(gx, i0 + 1, ynx), x, w.wx0, w.wx1);
+  fd2_interp(subline(gy, j0, yny), subline(gy, j0 + 1, yny), y, w.wy0, w.wy1);
+}
+
+// m1 outer, m2 inner, from zero (markmart.c:1415-1422)
+I2E_HD double isum(const W4 &w, int Y, const double *f) {
+  int m = w.i0 * Y + w.j0;
+  double e = 0;
+  e += (w.wx0 * w.wy0) * f[m];
+  e += (w.wx0 * w.wy1) * f[m + 1];
+  e += (w.wx1 * w.wy0) * f[m + Y];
+  e += (w.wx1 * w.wy1) * f[m + Y + 1];
+  return e;
+}
+
+I2E_HD double isum2(const W4 &w, int Y, const double *f, const double *h) {
+  int m = w.i0 * Y + w.j0;
+  double e = 0;
+  e += (w.wx0 * w.wy0) * f[m] * h[m];
+  e += (w.wx0 * w.wy1) * f[m + 1] * h[m + 1];
+  e += (w.wx1 * w.wy0) * f[m + Y] * h[m + Y];
+  e += (w.wx1 * w.wy1) * f[m + Y + 1] * h[m + Y + 1];
+  return e;
+}
+
+// stencil origins of the staggered sub-grids (allinterv/allintere,
+// markmart.c:1378-1543; intermod = 0)
+I2E_HD int vx_j0(const double *gy, int Y, int m20, double y) {
+  int j = m20;
+  if (y < (gy[m20] + gy[m20 + 1]) / 2.0) j -= 1;
+  if (j < 0) j = 0;
+  if (j > Y - 3) j = Y - 3;
+  return j;
+}
+I2E_HD int vy_i0(const double *gx, int X, int m10, double x) {
+  int i = m10;
+  if (x < (gx[m10] + gx[m10 + 1]) / 2.0) i -= 1;
+  if (i < 0) i = 0;
+  if (i > X - 3) i = X - 3;
+  return i;
+}
+I2E_HD int bn_i0(int m10, int n) {
+  int i = m10;
+  if (i < 1) i = 1;
+  if (i > n - 3) i = n - 3;
+  return i;
+}
+I2E_HD int ce_i0(const double *g, int m10, double x, int n) {
+  int i = m10;
+  if (x > (g[m10] + g[m10 + 1]) / 2.0) i += 1;
+  if (i < 1) i = 1;
+  if (i > n - 2) i = n - 2;
+  return i;
+}
+
+I2E_HD double dabs(double a) { return a >= 0 ? a : -a; }
+
+inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace i2e

@@ -1,0 +1,858 @@
+// ctx.hip -- the C-ABI of include/i2e.h: context, host<->device transfer and
+// the reference's step functions on device-resident state.
+//
+// Call order and time-step coupling follow main() in i2mart.c:60-153:
+// maxvelstep -> viterate (gravity, Stokes) -> maxvelstep -> titerate (may
+// shrink the global timestep, heatmart.c:387) -> movemark -> ronurecalc
+// (+ tkrecalc) -> timesum += timestep.
+#include <stdarg.h>
+#include <string.h>
+
+#include <chrono>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "krylov.hpp"
+
+namespace i2e {
+
+static thread_local char g_err[512] = "";
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+static inline dim3 nb2(int X, int Y) { return dim3((Y + 255) / 256, X); }
+
+// ---------------------------------------------------------------- small kernels
+__global__ void k_copy(int n, const double *a, double *b) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) b[k] = a[k];
+}
+__global__ void k_fill(int n, double v, double *a) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) a[k] = v;
+}
+
+// Stokes preconditioner pieces (lower block-triangular):
+//   z_v = MG(r_v);  z_p = -sc * eta_cell / mu * (r_p - mu * div z_v)  on continuity rows,
+//   z_p = r_p / mu + sum k z_p(nbr) on P boundary rows, r_p / mu on ghost rows
+template <class T>
+__global__ void k_split_v(int X, int Y, const double *r, T *fx, T *fy) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= Y) return;
+  const int m = i * Y + j;
+  fx[m] = T(r[3 * m + 1]);
+  fy[m] = T(r[3 * m + 2]);
+}
+template <class T>
+__global__ void k_schur(SOp<double> o, BcTab bc, double sc, const double *r, const T *ux, const T *uy, double *z) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int Y = o.Y;
+  if (j >= Y) return;
+  const int m = i * Y + j;
+  z[3 * m + 1] = (double)ux[m];
+  z[3 * m + 2] = (double)uy[m];
+  double zp;
+  if (i && j && !bc.bondm[3 * m]) {
+    // divergence of z_v in double from the just-written values
+    double dv = ((double)ux[i * Y + j - 1] - (double)ux[(i - 1) * Y + j - 1]) / (o.gx[i] - o.gx[i - 1]) +
+                ((double)uy[(i - 1) * Y + j] - (double)uy[(i - 1) * Y + j - 1]) / (o.gy[j] - o.gy[j - 1]);
+    zp = -sc * o.nd[m] / o.mu * (r[3 * m] - o.mu * dv);
+  } else
+    zp = r[3 * m] / o.mu;
+  z[3 * m] = zp;
+}
+__global__ void k_schur_bc(SOp<double> o, BcTab bc, const double *r, double *z) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int Y = o.Y;
+  if (j >= Y || !(i && j)) return;
+  const int m = i * Y + j;
+  const int b = bc.bondm[3 * m];
+  if (!b) return;
+  double v = r[3 * m] / o.mu;
+  for (int n = 0; n < 3; n++) {
+    int nb = bc.bondn[3 * b + n];
+    if (nb) v += bc.bondv[4 * b + n + 1] * z[nb - 1];
+  }
+  z[3 * m] = v;
+}
+
+// heat preconditioner input/output: f = rho*cp*r on equation rows, r on BC rows
+template <class T>
+__global__ void k_heat_pre(int N, const double *r, const double *ro, const double *cp, const int8_t *mask, T *f) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < N) f[m] = T(mask[m] ? r[m] : r[m] * ro[m] * cp[m]);
+}
+template <class T>
+__global__ void k_to_double(int N, const T *u, double *z) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < N) z[m] = (double)u[m];
+}
+__global__ void k_heat_fields(int N, const double *ro, const double *cp, double dt, const int *bondm, double *c,
+                              int8_t *mask, double *dm) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  c[m] = ro[m] * cp[m] / dt;
+  mask[m] = bondm[3 * N + m] ? 1 : 0;
+  dm[m] = 1.0;
+}
+
+// gravity: scalar grid = cells (X-1)x(Y-1); unmasked rows use -Laplacian
+struct GravPre {
+  int X, Y;
+  double mu;
+};
+template <class T>
+__global__ void k_grav_pre(GravPre g, const double *r, const int8_t *mask, T *f) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= g.Y - 1) return;
+  const int q = i * (g.Y - 1) + j, m = (i + 1) * g.Y + (j + 1);
+  f[q] = T(mask[q] ? r[m] : -r[m]);
+}
+template <class T>
+__global__ void k_grav_post(GravPre g, const double *r, const T *u, double *z) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= g.Y) return;
+  const int m = i * g.Y + j;
+  z[m] = (i && j) ? (double)u[(i - 1) * (g.Y - 1) + (j - 1)] : r[m] / g.mu;
+}
+__global__ void k_grav_fields(Grid g, double R, double mu, double *k, double *c, int8_t *mask, double *dm) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;  // cell-grid indices
+  const int X = g.X, Y = g.Y;
+  if (j >= Y - 1) return;
+  const int q = i * (Y - 1) + j, m1 = i + 1, m2 = j + 1;
+  k[q] = 1.0;
+  c[q] = 0.0;
+  if (m1 == 1 || m1 == X - 1 || m2 == 1 || m2 == Y - 1) {
+    mask[q] = 1;
+    dm[q] = mu;
+    return;
+  }
+  double dx = (g.gx[m1 - 1] + g.gx[m1] - g.xsize) / 2.0, dy = (g.gy[m2 - 1] + g.gy[m2] - g.ysize) / 2.0;
+  double d = pow(dx * dx + dy * dy, 0.5);
+  if (d > R) {
+    const double *gx = g.gx, *gy = g.gy;
+    double dx02 = (gx[m1] - gx[m1 - 2]) / 2.0, dx24 = (gx[m1 + 1] - gx[m1 - 1]) / 2.0;
+    double dy12 = (gy[m2] - gy[m2 - 2]) / 2.0, dy23 = (gy[m2 + 1] - gy[m2 - 1]) / 2.0;
+    mask[q] = 1;
+    dm[q] = -2.0 / dx02 / (dx02 + dx24) - 2.0 / dx24 / (dx02 + dx24) - 2.0 / dy12 / (dy12 + dy23) -
+            2.0 / dy23 / (dy12 + dy23) + mu * 1e+18;
+  } else {
+    mask[q] = 0;
+    dm[q] = 1.0;
+  }
+}
+
+}  // namespace i2e
+
+using namespace i2e;
+
+struct i2e_ctx {
+  i2e_params P;
+  i2e_params *dP = nullptr;
+  int X = 0, Y = 0, N = 0;
+  int64_t Mmax = 0, M = 0;
+  hipStream_t s = nullptr;
+  DBuf<double> gx, gy, nodes, acc, work, sol1, tk2, red, xs, bs;
+  DBuf<float> mfl;
+  DBuf<int8_t> mt;
+  DBuf<int> bondm, bondn;
+  DBuf<double> bondv;
+  NodeF F;
+  Markers mk;
+  Acc A;
+  BcTab bc;
+  int need_eii = 0;
+  bool setup_done = false;
+  std::vector<GeoLevel> geo;
+  VelMG<float> v32;
+  VelMG<double> v64;
+  ScalMG<float> h32, g32;
+  ScalMG<double> h64, g64;
+  DBuf<double> hk, hc, hdm, gk, gc, gdm;
+  DBuf<int8_t> hmask, gmask;
+  Fgmres fs, fh, fgv;
+  i2e_solver_opts o;
+  i2e_stats st;
+  double timestep = 0, timesum = 0, gxkoef = 1.0;
+  HBuf<double> pin;
+
+  Grid grid() const { return Grid{X, Y, P.xsize, P.ysize, gx.p, gy.p}; }
+  SOp<double> sop() const { return SOp<double>{X, Y, gx.p, gy.p, F.nu, F.nd, F.ro, F.gp, timestep, P.nubeg * P.strmin}; }
+};
+
+extern "C" {
+
+const char *i2e_last_error(void) { return g_err; }
+
+int i2e_default_solver(i2e_solver_opts *o) {
+  if (!o) return I2E_ERR_ARG;
+  memset(o, 0, sizeof *o);
+  o->precision = I2E_PREC_MIXED;
+  o->restart = 60;
+  o->maxit = 600;
+  o->nsmooth = 2;
+  o->stokes_rtol = 1e-8;
+  o->heat_rtol = 1e-10;
+  o->grav_rtol = 1e-10;
+  o->schur_scale = 2.0;
+  o->omega = 1.0;
+  o->coarse_max = 8;
+  return I2E_OK;
+}
+
+int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
+  if (!p || !out || max_markers < 0) return I2E_ERR_ARG;
+  if (p->xnumx < 9 || p->ynumy < 9 || p->xnumx > 16385 || p->ynumy > 16385) {
+    set_error("grid %lld x %lld outside [9, 16385]", (long long)p->xnumx, (long long)p->ynumy);
+    return I2E_ERR_CAPACITY;
+  }
+  if ((int64_t)p->xnumx * p->ynumy > (int64_t)1 << 28) return I2E_ERR_CAPACITY;
+  if (p->densimod > 1 || p->adiabyn > 0 || p->markmod != 2) {
+    set_error("densimod>1, adiabyn>0 or markmod!=2 not supported");
+    return I2E_ERR_UNSUPPORTED;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no HIP device visible");
+    return I2E_ERR_NODEVICE;
+  }
+  i2e_ctx *c = new i2e_ctx();
+  c->P = *p;
+  c->X = (int)p->xnumx;
+  c->Y = (int)p->ynumy;
+  c->N = c->X * c->Y;
+  c->Mmax = max_markers;
+  i2e_default_solver(&c->o);
+  memset(&c->st, 0, sizeof c->st);
+  for (int t = 0; t < I2E_MAXTYPES; t++)
+    if (p->rock[t].ss != 0 && p->rock[t].mm != 1.0) c->need_eii = 1;
+  int st = I2E_OK;
+  auto fail = [&](int s_) {
+    i2e_destroy(c);
+    return s_;
+  };
+  if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return fail(I2E_ERR_HIP);
+  if (hipMalloc((void **)&c->dP, sizeof(i2e_params)) != hipSuccess) return fail(I2E_ERR_HIP);
+  if (hipMemcpy(c->dP, p, sizeof(i2e_params), hipMemcpyHostToDevice) != hipSuccess) return fail(I2E_ERR_HIP);
+  const int N = c->N;
+  if ((st = c->gx.alloc(c->X)) || (st = c->gy.alloc(c->Y)) || (st = c->nodes.alloc((size_t)19 * N)) ||
+      (st = c->acc.alloc((size_t)12 * N)) || (st = c->sol1.alloc(N)) || (st = c->tk2.alloc(N)) ||
+      (st = c->red.alloc(4096)) || (st = c->xs.alloc((size_t)3 * N)) || (st = c->bs.alloc((size_t)3 * N)) ||
+      (st = c->work.alloc((size_t)3 * N)) || (st = c->bondm.alloc((size_t)4 * N)) || (st = c->pin.alloc(4096)) ||
+      (st = c->hk.alloc(N)) || (st = c->hc.alloc(N)) || (st = c->hdm.alloc(N)) || (st = c->hmask.alloc(N)) ||
+      (st = c->gk.alloc(N)) || (st = c->gc.alloc(N)) || (st = c->gdm.alloc(N)) || (st = c->gmask.alloc(N)))
+    return fail(st);
+  if (max_markers > 0) {
+    if ((st = c->mfl.alloc((size_t)7 * max_markers)) || (st = c->mt.alloc(max_markers))) return fail(st);
+  }
+  double *n0 = c->nodes.p;
+  double **f[19] = {&c->F.vx, &c->F.vy, &c->F.pr, &c->F.gp, &c->F.ga, &c->F.exx, &c->F.eyy,
+                    &c->F.exy, &c->F.sxx, &c->F.syy, &c->F.sxy, &c->F.nu, &c->F.nd, &c->F.ro,
+                    &c->F.tk, &c->F.cp, &c->F.kt, &c->F.ht, &c->F.et};
+  for (int k = 0; k < 19; k++) *f[k] = n0 + (size_t)k * N;
+  float *mf = c->mfl.p;
+  size_t MM = (size_t)max_markers;
+  c->mk = Markers{mf, mf + MM, mf + 2 * MM, mf + 3 * MM, mf + 4 * MM, mf + 5 * MM, mf + 6 * MM, c->mt.p};
+  double *a0 = c->acc.p;
+  c->A = Acc{a0, a0 + N, a0 + 2 * (size_t)N, a0 + 3 * (size_t)N, a0 + 4 * (size_t)N, a0 + 5 * (size_t)N,
+             a0 + 6 * (size_t)N, a0 + 7 * (size_t)N, a0 + 8 * (size_t)N, a0 + 9 * (size_t)N, a0 + 10 * (size_t)N,
+             a0 + 11 * (size_t)N};
+  HIP_OK(hipMemset(c->nodes.p, 0, sizeof(double) * 19 * N));
+  *out = c;
+  return I2E_OK;
+}
+
+int i2e_destroy(i2e_ctx *c) {
+  if (!c) return I2E_OK;
+  if (c->s) hipStreamSynchronize(c->s);
+  c->v32.release();
+  c->v64.release();
+  c->h32.release();
+  c->g32.release();
+  c->h64.release();
+  c->g64.release();
+  c->fs.release();
+  c->fh.release();
+  c->fgv.release();
+  for (auto &g : c->geo) {
+    g.gx.release();
+    g.gy.release();
+  }
+  DBuf<double> *d[] = {&c->gx, &c->gy, &c->nodes, &c->acc, &c->work, &c->sol1, &c->tk2, &c->red, &c->xs,
+                       &c->bs, &c->bondv, &c->hk, &c->hc, &c->hdm, &c->gk, &c->gc, &c->gdm};
+  for (auto *b : d) b->release();
+  c->mfl.release();
+  c->mt.release();
+  c->bondm.release();
+  c->bondn.release();
+  c->hmask.release();
+  c->gmask.release();
+  if (c->pin.p) hipHostFree(c->pin.p);
+  if (c->dP) hipFree(c->dP);
+  if (c->s) hipStreamDestroy(c->s);
+  delete c;
+  return I2E_OK;
+}
+
+int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
+  if (!c || !o) return I2E_ERR_ARG;
+  if (o->restart < 1 || o->restart > 200 || o->maxit < 1 || o->nsmooth < 0 || o->coarse_max < 2) return I2E_ERR_ARG;
+  bool regeo = c->setup_done && o->coarse_max != c->o.coarse_max;
+  c->o = *o;
+  if (regeo) c->setup_done = false;
+  return I2E_OK;
+}
+
+int i2e_get_stats(i2e_ctx *c, i2e_stats *st) {
+  if (!c || !st) return I2E_ERR_ARG;
+  *st = c->st;
+  return I2E_OK;
+}
+
+int i2e_sync(i2e_ctx *c) {
+  if (!c) return I2E_ERR_ARG;
+  HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_get_scalars(i2e_ctx *c, double *timestep, double *timesum, double *gxkoef) {
+  if (!c) return I2E_ERR_ARG;
+  if (timestep) *timestep = c->timestep;
+  if (timesum) *timesum = c->timesum;
+  if (gxkoef) *gxkoef = c->gxkoef;
+  return I2E_OK;
+}
+
+int i2e_set_scalars(i2e_ctx *c, double timestep, double timesum) {
+  if (!c) return I2E_ERR_ARG;
+  c->timestep = timestep;
+  c->timesum = timesum;
+  return I2E_OK;
+}
+
+void *i2e_device_ptr(i2e_ctx *c, const char *name) {
+  if (!c || !name) return nullptr;
+  static const char *nn[19] = {"vx", "vy", "pr", "gp", "ga", "exx", "eyy", "exy", "sxx", "syy",
+                               "sxy", "nu", "nd", "ro", "tk", "cp", "kt", "ht", "et"};
+  for (int k = 0; k < 19; k++)
+    if (!strcmp(name, nn[k])) return c->nodes.p + (size_t)k * c->N;
+  static const char *mn[7] = {"markx", "marky", "markk", "markv", "markp", "markpor", "marktmax"};
+  for (int k = 0; k < 7; k++)
+    if (!strcmp(name, mn[k])) return c->mfl.p + (size_t)k * c->Mmax;
+  if (!strcmp(name, "markt")) return c->mt.p;
+  if (!strcmp(name, "gx")) return c->gx.p;
+  if (!strcmp(name, "gy")) return c->gy.p;
+  return nullptr;
+}
+
+// solver hierarchies; rebuilt when the grid, the BC tables or coarse_max change
+static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
+  const int X = c->X, Y = c->Y, N = c->N;
+  for (auto &g : c->geo) {
+    g.gx.release();
+    g.gy.release();
+  }
+  ST_OK(build_geometry(s->gx, s->gy, X, Y, c->o.coarse_max, c->geo));
+  // velocity-block row kinds of the finest grid from the BC tables
+  std::vector<int8_t> kx(N, 0), ky(N, 0);
+  std::vector<int> nbx(N, -1), nby(N, -1);
+  std::vector<double> kkx(N, 0), kky(N, 0);
+  for (int i = 0; i < X; i++)
+    for (int j = 0; j < Y; j++) {
+      const int m = i * Y + j;
+      for (int comp = 1; comp <= 2; comp++) {
+        int8_t &k = comp == 1 ? kx[m] : ky[m];
+        int &nb = comp == 1 ? nbx[m] : nby[m];
+        double &kk = comp == 1 ? kkx[m] : kky[m];
+        bool ghost = comp == 1 ? (j == Y - 1) : (i == X - 1);
+        if (ghost) {
+          k = 2;
+          continue;
+        }
+        int64_t b = s->bondm[3 * m + comp];
+        if (!b) {
+          bool ok = comp == 1 ? (i >= 1 && i <= X - 2 && j <= Y - 2) : (j >= 1 && j <= Y - 2 && i <= X - 2);
+          if (!ok) {
+            set_error("Stokes row of unknown %d has no BC but sits on the grid edge", 3 * m + comp);
+            return I2E_ERR_UNSUPPORTED;
+          }
+          k = 0;
+          continue;
+        }
+        if (s->bondv[4 * b + 1] > 1e30) {
+          set_error("continuity-type velocity BC (movemart.c:863) not supported");
+          return I2E_ERR_UNSUPPORTED;
+        }
+        k = 1;
+        int cnt = 0;
+        for (int q = 0; q < 3; q++) {
+          int64_t u = s->bondn[3 * b + q];
+          if (!u) continue;
+          cnt++;
+          if ((u - 1) % 3 != comp || u - 1 >= 3 * (int64_t)N) {
+            set_error("velocity BC of unknown %d couples another component", 3 * m + comp);
+            return I2E_ERR_UNSUPPORTED;
+          }
+          nb = (int)((u - 1) / 3);
+          kk = s->bondv[4 * b + q + 1];
+        }
+        if (cnt > 1) {
+          set_error("velocity BC with more than one neighbour not supported by the multigrid");
+          return I2E_ERR_UNSUPPORTED;
+        }
+      }
+    }
+  // heat equation rows need the 3x3 neighbourhood (heaterr, heatmart.c:16-23);
+  // temperature BC rows must reference free nodes (explicit projection)
+  for (int m = 0; m < N; m++) {
+    int64_t b = s->bondm[3 * N + m];
+    if (!b) {
+      int i = m / Y, j = m % Y;
+      if (i == 0 || j == 0 || i == X - 1 || j == Y - 1) {
+        set_error("heat equation row at edge node %d (needs a temperature BC)", m);
+        return I2E_ERR_UNSUPPORTED;
+      }
+      continue;
+    }
+    for (int q = 0; q < 3; q++) {
+      int64_t u = s->bondn[3 * b + q];
+      if (!u) continue;
+      int64_t nm = u - 1 - 3 * (int64_t)N;
+      if (nm < 0 || nm >= N || s->bondm[3 * N + nm]) {
+        set_error("temperature BC of node %d references another BC node", m);
+        return I2E_ERR_UNSUPPORTED;
+      }
+    }
+  }
+  ST_OK(c->v32.setup(c->geo, kx, ky, nbx, nby, kkx, kky));
+  ST_OK(c->v64.setup(c->geo, kx, ky, nbx, nby, kkx, kky));
+  std::vector<double> hx(s->gx, s->gx + X), hy(s->gy, s->gy + Y), cxs, cys;
+  for (int i = 0; i + 1 < X; i++) cxs.push_back((s->gx[i] + s->gx[i + 1]) / 2.0);
+  for (int j = 0; j + 1 < Y; j++) cys.push_back((s->gy[j] + s->gy[j + 1]) / 2.0);
+  ST_OK(c->h32.setup(hx, hy, c->o.coarse_max));
+  ST_OK(c->h64.setup(hx, hy, c->o.coarse_max));
+  ST_OK(c->g32.setup(cxs, cys, c->o.coarse_max));
+  ST_OK(c->g64.setup(cxs, cys, c->o.coarse_max));
+  ST_OK(c->fs.init(3 * N, c->o.restart));
+  ST_OK(c->fh.init(N, c->o.restart));
+  ST_OK(c->fgv.init(N, c->o.restart));
+  // gravity operator fields depend on geometry only
+  Grid g = c->grid();
+  double xstpx = c->P.xsize / (double)(X - 1), ystpy = c->P.ysize / (double)(Y - 1);
+  double mug = 2.0 / xstpx / xstpx + 2 / ystpy / ystpy;
+  hipLaunchKernelGGL(k_grav_fields, nb2(X - 1, Y - 1), dim3(256), 0, c->s, g, c->P.gykoef, mug, c->gk.p, c->gc.p,
+                     c->gmask.p, c->gdm.p);
+  ST_OK(c->g32.refresh(c->gk.p, c->gc.p, c->gmask.p, c->gdm.p, c->s));
+  ST_OK(c->g64.refresh(c->gk.p, c->gc.p, c->gmask.p, c->gdm.p, c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
+  c->setup_done = true;
+  return I2E_OK;
+}
+
+int i2e_upload(i2e_ctx *c, const i2e_host_state *s) {
+  if (!c || !s || !s->gx || !s->gy) return I2E_ERR_ARG;
+  if (s->marknum < 0 || s->marknum > c->Mmax) {
+    set_error("marknum %lld exceeds max_markers %lld", (long long)s->marknum, (long long)c->Mmax);
+    return I2E_ERR_CAPACITY;
+  }
+  const int N = c->N;
+  HIP_OK(hipMemcpy(c->gx.p, s->gx, sizeof(double) * c->X, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->gy.p, s->gy, sizeof(double) * c->Y, hipMemcpyHostToDevice));
+  const double *src[19] = {s->vx, s->vy, s->pr, s->gp, s->ga, s->exx, s->eyy, s->exy, s->sxx, s->syy,
+                           s->sxy, s->nu, s->nd, s->ro, s->tk, s->cp, s->kt, s->ht, s->et};
+  for (int k = 0; k < 19; k++) {
+    if (!src[k]) return I2E_ERR_ARG;
+    HIP_OK(hipMemcpy(c->nodes.p + (size_t)k * N, src[k], sizeof(double) * N, hipMemcpyHostToDevice));
+  }
+  c->M = s->marknum;
+  const float *ms[7] = {s->markx, s->marky, s->markk, s->markv, s->markp, s->markpor, s->marktmax};
+  for (int k = 0; k < 7; k++)
+    if (c->M) HIP_OK(hipMemcpy(c->mfl.p + (size_t)k * c->Mmax, ms[k], sizeof(float) * c->M, hipMemcpyHostToDevice));
+  if (c->M) HIP_OK(hipMemcpy(c->mt.p, s->markt, c->M, hipMemcpyHostToDevice));
+  // BC tables (int64 -> int32)
+  if (s->bondnum < 1 || s->bondnum > (int64_t)1 << 30) return I2E_ERR_ARG;
+  std::vector<int> bm(4 * (size_t)N), bn(3 * (size_t)s->bondnum);
+  for (size_t k = 0; k < bm.size(); k++) {
+    if (s->bondm[k] < 0 || s->bondm[k] >= s->bondnum) return I2E_ERR_ARG;
+    bm[k] = (int)s->bondm[k];
+  }
+  for (size_t k = 0; k < bn.size(); k++) {
+    if (s->bondn[k] < 0 || s->bondn[k] > 4 * (int64_t)N) return I2E_ERR_ARG;
+    bn[k] = (int)s->bondn[k];
+  }
+  ST_OK(c->bondn.alloc(bn.size()));
+  ST_OK(c->bondv.alloc(4 * (size_t)s->bondnum));
+  HIP_OK(hipMemcpy(c->bondm.p, bm.data(), sizeof(int) * bm.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->bondn.p, bn.data(), sizeof(int) * bn.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c->bondv.p, s->bondv, sizeof(double) * 4 * s->bondnum, hipMemcpyHostToDevice));
+  c->bc = BcTab{c->bondm.p, c->bondn.p, c->bondv.p};
+  c->timestep = s->timestep;
+  c->timesum = s->timesum;
+  c->gxkoef = s->gxkoef;
+  return setup_solvers(c, s);
+}
+
+int i2e_download(i2e_ctx *c, i2e_host_state *s) {
+  if (!c || !s) return I2E_ERR_ARG;
+  HIP_OK(hipStreamSynchronize(c->s));
+  const int N = c->N;
+  double *dst[19] = {s->vx, s->vy, s->pr, s->gp, s->ga, s->exx, s->eyy, s->exy, s->sxx, s->syy,
+                     s->sxy, s->nu, s->nd, s->ro, s->tk, s->cp, s->kt, s->ht, s->et};
+  for (int k = 0; k < 19; k++)
+    if (dst[k]) HIP_OK(hipMemcpy(dst[k], c->nodes.p + (size_t)k * N, sizeof(double) * N, hipMemcpyDeviceToHost));
+  float *ms[7] = {s->markx, s->marky, s->markk, s->markv, s->markp, s->markpor, s->marktmax};
+  for (int k = 0; k < 7; k++)
+    if (ms[k] && c->M)
+      HIP_OK(hipMemcpy(ms[k], c->mfl.p + (size_t)k * c->Mmax, sizeof(float) * c->M, hipMemcpyDeviceToHost));
+  if (s->markt && c->M) HIP_OK(hipMemcpy(s->markt, c->mt.p, c->M, hipMemcpyDeviceToHost));
+  s->marknum = c->M;
+  s->timestep = c->timestep;
+  s->timesum = c->timesum;
+  s->gxkoef = c->gxkoef;
+  return I2E_OK;
+}
+
+// ---------------------------------------------------------------- step functions
+int i2e_maxvelstep(i2e_ctx *c, double *timestep) {
+  if (!c) return I2E_ERR_ARG;
+  HIP_OK(hipMemsetAsync(c->red.p, 0, 2 * sizeof(double), c->s));
+  hipLaunchKernelGGL(k_maxvel, dim3(512), dim3(256), 0, c->s, c->X, c->Y, c->F.vx, c->F.vy, c->red.p);
+  HIP_OK(hipMemcpyAsync(c->pin.p, c->red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
+  double maxvx = c->pin.p[0], maxvy = c->pin.p[1];
+  double xstpx = c->P.xsize / (double)(c->X - 1), ystpy = c->P.ysize / (double)(c->Y - 1);
+  if (maxvx) {
+    maxvx = (c->P.maxxystep * xstpx) / maxvx;
+    c->timestep = maxvx <= c->timestep ? maxvx : c->timestep;
+  }
+  if (maxvy) {
+    maxvy = (c->P.maxxystep * ystpy) / maxvy;
+    c->timestep = maxvy <= c->timestep ? maxvy : c->timestep;
+  }
+  if (timestep) *timestep = c->timestep;
+  return I2E_OK;
+}
+
+static inline bool mixed(const i2e_ctx *c) { return c->o.precision == I2E_PREC_MIXED; }
+
+static int gravity_solve(i2e_ctx *c) {
+  const int X = c->X, Y = c->Y, N = c->N;
+  Grid g = c->grid();
+  const int nbm = 256;
+  hipLaunchKernelGGL(k_grav_mean, dim3(nbm), dim3(256), 0, c->s, g, c->F.ro, c->P.gykoef, c->red.p);
+  HIP_OK(hipMemcpyAsync(c->pin.p, c->red.p, 2 * nbm * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
+  double rosum = 0, ronum = 0;
+  for (int b = 0; b < nbm; b++) {
+    rosum += c->pin.p[2 * b];
+    ronum += c->pin.p[2 * b + 1];
+  }
+  if (ronum > 0) {
+    rosum /= ronum;
+    const double gamma = 6.67384e-11, pival = 2.0 * asin(1.0);
+    c->gxkoef = 4.0 / 3.0 * pival * c->P.gykoef * rosum * gamma;
+  }
+  double xstpx = c->P.xsize / (double)(X - 1), ystpy = c->P.ysize / (double)(Y - 1);
+  GravGeo q{c->P.gykoef, c->gxkoef, 2.0 / xstpx / xstpx + 2 / ystpy / ystpy};
+  hipLaunchKernelGGL(k_grav_rhs, nb2(X, Y), dim3(256), 0, c->s, g, q, c->F.ro, c->bs.p);
+  hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->F.gp, c->xs.p);
+  GravPre gp{X, Y, q.mu};
+  auto A = [&](const double *x, double *y) -> int {
+    hipLaunchKernelGGL(k_grav_apply, nb2(X, Y), dim3(256), 0, c->s, g, q, x, y);
+    return I2E_OK;
+  };
+  auto Mf = [&](const double *r, double *z) -> int {
+    if (mixed(c)) {
+      hipLaunchKernelGGL(k_grav_pre<float>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, gp, r, c->gmask.p,
+                         c->g32.L[0].f.p);
+      ST_OK(c->g32.vcycle(0, c->s));
+      hipLaunchKernelGGL(k_grav_post<float>, nb2(X, Y), dim3(256), 0, c->s, gp, r, c->g32.L[0].u.p, z);
+    } else {
+      hipLaunchKernelGGL(k_grav_pre<double>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, gp, r, c->gmask.p,
+                         c->g64.L[0].f.p);
+      ST_OK(c->g64.vcycle(0, c->s));
+      hipLaunchKernelGGL(k_grav_post<double>, nb2(X, Y), dim3(256), 0, c->s, gp, r, c->g64.L[0].u.p, z);
+    }
+    return I2E_OK;
+  };
+  c->g32.nsmooth = c->g64.nsmooth = c->o.nsmooth;
+  int it = 0;
+  double rr = 0;
+  int st = c->fgv.solve(A, Mf, c->bs.p, c->xs.p, c->o.grav_rtol, c->o.maxit, c->s, &it, &rr);
+  c->st.grav_iters = it;
+  c->st.grav_relres = rr;
+  if (st) return st;
+  hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->xs.p, c->F.gp);
+  hipLaunchKernelGGL(k_grav_ga, nb2(X, Y), dim3(256), 0, c->s, g, c->F.gp, c->F.ga);
+  return I2E_OK;
+}
+
+static int stokes_solve(i2e_ctx *c) {
+  const int X = c->X, Y = c->Y, N = c->N;
+  SOp<double> o = c->sop();
+  hipLaunchKernelGGL(k_stokes_rhs, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->bs.p);
+  hipLaunchKernelGGL(k_stokes_pack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->F.pr, c->F.vx, c->F.vy, c->xs.p);
+  const bool mx = mixed(c);
+  if (mx) {
+    ST_OK(c->v32.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
+    c->v32.nsmooth = c->o.nsmooth;
+  } else {
+    ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
+    c->v64.nsmooth = c->o.nsmooth;
+  }
+  auto A = [&](const double *x, double *y) -> int {
+    hipLaunchKernelGGL(k_stokes_apply, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, x, y);
+    return I2E_OK;
+  };
+  auto Mf = [&](const double *r, double *z) -> int {
+    if (mx) {
+      hipLaunchKernelGGL(k_split_v<float>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v32.L[0].fx.p, c->v32.L[0].fy.p);
+      ST_OK(c->v32.vcycle(0, c->s));
+      hipLaunchKernelGGL(k_schur<float>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->o.schur_scale, r,
+                         c->v32.L[0].ux.p, c->v32.L[0].uy.p, z);
+    } else {
+      hipLaunchKernelGGL(k_split_v<double>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v64.L[0].fx.p,
+                         c->v64.L[0].fy.p);
+      ST_OK(c->v64.vcycle(0, c->s));
+      hipLaunchKernelGGL(k_schur<double>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->o.schur_scale, r,
+                         c->v64.L[0].ux.p, c->v64.L[0].uy.p, z);
+    }
+    hipLaunchKernelGGL(k_schur_bc, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, r, z);
+    return I2E_OK;
+  };
+  int it = 0;
+  double rr = 0;
+  int st = c->fs.solve(A, Mf, c->bs.p, c->xs.p, c->o.stokes_rtol, c->o.maxit, c->s, &it, &rr);
+  c->st.stokes_iters = it;
+  c->st.stokes_relres = rr;
+  if (st) return st;
+  hipLaunchKernelGGL(k_stokes_unpack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->xs.p, c->F.pr, c->F.vx, c->F.vy);
+  hipLaunchKernelGGL(k_strain, nb2(X, Y), dim3(256), 0, c->s, c->grid(), c->F);
+  (void)N;
+  return I2E_OK;
+}
+
+int i2e_viterate(i2e_ctx *c, int n0) {
+  (void)n0;
+  if (!c || !c->setup_done) return I2E_ERR_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  ST_OK(gravity_solve(c));
+  HIP_OK(hipStreamSynchronize(c->s));
+  auto t1 = std::chrono::steady_clock::now();
+  ST_OK(stokes_solve(c));
+  HIP_OK(hipStreamSynchronize(c->s));
+  auto t2 = std::chrono::steady_clock::now();
+  c->st.ms_grav = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  c->st.ms_stokes = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  return I2E_OK;
+}
+
+int i2e_titerate(i2e_ctx *c, int n0) {
+  (void)n0;
+  if (!c || !c->setup_done) return I2E_ERR_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  const int X = c->X, Y = c->Y, N = c->N;
+  Grid g = c->grid();
+  HIP_OK(hipMemsetAsync(c->red.p, 0, sizeof(double), c->s));
+  hipLaunchKernelGGL(k_heat_source, nb2(X, Y), dim3(256), 0, c->s, c->dP, g, c->F, c->bc, c->timesum, c->sol1.p,
+                     c->red.p);
+  HIP_OK(hipMemcpyAsync(c->pin.p, c->red.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
+  double maxrate = c->pin.p[0], tktimestep = 0;
+  if (maxrate) tktimestep = c->P.maxtkstep / maxrate;
+  if (c->timestep == 0 || tktimestep == 0) return I2E_OK;
+  if (c->P.heatmod == 0) c->timestep = c->timestep <= tktimestep ? c->timestep : tktimestep;
+  const double dt = c->timestep;
+  hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->F.tk, c->tk2.p);
+  hipLaunchKernelGGL(k_heat_rhs, nb2(X, Y), dim3(256), 0, c->s, g, c->F, c->bc, dt, c->sol1.p, c->bs.p);
+  hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->F.tk, c->xs.p);
+  hipLaunchKernelGGL(k_heat_fields, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, c->F.ro, c->F.cp, dt, c->bondm.p,
+                     c->hc.p, c->hmask.p, c->hdm.p);
+  const bool mx = mixed(c);
+  if (mx) {
+    ST_OK(c->h32.refresh(c->F.kt, c->hc.p, c->hmask.p, c->hdm.p, c->s));
+    c->h32.nsmooth = c->o.nsmooth;
+  } else {
+    ST_OK(c->h64.refresh(c->F.kt, c->hc.p, c->hmask.p, c->hdm.p, c->s));
+    c->h64.nsmooth = c->o.nsmooth;
+  }
+  auto A = [&](const double *x, double *y) -> int {
+    hipLaunchKernelGGL(k_heat_apply, nb2(X, Y), dim3(256), 0, c->s, g, c->F, c->bc, dt, x, y);
+    return I2E_OK;
+  };
+  auto Mf = [&](const double *r, double *z) -> int {
+    if (mx) {
+      hipLaunchKernelGGL(k_heat_pre<float>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, r, c->F.ro, c->F.cp,
+                         c->hmask.p, c->h32.L[0].f.p);
+      ST_OK(c->h32.vcycle(0, c->s));
+      hipLaunchKernelGGL(k_to_double<float>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, c->h32.L[0].u.p, z);
+    } else {
+      hipLaunchKernelGGL(k_heat_pre<double>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, r, c->F.ro, c->F.cp,
+                         c->hmask.p, c->h64.L[0].f.p);
+      ST_OK(c->h64.vcycle(0, c->s));
+      hipLaunchKernelGGL(k_to_double<double>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, c->h64.L[0].u.p, z);
+    }
+    return I2E_OK;
+  };
+  int it = 0;
+  double rr = 0;
+  int st = c->fh.solve(A, Mf, c->bs.p, c->xs.p, c->o.heat_rtol, c->o.maxit, c->s, &it, &rr);
+  c->st.heat_iters = it;
+  c->st.heat_relres = rr;
+  if (st) return st;
+  hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->xs.p, c->F.tk);
+  // marker temperatures (heatmart.c:515-689); sol0/sol1 live in the ronurecalc accumulators
+  double *s0 = c->A.w0, *s1 = c->A.w1;
+  HIP_OK(hipMemsetAsync(s0, 0, sizeof(double) * N, c->s));
+  HIP_OK(hipMemsetAsync(s1, 0, sizeof(double) * N, c->s));
+  if (c->M) {
+    unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+    hipLaunchKernelGGL(k_heat_markers_a, dim3(gb), dim3(256), 0, c->s, c->dP, g, c->F, c->tk2.p, c->mk, c->M, dt, s0,
+                       s1);
+    if (c->P.heatdif) {
+      hipLaunchKernelGGL(k_heat_nodes_norm, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, s0, s1);
+      hipLaunchKernelGGL(k_heat_markers_b, dim3(gb), dim3(256), 0, c->s, g, c->mk, c->M, s1);
+    }
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->s));
+  c->st.ms_heat = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return I2E_OK;
+}
+
+int i2e_movemark(i2e_ctx *c) {
+  if (!c) return I2E_ERR_ARG;
+  if (!c->M) return I2E_OK;
+  unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+  hipLaunchKernelGGL(k_movemark, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->timestep,
+                     c->need_eii);
+  HIP_OK(hipGetLastError());
+  return I2E_OK;
+}
+
+int i2e_ronurecalc(i2e_ctx *c) {
+  if (!c) return I2E_ERR_ARG;
+  const int N = c->N;
+  HIP_OK(hipMemsetAsync(c->acc.p, 0, sizeof(double) * 12 * N, c->s));
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->s, 1, 1e300, c->red.p + 16);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->s, 1, 0.0, c->red.p + 17);
+  if (c->M) {
+    unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+    hipLaunchKernelGGL(k_ronu_markers, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->A,
+                       c->timesum, c->timestep, c->need_eii);
+  }
+  hipLaunchKernelGGL(k_ronu_nodes, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->dP, c->F, c->A, N, c->red.p + 16);
+  hipLaunchKernelGGL(k_contrast, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->dP, c->F, N, c->red.p + 16);
+  hipLaunchKernelGGL(k_tbc, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->F, c->bc, N);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(c->pin.p + 16, c->red.p + 16, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
+  c->st.numin = c->pin.p[16] >= 1e300 ? 0 : c->pin.p[16];
+  c->st.numax = c->pin.p[17];
+  return I2E_OK;
+}
+
+int i2e_step(i2e_ctx *c) {
+  if (!c) return I2E_ERR_ARG;
+  c->timestep = c->P.maxtmstep;
+  ST_OK(i2e_maxvelstep(c, nullptr));
+  ST_OK(i2e_viterate(c, 0));
+  c->timestep = c->P.maxtmstep;
+  ST_OK(i2e_maxvelstep(c, nullptr));
+  if (c->timestep) ST_OK(i2e_titerate(c, 0));
+  auto t0 = std::chrono::steady_clock::now();
+  if (c->timestep) ST_OK(i2e_movemark(c));
+  ST_OK(i2e_ronurecalc(c));
+  c->st.ms_markers = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  c->timesum += c->timestep;
+  return I2E_OK;
+}
+
+int i2e_stokes_apply_dev(i2e_ctx *c, const double *x, double *y) {
+  if (!c || !x || !y) return I2E_ERR_ARG;
+  hipLaunchKernelGGL(k_stokes_apply, nb2(c->X, c->Y), dim3(256), 0, c->s, c->sop(), c->bc, x, y);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_gravity_apply_dev(i2e_ctx *c, const double *x, double *y) {
+  if (!c || !x || !y) return I2E_ERR_ARG;
+  double xstpx = c->P.xsize / (double)(c->X - 1), ystpy = c->P.ysize / (double)(c->Y - 1);
+  GravGeo q{c->P.gykoef, c->gxkoef, 2.0 / xstpx / xstpx + 2 / ystpy / ystpy};
+  hipLaunchKernelGGL(k_grav_apply, nb2(c->X, c->Y), dim3(256), 0, c->s, c->grid(), q, x, y);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_heat_apply_dev(i2e_ctx *c, const double *x, double *y) {
+  if (!c || !x || !y) return I2E_ERR_ARG;
+  hipLaunchKernelGGL(k_heat_apply, nb2(c->X, c->Y), dim3(256), 0, c->s, c->grid(), c->F, c->bc, c->timestep, x, y);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g, double *ms_stokes_apply) {
+  if (!c || reps < 1) return I2E_ERR_ARG;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  float ms = 0;
+  const size_t MM = (size_t)c->Mmax;
+  // advection from the same start positions each repetition
+  DBuf<float> save;
+  ST_OK(save.alloc(3 * MM + 1));
+  HIP_OK(hipMemcpyAsync(save.p, c->mk.x, sizeof(float) * c->M, hipMemcpyDeviceToDevice, c->s));
+  HIP_OK(hipMemcpyAsync(save.p + MM, c->mk.y, sizeof(float) * c->M, hipMemcpyDeviceToDevice, c->s));
+  HIP_OK(hipMemcpyAsync(save.p + 2 * MM, c->mk.k, sizeof(float) * c->M, hipMemcpyDeviceToDevice, c->s));
+  double tadv = 0;
+  unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+  for (int r = 0; r < reps; r++) {
+    HIP_OK(hipEventRecord(e0, c->s));
+    hipLaunchKernelGGL(k_movemark, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->timestep,
+                       c->need_eii);
+    HIP_OK(hipEventRecord(e1, c->s));
+    HIP_OK(hipEventSynchronize(e1));
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    tadv += ms;
+    HIP_OK(hipMemcpyAsync(c->mk.x, save.p, sizeof(float) * c->M, hipMemcpyDeviceToDevice, c->s));
+    HIP_OK(hipMemcpyAsync(c->mk.y, save.p + MM, sizeof(float) * c->M, hipMemcpyDeviceToDevice, c->s));
+    HIP_OK(hipMemcpyAsync(c->mk.k, save.p + 2 * MM, sizeof(float) * c->M, hipMemcpyDeviceToDevice, c->s));
+  }
+  double tp2g = 0;
+  for (int r = 0; r < reps; r++) {
+    HIP_OK(hipMemsetAsync(c->acc.p, 0, sizeof(double) * 12 * c->N, c->s));
+    HIP_OK(hipEventRecord(e0, c->s));
+    hipLaunchKernelGGL(k_ronu_markers, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->A,
+                       c->timesum, c->timestep, c->need_eii);
+    HIP_OK(hipEventRecord(e1, c->s));
+    HIP_OK(hipEventSynchronize(e1));
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    tp2g += ms;
+  }
+  double tst = 0;
+  hipLaunchKernelGGL(k_stokes_pack, nb2(c->X, c->Y), dim3(256), 0, c->s, c->X, c->Y, c->F.pr, c->F.vx, c->F.vy,
+                     c->xs.p);
+  for (int r = 0; r < reps; r++) {
+    HIP_OK(hipEventRecord(e0, c->s));
+    hipLaunchKernelGGL(k_stokes_apply, nb2(c->X, c->Y), dim3(256), 0, c->s, c->sop(), c->bc, c->xs.p, c->work.p);
+    HIP_OK(hipEventRecord(e1, c->s));
+    HIP_OK(hipEventSynchronize(e1));
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    tst += ms;
+  }
+  save.release();
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ms_advect) *ms_advect = tadv / reps;
+  if (ms_p2g) *ms_p2g = tp2g / reps;
+  if (ms_stokes_apply) *ms_stokes_apply = tst / reps;
+  return I2E_OK;
+}
+
+}  // extern "C"

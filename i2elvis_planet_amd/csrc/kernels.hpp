@@ -1,0 +1,47 @@
+// kernels.hpp -- device data views and kernel declarations shared by the
+// translation units of libi2e.
+#pragma once
+#include "common.hpp"
+
+namespace i2e {
+
+// node fields (reference names, headmart.c:87-99)
+struct NodeF {
+  double *vx, *vy, *pr, *gp, *ga;
+  double *exx, *eyy, *exy, *sxx, *syy, *sxy;
+  double *nu, *nd, *ro, *tk, *cp, *kt, *ht, *et;
+};
+
+// marker SoA (headmart.c:122-127)
+struct Markers {
+  float *x, *y, *k, *v, *p, *por, *tmax;
+  int8_t *t;
+};
+
+// ronurecalc accumulators (ro0..tk0, sol0/sol1 and the viscosity weights)
+struct Acc {
+  double *ro0, *et0, *cp0, *kt0, *ht0, *tk0, *nu0, *nd0, *w0, *w1, *wnu, *wnd;
+};
+
+// boundary tables on device: bondm[4N] (row, 0 = none), bondn[3*nb]
+// (1-based unknown numbers, 0 = none), bondv[4*nb]
+struct BcTab {
+  const int *bondm;
+  const int *bondn;
+  const double *bondv;
+};
+
+// markers.hip
+__global__ void k_movemark(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, double timestep, int need_eii);
+__global__ void k_ronu_markers(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, Acc A, double timesum,
+                               double timestep, int need_eii);
+__global__ void k_ronu_nodes(const i2e_params *p, NodeF F, Acc A, int N, double *mm);
+__global__ void k_contrast(const i2e_params *p, NodeF F, int N, const double *mm);
+__global__ void k_tbc(NodeF F, BcTab bc, int N);
+__global__ void k_heat_markers_a(const i2e_params *p, Grid g, NodeF F, const double *tk2, Markers mk, int64_t M,
+                                 double timestep, double *sol0, double *sol1);
+__global__ void k_heat_nodes_norm(int N, const double *sol0, double *sol1);
+__global__ void k_heat_markers_b(Grid g, Markers mk, int64_t M, const double *sol1);
+__global__ void k_type_counts(Grid g, Markers mk, int64_t M, int ntypes, int *cnt);
+
+}  // namespace i2e

@@ -1,0 +1,40 @@
+// krylov.hpp -- FP64 flexible GMRES on device vectors
+#pragma once
+#include <functional>
+#include <vector>
+
+#include "mg.hpp"
+
+namespace i2e {
+
+// pinned host buffer (hipHostMalloc, never registered after the fact)
+template <class T>
+struct HBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    if (count <= n && p) return I2E_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    HIP_OK(hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault));
+    n = count;
+    return I2E_OK;
+  }
+};
+
+struct Fgmres {
+  using Op = std::function<int(const double *, double *)>;
+  int n = 0, m = 0;
+  size_t ld = 0;
+  DBuf<double> V, Z, w, part, hd;
+  HBuf<double> hh;
+  int init(int n, int m);
+  int norm(const double *x, double *out_dev, hipStream_t s);
+  // solves A x = b to ||b - A x|| <= rtol ||b||; x holds the initial guess
+  int solve(const Op &A, const Op &M, const double *b, double *x, double rtol, int maxit, hipStream_t s, int *iters,
+            double *relres);
+  void release();
+};
+
+}  // namespace i2e

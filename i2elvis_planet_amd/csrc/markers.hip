@@ -1,0 +1,498 @@
+// markers.hip -- per-marker kernels of the step: RK4 advection (movemark,
+// markmart.c:3-188), marker properties + marker->node scatter and node
+// normalisation (ronurecalc, markmart.c:192-1009), temperature boundary
+// projection (tkrecalc, heatmart.c:758-826) and the marker temperature update
+// after the implicit heat solve (titerate, heatmart.c:515-689).
+//
+// Markers stay in the cell-major order the initial lattice gives them
+// (in2mart.c:153-182) and move at most maxxystep cells per step, so a
+// wavefront touches a handful of neighbouring cells: node reads hit L2, and
+// scatters use hardware FP64 atomics (global_atomic_add_f64).
+#include "kernels.hpp"
+
+namespace i2e {
+
+// ---------------------------------------------------------------- rheology
+// viscalc, markmart.c:1855-2063 (ductile branches A-D and the limits; the
+// brittle and Peierls branches need rock constants the configs set to zero)
+__device__ double viscalc_d(const i2e_params *p, double mtk, double mpb, int mm2, double epsin) {
+  const i2e_rock &r = p->rock[mm2];
+  double rt = 8.314 * mtk, e1, k1, sduct, nunewt = 0, nupowl = 0, nueff;
+  if (mm2 > 20) return r.n0;
+  if (r.nu0 != 0) {
+    if (r.dh == 0 && r.dv == 0 && (r.ss == 0 || r.mm == 1.0)) nunewt = 1.0 / r.nu0;
+    if ((r.dh != 0 || r.dv != 0) && (r.ss == 0 || r.mm == 1.0)) {
+      e1 = (r.dh + r.dv * mpb) / rt;
+      if (e1 > 150.0) e1 = 150.0;
+      if (r.dh < 0 && r.dv > 0) {
+        e1 = r.dh * (mtk - r.dv);
+        if (e1 < -150.0) e1 = -150.0;
+      }
+      if (r.dh < 0 && r.dv < 0) {
+        e1 = (-r.dh) * (1.0 - (mtk - (-r.dv)) / (-r.dv)) / 8.314 / (-r.dv);
+        if (e1 > 150.0) e1 = 150.0;
+      }
+      nunewt = 1.0 / (r.nu0 * exp(e1));
+    }
+    if ((r.dh == 0 && r.dv == 0) && r.ss != 0 && r.mm != 1.0) {
+      k1 = r.nu0 / pow(r.ss, r.mm - 1.0);
+      nunewt = 1.0 / (0.5 * k1);
+      sduct = pow(r.nu0 * epsin, 1.0 / r.mm);
+      if (epsin > 0) nupowl = 1.0 / (0.5 * sduct / epsin);
+    }
+    if ((r.dh != 0 || r.dv != 0) && r.ss != 0 && r.mm != 1.0) {
+      e1 = (r.dh + r.dv * mpb) / rt;
+      if (e1 > 150.0) e1 = 150.0;
+      e1 = exp(e1);
+      k1 = r.nu0 / pow(r.ss, r.mm - 1.0);
+      nunewt = 1.0 / (0.5 * k1 * e1);
+      sduct = pow(r.nu0 * e1 * epsin, 1.0 / r.mm);
+      if (epsin > 0) nupowl = 1.0 / (0.5 * sduct / epsin);
+    }
+  }
+  nueff = nunewt + nupowl;
+  if (nueff <= 0) nueff = 1.0 / r.n1;
+  nueff = 1.0 / nueff;
+  if (nueff < r.n0) nueff = r.n0;
+  if (nueff > r.n1) nueff = r.n1;
+  if (nueff < p->nubeg) nueff = p->nubeg;
+  if (nueff > p->nuend) nueff = p->nuend;
+  return nueff;
+}
+
+// allintere, markmart.c:1458-1543: strain rate, stress, P and |g| at (x,y)
+struct Ev {
+  double exy, sxy, exx, sxx, eyy, syy, pr, ga;
+  int m10, m20;
+};
+
+__device__ __forceinline__ void clampxy(const Grid &g, double &x, double &y) {
+  if (x < 0)
+    x = 0;
+  else if (x > g.xsize)
+    x = g.xsize;
+  if (y < 0)
+    y = 0;
+  else if (y > g.ysize)
+    y = g.ysize;
+}
+
+__device__ void allintere_d(const Grid &g, const NodeF &F, double x, double y, Ev &e, bool full) {
+  W4 w;
+  clampxy(g, x, y);
+  e.m10 = serch(g.gx, g.X, x);
+  e.m20 = serch(g.gy, g.Y, y);
+  if (full) {
+    nodewt2(g.gx, g.gy, bn_i0(e.m10, g.X), bn_i0(e.m20, g.Y), x, y, 0, 0, w);
+    e.exy = isum(w, g.Y, F.exy);
+    e.sxy = isum(w, g.Y, F.sxy);
+  }
+  nodewt2(g.gx, g.gy, ce_i0(g.gx, e.m10, x, g.X), ce_i0(g.gy, e.m20, y, g.Y), x, y, -1, -1, w);
+  if (full) {
+    e.exx = isum(w, g.Y, F.exx);
+    e.sxx = isum(w, g.Y, F.sxx);
+    e.eyy = isum(w, g.Y, F.eyy);
+    e.syy = isum(w, g.Y, F.syy);
+  }
+  e.pr = isum(w, g.Y, F.pr);
+  e.ga = isum(w, g.Y, F.ga);
+}
+
+// allinterv, markmart.c:1378-1453
+__device__ __forceinline__ void allinterv_d(const Grid &g, const double *vx, const double *vy, double x, double y,
+                                            double &ovx, double &ovy) {
+  W4 w;
+  int m10, m20;
+  clampxy(g, x, y);
+  m10 = serch(g.gx, g.X, x);
+  m20 = serch(g.gy, g.Y, y);
+  nodewt2(g.gx, g.gy, m10, vx_j0(g.gy, g.Y, m20, y), x, y, 0, +1, w);
+  ovx = isum(w, g.Y, vx);
+  nodewt2(g.gx, g.gy, vy_i0(g.gx, g.X, m10, x), m20, x, y, +1, 0, w);
+  ovy = isum(w, g.Y, vy);
+}
+
+// ---------------------------------------------------------------- movemark
+__global__ void k_movemark(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk, int64_t M, double timestep,
+                           int need_eii) {
+  for (int64_t mm1 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; mm1 < M; mm1 += (int64_t)gridDim.x * blockDim.x) {
+    int t = mk.t[mm1];
+    int mm2 = t >= 100 ? t - 100 : t;
+    float fx = mk.x[mm1], fy = mk.y[mm1];
+    if (!((fx >= 0 && fy >= 0 && (double)fx <= g.xsize && (double)fy <= g.ysize) || p->outgrid != 1)) continue;
+    Ev e;
+    allintere_d(g, F, (double)fx, (double)fy, e, need_eii != 0);
+    double mpb = e.pr * 1e-5, mtk = (double)mk.k[mm1];
+    mk.p[mm1] = (float)e.pr;
+    if (t < 50 && mtk > 0) {
+      double eii = need_eii ? pow(0.5 * (e.exx * e.exx + e.eyy * e.eyy) + e.exy * e.exy, 0.5) : 0.0;
+      mk.v[mm1] = (float)viscalc_d(p, mtk, mpb, t < 20 ? mm2 : mm2 - 20, eii);
+    }
+    double x0 = (double)fx, y0 = (double)fy, vx1, vy1, vx2, vy2, vx3, vy3, vx4, vy4;
+    allinterv_d(g, F.vx, F.vy, x0, y0, vx1, vy1);
+    allinterv_d(g, F.vx, F.vy, x0 + vx1 * timestep / 2.0, y0 + vy1 * timestep / 2.0, vx2, vy2);
+    allinterv_d(g, F.vx, F.vy, x0 + vx2 * timestep / 2.0, y0 + vy2 * timestep / 2.0, vx3, vy3);
+    allinterv_d(g, F.vx, F.vy, x0 + vx3 * timestep, y0 + vy3 * timestep, vx4, vy4);
+    double vx0 = (vx1 + 2.0 * vx2 + 2.0 * vx3 + vx4) / 6.0;
+    double vy0 = (vy1 + 2.0 * vy2 + 2.0 * vy3 + vy4) / 6.0;
+    if (p->outgrid == 2) {
+      if (fx < 0 || (double)fx > g.xsize) vy0 = 0;
+      if (fy < 0 || (double)fy > g.ysize) vx0 = 0;
+    }
+    fx += (float)(timestep * vx0);
+    fy += (float)(timestep * vy0);
+    mk.x[mm1] = fx;
+    mk.y[mm1] = fy;
+    if (fx < 0 || fy < 0 || (double)fx > g.xsize || (double)fy > g.ysize) mk.k[mm1] = 0;
+  }
+}
+
+// ---------------------------------------------------------------- ronurecalc
+// Henke et al. cold pressing / sintering (markmart.c:363-468); the Newton
+// restarts the reference repeats without bound stop after 64 here.
+__device__ void porosity_update_d(const i2e_params *p, float &por, float markk, double mpb, double mtk,
+                                  double timestep) {
+  const double E_act = 85000.000, fact_a = 4.000e-5, grain_orig = 1.000e-6, pivalue = 3.141592654, r_gas = 1.9872,
+               p_ref = 0.130;
+  if (markk < 700.000) {
+    double lim = 0.420 + 0.460 * pow((pow((mpb / p_ref), 1.720) + 1.000), (-1.000));
+    double pd = (double)por;
+    por = (float)(pd <= lim ? pd : lim);
+    pd = (double)por;
+    por = (float)(pd <= 0.850 ? pd : 0.850);
+  }
+  if (markk >= 700.000) {
+    double grain_corr = 0, grain_guess, grain_old = 0, por_change = 0, por_orig, funct_error;
+    int n_count = 1;
+    por_orig = 0.420 + 0.460 * pow((pow((mpb / p_ref), 1.720) + 1.000), (-1.000));
+    if (p->por_init <= por_orig) por_orig = p->por_init;
+    if (por_orig > 0.850) por_orig = 0.850;
+    grain_guess = grain_orig;
+    double q = (1.000 - por_orig) / (1.000 - (double)por);
+    for (;;) {
+      for (int it = 1; it < 100; it++) {
+        grain_old = grain_guess;
+        double funct = pow(grain_guess, 3.000) + pow(grain_orig, 3.000) -
+                       3.000 * pow(grain_guess, 2.000) * grain_orig * pow(q, (1.000 / 3.000)) +
+                       pow(grain_orig, 3.000) * (1.000 - por_orig) / (1.000 - (double)por);
+        double funct_drv = 3.000 * pow(grain_guess, 2.000) - 6.000 * grain_guess * grain_orig * pow(q, (1.000 / 3.000));
+        grain_guess = grain_old - (funct / funct_drv);
+      }
+      funct_error = dabs(grain_guess - grain_old);
+      if (((funct_error > 5.000e-2) || (grain_guess <= grain_orig)) && n_count < 64) {
+        n_count++;
+        grain_guess = 1.250 * (double)n_count * grain_orig;
+        continue;
+      }
+      break;
+    }
+    if (grain_guess > grain_orig) grain_corr = grain_guess;
+    double cont_r = sqrt(pow(grain_corr, 2.000) - pow(grain_orig, 2.000) * pow(q, (2.000 / 3.000)));
+    double c_fact = 2.000 * sqrt(3.000) * (pow(grain_corr, 2.000) - pow(cont_r, 2.000));
+    double grain_stress = c_fact * mpb / (pivalue * pow(cont_r, 2.000));
+    if (por > 0.100)
+      por_change = fact_a * (1.000 - (double)por) * pow(grain_stress, 1.500) / pow((100.000 * grain_corr), 3.000) *
+                   exp(-E_act / (r_gas * mtk)) * timestep;
+    if (por <= 0.100)
+      por_change = 10.000 * (double)por * fact_a * (1.000 - (double)por) * pow(grain_stress, 1.500) /
+                   pow((100.000 * grain_corr), 3.000) * exp(-E_act / (r_gas * mtk)) * timestep;
+    if (por_change > 0.000) por = (float)((double)por - por_change);
+  }
+}
+
+__device__ __forceinline__ void aadd(double *a, double v) { unsafeAtomicAdd(a, v); }
+
+__global__ void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk, int64_t M, Acc A,
+                               double timesum, double timestep, int need_eii) {
+  for (int64_t mm1 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; mm1 < M; mm1 += (int64_t)gridDim.x * blockDim.x) {
+    int t = mk.t[mm1];
+    int mm2 = t >= 100 ? t - 100 : t;
+    float fx = mk.x[mm1], fy = mk.y[mm1], fk = mk.k[mm1];
+    if (!(fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize && fk > 0 && t < 50)) continue;
+    const double x = (double)fx, y = (double)fy;
+    Ev e;
+    allintere_d(g, F, x, y, e, need_eii != 0);
+    double mpb = e.pr * 1.0000e-5, mtk = (double)fk;
+    if (mm2 < 2) {
+      fk = (float)p->tmp_ambient;
+      mk.k[mm1] = fk;
+      mtk = (double)fk;
+    }
+    double mnu = mk.v[mm1];
+    if (mnu <= 0) {
+      double eii = need_eii ? pow(0.5 * (e.exx * e.exx + e.eyy * e.eyy) + e.exy * e.exy, 0.5) : 0.0;
+      mnu = viscalc_d(p, mtk, mpb, t < 20 ? mm2 : mm2 - 20, eii);
+    }
+    double celdx = fx - g.xsize / 2.0, celdy = fy - g.ysize / 2.0;
+    celdy = pow(celdx * celdx + celdy * celdy, 0.5);
+    if (celdy > g.xsize / 2.0 * p->radmax_frac) mnu = p->eta_outer;
+    float por = mk.por[mm1];
+    if (mm2 == 0 || mm2 >= 7) por = 0.000f;
+    if (t != 0)
+      if (mk.tmax[mm1] < fk) mk.tmax[mm1] = fk;
+    if (p->porosity_on && p->porous[mm2] && (por > 0.000f && por <= 0.850f))
+      porosity_update_d(p, por, fk, mpb, mtk, timestep);
+    if (por < 0.000f) por = 0.000f;
+    if (por > 1.000f) por = 1.000f;
+    mk.por[mm1] = por;
+    const i2e_rock &r = p->rock[mm2];
+    double mro, mbb = 0;
+    if (p->densimod == 0)
+      mro = r.ro;
+    else {
+      mbb = r.bb / (1.0 - r.bb * (mtk - 298.15));
+      mro = r.ro * (1.0 - r.bb * (mtk - 298.15)) * (1.0 + r.aa * (mpb - 1.0) * 1e-3) * (1.000 - (double)por);
+    }
+    double mcp = r.cp;
+    double mkt = (r.kt + r.kf / (mtk + 77.0)) * exp(r.kp * mpb);
+    if (p->melt_on && p->meltable[mm2 < 20 ? mm2 : mm2 - 20]) {
+      double xm = (mtk - p->melt_ts) / (p->melt_tl - p->melt_ts);
+      if (xm < 0.0) xm = 0.0;
+      if (xm > 1.0) xm = 1.0;
+      int tnew = t;
+      if (xm > 0 && tnew < 20) tnew += 20;
+      if (xm <= 0 && tnew > 20) tnew -= 20;
+      mk.t[mm1] = (int8_t)tnew;
+      if (tnew > 20) {
+        const i2e_rock &rm = p->rock[tnew];
+        const i2e_rock &rs = p->rock[tnew - 20];
+        mro = mro * (1.0 - xm) + xm * rm.ro * (1.0 - rm.bb * (mtk - 298.15)) * (1.0 + rm.aa * (mpb - 1.0) * 1e-3);
+        if (xm > p->melt_switch) {
+          double nueff = p->melt_eta;
+          if (nueff < p->nubeg) nueff = p->nubeg;
+          if (nueff > p->nuend) nueff = p->nuend;
+          mnu = nueff;
+        }
+        mkt = ((rm.kt + rm.kf / (mtk + 77.0)) * exp(rm.kp * mpb)) * xm +
+              ((rs.kt + rs.kf / (mtk + 77.0)) * exp(rs.kp * mpb)) * (1.0 - xm);
+        if (xm > 0 && xm < 1.0) {
+          double xp = (mtk + 1.0 - p->melt_ts) / (p->melt_tl - p->melt_ts);
+          double xq = (mtk - 1.0 - p->melt_ts) / (p->melt_tl - p->melt_ts);
+          xp = xp < 0.0 ? 0.0 : (xp > 1.0 ? 1.0 : xp);
+          xq = xq < 0.0 ? 0.0 : (xq > 1.0 ? 1.0 : xq);
+          mcp += (p->melt_hlat * xp - p->melt_hlat * xq) / (2.0 * 1.0);
+        }
+      }
+    }
+    if (p->porous[mm2]) {
+      double kt0 = r.kt, pr0 = (double)por;
+      if (por < 0.200f) mkt = kt0 * exp(-pr0 / 0.080);
+      if (por >= 0.200f && por <= 0.400f)
+        mkt = pow((pow((kt0 * exp(-pr0 / 0.080)), 4.000) + pow((kt0 * exp(-1.200 - (pr0 / 0.167))), 4.000)), 0.250);
+      if (por > 0.400f) mkt = kt0 * exp(-1.200 - (pr0 / 0.167));
+    }
+    if (mkt <= 0.0010) mkt = 0.0010;
+    if (mkt >= p->k_cutoff) mkt = p->k_cutoff;
+    double mht = 0;
+    double timesumA = timesum / (365.250 * 24.000 * 3600.000);
+    if (p->is_silicate[mm2]) {
+      double init_al = 1.89716e+23 * p->al2627_init * 4.999e-13 / (3.264e+13);
+      double ll = 0;
+      if (p->longlived != 0)
+        ll = 0.40689 * 1.430e-11 * exp(-timesumA * 4.9867e-10) + 2.990e-12 * exp(-timesumA * 9.8458e-10) +
+             1.600e-12 * exp(-timesumA * 1.5541e-10) + 1.000e-12 * exp(-timesumA * 4.9511e-11);
+      mht = (init_al * exp(-timesumA * 9.6673e-7) * p->h0_scale + ll) * mro;
+    } else if (p->is_iron[mm2]) {
+      double init_fe = 1.971e+24 * p->fe6056_init * 4.3451e-13 / (1.183e+14), ka = 0;
+      if (p->longlived != 0) ka = 0.090625 * 1.430e-11 * exp(-timesumA * 4.9867e-10);
+      mht = (init_fe * exp(-timesumA * 2.6660e-7) + ka) * mro;
+    }
+    // scatter (markmart.c:840-911)
+    W4 w;
+    nodewt2(g.gx, g.gy, e.m10, e.m20, x, y, 0, 0, w);
+    {
+      double ddx = 1.0 - dabs(w.wx0 - 0.5), ddy = 1.0 - dabs(w.wy0 - 0.5);
+      if (ddx > 0.5 && ddy > 0.5) {
+        double swt2 = 1.0 * 4.0 * (ddx - 0.5) * (ddy - 0.5);
+        int m3 = (e.m10 + 1) * g.Y + e.m20 + 1;
+        aadd(&A.nd0[m3], mnu * swt2);
+        aadd(&A.wnd[m3], swt2);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        int m3 = (e.m10 + a) * g.Y + e.m20 + b;
+        double ddx = a ? w.wx1 : w.wx0, ddy = b ? w.wy1 : w.wy0;
+        double swt = 1.0 * ddx * ddy;
+        if (ddx > 0.5 && ddy > 0.5) {
+          double swt2 = 1.0 * 4.0 * (ddx - 0.5) * (ddy - 0.5);
+          aadd(&A.nu0[m3], mnu * swt2);
+          aadd(&A.wnu[m3], swt2);
+        }
+        aadd(&A.ro0[m3], mro * swt);
+        aadd(&A.et0[m3], mbb * swt);
+        aadd(&A.cp0[m3], mcp * swt);
+        aadd(&A.kt0[m3], mkt * swt);
+        aadd(&A.ht0[m3], mht * swt);
+        aadd(&A.w0[m3], swt);
+        aadd(&A.tk0[m3], mtk * swt);
+        aadd(&A.w1[m3], swt);
+      }
+  }
+}
+
+// positive doubles order like their bit patterns
+__device__ __forceinline__ void amin_pos(double *a, double v) {
+  atomicMin((unsigned long long *)a, (unsigned long long)__double_as_longlong(v));
+}
+__device__ __forceinline__ void amax_pos(double *a, double v) {
+  atomicMax((unsigned long long *)a, (unsigned long long)__double_as_longlong(v));
+}
+
+// node normalisation, markmart.c:917-994; mm[0]=numin, mm[1]=numax
+// (initialised to +huge / 0 by the caller)
+__global__ void k_ronu_nodes(const i2e_params *__restrict__ p, NodeF F, Acc A, int N, double *mm) {
+  int m3 = blockIdx.x * blockDim.x + threadIdx.x;
+  double lmin = 1e300, lmax = 0;
+  if (m3 < N) {
+    if (A.wnu[m3]) {
+      double v = A.nu0[m3] / A.wnu[m3];
+      if (v < p->nubeg) v = p->nubeg;
+      if (v > p->nuend) v = p->nuend;
+      lmin = fmin(lmin, v);
+      lmax = fmax(lmax, v);
+      F.nu[m3] = v;
+    }
+    if (A.wnd[m3]) {
+      double v = A.nd0[m3] / A.wnd[m3];
+      if (v < p->nubeg) v = p->nubeg;
+      if (v > p->nuend) v = p->nuend;
+      lmin = fmin(lmin, v);
+      lmax = fmax(lmax, v);
+      F.nd[m3] = v;
+    }
+    double w0 = A.w0[m3];
+    if (w0) {
+      F.ro[m3] = A.ro0[m3] / w0;
+      F.et[m3] = A.et0[m3] / w0;
+      F.cp[m3] = A.cp0[m3] / w0;
+      F.kt[m3] = A.kt0[m3] / w0;
+      F.ht[m3] = A.ht0[m3] / w0;
+      if (A.w1[m3]) F.tk[m3] = A.tk0[m3] / A.w1[m3];
+    }
+  }
+  // wavefront reduction then one atomic per wavefront
+  for (int o = 32; o > 0; o >>= 1) {
+    lmin = fmin(lmin, __shfl_xor(lmin, o));
+    lmax = fmax(lmax, __shfl_xor(lmax, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (lmin < 1e300) amin_pos(&mm[0], lmin);
+    if (lmax > 0) amax_pos(&mm[1], lmax);
+  }
+}
+
+// viscosity contrast limit (markmart.c:1002-1009)
+__global__ void k_contrast(const i2e_params *__restrict__ p, NodeF F, int N, const double *mm) {
+  int m3 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m3 >= N) return;
+  double numin = mm[0], numax = mm[1];
+  if (numin >= 1e300) numin = 0;  // no weighted node at all: the reference's numin stays 0
+  if (p->nucontr >= 1.0)
+    numax = numin * p->nucontr;
+  else
+    numin = numax * p->nucontr;
+  double a = F.nu[m3], b = F.nd[m3];
+  if (a < numin) a = numin;
+  if (a > numax) a = numax;
+  if (b < numin) b = numin;
+  if (b > numax) b = numax;
+  F.nu[m3] = a;
+  F.nd[m3] = b;
+}
+
+// tkrecalc (heatmart.c:758-826) for BC rows whose neighbours are free nodes:
+// T = const + sum k T_nbr; free nodes keep their value.
+__global__ void k_tbc(NodeF F, BcTab bc, int N) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  int b = bc.bondm[3 * N + m];
+  if (!b) return;
+  double v = bc.bondv[4 * b];
+  for (int n1 = 0; n1 < 3; n1++) {
+    int nb = bc.bondn[3 * b + n1];
+    if (nb) v += bc.bondv[4 * b + n1 + 1] * F.tk[nb - 1 - 3 * N];
+  }
+  F.tk[m] = v;
+}
+
+// ---------------------------------------------------------------- marker T
+// heatmart.c:529-627: subgrid diffusion + scatter of the diffusion change
+__global__ void k_heat_markers_a(const i2e_params *__restrict__ p, Grid g, NodeF F, const double *__restrict__ tk2,
+                                 Markers mk, int64_t M, double timestep, double *sol0, double *sol1) {
+  for (int64_t m3 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m3 < M; m3 += (int64_t)gridDim.x * blockDim.x) {
+    float fx = mk.x[m3], fy = mk.y[m3];
+    if (!(fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize && mk.t[m3] < 50)) continue;
+    double x = fx, y = fy;
+    int m10 = serch(g.gx, g.X, x), m20 = serch(g.gy, g.Y, y);
+    W4 w;
+    nodewt2(g.gx, g.gy, m10, m20, x, y, 0, 0, w);
+    double e2 = isum(w, g.Y, F.tk), e3 = isum(w, g.Y, tk2);
+    float fk = mk.k[m3];
+    if (fk <= 0) fk = (float)e3;
+    if (p->heatdif) {
+      double dtk = e3 - (double)fk;
+      double celdx = g.gx[m10 + 1] - g.gx[m10], celdy = g.gy[m20 + 1] - g.gy[m20];
+      double mkt = isum(w, g.Y, F.kt), mcp = isum(w, g.Y, F.cp), mro = isum(w, g.Y, F.ro);
+      double dtk1 = -p->heatdif * mkt / mro / mcp * 2.0 * (1.0 / celdx / celdx + 1.0 / celdy / celdy) * timestep;
+      if (dtk1 < -150.0) dtk1 = -150.0;
+      dtk1 = dtk * (1.0 - exp(dtk1));
+      fk += (float)dtk1;
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          int m4 = (m10 + a) * g.Y + m20 + b;
+          double ival = (a ? w.wx1 : w.wx0) * (b ? w.wy1 : w.wy0) * 1.0;
+          aadd(&sol0[m4], ival);
+          aadd(&sol1[m4], dtk1 * ival);
+        }
+    }
+    fk += (float)(e2 - e3);
+    mk.k[m3] = fk;
+  }
+}
+
+__global__ void k_heat_nodes_norm(int N, const double *sol0, double *sol1) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < N && sol0[m]) sol1[m] /= sol0[m];
+}
+
+// heatmart.c:635-688: antidiffusion
+__global__ void k_heat_markers_b(Grid g, Markers mk, int64_t M, const double *__restrict__ sol1) {
+  for (int64_t m3 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m3 < M; m3 += (int64_t)gridDim.x * blockDim.x) {
+    float fx = mk.x[m3], fy = mk.y[m3];
+    if (!(fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize && mk.t[m3] < 50)) continue;
+    double x = fx, y = fy;
+    int m10 = serch(g.gx, g.X, x), m20 = serch(g.gy, g.Y, y);
+    W4 w;
+    nodewt2(g.gx, g.gy, m10, m20, x, y, 0, 0, w);
+    float fk = mk.k[m3];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        int m4 = (m10 + a) * g.Y + m20 + b;
+        double ival = (a ? w.wx1 : w.wx0) * (b ? w.wy1 : w.wy0);
+        double s = sol1[m4];
+        if (s) fk -= (float)(s * ival);
+      }
+    mk.k[m3] = fk;
+  }
+}
+
+// per-cell marker-type histogram (cell = m1serch,m2serch of the position)
+__global__ void k_type_counts(Grid g, Markers mk, int64_t M, int ntypes, int *cnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
+    float fx = mk.x[i], fy = mk.y[i];
+    int t = mk.t[i];
+    if (!(fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize)) continue;
+    if (t < 0 || t >= ntypes) continue;
+    int c = serch(g.gx, g.X, (double)fx) * (g.Y - 1) + serch(g.gy, g.Y, (double)fy);
+    atomicAdd(&cnt[c * ntypes + t], 1);
+  }
+}
+
+}  // namespace i2e

@@ -1,0 +1,768 @@
+// mg.hip -- geometric multigrid for the Stokes velocity block and for the
+// scalar (heat, gravity) operators.  See mg.hpp for the design.
+#include <algorithm>
+#include <cmath>
+
+#include "mg.hpp"
+
+namespace i2e {
+
+static inline dim3 blk2() { return dim3(64, 4); }
+static inline dim3 grd2(int X, int Y) { return dim3((Y + 63) / 64, (X + 3) / 4); }
+
+#define IDX2                                                   \
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;         \
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+
+std::vector<int> coarse_lines(int n) {
+  std::vector<int> v;
+  for (int i = 0; i < n; i += 2) v.push_back(i);
+  if (v.back() != n - 1) v.push_back(n - 1);
+  return v;
+}
+
+int build_geometry(const double *gx, const double *gy, int X, int Y, int coarse_max, std::vector<GeoLevel> &geo) {
+  geo.clear();
+  GeoLevel g0;
+  g0.X = X;
+  g0.Y = Y;
+  g0.hx.assign(gx, gx + X);
+  g0.hy.assign(gy, gy + Y);
+  geo.push_back(std::move(g0));
+  while (true) {
+    GeoLevel &f = geo.back();
+    if (std::min(f.X, f.Y) - 1 <= coarse_max || std::min(f.X, f.Y) < 9) break;
+    std::vector<int> ci = coarse_lines(f.X), cj = coarse_lines(f.Y);
+    f.fi = ci;
+    f.fj = cj;
+    GeoLevel c;
+    c.X = (int)ci.size();
+    c.Y = (int)cj.size();
+    for (int a : ci) c.hx.push_back(f.hx[a]);
+    for (int a : cj) c.hy.push_back(f.hy[a]);
+    geo.push_back(std::move(c));
+  }
+  for (auto &g : geo) {
+    ST_OK(g.gx.alloc(g.X));
+    ST_OK(g.gy.alloc(g.Y));
+    HIP_OK(hipMemcpy(g.gx.p, g.hx.data(), g.X * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(g.gy.p, g.hy.data(), g.Y * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return I2E_OK;
+}
+
+int Map1::build(const std::vector<double> &cf, const std::vector<double> &cc) {
+  nf = (int)cf.size();
+  nc = (int)cc.size();
+  std::vector<int> hi0(nf), hrb(nc, nf), hre(nc, 0);
+  std::vector<double> hw(nf);
+  for (int k = 0; k < nf; k++) {
+    int I = (int)(std::upper_bound(cc.begin(), cc.end(), cf[k]) - cc.begin()) - 1;
+    if (I < 0) I = 0;
+    if (I > nc - 2) I = nc - 2;
+    double w = (cf[k] - cc[I]) / (cc[I + 1] - cc[I]);
+    if (w < 0) w = 0;
+    if (w > 1) w = 1;
+    hi0[k] = I;
+    hw[k] = w;
+    if (w < 1) {
+      hrb[I] = std::min(hrb[I], k);
+      hre[I] = std::max(hre[I], k + 1);
+    }
+    if (w > 0) {
+      hrb[I + 1] = std::min(hrb[I + 1], k);
+      hre[I + 1] = std::max(hre[I + 1], k + 1);
+    }
+  }
+  for (int I = 0; I < nc; I++)
+    if (hre[I] <= hrb[I]) hrb[I] = hre[I] = 0;
+  ST_OK(i0.alloc(nf));
+  ST_OK(w.alloc(nf));
+  ST_OK(rb.alloc(nc));
+  ST_OK(re.alloc(nc));
+  HIP_OK(hipMemcpy(i0.p, hi0.data(), nf * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(w.p, hw.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(rb.p, hrb.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(re.p, hre.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+  return I2E_OK;
+}
+
+__device__ __forceinline__ double wgt(const MapView &m, int k, int I) {
+  int a = m.i0[k];
+  double w = m.w[k];
+  return (a == I ? 1.0 - w : 0.0) + (a + 1 == I ? w : 0.0);
+}
+
+// ---------------------------------------------------------------- generic kernels
+template <class T>
+__global__ void k_cast(int n, const double *a, T *b) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = T(a[t]);
+}
+template <class T>
+__global__ void k_zero(int n, T *a) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = T(0);
+}
+
+// node field restriction through the node maps; geometric mean if lg
+template <class T>
+__global__ void k_restrict_node(int Xc, int Yc, int Yf, MapView mx, MapView my, const T *f, T *c, int lg) {
+  IDX2;
+  if (i >= Xc || j >= Yc) return;
+  double s = 0, ws = 0;
+  for (int k = mx.rb[i]; k < mx.re[i]; k++) {
+    double wx = wgt(mx, k, i);
+    if (wx == 0) continue;
+    for (int l = my.rb[j]; l < my.re[j]; l++) {
+      double w = wx * wgt(my, l, j);
+      if (w == 0) continue;
+      double v = (double)f[k * Yf + l];
+      s += w * (lg ? log(v) : v);
+      ws += w;
+    }
+  }
+  double r = ws > 0 ? s / ws : 0.0;
+  c[i * Yc + j] = T(lg ? exp(r) : r);
+}
+
+// cell field restriction: average over the fine cells inside coarse cell
+// (cells are indexed by their lower-right node, i,j >= 1)
+template <class T>
+__global__ void k_restrict_cell(int Xc, int Yc, int Yf, const int *fi, const int *fj, const T *f, T *c, int lg) {
+  IDX2;
+  if (i >= Xc || j >= Yc) return;
+  if (i == 0 || j == 0) {
+    c[i * Yc + j] = f[fi[i] * Yf + fj[j]];
+    return;
+  }
+  double s = 0, ws = 0;
+  for (int a = fi[i - 1] + 1; a <= fi[i]; a++)
+    for (int b = fj[j - 1] + 1; b <= fj[j]; b++) {
+      double v = (double)f[a * Yf + b];
+      s += lg ? log(v) : v;
+      ws += 1;
+    }
+  double r = s / ws;
+  c[i * Yc + j] = T(lg ? exp(r) : r);
+}
+
+// ---------------------------------------------------------------- velocity rows
+template <class T, class A>
+__device__ __forceinline__ T vrow(const VLev<T> &L, const A &v, int comp, int i, int j) {
+  const int Y = L.o.Y, m = i * Y + j;
+  if (comp == 1) {
+    int k = L.kx[m];
+    if (k == 0) return xmom(L.o, v, i, j, false);
+    T s = v.vx(i, j);
+    if (k == 1 && L.nbx[m] >= 0) s -= L.kkx[m] * v.vx(L.nbx[m] / Y, L.nbx[m] % Y);
+    return T(L.o.mu) * s;
+  } else {
+    int k = L.ky[m];
+    if (k == 0) return ymom(L.o, v, i, j, false);
+    T s = v.vy(i, j);
+    if (k == 1 && L.nby[m] >= 0) s -= L.kky[m] * v.vy(L.nby[m] / Y, L.nby[m] % Y);
+    return T(L.o.mu) * s;
+  }
+}
+
+template <class T>
+__global__ void k_vdiag(VLev<T> L, T *dx, T *dy) {
+  IDX2;
+  const int X = L.o.X, Y = L.o.Y;
+  if (i >= X || j >= Y) return;
+  const int m = i * Y + j;
+  dx[m] = vrow(L, VUnit<T>{1, m, Y}, 1, i, j);
+  dy[m] = vrow(L, VUnit<T>{2, m, Y}, 2, i, j);
+}
+
+// red-black Gauss-Seidel half sweep on one component
+template <class T>
+__global__ void k_vgs(VLev<T> L, int comp, int color, T *ux, T *uy, const T *fx, const T *fy, const T *dx,
+                      const T *dy) {
+  IDX2;
+  const int X = L.o.X, Y = L.o.Y;
+  if (i >= X || j >= Y || ((i + j) & 1) != color) return;
+  const int m = i * Y + j;
+  VSplit<T> v{ux, uy, nullptr, Y};
+  T r = vrow(L, v, comp, i, j);
+  if (comp == 1)
+    ux[m] += (fx[m] - r) / dx[m];
+  else
+    uy[m] += (fy[m] - r) / dy[m];
+}
+
+template <class T>
+__global__ void k_vresid(VLev<T> L, const T *ux, const T *uy, const T *fx, const T *fy, T *rx, T *ry) {
+  IDX2;
+  const int X = L.o.X, Y = L.o.Y;
+  if (i >= X || j >= Y) return;
+  const int m = i * Y + j;
+  VSplit<T> v{ux, uy, nullptr, Y};
+  rx[m] = L.kx[m] == 0 ? fx[m] - vrow(L, v, 1, i, j) : T(0);
+  ry[m] = L.ky[m] == 0 ? fy[m] - vrow(L, v, 2, i, j) : T(0);
+}
+
+// coarse rhs = normalised transpose interpolation of the fine residual
+// (fine BC rows were zeroed in k_vresid)
+template <class T>
+__global__ void k_vrestrict(int Xc, int Yc, int Yf, MapView xn, MapView xm, MapView yn, MapView ym, const int8_t *ckx,
+                            const int8_t *cky, const T *rx, const T *ry, T *fx, T *fy) {
+  IDX2;
+  if (i >= Xc || j >= Yc) return;
+  const int m = i * Yc + j;
+  // Vx: x at nodes, y at mids (only j <= Yc-2 carry Vx)
+  double s = 0, ws = 0;
+  if (ckx[m] == 0 && j <= Yc - 2) {
+    for (int k = xn.rb[i]; k < xn.re[i]; k++) {
+      double wx = wgt(xn, k, i);
+      if (wx == 0) continue;
+      for (int l = ym.rb[j]; l < ym.re[j]; l++) {
+        double w = wx * wgt(ym, l, j);
+        if (w == 0) continue;
+        s += w * (double)rx[k * Yf + l];
+        ws += w;
+      }
+    }
+  }
+  fx[m] = T(ws > 0 ? s / ws : 0.0);
+  s = 0;
+  ws = 0;
+  if (cky[m] == 0 && i <= Xc - 2) {
+    for (int k = xm.rb[i]; k < xm.re[i]; k++) {
+      double wx = wgt(xm, k, i);
+      if (wx == 0) continue;
+      for (int l = yn.rb[j]; l < yn.re[j]; l++) {
+        double w = wx * wgt(yn, l, j);
+        if (w == 0) continue;
+        s += w * (double)ry[k * Yf + l];
+        ws += w;
+      }
+    }
+  }
+  fy[m] = T(ws > 0 ? s / ws : 0.0);
+}
+
+template <class T>
+__global__ void k_vprolong(int Xf, int Yf, int Yc, MapView xn, MapView xm, MapView yn, MapView ym, const int8_t *fkx,
+                           const int8_t *fky, const T *cx, const T *cy, T *ux, T *uy) {
+  IDX2;
+  if (i >= Xf || j >= Yf) return;
+  const int m = i * Yf + j;
+  if (fkx[m] != 2 && j <= Yf - 2) {
+    int a = xn.i0[i], b = ym.i0[j];
+    double wa = xn.w[i], wb = ym.w[j];
+    double v = (1 - wa) * ((1 - wb) * (double)cx[a * Yc + b] + wb * (double)cx[a * Yc + b + 1]) +
+               wa * ((1 - wb) * (double)cx[(a + 1) * Yc + b] + wb * (double)cx[(a + 1) * Yc + b + 1]);
+    ux[m] += T(v);
+  }
+  if (fky[m] != 2 && i <= Xf - 2) {
+    int a = xm.i0[i], b = yn.i0[j];
+    double wa = xm.w[i], wb = yn.w[j];
+    double v = (1 - wa) * ((1 - wb) * (double)cy[a * Yc + b] + wb * (double)cy[a * Yc + b + 1]) +
+               wa * ((1 - wb) * (double)cy[(a + 1) * Yc + b] + wb * (double)cy[(a + 1) * Yc + b + 1]);
+    uy[m] += T(v);
+  }
+}
+
+// dense matrix of the coarsest level, column c = A e_c, rows/cols ordered
+// [Vx(0..N-1), Vy(0..N-1)]; stored into the left half of an n x 2n array
+template <class T>
+__global__ void k_vdense(VLev<T> L, int n, double *aug) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (c >= n) return;
+  const int N = n / 2, Y = L.o.Y;
+  int comp = r < N ? 1 : 2, m = r % N;
+  VUnit<T> e{c < N ? 1 : 2, c % N, Y};
+  aug[(size_t)r * 2 * n + c] = (double)vrow(L, e, comp, m / Y, m % Y);
+  aug[(size_t)r * 2 * n + n + c] = (r == c) ? 1.0 : 0.0;
+}
+
+// Gauss-Jordan inversion with partial pivoting in one workgroup
+template <class T>
+__global__ void k_gauss_jordan(int n, double *a, T *inv) {
+  __shared__ double sv[1024];
+  __shared__ int si[1024];
+  __shared__ int piv;
+  const int t = threadIdx.x, nt = blockDim.x, w = 2 * n;
+  for (int p = 0; p < n; p++) {
+    double best = -1;
+    int bi = p;
+    for (int r = p + t; r < n; r += nt) {
+      double v = fabs(a[(size_t)r * w + p]);
+      if (v > best) {
+        best = v;
+        bi = r;
+      }
+    }
+    sv[t] = best;
+    si[t] = bi;
+    __syncthreads();
+    for (int o = nt / 2; o > 0; o >>= 1) {
+      if (t < o && sv[t + o] > sv[t]) {
+        sv[t] = sv[t + o];
+        si[t] = si[t + o];
+      }
+      __syncthreads();
+    }
+    if (t == 0) piv = si[0];
+    __syncthreads();
+    const int q = piv;
+    if (q != p)
+      for (int c = t; c < w; c += nt) {
+        double x = a[(size_t)p * w + c];
+        a[(size_t)p * w + c] = a[(size_t)q * w + c];
+        a[(size_t)q * w + c] = x;
+      }
+    __syncthreads();
+    double d = a[(size_t)p * w + p];
+    __syncthreads();
+    for (int c = t; c < w; c += nt) a[(size_t)p * w + c] /= d;
+    __syncthreads();
+    for (int idx = t; idx < n * w; idx += nt) {
+      int r = idx / w, c = idx % w;
+      if (r == p) continue;
+      double f = a[(size_t)r * w + p];
+      if (c != p) a[(size_t)r * w + c] -= f * a[(size_t)p * w + c];
+    }
+    __syncthreads();
+    for (int r = t; r < n; r += nt)
+      if (r != p) a[(size_t)r * w + p] = 0.0;
+    __syncthreads();
+  }
+  for (int idx = t; idx < n * n; idx += nt) inv[idx] = T(a[(size_t)(idx / n) * w + n + idx % n]);
+}
+
+// u = Ainv f with f = [f0 (n0 entries), f1]
+template <class T>
+__global__ void k_dense_solve(int n, int n0, const T *ainv, const T *f0, const T *f1, T *u0, T *u1) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0;
+  for (int c = 0; c < n; c++) s += (double)ainv[(size_t)r * n + c] * (double)(c < n0 ? f0[c] : f1[c - n0]);
+  if (r < n0)
+    u0[r] = T(s);
+  else
+    u1[r - n0] = T(s);
+}
+
+// ---------------------------------------------------------------- VelMG host
+template <class T>
+VLev<T> VelMG<T>::lev(int l) const {
+  const VelLevel<T> &v = L[l];
+  const GeoLevel &g = (*geo)[l];
+  VLev<T> o;
+  o.o = SOp<T>{v.X, v.Y, g.gx.p, g.gy.p, v.nu.p, v.nd.p, v.ro.p, v.gp.p, dt, mu};
+  o.kx = v.kx.p;
+  o.ky = v.ky.p;
+  o.nbx = v.nbx.p;
+  o.nby = v.nby.p;
+  o.kkx = v.kkx.p;
+  o.kky = v.kky.p;
+  return o;
+}
+
+// BC pattern of a coarse level from the finer one: each coarse unknown copies
+// kind and neighbour offset of the fine unknown on the same wall position
+static int map_index(int I, int nc, int nf, const std::vector<int> &fi, int lastpad) {
+  // lastpad: 1 for mid-point sets (last valid index nc-2 <-> nf-2)
+  if (I == 0) return 0;
+  if (I >= nc - 1 - lastpad) return nf - (nc - I);
+  int a = fi[I];
+  if (a < 1) a = 1;
+  if (a > nf - 2 - lastpad) a = nf - 2 - lastpad;
+  return a;
+}
+
+template <class T>
+int VelMG<T>::setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,
+                    const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,
+                    const std::vector<double> &kky0) {
+  geo = &g;
+  const int nl = (int)g.size();
+  L.clear();
+  L.resize(nl);
+  for (int l = 0; l < nl; l++) {
+    VelLevel<T> &v = L[l];
+    v.X = g[l].X;
+    v.Y = g[l].Y;
+    v.N = v.X * v.Y;
+    if (l == 0) {
+      v.hkx = kx0;
+      v.hky = ky0;
+      v.hnbx = nbx0;
+      v.hnby = nby0;
+      v.hkkx.assign(kkx0.begin(), kkx0.end());
+      v.hkky.assign(kky0.begin(), kky0.end());
+    } else {
+      const VelLevel<T> &f = L[l - 1];
+      const GeoLevel &gf = g[l - 1];
+      v.hkx.assign(v.N, 0);
+      v.hky.assign(v.N, 0);
+      v.hnbx.assign(v.N, -1);
+      v.hnby.assign(v.N, -1);
+      v.hkkx.assign(v.N, T(0));
+      v.hkky.assign(v.N, T(0));
+      for (int I = 0; I < v.X; I++)
+        for (int J = 0; J < v.Y; J++) {
+          int m = I * v.Y + J;
+          // Vx: x nodes, y mids (ghost row J = Y-1)
+          {
+            int a = map_index(I, v.X, f.X, gf.fi, 0), b = (J == v.Y - 1) ? f.Y - 1 : map_index(J, v.Y, f.Y, gf.fj, 1);
+            int fm = a * f.Y + b;
+            v.hkx[m] = f.hkx[fm];
+            if (f.hkx[fm] == 1 && f.hnbx[fm] >= 0) {
+              int da = f.hnbx[fm] / f.Y - a, db = f.hnbx[fm] % f.Y - b;
+              v.hnbx[m] = (I + da) * v.Y + (J + db);
+              v.hkkx[m] = f.hkkx[fm];
+            }
+          }
+          {
+            int a = (I == v.X - 1) ? f.X - 1 : map_index(I, v.X, f.X, gf.fi, 1), b = map_index(J, v.Y, f.Y, gf.fj, 0);
+            int fm = a * f.Y + b;
+            v.hky[m] = f.hky[fm];
+            if (f.hky[fm] == 1 && f.hnby[fm] >= 0) {
+              int da = f.hnby[fm] / f.Y - a, db = f.hnby[fm] % f.Y - b;
+              v.hnby[m] = (I + da) * v.Y + (J + db);
+              v.hkky[m] = f.hkky[fm];
+            }
+          }
+        }
+    }
+    ST_OK(v.nu.alloc(v.N));
+    ST_OK(v.nd.alloc(v.N));
+    ST_OK(v.ro.alloc(v.N));
+    ST_OK(v.gp.alloc(v.N));
+    ST_OK(v.kx.alloc(v.N));
+    ST_OK(v.ky.alloc(v.N));
+    ST_OK(v.nbx.alloc(v.N));
+    ST_OK(v.nby.alloc(v.N));
+    ST_OK(v.kkx.alloc(v.N));
+    ST_OK(v.kky.alloc(v.N));
+    ST_OK(v.dx.alloc(v.N));
+    ST_OK(v.dy.alloc(v.N));
+    ST_OK(v.ux.alloc(v.N));
+    ST_OK(v.uy.alloc(v.N));
+    ST_OK(v.fx.alloc(v.N));
+    ST_OK(v.fy.alloc(v.N));
+    ST_OK(v.rx.alloc(v.N));
+    ST_OK(v.ry.alloc(v.N));
+    HIP_OK(hipMemcpy(v.kx.p, v.hkx.data(), v.N, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(v.ky.p, v.hky.data(), v.N, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(v.nbx.p, v.hnbx.data(), v.N * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(v.nby.p, v.hnby.data(), v.N * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(v.kkx.p, v.hkkx.data(), v.N * sizeof(T), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(v.kky.p, v.hkky.data(), v.N * sizeof(T), hipMemcpyHostToDevice));
+  }
+  gfi.resize(nl);
+  gfj.resize(nl);
+  for (int l = 0; l + 1 < nl; l++) {
+    const GeoLevel &gf = g[l], &gc = g[l + 1];
+    ST_OK(gfi[l].alloc(gf.fi.size()));
+    ST_OK(gfj[l].alloc(gf.fj.size()));
+    HIP_OK(hipMemcpy(gfi[l].p, gf.fi.data(), gf.fi.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(gfj[l].p, gf.fj.data(), gf.fj.size() * sizeof(int), hipMemcpyHostToDevice));
+    auto mids = [](const std::vector<double> &a) {
+      std::vector<double> m;
+      for (size_t k = 0; k + 1 < a.size(); k++) m.push_back((a[k] + a[k + 1]) / 2.0);
+      return m;
+    };
+    ST_OK(L[l].mxn.build(gf.hx, gc.hx));
+    ST_OK(L[l].mxm.build(mids(gf.hx), mids(gc.hx)));
+    ST_OK(L[l].myn.build(gf.hy, gc.hy));
+    ST_OK(L[l].mym.build(mids(gf.hy), mids(gc.hy)));
+  }
+  nc = 2 * L[nl - 1].N;
+  ST_OK(dense.alloc((size_t)nc * 2 * nc));
+  ST_OK(ainv.alloc((size_t)nc * nc));
+  return I2E_OK;
+}
+
+template <class T>
+int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, const double *gp, double mu_, double dt_,
+                      hipStream_t s) {
+  mu = mu_;
+  dt = dt_;
+  const int nl = (int)L.size();
+  const int N0 = L[0].N;
+  hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, nu, L[0].nu.p);
+  hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, nd, L[0].nd.p);
+  hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, ro, L[0].ro.p);
+  hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, gp, L[0].gp.p);
+  for (int l = 0; l + 1 < nl; l++) {
+    VelLevel<T> &f = L[l], &c = L[l + 1];
+    hipLaunchKernelGGL(k_restrict_node<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, f.Y, view(f.mxn), view(f.myn),
+                       f.nu.p, c.nu.p, 1);
+    hipLaunchKernelGGL(k_restrict_node<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, f.Y, view(f.mxn), view(f.myn),
+                       f.ro.p, c.ro.p, 0);
+    hipLaunchKernelGGL(k_restrict_cell<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, f.Y, gfi[l].p, gfj[l].p, f.nd.p,
+                       c.nd.p, 1);
+    hipLaunchKernelGGL(k_restrict_cell<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, f.Y, gfi[l].p, gfj[l].p, f.gp.p,
+                       c.gp.p, 0);
+  }
+  for (int l = 0; l < nl; l++)
+    hipLaunchKernelGGL(k_vdiag<T>, grd2(L[l].X, L[l].Y), blk2(), 0, s, lev(l), L[l].dx.p, L[l].dy.p);
+  const int n = nc;
+  hipLaunchKernelGGL(k_vdense<T>, dim3(nblk(n, 256), n), dim3(256), 0, s, lev(nl - 1), n, dense.p);
+  hipLaunchKernelGGL(k_gauss_jordan<T>, dim3(1), dim3(1024), 0, s, n, dense.p, ainv.p);
+  HIP_OK(hipGetLastError());
+  return I2E_OK;
+}
+
+template <class T>
+int VelMG<T>::vcycle(int l, hipStream_t s) {
+  const int nl = (int)L.size();
+  VelLevel<T> &v = L[l];
+  if (l == nl - 1) {
+    hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(nc, 64)), dim3(64), 0, s, nc, v.N, ainv.p, v.fx.p, v.fy.p, v.ux.p,
+                       v.uy.p);
+    return I2E_OK;
+  }
+  const dim3 G = grd2(v.X, v.Y), B = blk2();
+  const VLev<T> o = lev(l);
+  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.ux.p);
+  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.uy.p);
+  for (int it = 0; it < nsmooth; it++)
+    for (int comp = 1; comp <= 2; comp++)
+      for (int col = 0; col < 2; col++)
+        hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
+  hipLaunchKernelGGL(k_vresid<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
+  VelLevel<T> &c = L[l + 1];
+  hipLaunchKernelGGL(k_vrestrict<T>, grd2(c.X, c.Y), B, 0, s, c.X, c.Y, v.Y, view(v.mxn), view(v.mxm), view(v.myn),
+                     view(v.mym), c.kx.p, c.ky.p, v.rx.p, v.ry.p, c.fx.p, c.fy.p);
+  ST_OK(vcycle(l + 1, s));
+  hipLaunchKernelGGL(k_vprolong<T>, G, B, 0, s, v.X, v.Y, c.Y, view(v.mxn), view(v.mxm), view(v.myn), view(v.mym),
+                     v.kx.p, v.ky.p, c.ux.p, c.uy.p, v.ux.p, v.uy.p);
+  for (int it = 0; it < nsmooth; it++)
+    for (int comp = 2; comp >= 1; comp--)
+      for (int col = 1; col >= 0; col--)
+        hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
+  return I2E_OK;
+}
+
+template <class T>
+void VelMG<T>::release() {
+  for (auto &v : L) {
+    v.nu.release(); v.nd.release(); v.ro.release(); v.gp.release();
+    v.kx.release(); v.ky.release(); v.nbx.release(); v.nby.release(); v.kkx.release(); v.kky.release();
+    v.dx.release(); v.dy.release(); v.ux.release(); v.uy.release(); v.fx.release(); v.fy.release();
+    v.rx.release(); v.ry.release();
+  }
+  for (auto &b : gfi) b.release();
+  for (auto &b : gfj) b.release();
+  dense.release();
+  ainv.release();
+}
+
+// ---------------------------------------------------------------- scalar MG
+template <class T>
+POp<T> ScalMG<T>::op(int l) const {
+  const ScalLevel<T> &v = L[l];
+  return POp<T>{v.X, v.Y, v.cx.p, v.cy.p, v.k.p, v.c.p, v.mask.p, v.dm.p};
+}
+
+template <class T>
+struct FSplit {
+  const T *u;
+  int Y;
+  __device__ __forceinline__ T operator()(int i, int j) const { return u[i * Y + j]; }
+};
+template <class T>
+struct FUnit {
+  int idx, Y;
+  __device__ __forceinline__ T operator()(int i, int j) const { return (i * Y + j == idx) ? T(1) : T(0); }
+};
+
+template <class T>
+__global__ void k_pdiag(POp<T> o, T *d) {
+  IDX2;
+  if (i >= o.X || j >= o.Y) return;
+  d[i * o.Y + j] = prow(o, FUnit<T>{i * o.Y + j, o.Y}, i, j);
+}
+template <class T>
+__global__ void k_pgs(POp<T> o, int color, T *u, const T *f, const T *d) {
+  IDX2;
+  if (i >= o.X || j >= o.Y || ((i + j) & 1) != color) return;
+  const int m = i * o.Y + j;
+  u[m] += (f[m] - prow(o, FSplit<T>{u, o.Y}, i, j)) / d[m];
+}
+template <class T>
+__global__ void k_presid(POp<T> o, const T *u, const T *f, T *r) {
+  IDX2;
+  if (i >= o.X || j >= o.Y) return;
+  const int m = i * o.Y + j;
+  r[m] = o.mask[m] ? T(0) : f[m] - prow(o, FSplit<T>{u, o.Y}, i, j);
+}
+template <class T>
+__global__ void k_prestrict(int Xc, int Yc, int Yf, MapView mx, MapView my, const int8_t *cmask, const T *r, T *f) {
+  IDX2;
+  if (i >= Xc || j >= Yc) return;
+  double s = 0, ws = 0;
+  if (!cmask[i * Yc + j])
+    for (int k = mx.rb[i]; k < mx.re[i]; k++) {
+      double wx = wgt(mx, k, i);
+      if (wx == 0) continue;
+      for (int l = my.rb[j]; l < my.re[j]; l++) {
+        double w = wx * wgt(my, l, j);
+        if (w == 0) continue;
+        s += w * (double)r[k * Yf + l];
+        ws += w;
+      }
+    }
+  f[i * Yc + j] = T(ws > 0 ? s / ws : 0.0);
+}
+template <class T>
+__global__ void k_pprolong(int Xf, int Yf, int Yc, MapView mx, MapView my, const int8_t *fmask, const T *c, T *u) {
+  IDX2;
+  if (i >= Xf || j >= Yf) return;
+  const int m = i * Yf + j;
+  if (fmask[m]) return;
+  int a = mx.i0[i], b = my.i0[j];
+  double wa = mx.w[i], wb = my.w[j];
+  double v = (1 - wa) * ((1 - wb) * (double)c[a * Yc + b] + wb * (double)c[a * Yc + b + 1]) +
+             wa * ((1 - wb) * (double)c[(a + 1) * Yc + b] + wb * (double)c[(a + 1) * Yc + b + 1]);
+  u[m] += T(v);
+}
+template <class T>
+__global__ void k_pdense(POp<T> o, int n, double *aug) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (c >= n) return;
+  aug[(size_t)r * 2 * n + c] = (double)prow(o, FUnit<T>{c, o.Y}, r / o.Y, r % o.Y);
+  aug[(size_t)r * 2 * n + n + c] = (r == c) ? 1.0 : 0.0;
+}
+// coarse mask by injection, coarse dm = 1 (homogeneous rows)
+template <class T>
+__global__ void k_pmask_inject(int Xc, int Yc, int Yf, const int *fi, const int *fj, const int8_t *fm, int8_t *cm,
+                               T *cdm) {
+  IDX2;
+  if (i >= Xc || j >= Yc) return;
+  cm[i * Yc + j] = fm[fi[i] * Yf + fj[j]];
+  cdm[i * Yc + j] = T(1);
+}
+__global__ void k_copy_mask(int n, const int8_t *a, int8_t *b) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = a[t];
+}
+
+template <class T>
+int ScalMG<T>::setup(const std::vector<double> &x0, const std::vector<double> &y0, int coarse_max) {
+  L.clear();
+  std::vector<std::vector<double>> xs{x0}, ys{y0};
+  std::vector<std::vector<int>> fis, fjs;
+  while (true) {
+    int X = (int)xs.back().size(), Y = (int)ys.back().size();
+    if (std::min(X, Y) - 1 <= coarse_max || std::min(X, Y) < 9) break;
+    std::vector<int> ci = coarse_lines(X), cj = coarse_lines(Y);
+    std::vector<double> cx, cy;
+    for (int a : ci) cx.push_back(xs.back()[a]);
+    for (int a : cj) cy.push_back(ys.back()[a]);
+    fis.push_back(ci);
+    fjs.push_back(cj);
+    xs.push_back(cx);
+    ys.push_back(cy);
+  }
+  const int nl = (int)xs.size();
+  L.resize(nl);
+  sfi.resize(nl);
+  sfj.resize(nl);
+  for (int l = 0; l < nl; l++) {
+    ScalLevel<T> &v = L[l];
+    v.X = (int)xs[l].size();
+    v.Y = (int)ys[l].size();
+    v.N = v.X * v.Y;
+    v.hcx = xs[l];
+    v.hcy = ys[l];
+    ST_OK(v.cx.alloc(v.X));
+    ST_OK(v.cy.alloc(v.Y));
+    HIP_OK(hipMemcpy(v.cx.p, v.hcx.data(), v.X * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(v.cy.p, v.hcy.data(), v.Y * sizeof(double), hipMemcpyHostToDevice));
+    ST_OK(v.k.alloc(v.N));
+    ST_OK(v.c.alloc(v.N));
+    ST_OK(v.dm.alloc(v.N));
+    ST_OK(v.diag.alloc(v.N));
+    ST_OK(v.mask.alloc(v.N));
+    ST_OK(v.u.alloc(v.N));
+    ST_OK(v.f.alloc(v.N));
+    ST_OK(v.r.alloc(v.N));
+    if (l + 1 < nl) {
+      ST_OK(v.mx.build(xs[l], xs[l + 1]));
+      ST_OK(v.my.build(ys[l], ys[l + 1]));
+      ST_OK(sfi[l].alloc(fis[l].size()));
+      ST_OK(sfj[l].alloc(fjs[l].size()));
+      HIP_OK(hipMemcpy(sfi[l].p, fis[l].data(), fis[l].size() * sizeof(int), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(sfj[l].p, fjs[l].data(), fjs[l].size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+  }
+  nc = L[nl - 1].N;
+  ST_OK(dense.alloc((size_t)nc * 2 * nc));
+  ST_OK(ainv.alloc((size_t)nc * nc));
+  return I2E_OK;
+}
+
+template <class T>
+int ScalMG<T>::refresh(const double *k, const double *c, const int8_t *mask, const double *dm, hipStream_t s) {
+  const int nl = (int)L.size();
+  const int N0 = L[0].N;
+  hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, k, L[0].k.p);
+  hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, c, L[0].c.p);
+  hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, dm, L[0].dm.p);
+  hipLaunchKernelGGL(k_copy_mask, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, mask, L[0].mask.p);
+  for (int l = 0; l + 1 < nl; l++) {
+    ScalLevel<T> &f = L[l], &cc = L[l + 1];
+    hipLaunchKernelGGL(k_restrict_node<T>, grd2(cc.X, cc.Y), blk2(), 0, s, cc.X, cc.Y, f.Y, view(f.mx), view(f.my),
+                       f.k.p, cc.k.p, 0);
+    hipLaunchKernelGGL(k_restrict_node<T>, grd2(cc.X, cc.Y), blk2(), 0, s, cc.X, cc.Y, f.Y, view(f.mx), view(f.my),
+                       f.c.p, cc.c.p, 0);
+    hipLaunchKernelGGL(k_pmask_inject<T>, grd2(cc.X, cc.Y), blk2(), 0, s, cc.X, cc.Y, f.Y, sfi[l].p, sfj[l].p,
+                       f.mask.p, cc.mask.p, cc.dm.p);
+  }
+  for (int l = 0; l < nl; l++) hipLaunchKernelGGL(k_pdiag<T>, grd2(L[l].X, L[l].Y), blk2(), 0, s, op(l), L[l].diag.p);
+  hipLaunchKernelGGL(k_pdense<T>, dim3(nblk(nc, 256), nc), dim3(256), 0, s, op(nl - 1), nc, dense.p);
+  hipLaunchKernelGGL(k_gauss_jordan<T>, dim3(1), dim3(1024), 0, s, nc, dense.p, ainv.p);
+  HIP_OK(hipGetLastError());
+  return I2E_OK;
+}
+
+template <class T>
+int ScalMG<T>::vcycle(int l, hipStream_t s) {
+  const int nl = (int)L.size();
+  ScalLevel<T> &v = L[l];
+  if (l == nl - 1) {
+    hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(nc, 64)), dim3(64), 0, s, nc, nc, ainv.p, v.f.p, v.f.p, v.u.p,
+                       v.u.p);
+    return I2E_OK;
+  }
+  const dim3 G = grd2(v.X, v.Y), B = blk2();
+  const POp<T> o = op(l);
+  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.u.p);
+  for (int it = 0; it < nsmooth; it++)
+    for (int col = 0; col < 2; col++) hipLaunchKernelGGL(k_pgs<T>, G, B, 0, s, o, col, v.u.p, v.f.p, v.diag.p);
+  hipLaunchKernelGGL(k_presid<T>, G, B, 0, s, o, v.u.p, v.f.p, v.r.p);
+  ScalLevel<T> &c = L[l + 1];
+  hipLaunchKernelGGL(k_prestrict<T>, grd2(c.X, c.Y), B, 0, s, c.X, c.Y, v.Y, view(v.mx), view(v.my), c.mask.p, v.r.p,
+                     c.f.p);
+  ST_OK(vcycle(l + 1, s));
+  hipLaunchKernelGGL(k_pprolong<T>, G, B, 0, s, v.X, v.Y, c.Y, view(v.mx), view(v.my), v.mask.p, c.u.p, v.u.p);
+  for (int it = 0; it < nsmooth; it++)
+    for (int col = 1; col >= 0; col--) hipLaunchKernelGGL(k_pgs<T>, G, B, 0, s, o, col, v.u.p, v.f.p, v.diag.p);
+  return I2E_OK;
+}
+
+template <class T>
+void ScalMG<T>::release() {
+  for (auto &v : L) {
+    v.cx.release(); v.cy.release(); v.k.release(); v.c.release(); v.dm.release(); v.diag.release();
+    v.mask.release(); v.u.release(); v.f.release(); v.r.release();
+  }
+  for (auto &b : sfi) b.release();
+  for (auto &b : sfj) b.release();
+  dense.release();
+  ainv.release();
+}
+
+template struct VelMG<float>;
+template struct VelMG<double>;
+template struct ScalMG<float>;
+template struct ScalMG<double>;
+
+}  // namespace i2e

@@ -1,0 +1,145 @@
+// mg.hpp -- geometric multigrid preconditioners (velocity block of Stokes,
+// scalar heat/gravity), templated on the arithmetic type.
+//
+// Coarse grids keep every second grid line of the finer grid plus the last
+// one (so odd cell counts such as 125 work, at the price of one narrow last
+// cell), operators are rediscretised with restricted fields (geometric mean
+// for viscosity, arithmetic for density / potential / conductivity), the
+// smoother is red-black Gauss-Seidel (per velocity component), transfers are
+// bilinear interpolation and its normalised transpose, and the coarsest grid
+// is solved with an explicit dense inverse.
+#pragma once
+#include <vector>
+
+#include "ops.hpp"
+
+namespace i2e {
+
+// device buffer
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    if (count <= n && p) return I2E_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (!count) return I2E_OK;
+    HIP_OK(hipMalloc((void **)&p, count * sizeof(T)));
+    n = count;
+    return I2E_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// 1D interpolation map fine -> coarse coordinate sets
+struct Map1 {
+  int nf = 0, nc = 0;
+  DBuf<int> i0;      // [nf]
+  DBuf<double> w;    // [nf] weight of i0+1
+  DBuf<int> rb, re;  // [nc] fine index range touching coarse index
+  int build(const std::vector<double> &cf, const std::vector<double> &cc);
+};
+
+struct MapView {
+  const int *i0;
+  const double *w;
+  const int *rb, *re;
+};
+inline MapView view(const Map1 &m) { return MapView{m.i0.p, m.w.p, m.rb.p, m.re.p}; }
+
+// level geometry shared by all solvers on one grid hierarchy
+struct GeoLevel {
+  int X, Y;
+  std::vector<double> hx, hy;   // host copies of the lines
+  DBuf<double> gx, gy;          // device lines
+  std::vector<int> fi, fj;      // fine line index of each coarse line (to the next finer level)
+};
+
+// ---------------------------------------------------------------- velocity MG
+template <class T>
+struct VLev {
+  SOp<T> o;
+  const int8_t *kx, *ky;  // 0 stencil row, 1 BC row, 2 ghost row
+  const int *nbx, *nby;   // BC neighbour (same component), -1 none
+  const T *kkx, *kky;     // BC coefficient
+};
+
+template <class T>
+struct VelLevel {
+  int X, Y, N;
+  DBuf<T> nu, nd, ro, gp;
+  DBuf<int8_t> kx, ky;
+  DBuf<int> nbx, nby;
+  DBuf<T> kkx, kky;
+  DBuf<T> dx, dy;           // diagonals
+  DBuf<T> ux, uy, fx, fy, rx, ry;
+  Map1 mxn, mxm, myn, mym;  // maps to the next coarser level: x nodes/mids, y nodes/mids
+  std::vector<int8_t> hkx, hky;
+  std::vector<int> hnbx, hnby;
+  std::vector<T> hkkx, hkky;
+};
+
+template <class T>
+struct VelMG {
+  std::vector<VelLevel<T>> L;
+  const std::vector<GeoLevel> *geo = nullptr;
+  int nc = 0;               // coarsest unknown count (2 N)
+  DBuf<double> dense;       // n x 2n work
+  DBuf<T> ainv;             // n x n
+  std::vector<DBuf<int>> gfi, gfj;  // fine line index of each coarse line, per level
+  double mu = 0, dt = 0;
+  int nsmooth = 2;
+  int setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,
+            const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,
+            const std::vector<double> &kky0);
+  // restrict fields from the finest level (FP64 node arrays) and refresh
+  // diagonals and the coarse inverse
+  int refresh(const double *nu, const double *nd, const double *ro, const double *gp, double mu, double dt,
+              hipStream_t s);
+  // z = M^-1 r for the velocity block; f/u live in L[0].fx/fy and L[0].ux/uy
+  int vcycle(int l, hipStream_t s);
+  VLev<T> lev(int l) const;
+  void release();
+};
+
+// ---------------------------------------------------------------- scalar MG
+template <class T>
+struct ScalLevel {
+  int X, Y, N;
+  DBuf<double> cx, cy;   // point coordinates (device)
+  std::vector<double> hcx, hcy;
+  DBuf<T> k, c, dm, diag;
+  DBuf<int8_t> mask;
+  std::vector<int8_t> hmask;
+  DBuf<T> u, f, r;
+  Map1 mx, my;
+};
+
+template <class T>
+struct ScalMG {
+  std::vector<ScalLevel<T>> L;
+  int nc = 0;
+  DBuf<double> dense;
+  DBuf<T> ainv;
+  std::vector<DBuf<int>> sfi, sfj;
+  int nsmooth = 2;
+  // coords of the finest points; fi/fj give the coarse point selection
+  int setup(const std::vector<double> &x0, const std::vector<double> &y0, int coarse_max);
+  // finest-level fields: k, c, mask, dm (FP64 device arrays, length N0)
+  int refresh(const double *k, const double *c, const int8_t *mask, const double *dm, hipStream_t s);
+  int vcycle(int l, hipStream_t s);
+  POp<T> op(int l) const;
+  void release();
+};
+
+// coarse line selection: every second line plus the last
+std::vector<int> coarse_lines(int n);
+int build_geometry(const double *gx, const double *gy, int X, int Y, int coarse_max, std::vector<GeoLevel> &geo);
+
+}  // namespace i2e

@@ -1,0 +1,27 @@
+// ops.hpp -- FP64 node kernels (exact operators, right-hand sides, reductions)
+#pragma once
+#include "kernels.hpp"
+#include "rows.hpp"
+
+namespace i2e {
+
+struct GravGeo {
+  double R, gxkoef, mu;
+};
+
+__global__ void k_stokes_apply(SOp<double> o, BcTab bc, const double *x, double *y);
+__global__ void k_stokes_rhs(SOp<double> o, BcTab bc, double *b);
+__global__ void k_stokes_pack(int X, int Y, const double *pr, const double *vx, const double *vy, double *x);
+__global__ void k_stokes_unpack(int X, int Y, const double *x, double *pr, double *vx, double *vy);
+__global__ void k_strain(Grid g, NodeF F);
+__global__ void k_maxvel(int X, int Y, const double *vx, const double *vy, double *out);
+__global__ void k_grav_mean(Grid g, const double *ro, double R, double *part);
+__global__ void k_grav_apply(Grid g, GravGeo q, const double *x, double *y);
+__global__ void k_grav_rhs(Grid g, GravGeo q, const double *ro, double *b);
+__global__ void k_grav_ga(Grid g, const double *gp, double *ga);
+__global__ void k_heat_source(const i2e_params *p, Grid g, NodeF F, BcTab bc, double timesum, double *sol1,
+                              double *out);
+__global__ void k_heat_apply(Grid g, NodeF F, BcTab bc, double dt, const double *x, double *y);
+__global__ void k_heat_rhs(Grid g, NodeF F, BcTab bc, double dt, const double *sol1, double *rhs);
+
+}  // namespace i2e

@@ -1,0 +1,207 @@
+/*
+ * i2e.h -- C-ABI of the MI355X-native I2ELVIS marker-in-cell time step.
+ *
+ * The five step functions replace the reference's serial C entry points
+ * (prototypes in headmart.c:321-366, called in fixed order by main() in
+ * i2mart.c:60-164):
+ *
+ *   i2e_maxvelstep  <- void maxvelstep(void)   movemart.c:1225-1262
+ *   i2e_viterate    <- void viterate(int)      movemart.c:603-1217
+ *                      (+ the three gausmat4(0,..) PARDISO solves it makes,
+ *                       gausmart.c:174-318, replaced by matrix-free solvers)
+ *   i2e_titerate    <- void titerate(int)      heatmart.c:320-695
+ *   i2e_movemark    <- void movemark(void)     markmart.c:3-188
+ *   i2e_ronurecalc  <- void ronurecalc(void)   markmart.c:192-1019
+ *                      (+ tkrecalc/gausmat3,   heatmart.c:758-826)
+ *
+ * The reference keeps all state in global arrays (headmart.c:38-167); here the
+ * caller owns host arrays described by i2e_host_state, and a context owns the
+ * device copies.  Index maps are the reference's: node m = m1*ynumy + m2 (y
+ * fastest); Stokes unknown 3m+{0:P,1:Vx,2:Vy}; temperature boundary slot
+ * 3N+m (movemart.c:800, heatmart.c:355).  Markers are float32 SoA plus a
+ * signed char type, nodes are FP64 (headmart.c:87-127).
+ *
+ * Every function returns an int status (I2E_OK == 0).  Nothing here exits the
+ * process; a caller that wants the reference's exit() behaviour maps the codes.
+ */
+#ifndef I2E_H
+#define I2E_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  I2E_OK = 0,
+  I2E_ERR_ARG = 1,          /* bad argument / inconsistent sizes            */
+  I2E_ERR_CAPACITY = 2,     /* size above a compiled-in maximum             */
+  I2E_ERR_HIP = 3,          /* a HIP runtime call failed                    */
+  I2E_ERR_NOCONV = 4,       /* a Krylov solve missed its tolerance          */
+  I2E_ERR_UNSUPPORTED = 5,  /* input valid for the reference, not handled   */
+  I2E_ERR_NODEVICE = 6      /* no GPU visible                               */
+};
+
+#define I2E_MAXTYPES 64     /* rock types (reference MAXTMR=200, 0..63 used) */
+
+/* ---- per-rock-type constants (reference: marknu..markkp, headmart.c:127-131,
+ *      read from the ROCKS table of init.t3c) --------------------------------*/
+typedef struct i2e_rock {
+  double nu0;   /* marknu: viscosity prefactor, Pa s                        */
+  double dh;    /* markdh: activation energy, J/mol                         */
+  double dv;    /* markdv: activation volume, J/bar                         */
+  double ss;    /* markss: transition stress (0 = Newtonian)                */
+  double mm;    /* markmm: stress exponent                                  */
+  double n0;    /* markn0: lower viscosity limit; molten types return this  */
+  double n1;    /* markn1: upper viscosity limit                            */
+  double ro;    /* markro: reference density, kg/m^3                        */
+  double aa;    /* markaa: compressibility term, 1/kbar                     */
+  double bb;    /* markbb: thermal expansion, 1/K                           */
+  double cp;    /* markcp: heat capacity, J/kg/K                            */
+  double kt;    /* markkt: conductivity, W/m/K                              */
+  double kf;    /* markkf: conductivity T-term                              */
+  double kp;    /* markkp: conductivity P-term                              */
+} i2e_rock;
+
+/* ---- run parameters (mode.t3c GENERAL/STOKES/HEAT blocks, loadmart.c:26-90,
+ *      init.t3c header, in2mart.c:67-125) -----------------------------------*/
+typedef struct i2e_params {
+  int64_t xnumx, ynumy;       /* grid lines                                 */
+  int64_t mnumx, mnumy;       /* markers per cell (lattice), for mardx      */
+  double xsize, ysize;        /* box size, m                                */
+  /* time-step controls (fl0stp, loadmart.c:44-46; maxtmstep in seconds)     */
+  double maxxystep, maxtkstep, maxtmstep;
+  /* STOKES_CONTINUITY_PARAMETERS                                            */
+  double nubeg, nuend, nucontr, strmin;
+  /* HEAT_TRANSFER_PARAMETERS                                                */
+  double heatdif, frictyn, adiabyn;
+  int32_t heatmod, densimod;
+  int32_t outgrid, markmod;   /* markmod 2 = RK4 (movemark, markmart.c:77)  */
+  /* planet                                                                  */
+  double gykoef;              /* GYKOEF: planet radius for gravity, m       */
+  double tmp_ambient;         /* sticky-air temperature, K                  */
+  double al2627_init;         /* 26Al/27Al (absolute, e.g. 5.25e-5)         */
+  double fe6056_init;         /* 60Fe/56Fe (absolute)                       */
+  double h0_scale;            /* multiplies the 26Al heating (1 = reference) */
+  double longlived;           /* 1 = add 40K/235U/238U/232Th terms           */
+  double por_init;            /* reference por_init (porosity before sinter) */
+  double k_cutoff;            /* conductivity upper cut-off                  */
+  double radmax_frac;         /* eta=1e22 beyond radmax_frac*xsize/2 (0.95)  */
+  double eta_outer;           /* that viscosity (1e22)                       */
+  /* partial-melting closure standing in for tdbasecalc's type switch
+   * (markmart.c:2178-2182, meltpart1 linear branch markmart.c:2599-2613):
+   * silicate types in melt_types[] switch to type+20 when the linear melt
+   * fraction (T-Ts)/(Tl-Ts) is positive, and back when it is not; above
+   * melt_switch the viscosity is melt_eta (clamped to [nubeg,nuend]), and
+   * inside the melting interval cp gains melt_hlat*dX/dT (central difference
+   * with 1 K steps, as markmart.c:2212-2219).                                */
+  double melt_ts, melt_tl, melt_switch, melt_eta, melt_hlat;
+  int32_t melt_on, porosity_on;
+  int32_t ntypes;
+  int32_t pad0;
+  i2e_rock rock[I2E_MAXTYPES];
+  uint8_t is_silicate[I2E_MAXTYPES];  /* radiogenic 26Al carrier (5,6,25,26) */
+  uint8_t is_iron[I2E_MAXTYPES];      /* 60Fe carrier (7-10,17-19)            */
+  uint8_t porous[I2E_MAXTYPES];       /* porosity closures (5,6)              */
+  uint8_t meltable[I2E_MAXTYPES];     /* melt switch applies                  */
+} i2e_params;
+
+/* ---- host state: the reference globals as raw pointers ------------------ */
+typedef struct i2e_host_state {
+  double *gx, *gy;                    /* grid lines [xnumx], [ynumy]          */
+  /* node fields, N = xnumx*ynumy each                                       */
+  double *vx, *vy, *pr, *gp, *ga;
+  double *exx, *eyy, *exy, *sxx, *syy, *sxy;
+  double *nu, *nd, *ro, *tk, *cp, *kt, *ht, *et;
+  /* markers                                                                 */
+  int64_t marknum;
+  float *markx, *marky, *markk, *markv, *markp, *markpor, *marktmax;
+  int8_t *markt;
+  /* boundary-condition tables in the reference layout (headmart.c:95-97):
+   * bondm[4N] -> row of bondv/bondn (0 = no BC), bondn[bondnum][3] holds
+   * 1-based unknown numbers of up to three neighbours (0 = none),
+   * bondv[bondnum][4] = {const, k1, k2, k3}: x = const + sum k_i x_nbr_i.  */
+  int64_t *bondm, *bondn;
+  double *bondv;
+  int64_t bondnum;
+  /* scalars                                                                 */
+  double timestep, timesum, gxkoef;
+} i2e_host_state;
+
+/* ---- solver controls ---------------------------------------------------- */
+enum { I2E_PREC_FP64 = 0, I2E_PREC_MIXED = 1 };
+
+typedef struct i2e_solver_opts {
+  int32_t precision;       /* I2E_PREC_FP64 or I2E_PREC_MIXED                 */
+  int32_t restart;         /* FGMRES restart length                          */
+  int32_t maxit;           /* max outer iterations per solve                 */
+  int32_t nsmooth;         /* smoothing sweeps per level (pre = post)        */
+  double stokes_rtol;      /* ||b-Ax||/||b|| target for Stokes               */
+  double heat_rtol;        /* same for heat                                  */
+  double grav_rtol;        /* same for gravity                               */
+  double schur_scale;      /* Schur diagonal = schur_scale * eta_cell / mukoef */
+  double omega;            /* smoother damping                               */
+  int32_t coarse_max;      /* coarsest grid: cells per direction <= this     */
+  int32_t pad1;
+} i2e_solver_opts;
+
+typedef struct i2e_stats {
+  int32_t stokes_iters, heat_iters, grav_iters, pad;
+  double stokes_relres, heat_relres, grav_relres;
+  double numin, numax;      /* viscosity range after ronurecalc           */
+  double ms_grav, ms_stokes, ms_heat, ms_markers;
+} i2e_stats;
+
+typedef struct i2e_ctx i2e_ctx;
+
+/* Context: one HIP stream, device copies of every field.  max_markers bounds
+ * marker storage (markers can only leave through the out-of-grid path).     */
+int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out);
+int i2e_destroy(i2e_ctx *c);
+int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o);
+int i2e_default_solver(i2e_solver_opts *o);
+/* Host <-> device.  upload copies everything the step reads (grid, nodes,
+ * markers, BC tables, scalars); download copies everything it writes.       */
+int i2e_upload(i2e_ctx *c, const i2e_host_state *s);
+int i2e_download(i2e_ctx *c, i2e_host_state *s);
+
+/* The reference's step functions, on device-resident state. */
+int i2e_maxvelstep(i2e_ctx *c, double *timestep);        /* movemart.c:1225 */
+int i2e_viterate(i2e_ctx *c, int n0);                    /* movemart.c:603  */
+int i2e_titerate(i2e_ctx *c, int n0);                    /* heatmart.c:320  */
+int i2e_movemark(i2e_ctx *c);                            /* markmart.c:3    */
+int i2e_ronurecalc(i2e_ctx *c);                          /* markmart.c:192  */
+/* One full step in i2mart.c:60-153 order (maxvelstep, viterate, maxvelstep,
+ * titerate, movemark, ronurecalc, timesum += timestep).                     */
+int i2e_step(i2e_ctx *c);
+int i2e_get_scalars(i2e_ctx *c, double *timestep, double *timesum, double *gxkoef);
+int i2e_set_scalars(i2e_ctx *c, double timestep, double timesum);
+int i2e_get_stats(i2e_ctx *c, i2e_stats *st);
+/* Synchronise the context's stream. */
+int i2e_sync(i2e_ctx *c);
+/* Last error text (static storage, never NULL). */
+const char *i2e_last_error(void);
+
+/* ---- kernel-level entry points used by parity tests and the benchmark ---- */
+/* y = K x with K the reference's assembled Stokes matrix (gausmat4 rows of
+ * viterate, movemart.c:789-967); x, y are 3N interleaved device arrays.     */
+int i2e_stokes_apply_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
+/* Same for the gravity rows (movemart.c:661-749) and heat rows
+ * (heatmart.c:406-432, needs titerate's RHS state); N-length arrays.        */
+int i2e_gravity_apply_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
+int i2e_heat_apply_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
+/* Device pointers of the context's node/marker arrays, by reference name
+ * ("vx", "tk", "markx", ...); NULL if unknown.                              */
+void *i2e_device_ptr(i2e_ctx *c, const char *name);
+/* Advection kernel alone, timed with HIP events on the context's stream:
+ * runs movemark's marker loop `reps` times from the same start positions
+ * and returns the mean kernel milliseconds.                                 */
+int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g,
+                     double *ms_stokes_apply);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* I2E_H */

@@ -1,0 +1,278 @@
+"""Python driver of the CPU FP64 oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, to check or time the HIP library against it.  It calls the C
+restatement in oracle/i2e_oracle.c for everything except the three sparse
+solves per step, which it does like the reference: a direct solve of the
+assembled system (gausmart.c:174-318).  With Intel MKL's runtime present the
+solve is PARDISO with the reference's iparm settings (mtype 11, minimum
+degree, MPS scaling, 1e-13 pivot perturbation, <= 20 refinement steps);
+otherwise scipy's SuperLU.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from i2elvis_planet_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "i2e_oracle.c")
+LIB = os.path.join(HERE, "liborc.so")
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+        subprocess.check_call(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+                               "-o", LIB + ".tmp", SRC, "-lm"])
+        os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+class Csr(C.Structure):
+    _fields_ = [("n", C.c_int64), ("cap", C.c_int64), ("nnz", C.c_int64),
+                ("ia", C.POINTER(C.c_int32)), ("ja", C.POINTER(C.c_int32)),
+                ("a", C.POINTER(C.c_double)), ("b", C.POINTER(C.c_double))]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB)
+        P = C.POINTER(abi.Params)
+        S = C.POINTER(abi.HostStateC)
+        Cp = C.POINTER(Csr)
+        dp = C.POINTER(C.c_double)
+        for name, args in [
+            ("orc_maxvelstep", [P, S, dp]), ("orc_movemark", [P, S]), ("orc_ronurecalc", [P, S, dp]),
+            ("orc_gravity_build", [P, S, Cp]), ("orc_stokes_build", [P, S, Cp]),
+            ("orc_heat_prepare", [P, S, dp, dp]), ("orc_heat_build", [P, S, dp, Cp]),
+            ("orc_tbc_build", [P, S, Cp]), ("orc_heat_markers", [P, S, dp]),
+            ("orc_type_counts", [P, S, C.c_int, C.POINTER(C.c_int32)]),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        for name, args in [("orc_gravity_post", [P, S, dp]), ("orc_stokes_post", [P, S, dp]),
+                           ("orc_allinterv", [P, S, C.c_double, C.c_double, dp]),
+                           ("orc_csr_matvec", [Cp, dp, dp])]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None
+        L.orc_m1serch.argtypes = [dp, C.c_int64, C.c_double]
+        L.orc_m1serch.restype = C.c_int64
+        L.orc_viscalc.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_double]
+        L.orc_viscalc.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _check(st, what):
+    if st != 0:
+        raise RuntimeError("oracle %s failed with status %d" % (what, st))
+
+
+class Matrix:
+    """CSR buffers filled by an orc_*_build call."""
+
+    def __init__(self, n, per_row=32):
+        self.n = n
+        self.ia = np.zeros(n + 1, np.int32)
+        self.ja = np.zeros(n * per_row, np.int32)
+        self.a = np.zeros(n * per_row)
+        self.b = np.zeros(n)
+        self.c = Csr(n, n * per_row, 0, self.ia.ctypes.data_as(C.POINTER(C.c_int32)),
+                     self.ja.ctypes.data_as(C.POINTER(C.c_int32)),
+                     self.a.ctypes.data_as(C.POINTER(C.c_double)),
+                     self.b.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def scipy(self):
+        nnz = int(self.c.nnz)
+        return sp.csr_matrix((self.a[:nnz].copy(), self.ja[:nnz].copy(), self.ia.copy()), shape=(self.n, self.n))
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.n)
+        lib().orc_csr_matvec(C.byref(self.c), x.ctypes.data_as(C.POINTER(C.c_double)),
+                             y.ctypes.data_as(C.POINTER(C.c_double)))
+        return y
+
+
+# --------------------------------------------------------------------------- solves
+_mkl = None
+
+
+def _mkl_lib():
+    global _mkl
+    if _mkl is None:
+        _mkl = False
+        for path in ("/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so", "libmkl_rt.so.1", "libmkl_rt.so.2"):
+            try:
+                _mkl = C.CDLL(path)
+                break
+            except OSError:
+                continue
+    return _mkl or None
+
+
+def pardiso_solve(A, b):
+    """PARDISO phase 13 + release with the reference's iparm (gausmart.c:203-260)."""
+    mkl = _mkl_lib()
+    if mkl is None:
+        return None
+    try:  # 32-bit integer interface, whatever an earlier loader selected
+        mkl.MKL_Set_Interface_Layer(C.c_int(0))
+    except AttributeError:
+        pass
+    n = A.shape[0]
+    ia = (A.indptr + 1).astype(np.int32)
+    ja = (A.indices + 1).astype(np.int32)
+    a = np.ascontiguousarray(A.data, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n)
+    pt = (C.c_void_p * 64)()
+    iparm = (C.c_int * 64)()
+    iparm[0], iparm[1], iparm[2], iparm[7], iparm[9], iparm[10] = 1, 0, 1, 20, 13, 1
+    iparm[17], iparm[18] = -1, -1
+    i = lambda v: C.byref(C.c_int(v))
+    err = C.c_int(0)
+    idum = C.c_int(0)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    for phase in (13, -1):
+        mkl.pardiso(pt, i(1), i(1), i(11), i(phase), i(n), a.ctypes.data_as(dp), ia.ctypes.data_as(ip),
+                    ja.ctypes.data_as(ip), C.byref(idum), i(1), iparm, i(0), b.ctypes.data_as(dp),
+                    x.ctypes.data_as(dp), C.byref(err))
+        if err.value != 0:
+            raise RuntimeError("PARDISO phase %d error %d" % (phase, err.value))
+    return x
+
+
+SOLVER = os.environ.get("I2E_ORACLE_SOLVER", "auto")
+
+
+def solve(M):
+    A = M.scipy()
+    if SOLVER in ("auto", "pardiso"):
+        try:
+            x = pardiso_solve(A, M.b)
+        except RuntimeError:
+            if SOLVER == "pardiso":
+                raise
+            x = None
+        if x is not None:
+            return x
+        if SOLVER == "pardiso":
+            raise RuntimeError("MKL runtime not found")
+    return spla.spsolve(A.tocsc(), M.b)
+
+
+def solver_kind():
+    return "pardiso" if (SOLVER in ("auto", "pardiso") and _mkl_lib() is not None) else "superlu"
+
+
+# --------------------------------------------------------------------------- step
+class Oracle:
+    """The reference's step functions on a HostState (i2mart.c:60-153)."""
+
+    def __init__(self, params, hs):
+        self.p = params
+        self.hs = hs
+        self.L = lib()
+        self.last = {}
+
+    def _s(self):
+        self._cs = self.hs.cstruct()
+        return C.byref(self._cs)
+
+    def _pull(self):
+        self.hs.pull_scalars(self._cs)
+
+    def maxvelstep(self):
+        ts = C.c_double(self.hs.timestep)
+        _check(self.L.orc_maxvelstep(C.byref(self.p), self._s(), C.byref(ts)), "maxvelstep")
+        self.hs.timestep = ts.value
+
+    def gravity(self):
+        M = Matrix(self.hs.N, 12)
+        s = self._s()
+        _check(self.L.orc_gravity_build(C.byref(self.p), s, C.byref(M.c)), "gravity_build")
+        self._pull()
+        x = solve(M)
+        self.last["grav"] = (M, x)
+        self.L.orc_gravity_post(C.byref(self.p), self._s(), x.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def stokes_matrix(self):
+        M = Matrix(3 * self.hs.N, 32)
+        _check(self.L.orc_stokes_build(C.byref(self.p), self._s(), C.byref(M.c)), "stokes_build")
+        return M
+
+    def viterate(self):
+        self.gravity()
+        M = self.stokes_matrix()
+        x = solve(M)
+        self.last["stokes"] = (M, x)
+        self.L.orc_stokes_post(C.byref(self.p), self._s(), x.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def titerate(self):
+        N = self.hs.N
+        sol1 = np.zeros(N)
+        tkts = C.c_double(0)
+        s = self._s()
+        _check(self.L.orc_heat_prepare(C.byref(self.p), s, sol1.ctypes.data_as(C.POINTER(C.c_double)),
+                                       C.byref(tkts)), "heat_prepare")
+        self._pull()
+        self.last["tktimestep"] = tkts.value
+        if self.hs.timestep == 0 or tkts.value == 0:
+            return
+        M = Matrix(N, 12)
+        _check(self.L.orc_heat_build(C.byref(self.p), self._s(), sol1.ctypes.data_as(C.POINTER(C.c_double)),
+                                     C.byref(M.c)), "heat_build")
+        tk2 = self.hs.tk.copy()
+        x = solve(M)
+        self.last["heat"] = (M, x, sol1)
+        self.hs.tk[:] = x
+        _check(self.L.orc_heat_markers(C.byref(self.p), self._s(), tk2.ctypes.data_as(C.POINTER(C.c_double))),
+               "heat_markers")
+
+    def movemark(self):
+        _check(self.L.orc_movemark(C.byref(self.p), self._s()), "movemark")
+
+    def ronurecalc(self):
+        mm = np.zeros(2)
+        _check(self.L.orc_ronurecalc(C.byref(self.p), self._s(), mm.ctypes.data_as(C.POINTER(C.c_double))),
+               "ronurecalc")
+        self.last["numinmax"] = mm.copy()
+        self.tkrecalc()
+
+    def tkrecalc(self):
+        M = Matrix(self.hs.N, 8)
+        _check(self.L.orc_tbc_build(C.byref(self.p), self._s(), C.byref(M.c)), "tbc_build")
+        self.hs.tk[:] = spla.spsolve(M.scipy().tocsc(), M.b)
+
+    def step(self):
+        self.hs.timestep = self.p.maxtmstep
+        self.maxvelstep()
+        self.viterate()
+        self.hs.timestep = self.p.maxtmstep
+        self.maxvelstep()
+        if self.hs.timestep:
+            self.titerate()
+        if self.hs.timestep:
+            self.movemark()
+        self.ronurecalc()
+        self.hs.timesum += self.hs.timestep
+
+    def type_counts(self, ntypes=64):
+        C_ = (self.p.xnumx - 1) * (self.p.ynumy - 1)
+        out = np.zeros(C_ * ntypes, np.int32)
+        _check(self.L.orc_type_counts(C.byref(self.p), self._s(), ntypes, out.ctypes.data_as(C.POINTER(C.c_int32))),
+               "type_counts")
+        return out.reshape(C_, ntypes)
