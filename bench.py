@@ -1,0 +1,152 @@
+"""Benchmark: I2ELVIS time steps per second at 501x501 nodes, 6.25M markers.
+
+One step = the reference's i2mart.c:60-153 sequence (maxvelstep, viterate
+with gravity + Stokes, maxvelstep, titerate, movemark, ronurecalc) on the
+501x501 / 25 markers-per-cell planetesimal (radius 150 km, Arrhenius rheology,
+linear melting, 400 iron blobs) with every array resident in HBM.  The
+mixed-precision solver (FP32 multigrid inside FP64 FGMRES) is the headline;
+the same steps with the FP64 multigrid run from the same state for the
+mixed/FP64 ratio.  Both solve Stokes to ||b - Ax|| / ||b|| <= 1e-8.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sample_steps=1):
+    """Oracle step (C restatement + PARDISO/SuperLU direct solves, as the
+    reference factorises every step) on the same 501x501 input, timed on
+    this host's cores."""
+    from i2elvis_planet_amd import configs
+    from oracle import oracle
+    p, hs, _ = configs.make("c3")
+    orc = oracle.Oracle(p, hs)
+    orc.ronurecalc()
+    t = time.time()
+    for _ in range(sample_steps):
+        orc.step()
+    dt = time.time() - t
+    threads = int(os.environ.get("MKL_NUM_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    return {"value": sample_steps / dt, "unit": "timesteps/s", "cores": threads, "kind": "port",
+            "sample": "%d step(s) of the 501x501 / 6.25M-marker setup from its initial state; marker loops "
+                      "serial C, sparse solves %s with %d threads" % (sample_steps, oracle.solver_kind(), threads)}
+
+
+def run(steps, warmup, skip_cpu):
+    import numpy as np
+    from i2elvis_planet_amd import Context, configs, default_solver
+
+    p, hs, cfg = configs.make("c3")
+    log("setup generated: %d markers" % hs.marknum)
+    o = default_solver()
+    o.precision = 1
+    ctx = Context(p, hs, solver=o)
+    ctx.ronurecalc()
+    for k in range(warmup):
+        ctx.step()
+        log("warmup step %d done" % k)
+    ctx.sync()
+    snap = ctx.download(hs.copy())
+    res = {}
+    for mode in ("mixed", "fp64"):
+        o.precision = 1 if mode == "mixed" else 0
+        ctx.set_solver(o)
+        ctx.upload(snap)
+        ctx.sync()
+        iters = []
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ctx.step()
+            st = ctx.stats()
+            iters.append((st.stokes_iters, st.stokes_relres, st.grav_iters, st.heat_iters, st.ms_stokes, st.ms_heat,
+                          st.ms_grav, st.ms_markers))
+        ctx.sync()
+        el = time.perf_counter() - t0
+        res[mode] = (steps / el, el * 1e3 / steps, iters)
+        log("%s: %.3f steps/s, %.1f ms/step; stokes its %s" % (mode, steps / el, el * 1e3 / steps,
+                                                                [i[0] for i in iters]))
+    # per-kernel timing with HIP events on the context stream
+    ms_adv, ms_p2g, ms_sap = ctx.time_kernels(5)
+    N, M = p.xnumx * p.ynumy, hs.marknum
+    kernels = {
+        # algorithmic bytes: markers 25 B (x,y,type,T read; x,y,eta,P written), nodes vx,vy 16 B
+        "advect_rk4": {"ms": ms_adv, "bytes": 25 * M + 16 * N},
+        # markers 30 B, 12 FP64 node accumulators written
+        "p2g_scatter": {"ms": ms_p2g, "bytes": 30 * M + 96 * N},
+        # FP64 Stokes apply: x (24 B) + nu, nd, ro, gp (32 B) read, y (24 B) written per node
+        "stokes_apply_fp64": {"ms": ms_sap, "bytes": 80 * N},
+    }
+    for k, v in kernels.items():
+        v["gbs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+    dom = "advect_rk4"
+    d = kernels[dom]
+    mixed_sps, mixed_ms, it = res["mixed"]
+    fp64_sps = res["fp64"][0]
+    line = {
+        "metric": "timesteps/s @501², 6.3M markers: mixed vs FP64 GPU at Stokes rel.resid 1e-8",
+        "value": mixed_sps,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": mixed_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (f32 multigrid preconditioner)",
+        "data": "synthetic",
+        "config": {"workload": "501x501 nodes, 25 markers/cell (6.25M), R=150 km, Arrhenius 300 kJ/mol, "
+                               "linear melting 1360-1850 K, 400 iron blobs; one i2mart step",
+                   "stokes_rtol": 1e-8, "precision": "mixed"},
+        "fp64_value": fp64_sps,
+        "mixed_over_fp64": mixed_sps / fp64_sps,
+        "stokes_iters_mixed": [i[0] for i in it],
+        "stokes_iters_fp64": [i[0] for i in res["fp64"][2]],
+        "stokes_relres_mixed": [i[1] for i in it],
+        "phase_ms_mixed": {"stokes": float(np.mean([i[4] for i in it])), "heat": float(np.mean([i[5] for i in it])),
+                           "gravity": float(np.mean([i[6] for i in it])),
+                           "markers": float(np.mean([i[7] for i in it]))},
+        "kernels": {k: {"ms": round(v["ms"], 4), "GB/s": round(v["gbs"], 1)} for k, v in kernels.items()},
+        "roofline": {"bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": d["gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom},
+        "cpu_baseline": None,
+    }
+    ctx.close()
+    if not skip_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # reported, never fatal to the GPU number
+            log("cpu baseline failed: %r" % (e,))
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs one GPU only: the multi-GPU decomposition of this step is not built "
+              "(see the sharding notes); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    line = run(a.steps, a.warmup, a.no_cpu)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -24,6 +24,8 @@ _EXPORTS = [
     "i2e_maxvelstep", "i2e_viterate", "i2e_titerate", "i2e_movemark", "i2e_ronurecalc", "i2e_step",
     "i2e_get_scalars", "i2e_set_scalars", "i2e_get_stats", "i2e_sync", "i2e_last_error",
     "i2e_stokes_apply_dev", "i2e_gravity_apply_dev", "i2e_heat_apply_dev", "i2e_device_ptr", "i2e_time_kernels",
+    "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
+    "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
 ]
 
 _lib = None
@@ -68,6 +70,13 @@ def load_library(path=LIB_PATH):
         "i2e_heat_apply_dev": ([vp, vp, vp], C.c_int),
         "i2e_device_ptr": ([vp, C.c_char_p], vp),
         "i2e_time_kernels": ([vp, C.c_int, dp, dp, dp], C.c_int),
+        "i2e_stokes_rhs_dev": ([vp, vp], C.c_int),
+        "i2e_stokes_precond_dev": ([vp, vp, vp], C.c_int),
+        "i2e_vcycle_dev": ([vp, vp, vp], C.c_int),
+        "i2e_dev_alloc": ([C.c_size_t, C.POINTER(vp)], C.c_int),
+        "i2e_dev_free": ([vp], C.c_int),
+        "i2e_copy_to_dev": ([vp, vp, C.c_size_t], C.c_int),
+        "i2e_copy_to_host": ([vp, vp, C.c_size_t], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -87,6 +96,44 @@ def default_solver():
     o = SolverOpts()
     _chk(load_library().i2e_default_solver(C.byref(o)), "i2e_default_solver")
     return o
+
+
+class DeviceArray:
+    """float64 device buffer allocated through libi2e's own HIP runtime."""
+
+    def __init__(self, n_or_array):
+        L = load_library()
+        a = None if np.isscalar(n_or_array) else np.ascontiguousarray(n_or_array, np.float64)
+        self.n = int(n_or_array) if a is None else a.size
+        self._p = C.c_void_p()
+        _chk(L.i2e_dev_alloc(8 * self.n, C.byref(self._p)), "i2e_dev_alloc")
+        self.upload(np.zeros(self.n) if a is None else a)
+
+    @property
+    def ptr(self):
+        return self._p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        if a.size != self.n:
+            raise ValueError("size %d != %d" % (a.size, self.n))
+        _chk(_lib.i2e_copy_to_dev(self._p, a.ctypes.data_as(C.c_void_p), 8 * self.n), "i2e_copy_to_dev")
+
+    def numpy(self):
+        out = np.empty(self.n)
+        _chk(_lib.i2e_copy_to_host(out.ctypes.data_as(C.c_void_p), self._p, 8 * self.n), "i2e_copy_to_host")
+        return out
+
+    def free(self):
+        if self._p:
+            _lib.i2e_dev_free(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -175,10 +222,19 @@ class Context:
              "heat": _lib.i2e_heat_apply_dev}[which]
         _chk(f(self._h, C.c_void_p(x_dev), C.c_void_p(y_dev)), which + "_apply")
 
+    def stokes_rhs(self, b_dev):
+        _chk(_lib.i2e_stokes_rhs_dev(self._h, C.c_void_p(b_dev)), "stokes_rhs")
+
+    def stokes_precond(self, r_dev, z_dev):
+        _chk(_lib.i2e_stokes_precond_dev(self._h, C.c_void_p(r_dev), C.c_void_p(z_dev)), "stokes_precond")
+
+    def vcycle(self, f_dev, u_dev):
+        _chk(_lib.i2e_vcycle_dev(self._h, C.c_void_p(f_dev), C.c_void_p(u_dev)), "vcycle")
+
     def time_kernels(self, reps=5):
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         _chk(_lib.i2e_time_kernels(self._h, reps, C.byref(a), C.byref(b), C.byref(c)), "time_kernels")
         return a.value, b.value, c.value
 
 
-__all__ = ["Context", "HostState", "Params", "SolverOpts", "Stats", "default_solver", "load_library", "I2EError"]
+__all__ = ["Context", "DeviceArray", "HostState", "Params", "SolverOpts", "Stats", "default_solver", "load_library", "I2EError"]

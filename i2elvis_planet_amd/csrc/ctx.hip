@@ -81,9 +81,13 @@ __global__ void k_schur_bc(SOp<double> o, BcTab bc, const double *r, double *z) 
 
 // heat preconditioner input/output: f = rho*cp*r on equation rows, r on BC rows
 template <class T>
-__global__ void k_heat_pre(int N, const double *r, const double *ro, const double *cp, const int8_t *mask, T *f) {
+__global__ void k_heat_pre(int N, const double *r, const double *sc, const double *ro, const double *cp,
+                           const int8_t *mask, T *f) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < N) f[m] = T(mask[m] ? r[m] : r[m] * ro[m] * cp[m]);
+  if (m < N) {
+    const double rr = r[m] / sc[m];
+    f[m] = T(mask[m] ? rr : rr * ro[m] * cp[m]);
+  }
 }
 template <class T>
 __global__ void k_to_double(int N, const T *u, double *z) {
@@ -104,19 +108,29 @@ struct GravPre {
   int X, Y;
   double mu;
 };
+// r arrives row-scaled (r_row / diag); the multigrid works on the unscaled rows
 template <class T>
-__global__ void k_grav_pre(GravPre g, const double *r, const int8_t *mask, T *f) {
+__global__ void k_grav_pre(GravPre g, const double *r, const double *sc, const int8_t *mask, T *f) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   if (j >= g.Y - 1) return;
   const int q = i * (g.Y - 1) + j, m = (i + 1) * g.Y + (j + 1);
-  f[q] = T(mask[q] ? r[m] : -r[m]);
+  const double rr = r[m] / sc[m];
+  f[q] = T(mask[q] ? rr : -rr);
 }
 template <class T>
-__global__ void k_grav_post(GravPre g, const double *r, const T *u, double *z) {
+__global__ void k_grav_post(GravPre g, const double *r, const double *sc, const T *u, double *z) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   if (j >= g.Y) return;
   const int m = i * g.Y + j;
-  z[m] = (i && j) ? (double)u[(i - 1) * (g.Y - 1) + (j - 1)] : r[m] / g.mu;
+  z[m] = (i && j) ? (double)u[(i - 1) * (g.Y - 1) + (j - 1)] : r[m] / sc[m] / g.mu;
+}
+template <class T>
+__global__ void k_cast_to(int n, const double *a, T *b) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = T(a[t]);
+}
+__global__ void k_scale_vec(int n, const double *sc, double *y) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) y[k] *= sc[k];
 }
 __global__ void k_grav_fields(Grid g, double R, double mu, double *k, double *c, int8_t *mask, double *dm) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;  // cell-grid indices
@@ -171,7 +185,7 @@ struct i2e_ctx {
   VelMG<double> v64;
   ScalMG<float> h32, g32;
   ScalMG<double> h64, g64;
-  DBuf<double> hk, hc, hdm, gk, gc, gdm;
+  DBuf<double> hk, hc, hdm, gk, gc, gdm, gsc, hsc;
   DBuf<int8_t> hmask, gmask;
   Fgmres fs, fh, fgv;
   i2e_solver_opts o;
@@ -191,7 +205,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   if (!o) return I2E_ERR_ARG;
   memset(o, 0, sizeof *o);
   o->precision = I2E_PREC_MIXED;
-  o->restart = 60;
+  o->restart = 200;
   o->maxit = 600;
   o->nsmooth = 2;
   o->stokes_rtol = 1e-8;
@@ -243,7 +257,8 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
       (st = c->red.alloc(4096)) || (st = c->xs.alloc((size_t)3 * N)) || (st = c->bs.alloc((size_t)3 * N)) ||
       (st = c->work.alloc((size_t)3 * N)) || (st = c->bondm.alloc((size_t)4 * N)) || (st = c->pin.alloc(4096)) ||
       (st = c->hk.alloc(N)) || (st = c->hc.alloc(N)) || (st = c->hdm.alloc(N)) || (st = c->hmask.alloc(N)) ||
-      (st = c->gk.alloc(N)) || (st = c->gc.alloc(N)) || (st = c->gdm.alloc(N)) || (st = c->gmask.alloc(N)))
+      (st = c->gk.alloc(N)) || (st = c->gc.alloc(N)) || (st = c->gdm.alloc(N)) || (st = c->gmask.alloc(N)) ||
+      (st = c->gsc.alloc(N)) || (st = c->hsc.alloc(N)))
     return fail(st);
   if (max_markers > 0) {
     if ((st = c->mfl.alloc((size_t)7 * max_markers)) || (st = c->mt.alloc(max_markers))) return fail(st);
@@ -282,7 +297,7 @@ int i2e_destroy(i2e_ctx *c) {
     g.gy.release();
   }
   DBuf<double> *d[] = {&c->gx, &c->gy, &c->nodes, &c->acc, &c->work, &c->sol1, &c->tk2, &c->red, &c->xs,
-                       &c->bs, &c->bondv, &c->hk, &c->hc, &c->hdm, &c->gk, &c->gc, &c->gdm};
+                       &c->bs, &c->bondv, &c->hk, &c->hc, &c->hdm, &c->gk, &c->gc, &c->gdm, &c->gsc, &c->hsc};
   for (auto *b : d) b->release();
   c->mfl.release();
   c->mt.release();
@@ -299,7 +314,7 @@ int i2e_destroy(i2e_ctx *c) {
 
 int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
   if (!c || !o) return I2E_ERR_ARG;
-  if (o->restart < 1 || o->restart > 200 || o->maxit < 1 || o->nsmooth < 0 || o->coarse_max < 2) return I2E_ERR_ARG;
+  if (o->restart < 1 || o->restart > 400 || o->maxit < 1 || o->nsmooth < 0 || o->coarse_max < 2) return I2E_ERR_ARG;
   bool regeo = c->setup_done && o->coarse_max != c->o.coarse_max;
   c->o = *o;
   if (regeo) c->setup_done = false;
@@ -445,6 +460,8 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
   double mug = 2.0 / xstpx / xstpx + 2 / ystpy / ystpy;
   hipLaunchKernelGGL(k_grav_fields, nb2(X - 1, Y - 1), dim3(256), 0, c->s, g, c->P.gykoef, mug, c->gk.p, c->gc.p,
                      c->gmask.p, c->gdm.p);
+  GravGeo q0{c->P.gykoef, 0.0, mug};
+  hipLaunchKernelGGL(k_grav_rowscale, nb2(X, Y), dim3(256), 0, c->s, g, q0, c->gsc.p);
   ST_OK(c->g32.refresh(c->gk.p, c->gc.p, c->gmask.p, c->gdm.p, c->s));
   ST_OK(c->g64.refresh(c->gk.p, c->gc.p, c->gmask.p, c->gdm.p, c->s));
   HIP_OK(hipStreamSynchronize(c->s));
@@ -557,24 +574,29 @@ static int gravity_solve(i2e_ctx *c) {
   }
   double xstpx = c->P.xsize / (double)(X - 1), ystpy = c->P.ysize / (double)(Y - 1);
   GravGeo q{c->P.gykoef, c->gxkoef, 2.0 / xstpx / xstpx + 2 / ystpy / ystpy};
+  // rows divided by their diagonal: the exterior-penalty rows (x 1e18) would
+  // otherwise make ||b|| blind to the interior Poisson rows
+  const double *sc = c->gsc.p;
   hipLaunchKernelGGL(k_grav_rhs, nb2(X, Y), dim3(256), 0, c->s, g, q, c->F.ro, c->bs.p);
+  hipLaunchKernelGGL(k_scale_vec, dim3(1024), dim3(256), 0, c->s, N, sc, c->bs.p);
   hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->F.gp, c->xs.p);
   GravPre gp{X, Y, q.mu};
   auto A = [&](const double *x, double *y) -> int {
     hipLaunchKernelGGL(k_grav_apply, nb2(X, Y), dim3(256), 0, c->s, g, q, x, y);
+    hipLaunchKernelGGL(k_scale_vec, dim3(1024), dim3(256), 0, c->s, N, sc, y);
     return I2E_OK;
   };
   auto Mf = [&](const double *r, double *z) -> int {
     if (mixed(c)) {
-      hipLaunchKernelGGL(k_grav_pre<float>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, gp, r, c->gmask.p,
+      hipLaunchKernelGGL(k_grav_pre<float>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, gp, r, sc, c->gmask.p,
                          c->g32.L[0].f.p);
       ST_OK(c->g32.vcycle(0, c->s));
-      hipLaunchKernelGGL(k_grav_post<float>, nb2(X, Y), dim3(256), 0, c->s, gp, r, c->g32.L[0].u.p, z);
+      hipLaunchKernelGGL(k_grav_post<float>, nb2(X, Y), dim3(256), 0, c->s, gp, r, sc, c->g32.L[0].u.p, z);
     } else {
-      hipLaunchKernelGGL(k_grav_pre<double>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, gp, r, c->gmask.p,
+      hipLaunchKernelGGL(k_grav_pre<double>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, gp, r, sc, c->gmask.p,
                          c->g64.L[0].f.p);
       ST_OK(c->g64.vcycle(0, c->s));
-      hipLaunchKernelGGL(k_grav_post<double>, nb2(X, Y), dim3(256), 0, c->s, gp, r, c->g64.L[0].u.p, z);
+      hipLaunchKernelGGL(k_grav_post<double>, nb2(X, Y), dim3(256), 0, c->s, gp, r, sc, c->g64.L[0].u.p, z);
     }
     return I2E_OK;
   };
@@ -584,54 +606,71 @@ static int gravity_solve(i2e_ctx *c) {
   int st = c->fgv.solve(A, Mf, c->bs.p, c->xs.p, c->o.grav_rtol, c->o.maxit, c->s, &it, &rr);
   c->st.grav_iters = it;
   c->st.grav_relres = rr;
-  if (st) return st;
+  if (st) {
+    set_error("gravity solve: %d iterations, relative residual %.3e > %.1e", it, rr, c->o.grav_rtol);
+    return st;
+  }
   hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->xs.p, c->F.gp);
   hipLaunchKernelGGL(k_grav_ga, nb2(X, Y), dim3(256), 0, c->s, g, c->F.gp, c->F.ga);
   return I2E_OK;
 }
 
-static int stokes_solve(i2e_ctx *c) {
-  const int X = c->X, Y = c->Y, N = c->N;
+// Stokes: right-hand side, initial guess and multigrid refresh for this step
+static int stokes_prepare(i2e_ctx *c) {
+  const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
   hipLaunchKernelGGL(k_stokes_rhs, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->bs.p);
   hipLaunchKernelGGL(k_stokes_pack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->F.pr, c->F.vx, c->F.vy, c->xs.p);
-  const bool mx = mixed(c);
-  if (mx) {
+  if (mixed(c)) {
     ST_OK(c->v32.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
     c->v32.nsmooth = c->o.nsmooth;
   } else {
     ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
     c->v64.nsmooth = c->o.nsmooth;
   }
+  return I2E_OK;
+}
+
+// z = P^-1 r, lower block-triangular: velocity V-cycle, then the Schur diagonal
+static int stokes_precond(i2e_ctx *c, const double *r, double *z) {
+  const int X = c->X, Y = c->Y;
+  SOp<double> o = c->sop();
+  if (mixed(c)) {
+    hipLaunchKernelGGL(k_split_v<float>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v32.L[0].fx.p, c->v32.L[0].fy.p);
+    ST_OK(c->v32.vcycle(0, c->s));
+    hipLaunchKernelGGL(k_schur<float>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->o.schur_scale, r,
+                       c->v32.L[0].ux.p, c->v32.L[0].uy.p, z);
+  } else {
+    hipLaunchKernelGGL(k_split_v<double>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v64.L[0].fx.p,
+                       c->v64.L[0].fy.p);
+    ST_OK(c->v64.vcycle(0, c->s));
+    hipLaunchKernelGGL(k_schur<double>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->o.schur_scale, r,
+                       c->v64.L[0].ux.p, c->v64.L[0].uy.p, z);
+  }
+  hipLaunchKernelGGL(k_schur_bc, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, r, z);
+  return I2E_OK;
+}
+
+static int stokes_solve(i2e_ctx *c) {
+  const int X = c->X, Y = c->Y;
+  SOp<double> o = c->sop();
+  ST_OK(stokes_prepare(c));
   auto A = [&](const double *x, double *y) -> int {
     hipLaunchKernelGGL(k_stokes_apply, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, x, y);
     return I2E_OK;
   };
-  auto Mf = [&](const double *r, double *z) -> int {
-    if (mx) {
-      hipLaunchKernelGGL(k_split_v<float>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v32.L[0].fx.p, c->v32.L[0].fy.p);
-      ST_OK(c->v32.vcycle(0, c->s));
-      hipLaunchKernelGGL(k_schur<float>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->o.schur_scale, r,
-                         c->v32.L[0].ux.p, c->v32.L[0].uy.p, z);
-    } else {
-      hipLaunchKernelGGL(k_split_v<double>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v64.L[0].fx.p,
-                         c->v64.L[0].fy.p);
-      ST_OK(c->v64.vcycle(0, c->s));
-      hipLaunchKernelGGL(k_schur<double>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->o.schur_scale, r,
-                         c->v64.L[0].ux.p, c->v64.L[0].uy.p, z);
-    }
-    hipLaunchKernelGGL(k_schur_bc, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, r, z);
-    return I2E_OK;
-  };
+  auto Mf = [&](const double *r, double *z) -> int { return stokes_precond(c, r, z); };
   int it = 0;
   double rr = 0;
   int st = c->fs.solve(A, Mf, c->bs.p, c->xs.p, c->o.stokes_rtol, c->o.maxit, c->s, &it, &rr);
   c->st.stokes_iters = it;
   c->st.stokes_relres = rr;
-  if (st) return st;
+  if (st) {
+    set_error("Stokes solve: %d iterations, relative residual %.3e > %.1e", it, rr, c->o.stokes_rtol);
+    return st;
+  }
   hipLaunchKernelGGL(k_stokes_unpack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->xs.p, c->F.pr, c->F.vx, c->F.vy);
   hipLaunchKernelGGL(k_strain, nb2(X, Y), dim3(256), 0, c->s, c->grid(), c->F);
-  (void)N;
   return I2E_OK;
 }
 
@@ -667,7 +706,10 @@ int i2e_titerate(i2e_ctx *c, int n0) {
   if (c->P.heatmod == 0) c->timestep = c->timestep <= tktimestep ? c->timestep : tktimestep;
   const double dt = c->timestep;
   hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->F.tk, c->tk2.p);
+  // rows divided by their diagonal (the fixed-T rows would otherwise dominate ||b||)
+  hipLaunchKernelGGL(k_heat_rowscale, nb2(X, Y), dim3(256), 0, c->s, g, c->F, c->bc, dt, c->hsc.p);
   hipLaunchKernelGGL(k_heat_rhs, nb2(X, Y), dim3(256), 0, c->s, g, c->F, c->bc, dt, c->sol1.p, c->bs.p);
+  hipLaunchKernelGGL(k_scale_vec, dim3(1024), dim3(256), 0, c->s, N, c->hsc.p, c->bs.p);
   hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->F.tk, c->xs.p);
   hipLaunchKernelGGL(k_heat_fields, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, c->F.ro, c->F.cp, dt, c->bondm.p,
                      c->hc.p, c->hmask.p, c->hdm.p);
@@ -681,17 +723,18 @@ int i2e_titerate(i2e_ctx *c, int n0) {
   }
   auto A = [&](const double *x, double *y) -> int {
     hipLaunchKernelGGL(k_heat_apply, nb2(X, Y), dim3(256), 0, c->s, g, c->F, c->bc, dt, x, y);
+    hipLaunchKernelGGL(k_scale_vec, dim3(1024), dim3(256), 0, c->s, N, c->hsc.p, y);
     return I2E_OK;
   };
   auto Mf = [&](const double *r, double *z) -> int {
     if (mx) {
-      hipLaunchKernelGGL(k_heat_pre<float>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, r, c->F.ro, c->F.cp,
-                         c->hmask.p, c->h32.L[0].f.p);
+      hipLaunchKernelGGL(k_heat_pre<float>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, r, c->hsc.p, c->F.ro,
+                         c->F.cp, c->hmask.p, c->h32.L[0].f.p);
       ST_OK(c->h32.vcycle(0, c->s));
       hipLaunchKernelGGL(k_to_double<float>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, c->h32.L[0].u.p, z);
     } else {
-      hipLaunchKernelGGL(k_heat_pre<double>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, r, c->F.ro, c->F.cp,
-                         c->hmask.p, c->h64.L[0].f.p);
+      hipLaunchKernelGGL(k_heat_pre<double>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, r, c->hsc.p, c->F.ro,
+                         c->F.cp, c->hmask.p, c->h64.L[0].f.p);
       ST_OK(c->h64.vcycle(0, c->s));
       hipLaunchKernelGGL(k_to_double<double>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, c->h64.L[0].u.p, z);
     }
@@ -702,7 +745,10 @@ int i2e_titerate(i2e_ctx *c, int n0) {
   int st = c->fh.solve(A, Mf, c->bs.p, c->xs.p, c->o.heat_rtol, c->o.maxit, c->s, &it, &rr);
   c->st.heat_iters = it;
   c->st.heat_relres = rr;
-  if (st) return st;
+  if (st) {
+    set_error("heat solve: %d iterations, relative residual %.3e > %.1e", it, rr, c->o.heat_rtol);
+    return st;
+  }
   hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->xs.p, c->F.tk);
   // marker temperatures (heatmart.c:515-689); sol0/sol1 live in the ronurecalc accumulators
   double *s0 = c->A.w0, *s1 = c->A.w1;
@@ -794,6 +840,74 @@ int i2e_heat_apply_dev(i2e_ctx *c, const double *x, double *y) {
   hipLaunchKernelGGL(k_heat_apply, nb2(c->X, c->Y), dim3(256), 0, c->s, c->grid(), c->F, c->bc, c->timestep, x, y);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_stokes_rhs_dev(i2e_ctx *c, double *b) {
+  if (!c || !b || !c->setup_done) return I2E_ERR_ARG;
+  hipLaunchKernelGGL(k_stokes_rhs, nb2(c->X, c->Y), dim3(256), 0, c->s, c->sop(), c->bc, b);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_stokes_precond_dev(i2e_ctx *c, const double *r, double *z) {
+  if (!c || !r || !z || !c->setup_done) return I2E_ERR_ARG;
+  ST_OK(stokes_prepare(c));
+  ST_OK(stokes_precond(c, r, z));
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_vcycle_dev(i2e_ctx *c, const double *f, double *u) {
+  if (!c || !f || !u || !c->setup_done) return I2E_ERR_ARG;
+  const int N = c->N;
+  ST_OK(stokes_prepare(c));
+  if (mixed(c)) {
+    hipLaunchKernelGGL(k_cast_to<float>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, f, c->v32.L[0].fx.p);
+    hipLaunchKernelGGL(k_cast_to<float>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, f + N, c->v32.L[0].fy.p);
+    ST_OK(c->v32.vcycle(0, c->s));
+    hipLaunchKernelGGL(k_to_double<float>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, c->v32.L[0].ux.p, u);
+    hipLaunchKernelGGL(k_to_double<float>, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, c->v32.L[0].uy.p, u + N);
+  } else {
+    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, f, c->v64.L[0].fx.p);
+    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, f + N, c->v64.L[0].fy.p);
+    ST_OK(c->v64.vcycle(0, c->s));
+    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->v64.L[0].ux.p, u);
+    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, N, c->v64.L[0].uy.p, u + N);
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_dev_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return I2E_ERR_ARG;
+  *ptr = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no GPU visible");
+    return I2E_ERR_NODEVICE;
+  }
+  HIP_OK(hipMalloc(ptr, bytes ? bytes : 1));
+  return I2E_OK;
+}
+
+int i2e_dev_free(void *ptr) {
+  if (ptr) HIP_OK(hipFree(ptr));
+  return I2E_OK;
+}
+
+int i2e_copy_to_dev(void *dst_dev, const void *src_host, size_t bytes) {
+  if ((!dst_dev || !src_host) && bytes) return I2E_ERR_ARG;
+  if (bytes) HIP_OK(hipMemcpy(dst_dev, src_host, bytes, hipMemcpyHostToDevice));
+  return I2E_OK;
+}
+
+int i2e_copy_to_host(void *dst_host, const void *src_dev, size_t bytes) {
+  if ((!dst_host || !src_dev) && bytes) return I2E_ERR_ARG;
+  if (bytes) HIP_OK(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost));
   return I2E_OK;
 }
 

@@ -7,6 +7,7 @@
 // fused multi-dot / multi-axpy passes over the basis); all reductions go
 // through fixed-order block partials, so results are bitwise reproducible.
 #include <cmath>
+#include <cstdlib>
 
 #include "krylov.hpp"
 
@@ -115,6 +116,7 @@ int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double r
     return I2E_OK;
   }
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  const bool verbose = std::getenv("I2E_VERBOSE") != nullptr;
   int its = 0;
   double rr = 1;
   while (true) {
@@ -126,6 +128,7 @@ int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double r
     HIP_OK(hipStreamSynchronize(s));
     double beta = hh.p[0];
     rr = beta / bn;
+    if (verbose) fprintf(stderr, "[fgmres n=%d] its %d true relres %.3e (||b|| %.3e)\n", n, its, rr, bn);
     if (rr <= rtol || its >= maxit || !(beta > 0)) break;
     hipLaunchKernelGGL(k_scale_by, dim3(gb), dim3(256), 0, s, n, w.p, hd.p, V.p);
     std::fill(H.begin(), H.end(), 0.0);
@@ -166,6 +169,7 @@ int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double r
       g[j] = cs[j] * g[j];
       its++;
       k = j + 1;
+      if (verbose && (j % 10 == 9)) fprintf(stderr, "[fgmres n=%d]   it %d est %.3e\n", n, its, std::fabs(g[j + 1]) / bn);
       if (std::fabs(g[j + 1]) / bn <= rtol || !(hn > 0)) break;
     }
     // back substitution and x += Z y

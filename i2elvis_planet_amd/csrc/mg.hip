@@ -147,6 +147,28 @@ __global__ void k_restrict_cell(int Xc, int Yc, int Yf, const int *fi, const int
   c[i * Yc + j] = T(lg ? exp(r) : r);
 }
 
+// viscosity restriction: maximum over a window of +-rad fine lines around the
+// fine point that carries the coarse node (cells: around their lower-right
+// fine cell).  Local averages (geometric, arithmetic, harmonic) make coarse
+// rows next to stiff regions much softer than the Galerkin operator R A P,
+// and the coarse correction then diverges.
+template <class T>
+__global__ void k_restrict_max(int Xc, int Yc, int Xf, int Yf, const int *fi, const int *fj, int rad, const T *f, T *c) {
+  IDX2;
+  if (i >= Xc || j >= Yc) return;
+  const int a0 = fi[i], b0 = fj[j];
+  T v = f[a0 * Yf + b0];
+  for (int a = a0 - rad; a <= a0 + rad; a++) {
+    if (a < 0 || a >= Xf) continue;
+    for (int b = b0 - rad; b <= b0 + rad; b++) {
+      if (b < 0 || b >= Yf) continue;
+      T u = f[a * Yf + b];
+      v = u > v ? u : v;
+    }
+  }
+  c[i * Yc + j] = v;
+}
+
 // ---------------------------------------------------------------- velocity rows
 template <class T, class A>
 __device__ __forceinline__ T vrow(const VLev<T> &L, const A &v, int comp, int i, int j) {
@@ -190,6 +212,20 @@ __global__ void k_vgs(VLev<T> L, int comp, int color, T *ux, T *uy, const T *fx,
     ux[m] += (fx[m] - r) / dx[m];
   else
     uy[m] += (fy[m] - r) / dy[m];
+}
+
+// re-solve the pointwise rows (boundary and ghost) exactly after smoothing:
+// their scale (mukoef) dwarfs the momentum rows, and the last half sweeps
+// leave them inconsistent with their freshly updated neighbours
+template <class T>
+__global__ void k_vbcfix(VLev<T> L, T *ux, T *uy, const T *fx, const T *fy) {
+  IDX2;
+  const int X = L.o.X, Y = L.o.Y;
+  if (i >= X || j >= Y) return;
+  const int m = i * Y + j;
+  const T imu = T(1.0 / L.o.mu);
+  if (L.kx[m]) ux[m] = fx[m] * imu + (L.kx[m] == 1 && L.nbx[m] >= 0 ? L.kkx[m] * ux[L.nbx[m]] : T(0));
+  if (L.ky[m]) uy[m] = fy[m] * imu + (L.ky[m] == 1 && L.nby[m] >= 0 ? L.kky[m] * uy[L.nby[m]] : T(0));
 }
 
 template <class T>
@@ -346,6 +382,195 @@ __global__ void k_dense_solve(int n, int n0, const T *ainv, const T *f0, const T
     u1[r - n0] = T(s);
 }
 
+
+// ---------------------------------------------------------------- stencil levels
+// slot of column component cc (0 Vx, 1 Vy) at offset (di,dj) in a window of
+// half-width W: rows store 2*(2W+1)^2 coefficients
+__device__ __forceinline__ int sslot(int W, int cc, int di, int dj) {
+  const int w = 2 * W + 1;
+  return cc * w * w + (di + W) * w + (dj + W);
+}
+
+template <class T>
+struct StView {
+  const T *S;
+  int X, Y, W, NS;
+  const int8_t *kx, *ky;
+};
+
+// level-0 coefficients by probing the matrix-free rows with unit vectors
+template <class T>
+__global__ void k_probe0(VLev<T> L, T *S) {
+  IDX2;
+  const int X = L.o.X, Y = L.o.Y, N = X * Y;
+  if (i >= X || j >= Y) return;
+  const int m = i * Y + j;
+  for (int comp = 1; comp <= 2; comp++) {
+    T *out = S + ((size_t)(comp - 1) * N + m) * 18;
+    for (int cc = 0; cc < 2; cc++)
+      for (int di = -1; di <= 1; di++)
+        for (int dj = -1; dj <= 1; dj++) {
+          const int a = i + di, b = j + dj;
+          T v = 0;
+          if (a >= 0 && a < X && b >= 0 && b < Y) v = vrow(L, VUnit<T>{cc + 1, a * Y + b, Y}, comp, i, j);
+          out[sslot(1, cc, di, dj)] = v;
+        }
+  }
+}
+
+// Galerkin coarse stencil R A P (R = normalised transpose of the bilinear
+// prolongation restricted to equation rows, as the residual restriction is);
+// pointwise rows (boundary, ghost) keep their mukoef-scaled form
+template <class T>
+__global__ void k_galerkin(StView<T> F, int Xc, int Yc, MapView xn, MapView xm, MapView yn, MapView ym,
+                           const int8_t *ckx, const int8_t *cky, const int *cnbx, const int *cnby, const T *ckkx,
+                           const T *ckky, double mu, T *Sc, int *overflow) {
+  IDX2;
+  if (i >= Xc || j >= Yc) return;
+  const int Nc = Xc * Yc, Nf = F.X * F.Y, m = i * Yc + j;
+  for (int comp = 1; comp <= 2; comp++) {
+    T *out = Sc + ((size_t)(comp - 1) * Nc + m) * 50;
+    double acc[50];
+    for (int q = 0; q < 50; q++) acc[q] = 0.0;
+    const int kind = comp == 1 ? ckx[m] : cky[m];
+    if (kind != 0) {
+      acc[sslot(2, comp - 1, 0, 0)] = mu;
+      const int nb = comp == 1 ? cnbx[m] : cnby[m];
+      if (kind == 1 && nb >= 0) {
+        const double kk = (double)(comp == 1 ? ckkx[m] : ckky[m]);
+        const int di = nb / Yc - i, dj = nb % Yc - j;
+        if (di >= -2 && di <= 2 && dj >= -2 && dj <= 2)
+          acc[sslot(2, comp - 1, di, dj)] -= mu * kk;
+        else
+          atomicAdd(overflow, 1);
+      }
+      for (int q = 0; q < 50; q++) out[q] = T(acc[q]);
+      continue;
+    }
+    const MapView mxr = comp == 1 ? xn : xm, myr = comp == 1 ? ym : yn;
+    double wsum = 0;
+    for (int k = mxr.rb[i]; k < mxr.re[i]; k++) {
+      const double wx = wgt(mxr, k, i);
+      if (wx == 0) continue;
+      for (int l = myr.rb[j]; l < myr.re[j]; l++) {
+        const double w = wx * wgt(myr, l, j);
+        if (w == 0) continue;
+        wsum += w;
+        const int a = k * F.Y + l;
+        if ((comp == 1 ? F.kx[a] : F.ky[a]) != 0) continue;
+        const T *srow = F.S + ((size_t)(comp - 1) * Nf + a) * F.NS;
+        for (int cc = 0; cc < 2; cc++) {
+          const MapView px = cc == 0 ? xn : xm, py = cc == 0 ? ym : yn;
+          for (int dk = -F.W; dk <= F.W; dk++)
+            for (int dl = -F.W; dl <= F.W; dl++) {
+              const double sv = (double)srow[sslot(F.W, cc, dk, dl)];
+              if (sv == 0) continue;
+              const int kb = k + dk, lb = l + dl;
+              if (kb < 0 || kb >= F.X || lb < 0 || lb >= F.Y) continue;
+              if (cc == 0 ? (lb == F.Y - 1) : (kb == F.X - 1)) continue;  // ghost: no prolongation
+              const int a0 = px.i0[kb], b0 = py.i0[lb];
+              const double wa = px.w[kb], wb = py.w[lb];
+              for (int ia = 0; ia < 2; ia++)
+                for (int ib = 0; ib < 2; ib++) {
+                  const double pw = (ia ? wa : 1.0 - wa) * (ib ? wb : 1.0 - wb);
+                  if (pw == 0) continue;
+                  const int di = a0 + ia - i, dj = b0 + ib - j;
+                  if (di < -2 || di > 2 || dj < -2 || dj > 2) {
+                    atomicAdd(overflow, 1);
+                    continue;
+                  }
+                  acc[sslot(2, cc, di, dj)] += w * sv * pw;
+                }
+            }
+        }
+      }
+    }
+    const double inv = wsum > 0 ? 1.0 / wsum : 0.0;
+    for (int q = 0; q < 50; q++) out[q] = T(acc[q] * inv);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T s_apply(const T *s, int X, int Y, const T *ux, const T *uy, int i, int j) {
+  T acc = 0;
+  for (int cc = 0; cc < 2; cc++) {
+    const T *u = cc ? uy : ux;
+    for (int di = -2; di <= 2; di++) {
+      const int a = i + di;
+      if (a < 0 || a >= X) continue;
+      for (int dj = -2; dj <= 2; dj++) {
+        const int b = j + dj;
+        if (b < 0 || b >= Y) continue;
+        acc += s[sslot(2, cc, di, dj)] * u[a * Y + b];
+      }
+    }
+  }
+  return acc;
+}
+
+// one colour of the coarse smoother: out = u + (f - S u)/diag on the colour,
+// out = u elsewhere (component comp only)
+template <class T>
+__global__ void k_sjac(const T *S, int X, int Y, int comp, int color, const T *ux, const T *uy, const T *f, T *out) {
+  IDX2;
+  if (i >= X || j >= Y) return;
+  const int N = X * Y, m = i * Y + j;
+  const T *u = comp == 1 ? ux : uy;
+  if (((i + j) & 1) != color) {
+    out[m] = u[m];
+    return;
+  }
+  const T *s = S + ((size_t)(comp - 1) * N + m) * 50;
+  const T d = s[sslot(2, comp - 1, 0, 0)];
+  out[m] = u[m] + (f[m] - s_apply(s, X, Y, ux, uy, i, j)) / d;
+}
+
+template <class T>
+__global__ void k_sbcfix(const T *S, int X, int Y, const int8_t *kx, const int8_t *ky, T *ux, T *uy, const T *fx,
+                         const T *fy) {
+  IDX2;
+  if (i >= X || j >= Y) return;
+  const int N = X * Y, m = i * Y + j;
+  for (int comp = 1; comp <= 2; comp++) {
+    if ((comp == 1 ? kx[m] : ky[m]) == 0) continue;
+    const T *s = S + ((size_t)(comp - 1) * N + m) * 50;
+    const T d = s[sslot(2, comp - 1, 0, 0)];
+    T *u = comp == 1 ? ux : uy;
+    const T f = comp == 1 ? fx[m] : fy[m];
+    u[m] = u[m] + (f - s_apply(s, X, Y, ux, uy, i, j)) / d;
+  }
+}
+
+template <class T>
+__global__ void k_sresid(const T *S, int X, int Y, const int8_t *kx, const int8_t *ky, const T *ux, const T *uy,
+                         const T *fx, const T *fy, T *rx, T *ry) {
+  IDX2;
+  if (i >= X || j >= Y) return;
+  const int N = X * Y, m = i * Y + j;
+  rx[m] = kx[m] == 0 ? fx[m] - s_apply(S + (size_t)m * 50, X, Y, ux, uy, i, j) : T(0);
+  ry[m] = ky[m] == 0 ? fy[m] - s_apply(S + ((size_t)N + m) * 50, X, Y, ux, uy, i, j) : T(0);
+}
+
+// dense [Vx | Vy] matrix of the coarsest level from its stencil
+template <class T>
+__global__ void k_sdense(const T *S, int X, int Y, double *aug) {
+  const int N = X * Y, n = 2 * N;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double *row = aug + (size_t)r * 2 * n;
+  for (int c = 0; c < 2 * n; c++) row[c] = 0.0;
+  row[n + r] = 1.0;
+  const int m = r % N, i = m / Y, j = m % Y;
+  const T *s = S + (size_t)r * 50;
+  for (int cc = 0; cc < 2; cc++)
+    for (int di = -2; di <= 2; di++)
+      for (int dj = -2; dj <= 2; dj++) {
+        const int a = i + di, b = j + dj;
+        if (a < 0 || a >= X || b < 0 || b >= Y) continue;
+        row[cc * N + a * Y + b] += (double)s[sslot(2, cc, di, dj)];
+      }
+}
+
 // ---------------------------------------------------------------- VelMG host
 template <class T>
 VLev<T> VelMG<T>::lev(int l) const {
@@ -447,6 +672,10 @@ int VelMG<T>::setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &k
     ST_OK(v.fy.alloc(v.N));
     ST_OK(v.rx.alloc(v.N));
     ST_OK(v.ry.alloc(v.N));
+    ST_OK(v.ux2.alloc(v.N));
+    ST_OK(v.uy2.alloc(v.N));
+    v.W = l == 0 ? 1 : 2;
+    ST_OK(v.S.alloc((size_t)2 * v.N * (l == 0 ? 18 : 50)));
     HIP_OK(hipMemcpy(v.kx.p, v.hkx.data(), v.N, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(v.ky.p, v.hky.data(), v.N, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(v.nbx.p, v.hnbx.data(), v.N * sizeof(int), hipMemcpyHostToDevice));
@@ -475,6 +704,8 @@ int VelMG<T>::setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &k
   nc = 2 * L[nl - 1].N;
   ST_OK(dense.alloc((size_t)nc * 2 * nc));
   ST_OK(ainv.alloc((size_t)nc * nc));
+  ST_OK(overflow.alloc(1));
+  HIP_OK(hipMemset(overflow.p, 0, sizeof(int)));
   return I2E_OK;
 }
 
@@ -489,23 +720,28 @@ int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, cons
   hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, nd, L[0].nd.p);
   hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, ro, L[0].ro.p);
   hipLaunchKernelGGL(k_cast<T>, dim3(nblk(N0, 256)), dim3(256), 0, s, N0, gp, L[0].gp.p);
+  // finest level: diagonal for the matrix-free smoother and probed stencil
+  hipLaunchKernelGGL(k_vdiag<T>, grd2(L[0].X, L[0].Y), blk2(), 0, s, lev(0), L[0].dx.p, L[0].dy.p);
+  hipLaunchKernelGGL(k_probe0<T>, grd2(L[0].X, L[0].Y), blk2(), 0, s, lev(0), L[0].S.p);
   for (int l = 0; l + 1 < nl; l++) {
     VelLevel<T> &f = L[l], &c = L[l + 1];
-    hipLaunchKernelGGL(k_restrict_node<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, f.Y, view(f.mxn), view(f.myn),
-                       f.nu.p, c.nu.p, 1);
-    hipLaunchKernelGGL(k_restrict_node<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, f.Y, view(f.mxn), view(f.myn),
-                       f.ro.p, c.ro.p, 0);
-    hipLaunchKernelGGL(k_restrict_cell<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, f.Y, gfi[l].p, gfj[l].p, f.nd.p,
-                       c.nd.p, 1);
-    hipLaunchKernelGGL(k_restrict_cell<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, f.Y, gfi[l].p, gfj[l].p, f.gp.p,
-                       c.gp.p, 0);
+    StView<T> fv{f.S.p, f.X, f.Y, f.W, f.W == 1 ? 18 : 50, f.kx.p, f.ky.p};
+    hipLaunchKernelGGL(k_galerkin<T>, grd2(c.X, c.Y), blk2(), 0, s, fv, c.X, c.Y, view(f.mxn), view(f.mxm),
+                       view(f.myn), view(f.mym), c.kx.p, c.ky.p, c.nbx.p, c.nby.p, c.kkx.p, c.kky.p, mu_, c.S.p,
+                       overflow.p);
   }
-  for (int l = 0; l < nl; l++)
-    hipLaunchKernelGGL(k_vdiag<T>, grd2(L[l].X, L[l].Y), blk2(), 0, s, lev(l), L[l].dx.p, L[l].dy.p);
   const int n = nc;
-  hipLaunchKernelGGL(k_vdense<T>, dim3(nblk(n, 256), n), dim3(256), 0, s, lev(nl - 1), n, dense.p);
+  hipLaunchKernelGGL(k_sdense<T>, dim3(nblk(n, 64)), dim3(64), 0, s, L[nl - 1].S.p, L[nl - 1].X, L[nl - 1].Y,
+                     dense.p);
   hipLaunchKernelGGL(k_gauss_jordan<T>, dim3(1), dim3(1024), 0, s, n, dense.p, ainv.p);
   HIP_OK(hipGetLastError());
+  int nover = 0;
+  HIP_OK(hipMemcpyAsync(&nover, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  if (nover) {
+    set_error("velocity multigrid: %d Galerkin couplings fall outside the 5x5 stencil window", nover);
+    return I2E_ERR_UNSUPPORTED;
+  }
   return I2E_OK;
 }
 
@@ -522,21 +758,46 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   const VLev<T> o = lev(l);
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.ux.p);
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.uy.p);
-  for (int it = 0; it < nsmooth; it++)
-    for (int comp = 1; comp <= 2; comp++)
-      for (int col = 0; col < 2; col++)
-        hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
-  hipLaunchKernelGGL(k_vresid<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
+  auto csmooth = [&](bool rev) {
+    // per-colour Jacobi on the Galerkin stencil, double-buffered per component
+    for (int it = 0; it < nsmooth; it++)
+      for (int q = 0; q < 4; q++) {
+        const int k = rev ? 3 - q : q, comp = 1 + k / 2, col = k % 2;
+        DBuf<T> &cur = comp == 1 ? v.ux : v.uy, &nxt = comp == 1 ? v.ux2 : v.uy2;
+        hipLaunchKernelGGL(k_sjac<T>, G, B, 0, s, (const T *)v.S.p, v.X, v.Y, comp, col, (const T *)v.ux.p,
+                           (const T *)v.uy.p, comp == 1 ? (const T *)v.fx.p : (const T *)v.fy.p, nxt.p);
+        std::swap(cur.p, nxt.p);
+      }
+    hipLaunchKernelGGL(k_sbcfix<T>, G, B, 0, s, (const T *)v.S.p, v.X, v.Y, (const int8_t *)v.kx.p,
+                       (const int8_t *)v.ky.p, v.ux.p, v.uy.p, (const T *)v.fx.p, (const T *)v.fy.p);
+  };
+  if (l == 0) {
+    for (int it = 0; it < nsmooth; it++)
+      for (int comp = 1; comp <= 2; comp++)
+        for (int col = 0; col < 2; col++)
+          hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
+    hipLaunchKernelGGL(k_vbcfix<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p);
+    hipLaunchKernelGGL(k_vresid<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
+  } else {
+    csmooth(false);
+    hipLaunchKernelGGL(k_sresid<T>, G, B, 0, s, (const T *)v.S.p, v.X, v.Y, (const int8_t *)v.kx.p,
+                       (const int8_t *)v.ky.p, (const T *)v.ux.p, (const T *)v.uy.p, (const T *)v.fx.p,
+                       (const T *)v.fy.p, v.rx.p, v.ry.p);
+  }
   VelLevel<T> &c = L[l + 1];
   hipLaunchKernelGGL(k_vrestrict<T>, grd2(c.X, c.Y), B, 0, s, c.X, c.Y, v.Y, view(v.mxn), view(v.mxm), view(v.myn),
                      view(v.mym), c.kx.p, c.ky.p, v.rx.p, v.ry.p, c.fx.p, c.fy.p);
   ST_OK(vcycle(l + 1, s));
   hipLaunchKernelGGL(k_vprolong<T>, G, B, 0, s, v.X, v.Y, c.Y, view(v.mxn), view(v.mxm), view(v.myn), view(v.mym),
                      v.kx.p, v.ky.p, c.ux.p, c.uy.p, v.ux.p, v.uy.p);
-  for (int it = 0; it < nsmooth; it++)
-    for (int comp = 2; comp >= 1; comp--)
-      for (int col = 1; col >= 0; col--)
-        hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
+  if (l == 0) {
+    for (int it = 0; it < nsmooth; it++)
+      for (int comp = 2; comp >= 1; comp--)
+        for (int col = 1; col >= 0; col--)
+          hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
+    hipLaunchKernelGGL(k_vbcfix<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p);
+  } else
+    csmooth(true);
   return I2E_OK;
 }
 
@@ -546,8 +807,9 @@ void VelMG<T>::release() {
     v.nu.release(); v.nd.release(); v.ro.release(); v.gp.release();
     v.kx.release(); v.ky.release(); v.nbx.release(); v.nby.release(); v.kkx.release(); v.kky.release();
     v.dx.release(); v.dy.release(); v.ux.release(); v.uy.release(); v.fx.release(); v.fy.release();
-    v.rx.release(); v.ry.release();
+    v.rx.release(); v.ry.release(); v.ux2.release(); v.uy2.release(); v.S.release();
   }
+  overflow.release();
   for (auto &b : gfi) b.release();
   for (auto &b : gfj) b.release();
   dense.release();

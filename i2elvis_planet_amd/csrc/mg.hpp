@@ -3,11 +3,21 @@
 //
 // Coarse grids keep every second grid line of the finer grid plus the last
 // one (so odd cell counts such as 125 work, at the price of one narrow last
-// cell), operators are rediscretised with restricted fields (geometric mean
-// for viscosity, arithmetic for density / potential / conductivity), the
-// smoother is red-black Gauss-Seidel (per velocity component), transfers are
-// bilinear interpolation and its normalised transpose, and the coarsest grid
-// is solved with an explicit dense inverse.
+// cell); transfers are bilinear interpolation and its normalised transpose.
+//
+// Velocity block: the finest level is matrix-free (the exact rows) with
+// red-black Gauss-Seidel per component; every coarser level is the Galerkin
+// product R A P stored as an explicit 5x5-per-component stencil, computed on
+// the device each solve, and smoothed by per-colour Jacobi (read one buffer,
+// write the other: deterministic although the stencil couples same-colour
+// points).  Rediscretised coarse operators with averaged viscosities were
+// tried first: near viscosity jumps they are far softer (or, with max
+// averaging, stiffer) than R A P and the V-cycle diverges or stalls.  Each
+// smoothing pass ends with an exact re-solve of the pointwise boundary rows.
+// The coarsest grid is solved with an explicit dense inverse.
+//
+// Scalar operators (heat, gravity): rediscretised with restricted
+// coefficients and red-black Gauss-Seidel.
 #pragma once
 #include <vector>
 
@@ -79,6 +89,9 @@ struct VelLevel {
   DBuf<T> kkx, kky;
   DBuf<T> dx, dy;           // diagonals
   DBuf<T> ux, uy, fx, fy, rx, ry;
+  DBuf<T> ux2, uy2;         // second buffers of the coarse-level smoother
+  DBuf<T> S;                // explicit stencil: level 0 probed (3x3), coarse levels Galerkin (5x5)
+  int W = 2;                // stencil half-width
   Map1 mxn, mxm, myn, mym;  // maps to the next coarser level: x nodes/mids, y nodes/mids
   std::vector<int8_t> hkx, hky;
   std::vector<int> hnbx, hnby;
@@ -93,6 +106,7 @@ struct VelMG {
   DBuf<double> dense;       // n x 2n work
   DBuf<T> ainv;             // n x n
   std::vector<DBuf<int>> gfi, gfj;  // fine line index of each coarse line, per level
+  DBuf<int> overflow;       // Galerkin entries that fell outside the 5x5 window (must stay 0)
   double mu = 0, dt = 0;
   int nsmooth = 2;
   int setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,

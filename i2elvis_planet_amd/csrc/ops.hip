@@ -207,6 +207,25 @@ __global__ void k_grav_apply(Grid g, GravGeo q, const double *__restrict__ x, do
   y[m] = r;
 }
 
+// 1/diagonal of every gravity row (row scaling of the solve)
+__global__ void k_grav_rowscale(Grid g, GravGeo q, double *sc) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int X = g.X, Y = g.Y;
+  if (j >= Y) return;
+  const int m = i * Y + j;
+  double d;
+  if (i > 0 && j > 0 && !(i == 1 || i == X - 1 || j == 1 || j == Y - 1)) {
+    const double *gx = g.gx, *gy = g.gy;
+    double dx02 = (gx[i] - gx[i - 2]) / 2.0, dx24 = (gx[i + 1] - gx[i - 1]) / 2.0;
+    double dy12 = (gy[j] - gy[j - 2]) / 2.0, dy23 = (gy[j + 1] - gy[j - 1]) / 2.0;
+    d = -2.0 / dx02 / (dx02 + dx24) - 2.0 / dx24 / (dx02 + dx24) - 2.0 / dy12 / (dy12 + dy23) -
+        2.0 / dy23 / (dy12 + dy23);
+    if (grav_dist(g, i, j) > q.R) d += q.mu * 1e+18;
+  } else
+    d = q.mu;
+  sc[m] = 1.0 / d;
+}
+
 __global__ void k_grav_rhs(Grid g, GravGeo q, const double *__restrict__ ro, double *__restrict__ b) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   const int X = g.X, Y = g.Y;
@@ -314,6 +333,23 @@ __global__ void k_heat_apply(Grid g, NodeF F, BcTab bc, double dt, const double 
     }
   }
   y[m] = r;
+}
+
+// 1/diagonal of every heat row
+__global__ void k_heat_rowscale(Grid g, NodeF F, BcTab bc, double dt, double *sc) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int X = g.X, Y = g.Y, N = X * Y;
+  if (j >= Y) return;
+  const int m = i * Y + j;
+  if (bc.bondm[3 * N + m]) {
+    sc[m] = 1.0;
+    return;
+  }
+  const double *kt = F.kt, *gx = g.gx, *gy = g.gy;
+  double xkf = (gx[i + 1] - gx[i - 1]) / 2.0, ykf = (gy[j + 1] - gy[j - 1]) / 2.0;
+  double kR = (kt[m] + kt[m + Y]) / 2.0 / (gx[i + 1] - gx[i]), kL = (kt[m - Y] + kt[m]) / 2.0 / (gx[i] - gx[i - 1]);
+  double kB = (kt[m] + kt[m + 1]) / 2.0 / (gy[j + 1] - gy[j]), kT = (kt[m - 1] + kt[m]) / 2.0 / (gy[j] - gy[j - 1]);
+  sc[m] = 1.0 / (1.0 / dt + ((kR + kL) / xkf + (kB + kT) / ykf) / F.cp[m] / F.ro[m]);
 }
 
 __global__ void k_heat_rhs(Grid g, NodeF F, BcTab bc, double dt, const double *sol1, double *rhs) {

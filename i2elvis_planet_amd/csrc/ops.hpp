@@ -18,6 +18,8 @@ __global__ void k_maxvel(int X, int Y, const double *vx, const double *vy, doubl
 __global__ void k_grav_mean(Grid g, const double *ro, double R, double *part);
 __global__ void k_grav_apply(Grid g, GravGeo q, const double *x, double *y);
 __global__ void k_grav_rhs(Grid g, GravGeo q, const double *ro, double *b);
+__global__ void k_grav_rowscale(Grid g, GravGeo q, double *sc);
+__global__ void k_heat_rowscale(Grid g, NodeF F, BcTab bc, double dt, double *sc);
 __global__ void k_grav_ga(Grid g, const double *gp, double *ga);
 __global__ void k_heat_source(const i2e_params *p, Grid g, NodeF F, BcTab bc, double timesum, double *sol1,
                               double *out);

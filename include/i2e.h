@@ -135,7 +135,7 @@ enum { I2E_PREC_FP64 = 0, I2E_PREC_MIXED = 1 };
 
 typedef struct i2e_solver_opts {
   int32_t precision;       /* I2E_PREC_FP64 or I2E_PREC_MIXED                 */
-  int32_t restart;         /* FGMRES restart length                          */
+  int32_t restart;         /* FGMRES restart length (1..400, default 200)    */
   int32_t maxit;           /* max outer iterations per solve                 */
   int32_t nsmooth;         /* smoothing sweeps per level (pre = post)        */
   double stokes_rtol;      /* ||b-Ax||/||b|| target for Stokes               */
@@ -192,6 +192,13 @@ int i2e_stokes_apply_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
  * (heatmart.c:406-432, needs titerate's RHS state); N-length arrays.        */
 int i2e_gravity_apply_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
 int i2e_heat_apply_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
+/* Diagnostics of the Stokes solver (no counterpart in the reference): the
+ * right-hand side of viterate's rows, one application of the block
+ * preconditioner z = P^-1 r (3N interleaved), and one velocity V-cycle on
+ * f = [fx(N), fy(N)] -> u = [ux(N), uy(N)].  All device pointers.           */
+int i2e_stokes_rhs_dev(i2e_ctx *c, double *b_dev);
+int i2e_stokes_precond_dev(i2e_ctx *c, const double *r_dev, double *z_dev);
+int i2e_vcycle_dev(i2e_ctx *c, const double *f_dev, double *u_dev);
 /* Device pointers of the context's node/marker arrays, by reference name
  * ("vx", "tk", "markx", ...); NULL if unknown.                              */
 void *i2e_device_ptr(i2e_ctx *c, const char *name);
@@ -200,6 +207,14 @@ void *i2e_device_ptr(i2e_ctx *c, const char *name);
  * and returns the mean kernel milliseconds.                                 */
 int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g,
                      double *ms_stokes_apply);
+
+/* Device buffers for callers without a device allocator of their own (no
+ * counterpart in the reference): allocation, release and blocking copies,
+ * all through the same HIP runtime as the library itself.                   */
+int i2e_dev_alloc(size_t bytes, void **ptr);
+int i2e_dev_free(void *ptr);
+int i2e_copy_to_dev(void *dst_dev, const void *src_host, size_t bytes);
+int i2e_copy_to_host(void *dst_host, const void *src_dev, size_t bytes);
 
 #ifdef __cplusplus
 }

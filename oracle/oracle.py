@@ -11,6 +11,7 @@ otherwise scipy's SuperLU.
 """
 import ctypes as C
 import os
+import sys
 import subprocess
 
 import numpy as np
@@ -158,9 +159,18 @@ def pardiso_solve(A, b):
 SOLVER = os.environ.get("I2E_ORACLE_SOLVER", "auto")
 
 
+def _use_pardiso():
+    """PARDISO unless torch is loaded: the MKL linked into the torch wheel clashes
+    with the standalone MKL runtime (PARDISO then fails or exhausts host memory),
+    so with torch in the process "auto" falls back to SuperLU."""
+    if SOLVER == "pardiso":
+        return True
+    return SOLVER == "auto" and "torch" not in sys.modules
+
+
 def solve(M):
     A = M.scipy()
-    if SOLVER in ("auto", "pardiso"):
+    if _use_pardiso():
         try:
             x = pardiso_solve(A, M.b)
         except RuntimeError:
@@ -171,11 +181,28 @@ def solve(M):
             return x
         if SOLVER == "pardiso":
             raise RuntimeError("MKL runtime not found")
-    return spla.spsolve(A.tocsc(), M.b)
+    return lu_solve(A, M.b)
+
+
+def lu_solve(A, b, refine=3):
+    """SuperLU with up to `refine` steps of iterative refinement, keeping the
+    iterate of smallest residual (PARDISO refines the same way, iparm[7])."""
+    A = A.tocsc()
+    lu = spla.splu(A)
+    x = lu.solve(b)
+    r = b - A @ x
+    best, bn = x.copy(), np.linalg.norm(r)
+    for _ in range(refine):
+        x = x + lu.solve(r)
+        r = b - A @ x
+        rn = np.linalg.norm(r)
+        if rn < bn:
+            best, bn = x.copy(), rn
+    return best
 
 
 def solver_kind():
-    return "pardiso" if (SOLVER in ("auto", "pardiso") and _mkl_lib() is not None) else "superlu"
+    return "pardiso" if (_use_pardiso() and _mkl_lib() is not None) else "superlu"
 
 
 # --------------------------------------------------------------------------- step
@@ -255,7 +282,7 @@ class Oracle:
     def tkrecalc(self):
         M = Matrix(self.hs.N, 8)
         _check(self.L.orc_tbc_build(C.byref(self.p), self._s(), C.byref(M.c)), "tbc_build")
-        self.hs.tk[:] = spla.spsolve(M.scipy().tocsc(), M.b)
+        self.hs.tk[:] = lu_solve(M.scipy(), M.b)
 
     def step(self):
         self.hs.timestep = self.p.maxtmstep

@@ -1,0 +1,255 @@
+"""GPU parity of libi2e (through its C-ABI) against the CPU oracle on the same
+seeded inputs.  Integer/index results bit-exact, FP64 fields within stated
+relative L2 tolerances."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from i2elvis_planet_amd import Context, DeviceArray, configs, default_solver
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    n = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / n if n > 0 else np.linalg.norm(a - b)
+
+
+def fp64_opts(rtol=1e-12):
+    o = default_solver()
+    o.precision = 0
+    o.stokes_rtol = rtol
+    o.heat_rtol = 1e-13
+    o.grav_rtol = 1e-13
+    return o
+
+
+def small_cfg(name, **kw):
+    cfg = dict(configs.CONFIGS[name], **kw)
+    key = "_t_" + name
+    configs.CONFIGS[key] = cfg
+    try:
+        return configs.make(key)
+    finally:
+        del configs.CONFIGS[key]
+
+
+def rotation_field(hs, om=2e-13, noise=0.0, seed=0):
+    X, Y = hs.xnumx, hs.ynumy
+    c = hs.gx[-1] / 2
+    xm = (hs.gx[:-1] + hs.gx[1:]) / 2
+    ym = (hs.gy[:-1] + hs.gy[1:]) / 2
+    vx = np.zeros((X, Y))
+    vy = np.zeros((X, Y))
+    vx[:, :Y - 1] = -om * (ym[None, :] - c) + 0 * hs.gx[:, None]
+    vy[:X - 1, :] = om * (xm[:, None] - c) + 0 * hs.gy[None, :]
+    if noise:
+        rng = np.random.default_rng(seed)
+        vx[:, :Y - 1] *= 1 + noise * rng.standard_normal((X, Y - 1))
+        vy[:X - 1, :] *= 1 + noise * rng.standard_normal((X - 1, Y))
+    hs.vx[:] = vx.ravel()
+    hs.vy[:] = vy.ravel()
+
+
+# ------------------------------------------------------------------ markers
+@pytest.mark.parametrize("name", ["c1", "c3"])
+def test_movemark_bitexact(orc, name):
+    say("case movemark", name)
+    p, hs, _ = configs.make(name)
+    rotation_field(hs, noise=0.3)
+    rng = np.random.default_rng(3)
+    hs.pr[:] = 1e6 * rng.random(hs.N)
+    hs.ga[:] = rng.random(hs.N)
+    hs.timestep = 0.05 * (hs.gx[1] - hs.gx[0]) / np.abs(hs.vx).max()
+    ref = hs.copy()
+    ctx = Context(p, hs, solver=fp64_opts())
+    ctx.set_scalars(hs.timestep, 0.0)
+    ctx.movemark()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    orc.Oracle(p, ref).movemark()
+    assert np.array_equal(got.markx, ref.markx)
+    assert np.array_equal(got.marky, ref.marky)
+    assert np.array_equal(got.markk, ref.markk)
+    assert np.array_equal(got.markp, ref.markp)
+    np.testing.assert_allclose(got.markv, ref.markv, rtol=2e-7)
+    # cell indices and per-cell type counts, bit-exact
+    ca = orc.Oracle(p, got).type_counts()
+    cb = orc.Oracle(p, ref).type_counts()
+    assert np.array_equal(ca, cb)
+
+
+@pytest.mark.parametrize("name,kw", [("c1", {}), ("c2", dict(n=81, size=160e3, radius=40e3)),
+                                     ("c3", dict(n=101, size=120e3, radius=30e3))])
+def test_ronurecalc_parity(orc, name, kw):
+    say("case ronurecalc", name)
+    p, hs, _ = small_cfg(name, **kw)
+    rng = np.random.default_rng(4)
+    # temperatures across solidus/liquidus and the 700 K sintering switch
+    hs.markk[:] = rng.uniform(150, 1900, hs.marknum).astype(np.float32)
+    hs.pr[:] = 1e7 * rng.random(hs.N)
+    hs.timestep = 7.9e10
+    hs.timesum = 3e12
+    ref = hs.copy()
+    ctx = Context(p, hs, solver=fp64_opts())
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.ronurecalc()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    orc.Oracle(p, ref).ronurecalc()
+    for f in ("nu", "nd", "ro", "cp", "kt", "ht", "tk", "et"):
+        assert rel(getattr(got, f), getattr(ref, f)) < 1e-10, f
+    assert np.array_equal(got.markt, ref.markt)
+    assert np.array_equal(got.markk, ref.markk)
+    np.testing.assert_allclose(got.markpor, ref.markpor, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(got.marktmax, ref.marktmax, rtol=0)
+
+
+# ------------------------------------------------------------------ operators
+def _dev(x):
+    return DeviceArray(x)
+
+
+@pytest.mark.parametrize("name", ["c1", "c2"])
+def test_operator_apply_matches_assembled_rows(orc, name):
+    say("case operators", name)
+    p, hs, _ = configs.make(name)
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.gravity()
+    ctx = Context(p, hs, solver=fp64_opts())
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    rng = np.random.default_rng(5)
+    # Stokes rows: scale unknowns like the solution (P ~ 1e7, v ~ 1e-10)
+    M = o.stokes_matrix()
+    x = rng.standard_normal(3 * hs.N)
+    x[0::3] *= 1e7
+    x[1::3] *= 1e-10
+    x[2::3] *= 1e-10
+    y_ref = M.matvec(x)
+    xd, yd = _dev(x), _dev(np.zeros_like(x))
+    ctx.apply("stokes", xd.ptr, yd.ptr)
+    y = yd.numpy()
+    # row-wise bound relative to sum |a_ij x_j|
+    A = M.scipy()
+    scale = abs(A) @ np.abs(x)
+    assert np.all(np.abs(y - y_ref) <= 1e-13 * scale + 1e-300)
+    # gravity rows
+    G = orc.Matrix(hs.N, 12)
+    import ctypes as C
+    assert orc.lib().orc_gravity_build(C.byref(p), C.byref(hs.cstruct()), C.byref(G.c)) == 0
+    xg = rng.standard_normal(hs.N)
+    yd = _dev(np.zeros(hs.N))
+    ctx.apply("gravity", _dev(xg).ptr, yd.ptr)
+    scale = abs(G.scipy()) @ np.abs(xg)
+    assert np.all(np.abs(yd.numpy() - G.matvec(xg)) <= 1e-13 * scale)
+    # heat rows at this timestep
+    H = orc.Matrix(hs.N, 12)
+    sol1 = np.zeros(hs.N)
+    assert orc.lib().orc_heat_build(C.byref(p), C.byref(hs.cstruct()), sol1.ctypes.data_as(C.POINTER(C.c_double)),
+                                    C.byref(H.c)) == 0
+    xh = 150 + rng.standard_normal(hs.N)
+    yd = _dev(np.zeros(hs.N))
+    ctx.apply("heat", _dev(xh).ptr, yd.ptr)
+    scale = abs(H.scipy()) @ np.abs(xh)
+    assert np.all(np.abs(yd.numpy() - H.matvec(xh)) <= 1e-13 * scale)
+    ctx.close()
+
+
+# ------------------------------------------------------------------ solves
+def test_viterate_parity_fp64(orc):
+    say("case viterate c1")
+    p, hs, _ = configs.make("c1")
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    ctx = Context(p, hs, solver=fp64_opts(1e-13))
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.viterate()
+    st = ctx.stats()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o.viterate()
+    say("stokes its", st.stokes_iters, "relres", st.stokes_relres, "grav its", st.grav_iters)
+    assert st.stokes_relres <= 1e-13
+    assert got.gxkoef == pytest.approx(hs.gxkoef, rel=1e-13)
+    for f in ("gp", "vx", "vy", "exx", "eyy", "exy", "sxy"):
+        assert rel(getattr(got, f), getattr(hs, f)) < 1e-8, f
+    # pressure is fixed only through the anchor cell: compare it in L2 too
+    assert rel(got.pr, hs.pr) < 1e-8
+
+
+def test_titerate_parity_fp64(orc):
+    say("case titerate c1")
+    p, hs, _ = configs.make("c1")
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    o.viterate()
+    hs.timesum = 5e12  # switches shear heating on
+    ref = hs.copy()
+    ctx = Context(p, hs, solver=fp64_opts())
+    ctx.set_scalars(p.maxtmstep, hs.timesum)
+    ctx.maxvelstep()
+    ctx.titerate()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    oo = orc.Oracle(p, ref)
+    ref.timestep = p.maxtmstep
+    oo.maxvelstep()
+    oo.titerate()
+    assert got.timestep == ref.timestep
+    assert rel(got.tk, ref.tk) < 1e-10
+    np.testing.assert_allclose(got.markk, ref.markk, rtol=1e-6)
+
+
+@pytest.mark.parametrize("name,nsteps", [("c1", 3)])
+def test_full_step_parity_fp64(orc, name, nsteps):
+    say("case steps", name, nsteps)
+    p, hs, _ = configs.make(name)
+    ref = hs.copy()
+    ctx = Context(p, hs, solver=fp64_opts(1e-13))
+    ctx.ronurecalc()
+    o = orc.Oracle(p, ref)
+    o.ronurecalc()
+    for k in range(nsteps):
+        ctx.step()
+        o.step()
+        got = ctx.download(hs.copy())
+        say("step", k, "dt", got.timestep, ref.timestep, "vx", rel(got.vx, ref.vx), "tk", rel(got.tk, ref.tk))
+        assert got.timestep == pytest.approx(ref.timestep, rel=1e-10)
+        for f in ("vx", "vy", "pr", "tk", "gp"):
+            assert rel(getattr(got, f), getattr(ref, f)) < 1e-8, (k, f)
+        assert np.array_equal(got.markt, ref.markt)
+        assert np.array_equal(o.type_counts(), orc.Oracle(p, got).type_counts())
+    ctx.close()
+
+
+def test_mixed_precision_error_vs_fp64(orc):
+    say("case mixed vs fp64 c1")
+    p, hs, _ = configs.make("c1")
+    a = Context(p, hs.copy(), solver=fp64_opts(1e-8))
+    om = default_solver()
+    b = Context(p, hs.copy(), solver=om)
+    for c in (a, b):
+        c.ronurecalc()
+        c.step()
+    ga = a.download(hs.copy())
+    gb = b.download(hs.copy())
+    sb = b.stats()
+    a.close()
+    b.close()
+    assert sb.stokes_relres <= 1e-8
+    for f in ("vx", "vy", "pr", "tk"):
+        e = rel(gb.__dict__[f], ga.__dict__[f])
+        say("mixed vs fp64", f, e)
+        assert e < 1e-5, f

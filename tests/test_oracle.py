@@ -330,3 +330,28 @@ def test_heat_uniform_source_energy(orc):
     assert np.allclose(sol1[interior], 1e-7 / 1000.0, rtol=1e-14)
     # tktimestep = maxtkstep / max rate (heatmart.c:378-386)
     assert tk_ts.value == pytest.approx(p.maxtkstep / (1e-7 / 1000.0), rel=1e-12)
+
+
+def test_direct_solvers_agree_on_stokes(orc):
+    """The oracle's Stokes solution does not depend on which exact sparse LU
+    solves it (PARDISO as gausmart.c:203-260 calls it, or SuperLU): the
+    residual of both is at round-off and the fields agree far below the
+    tolerances of the GPU parity tests."""
+    import sys
+    p, hs, _ = configs.make("c1")
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    o.gravity()
+    M = o.stokes_matrix()
+    A = M.scipy().tocsc()
+    xs = orc.lu_solve(A, M.b)
+    assert np.linalg.norm(A @ xs - M.b) <= 1e-12 * np.linalg.norm(M.b)
+    if "torch" in sys.modules or orc._mkl_lib() is None:
+        pytest.skip("standalone MKL runtime unusable in this process")
+    xp = orc.pardiso_solve(M.scipy(), M.b)
+    assert np.linalg.norm(A @ xp - M.b) <= 1e-12 * np.linalg.norm(M.b)
+    for k in (1, 2):  # velocities
+        assert np.linalg.norm(xp[k::3] - xs[k::3]) <= 1e-9 * np.linalg.norm(xs[k::3])
+    assert np.linalg.norm(xp[0::3] - xs[0::3]) <= 1e-9 * np.linalg.norm(xs[0::3])
