@@ -27,9 +27,9 @@ def log(*a):
 
 
 def cpu_baseline(sample_steps=1):
-    """Oracle step (C restatement + PARDISO/SuperLU direct solves, as the
-    reference factorises every step) on the same 501x501 input, timed on
-    this host's cores."""
+    """One full step of the CPU oracle on the same 501x501 / 6.25M-marker
+    input (serial C marker loops and node assembly, PARDISO direct solves
+    as the reference factorises every step), timed on this host's cores."""
     from i2elvis_planet_amd import configs
     from oracle import oracle
     p, hs, _ = configs.make("c3")
@@ -41,11 +41,24 @@ def cpu_baseline(sample_steps=1):
     dt = time.time() - t
     threads = int(os.environ.get("MKL_NUM_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     return {"value": sample_steps / dt, "unit": "timesteps/s", "cores": threads, "kind": "port",
-            "sample": "%d step(s) of the 501x501 / 6.25M-marker setup from its initial state; marker loops "
-                      "serial C, sparse solves %s with %d threads" % (sample_steps, oracle.solver_kind(), threads)}
+            "sample": "%d full step(s) of the 501x501 / 6.25M-marker setup after its initial ronurecalc "
+                      "(%.1f s): marker loops and assembly serial C, sparse solves %s with %d threads"
+                      % (sample_steps, dt, oracle.solver_kind(), threads)}
 
 
-def run(steps, warmup, skip_cpu):
+# algorithmic-byte roofline kernels (HBM bound), selectable with --roofline-kernel
+ROOFLINE_KERNELS = {
+    "gsdot": "k_gsdot (Gram-Schmidt projection pass of FGMRES)",
+    "gsaxpy": "k_gsaxpy (Gram-Schmidt update pass of FGMRES)",
+    "vsmoothc": "k_sjac (coarse-level velocity smoother colour step)",
+    "vsmooth0": "k_vgs (finest-level velocity smoother colour step)",
+    "stokes_apply": "k_stokes_apply (matrix-free Stokes operator)",
+    "advect": "k_movemark (RK4 marker advection)",
+    "p2g": "k_ronu_markers (marker-to-node scatter)",
+}
+
+
+def run(steps, warmup, skip_cpu, rk):
     import numpy as np
     from i2elvis_planet_amd import Context, configs, default_solver
 
@@ -61,11 +74,14 @@ def run(steps, warmup, skip_cpu):
     ctx.sync()
     snap = ctx.download(hs.copy())
     res = {}
+    roof = None
     for mode in ("mixed", "fp64"):
         o.precision = 1 if mode == "mixed" else 0
         ctx.set_solver(o)
         ctx.upload(snap)
         ctx.sync()
+        if mode == "mixed":
+            ctx.kernel_timer(rk, 1 << 17)
         iters = []
         t0 = time.perf_counter()
         for k in range(steps):
@@ -75,28 +91,21 @@ def run(steps, warmup, skip_cpu):
                           st.ms_grav, st.ms_markers))
         ctx.sync()
         el = time.perf_counter() - t0
+        if mode == "mixed":
+            roof = ctx.kernel_timer_read()
+            ctx.kernel_timer(None, 0)
         res[mode] = (steps / el, el * 1e3 / steps, iters)
         log("%s: %.3f steps/s, %.1f ms/step; stokes its %s" % (mode, steps / el, el * 1e3 / steps,
                                                                 [i[0] for i in iters]))
-    # per-kernel timing with HIP events on the context stream
+    # standalone kernel timings (HIP events, outside the timed region)
     ms_adv, ms_p2g, ms_sap = ctx.time_kernels(5)
-    N, M = p.xnumx * p.ynumy, hs.marknum
-    kernels = {
-        # algorithmic bytes: markers 25 B (x,y,type,T read; x,y,eta,P written), nodes vx,vy 16 B
-        "advect_rk4": {"ms": ms_adv, "bytes": 25 * M + 16 * N},
-        # markers 30 B, 12 FP64 node accumulators written
-        "p2g_scatter": {"ms": ms_p2g, "bytes": 30 * M + 96 * N},
-        # FP64 Stokes apply: x (24 B) + nu, nd, ro, gp (32 B) read, y (24 B) written per node
-        "stokes_apply_fp64": {"ms": ms_sap, "bytes": 80 * N},
-    }
-    for k, v in kernels.items():
-        v["gbs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
-    dom = "advect_rk4"
-    d = kernels[dom]
     mixed_sps, mixed_ms, it = res["mixed"]
     fp64_sps = res["fp64"][0]
+    launches, ms_tot, bytes_tot = roof
+    avg_ms = ms_tot / max(launches, 1)
+    achieved = bytes_tot / max(ms_tot, 1e-30) * 1e3 / 1e9
     line = {
-        "metric": "timesteps/s @501², 6.3M markers: mixed vs FP64 GPU at Stokes rel.resid 1e-8",
+        "metric": "timesteps/s @501\u00b2, 6.3M markers: mixed vs FP64 GPU at Stokes rel.resid 1e-8",
         "value": mixed_sps,
         "unit": "timesteps/s",
         "n_gpus": 1,
@@ -119,9 +128,12 @@ def run(steps, warmup, skip_cpu):
         "phase_ms_mixed": {"stokes": float(np.mean([i[4] for i in it])), "heat": float(np.mean([i[5] for i in it])),
                            "gravity": float(np.mean([i[6] for i in it])),
                            "markers": float(np.mean([i[7] for i in it]))},
-        "kernels": {k: {"ms": round(v["ms"], 4), "GB/s": round(v["gbs"], 1)} for k, v in kernels.items()},
-        "roofline": {"bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": d["gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom},
+        "kernels_standalone_ms": {"advect_rk4": round(ms_adv, 4), "p2g_scatter": round(ms_p2g, 4),
+                                  "stokes_apply_fp64": round(ms_sap, 4)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": ROOFLINE_KERNELS[rk],
+                     "launches": launches, "avg_us": avg_ms * 1e3,
+                     "bytes_per_launch": bytes_tot / max(launches, 1)},
         "cpu_baseline": None,
     }
     ctx.close()
@@ -139,12 +151,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-kernel", default="gsdot", choices=sorted(ROOFLINE_KERNELS))
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs one GPU only: the multi-GPU decomposition of this step is not built "
               "(see the sharding notes); use --gpus 1", file=sys.stderr)
         sys.exit(2)
-    line = run(a.steps, a.warmup, a.no_cpu)
+    line = run(a.steps, a.warmup, a.no_cpu, a.roofline_kernel)
     print(json.dumps(line), flush=True)
 
 

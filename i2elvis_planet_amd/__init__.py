@@ -26,6 +26,7 @@ _EXPORTS = [
     "i2e_stokes_apply_dev", "i2e_gravity_apply_dev", "i2e_heat_apply_dev", "i2e_device_ptr", "i2e_time_kernels",
     "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
+    "i2e_kernel_timer", "i2e_kernel_timer_read",
 ]
 
 _lib = None
@@ -73,6 +74,8 @@ def load_library(path=LIB_PATH):
         "i2e_stokes_rhs_dev": ([vp, vp], C.c_int),
         "i2e_stokes_precond_dev": ([vp, vp, vp], C.c_int),
         "i2e_vcycle_dev": ([vp, vp, vp], C.c_int),
+        "i2e_kernel_timer": ([vp, C.c_char_p, C.c_int], C.c_int),
+        "i2e_kernel_timer_read": ([vp, C.POINTER(C.c_int64), dp, dp], C.c_int),
         "i2e_dev_alloc": ([C.c_size_t, C.POINTER(vp)], C.c_int),
         "i2e_dev_free": ([vp], C.c_int),
         "i2e_copy_to_dev": ([vp, vp, C.c_size_t], C.c_int),
@@ -230,6 +233,16 @@ class Context:
 
     def vcycle(self, f_dev, u_dev):
         _chk(_lib.i2e_vcycle_dev(self._h, C.c_void_p(f_dev), C.c_void_p(u_dev)), "vcycle")
+
+    def kernel_timer(self, kernel, max_launches=65536):
+        """Time every launch of `kernel` from now on with HIP events (None stops)."""
+        _chk(_lib.i2e_kernel_timer(self._h, (kernel or "").encode(), max_launches), "kernel_timer")
+
+    def kernel_timer_read(self):
+        """(launches, total ms, total algorithmic bytes) since kernel_timer()."""
+        n, ms, b = C.c_int64(), C.c_double(), C.c_double()
+        _chk(_lib.i2e_kernel_timer_read(self._h, C.byref(n), C.byref(ms), C.byref(b)), "kernel_timer_read")
+        return n.value, ms.value, b.value
 
     def time_kernels(self, reps=5):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

@@ -9,6 +9,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <vector>
+
 #include "../../include/i2e.h"
 
 #define I2E_HD __host__ __device__ __forceinline__
@@ -31,6 +33,36 @@ void set_error(const char *fmt, ...);
     int s_ = (call);             \
     if (s_ != I2E_OK) return s_; \
   } while (0)
+
+// Per-launch HIP-event timing of one kernel class on its launch stream, for
+// roofline figures measured inside a normal run.  Host-side only: launch
+// sites bracket the launch with begin/end and state its algorithmic bytes.
+enum KId { K_ADVECT = 0, K_P2G, K_GSDOT, K_GSAXPY, K_VSMOOTH0, K_VSMOOTHC, K_STOKES_APPLY, K_COUNT };
+
+struct KTimer {
+  int target = -1;
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  int64_t launches = 0;
+  double bytes = 0;
+  bool active(int id) const { return id == target && used + 2 <= ev.size(); }
+  void begin(int id, hipStream_t s) {
+    if (active(id)) (void)hipEventRecord(ev[used], s);
+  }
+  void end(int id, hipStream_t s, double b) {
+    if (!active(id)) return;
+    (void)hipEventRecord(ev[used + 1], s);
+    used += 2;
+    launches++;
+    bytes += b;
+  }
+};
+
+// butterfly sum over the 64 lanes of a wavefront (fixed order: reproducible)
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 
 // Grid lines of one level.  Node m = m1*Y + m2.
 struct Grid {

@@ -192,6 +192,7 @@ struct i2e_ctx {
   i2e_stats st;
   double timestep = 0, timesum = 0, gxkoef = 1.0;
   HBuf<double> pin;
+  KTimer kt;
 
   Grid grid() const { return Grid{X, Y, P.xsize, P.ysize, gx.p, gy.p}; }
   SOp<double> sop() const { return SOp<double>{X, Y, gx.p, gy.p, F.nu, F.nd, F.ro, F.gp, timestep, P.nubeg * P.strmin}; }
@@ -249,6 +250,9 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
     return s_;
   };
   if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return fail(I2E_ERR_HIP);
+  c->fs.kt = c->fh.kt = c->fgv.kt = &c->kt;
+  c->v32.kt = &c->kt;
+  c->v64.kt = &c->kt;
   if (hipMalloc((void **)&c->dP, sizeof(i2e_params)) != hipSuccess) return fail(I2E_ERR_HIP);
   if (hipMemcpy(c->dP, p, sizeof(i2e_params), hipMemcpyHostToDevice) != hipSuccess) return fail(I2E_ERR_HIP);
   const int N = c->N;
@@ -283,6 +287,8 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
 int i2e_destroy(i2e_ctx *c) {
   if (!c) return I2E_OK;
   if (c->s) hipStreamSynchronize(c->s);
+  for (auto e : c->kt.ev) (void)hipEventDestroy(e);
+  c->kt.ev.clear();
   c->v32.release();
   c->v64.release();
   c->h32.release();
@@ -655,8 +661,13 @@ static int stokes_solve(i2e_ctx *c) {
   const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
   ST_OK(stokes_prepare(c));
+  // algorithmic bytes: x and y (3 FP64 unknowns per node), nu, nd, ro, gp
+  // (FP64 per node) and the 4 BC row numbers (int32) per node
+  const double bytes = (24.0 + 24.0 + 32.0 + 16.0) * (double)c->N;
   auto A = [&](const double *x, double *y) -> int {
+    c->kt.begin(K_STOKES_APPLY, c->s);
     hipLaunchKernelGGL(k_stokes_apply, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, x, y);
+    c->kt.end(K_STOKES_APPLY, c->s, bytes);
     return I2E_OK;
   };
   auto Mf = [&](const double *r, double *z) -> int { return stokes_precond(c, r, z); };
@@ -769,12 +780,28 @@ int i2e_titerate(i2e_ctx *c, int n0) {
   return I2E_OK;
 }
 
+// Algorithmic HBM bytes of the marker kernels: every marker field they read
+// or write once, every node field they gather once.
+// movemark: reads type (1 B), x, y, T (12 B); writes P, eta, x, y (16 B);
+// gathers vx, vy, P (+ exx, eyy, exy for strain-rate-dependent viscosity).
+static double advect_bytes(const i2e_ctx *c) {
+  return 29.0 * (double)c->M + (24.0 + 24.0 * c->need_eii) * (double)c->N;
+}
+// ronurecalc marker pass: reads x, y, T, eta, P, porosity, Tmax, type
+// (29 B), writes type, porosity, Tmax (9 B); 12 FP64 node accumulators
+// written once, node P, T (+ strain rates) gathered.
+static double p2g_bytes(const i2e_ctx *c) {
+  return 38.0 * (double)c->M + (96.0 + 16.0 + 24.0 * c->need_eii) * (double)c->N;
+}
+
 int i2e_movemark(i2e_ctx *c) {
   if (!c) return I2E_ERR_ARG;
   if (!c->M) return I2E_OK;
   unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+  c->kt.begin(K_ADVECT, c->s);
   hipLaunchKernelGGL(k_movemark, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->timestep,
                      c->need_eii);
+  c->kt.end(K_ADVECT, c->s, advect_bytes(c));
   HIP_OK(hipGetLastError());
   return I2E_OK;
 }
@@ -787,8 +814,10 @@ int i2e_ronurecalc(i2e_ctx *c) {
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->s, 1, 0.0, c->red.p + 17);
   if (c->M) {
     unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+    c->kt.begin(K_P2G, c->s);
     hipLaunchKernelGGL(k_ronu_markers, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->A,
                        c->timesum, c->timestep, c->need_eii);
+    c->kt.end(K_P2G, c->s, p2g_bytes(c));
   }
   hipLaunchKernelGGL(k_ronu_nodes, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->dP, c->F, c->A, N, c->red.p + 16);
   hipLaunchKernelGGL(k_contrast, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->dP, c->F, N, c->red.p + 16);
@@ -908,6 +937,47 @@ int i2e_copy_to_dev(void *dst_dev, const void *src_host, size_t bytes) {
 int i2e_copy_to_host(void *dst_host, const void *src_dev, size_t bytes) {
   if ((!dst_host || !src_dev) && bytes) return I2E_ERR_ARG;
   if (bytes) HIP_OK(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost));
+  return I2E_OK;
+}
+
+int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches) {
+  if (!c || max_launches < 0) return I2E_ERR_ARG;
+  static const char *names[K_COUNT] = {"advect", "p2g", "gsdot", "gsaxpy", "vsmooth0", "vsmoothc", "stokes_apply"};
+  int id = -1;
+  if (kernel && kernel[0]) {
+    for (int k = 0; k < K_COUNT; k++)
+      if (!strcmp(kernel, names[k])) id = k;
+    if (id < 0) {
+      set_error("unknown kernel timer %s", kernel);
+      return I2E_ERR_ARG;
+    }
+  }
+  HIP_OK(hipStreamSynchronize(c->s));
+  const size_t need = 2 * (size_t)max_launches;
+  while (c->kt.ev.size() < need) {
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    c->kt.ev.push_back(e);
+  }
+  c->kt.target = id;
+  c->kt.used = 0;
+  c->kt.launches = 0;
+  c->kt.bytes = 0;
+  return I2E_OK;
+}
+
+int i2e_kernel_timer_read(i2e_ctx *c, int64_t *launches, double *ms_total, double *bytes_total) {
+  if (!c) return I2E_ERR_ARG;
+  double tot = 0;
+  if (c->kt.used) HIP_OK(hipEventSynchronize(c->kt.ev[c->kt.used - 1]));
+  for (size_t k = 0; k + 1 < c->kt.used; k += 2) {
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, c->kt.ev[k], c->kt.ev[k + 1]));
+    tot += ms;
+  }
+  if (launches) *launches = c->kt.launches;
+  if (ms_total) *ms_total = tot;
+  if (bytes_total) *bytes_total = c->kt.bytes;
   return I2E_OK;
 }
 

@@ -3,9 +3,10 @@
 // Replaces the direct PARDISO solves of gausmat4 (gausmart.c:174-318).  Each
 // restart recomputes the true FP64 residual b - A x, which makes the restart
 // loop an iterative refinement around whatever precision the preconditioner
-// runs in.  Orthogonalisation is classical Gram-Schmidt applied twice (two
-// fused multi-dot / multi-axpy passes over the basis); all reductions go
-// through fixed-order block partials, so results are bitwise reproducible.
+// runs in.  All reductions go through fixed-order block partials, so results are bitwise reproducible.
+// Orthogonalisation is classical Gram-Schmidt (one fused projection pass and
+// one fused update pass, each a single read of the basis), repeated when the
+// first pass cancelled most of the vector.
 #include <cmath>
 #include <cstdlib>
 
@@ -13,51 +14,100 @@
 
 namespace i2e {
 
-static constexpr int RB = 256;   // threads per block
-static constexpr int RG = 480;   // reduction blocks
+static constexpr int RB = 256;        // threads per block
+static constexpr int EPT = 4;        // vector elements per thread in the Gram-Schmidt kernels
+static constexpr int CH = RB * EPT;  // vector elements per block
 
-// part[i*RG + blk] = sum_k V[i][k] w[k] over this block's slice, i < nv
-__global__ void k_multidot(int n, int nv, const double *__restrict__ V, size_t ld, const double *__restrict__ w,
-                           double *part) {
-  __shared__ double sh[RB];
-  for (int i0 = 0; i0 < nv; i0 += 4) {
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    const int nb = nv - i0 < 4 ? nv - i0 : 4;
-    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += RG * RB) {
-      double wk = w[k];
-      a0 += V[(size_t)i0 * ld + k] * wk;
-      if (nb > 1) a1 += V[(size_t)(i0 + 1) * ld + k] * wk;
-      if (nb > 2) a2 += V[(size_t)(i0 + 2) * ld + k] * wk;
-      if (nb > 3) a3 += V[(size_t)(i0 + 3) * ld + k] * wk;
-    }
-    double acc[4] = {a0, a1, a2, a3};
-    for (int q = 0; q < nb; q++) {
-      sh[threadIdx.x] = acc[q];
-      __syncthreads();
-      for (int o = RB / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) part[(size_t)(i0 + q) * RG + blockIdx.x] = sh[0];
-      __syncthreads();
-    }
+static inline int gs_blocks(int n) { return (n + CH - 1) / CH; }
+
+// Classical Gram-Schmidt projection coefficients in one pass over the basis:
+// part[i*G + blk] = <V_i, w> over this block's chunk (i < nv) and
+// part[nv*G + blk] = <w, w>.  w stays in registers; each wavefront reduces
+// with cross-lane butterflies, the block's four wavefronts meet in LDS once.
+__global__ __launch_bounds__(RB) void k_gsdot(int n, int nv, const double *__restrict__ V, size_t ld,
+                                              const double *__restrict__ w, double *__restrict__ part) {
+  extern __shared__ double sh[];  // [4][nv + 1]
+  const int G = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6, S = nv + 1;
+  const size_t base = (size_t)blockIdx.x * CH + threadIdx.x;
+  double we[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    const size_t k = base + (size_t)e * RB;
+    we[e] = k < (size_t)n ? w[k] : 0.0;
   }
+  for (int i = 0; i < nv; i++) {
+    const double *v = V + (size_t)i * ld;
+    double a = 0;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const size_t k = base + (size_t)e * RB;
+      if (k < (size_t)n) a += v[k] * we[e];
+    }
+    a = wave_sum(a);
+    if (lane == 0) sh[wv * S + i] = a;
+  }
+  double a = 0;
+#pragma unroll
+  for (int e = 0; e < EPT; e++) a += we[e] * we[e];
+  a = wave_sum(a);
+  if (lane == 0) sh[wv * S + nv] = a;
+  __syncthreads();
+  for (int i = threadIdx.x; i < S; i += RB)
+    part[(size_t)i * G + blockIdx.x] = ((sh[i] + sh[S + i]) + sh[2 * S + i]) + sh[3 * S + i];
 }
 
-// out[i] = sum_blk part[i*RG+blk] (fixed order); if sq, out[i] = sqrt(out[i])
-__global__ void k_reduce_parts(int nv, const double *part, double *out, int sq) {
+// w -= sum_i h[i] V_i (i < nv), and part[blk] = <w, w> of the updated w over
+// this block's chunk
+__global__ __launch_bounds__(RB) void k_gsaxpy(int n, int nv, const double *__restrict__ V, size_t ld,
+                                               const double *__restrict__ h, double *__restrict__ w,
+                                               double *__restrict__ part) {
+  __shared__ double sh[RB / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const size_t base = (size_t)blockIdx.x * CH + threadIdx.x;
+  double we[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    const size_t k = base + (size_t)e * RB;
+    we[e] = k < (size_t)n ? w[k] : 0.0;
+  }
+  for (int i = 0; i < nv; i++) {
+    const double hi = h[i];
+    const double *v = V + (size_t)i * ld;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const size_t k = base + (size_t)e * RB;
+      if (k < (size_t)n) we[e] -= hi * v[k];
+    }
+  }
+  double a = 0;
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    const size_t k = base + (size_t)e * RB;
+    if (k < (size_t)n) {
+      w[k] = we[e];
+      a += we[e] * we[e];
+    }
+  }
+  a = wave_sum(a);
+  if (lane == 0) sh[wv] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// out[i] = sum_blk part[i*G + blk] in a fixed order, i < nv (one block per i)
+__global__ void k_reduce_parts(int nv, int G, const double *part, double *out) {
   __shared__ double sh[RB];
   const int i = blockIdx.x;
   if (i >= nv) return;
   double s = 0;
-  for (int b = threadIdx.x; b < RG; b += RB) s += part[(size_t)i * RG + b];
+  for (int b = threadIdx.x; b < G; b += RB) s += part[(size_t)i * G + b];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int o = RB / 2; o > 0; o >>= 1) {
     if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[i] = sq ? sqrt(sh[0]) : sh[0];
+  if (threadIdx.x == 0) out[i] = sh[0];
 }
 
 // w += sgn * sum_i h[i] V[i]
@@ -70,10 +120,9 @@ __global__ void k_multiaxpy(int n, int nv, const double *__restrict__ V, size_t 
   }
 }
 
-// v = w / *nrm
-__global__ void k_scale_by(int n, const double *__restrict__ w, const double *nrm, double *__restrict__ v) {
-  const double inv = 1.0 / nrm[0];
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) v[k] = w[k] * inv;
+// v = w * f
+__global__ void k_scale(int n, const double *__restrict__ w, double f, double *__restrict__ v) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) v[k] = w[k] * f;
 }
 
 // r = b - r
@@ -89,15 +138,36 @@ int Fgmres::init(int n_, int m_) {
   ST_OK(V.alloc(ld * (m + 1)));
   ST_OK(Z.alloc(ld * m));
   ST_OK(w.alloc(ld));
-  ST_OK(part.alloc((size_t)RG * (m + 2)));
+  ST_OK(part.alloc((size_t)gs_blocks(n) * (m + 2)));
   ST_OK(hd.alloc(4 * (m + 2)));
   ST_OK(hh.alloc(4 * (m + 2)));
   return I2E_OK;
 }
 
-int Fgmres::norm(const double *x, double *out_dev, hipStream_t s) {
-  hipLaunchKernelGGL(k_multidot, dim3(RG), dim3(RB), 0, s, n, 1, x, ld, x, part.p);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(RB), 0, s, 1, part.p, out_dev, 1);
+// one Gram-Schmidt pass of w against V[0..nv): h = V^T w and <w,w> into
+// hd[off .. off+nv], w -= V h, then <w,w> of the result into hd[off+nv+1]
+int Fgmres::gs_pass(int nv, double *w_, int off, hipStream_t s) {
+  const int G = gs_blocks(n);
+  const double bytes_dot = 8.0 * ((double)(nv + 1) * n + (double)(nv + 1) * G);
+  const double bytes_axpy = 8.0 * ((double)nv * n + 2.0 * n + G);
+  if (kt) kt->begin(K_GSDOT, s);
+  hipLaunchKernelGGL(k_gsdot, dim3(G), dim3(RB), sizeof(double) * 4 * (nv + 1), s, n, nv, V.p, ld, w_, part.p);
+  if (kt) kt->end(K_GSDOT, s, bytes_dot);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(nv + 1), dim3(RB), 0, s, nv + 1, G, part.p, hd.p + off);
+  if (nv > 0) {
+    if (kt) kt->begin(K_GSAXPY, s);
+    hipLaunchKernelGGL(k_gsaxpy, dim3(G), dim3(RB), 0, s, n, nv, V.p, ld, hd.p + off, w_, part.p);
+    if (kt) kt->end(K_GSAXPY, s, bytes_axpy);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(RB), 0, s, 1, G, part.p, hd.p + off + nv + 1);
+  }
+  return I2E_OK;
+}
+
+int Fgmres::norm(const double *x, double *out, hipStream_t s) {
+  ST_OK(gs_pass(0, const_cast<double *>(x), 0, s));
+  HIP_OK(hipMemcpyAsync(hh.p, hd.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  *out = std::sqrt(hh.p[0]);
   return I2E_OK;
 }
 
@@ -105,32 +175,29 @@ int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double r
                   int *iters, double *relres) {
   const unsigned gb = 1024;
   double bn;
-  ST_OK(norm(b, hd.p, s));
-  HIP_OK(hipMemcpyAsync(hh.p, hd.p, sizeof(double), hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
-  bn = hh.p[0];
+  ST_OK(norm(b, &bn, s));
   *iters = 0;
   *relres = 0;
+  reorth = 0;
   if (bn == 0) {
     HIP_OK(hipMemsetAsync(x, 0, sizeof(double) * n, s));
     return I2E_OK;
   }
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
   const bool verbose = std::getenv("I2E_VERBOSE") != nullptr;
+  const size_t P2 = (size_t)(m + 2);  // offset of the second pass's coefficients in hd
   int its = 0;
   double rr = 1;
   while (true) {
     // true residual
     ST_OK(A(x, w.p));
     hipLaunchKernelGGL(k_bminus, dim3(gb), dim3(256), 0, s, n, b, w.p);
-    ST_OK(norm(w.p, hd.p, s));
-    HIP_OK(hipMemcpyAsync(hh.p, hd.p, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    double beta = hh.p[0];
+    double beta;
+    ST_OK(norm(w.p, &beta, s));
     rr = beta / bn;
     if (verbose) fprintf(stderr, "[fgmres n=%d] its %d true relres %.3e (||b|| %.3e)\n", n, its, rr, bn);
     if (rr <= rtol || its >= maxit || !(beta > 0)) break;
-    hipLaunchKernelGGL(k_scale_by, dim3(gb), dim3(256), 0, s, n, w.p, hd.p, V.p);
+    hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, s, n, w.p, 1.0 / beta, V.p);
     std::fill(H.begin(), H.end(), 0.0);
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
@@ -139,20 +206,23 @@ int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double r
       double *vj = V.p + (size_t)j * ld, *zj = Z.p + (size_t)j * ld;
       ST_OK(M(vj, zj));
       ST_OK(A(zj, w.p));
-      // CGS2: two passes, h accumulated on the host from one packed copy
-      for (int pass = 0; pass < 2; pass++) {
-        double *hp = hd.p + (size_t)pass * (m + 2);
-        hipLaunchKernelGGL(k_multidot, dim3(RG), dim3(RB), 0, s, n, j + 1, V.p, ld, w.p, part.p);
-        hipLaunchKernelGGL(k_reduce_parts, dim3(j + 1), dim3(RB), 0, s, j + 1, part.p, hp, 0);
-        hipLaunchKernelGGL(k_multiaxpy, dim3(gb), dim3(256), 0, s, n, j + 1, V.p, ld, hp, -1.0, w.p);
-      }
-      double *nrm = hd.p + 2 * (m + 2);
-      ST_OK(norm(w.p, nrm, s));
-      hipLaunchKernelGGL(k_scale_by, dim3(gb), dim3(256), 0, s, n, w.p, nrm, V.p + (size_t)(j + 1) * ld);
-      HIP_OK(hipMemcpyAsync(hh.p, hd.p, sizeof(double) * 3 * (m + 2), hipMemcpyDeviceToHost, s));
+      // classical Gram-Schmidt, repeated only when the projection removed
+      // most of w (||w'|| < ||w|| / sqrt 2: Daniel-Gragg-Kaufman-Stewart)
+      ST_OK(gs_pass(j + 1, w.p, 0, s));
+      HIP_OK(hipMemcpyAsync(hh.p, hd.p, sizeof(double) * (j + 3), hipMemcpyDeviceToHost, s));
       HIP_OK(hipStreamSynchronize(s));
-      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hh.p[i] + hh.p[(m + 2) + i];
-      double hn = hh.p[2 * (m + 2)];
+      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hh.p[i];
+      double ww = hh.p[j + 1], nn = hh.p[j + 2];
+      if (!(nn >= 0.5 * ww)) {
+        ST_OK(gs_pass(j + 1, w.p, (int)P2, s));
+        HIP_OK(hipMemcpyAsync(hh.p + P2, hd.p + P2, sizeof(double) * (j + 3), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        for (int i = 0; i <= j; i++) H[(size_t)i * m + j] += hh.p[P2 + i];
+        nn = hh.p[P2 + j + 2];
+        reorth++;
+      }
+      const double hn = std::sqrt(nn > 0 ? nn : 0.0);
+      if (hn > 0) hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, s, n, w.p, 1.0 / hn, V.p + (size_t)(j + 1) * ld);
       H[(size_t)(j + 1) * m + j] = hn;
       for (int i = 0; i < j; i++) {
         double a = H[(size_t)i * m + j], c = H[(size_t)(i + 1) * m + j];
@@ -169,7 +239,8 @@ int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double r
       g[j] = cs[j] * g[j];
       its++;
       k = j + 1;
-      if (verbose && (j % 10 == 9)) fprintf(stderr, "[fgmres n=%d]   it %d est %.3e\n", n, its, std::fabs(g[j + 1]) / bn);
+      if (verbose && (j % 10 == 9))
+        fprintf(stderr, "[fgmres n=%d]   it %d est %.3e reorth %d\n", n, its, std::fabs(g[j + 1]) / bn, reorth);
       if (std::fabs(g[j + 1]) / bn <= rtol || !(hn > 0)) break;
     }
     // back substitution and x += Z y

@@ -29,8 +29,11 @@ struct Fgmres {
   size_t ld = 0;
   DBuf<double> V, Z, w, part, hd;
   HBuf<double> hh;
+  KTimer *kt = nullptr;  // optional per-launch timing of the Gram-Schmidt kernels
+  int reorth = 0;        // second Gram-Schmidt passes in the last solve
   int init(int n, int m);
-  int norm(const double *x, double *out_dev, hipStream_t s);
+  int gs_pass(int nv, double *w, int off, hipStream_t s);
+  int norm(const double *x, double *out, hipStream_t s);
   // solves A x = b to ||b - A x|| <= rtol ||b||; x holds the initial guess
   int solve(const Op &A, const Op &M, const double *b, double *x, double rtol, int maxit, hipStream_t s, int *iters,
             double *relres);

@@ -372,14 +372,18 @@ __global__ void k_gauss_jordan(int n, double *a, T *inv) {
 // u = Ainv f with f = [f0 (n0 entries), f1]
 template <class T>
 __global__ void k_dense_solve(int n, int n0, const T *ainv, const T *f0, const T *f1, T *u0, T *u1) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  // one wavefront per row of the inverse: coalesced row reads, lane butterfly sum
+  const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= n) return;
   double s = 0;
-  for (int c = 0; c < n; c++) s += (double)ainv[(size_t)r * n + c] * (double)(c < n0 ? f0[c] : f1[c - n0]);
-  if (r < n0)
-    u0[r] = T(s);
-  else
-    u1[r - n0] = T(s);
+  for (int c = lane; c < n; c += 64) s += (double)ainv[(size_t)r * n + c] * (double)(c < n0 ? f0[c] : f1[c - n0]);
+  s = wave_sum(s);
+  if (lane == 0) {
+    if (r < n0)
+      u0[r] = T(s);
+    else
+      u1[r - n0] = T(s);
+  }
 }
 
 
@@ -750,7 +754,7 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   const int nl = (int)L.size();
   VelLevel<T> &v = L[l];
   if (l == nl - 1) {
-    hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(nc, 64)), dim3(64), 0, s, nc, v.N, ainv.p, v.fx.p, v.fy.p, v.ux.p,
+    hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(nc, 4)), dim3(256), 0, s, nc, v.N, ainv.p, v.fx.p, v.fy.p, v.ux.p,
                        v.uy.p);
     return I2E_OK;
   }
@@ -758,14 +762,26 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   const VLev<T> o = lev(l);
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.ux.p);
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.uy.p);
+  // algorithmic bytes of one colour step: the colour's stencil rows, both
+  // components of u, its rhs, the full output component; level 0 reads the
+  // four coefficient fields instead of a stencil (plus kinds and diagonal)
+  const double bytes_c = sizeof(T) * (25.0 + 2.0 + 0.5 + 1.0) * v.N;
+  const double bytes_0 = sizeof(T) * (4.0 + 2.0 + 0.5 + 0.5 + 0.5) * v.N + 2.0 * v.N;
+  auto gs0 = [&](int comp, int col) {
+    if (kt) kt->begin(K_VSMOOTH0, s);
+    hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
+    if (kt) kt->end(K_VSMOOTH0, s, bytes_0);
+  };
   auto csmooth = [&](bool rev) {
     // per-colour Jacobi on the Galerkin stencil, double-buffered per component
     for (int it = 0; it < nsmooth; it++)
       for (int q = 0; q < 4; q++) {
         const int k = rev ? 3 - q : q, comp = 1 + k / 2, col = k % 2;
         DBuf<T> &cur = comp == 1 ? v.ux : v.uy, &nxt = comp == 1 ? v.ux2 : v.uy2;
+        if (kt) kt->begin(K_VSMOOTHC, s);
         hipLaunchKernelGGL(k_sjac<T>, G, B, 0, s, (const T *)v.S.p, v.X, v.Y, comp, col, (const T *)v.ux.p,
                            (const T *)v.uy.p, comp == 1 ? (const T *)v.fx.p : (const T *)v.fy.p, nxt.p);
+        if (kt) kt->end(K_VSMOOTHC, s, bytes_c);
         std::swap(cur.p, nxt.p);
       }
     hipLaunchKernelGGL(k_sbcfix<T>, G, B, 0, s, (const T *)v.S.p, v.X, v.Y, (const int8_t *)v.kx.p,
@@ -774,8 +790,7 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   if (l == 0) {
     for (int it = 0; it < nsmooth; it++)
       for (int comp = 1; comp <= 2; comp++)
-        for (int col = 0; col < 2; col++)
-          hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
+        for (int col = 0; col < 2; col++) gs0(comp, col);
     hipLaunchKernelGGL(k_vbcfix<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p);
     hipLaunchKernelGGL(k_vresid<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
   } else {
@@ -793,8 +808,7 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   if (l == 0) {
     for (int it = 0; it < nsmooth; it++)
       for (int comp = 2; comp >= 1; comp--)
-        for (int col = 1; col >= 0; col--)
-          hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
+        for (int col = 1; col >= 0; col--) gs0(comp, col);
     hipLaunchKernelGGL(k_vbcfix<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p);
   } else
     csmooth(true);
@@ -990,7 +1004,7 @@ int ScalMG<T>::vcycle(int l, hipStream_t s) {
   const int nl = (int)L.size();
   ScalLevel<T> &v = L[l];
   if (l == nl - 1) {
-    hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(nc, 64)), dim3(64), 0, s, nc, nc, ainv.p, v.f.p, v.f.p, v.u.p,
+    hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(nc, 4)), dim3(256), 0, s, nc, nc, ainv.p, v.f.p, v.f.p, v.u.p,
                        v.u.p);
     return I2E_OK;
   }

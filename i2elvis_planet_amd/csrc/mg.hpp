@@ -109,6 +109,7 @@ struct VelMG {
   DBuf<int> overflow;       // Galerkin entries that fell outside the 5x5 window (must stay 0)
   double mu = 0, dt = 0;
   int nsmooth = 2;
+  KTimer *kt = nullptr;     // optional per-launch timing of the smoothers
   int setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,
             const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,
             const std::vector<double> &kky0);

@@ -208,6 +208,20 @@ void *i2e_device_ptr(i2e_ctx *c, const char *name);
 int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g,
                      double *ms_stokes_apply);
 
+/* Per-launch HIP-event timing of one kernel on the context's stream (no
+ * counterpart in the reference), for roofline figures taken inside a normal
+ * run: i2e_kernel_timer selects the kernel ("advect" = movemark's marker
+ * pass, "p2g" = ronurecalc's marker scatter, "gsdot" / "gsaxpy" = the
+ * Gram-Schmidt projection / update passes of the Krylov solvers,
+ * "vsmooth0" / "vsmoothc" = one colour step of the velocity smoother on the
+ * finest / a coarse multigrid level, "stokes_apply"; NULL or ""
+ * stops timing), reserves events for up to max_launches launches and clears
+ * the counters; i2e_kernel_timer_read returns the launches timed, their
+ * summed milliseconds and summed algorithmic HBM bytes.                     */
+int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches);
+int i2e_kernel_timer_read(i2e_ctx *c, int64_t *launches, double *ms_total,
+                          double *bytes_total);
+
 /* Device buffers for callers without a device allocator of their own (no
  * counterpart in the reference): allocation, release and blocking copies,
  * all through the same HIP runtime as the library itself.                   */
