@@ -419,6 +419,10 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
           }
           nb = (int)((u - 1) / 3);
           kk = s->bondv[4 * b + q + 1];
+          if (std::abs(nb / Y - i) > 1 || std::abs(nb % Y - j) > 1) {
+            set_error("velocity BC of unknown %d references a non-adjacent node", 3 * m + comp);
+            return I2E_ERR_UNSUPPORTED;
+          }
         }
         if (cnt > 1) {
           set_error("velocity BC with more than one neighbour not supported by the multigrid");

@@ -35,7 +35,25 @@ __global__ __launch_bounds__(RB) void k_gsdot(int n, int nv, const double *__res
     const size_t k = base + (size_t)e * RB;
     we[e] = k < (size_t)n ? w[k] : 0.0;
   }
-  for (int i = 0; i < nv; i++) {
+  // four basis vectors per round keep 16 loads in flight per lane
+  int i = 0;
+  for (; i + 4 <= nv; i += 4) {
+    double a[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const size_t k = base + (size_t)e * RB;
+      if (k < (size_t)n) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) a[q] += V[(size_t)(i + q) * ld + k] * we[e];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double t = wave_sum(a[q]);
+      if (lane == 0) sh[wv * S + i + q] = t;
+    }
+  }
+  for (; i < nv; i++) {
     const double *v = V + (size_t)i * ld;
     double a = 0;
 #pragma unroll
@@ -70,7 +88,20 @@ __global__ __launch_bounds__(RB) void k_gsaxpy(int n, int nv, const double *__re
     const size_t k = base + (size_t)e * RB;
     we[e] = k < (size_t)n ? w[k] : 0.0;
   }
-  for (int i = 0; i < nv; i++) {
+  int i = 0;
+  for (; i + 4 <= nv; i += 4) {
+    const double h0 = h[i], h1 = h[i + 1], h2 = h[i + 2], h3 = h[i + 3];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const size_t k = base + (size_t)e * RB;
+      if (k < (size_t)n) {
+        const double v0 = V[(size_t)i * ld + k], v1 = V[(size_t)(i + 1) * ld + k], v2 = V[(size_t)(i + 2) * ld + k],
+                     v3 = V[(size_t)(i + 3) * ld + k];
+        we[e] = (((we[e] - h0 * v0) - h1 * v1) - h2 * v2) - h3 * v3;
+      }
+    }
+  }
+  for (; i < nv; i++) {
     const double hi = h[i];
     const double *v = V + (size_t)i * ld;
 #pragma unroll

@@ -198,36 +198,6 @@ __global__ void k_vdiag(VLev<T> L, T *dx, T *dy) {
   dy[m] = vrow(L, VUnit<T>{2, m, Y}, 2, i, j);
 }
 
-// red-black Gauss-Seidel half sweep on one component
-template <class T>
-__global__ void k_vgs(VLev<T> L, int comp, int color, T *ux, T *uy, const T *fx, const T *fy, const T *dx,
-                      const T *dy) {
-  IDX2;
-  const int X = L.o.X, Y = L.o.Y;
-  if (i >= X || j >= Y || ((i + j) & 1) != color) return;
-  const int m = i * Y + j;
-  VSplit<T> v{ux, uy, nullptr, Y};
-  T r = vrow(L, v, comp, i, j);
-  if (comp == 1)
-    ux[m] += (fx[m] - r) / dx[m];
-  else
-    uy[m] += (fy[m] - r) / dy[m];
-}
-
-// re-solve the pointwise rows (boundary and ghost) exactly after smoothing:
-// their scale (mukoef) dwarfs the momentum rows, and the last half sweeps
-// leave them inconsistent with their freshly updated neighbours
-template <class T>
-__global__ void k_vbcfix(VLev<T> L, T *ux, T *uy, const T *fx, const T *fy) {
-  IDX2;
-  const int X = L.o.X, Y = L.o.Y;
-  if (i >= X || j >= Y) return;
-  const int m = i * Y + j;
-  const T imu = T(1.0 / L.o.mu);
-  if (L.kx[m]) ux[m] = fx[m] * imu + (L.kx[m] == 1 && L.nbx[m] >= 0 ? L.kkx[m] * ux[L.nbx[m]] : T(0));
-  if (L.ky[m]) uy[m] = fy[m] * imu + (L.ky[m] == 1 && L.nby[m] >= 0 ? L.kky[m] * uy[L.nby[m]] : T(0));
-}
-
 template <class T>
 __global__ void k_vresid(VLev<T> L, const T *ux, const T *uy, const T *fx, const T *fy, T *rx, T *ry) {
   IDX2;
@@ -277,28 +247,6 @@ __global__ void k_vrestrict(int Xc, int Yc, int Yf, MapView xn, MapView xm, MapV
     }
   }
   fy[m] = T(ws > 0 ? s / ws : 0.0);
-}
-
-template <class T>
-__global__ void k_vprolong(int Xf, int Yf, int Yc, MapView xn, MapView xm, MapView yn, MapView ym, const int8_t *fkx,
-                           const int8_t *fky, const T *cx, const T *cy, T *ux, T *uy) {
-  IDX2;
-  if (i >= Xf || j >= Yf) return;
-  const int m = i * Yf + j;
-  if (fkx[m] != 2 && j <= Yf - 2) {
-    int a = xn.i0[i], b = ym.i0[j];
-    double wa = xn.w[i], wb = ym.w[j];
-    double v = (1 - wa) * ((1 - wb) * (double)cx[a * Yc + b] + wb * (double)cx[a * Yc + b + 1]) +
-               wa * ((1 - wb) * (double)cx[(a + 1) * Yc + b] + wb * (double)cx[(a + 1) * Yc + b + 1]);
-    ux[m] += T(v);
-  }
-  if (fky[m] != 2 && i <= Xf - 2) {
-    int a = xm.i0[i], b = yn.i0[j];
-    double wa = xm.w[i], wb = yn.w[j];
-    double v = (1 - wa) * ((1 - wb) * (double)cy[a * Yc + b] + wb * (double)cy[a * Yc + b + 1]) +
-               wa * ((1 - wb) * (double)cy[(a + 1) * Yc + b] + wb * (double)cy[(a + 1) * Yc + b + 1]);
-    uy[m] += T(v);
-  }
 }
 
 // dense matrix of the coarsest level, column c = A e_c, rows/cols ordered
@@ -371,12 +319,12 @@ __global__ void k_gauss_jordan(int n, double *a, T *inv) {
 
 // u = Ainv f with f = [f0 (n0 entries), f1]
 template <class T>
-__global__ void k_dense_solve(int n, int n0, const T *ainv, const T *f0, const T *f1, T *u0, T *u1) {
+__global__ void k_dense_solve(int n, int n0, const double *ainv, const T *f0, const T *f1, T *u0, T *u1) {
   // one wavefront per row of the inverse: coalesced row reads, lane butterfly sum
   const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= n) return;
   double s = 0;
-  for (int c = lane; c < n; c += 64) s += (double)ainv[(size_t)r * n + c] * (double)(c < n0 ? f0[c] : f1[c - n0]);
+  for (int c = lane; c < n; c += 64) s += ainv[(size_t)r * n + c] * (double)(c < n0 ? f0[c] : f1[c - n0]);
   s = wave_sum(s);
   if (lane == 0) {
     if (r < n0)
@@ -388,8 +336,10 @@ __global__ void k_dense_solve(int n, int n0, const T *ainv, const T *f0, const T
 
 
 // ---------------------------------------------------------------- stencil levels
-// slot of column component cc (0 Vx, 1 Vy) at offset (di,dj) in a window of
-// half-width W: rows store 2*(2W+1)^2 coefficients
+// Explicit stencils are stored slot-major: coefficient `slot` of row r
+// (r = (comp-1)*N + m) at S[slot * 2N + r], so that the lanes of a wavefront
+// (consecutive m) read consecutive addresses.  A window of half-width W
+// holds 2*(2W+1)^2 slots: column component cc (0 Vx, 1 Vy) at offset (di,dj).
 __device__ __forceinline__ int sslot(int W, int cc, int di, int dj) {
   const int w = 2 * W + 1;
   return cc * w * w + (di + W) * w + (dj + W);
@@ -398,26 +348,27 @@ __device__ __forceinline__ int sslot(int W, int cc, int di, int dj) {
 template <class T>
 struct StView {
   const T *S;
-  int X, Y, W, NS;
+  int X, Y, W;
   const int8_t *kx, *ky;
 };
 
-// level-0 coefficients by probing the matrix-free rows with unit vectors
+// finest-level coefficients by probing the matrix-free rows with unit vectors
 template <class T>
 __global__ void k_probe0(VLev<T> L, T *S) {
   IDX2;
   const int X = L.o.X, Y = L.o.Y, N = X * Y;
   if (i >= X || j >= Y) return;
   const int m = i * Y + j;
+  const size_t R = 2 * (size_t)N;
   for (int comp = 1; comp <= 2; comp++) {
-    T *out = S + ((size_t)(comp - 1) * N + m) * 18;
+    const size_t r = (size_t)(comp - 1) * N + m;
     for (int cc = 0; cc < 2; cc++)
       for (int di = -1; di <= 1; di++)
         for (int dj = -1; dj <= 1; dj++) {
           const int a = i + di, b = j + dj;
           T v = 0;
           if (a >= 0 && a < X && b >= 0 && b < Y) v = vrow(L, VUnit<T>{cc + 1, a * Y + b, Y}, comp, i, j);
-          out[sslot(1, cc, di, dj)] = v;
+          S[(size_t)sslot(1, cc, di, dj) * R + r] = v;
         }
   }
 }
@@ -432,8 +383,9 @@ __global__ void k_galerkin(StView<T> F, int Xc, int Yc, MapView xn, MapView xm, 
   IDX2;
   if (i >= Xc || j >= Yc) return;
   const int Nc = Xc * Yc, Nf = F.X * F.Y, m = i * Yc + j;
+  const size_t Rc = 2 * (size_t)Nc, Rf = 2 * (size_t)Nf;
   for (int comp = 1; comp <= 2; comp++) {
-    T *out = Sc + ((size_t)(comp - 1) * Nc + m) * 50;
+    const size_t rc = (size_t)(comp - 1) * Nc + m;
     double acc[50];
     for (int q = 0; q < 50; q++) acc[q] = 0.0;
     const int kind = comp == 1 ? ckx[m] : cky[m];
@@ -448,7 +400,7 @@ __global__ void k_galerkin(StView<T> F, int Xc, int Yc, MapView xn, MapView xm, 
         else
           atomicAdd(overflow, 1);
       }
-      for (int q = 0; q < 50; q++) out[q] = T(acc[q]);
+      for (int q = 0; q < 50; q++) Sc[(size_t)q * Rc + rc] = T(acc[q]);
       continue;
     }
     const MapView mxr = comp == 1 ? xn : xm, myr = comp == 1 ? ym : yn;
@@ -462,12 +414,12 @@ __global__ void k_galerkin(StView<T> F, int Xc, int Yc, MapView xn, MapView xm, 
         wsum += w;
         const int a = k * F.Y + l;
         if ((comp == 1 ? F.kx[a] : F.ky[a]) != 0) continue;
-        const T *srow = F.S + ((size_t)(comp - 1) * Nf + a) * F.NS;
+        const size_t rf = (size_t)(comp - 1) * Nf + a;
         for (int cc = 0; cc < 2; cc++) {
           const MapView px = cc == 0 ? xn : xm, py = cc == 0 ? ym : yn;
           for (int dk = -F.W; dk <= F.W; dk++)
             for (int dl = -F.W; dl <= F.W; dl++) {
-              const double sv = (double)srow[sslot(F.W, cc, dk, dl)];
+              const double sv = (double)F.S[(size_t)sslot(F.W, cc, dk, dl) * Rf + rf];
               if (sv == 0) continue;
               const int kb = k + dk, lb = l + dl;
               if (kb < 0 || kb >= F.X || lb < 0 || lb >= F.Y) continue;
@@ -490,69 +442,8 @@ __global__ void k_galerkin(StView<T> F, int Xc, int Yc, MapView xn, MapView xm, 
       }
     }
     const double inv = wsum > 0 ? 1.0 / wsum : 0.0;
-    for (int q = 0; q < 50; q++) out[q] = T(acc[q] * inv);
+    for (int q = 0; q < 50; q++) Sc[(size_t)q * Rc + rc] = T(acc[q] * inv);
   }
-}
-
-template <class T>
-__device__ __forceinline__ T s_apply(const T *s, int X, int Y, const T *ux, const T *uy, int i, int j) {
-  T acc = 0;
-  for (int cc = 0; cc < 2; cc++) {
-    const T *u = cc ? uy : ux;
-    for (int di = -2; di <= 2; di++) {
-      const int a = i + di;
-      if (a < 0 || a >= X) continue;
-      for (int dj = -2; dj <= 2; dj++) {
-        const int b = j + dj;
-        if (b < 0 || b >= Y) continue;
-        acc += s[sslot(2, cc, di, dj)] * u[a * Y + b];
-      }
-    }
-  }
-  return acc;
-}
-
-// one colour of the coarse smoother: out = u + (f - S u)/diag on the colour,
-// out = u elsewhere (component comp only)
-template <class T>
-__global__ void k_sjac(const T *S, int X, int Y, int comp, int color, const T *ux, const T *uy, const T *f, T *out) {
-  IDX2;
-  if (i >= X || j >= Y) return;
-  const int N = X * Y, m = i * Y + j;
-  const T *u = comp == 1 ? ux : uy;
-  if (((i + j) & 1) != color) {
-    out[m] = u[m];
-    return;
-  }
-  const T *s = S + ((size_t)(comp - 1) * N + m) * 50;
-  const T d = s[sslot(2, comp - 1, 0, 0)];
-  out[m] = u[m] + (f[m] - s_apply(s, X, Y, ux, uy, i, j)) / d;
-}
-
-template <class T>
-__global__ void k_sbcfix(const T *S, int X, int Y, const int8_t *kx, const int8_t *ky, T *ux, T *uy, const T *fx,
-                         const T *fy) {
-  IDX2;
-  if (i >= X || j >= Y) return;
-  const int N = X * Y, m = i * Y + j;
-  for (int comp = 1; comp <= 2; comp++) {
-    if ((comp == 1 ? kx[m] : ky[m]) == 0) continue;
-    const T *s = S + ((size_t)(comp - 1) * N + m) * 50;
-    const T d = s[sslot(2, comp - 1, 0, 0)];
-    T *u = comp == 1 ? ux : uy;
-    const T f = comp == 1 ? fx[m] : fy[m];
-    u[m] = u[m] + (f - s_apply(s, X, Y, ux, uy, i, j)) / d;
-  }
-}
-
-template <class T>
-__global__ void k_sresid(const T *S, int X, int Y, const int8_t *kx, const int8_t *ky, const T *ux, const T *uy,
-                         const T *fx, const T *fy, T *rx, T *ry) {
-  IDX2;
-  if (i >= X || j >= Y) return;
-  const int N = X * Y, m = i * Y + j;
-  rx[m] = kx[m] == 0 ? fx[m] - s_apply(S + (size_t)m * 50, X, Y, ux, uy, i, j) : T(0);
-  ry[m] = ky[m] == 0 ? fy[m] - s_apply(S + ((size_t)N + m) * 50, X, Y, ux, uy, i, j) : T(0);
 }
 
 // dense [Vx | Vy] matrix of the coarsest level from its stencil
@@ -565,14 +456,222 @@ __global__ void k_sdense(const T *S, int X, int Y, double *aug) {
   for (int c = 0; c < 2 * n; c++) row[c] = 0.0;
   row[n + r] = 1.0;
   const int m = r % N, i = m / Y, j = m % Y;
-  const T *s = S + (size_t)r * 50;
   for (int cc = 0; cc < 2; cc++)
     for (int di = -2; di <= 2; di++)
       for (int dj = -2; dj <= 2; dj++) {
         const int a = i + di, b = j + dj;
         if (a < 0 || a >= X || b < 0 || b >= Y) continue;
-        row[cc * N + a * Y + b] += (double)s[sslot(2, cc, di, dj)];
+        row[cc * N + a * Y + b] += (double)S[(size_t)sslot(2, cc, di, dj) * n + r];
       }
+}
+
+// coarse-level residual r = f - S u on equation rows (0 on pointwise rows).
+// Columns outside the grid carry zero coefficients, so clamped reads keep the
+// 50-term sum free of branches.
+template <class T>
+__global__ void k_sresid(const T *S, int X, int Y, const int8_t *kx, const int8_t *ky, const T *ux, const T *uy,
+                         const T *fx, const T *fy, T *rx, T *ry) {
+  IDX2;
+  if (i >= X || j >= Y) return;
+  const int N = X * Y, m = i * Y + j;
+  const size_t R = 2 * (size_t)N;
+  int ia[5], jb[5];
+#pragma unroll
+  for (int d = 0; d < 5; d++) {
+    const int a = i + d - 2, b = j + d - 2;
+    ia[d] = (a < 0 ? 0 : (a >= X ? X - 1 : a)) * Y;
+    jb[d] = b < 0 ? 0 : (b >= Y ? Y - 1 : b);
+  }
+  for (int comp = 0; comp < 2; comp++) {
+    const T *Sr = S + (size_t)comp * N + m;
+    T acc = 0;
+    for (int cc = 0; cc < 2; cc++) {
+      const T *u = cc ? uy : ux;
+#pragma unroll
+      for (int di = 0; di < 5; di++)
+#pragma unroll
+        for (int dj = 0; dj < 5; dj++) acc += Sr[(size_t)sslot(2, cc, di - 2, dj - 2) * R] * u[ia[di] + jb[dj]];
+    }
+    if (comp == 0)
+      rx[m] = kx[m] == 0 ? fx[m] - acc : T(0);
+    else
+      ry[m] = ky[m] == 0 ? fy[m] - acc : T(0);
+  }
+}
+
+// ---------------------------------------------------------------- tiled smoothers
+// One workgroup smooths a TS x TS tile of a level in LDS: the tile and a halo
+// of TH points are loaded once (with the coarse-grid correction added on the
+// way up), nsmooth sweeps of the four colour steps (Vx red, Vx black, Vy red,
+// Vy black; reversed after the coarse-grid correction) run between workgroup
+// barriers, the pointwise boundary rows are re-solved, and the tile interior
+// goes to the output buffers.  Halo values keep their kernel-start state
+// (block Gauss-Seidel across tiles): no workgroup reads what another writes,
+// so the result is deterministic.  Within a colour step every point of the
+// colour is updated from the values before the step (on the Galerkin levels
+// same-colour points couple; on the finest level they do not, and this is
+// red-black Gauss-Seidel).
+constexpr int TS = 16, TH = 2, TW = TS + 2 * TH, TB = 256, TQ = TS * TS / TB;
+
+template <class T>
+struct TileArgs {
+  int X, Y;
+  const int8_t *kx, *ky;
+  const T *fx, *fy;  // right-hand side
+  const T *ux, *uy;  // input iterate (not read when zero != 0)
+  T *ox, *oy;        // output iterate
+  const T *ex, *ey;  // correction on the next coarser level, added on load (nullptr: none)
+  int Yc;
+  MapView xn, xm, yn, ym;
+  int zero, nsmooth, rev;
+};
+
+template <class T>
+__device__ __forceinline__ double prolong_at(const T *e, int Yc, const MapView &mx, const MapView &my, int i, int j) {
+  const int a = mx.i0[i], b = my.i0[j];
+  const double wa = mx.w[i], wb = my.w[j];
+  return (1 - wa) * ((1 - wb) * (double)e[a * Yc + b] + wb * (double)e[a * Yc + b + 1]) +
+         wa * ((1 - wb) * (double)e[(a + 1) * Yc + b] + wb * (double)e[(a + 1) * Yc + b + 1]);
+}
+
+template <class T>
+__device__ __forceinline__ void tile_load(const TileArgs<T> &A, T *lx, T *ly, int i0, int j0) {
+  for (int q = threadIdx.x; q < TW * TW; q += TB) {
+    const int i = i0 + q / TW, j = j0 + q % TW;
+    T vx = 0, vy = 0;
+    if (i >= 0 && i < A.X && j >= 0 && j < A.Y) {
+      const int m = i * A.Y + j;
+      if (!A.zero) {
+        vx = A.ux[m];
+        vy = A.uy[m];
+      }
+      if (A.ex) {
+        if (A.kx[m] != 2 && j <= A.Y - 2) vx += T(prolong_at(A.ex, A.Yc, A.xn, A.ym, i, j));
+        if (A.ky[m] != 2 && i <= A.X - 2) vy += T(prolong_at(A.ey, A.Yc, A.xm, A.yn, i, j));
+      }
+    }
+    lx[q] = vx;
+    ly[q] = vy;
+  }
+}
+
+// unknowns of the finest level read from the LDS tile (global indices)
+template <class T>
+struct LdsV {
+  const T *lx, *ly;
+  int i0, j0;
+  __device__ __forceinline__ T vx(int i, int j) const { return lx[(i - i0) * TW + (j - j0)]; }
+  __device__ __forceinline__ T vy(int i, int j) const { return ly[(i - i0) * TW + (j - j0)]; }
+  __device__ __forceinline__ T pp(int, int) const { return T(0); }
+};
+
+// S u at a tile point from LDS (Galerkin levels; out-of-grid halo entries are 0)
+template <class T>
+__device__ __forceinline__ T s_apply_lds(const T *S, size_t R, size_t r, const T *lx, const T *ly, int la, int lb) {
+  T acc = 0;
+#pragma unroll
+  for (int cc = 0; cc < 2; cc++) {
+    const T *u = cc ? ly : lx;
+#pragma unroll
+    for (int di = -2; di <= 2; di++)
+#pragma unroll
+      for (int dj = -2; dj <= 2; dj++) acc += S[(size_t)sslot(2, cc, di, dj) * R + r] * u[(la + di) * TW + lb + dj];
+  }
+  return acc;
+}
+
+// fine = true: finest level (matrix-free rows, diagonals dx/dy); false:
+// Galerkin level with stencil S
+template <class T, bool FINE>
+__global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, VLev<T> L, const T *dx, const T *dy, const T *S) {
+  __shared__ T lx[TW * TW], ly[TW * TW];
+  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
+  const int X = A.X, Y = A.Y, N = X * Y;
+  const size_t R = 2 * (size_t)N;
+  tile_load(A, lx, ly, i0, j0);
+  int gi[TQ], gj[TQ];
+  T fxr[TQ], fyr[TQ];
+  int8_t kxr[TQ], kyr[TQ];
+#pragma unroll
+  for (int q = 0; q < TQ; q++) {
+    const int p = threadIdx.x + q * TB;
+    gi[q] = i0 + TH + p / TS;
+    gj[q] = j0 + TH + p % TS;
+    const bool ok = gi[q] < X && gj[q] < Y;
+    const int m = ok ? gi[q] * Y + gj[q] : 0;
+    fxr[q] = ok ? A.fx[m] : T(0);
+    fyr[q] = ok ? A.fy[m] : T(0);
+    kxr[q] = ok ? A.kx[m] : int8_t(-1);
+    kyr[q] = ok ? A.ky[m] : int8_t(-1);
+  }
+  __syncthreads();
+  const LdsV<T> lv{lx, ly, i0, j0};
+  for (int it = 0; it < A.nsmooth; it++)
+    for (int st = 0; st < 4; st++) {
+      const int k = A.rev ? 3 - st : st, comp = 1 + k / 2, col = k & 1;
+      T nv[TQ];
+#pragma unroll
+      for (int q = 0; q < TQ; q++) {
+        const int i = gi[q], j = gj[q];
+        if ((comp == 1 ? kxr[q] : kyr[q]) < 0 || ((i + j) & 1) != col) continue;
+        const int m = i * Y + j, li = (i - i0) * TW + (j - j0);
+        const T u = comp == 1 ? lx[li] : ly[li];
+        const T f = comp == 1 ? fxr[q] : fyr[q];
+        if (FINE) {
+          nv[q] = u + (f - vrow(L, lv, comp, i, j)) / (comp == 1 ? dx[m] : dy[m]);
+        } else {
+          const size_t r = (size_t)(comp - 1) * N + m;
+          const T d = S[(size_t)sslot(2, comp - 1, 0, 0) * R + r];
+          nv[q] = u + (f - s_apply_lds(S, R, r, lx, ly, i - i0, j - j0)) / d;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < TQ; q++) {
+        const int i = gi[q], j = gj[q];
+        if ((comp == 1 ? kxr[q] : kyr[q]) < 0 || ((i + j) & 1) != col) continue;
+        (comp == 1 ? lx : ly)[(i - i0) * TW + (j - j0)] = nv[q];
+      }
+      __syncthreads();
+    }
+  // re-solve the pointwise rows exactly: their scale (mukoef) dwarfs the
+  // momentum rows, and the last colour steps left them inconsistent
+  T bx[TQ], by[TQ];
+#pragma unroll
+  for (int q = 0; q < TQ; q++) {
+    const int i = gi[q], j = gj[q], li = (i - i0) * TW + (j - j0);
+    bx[q] = kxr[q] >= 0 ? lx[li] : T(0);
+    by[q] = kyr[q] >= 0 ? ly[li] : T(0);
+    if (kxr[q] <= 0 && kyr[q] <= 0) continue;
+    const int m = i * Y + j;
+    if (FINE) {
+      const T imu = T(1.0 / L.o.mu);
+      if (kxr[q] > 0) {
+        const int nb = L.nbx[m];
+        bx[q] = fxr[q] * imu + (kxr[q] == 1 && nb >= 0 ? L.kkx[m] * lv.vx(nb / Y, nb % Y) : T(0));
+      }
+      if (kyr[q] > 0) {
+        const int nb = L.nby[m];
+        by[q] = fyr[q] * imu + (kyr[q] == 1 && nb >= 0 ? L.kky[m] * lv.vy(nb / Y, nb % Y) : T(0));
+      }
+    } else {
+      if (kxr[q] > 0) {
+        const size_t r = m;
+        bx[q] = bx[q] + (fxr[q] - s_apply_lds(S, R, r, lx, ly, i - i0, j - j0)) / S[(size_t)sslot(2, 0, 0, 0) * R + r];
+      }
+      if (kyr[q] > 0) {
+        const size_t r = (size_t)N + m;
+        by[q] = by[q] + (fyr[q] - s_apply_lds(S, R, r, lx, ly, i - i0, j - j0)) / S[(size_t)sslot(2, 1, 0, 0) * R + r];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < TQ; q++) {
+    if (kxr[q] < 0) continue;
+    const int m = gi[q] * Y + gj[q];
+    A.ox[m] = bx[q];
+    A.oy[m] = by[q];
+  }
 }
 
 // ---------------------------------------------------------------- VelMG host
@@ -729,15 +828,18 @@ int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, cons
   hipLaunchKernelGGL(k_probe0<T>, grd2(L[0].X, L[0].Y), blk2(), 0, s, lev(0), L[0].S.p);
   for (int l = 0; l + 1 < nl; l++) {
     VelLevel<T> &f = L[l], &c = L[l + 1];
-    StView<T> fv{f.S.p, f.X, f.Y, f.W, f.W == 1 ? 18 : 50, f.kx.p, f.ky.p};
+    StView<T> fv{f.S.p, f.X, f.Y, f.W, f.kx.p, f.ky.p};
     hipLaunchKernelGGL(k_galerkin<T>, grd2(c.X, c.Y), blk2(), 0, s, fv, c.X, c.Y, view(f.mxn), view(f.mxm),
                        view(f.myn), view(f.mym), c.kx.p, c.ky.p, c.nbx.p, c.nby.p, c.kkx.p, c.kky.p, mu_, c.S.p,
                        overflow.p);
   }
   const int n = nc;
-  hipLaunchKernelGGL(k_sdense<T>, dim3(nblk(n, 64)), dim3(64), 0, s, L[nl - 1].S.p, L[nl - 1].X, L[nl - 1].Y,
-                     dense.p);
-  hipLaunchKernelGGL(k_gauss_jordan<T>, dim3(1), dim3(1024), 0, s, n, dense.p, ainv.p);
+  if (nl > 1)
+    hipLaunchKernelGGL(k_sdense<T>, dim3(nblk(n, 64)), dim3(64), 0, s, L[nl - 1].S.p, L[nl - 1].X, L[nl - 1].Y,
+                       dense.p);
+  else
+    hipLaunchKernelGGL(k_vdense<T>, dim3(nblk(n, 64), n), dim3(64), 0, s, lev(0), n, dense.p);
+  hipLaunchKernelGGL(k_gauss_jordan<double>, dim3(1), dim3(1024), 0, s, n, dense.p, ainv.p);
   HIP_OK(hipGetLastError());
   int nover = 0;
   HIP_OK(hipMemcpyAsync(&nover, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -758,60 +860,43 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
                        v.uy.p);
     return I2E_OK;
   }
-  const dim3 G = grd2(v.X, v.Y), B = blk2();
+  const dim3 TG((v.Y + TS - 1) / TS, (v.X + TS - 1) / TS), TBK(TB);
   const VLev<T> o = lev(l);
-  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.ux.p);
-  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.uy.p);
-  // algorithmic bytes of one colour step: the colour's stencil rows, both
-  // components of u, its rhs, the full output component; level 0 reads the
-  // four coefficient fields instead of a stencil (plus kinds and diagonal)
-  const double bytes_c = sizeof(T) * (25.0 + 2.0 + 0.5 + 1.0) * v.N;
-  const double bytes_0 = sizeof(T) * (4.0 + 2.0 + 0.5 + 0.5 + 0.5) * v.N + 2.0 * v.N;
-  auto gs0 = [&](int comp, int col) {
-    if (kt) kt->begin(K_VSMOOTH0, s);
-    hipLaunchKernelGGL(k_vgs<T>, G, B, 0, s, o, comp, col, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.dx.p, v.dy.p);
-    if (kt) kt->end(K_VSMOOTH0, s, bytes_0);
+  const int kid = l == 0 ? K_VSMOOTH0 : K_VSMOOTHC;
+  // algorithmic bytes of one smoothing pass: rhs, input and output iterate
+  // (two components each), the row kinds, and the coefficients once (finest:
+  // nu, nd, ro, gp and the diagonals; Galerkin levels: 50 stencil slots per row)
+  const double coef = l == 0 ? 6.0 : 100.0;
+  const double bytes = sizeof(T) * (6.0 + coef) * v.N + 2.0 * v.N;
+  auto smooth = [&](const T *ex, const T *ey, int Yc, MapView xn, MapView xm, MapView yn, MapView ym, int zero,
+                    int rev) {
+    TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p,
+                  ex, ey, Yc, xn, xm, yn, ym, zero, nsmooth, rev};
+    if (kt) kt->begin(kid, s);
+    if (l == 0)
+      hipLaunchKernelGGL((k_tsmooth<T, true>), TG, TBK, 0, s, A, o, (const T *)v.dx.p, (const T *)v.dy.p,
+                         (const T *)nullptr);
+    else
+      hipLaunchKernelGGL((k_tsmooth<T, false>), TG, TBK, 0, s, A, o, (const T *)nullptr, (const T *)nullptr,
+                         (const T *)v.S.p);
+    if (kt) kt->end(kid, s, bytes);
+    std::swap(v.ux.p, v.ux2.p);
+    std::swap(v.uy.p, v.uy2.p);
   };
-  auto csmooth = [&](bool rev) {
-    // per-colour Jacobi on the Galerkin stencil, double-buffered per component
-    for (int it = 0; it < nsmooth; it++)
-      for (int q = 0; q < 4; q++) {
-        const int k = rev ? 3 - q : q, comp = 1 + k / 2, col = k % 2;
-        DBuf<T> &cur = comp == 1 ? v.ux : v.uy, &nxt = comp == 1 ? v.ux2 : v.uy2;
-        if (kt) kt->begin(K_VSMOOTHC, s);
-        hipLaunchKernelGGL(k_sjac<T>, G, B, 0, s, (const T *)v.S.p, v.X, v.Y, comp, col, (const T *)v.ux.p,
-                           (const T *)v.uy.p, comp == 1 ? (const T *)v.fx.p : (const T *)v.fy.p, nxt.p);
-        if (kt) kt->end(K_VSMOOTHC, s, bytes_c);
-        std::swap(cur.p, nxt.p);
-      }
-    hipLaunchKernelGGL(k_sbcfix<T>, G, B, 0, s, (const T *)v.S.p, v.X, v.Y, (const int8_t *)v.kx.p,
-                       (const int8_t *)v.ky.p, v.ux.p, v.uy.p, (const T *)v.fx.p, (const T *)v.fy.p);
-  };
-  if (l == 0) {
-    for (int it = 0; it < nsmooth; it++)
-      for (int comp = 1; comp <= 2; comp++)
-        for (int col = 0; col < 2; col++) gs0(comp, col);
-    hipLaunchKernelGGL(k_vbcfix<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p);
-    hipLaunchKernelGGL(k_vresid<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
-  } else {
-    csmooth(false);
-    hipLaunchKernelGGL(k_sresid<T>, G, B, 0, s, (const T *)v.S.p, v.X, v.Y, (const int8_t *)v.kx.p,
-                       (const int8_t *)v.ky.p, (const T *)v.ux.p, (const T *)v.uy.p, (const T *)v.fx.p,
-                       (const T *)v.fy.p, v.rx.p, v.ry.p);
-  }
+  const MapView none{nullptr, nullptr, nullptr, nullptr};
+  smooth(nullptr, nullptr, 0, none, none, none, none, 1, 0);
   VelLevel<T> &c = L[l + 1];
-  hipLaunchKernelGGL(k_vrestrict<T>, grd2(c.X, c.Y), B, 0, s, c.X, c.Y, v.Y, view(v.mxn), view(v.mxm), view(v.myn),
-                     view(v.mym), c.kx.p, c.ky.p, v.rx.p, v.ry.p, c.fx.p, c.fy.p);
-  ST_OK(vcycle(l + 1, s));
-  hipLaunchKernelGGL(k_vprolong<T>, G, B, 0, s, v.X, v.Y, c.Y, view(v.mxn), view(v.mxm), view(v.myn), view(v.mym),
-                     v.kx.p, v.ky.p, c.ux.p, c.uy.p, v.ux.p, v.uy.p);
   if (l == 0) {
-    for (int it = 0; it < nsmooth; it++)
-      for (int comp = 2; comp >= 1; comp--)
-        for (int col = 1; col >= 0; col--) gs0(comp, col);
-    hipLaunchKernelGGL(k_vbcfix<T>, G, B, 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p);
-  } else
-    csmooth(true);
+    hipLaunchKernelGGL(k_vresid<T>, grd2(v.X, v.Y), blk2(), 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
+  } else {
+    hipLaunchKernelGGL(k_sresid<T>, grd2(v.X, v.Y), blk2(), 0, s, (const T *)v.S.p, v.X, v.Y,
+                       (const int8_t *)v.kx.p, (const int8_t *)v.ky.p, (const T *)v.ux.p, (const T *)v.uy.p,
+                       (const T *)v.fx.p, (const T *)v.fy.p, v.rx.p, v.ry.p);
+  }
+  hipLaunchKernelGGL(k_vrestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mxn), view(v.mxm),
+                     view(v.myn), view(v.mym), c.kx.p, c.ky.p, v.rx.p, v.ry.p, c.fx.p, c.fy.p);
+  ST_OK(vcycle(l + 1, s));
+  smooth(c.ux.p, c.uy.p, c.Y, view(v.mxn), view(v.mxm), view(v.myn), view(v.mym), 0, 1);
   return I2E_OK;
 }
 
@@ -994,7 +1079,7 @@ int ScalMG<T>::refresh(const double *k, const double *c, const int8_t *mask, con
   }
   for (int l = 0; l < nl; l++) hipLaunchKernelGGL(k_pdiag<T>, grd2(L[l].X, L[l].Y), blk2(), 0, s, op(l), L[l].diag.p);
   hipLaunchKernelGGL(k_pdense<T>, dim3(nblk(nc, 256), nc), dim3(256), 0, s, op(nl - 1), nc, dense.p);
-  hipLaunchKernelGGL(k_gauss_jordan<T>, dim3(1), dim3(1024), 0, s, nc, dense.p, ainv.p);
+  hipLaunchKernelGGL(k_gauss_jordan<double>, dim3(1), dim3(1024), 0, s, nc, dense.p, ainv.p);
   HIP_OK(hipGetLastError());
   return I2E_OK;
 }

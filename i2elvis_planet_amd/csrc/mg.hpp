@@ -104,7 +104,7 @@ struct VelMG {
   const std::vector<GeoLevel> *geo = nullptr;
   int nc = 0;               // coarsest unknown count (2 N)
   DBuf<double> dense;       // n x 2n work
-  DBuf<T> ainv;             // n x n
+  DBuf<double> ainv;        // n x n, FP64 in every precision (the coarsest solve stays exact)
   std::vector<DBuf<int>> gfi, gfj;  // fine line index of each coarse line, per level
   DBuf<int> overflow;       // Galerkin entries that fell outside the 5x5 window (must stay 0)
   double mu = 0, dt = 0;
@@ -141,7 +141,7 @@ struct ScalMG {
   std::vector<ScalLevel<T>> L;
   int nc = 0;
   DBuf<double> dense;
-  DBuf<T> ainv;
+  DBuf<double> ainv;  // FP64 in every precision
   std::vector<DBuf<int>> sfi, sfj;
   int nsmooth = 2;
   // coords of the finest points; fi/fj give the coarse point selection
