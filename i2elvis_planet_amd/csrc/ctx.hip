@@ -632,7 +632,8 @@ static int stokes_prepare(i2e_ctx *c) {
   hipLaunchKernelGGL(k_stokes_rhs, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->bs.p);
   hipLaunchKernelGGL(k_stokes_pack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->F.pr, c->F.vx, c->F.vy, c->xs.p);
   if (mixed(c)) {
-    ST_OK(c->v32.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
+    ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
+    ST_OK(c->v32.refresh_from(c->v64, c->s));
     c->v32.nsmooth = c->o.nsmooth;
   } else {
     ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));

@@ -76,6 +76,24 @@ int Map1::build(const std::vector<double> &cf, const std::vector<double> &cc) {
   }
   for (int I = 0; I < nc; I++)
     if (hre[I] <= hrb[I]) hrb[I] = hre[I] = 0;
+  // normalised restriction weights (separable: the 2-D normalisation is the
+  // product of the 1-D ones)
+  std::vector<double> hrw((size_t)nc * RW, 0.0);
+  for (int I = 0; I < nc; I++) {
+    if (hre[I] - hrb[I] > RW) {
+      set_error("multigrid restriction gathers %d fine lines (at most %d)", hre[I] - hrb[I], RW);
+      return I2E_ERR_UNSUPPORTED;
+    }
+    double sum = 0;
+    for (int k = hrb[I]; k < hre[I]; k++) {
+      const double wk = (hi0[k] == I ? 1.0 - hw[k] : 0.0) + (hi0[k] + 1 == I ? hw[k] : 0.0);
+      hrw[(size_t)I * RW + (k - hrb[I])] = wk;
+      sum += wk;
+    }
+    for (int a = 0; a < RW; a++) hrw[(size_t)I * RW + a] = sum > 0 ? hrw[(size_t)I * RW + a] / sum : 0.0;
+  }
+  ST_OK(rw.alloc((size_t)nc * RW));
+  HIP_OK(hipMemcpy(rw.p, hrw.data(), (size_t)nc * RW * sizeof(double), hipMemcpyHostToDevice));
   ST_OK(i0.alloc(nf));
   ST_OK(w.alloc(nf));
   ST_OK(rb.alloc(nc));
@@ -212,41 +230,34 @@ __global__ void k_vresid(VLev<T> L, const T *ux, const T *uy, const T *fx, const
 // coarse rhs = normalised transpose interpolation of the fine residual
 // (fine BC rows were zeroed in k_vresid)
 template <class T>
+__device__ __forceinline__ T restrict_rw(const T *r, int Yf, const MapView &mx, const MapView &my, int i, int j) {
+  const int k0 = mx.rb[i], l0 = my.rb[j];
+  double wx[RW], wy[RW];
+#pragma unroll
+  for (int a = 0; a < RW; a++) {
+    wx[a] = mx.rw[i * RW + a];
+    wy[a] = my.rw[j * RW + a];
+  }
+  double s = 0;
+#pragma unroll
+  for (int a = 0; a < RW; a++)
+#pragma unroll
+    for (int b = 0; b < RW; b++)
+      if (wx[a] != 0 && wy[b] != 0) s += wx[a] * wy[b] * (double)r[(k0 + a) * Yf + l0 + b];
+  return T(s);
+}
+
+// coarse rhs = normalised transpose interpolation of the fine residual
+// (fine pointwise rows carry zero residual)
+template <class T>
 __global__ void k_vrestrict(int Xc, int Yc, int Yf, MapView xn, MapView xm, MapView yn, MapView ym, const int8_t *ckx,
                             const int8_t *cky, const T *rx, const T *ry, T *fx, T *fy) {
   IDX2;
   if (i >= Xc || j >= Yc) return;
   const int m = i * Yc + j;
-  // Vx: x at nodes, y at mids (only j <= Yc-2 carry Vx)
-  double s = 0, ws = 0;
-  if (ckx[m] == 0 && j <= Yc - 2) {
-    for (int k = xn.rb[i]; k < xn.re[i]; k++) {
-      double wx = wgt(xn, k, i);
-      if (wx == 0) continue;
-      for (int l = ym.rb[j]; l < ym.re[j]; l++) {
-        double w = wx * wgt(ym, l, j);
-        if (w == 0) continue;
-        s += w * (double)rx[k * Yf + l];
-        ws += w;
-      }
-    }
-  }
-  fx[m] = T(ws > 0 ? s / ws : 0.0);
-  s = 0;
-  ws = 0;
-  if (cky[m] == 0 && i <= Xc - 2) {
-    for (int k = xm.rb[i]; k < xm.re[i]; k++) {
-      double wx = wgt(xm, k, i);
-      if (wx == 0) continue;
-      for (int l = yn.rb[j]; l < yn.re[j]; l++) {
-        double w = wx * wgt(yn, l, j);
-        if (w == 0) continue;
-        s += w * (double)ry[k * Yf + l];
-        ws += w;
-      }
-    }
-  }
-  fy[m] = T(ws > 0 ? s / ws : 0.0);
+  // Vx: x at nodes, y at mids (only j <= Yc-2 carry Vx); Vy the other way round
+  fx[m] = (ckx[m] == 0 && j <= Yc - 2) ? restrict_rw(rx, Yf, xn, ym, i, j) : T(0);
+  fy[m] = (cky[m] == 0 && i <= Xc - 2) ? restrict_rw(ry, Yf, xm, yn, i, j) : T(0);
 }
 
 // dense matrix of the coarsest level, column c = A e_c, rows/cols ordered
@@ -510,8 +521,9 @@ __global__ void k_sresid(const T *S, int X, int Y, const int8_t *kx, const int8_
 // so the result is deterministic.  Within a colour step every point of the
 // colour is updated from the values before the step (on the Galerkin levels
 // same-colour points couple; on the finest level they do not, and this is
-// red-black Gauss-Seidel).
-constexpr int TS = 16, TH = 2, TW = TS + 2 * TH, TB = 256, TQ = TS * TS / TB;
+// red-black Gauss-Seidel).  Every level smooths with its explicit stencil
+// (finest: 3x3 window probed from the exact rows; coarser: 5x5 Galerkin).
+constexpr int TS = 16, TH = 2, TW = TS + 2 * TH, TB = 256;
 
 template <class T>
 struct TileArgs {
@@ -520,9 +532,10 @@ struct TileArgs {
   const T *fx, *fy;  // right-hand side
   const T *ux, *uy;  // input iterate (not read when zero != 0)
   T *ox, *oy;        // output iterate
-  const T *ex, *ey;  // correction on the next coarser level, added on load (nullptr: none)
+  // correction on the next coarser level, added on load (ex == nullptr: none)
+  const T *ex, *ey;
   int Yc;
-  MapView xn, xm, yn, ym;
+  MapView xn, xm, yn, ym;  // this level -> next coarser
   int zero, nsmooth, rev;
 };
 
@@ -555,6 +568,66 @@ __device__ __forceinline__ void tile_load(const TileArgs<T> &A, T *lx, T *ly, in
   }
 }
 
+// S u at a tile point from LDS (out-of-grid halo entries are 0, and so are
+// the coefficients that would multiply them)
+template <class T, int W>
+__device__ __forceinline__ T s_apply_lds(const T *S, size_t R, size_t r, const T *lx, const T *ly, int la, int lb) {
+  T acc = 0;
+#pragma unroll
+  for (int cc = 0; cc < 2; cc++) {
+    const T *u = cc ? ly : lx;
+#pragma unroll
+    for (int di = -W; di <= W; di++)
+#pragma unroll
+      for (int dj = -W; dj <= W; dj++) acc += S[(size_t)sslot(W, cc, di, dj) * R + r] * u[(la + di) * TW + lb + dj];
+  }
+  return acc;
+}
+
+template <class T, int W>
+__global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const T *S) {
+  __shared__ T lx[TW * TW], ly[TW * TW];
+  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
+  const int X = A.X, Y = A.Y, N = X * Y;
+  const size_t R = 2 * (size_t)N;
+  tile_load(A, lx, ly, i0, j0);
+  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
+  const int li = (i - i0) * TW + (j - j0);
+  const bool ok = i < X && j < Y;
+  const int m = ok ? i * Y + j : 0;
+  const T fx = ok ? A.fx[m] : T(0), fy = ok ? A.fy[m] : T(0);
+  const int kx = ok ? A.kx[m] : -1, ky = ok ? A.ky[m] : -1;
+  __syncthreads();
+  for (int it = 0; it < A.nsmooth; it++)
+    for (int st = 0; st < 4; st++) {
+      const int k = A.rev ? 3 - st : st, comp = 1 + k / 2, col = k & 1;
+      const bool act = ok && ((i + j) & 1) == col;
+      T nv = 0;
+      if (act) {
+        // row index depends on the step: the stencil is re-read per colour
+        // step rather than held in registers for both components
+        const size_t r = (size_t)(comp - 1) * N + m;
+        const T d = S[(size_t)sslot(W, comp - 1, 0, 0) * R + r];
+        nv = (comp == 1 ? lx : ly)[li] +
+             ((comp == 1 ? fx : fy) - s_apply_lds<T, W>(S, R, r, lx, ly, li / TW, li % TW)) / d;
+      }
+      __syncthreads();
+      if (act) (comp == 1 ? lx : ly)[li] = nv;
+      __syncthreads();
+    }
+  if (!ok) return;
+  // re-solve the pointwise rows exactly: their scale (mukoef) dwarfs the
+  // momentum rows, and the last colour steps left them inconsistent
+  T bx = lx[li], by = ly[li];
+  if (kx > 0)
+    bx += (fx - s_apply_lds<T, W>(S, R, m, lx, ly, li / TW, li % TW)) / S[(size_t)sslot(W, 0, 0, 0) * R + m];
+  if (ky > 0)
+    by += (fy - s_apply_lds<T, W>(S, R, (size_t)N + m, lx, ly, li / TW, li % TW)) /
+          S[(size_t)sslot(W, 1, 0, 0) * R + (size_t)N + m];
+  A.ox[m] = bx;
+  A.oy[m] = by;
+}
+
 // unknowns of the finest level read from the LDS tile (global indices)
 template <class T>
 struct LdsV {
@@ -565,113 +638,64 @@ struct LdsV {
   __device__ __forceinline__ T pp(int, int) const { return T(0); }
 };
 
-// S u at a tile point from LDS (Galerkin levels; out-of-grid halo entries are 0)
+// finest level: the same tile scheme on the matrix-free rows (the nodal
+// coefficient fields are read instead of an explicit 3x3 stencil, which
+// would be several times the bytes), red-black Gauss-Seidel
 template <class T>
-__device__ __forceinline__ T s_apply_lds(const T *S, size_t R, size_t r, const T *lx, const T *ly, int la, int lb) {
-  T acc = 0;
-#pragma unroll
-  for (int cc = 0; cc < 2; cc++) {
-    const T *u = cc ? ly : lx;
-#pragma unroll
-    for (int di = -2; di <= 2; di++)
-#pragma unroll
-      for (int dj = -2; dj <= 2; dj++) acc += S[(size_t)sslot(2, cc, di, dj) * R + r] * u[(la + di) * TW + lb + dj];
+__global__ __launch_bounds__(TB) void k_tsmooth0(TileArgs<T> A, VLev<T> L, const T *dxa, const T *dya) {
+  __shared__ T lx[TW * TW], ly[TW * TW];
+  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
+  const int X = A.X, Y = A.Y;
+  tile_load(A, lx, ly, i0, j0);
+  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
+  const int li = (i - i0) * TW + (j - j0);
+  const bool ok = i < X && j < Y;
+  const int m = ok ? i * Y + j : 0;
+  const T fx = ok ? A.fx[m] : T(0), fy = ok ? A.fy[m] : T(0);
+  const int kx = ok ? A.kx[m] : -1, ky = ok ? A.ky[m] : -1;
+  const T dx = ok ? dxa[m] : T(1), dy = ok ? dya[m] : T(1);
+  const LdsV<T> lv{lx, ly, i0, j0};
+  __syncthreads();
+  for (int it = 0; it < A.nsmooth; it++)
+    for (int st = 0; st < 4; st++) {
+      const int k = A.rev ? 3 - st : st, comp = 1 + k / 2, col = k & 1;
+      const bool act = ok && ((i + j) & 1) == col;
+      T nv = 0;
+      if (act) nv = (comp == 1 ? lx[li] : ly[li]) + ((comp == 1 ? fx : fy) - vrow(L, lv, comp, i, j)) / (comp == 1 ? dx : dy);
+      __syncthreads();
+      if (act) (comp == 1 ? lx : ly)[li] = nv;
+      __syncthreads();
+    }
+  if (!ok) return;
+  // re-solve the pointwise rows exactly (their neighbour is adjacent, so in the tile)
+  T bx = lx[li], by = ly[li];
+  const T imu = T(1.0 / L.o.mu);
+  if (kx > 0) {
+    const int nb = L.nbx[m];
+    bx = fx * imu + (kx == 1 && nb >= 0 ? L.kkx[m] * lv.vx(nb / Y, nb % Y) : T(0));
   }
-  return acc;
+  if (ky > 0) {
+    const int nb = L.nby[m];
+    by = fy * imu + (ky == 1 && nb >= 0 ? L.kky[m] * lv.vy(nb / Y, nb % Y) : T(0));
+  }
+  A.ox[m] = bx;
+  A.oy[m] = by;
 }
 
-// fine = true: finest level (matrix-free rows, diagonals dx/dy); false:
-// Galerkin level with stencil S
-template <class T, bool FINE>
-__global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, VLev<T> L, const T *dx, const T *dy, const T *S) {
+// r = f - S u on equation rows, 0 on pointwise rows (u staged through LDS)
+template <class T, int W>
+__global__ __launch_bounds__(TB) void k_tresid(TileArgs<T> A, const T *S, T *rxo, T *ryo) {
   __shared__ T lx[TW * TW], ly[TW * TW];
   const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
   const int X = A.X, Y = A.Y, N = X * Y;
   const size_t R = 2 * (size_t)N;
   tile_load(A, lx, ly, i0, j0);
-  int gi[TQ], gj[TQ];
-  T fxr[TQ], fyr[TQ];
-  int8_t kxr[TQ], kyr[TQ];
-#pragma unroll
-  for (int q = 0; q < TQ; q++) {
-    const int p = threadIdx.x + q * TB;
-    gi[q] = i0 + TH + p / TS;
-    gj[q] = j0 + TH + p % TS;
-    const bool ok = gi[q] < X && gj[q] < Y;
-    const int m = ok ? gi[q] * Y + gj[q] : 0;
-    fxr[q] = ok ? A.fx[m] : T(0);
-    fyr[q] = ok ? A.fy[m] : T(0);
-    kxr[q] = ok ? A.kx[m] : int8_t(-1);
-    kyr[q] = ok ? A.ky[m] : int8_t(-1);
-  }
   __syncthreads();
-  const LdsV<T> lv{lx, ly, i0, j0};
-  for (int it = 0; it < A.nsmooth; it++)
-    for (int st = 0; st < 4; st++) {
-      const int k = A.rev ? 3 - st : st, comp = 1 + k / 2, col = k & 1;
-      T nv[TQ];
-#pragma unroll
-      for (int q = 0; q < TQ; q++) {
-        const int i = gi[q], j = gj[q];
-        if ((comp == 1 ? kxr[q] : kyr[q]) < 0 || ((i + j) & 1) != col) continue;
-        const int m = i * Y + j, li = (i - i0) * TW + (j - j0);
-        const T u = comp == 1 ? lx[li] : ly[li];
-        const T f = comp == 1 ? fxr[q] : fyr[q];
-        if (FINE) {
-          nv[q] = u + (f - vrow(L, lv, comp, i, j)) / (comp == 1 ? dx[m] : dy[m]);
-        } else {
-          const size_t r = (size_t)(comp - 1) * N + m;
-          const T d = S[(size_t)sslot(2, comp - 1, 0, 0) * R + r];
-          nv[q] = u + (f - s_apply_lds(S, R, r, lx, ly, i - i0, j - j0)) / d;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < TQ; q++) {
-        const int i = gi[q], j = gj[q];
-        if ((comp == 1 ? kxr[q] : kyr[q]) < 0 || ((i + j) & 1) != col) continue;
-        (comp == 1 ? lx : ly)[(i - i0) * TW + (j - j0)] = nv[q];
-      }
-      __syncthreads();
-    }
-  // re-solve the pointwise rows exactly: their scale (mukoef) dwarfs the
-  // momentum rows, and the last colour steps left them inconsistent
-  T bx[TQ], by[TQ];
-#pragma unroll
-  for (int q = 0; q < TQ; q++) {
-    const int i = gi[q], j = gj[q], li = (i - i0) * TW + (j - j0);
-    bx[q] = kxr[q] >= 0 ? lx[li] : T(0);
-    by[q] = kyr[q] >= 0 ? ly[li] : T(0);
-    if (kxr[q] <= 0 && kyr[q] <= 0) continue;
-    const int m = i * Y + j;
-    if (FINE) {
-      const T imu = T(1.0 / L.o.mu);
-      if (kxr[q] > 0) {
-        const int nb = L.nbx[m];
-        bx[q] = fxr[q] * imu + (kxr[q] == 1 && nb >= 0 ? L.kkx[m] * lv.vx(nb / Y, nb % Y) : T(0));
-      }
-      if (kyr[q] > 0) {
-        const int nb = L.nby[m];
-        by[q] = fyr[q] * imu + (kyr[q] == 1 && nb >= 0 ? L.kky[m] * lv.vy(nb / Y, nb % Y) : T(0));
-      }
-    } else {
-      if (kxr[q] > 0) {
-        const size_t r = m;
-        bx[q] = bx[q] + (fxr[q] - s_apply_lds(S, R, r, lx, ly, i - i0, j - j0)) / S[(size_t)sslot(2, 0, 0, 0) * R + r];
-      }
-      if (kyr[q] > 0) {
-        const size_t r = (size_t)N + m;
-        by[q] = by[q] + (fyr[q] - s_apply_lds(S, R, r, lx, ly, i - i0, j - j0)) / S[(size_t)sslot(2, 1, 0, 0) * R + r];
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < TQ; q++) {
-    if (kxr[q] < 0) continue;
-    const int m = gi[q] * Y + gj[q];
-    A.ox[m] = bx[q];
-    A.oy[m] = by[q];
-  }
+  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
+  if (i >= X || j >= Y) return;
+  const int m = i * Y + j, la = i - i0, lb = j - j0;
+  rxo[m] = A.kx[m] == 0 ? A.fx[m] - s_apply_lds<T, W>(S, R, m, lx, ly, la, lb) : T(0);
+  ryo[m] = A.ky[m] == 0 ? A.fy[m] - s_apply_lds<T, W>(S, R, (size_t)N + m, lx, ly, la, lb) : T(0);
 }
 
 // ---------------------------------------------------------------- VelMG host
@@ -852,6 +876,31 @@ int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, cons
 }
 
 template <class T>
+int VelMG<T>::refresh_from(const VelMG<double> &d, hipStream_t s) {
+  mu = d.mu;
+  dt = d.dt;
+  const int nl = (int)L.size();
+  if ((int)d.L.size() != nl || d.nc != nc) {
+    set_error("velocity multigrid hierarchies differ");
+    return I2E_ERR_ARG;
+  }
+  auto cast = [&](const DBuf<double> &a, DBuf<T> &b, size_t n) {
+    hipLaunchKernelGGL(k_cast<T>, dim3(nblk(n, 256)), dim3(256), 0, s, (int)n, a.p, b.p);
+  };
+  const int N0 = L[0].N;
+  cast(d.L[0].nu, L[0].nu, N0);
+  cast(d.L[0].nd, L[0].nd, N0);
+  cast(d.L[0].ro, L[0].ro, N0);
+  cast(d.L[0].gp, L[0].gp, N0);
+  cast(d.L[0].dx, L[0].dx, N0);
+  cast(d.L[0].dy, L[0].dy, N0);
+  for (int l = 1; l < nl; l++) cast(d.L[l].S, L[l].S, (size_t)2 * L[l].N * 50);
+  HIP_OK(hipMemcpyAsync(ainv.p, d.ainv.p, sizeof(double) * (size_t)nc * nc, hipMemcpyDeviceToDevice, s));
+  HIP_OK(hipGetLastError());
+  return I2E_OK;
+}
+
+template <class T>
 int VelMG<T>::vcycle(int l, hipStream_t s) {
   const int nl = (int)L.size();
   VelLevel<T> &v = L[l];
@@ -861,42 +910,54 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
     return I2E_OK;
   }
   const dim3 TG((v.Y + TS - 1) / TS, (v.X + TS - 1) / TS), TBK(TB);
-  const VLev<T> o = lev(l);
   const int kid = l == 0 ? K_VSMOOTH0 : K_VSMOOTHC;
+  const VLev<T> o = lev(l);
   // algorithmic bytes of one smoothing pass: rhs, input and output iterate
   // (two components each), the row kinds, and the coefficients once (finest:
-  // nu, nd, ro, gp and the diagonals; Galerkin levels: 50 stencil slots per row)
+  // nu, nd, ro, gp and the two diagonals; Galerkin levels: the 50-slot stencil)
   const double coef = l == 0 ? 6.0 : 100.0;
   const double bytes = sizeof(T) * (6.0 + coef) * v.N + 2.0 * v.N;
-  auto smooth = [&](const T *ex, const T *ey, int Yc, MapView xn, MapView xm, MapView yn, MapView ym, int zero,
-                    int rev) {
-    TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p,
-                  ex, ey, Yc, xn, xm, yn, ym, zero, nsmooth, rev};
+  const MapView none{nullptr, nullptr, nullptr, nullptr};
+  TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr, 0,
+                none, none, none, none, 1, nsmooth, 0};
+  auto launch = [&](const TileArgs<T> &a) {
     if (kt) kt->begin(kid, s);
     if (l == 0)
-      hipLaunchKernelGGL((k_tsmooth<T, true>), TG, TBK, 0, s, A, o, (const T *)v.dx.p, (const T *)v.dy.p,
-                         (const T *)nullptr);
+      hipLaunchKernelGGL(k_tsmooth0<T>, TG, TBK, 0, s, a, o, (const T *)v.dx.p, (const T *)v.dy.p);
     else
-      hipLaunchKernelGGL((k_tsmooth<T, false>), TG, TBK, 0, s, A, o, (const T *)nullptr, (const T *)nullptr,
-                         (const T *)v.S.p);
+      hipLaunchKernelGGL((k_tsmooth<T, 2>), TG, TBK, 0, s, a, (const T *)v.S.p);
     if (kt) kt->end(kid, s, bytes);
     std::swap(v.ux.p, v.ux2.p);
     std::swap(v.uy.p, v.uy2.p);
   };
-  const MapView none{nullptr, nullptr, nullptr, nullptr};
-  smooth(nullptr, nullptr, 0, none, none, none, none, 1, 0);
-  VelLevel<T> &c = L[l + 1];
-  if (l == 0) {
+  launch(A);
+  // residual of the smoothed iterate, restricted to the next level
+  TileArgs<T> R = A;
+  R.ux = v.ux.p;
+  R.uy = v.uy.p;
+  R.zero = 0;
+  if (l == 0)
     hipLaunchKernelGGL(k_vresid<T>, grd2(v.X, v.Y), blk2(), 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
-  } else {
-    hipLaunchKernelGGL(k_sresid<T>, grd2(v.X, v.Y), blk2(), 0, s, (const T *)v.S.p, v.X, v.Y,
-                       (const int8_t *)v.kx.p, (const int8_t *)v.ky.p, (const T *)v.ux.p, (const T *)v.uy.p,
-                       (const T *)v.fx.p, (const T *)v.fy.p, v.rx.p, v.ry.p);
-  }
+  else
+    hipLaunchKernelGGL((k_tresid<T, 2>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
+  VelLevel<T> &c = L[l + 1];
   hipLaunchKernelGGL(k_vrestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mxn), view(v.mxm),
                      view(v.myn), view(v.mym), c.kx.p, c.ky.p, v.rx.p, v.ry.p, c.fx.p, c.fy.p);
   ST_OK(vcycle(l + 1, s));
-  smooth(c.ux.p, c.uy.p, c.Y, view(v.mxn), view(v.mxm), view(v.myn), view(v.mym), 0, 1);
+  TileArgs<T> P = R;
+  P.ux = v.ux.p;
+  P.uy = v.uy.p;
+  P.ox = v.ux2.p;
+  P.oy = v.uy2.p;
+  P.ex = c.ux.p;
+  P.ey = c.uy.p;
+  P.Yc = c.Y;
+  P.xn = view(v.mxn);
+  P.xm = view(v.mxm);
+  P.yn = view(v.myn);
+  P.ym = view(v.mym);
+  P.rev = 1;
+  launch(P);
   return I2E_OK;
 }
 

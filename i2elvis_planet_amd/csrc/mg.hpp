@@ -48,11 +48,15 @@ struct DBuf {
 };
 
 // 1D interpolation map fine -> coarse coordinate sets
+// restriction of one coarse index gathers at most RW consecutive fine indices
+constexpr int RW = 4;
+
 struct Map1 {
   int nf = 0, nc = 0;
   DBuf<int> i0;      // [nf]
   DBuf<double> w;    // [nf] weight of i0+1
   DBuf<int> rb, re;  // [nc] fine index range touching coarse index
+  DBuf<double> rw;   // [nc*RW] normalised restriction weights of fine indices rb.. (0 past re)
   int build(const std::vector<double> &cf, const std::vector<double> &cc);
 };
 
@@ -60,8 +64,9 @@ struct MapView {
   const int *i0;
   const double *w;
   const int *rb, *re;
+  const double *rw;
 };
-inline MapView view(const Map1 &m) { return MapView{m.i0.p, m.w.p, m.rb.p, m.re.p}; }
+inline MapView view(const Map1 &m) { return MapView{m.i0.p, m.w.p, m.rb.p, m.re.p, m.rw.p}; }
 
 // level geometry shared by all solvers on one grid hierarchy
 struct GeoLevel {
@@ -117,6 +122,11 @@ struct VelMG {
   // diagonals and the coarse inverse
   int refresh(const double *nu, const double *nd, const double *ro, const double *gp, double mu, double dt,
               hipStream_t s);
+  // the same hierarchy rounded from an FP64 one on the same geometry: the
+  // Galerkin products and the coarsest inverse are formed in FP64 (products
+  // of rounded fine-level entries lose the near-cancellation that carries
+  // rigid-body modes of stiff regions), only the stored operators are T
+  int refresh_from(const VelMG<double> &d, hipStream_t s);
   // z = M^-1 r for the velocity block; f/u live in L[0].fx/fy and L[0].ux/uy
   int vcycle(int l, hipStream_t s);
   VLev<T> lev(int l) const;
