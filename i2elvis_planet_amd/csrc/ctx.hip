@@ -208,7 +208,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->precision = I2E_PREC_MIXED;
   o->restart = 200;
   o->maxit = 600;
-  o->nsmooth = 2;
+  o->nsmooth = 3;
   o->stokes_rtol = 1e-8;
   o->heat_rtol = 1e-10;
   o->grav_rtol = 1e-10;

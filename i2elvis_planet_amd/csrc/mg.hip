@@ -597,20 +597,35 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const T *S) {
   const int m = ok ? i * Y + j : 0;
   const T fx = ok ? A.fx[m] : T(0), fy = ok ? A.fy[m] : T(0);
   const int kx = ok ? A.kx[m] : -1, ky = ok ? A.ky[m] : -1;
+  // both stencil rows of this thread's point stay in registers for the pass
+  constexpr int NSL = 2 * (2 * W + 1) * (2 * W + 1);
+  T sx[NSL], sy[NSL];
+#pragma unroll
+  for (int q = 0; q < NSL; q++) {
+    sx[q] = ok ? S[(size_t)q * R + m] : T(0);
+    sy[q] = ok ? S[(size_t)q * R + (size_t)N + m] : T(0);
+  }
+  const T dx = ok ? sx[sslot(W, 0, 0, 0)] : T(1), dy = ok ? sy[sslot(W, 1, 0, 0)] : T(1);
+  const int la = li / TW, lb = li % TW;
+  auto apply = [&](const T *sv) {
+    T acc = 0;
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++) {
+      const T *u = cc ? ly : lx;
+#pragma unroll
+      for (int di = -W; di <= W; di++)
+#pragma unroll
+        for (int dj = -W; dj <= W; dj++) acc += sv[sslot(W, cc, di, dj)] * u[(la + di) * TW + lb + dj];
+    }
+    return acc;
+  };
   __syncthreads();
   for (int it = 0; it < A.nsmooth; it++)
     for (int st = 0; st < 4; st++) {
       const int k = A.rev ? 3 - st : st, comp = 1 + k / 2, col = k & 1;
       const bool act = ok && ((i + j) & 1) == col;
       T nv = 0;
-      if (act) {
-        // row index depends on the step: the stencil is re-read per colour
-        // step rather than held in registers for both components
-        const size_t r = (size_t)(comp - 1) * N + m;
-        const T d = S[(size_t)sslot(W, comp - 1, 0, 0) * R + r];
-        nv = (comp == 1 ? lx : ly)[li] +
-             ((comp == 1 ? fx : fy) - s_apply_lds<T, W>(S, R, r, lx, ly, li / TW, li % TW)) / d;
-      }
+      if (act) nv = comp == 1 ? lx[li] + (fx - apply(sx)) / dx : ly[li] + (fy - apply(sy)) / dy;
       __syncthreads();
       if (act) (comp == 1 ? lx : ly)[li] = nv;
       __syncthreads();
@@ -619,11 +634,8 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const T *S) {
   // re-solve the pointwise rows exactly: their scale (mukoef) dwarfs the
   // momentum rows, and the last colour steps left them inconsistent
   T bx = lx[li], by = ly[li];
-  if (kx > 0)
-    bx += (fx - s_apply_lds<T, W>(S, R, m, lx, ly, li / TW, li % TW)) / S[(size_t)sslot(W, 0, 0, 0) * R + m];
-  if (ky > 0)
-    by += (fy - s_apply_lds<T, W>(S, R, (size_t)N + m, lx, ly, li / TW, li % TW)) /
-          S[(size_t)sslot(W, 1, 0, 0) * R + (size_t)N + m];
+  if (kx > 0) bx += (fx - apply(sx)) / dx;
+  if (ky > 0) by += (fy - apply(sy)) / dy;
   A.ox[m] = bx;
   A.oy[m] = by;
 }
