@@ -17,6 +17,8 @@ I2E_ERR_NODEVICE = 6
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+SCHUR_BFBT = 0
+SCHUR_MASS = 1
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -77,7 +79,7 @@ class SolverOpts(C.Structure):
         ("precision", C.c_int32), ("restart", C.c_int32), ("maxit", C.c_int32), ("nsmooth", C.c_int32),
         ("stokes_rtol", C.c_double), ("heat_rtol", C.c_double), ("grav_rtol", C.c_double),
         ("schur_scale", C.c_double), ("omega", C.c_double),
-        ("coarse_max", C.c_int32), ("pad1", C.c_int32),
+        ("coarse_max", C.c_int32), ("schur", C.c_int32),
     ]
 
 

@@ -79,6 +79,148 @@ __global__ void k_schur_bc(SOp<double> o, BcTab bc, const double *r, double *z) 
   z[3 * m] = v;
 }
 
+// ---------------------------------------------------------------- BFBt Schur
+// (B A^-1 B^T)^-1 ~ L^-1 (B C^-1 A C^-1 B^T) L^-1 with L = B C^-1 B^T, B the
+// divergence of the continuity rows (unscaled), B^T the pressure-gradient
+// columns of the momentum rows, C = diag(A) on momentum rows (Elman et al.
+// 2006, scaled BFBt).  Equation cells E: i, j >= 1 without a P boundary row;
+// momentum unknowns: kind 0 of the velocity multigrid.  The cell grid is
+// (X-1) x (Y-1), cell (i,j) of the node grid at (i-1, j-1).
+
+// C^-1 on momentum rows (0 elsewhere) and the explicit 5-point L on cells
+__global__ void k_bf_setup(Grid g, const int *bondm, const int8_t *kx, const int8_t *ky, const double *dx,
+                           const double *dy, double *cix, double *ciy, double *LS, int8_t *lmask) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int X = g.X, Y = g.Y;
+  if (j >= Y) return;
+  const int m = i * Y + j;
+  cix[m] = kx[m] == 0 ? 1.0 / dx[m] : 0.0;
+  ciy[m] = ky[m] == 0 ? 1.0 / dy[m] : 0.0;
+}
+__global__ void k_bf_stencil(Grid g, const int *bondm, const double *cix, const double *ciy, double *LS,
+                             int8_t *lmask) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x, I = blockIdx.y;
+  const int X = g.X, Y = g.Y, Yc = Y - 1, Nc = (X - 1) * (Y - 1);
+  if (J >= Yc) return;
+  const int i = I + 1, j = J + 1, c = I * Yc + J;
+  const double *gx = g.gx, *gy = g.gy;
+  auto isE = [&](int a, int b) { return a >= 1 && b >= 1 && a <= X - 1 && b <= Y - 1 && !bondm[3 * (a * Y + b)]; };
+  double st[9];
+  for (int q = 0; q < 9; q++) st[q] = 0.0;
+  if (!isE(i, j)) {
+    st[4] = 1.0;
+    lmask[c] = 1;
+  } else {
+    lmask[c] = 0;
+    const double hx = gx[i] - gx[i - 1], hy = gy[j] - gy[j - 1];
+    // right face Vx(i, j-1): B +1/hx; B^T couples (i,j) +1/xkf(i), (i+1,j) -1/xkf(i)
+    if (i <= X - 2) {
+      const double w = cix[i * Y + j - 1] / hx, xk = (gx[i + 1] - gx[i - 1]) / 2.0;
+      st[4] += w / xk;
+      if (isE(i + 1, j)) st[7] -= w / xk;
+    }
+    // left face Vx(i-1, j-1): B -1/hx; B^T couples (i,j) -1/xkf(i-1), (i-1,j) +1/xkf(i-1)
+    if (i - 1 >= 1) {
+      const double w = cix[(i - 1) * Y + j - 1] / hx, xk = (gx[i] - gx[i - 2]) / 2.0;
+      st[4] += w / xk;
+      if (isE(i - 1, j)) st[1] -= w / xk;
+    }
+    // bottom face Vy(i-1, j): B +1/hy; B^T couples (i,j) +1/ykf(j), (i,j+1) -1/ykf(j)
+    if (j <= Y - 2) {
+      const double w = ciy[(i - 1) * Y + j] / hy, yk = (gy[j + 1] - gy[j - 1]) / 2.0;
+      st[4] += w / yk;
+      if (isE(i, j + 1)) st[5] -= w / yk;
+    }
+    // top face Vy(i-1, j-1): B -1/hy; B^T couples (i,j) -1/ykf(j-1), (i,j-1) +1/ykf(j-1)
+    if (j - 1 >= 1) {
+      const double w = ciy[(i - 1) * Y + j - 1] / hy, yk = (gy[j] - gy[j - 2]) / 2.0;
+      st[4] += w / yk;
+      if (isE(i, j - 1)) st[3] -= w / yk;
+    }
+  }
+  for (int q = 0; q < 9; q++) LS[(size_t)q * Nc + c] = st[q];
+}
+
+// z velocity part from the V-cycle; cell rhs f = r_p - mu div z_v on E
+template <class T>
+__global__ void k_bf_in(SOp<double> o, BcTab bc, const double *r, const T *ux, const T *uy, double *z, T *f) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int Y = o.Y;
+  if (j >= Y) return;
+  const int m = i * Y + j;
+  z[3 * m + 1] = (double)ux[m];
+  z[3 * m + 2] = (double)uy[m];
+  if (!(i && j)) {
+    z[3 * m] = r[3 * m] / o.mu;  // ghost row
+    return;
+  }
+  const int c = (i - 1) * (Y - 1) + (j - 1);
+  if (bc.bondm[3 * m]) {
+    f[c] = 0;
+    return;
+  }
+  const double dv = ((double)ux[i * Y + j - 1] - (double)ux[(i - 1) * Y + j - 1]) / (o.gx[i] - o.gx[i - 1]) +
+                    ((double)uy[(i - 1) * Y + j] - (double)uy[(i - 1) * Y + j - 1]) / (o.gy[j] - o.gy[j - 1]);
+  f[c] = T(r[3 * m] - o.mu * dv);
+}
+
+// t = C^-1 B^T q on momentum rows (q on the cell grid, 0 off E)
+template <class T>
+__global__ void k_bf_grad(Grid g, const double *cix, const double *ciy, const int8_t *lmask, const T *q, T *tx,
+                          T *ty) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int X = g.X, Y = g.Y, Yc = Y - 1;
+  if (j >= Y) return;
+  const int m = i * Y + j;
+  auto qa = [&](int a, int b) -> double {  // q at node-cell (a,b)
+    if (a < 1 || b < 1 || a > X - 1 || b > Y - 1) return 0.0;
+    const int c = (a - 1) * Yc + (b - 1);
+    return lmask[c] ? 0.0 : (double)q[c];
+  };
+  double vx = 0, vy = 0;
+  if (cix[m] != 0) vx = -cix[m] * (qa(i + 1, j + 1) - qa(i, j + 1)) / ((g.gx[i + 1] - g.gx[i - 1]) / 2.0);
+  if (ciy[m] != 0) vy = -ciy[m] * (qa(i + 1, j + 1) - qa(i + 1, j)) / ((g.gy[j + 1] - g.gy[j - 1]) / 2.0);
+  tx[m] = T(vx);
+  ty[m] = T(vy);
+}
+
+// w = C^-1 A t on momentum rows (0 elsewhere), A the finest multigrid operator
+template <class T>
+__global__ void k_bf_apply(VLev<T> L, const double *cix, const double *ciy, const T *tx, const T *ty, T *wx, T *wy) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int Y = L.o.Y;
+  if (j >= Y) return;
+  const int m = i * Y + j;
+  VSplit<T> v{tx, ty, nullptr, Y};
+  wx[m] = cix[m] != 0 ? T(cix[m]) * xmom(L.o, v, i, j, false) : T(0);
+  wy[m] = ciy[m] != 0 ? T(ciy[m]) * ymom(L.o, v, i, j, false) : T(0);
+}
+
+// cell rhs = B w on E
+template <class T>
+__global__ void k_bf_div(Grid g, const int8_t *lmask, const T *wx, const T *wy, T *f) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x, I = blockIdx.y;
+  const int Y = g.Y, Yc = Y - 1;
+  if (J >= Yc) return;
+  const int i = I + 1, j = J + 1, c = I * Yc + J;
+  if (lmask[c]) {
+    f[c] = 0;
+    return;
+  }
+  f[c] = T(((double)wx[i * Y + j - 1] - (double)wx[(i - 1) * Y + j - 1]) / (g.gx[i] - g.gx[i - 1]) +
+           ((double)wy[(i - 1) * Y + j] - (double)wy[(i - 1) * Y + j - 1]) / (g.gy[j] - g.gy[j - 1]));
+}
+
+// z_p = -q / mu on E (P boundary rows follow in k_schur_bc)
+template <class T>
+__global__ void k_bf_out(Grid g, double mu, const int8_t *lmask, const T *q, double *z) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x, I = blockIdx.y;
+  const int Y = g.Y, Yc = Y - 1;
+  if (J >= Yc) return;
+  const int c = I * Yc + J, m = (I + 1) * Y + J + 1;
+  if (!lmask[c]) z[3 * m] = -(double)q[c] / mu;
+}
+
 // heat preconditioner input/output: f = rho*cp*r on equation rows, r on BC rows
 template <class T>
 __global__ void k_heat_pre(int N, const double *r, const double *sc, const double *ro, const double *cp,
@@ -183,6 +325,10 @@ struct i2e_ctx {
   std::vector<GeoLevel> geo;
   VelMG<float> v32;
   VelMG<double> v64;
+  CellMG<float> l32;
+  CellMG<double> l64;
+  DBuf<double> bcix, bciy, bLS;  // BFBt: C^-1 on momentum rows, FP64 cell stencil
+  DBuf<int8_t> blmask;
   ScalMG<float> h32, g32;
   ScalMG<double> h64, g64;
   DBuf<double> hk, hc, hdm, gk, gc, gdm, gsc, hsc;
@@ -215,6 +361,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->schur_scale = 2.0;
   o->omega = 1.0;
   o->coarse_max = 8;
+  o->schur = I2E_SCHUR_BFBT;
   return I2E_OK;
 }
 
@@ -291,6 +438,8 @@ int i2e_destroy(i2e_ctx *c) {
   c->kt.ev.clear();
   c->v32.release();
   c->v64.release();
+  c->l32.release();
+  c->l64.release();
   c->h32.release();
   c->g32.release();
   c->h64.release();
@@ -303,7 +452,9 @@ int i2e_destroy(i2e_ctx *c) {
     g.gy.release();
   }
   DBuf<double> *d[] = {&c->gx, &c->gy, &c->nodes, &c->acc, &c->work, &c->sol1, &c->tk2, &c->red, &c->xs,
-                       &c->bs, &c->bondv, &c->hk, &c->hc, &c->hdm, &c->gk, &c->gc, &c->gdm, &c->gsc, &c->hsc};
+                       &c->bs, &c->bondv, &c->hk, &c->hc, &c->hdm, &c->gk, &c->gc, &c->gdm, &c->gsc, &c->hsc,
+                       &c->bcix, &c->bciy, &c->bLS};
+  c->blmask.release();
   for (auto *b : d) b->release();
   c->mfl.release();
   c->mt.release();
@@ -320,7 +471,9 @@ int i2e_destroy(i2e_ctx *c) {
 
 int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
   if (!c || !o) return I2E_ERR_ARG;
-  if (o->restart < 1 || o->restart > 400 || o->maxit < 1 || o->nsmooth < 0 || o->coarse_max < 2) return I2E_ERR_ARG;
+  if (o->restart < 1 || o->restart > 400 || o->maxit < 1 || o->nsmooth < 0 || o->coarse_max < 2 ||
+      (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS))
+    return I2E_ERR_ARG;
   bool regeo = c->setup_done && o->coarse_max != c->o.coarse_max;
   c->o = *o;
   if (regeo) c->setup_done = false;
@@ -461,6 +614,15 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
   ST_OK(c->h64.setup(hx, hy, c->o.coarse_max));
   ST_OK(c->g32.setup(cxs, cys, c->o.coarse_max));
   ST_OK(c->g64.setup(cxs, cys, c->o.coarse_max));
+  ST_OK(c->l32.setup(cxs, cys, c->o.coarse_max));
+  ST_OK(c->l64.setup(cxs, cys, c->o.coarse_max));
+  {
+    const size_t Nc = (size_t)(X - 1) * (Y - 1);
+    ST_OK(c->bcix.alloc(N));
+    ST_OK(c->bciy.alloc(N));
+    ST_OK(c->bLS.alloc(9 * Nc));
+    ST_OK(c->blmask.alloc(Nc));
+  }
   ST_OK(c->fs.init(3 * N, c->o.restart));
   ST_OK(c->fh.init(N, c->o.restart));
   ST_OK(c->fgv.init(N, c->o.restart));
@@ -631,21 +793,62 @@ static int stokes_prepare(i2e_ctx *c) {
   SOp<double> o = c->sop();
   hipLaunchKernelGGL(k_stokes_rhs, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->bs.p);
   hipLaunchKernelGGL(k_stokes_pack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->F.pr, c->F.vx, c->F.vy, c->xs.p);
+  ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
+  c->v64.nsmooth = c->o.nsmooth;
   if (mixed(c)) {
-    ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
     ST_OK(c->v32.refresh_from(c->v64, c->s));
     c->v32.nsmooth = c->o.nsmooth;
-  } else {
-    ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
-    c->v64.nsmooth = c->o.nsmooth;
+  }
+  if (c->o.schur == I2E_SCHUR_BFBT) {
+    const VelLevel<double> &f0 = c->v64.L[0];
+    hipLaunchKernelGGL(k_bf_setup, nb2(X, Y), dim3(256), 0, c->s, c->grid(), c->bondm.p, f0.kx.p, f0.ky.p, f0.dx.p,
+                       f0.dy.p, c->bcix.p, c->bciy.p, c->bLS.p, c->blmask.p);
+    hipLaunchKernelGGL(k_bf_stencil, nb2(X - 1, Y - 1), dim3(256), 0, c->s, c->grid(), c->bondm.p, c->bcix.p,
+                       c->bciy.p, c->bLS.p, c->blmask.p);
+    ST_OK(c->l64.refresh(c->bLS.p, c->blmask.p, c->s));
+    c->l64.nsmooth = c->o.nsmooth;
+    if (mixed(c)) {
+      ST_OK(c->l32.refresh_from(c->l64, c->s));
+      c->l32.nsmooth = c->o.nsmooth;
+    }
   }
   return I2E_OK;
 }
 
 // z = P^-1 r, lower block-triangular: velocity V-cycle, then the Schur diagonal
+extern "C++" {
+template <class T>
+static int bfbt_precond(i2e_ctx *c, VelMG<T> &v, CellMG<T> &lm, const double *r, double *z) {
+  const int X = c->X, Y = c->Y;
+  SOp<double> o = c->sop();
+  Grid g = c->grid();
+  VelLevel<T> &f0 = v.L[0];
+  CellLevel<T> &l0 = lm.L[0];
+  hipLaunchKernelGGL(k_split_v<T>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, f0.fx.p, f0.fy.p);
+  ST_OK(v.vcycle(0, c->s));
+  hipLaunchKernelGGL(k_bf_in<T>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, r, (const T *)f0.ux.p,
+                     (const T *)f0.uy.p, z, l0.f.p);
+  ST_OK(lm.vcycle(0, c->s));
+  // the velocity level's residual and second iterate buffers are free here
+  hipLaunchKernelGGL(k_bf_grad<T>, nb2(X, Y), dim3(256), 0, c->s, g, c->bcix.p, c->bciy.p, lm.L[0].mask.p,
+                     (const T *)l0.u.p, f0.rx.p, f0.ry.p);
+  hipLaunchKernelGGL(k_bf_apply<T>, nb2(X, Y), dim3(256), 0, c->s, v.lev(0), c->bcix.p, c->bciy.p,
+                     (const T *)f0.rx.p, (const T *)f0.ry.p, f0.ux2.p, f0.uy2.p);
+  hipLaunchKernelGGL(k_bf_div<T>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, g, lm.L[0].mask.p, (const T *)f0.ux2.p,
+                     (const T *)f0.uy2.p, l0.f.p);
+  ST_OK(lm.vcycle(0, c->s));
+  hipLaunchKernelGGL(k_bf_out<T>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, g, o.mu, lm.L[0].mask.p,
+                     (const T *)l0.u.p, z);
+  hipLaunchKernelGGL(k_schur_bc, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, r, z);
+  return I2E_OK;
+}
+}  // extern "C++"
+
 static int stokes_precond(i2e_ctx *c, const double *r, double *z) {
   const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
+  if (c->o.schur == I2E_SCHUR_BFBT) return mixed(c) ? bfbt_precond(c, c->v32, c->l32, r, z)
+                                                     : bfbt_precond(c, c->v64, c->l64, r, z);
   if (mixed(c)) {
     hipLaunchKernelGGL(k_split_v<float>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v32.L[0].fx.p, c->v32.L[0].fy.p);
     ST_OK(c->v32.vcycle(0, c->s));

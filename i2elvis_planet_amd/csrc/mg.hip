@@ -1194,6 +1194,285 @@ void ScalMG<T>::release() {
   ainv.release();
 }
 
+// ---------------------------------------------------------------- cell MG
+template <class T>
+__global__ void k_cgalerkin(const double *Sf, const int8_t *fmask, int Xf, int Yf, int Xc, int Yc, MapView mx,
+                            MapView my, double *Sc, int8_t *cmask, int *overflow) {
+  IDX2;
+  if (i >= Xc || j >= Yc) return;
+  const int Nc = Xc * Yc, Nf = Xf * Yf, m = i * Yc + j;
+  double acc[9];
+  for (int q = 0; q < 9; q++) acc[q] = 0.0;
+  double wsum = 0, weq = 0;
+  for (int k = mx.rb[i]; k < mx.re[i]; k++) {
+    const double wx = wgt(mx, k, i);
+    if (wx == 0) continue;
+    for (int l = my.rb[j]; l < my.re[j]; l++) {
+      const double w = wx * wgt(my, l, j);
+      if (w == 0) continue;
+      wsum += w;
+      const int a = k * Yf + l;
+      if (fmask[a]) continue;
+      weq += w;
+      for (int dk = -1; dk <= 1; dk++)
+        for (int dl = -1; dl <= 1; dl++) {
+          const double sv = Sf[(size_t)((dk + 1) * 3 + dl + 1) * Nf + a];
+          if (sv == 0) continue;
+          const int kb = k + dk, lb = l + dl;
+          if (kb < 0 || kb >= Xf || lb < 0 || lb >= Yf) continue;
+          const int a0 = mx.i0[kb], b0 = my.i0[lb];
+          const double wa = mx.w[kb], wb = my.w[lb];
+          for (int ia = 0; ia < 2; ia++)
+            for (int ib = 0; ib < 2; ib++) {
+              const double pw = (ia ? wa : 1.0 - wa) * (ib ? wb : 1.0 - wb);
+              if (pw == 0) continue;
+              const int di = a0 + ia - i, dj = b0 + ib - j;
+              if (di < -1 || di > 1 || dj < -1 || dj > 1) {
+                atomicAdd(overflow, 1);
+                continue;
+              }
+              acc[(di + 1) * 3 + dj + 1] += w * sv * pw;
+            }
+        }
+    }
+  }
+  const bool id = !(weq > 0);
+  cmask[m] = id ? 1 : 0;
+  const double inv = wsum > 0 ? 1.0 / wsum : 0.0;
+  for (int q = 0; q < 9; q++) Sc[(size_t)q * Nc + m] = id ? (q == 4 ? 1.0 : 0.0) : acc[q] * inv;
+}
+
+__global__ void k_cdense(const double *S, int X, int Y, double *aug) {
+  const int n = X * Y;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double *row = aug + (size_t)r * 2 * n;
+  for (int c = 0; c < 2 * n; c++) row[c] = 0.0;
+  row[n + r] = 1.0;
+  const int i = r / Y, j = r % Y;
+  for (int di = -1; di <= 1; di++)
+    for (int dj = -1; dj <= 1; dj++) {
+      const int a = i + di, b = j + dj;
+      if (a < 0 || a >= X || b < 0 || b >= Y) continue;
+      row[a * Y + b] += S[(size_t)((di + 1) * 3 + dj + 1) * n + r];
+    }
+}
+
+template <class T>
+struct CellArgs {
+  int X, Y;
+  const T *S;
+  const int8_t *mask;
+  const T *f;
+  const T *u;  // input iterate (not read when zero)
+  T *o;        // output iterate
+  const T *e;  // correction on the next coarser level (nullptr: none)
+  int Yc;
+  MapView mx, my;
+  int zero, nsmooth, rev;
+};
+
+// tiled colour-Jacobi smoothing of a 3x3-stencil level (see the velocity
+// smoother for the tile scheme); the correction from the next coarser level
+// is added on load on the way up
+template <class T>
+__global__ __launch_bounds__(TB) void k_ctsmooth(CellArgs<T> A) {
+  __shared__ T lu[TW * TW];
+  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
+  const int X = A.X, Y = A.Y, N = X * Y;
+  for (int q = threadIdx.x; q < TW * TW; q += TB) {
+    const int i = i0 + q / TW, j = j0 + q % TW;
+    T v = 0;
+    if (i >= 0 && i < X && j >= 0 && j < Y) {
+      const int m = i * Y + j;
+      if (!A.zero) v = A.u[m];
+      if (A.e && !A.mask[m]) v += T(prolong_at(A.e, A.Yc, A.mx, A.my, i, j));
+    }
+    lu[q] = v;
+  }
+  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
+  const int la = i - i0, lb = j - j0, li = la * TW + lb;
+  const bool ok = i < X && j < Y;
+  const int m = ok ? i * Y + j : 0;
+  T sv[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) sv[q] = ok ? A.S[(size_t)q * N + m] : T(0);
+  const T f = ok ? A.f[m] : T(0), d = ok ? sv[4] : T(1);
+  __syncthreads();
+  for (int it = 0; it < A.nsmooth; it++)
+    for (int st = 0; st < 2; st++) {
+      const int col = A.rev ? 1 - st : st;
+      const bool act = ok && ((i + j) & 1) == col;
+      T nv = 0;
+      if (act) {
+        T acc = 0;
+#pragma unroll
+        for (int di = -1; di <= 1; di++)
+#pragma unroll
+          for (int dj = -1; dj <= 1; dj++) acc += sv[(di + 1) * 3 + dj + 1] * lu[(la + di) * TW + lb + dj];
+        nv = lu[li] + (f - acc) / d;
+      }
+      __syncthreads();
+      if (act) lu[li] = nv;
+      __syncthreads();
+    }
+  if (ok) A.o[m] = lu[li];
+}
+
+// r = f - S u on equation rows (0 on identity rows)
+template <class T>
+__global__ void k_cresid(int X, int Y, const T *S, const int8_t *mask, const T *u, const T *f, T *r) {
+  IDX2;
+  if (i >= X || j >= Y) return;
+  const int N = X * Y, m = i * Y + j;
+  if (mask[m]) {
+    r[m] = 0;
+    return;
+  }
+  T acc = 0;
+#pragma unroll
+  for (int di = -1; di <= 1; di++)
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++) {
+      int a = i + di, b = j + dj;
+      a = a < 0 ? 0 : (a >= X ? X - 1 : a);
+      b = b < 0 ? 0 : (b >= Y ? Y - 1 : b);
+      acc += S[(size_t)((di + 1) * 3 + dj + 1) * N + m] * u[a * Y + b];
+    }
+  r[m] = f[m] - acc;
+}
+
+template <class T>
+__global__ void k_crestrict(int Xc, int Yc, int Yf, MapView mx, MapView my, const int8_t *cmask, const T *r, T *f) {
+  IDX2;
+  if (i >= Xc || j >= Yc) return;
+  const int m = i * Yc + j;
+  f[m] = cmask[m] ? T(0) : restrict_rw(r, Yf, mx, my, i, j);
+}
+
+template <class T>
+int CellMG<T>::setup(const std::vector<double> &x0, const std::vector<double> &y0, int coarse_max) {
+  L.clear();
+  std::vector<std::vector<double>> xs{x0}, ys{y0};
+  while (true) {
+    int X = (int)xs.back().size(), Y = (int)ys.back().size();
+    if (std::min(X, Y) - 1 <= coarse_max || std::min(X, Y) < 9) break;
+    std::vector<int> ci = coarse_lines(X), cj = coarse_lines(Y);
+    std::vector<double> cx, cy;
+    for (int a : ci) cx.push_back(xs.back()[a]);
+    for (int a : cj) cy.push_back(ys.back()[a]);
+    xs.push_back(cx);
+    ys.push_back(cy);
+  }
+  const int nl = (int)xs.size();
+  L.resize(nl);
+  for (int l = 0; l < nl; l++) {
+    CellLevel<T> &v = L[l];
+    v.X = (int)xs[l].size();
+    v.Y = (int)ys[l].size();
+    v.N = v.X * v.Y;
+    ST_OK(v.S.alloc((size_t)9 * v.N));
+    ST_OK(v.mask.alloc(v.N));
+    ST_OK(v.u.alloc(v.N));
+    ST_OK(v.u2.alloc(v.N));
+    ST_OK(v.f.alloc(v.N));
+    ST_OK(v.r.alloc(v.N));
+    if (l + 1 < nl) {
+      ST_OK(v.mx.build(xs[l], xs[l + 1]));
+      ST_OK(v.my.build(ys[l], ys[l + 1]));
+    }
+  }
+  nc = L[nl - 1].N;
+  ST_OK(dense.alloc((size_t)nc * 2 * nc));
+  ST_OK(ainv.alloc((size_t)nc * nc));
+  ST_OK(overflow.alloc(1));
+  HIP_OK(hipMemset(overflow.p, 0, sizeof(int)));
+  return I2E_OK;
+}
+
+template <class T>
+int CellMG<T>::refresh(const double *S0, const int8_t *mask0, hipStream_t s) {
+  const int nl = (int)L.size();
+  if (sizeof(T) != sizeof(double)) {
+    set_error("cell multigrid: form the hierarchy in FP64 and round it with refresh_from");
+    return I2E_ERR_ARG;
+  }
+  HIP_OK(hipMemcpyAsync(L[0].S.p, S0, sizeof(double) * 9 * (size_t)L[0].N, hipMemcpyDeviceToDevice, s));
+  HIP_OK(hipMemcpyAsync(L[0].mask.p, mask0, (size_t)L[0].N, hipMemcpyDeviceToDevice, s));
+  for (int l = 0; l + 1 < nl; l++) {
+    CellLevel<T> &f = L[l], &c = L[l + 1];
+    hipLaunchKernelGGL(k_cgalerkin<T>, grd2(c.X, c.Y), blk2(), 0, s, (const double *)f.S.p, f.mask.p, f.X, f.Y, c.X,
+                       c.Y, view(f.mx), view(f.my), (double *)c.S.p, c.mask.p, overflow.p);
+  }
+  hipLaunchKernelGGL(k_cdense, dim3(nblk(nc, 64)), dim3(64), 0, s, (const double *)L[nl - 1].S.p, L[nl - 1].X,
+                     L[nl - 1].Y, dense.p);
+  hipLaunchKernelGGL(k_gauss_jordan<double>, dim3(1), dim3(1024), 0, s, nc, dense.p, ainv.p);
+  HIP_OK(hipGetLastError());
+  int nover = 0;
+  HIP_OK(hipMemcpyAsync(&nover, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  if (nover) {
+    set_error("cell multigrid: %d Galerkin couplings fall outside the 3x3 window", nover);
+    return I2E_ERR_UNSUPPORTED;
+  }
+  return I2E_OK;
+}
+
+template <class T>
+int CellMG<T>::refresh_from(const CellMG<double> &d, hipStream_t s) {
+  const int nl = (int)L.size();
+  if ((int)d.L.size() != nl || d.nc != nc) {
+    set_error("cell multigrid hierarchies differ");
+    return I2E_ERR_ARG;
+  }
+  for (int l = 0; l < nl; l++) {
+    const size_t n9 = (size_t)9 * L[l].N;
+    hipLaunchKernelGGL(k_cast<T>, dim3(nblk(n9, 256)), dim3(256), 0, s, (int)n9, d.L[l].S.p, L[l].S.p);
+    HIP_OK(hipMemcpyAsync(L[l].mask.p, d.L[l].mask.p, (size_t)L[l].N, hipMemcpyDeviceToDevice, s));
+  }
+  HIP_OK(hipMemcpyAsync(ainv.p, d.ainv.p, sizeof(double) * (size_t)nc * nc, hipMemcpyDeviceToDevice, s));
+  HIP_OK(hipGetLastError());
+  return I2E_OK;
+}
+
+template <class T>
+int CellMG<T>::vcycle(int l, hipStream_t s) {
+  const int nl = (int)L.size();
+  CellLevel<T> &v = L[l];
+  if (l == nl - 1) {
+    hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(nc, 4)), dim3(256), 0, s, nc, nc, ainv.p, v.f.p, v.f.p, v.u.p,
+                       v.u.p);
+    return I2E_OK;
+  }
+  const dim3 TG((v.Y + TS - 1) / TS, (v.X + TS - 1) / TS), TBK(TB);
+  const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
+  CellArgs<T> A{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, nullptr, 0, none, none, 1, nsmooth, 0};
+  hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, A);
+  std::swap(v.u.p, v.u2.p);
+  CellLevel<T> &c = L[l + 1];
+  hipLaunchKernelGGL(k_cresid<T>, grd2(v.X, v.Y), blk2(), 0, s, v.X, v.Y, (const T *)v.S.p,
+                     (const int8_t *)v.mask.p, (const T *)v.u.p, (const T *)v.f.p, v.r.p);
+  hipLaunchKernelGGL(k_crestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mx), view(v.my),
+                     (const int8_t *)c.mask.p, (const T *)v.r.p, c.f.p);
+  ST_OK(vcycle(l + 1, s));
+  CellArgs<T> P{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, nsmooth, 1};
+  hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, P);
+  std::swap(v.u.p, v.u2.p);
+  return I2E_OK;
+}
+
+template <class T>
+void CellMG<T>::release() {
+  for (auto &v : L) {
+    v.S.release(); v.mask.release(); v.u.release(); v.u2.release(); v.f.release(); v.r.release();
+  }
+  dense.release();
+  ainv.release();
+  overflow.release();
+}
+
+template struct CellMG<float>;
+template struct CellMG<double>;
 template struct VelMG<float>;
 template struct VelMG<double>;
 template struct ScalMG<float>;

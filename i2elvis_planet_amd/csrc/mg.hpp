@@ -163,6 +163,38 @@ struct ScalMG {
   void release();
 };
 
+// ---------------------------------------------------------------- cell MG
+// Galerkin multigrid for a scalar 5-point operator on the pressure cells
+// (the BFBt operator B C^-1 B^T of the Stokes Schur preconditioner): 3x3
+// stencils (slot-major, 9 slots), rows with mask = 1 are identity rows,
+// tiled colour-Jacobi smoothing, hierarchy formed in FP64 and rounded to T.
+template <class T>
+struct CellLevel {
+  int X, Y, N;
+  DBuf<T> S;           // 9 x N
+  DBuf<int8_t> mask;   // 1: identity row
+  DBuf<T> u, u2, f, r;
+  Map1 mx, my;         // to the next coarser level
+};
+
+template <class T>
+struct CellMG {
+  std::vector<CellLevel<T>> L;
+  int nc = 0;
+  DBuf<double> dense, ainv;
+  DBuf<int> overflow;
+  int nsmooth = 2;
+  int setup(const std::vector<double> &x0, const std::vector<double> &y0, int coarse_max);
+  // fine stencil (FP64, 9 x N0) and mask given on device; forms the
+  // Galerkin hierarchy and the coarsest inverse in FP64
+  int refresh(const double *S0, const int8_t *mask0, hipStream_t s);
+  // the same hierarchy rounded from an FP64 one
+  int refresh_from(const CellMG<double> &d, hipStream_t s);
+  // one V-cycle: u = M^-1 f on L[0].f -> L[0].u
+  int vcycle(int l, hipStream_t s);
+  void release();
+};
+
 // coarse line selection: every second line plus the last
 std::vector<int> coarse_lines(int n);
 int build_geometry(const double *gx, const double *gy, int X, int Y, int coarse_max, std::vector<GeoLevel> &geo);

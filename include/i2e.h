@@ -132,6 +132,10 @@ typedef struct i2e_host_state {
 
 /* ---- solver controls ---------------------------------------------------- */
 enum { I2E_PREC_FP64 = 0, I2E_PREC_MIXED = 1 };
+/* Pressure Schur-complement approximation of the Stokes preconditioner:
+ * BFBt (two Galerkin-multigrid solves with B diag(A)^-1 B^T around one
+ * velocity-block apply) or the viscosity-weighted pressure mass matrix.   */
+enum { I2E_SCHUR_BFBT = 0, I2E_SCHUR_MASS = 1 };
 
 typedef struct i2e_solver_opts {
   int32_t precision;       /* I2E_PREC_FP64 or I2E_PREC_MIXED                 */
@@ -141,10 +145,10 @@ typedef struct i2e_solver_opts {
   double stokes_rtol;      /* ||b-Ax||/||b|| target for Stokes               */
   double heat_rtol;        /* same for heat                                  */
   double grav_rtol;        /* same for gravity                               */
-  double schur_scale;      /* Schur diagonal = schur_scale * eta_cell / mukoef */
+  double schur_scale;      /* mass Schur: diagonal schur_scale*eta/mukoef     */
   double omega;            /* smoother damping                               */
   int32_t coarse_max;      /* coarsest grid: cells per direction <= this     */
-  int32_t pad1;
+  int32_t schur;           /* Schur preconditioner: I2E_SCHUR_BFBT or _MASS  */
 } i2e_solver_opts;
 
 typedef struct i2e_stats {
