@@ -80,6 +80,7 @@ class SolverOpts(C.Structure):
         ("stokes_rtol", C.c_double), ("heat_rtol", C.c_double), ("grav_rtol", C.c_double),
         ("schur_scale", C.c_double), ("omega", C.c_double),
         ("coarse_max", C.c_int32), ("schur", C.c_int32),
+        ("graphs", C.c_int32), ("pad1", C.c_int32),
     ]
 
 

@@ -339,6 +339,7 @@ struct i2e_ctx {
   double timestep = 0, timesum = 0, gxkoef = 1.0;
   HBuf<double> pin;
   KTimer kt;
+  DBuf<double> rstage, zstage;  // fixed input/output of the captured Stokes preconditioner
 
   Grid grid() const { return Grid{X, Y, P.xsize, P.ysize, gx.p, gy.p}; }
   SOp<double> sop() const { return SOp<double>{X, Y, gx.p, gy.p, F.nu, F.nd, F.ro, F.gp, timestep, P.nubeg * P.strmin}; }
@@ -362,6 +363,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->omega = 1.0;
   o->coarse_max = 8;
   o->schur = I2E_SCHUR_BFBT;
+  o->graphs = 1;
   return I2E_OK;
 }
 
@@ -453,7 +455,7 @@ int i2e_destroy(i2e_ctx *c) {
   }
   DBuf<double> *d[] = {&c->gx, &c->gy, &c->nodes, &c->acc, &c->work, &c->sol1, &c->tk2, &c->red, &c->xs,
                        &c->bs, &c->bondv, &c->hk, &c->hc, &c->hdm, &c->gk, &c->gc, &c->gdm, &c->gsc, &c->hsc,
-                       &c->bcix, &c->bciy, &c->bLS};
+                       &c->bcix, &c->bciy, &c->bLS, &c->rstage, &c->zstage};
   c->blmask.release();
   for (auto *b : d) b->release();
   c->mfl.release();
@@ -622,6 +624,8 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
     ST_OK(c->bciy.alloc(N));
     ST_OK(c->bLS.alloc(9 * Nc));
     ST_OK(c->blmask.alloc(Nc));
+    ST_OK(c->rstage.alloc((size_t)3 * N));
+    ST_OK(c->zstage.alloc((size_t)3 * N));
   }
   ST_OK(c->fs.init(3 * N, c->o.restart));
   ST_OK(c->fh.init(N, c->o.restart));
@@ -878,10 +882,43 @@ static int stokes_solve(i2e_ctx *c) {
     c->kt.end(K_STOKES_APPLY, c->s, bytes);
     return I2E_OK;
   };
-  auto Mf = [&](const double *r, double *z) -> int { return stokes_precond(c, r, z); };
+  // The preconditioner (~100 small launches) is captured once per solve
+  // (its kernel arguments carry this step's dt) and replayed per iteration
+  // between two copies through fixed staging vectors.  No memset is captured.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const size_t n3 = (size_t)3 * c->N;
+  if (c->o.graphs) {
+    KTimer *k64 = c->v64.kt, *k32 = c->v32.kt;
+    c->v64.kt = c->v32.kt = nullptr;  // per-launch events are not captured
+    HIP_OK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    const int sp = stokes_precond(c, c->rstage.p, c->zstage.p);
+    const hipError_t ec = hipStreamEndCapture(c->s, &graph);
+    c->v64.kt = k64;
+    c->v32.kt = k32;
+    if (sp) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return sp;
+    }
+    HIP_OK(ec);
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      HIP_OK(ei);
+    }
+  }
+  auto Mf = [&](const double *r, double *z) -> int {
+    if (!exec) return stokes_precond(c, r, z);
+    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, (int)n3, r, c->rstage.p);
+    HIP_OK(hipGraphLaunch(exec, c->s));
+    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, (int)n3, (const double *)c->zstage.p, z);
+    return I2E_OK;
+  };
   int it = 0;
   double rr = 0;
   int st = c->fs.solve(A, Mf, c->bs.p, c->xs.p, c->o.stokes_rtol, c->o.maxit, c->s, &it, &rr);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   c->st.stokes_iters = it;
   c->st.stokes_relres = rr;
   if (st) {

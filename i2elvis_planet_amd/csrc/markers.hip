@@ -113,8 +113,8 @@ __device__ __forceinline__ void allinterv_d(const Grid &g, const double *vx, con
 }
 
 // ---------------------------------------------------------------- movemark
-__global__ void k_movemark(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk, int64_t M, double timestep,
-                           int need_eii) {
+__global__ __launch_bounds__(256) void k_movemark(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
+                                                 int64_t M, double timestep, int need_eii) {
   for (int64_t mm1 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; mm1 < M; mm1 += (int64_t)gridDim.x * blockDim.x) {
     int t = mk.t[mm1];
     int mm2 = t >= 100 ? t - 100 : t;
@@ -202,134 +202,210 @@ __device__ void porosity_update_d(const i2e_params *p, float &por, float markk, 
 
 __device__ __forceinline__ void aadd(double *a, double v) { unsafeAtomicAdd(a, v); }
 
-__global__ void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk, int64_t M, Acc A,
-                               double timesum, double timestep, int need_eii) {
-  for (int64_t mm1 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; mm1 < M; mm1 += (int64_t)gridDim.x * blockDim.x) {
-    int t = mk.t[mm1];
-    int mm2 = t >= 100 ? t - 100 : t;
-    float fx = mk.x[mm1], fy = mk.y[mm1], fk = mk.k[mm1];
-    if (!(fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize && fk > 0 && t < 50)) continue;
-    const double x = (double)fx, y = (double)fy;
-    Ev e;
-    allintere_d(g, F, x, y, e, need_eii != 0);
-    double mpb = e.pr * 1.0000e-5, mtk = (double)fk;
-    if (mm2 < 2) {
-      fk = (float)p->tmp_ambient;
-      mk.k[mm1] = fk;
-      mtk = (double)fk;
-    }
-    double mnu = mk.v[mm1];
-    if (mnu <= 0) {
-      double eii = need_eii ? pow(0.5 * (e.exx * e.exx + e.eyy * e.eyy) + e.exy * e.exy, 0.5) : 0.0;
-      mnu = viscalc_d(p, mtk, mpb, t < 20 ? mm2 : mm2 - 20, eii);
-    }
-    double celdx = fx - g.xsize / 2.0, celdy = fy - g.ysize / 2.0;
-    celdy = pow(celdx * celdx + celdy * celdy, 0.5);
-    if (celdy > g.xsize / 2.0 * p->radmax_frac) mnu = p->eta_outer;
-    float por = mk.por[mm1];
-    if (mm2 == 0 || mm2 >= 7) por = 0.000f;
-    if (t != 0)
-      if (mk.tmax[mm1] < fk) mk.tmax[mm1] = fk;
-    if (p->porosity_on && p->porous[mm2] && (por > 0.000f && por <= 0.850f))
-      porosity_update_d(p, por, fk, mpb, mtk, timestep);
-    if (por < 0.000f) por = 0.000f;
-    if (por > 1.000f) por = 1.000f;
-    mk.por[mm1] = por;
-    const i2e_rock &r = p->rock[mm2];
-    double mro, mbb = 0;
-    if (p->densimod == 0)
-      mro = r.ro;
-    else {
-      mbb = r.bb / (1.0 - r.bb * (mtk - 298.15));
-      mro = r.ro * (1.0 - r.bb * (mtk - 298.15)) * (1.0 + r.aa * (mpb - 1.0) * 1e-3) * (1.000 - (double)por);
-    }
-    double mcp = r.cp;
-    double mkt = (r.kt + r.kf / (mtk + 77.0)) * exp(r.kp * mpb);
-    if (p->melt_on && p->meltable[mm2 < 20 ? mm2 : mm2 - 20]) {
-      double xm = (mtk - p->melt_ts) / (p->melt_tl - p->melt_ts);
-      if (xm < 0.0) xm = 0.0;
-      if (xm > 1.0) xm = 1.0;
-      int tnew = t;
-      if (xm > 0 && tnew < 20) tnew += 20;
-      if (xm <= 0 && tnew > 20) tnew -= 20;
-      mk.t[mm1] = (int8_t)tnew;
-      if (tnew > 20) {
-        const i2e_rock &rm = p->rock[tnew];
-        const i2e_rock &rs = p->rock[tnew - 20];
-        mro = mro * (1.0 - xm) + xm * rm.ro * (1.0 - rm.bb * (mtk - 298.15)) * (1.0 + rm.aa * (mpb - 1.0) * 1e-3);
-        if (xm > p->melt_switch) {
-          double nueff = p->melt_eta;
-          if (nueff < p->nubeg) nueff = p->nubeg;
-          if (nueff > p->nuend) nueff = p->nuend;
-          mnu = nueff;
-        }
-        mkt = ((rm.kt + rm.kf / (mtk + 77.0)) * exp(rm.kp * mpb)) * xm +
-              ((rs.kt + rs.kf / (mtk + 77.0)) * exp(rs.kp * mpb)) * (1.0 - xm);
-        if (xm > 0 && xm < 1.0) {
-          double xp = (mtk + 1.0 - p->melt_ts) / (p->melt_tl - p->melt_ts);
-          double xq = (mtk - 1.0 - p->melt_ts) / (p->melt_tl - p->melt_ts);
-          xp = xp < 0.0 ? 0.0 : (xp > 1.0 ? 1.0 : xp);
-          xq = xq < 0.0 ? 0.0 : (xq > 1.0 ? 1.0 : xq);
-          mcp += (p->melt_hlat * xp - p->melt_hlat * xq) / (2.0 * 1.0);
-        }
-      }
-    }
-    if (p->porous[mm2]) {
-      double kt0 = r.kt, pr0 = (double)por;
-      if (por < 0.200f) mkt = kt0 * exp(-pr0 / 0.080);
-      if (por >= 0.200f && por <= 0.400f)
-        mkt = pow((pow((kt0 * exp(-pr0 / 0.080)), 4.000) + pow((kt0 * exp(-1.200 - (pr0 / 0.167))), 4.000)), 0.250);
-      if (por > 0.400f) mkt = kt0 * exp(-1.200 - (pr0 / 0.167));
-    }
-    if (mkt <= 0.0010) mkt = 0.0010;
-    if (mkt >= p->k_cutoff) mkt = p->k_cutoff;
-    double mht = 0;
-    double timesumA = timesum / (365.250 * 24.000 * 3600.000);
-    if (p->is_silicate[mm2]) {
-      double init_al = 1.89716e+23 * p->al2627_init * 4.999e-13 / (3.264e+13);
-      double ll = 0;
-      if (p->longlived != 0)
-        ll = 0.40689 * 1.430e-11 * exp(-timesumA * 4.9867e-10) + 2.990e-12 * exp(-timesumA * 9.8458e-10) +
-             1.600e-12 * exp(-timesumA * 1.5541e-10) + 1.000e-12 * exp(-timesumA * 4.9511e-11);
-      mht = (init_al * exp(-timesumA * 9.6673e-7) * p->h0_scale + ll) * mro;
-    } else if (p->is_iron[mm2]) {
-      double init_fe = 1.971e+24 * p->fe6056_init * 4.3451e-13 / (1.183e+14), ka = 0;
-      if (p->longlived != 0) ka = 0.090625 * 1.430e-11 * exp(-timesumA * 4.9867e-10);
-      mht = (init_fe * exp(-timesumA * 2.6660e-7) + ka) * mro;
-    }
-    // scatter (markmart.c:840-911)
-    W4 w;
-    nodewt2(g.gx, g.gy, e.m10, e.m20, x, y, 0, 0, w);
-    {
-      double ddx = 1.0 - dabs(w.wx0 - 0.5), ddy = 1.0 - dabs(w.wy0 - 0.5);
-      if (ddx > 0.5 && ddy > 0.5) {
-        double swt2 = 1.0 * 4.0 * (ddx - 0.5) * (ddy - 0.5);
-        int m3 = (e.m10 + 1) * g.Y + e.m20 + 1;
-        aadd(&A.nd0[m3], mnu * swt2);
-        aadd(&A.wnd[m3], swt2);
-      }
-    }
+// Scatter contributions of one marker: per corner (a,b) of its cell the ten
+// weighted sums of markmart.c:840-911, plus the cell-centre weights of nd.
+struct RonuC {
+  double v[4][10];  // ro, et, cp, kt, ht, w0, tk, w1, nu*swt2, swt2 (wnu)
+  double nd, wnd;
+};
+
+// segmented sum over the lanes of a wavefront whose segment ids are equal
+// and contiguous: afterwards the first lane of each segment holds its total
+__device__ __forceinline__ double seg_sum(double v, int seg, int lane) {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+  for (int off = 1; off < 64; off <<= 1) {
+    const double o = __shfl_down(v, off, 64);
+    const int so = __shfl_down(seg, off, 64);
+    if (lane + off < 64 && so == seg) v += o;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
+                                                     int64_t M, Acc A, double timesum, double timestep,
+                                                     int need_eii) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  // whole wavefronts iterate together: the scatter reduces across lanes
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < M; base += stride) {
+    const int64_t mm1 = base + threadIdx.x;
+    bool act = mm1 < M;
+    RonuC C;
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
-        int m3 = (e.m10 + a) * g.Y + e.m20 + b;
-        double ddx = a ? w.wx1 : w.wx0, ddy = b ? w.wy1 : w.wy0;
-        double swt = 1.0 * ddx * ddy;
-        if (ddx > 0.5 && ddy > 0.5) {
-          double swt2 = 1.0 * 4.0 * (ddx - 0.5) * (ddy - 0.5);
-          aadd(&A.nu0[m3], mnu * swt2);
-          aadd(&A.wnu[m3], swt2);
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int q = 0; q < 10; q++) C.v[c][q] = 0.0;
+    C.nd = C.wnd = 0.0;
+    int key = -1;
+    if (act) {
+      int t = mk.t[mm1];
+      int mm2 = t >= 100 ? t - 100 : t;
+      float fx = mk.x[mm1], fy = mk.y[mm1], fk = mk.k[mm1];
+      if (!(fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize && fk > 0 && t < 50)) act = false;
+      if (act) {
+        const double x = (double)fx, y = (double)fy;
+        Ev e;
+        allintere_d(g, F, x, y, e, need_eii != 0);
+        double mpb = e.pr * 1.0000e-5, mtk = (double)fk;
+        if (mm2 < 2) {
+          fk = (float)p->tmp_ambient;
+          mk.k[mm1] = fk;
+          mtk = (double)fk;
         }
-        aadd(&A.ro0[m3], mro * swt);
-        aadd(&A.et0[m3], mbb * swt);
-        aadd(&A.cp0[m3], mcp * swt);
-        aadd(&A.kt0[m3], mkt * swt);
-        aadd(&A.ht0[m3], mht * swt);
-        aadd(&A.w0[m3], swt);
-        aadd(&A.tk0[m3], mtk * swt);
-        aadd(&A.w1[m3], swt);
+        double mnu = mk.v[mm1];
+        if (mnu <= 0) {
+          double eii = need_eii ? pow(0.5 * (e.exx * e.exx + e.eyy * e.eyy) + e.exy * e.exy, 0.5) : 0.0;
+          mnu = viscalc_d(p, mtk, mpb, t < 20 ? mm2 : mm2 - 20, eii);
+        }
+        double celdx = fx - g.xsize / 2.0, celdy = fy - g.ysize / 2.0;
+        celdy = pow(celdx * celdx + celdy * celdy, 0.5);
+        if (celdy > g.xsize / 2.0 * p->radmax_frac) mnu = p->eta_outer;
+        float por = mk.por[mm1];
+        if (mm2 == 0 || mm2 >= 7) por = 0.000f;
+        if (t != 0)
+          if (mk.tmax[mm1] < fk) mk.tmax[mm1] = fk;
+        if (p->porosity_on && p->porous[mm2] && (por > 0.000f && por <= 0.850f))
+          porosity_update_d(p, por, fk, mpb, mtk, timestep);
+        if (por < 0.000f) por = 0.000f;
+        if (por > 1.000f) por = 1.000f;
+        mk.por[mm1] = por;
+        const i2e_rock &r = p->rock[mm2];
+        double mro, mbb = 0;
+        if (p->densimod == 0)
+          mro = r.ro;
+        else {
+          mbb = r.bb / (1.0 - r.bb * (mtk - 298.15));
+          mro = r.ro * (1.0 - r.bb * (mtk - 298.15)) * (1.0 + r.aa * (mpb - 1.0) * 1e-3) * (1.000 - (double)por);
+        }
+        double mcp = r.cp;
+        double mkt = (r.kt + r.kf / (mtk + 77.0)) * exp(r.kp * mpb);
+        if (p->melt_on && p->meltable[mm2 < 20 ? mm2 : mm2 - 20]) {
+          double xm = (mtk - p->melt_ts) / (p->melt_tl - p->melt_ts);
+          if (xm < 0.0) xm = 0.0;
+          if (xm > 1.0) xm = 1.0;
+          int tnew = t;
+          if (xm > 0 && tnew < 20) tnew += 20;
+          if (xm <= 0 && tnew > 20) tnew -= 20;
+          mk.t[mm1] = (int8_t)tnew;
+          if (tnew > 20) {
+            const i2e_rock &rm = p->rock[tnew];
+            const i2e_rock &rs = p->rock[tnew - 20];
+            mro = mro * (1.0 - xm) + xm * rm.ro * (1.0 - rm.bb * (mtk - 298.15)) * (1.0 + rm.aa * (mpb - 1.0) * 1e-3);
+            if (xm > p->melt_switch) {
+              double nueff = p->melt_eta;
+              if (nueff < p->nubeg) nueff = p->nubeg;
+              if (nueff > p->nuend) nueff = p->nuend;
+              mnu = nueff;
+            }
+            mkt = ((rm.kt + rm.kf / (mtk + 77.0)) * exp(rm.kp * mpb)) * xm +
+                  ((rs.kt + rs.kf / (mtk + 77.0)) * exp(rs.kp * mpb)) * (1.0 - xm);
+            if (xm > 0 && xm < 1.0) {
+              double xp = (mtk + 1.0 - p->melt_ts) / (p->melt_tl - p->melt_ts);
+              double xq = (mtk - 1.0 - p->melt_ts) / (p->melt_tl - p->melt_ts);
+              xp = xp < 0.0 ? 0.0 : (xp > 1.0 ? 1.0 : xp);
+              xq = xq < 0.0 ? 0.0 : (xq > 1.0 ? 1.0 : xq);
+              mcp += (p->melt_hlat * xp - p->melt_hlat * xq) / (2.0 * 1.0);
+            }
+          }
+        }
+        if (p->porous[mm2]) {
+          double kt0 = r.kt, pr0 = (double)por;
+          if (por < 0.200f) mkt = kt0 * exp(-pr0 / 0.080);
+          if (por >= 0.200f && por <= 0.400f)
+            mkt = pow((pow((kt0 * exp(-pr0 / 0.080)), 4.000) + pow((kt0 * exp(-1.200 - (pr0 / 0.167))), 4.000)), 0.250);
+          if (por > 0.400f) mkt = kt0 * exp(-1.200 - (pr0 / 0.167));
+        }
+        if (mkt <= 0.0010) mkt = 0.0010;
+        if (mkt >= p->k_cutoff) mkt = p->k_cutoff;
+        double mht = 0;
+        double timesumA = timesum / (365.250 * 24.000 * 3600.000);
+        if (p->is_silicate[mm2]) {
+          double init_al = 1.89716e+23 * p->al2627_init * 4.999e-13 / (3.264e+13);
+          double ll = 0;
+          if (p->longlived != 0)
+            ll = 0.40689 * 1.430e-11 * exp(-timesumA * 4.9867e-10) + 2.990e-12 * exp(-timesumA * 9.8458e-10) +
+                 1.600e-12 * exp(-timesumA * 1.5541e-10) + 1.000e-12 * exp(-timesumA * 4.9511e-11);
+          mht = (init_al * exp(-timesumA * 9.6673e-7) * p->h0_scale + ll) * mro;
+        } else if (p->is_iron[mm2]) {
+          double init_fe = 1.971e+24 * p->fe6056_init * 4.3451e-13 / (1.183e+14), ka = 0;
+          if (p->longlived != 0) ka = 0.090625 * 1.430e-11 * exp(-timesumA * 4.9867e-10);
+          mht = (init_fe * exp(-timesumA * 2.6660e-7) + ka) * mro;
+        }
+        // scatter weights (markmart.c:840-911)
+        W4 w;
+        nodewt2(g.gx, g.gy, e.m10, e.m20, x, y, 0, 0, w);
+        key = e.m10 * g.Y + e.m20;
+        {
+          double ddx = 1.0 - dabs(w.wx0 - 0.5), ddy = 1.0 - dabs(w.wy0 - 0.5);
+          if (ddx > 0.5 && ddy > 0.5) {
+            double swt2 = 1.0 * 4.0 * (ddx - 0.5) * (ddy - 0.5);
+            C.nd = mnu * swt2;
+            C.wnd = swt2;
+          }
+        }
+    #pragma unroll
+        for (int a = 0; a < 2; a++)
+    #pragma unroll
+          for (int b = 0; b < 2; b++) {
+            double *v = C.v[a * 2 + b];
+            double ddx = a ? w.wx1 : w.wx0, ddy = b ? w.wy1 : w.wy0;
+            double swt = 1.0 * ddx * ddy;
+            if (ddx > 0.5 && ddy > 0.5) {
+              double swt2 = 1.0 * 4.0 * (ddx - 0.5) * (ddy - 0.5);
+              v[8] = mnu * swt2;
+              v[9] = swt2;
+            }
+            v[0] = mro * swt;
+            v[1] = mbb * swt;
+            v[2] = mcp * swt;
+            v[3] = mkt * swt;
+            v[4] = mht * swt;
+            v[5] = swt;
+            v[6] = mtk * swt;
+            v[7] = swt;
+          }
+
       }
+    }
+    // one atomic per run of lanes in the same cell (lanes are cell-sorted in
+    // the common case; any order stays correct, just with more runs)
+    const int kprev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || kprev != key;
+    const unsigned long long heads = __ballot(head);
+    const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+      for (int q = 0; q < 10; q++) C.v[c][q] = seg_sum(C.v[c][q], seg, lane);
+    C.nd = seg_sum(C.nd, seg, lane);
+    C.wnd = seg_sum(C.wnd, seg, lane);
+    if (head && key >= 0) {
+      const int m10 = key / g.Y, m20 = key % g.Y;
+      if (C.wnd != 0) {
+        const int m3 = (m10 + 1) * g.Y + m20 + 1;
+        aadd(&A.nd0[m3], C.nd);
+        aadd(&A.wnd[m3], C.wnd);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          const int m3 = (m10 + a) * g.Y + m20 + b;
+          const double *v = C.v[a * 2 + b];
+          if (v[9] != 0) {
+            aadd(&A.nu0[m3], v[8]);
+            aadd(&A.wnu[m3], v[9]);
+          }
+          aadd(&A.ro0[m3], v[0]);
+          aadd(&A.et0[m3], v[1]);
+          aadd(&A.cp0[m3], v[2]);
+          aadd(&A.kt0[m3], v[3]);
+          aadd(&A.ht0[m3], v[4]);
+          aadd(&A.w0[m3], v[5]);
+          aadd(&A.tk0[m3], v[6]);
+          aadd(&A.w1[m3], v[7]);
+        }
+    }
   }
 }
 
@@ -420,38 +496,70 @@ __global__ void k_tbc(NodeF F, BcTab bc, int N) {
 
 // ---------------------------------------------------------------- marker T
 // heatmart.c:529-627: subgrid diffusion + scatter of the diffusion change
-__global__ void k_heat_markers_a(const i2e_params *__restrict__ p, Grid g, NodeF F, const double *__restrict__ tk2,
-                                 Markers mk, int64_t M, double timestep, double *sol0, double *sol1) {
-  for (int64_t m3 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m3 < M; m3 += (int64_t)gridDim.x * blockDim.x) {
-    float fx = mk.x[m3], fy = mk.y[m3];
-    if (!(fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize && mk.t[m3] < 50)) continue;
-    double x = fx, y = fy;
-    int m10 = serch(g.gx, g.X, x), m20 = serch(g.gy, g.Y, y);
-    W4 w;
-    nodewt2(g.gx, g.gy, m10, m20, x, y, 0, 0, w);
-    double e2 = isum(w, g.Y, F.tk), e3 = isum(w, g.Y, tk2);
-    float fk = mk.k[m3];
-    if (fk <= 0) fk = (float)e3;
-    if (p->heatdif) {
-      double dtk = e3 - (double)fk;
-      double celdx = g.gx[m10 + 1] - g.gx[m10], celdy = g.gy[m20 + 1] - g.gy[m20];
-      double mkt = isum(w, g.Y, F.kt), mcp = isum(w, g.Y, F.cp), mro = isum(w, g.Y, F.ro);
-      double dtk1 = -p->heatdif * mkt / mro / mcp * 2.0 * (1.0 / celdx / celdx + 1.0 / celdy / celdy) * timestep;
-      if (dtk1 < -150.0) dtk1 = -150.0;
-      dtk1 = dtk * (1.0 - exp(dtk1));
-      fk += (float)dtk1;
+__global__ __launch_bounds__(256) void k_heat_markers_a(const i2e_params *__restrict__ p, Grid g, NodeF F,
+                                                       const double *__restrict__ tk2, Markers mk, int64_t M,
+                                                       double timestep, double *sol0, double *sol1) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  // whole wavefronts iterate together: the scatter reduces across lanes
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < M; base += stride) {
+    const int64_t m3 = base + threadIdx.x;
+    double s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
+    int key = -1;
+    if (m3 < M) {
+      float fx = mk.x[m3], fy = mk.y[m3];
+      if (fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize && mk.t[m3] < 50) {
+        double x = fx, y = fy;
+        int m10 = serch(g.gx, g.X, x), m20 = serch(g.gy, g.Y, y);
+        W4 w;
+        nodewt2(g.gx, g.gy, m10, m20, x, y, 0, 0, w);
+        double e2 = isum(w, g.Y, F.tk), e3 = isum(w, g.Y, tk2);
+        float fk = mk.k[m3];
+        if (fk <= 0) fk = (float)e3;
+        if (p->heatdif) {
+          double dtk = e3 - (double)fk;
+          double celdx = g.gx[m10 + 1] - g.gx[m10], celdy = g.gy[m20 + 1] - g.gy[m20];
+          double mkt = isum(w, g.Y, F.kt), mcp = isum(w, g.Y, F.cp), mro = isum(w, g.Y, F.ro);
+          double dtk1 = -p->heatdif * mkt / mro / mcp * 2.0 * (1.0 / celdx / celdx + 1.0 / celdy / celdy) * timestep;
+          if (dtk1 < -150.0) dtk1 = -150.0;
+          dtk1 = dtk * (1.0 - exp(dtk1));
+          fk += (float)dtk1;
+          key = m10 * g.Y + m20;
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+              double ival = (a ? w.wx1 : w.wx0) * (b ? w.wy1 : w.wy0) * 1.0;
+              s0[a * 2 + b] = ival;
+              s1[a * 2 + b] = dtk1 * ival;
+            }
+        }
+        fk += (float)(e2 - e3);
+        mk.k[m3] = fk;
+      }
+    }
+    if (!p->heatdif) continue;
+    // one atomic per run of lanes in the same cell
+    const int kprev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || kprev != key;
+    const unsigned long long heads = __ballot(head);
+    const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      s0[c] = seg_sum(s0[c], seg, lane);
+      s1[c] = seg_sum(s1[c], seg, lane);
+    }
+    if (head && key >= 0) {
+      const int m10 = key / g.Y, m20 = key % g.Y;
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
-          int m4 = (m10 + a) * g.Y + m20 + b;
-          double ival = (a ? w.wx1 : w.wx0) * (b ? w.wy1 : w.wy0) * 1.0;
-          aadd(&sol0[m4], ival);
-          aadd(&sol1[m4], dtk1 * ival);
+          const int m4 = (m10 + a) * g.Y + m20 + b;
+          aadd(&sol0[m4], s0[a * 2 + b]);
+          aadd(&sol1[m4], s1[a * 2 + b]);
         }
     }
-    fk += (float)(e2 - e3);
-    mk.k[m3] = fk;
   }
 }
 

@@ -273,7 +273,64 @@ __global__ void k_vdense(VLev<T> L, int n, double *aug) {
   aug[(size_t)r * 2 * n + n + c] = (r == c) ? 1.0 : 0.0;
 }
 
-// Gauss-Jordan inversion with partial pivoting in one workgroup
+// Gauss-Jordan inversion with partial pivoting in one workgroup, on the
+// n x 2n augmented matrix held in LDS (n <= GJ_LDS_N)
+constexpr int GJ_LDS_N = 90;
+__global__ __launch_bounds__(1024) void k_gauss_jordan_lds(int n, const double *ag, double *inv) {
+  __shared__ double a[GJ_LDS_N * 2 * GJ_LDS_N];
+  __shared__ double sv[1024];
+  __shared__ int si[1024];
+  const int t = threadIdx.x, nt = blockDim.x, w = 2 * n;
+  for (int idx = t; idx < n * w; idx += nt) a[idx] = ag[idx];
+  __syncthreads();
+  for (int p = 0; p < n; p++) {
+    double best = -1;
+    int bi = p;
+    for (int r = p + t; r < n; r += nt) {
+      const double v = fabs(a[r * w + p]);
+      if (v > best) {
+        best = v;
+        bi = r;
+      }
+    }
+    sv[t] = best;
+    si[t] = bi;
+    __syncthreads();
+    for (int o = nt / 2; o > 0; o >>= 1) {
+      if (t < o && sv[t + o] > sv[t]) {
+        sv[t] = sv[t + o];
+        si[t] = si[t + o];
+      }
+      __syncthreads();
+    }
+    const int q = si[0];
+    if (q != p)
+      for (int c = t; c < w; c += nt) {
+        const double x = a[p * w + c];
+        a[p * w + c] = a[q * w + c];
+        a[q * w + c] = x;
+      }
+    __syncthreads();
+    const double d = a[p * w + p];
+    __syncthreads();
+    for (int c = t; c < w; c += nt) a[p * w + c] /= d;
+    __syncthreads();
+    for (int idx = t; idx < n * w; idx += nt) {
+      const int r = idx / w, c = idx % w;
+      if (r == p || c == p) continue;
+      a[r * w + c] -= a[r * w + p] * a[p * w + c];
+    }
+    __syncthreads();
+    for (int r = t; r < n; r += nt)
+      if (r != p) a[r * w + p] = 0.0;
+    __syncthreads();
+  }
+  for (int idx = t; idx < n * n; idx += nt) inv[idx] = a[(idx / n) * w + n + idx % n];
+}
+
+static void gauss_jordan(int n, double *aug, double *inv, hipStream_t s);
+
+// Gauss-Jordan inversion with partial pivoting in one workgroup (global memory)
 template <class T>
 __global__ void k_gauss_jordan(int n, double *a, T *inv) {
   __shared__ double sv[1024];
@@ -326,6 +383,13 @@ __global__ void k_gauss_jordan(int n, double *a, T *inv) {
     __syncthreads();
   }
   for (int idx = t; idx < n * n; idx += nt) inv[idx] = T(a[(size_t)(idx / n) * w + n + idx % n]);
+}
+
+static void gauss_jordan(int n, double *aug, double *inv, hipStream_t s) {
+  if (n <= GJ_LDS_N)
+    hipLaunchKernelGGL(k_gauss_jordan_lds, dim3(1), dim3(1024), 0, s, n, (const double *)aug, inv);
+  else
+    hipLaunchKernelGGL(k_gauss_jordan<double>, dim3(1), dim3(1024), 0, s, n, aug, inv);
 }
 
 // u = Ainv f with f = [f0 (n0 entries), f1]
@@ -875,7 +939,7 @@ int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, cons
                        dense.p);
   else
     hipLaunchKernelGGL(k_vdense<T>, dim3(nblk(n, 64), n), dim3(64), 0, s, lev(0), n, dense.p);
-  hipLaunchKernelGGL(k_gauss_jordan<double>, dim3(1), dim3(1024), 0, s, n, dense.p, ainv.p);
+  gauss_jordan(n, dense.p, ainv.p, s);
   HIP_OK(hipGetLastError());
   int nover = 0;
   HIP_OK(hipMemcpyAsync(&nover, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1152,7 +1216,7 @@ int ScalMG<T>::refresh(const double *k, const double *c, const int8_t *mask, con
   }
   for (int l = 0; l < nl; l++) hipLaunchKernelGGL(k_pdiag<T>, grd2(L[l].X, L[l].Y), blk2(), 0, s, op(l), L[l].diag.p);
   hipLaunchKernelGGL(k_pdense<T>, dim3(nblk(nc, 256), nc), dim3(256), 0, s, op(nl - 1), nc, dense.p);
-  hipLaunchKernelGGL(k_gauss_jordan<double>, dim3(1), dim3(1024), 0, s, nc, dense.p, ainv.p);
+  gauss_jordan(nc, dense.p, ainv.p, s);
   HIP_OK(hipGetLastError());
   return I2E_OK;
 }
@@ -1406,7 +1470,7 @@ int CellMG<T>::refresh(const double *S0, const int8_t *mask0, hipStream_t s) {
   }
   hipLaunchKernelGGL(k_cdense, dim3(nblk(nc, 64)), dim3(64), 0, s, (const double *)L[nl - 1].S.p, L[nl - 1].X,
                      L[nl - 1].Y, dense.p);
-  hipLaunchKernelGGL(k_gauss_jordan<double>, dim3(1), dim3(1024), 0, s, nc, dense.p, ainv.p);
+  gauss_jordan(nc, dense.p, ainv.p, s);
   HIP_OK(hipGetLastError());
   int nover = 0;
   HIP_OK(hipMemcpyAsync(&nover, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));

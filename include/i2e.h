@@ -149,6 +149,8 @@ typedef struct i2e_solver_opts {
   double omega;            /* smoother damping                               */
   int32_t coarse_max;      /* coarsest grid: cells per direction <= this     */
   int32_t schur;           /* Schur preconditioner: I2E_SCHUR_BFBT or _MASS  */
+  int32_t graphs;          /* 1: replay the Stokes preconditioner as a HIP graph */
+  int32_t pad1;
 } i2e_solver_opts;
 
 typedef struct i2e_stats {

@@ -253,3 +253,33 @@ def test_mixed_precision_error_vs_fp64(orc):
         e = rel(gb.__dict__[f], ga.__dict__[f])
         say("mixed vs fp64", f, e)
         assert e < 1e-5, f
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_replay_matches_eager_precond(orc, precision):
+    """The Stokes preconditioner replayed as a captured HIP graph (once per
+    FGMRES iteration, with eager kernels and host synchronisations between
+    the replays) gives bitwise the same solve as launching it eagerly."""
+    say("case graph replay", precision)
+    p, hs, _ = configs.make("c2")
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    out = []
+    for graphs in (0, 1):
+        opts = default_solver()
+        opts.precision = precision
+        opts.graphs = graphs
+        ctx = Context(p, hs.copy(), solver=opts)
+        ctx.set_scalars(hs.timestep, hs.timesum)
+        ctx.viterate()
+        st = ctx.stats()
+        got = ctx.download(hs.copy())
+        ctx.close()
+        out.append((st.stokes_iters, got))
+    (i0, a), (i1, b) = out
+    say("iterations eager", i0, "graph", i1)
+    assert i0 == i1 and i0 > 10
+    for f in ("vx", "vy", "pr"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
