@@ -37,7 +37,7 @@ void set_error(const char *fmt, ...);
 // Per-launch HIP-event timing of one kernel class on its launch stream, for
 // roofline figures measured inside a normal run.  Host-side only: launch
 // sites bracket the launch with begin/end and state its algorithmic bytes.
-enum KId { K_ADVECT = 0, K_P2G, K_GSDOT, K_GSAXPY, K_VSMOOTH0, K_VSMOOTHC, K_STOKES_APPLY, K_COUNT };
+enum KId { K_ADVECT = 0, K_P2G, K_GSDOT, K_GSAXPY, K_VSMOOTH0, K_VSMOOTHC, K_STOKES_APPLY, K_PRECOND, K_COUNT };
 
 struct KTimer {
   int target = -1;

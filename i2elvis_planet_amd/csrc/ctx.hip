@@ -363,7 +363,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->omega = 1.0;
   o->coarse_max = 8;
   o->schur = I2E_SCHUR_BFBT;
-  o->graphs = 1;
+  o->graphs = 0;
   return I2E_OK;
 }
 
@@ -908,10 +908,15 @@ static int stokes_solve(i2e_ctx *c) {
     }
   }
   auto Mf = [&](const double *r, double *z) -> int {
-    if (!exec) return stokes_precond(c, r, z);
-    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, (int)n3, r, c->rstage.p);
-    HIP_OK(hipGraphLaunch(exec, c->s));
-    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, (int)n3, (const double *)c->zstage.p, z);
+    c->kt.begin(K_PRECOND, c->s);
+    if (!exec) {
+      ST_OK(stokes_precond(c, r, z));
+    } else {
+      hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, (int)n3, r, c->rstage.p);
+      HIP_OK(hipGraphLaunch(exec, c->s));
+      hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, (int)n3, (const double *)c->zstage.p, z);
+    }
+    c->kt.end(K_PRECOND, c->s, 0.0);
     return I2E_OK;
   };
   int it = 0;
@@ -1187,7 +1192,8 @@ int i2e_copy_to_host(void *dst_host, const void *src_dev, size_t bytes) {
 
 int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches) {
   if (!c || max_launches < 0) return I2E_ERR_ARG;
-  static const char *names[K_COUNT] = {"advect", "p2g", "gsdot", "gsaxpy", "vsmooth0", "vsmoothc", "stokes_apply"};
+  static const char *names[K_COUNT] = {"advect", "p2g",          "gsdot",  "gsaxpy",
+                                       "vsmooth0", "vsmoothc", "stokes_apply", "precond"};
   int id = -1;
   if (kernel && kernel[0]) {
     for (int k = 0; k < K_COUNT; k++)

@@ -1414,6 +1414,170 @@ __global__ void k_crestrict(int Xc, int Yc, int Yf, MapView mx, MapView my, cons
   f[m] = cmask[m] ? T(0) : restrict_rw(r, Yf, mx, my, i, j);
 }
 
+// The small levels of the cell multigrid (from the first with N <= CT_N
+// down to the coarsest) run as one workgroup: iterates, right-hand sides and
+// residuals in LDS, each thread's stencil rows in registers, phases
+// separated by workgroup barriers.  Replaces ~4 launches per level.
+constexpr int CT_N = 1156, CT_THR = 1024, CT_L = 5, CT_TOT = 1600, CT_K = (CT_N + CT_THR - 1) / CT_THR;
+
+template <class T>
+struct CTLev {
+  int X, Y, N, off;
+  const T *S;
+  const int8_t *mask;
+  MapView mx, my;  // to the next coarser level
+};
+template <class T>
+struct CTArgs {
+  int nl, nc;
+  CTLev<T> L[CT_L];
+  const double *ainv;
+  const T *fin;
+  T *uout;
+  int nsmooth;
+};
+
+template <class T>
+__device__ __forceinline__ T ct_apply(const T (&sv)[9], const T *u, int X, int Y, int i, int j) {
+  T acc = 0;
+#pragma unroll
+  for (int di = -1; di <= 1; di++)
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++) {
+      int a = i + di, b = j + dj;
+      a = a < 0 ? 0 : (a >= X ? X - 1 : a);
+      b = b < 0 ? 0 : (b >= Y ? Y - 1 : b);
+      acc += sv[(di + 1) * 3 + dj + 1] * u[a * Y + b];
+    }
+  return acc;
+}
+
+// stencil rows of this thread's points on one tail level (global loads)
+template <class T>
+__device__ __forceinline__ void ct_rows(const CTLev<T> &L, T (&sv)[CT_K][9]) {
+#pragma unroll
+  for (int k = 0; k < CT_K; k++) {
+    const int p = threadIdx.x + k * CT_THR;
+#pragma unroll
+    for (int q = 0; q < 9; q++) sv[k][q] = p < L.N ? L.S[(size_t)q * L.N + p] : T(q == 4 ? 1 : 0);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void ct_smooth(const CTLev<T> &L, const T (&svr)[CT_K][9], bool inreg, T *u, const T *f,
+                                          int nsmooth, bool rev) {
+  for (int it = 0; it < nsmooth; it++)
+    for (int st = 0; st < 2; st++) {
+      const int col = rev ? 1 - st : st;
+      T sv[CT_K][9];
+      if (inreg) {
+#pragma unroll
+        for (int k = 0; k < CT_K; k++)
+#pragma unroll
+          for (int q = 0; q < 9; q++) sv[k][q] = svr[k][q];
+      } else
+        ct_rows(L, sv);
+      T nv[CT_K];
+#pragma unroll
+      for (int k = 0; k < CT_K; k++) {
+        const int p = threadIdx.x + k * CT_THR;
+        nv[k] = 0;
+        if (p < L.N) {
+          const int i = p / L.Y, j = p % L.Y;
+          if (((i + j) & 1) == col)
+            nv[k] = u[L.off + p] + (f[L.off + p] - ct_apply(sv[k], u + L.off, L.X, L.Y, i, j)) / sv[k][4];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < CT_K; k++) {
+        const int p = threadIdx.x + k * CT_THR;
+        if (p < L.N && (((p / L.Y) + (p % L.Y)) & 1) == col) u[L.off + p] = nv[k];
+      }
+      __syncthreads();
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(CT_THR) void k_ctail(CTArgs<T> A) {
+  __shared__ T u[CT_TOT], f[CT_TOT], r[CT_TOT];
+  const int t = threadIdx.x;
+  // the first two tail levels keep their rows in registers; smaller ones re-read them
+  constexpr int NR = sizeof(T) == 4 ? 2 : 1;
+  T sv[NR][CT_K][9];
+#pragma unroll
+  for (int l = 0; l < NR; l++) {
+    if (l < A.nl - 1)
+      ct_rows(A.L[l], sv[l]);
+    else
+#pragma unroll
+      for (int k = 0; k < CT_K; k++)
+#pragma unroll
+        for (int q = 0; q < 9; q++) sv[l][k][q] = T(q == 4 ? 1 : 0);
+  }
+  for (int p = t; p < A.L[0].N; p += CT_THR) {
+    f[A.L[0].off + p] = A.fin[p];
+    u[A.L[0].off + p] = 0;
+  }
+  __syncthreads();
+  // down: pre-smooth, residual, restriction
+#pragma unroll
+  for (int l = 0; l < CT_L - 1; l++) {
+    if (l >= A.nl - 1) break;
+    const CTLev<T> &L = A.L[l], &C = A.L[l + 1];
+    const bool inreg = l < NR;
+    ct_smooth(L, sv[l < NR ? l : 0], inreg, u, f, A.nsmooth, false);
+    T sr[CT_K][9];
+    if (inreg) {
+#pragma unroll
+      for (int k = 0; k < CT_K; k++)
+#pragma unroll
+        for (int q = 0; q < 9; q++) sr[k][q] = sv[l < NR ? l : 0][k][q];
+    } else
+      ct_rows(L, sr);
+#pragma unroll
+    for (int k = 0; k < CT_K; k++) {
+      const int p = t + k * CT_THR;
+      if (p < L.N) {
+        const int i = p / L.Y, j = p % L.Y;
+        r[L.off + p] = L.mask[p] ? T(0) : f[L.off + p] - ct_apply(sr[k], u + L.off, L.X, L.Y, i, j);
+      }
+    }
+    __syncthreads();
+    for (int p = t; p < C.N; p += CT_THR) {
+      const int i = p / C.Y, j = p % C.Y;
+      f[C.off + p] = C.mask[p] ? T(0) : restrict_rw(r + L.off, L.Y, L.mx, L.my, i, j);
+      u[C.off + p] = 0;
+    }
+    __syncthreads();
+  }
+  // coarsest: dense FP64 inverse, one wavefront per row
+  {
+    const CTLev<T> &L = A.L[A.nl - 1];
+    const int wv = t >> 6, lane = t & 63, nw = CT_THR / 64;
+    for (int row = wv; row < A.nc; row += nw) {
+      double acc = 0;
+      for (int c = lane; c < A.nc; c += 64) acc += A.ainv[(size_t)row * A.nc + c] * (double)f[L.off + c];
+      acc = wave_sum(acc);
+      if (lane == 0) u[L.off + row] = T(acc);
+    }
+    __syncthreads();
+  }
+  // up: prolongation, post-smooth
+#pragma unroll
+  for (int l = CT_L - 2; l >= 0; l--) {
+    if (l >= A.nl - 1) continue;
+    const CTLev<T> &L = A.L[l], &C = A.L[l + 1];
+    for (int p = t; p < L.N; p += CT_THR) {
+      const int i = p / L.Y, j = p % L.Y;
+      if (!L.mask[p]) u[L.off + p] += T(prolong_at(u + C.off, C.Y, L.mx, L.my, i, j));
+    }
+    __syncthreads();
+    ct_smooth(L, sv[l < NR ? l : 0], l < NR, u, f, A.nsmooth, true);
+  }
+  for (int p = t; p < A.L[0].N; p += CT_THR) A.uout[p] = u[A.L[0].off + p];
+}
+
 template <class T>
 int CellMG<T>::setup(const std::vector<double> &x0, const std::vector<double> &y0, int coarse_max) {
   L.clear();
@@ -1503,6 +1667,29 @@ template <class T>
 int CellMG<T>::vcycle(int l, hipStream_t s) {
   const int nl = (int)L.size();
   CellLevel<T> &v = L[l];
+  if (v.N <= CT_N && nl - l <= CT_L && nc <= GJ_LDS_N) {
+    CTArgs<T> A{};
+    A.nl = nl - l;
+    A.nc = nc;
+    int off = 0;
+    for (int q = 0; q < A.nl; q++) {
+      CellLevel<T> &w = L[l + q];
+      const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
+      A.L[q] = CTLev<T>{w.X, w.Y, w.N, off, w.S.p, w.mask.p, q + 1 < A.nl ? view(w.mx) : none,
+                        q + 1 < A.nl ? view(w.my) : none};
+      off += w.N;
+    }
+    if (off > CT_TOT) {
+      set_error("cell multigrid tail holds %d points (at most %d)", off, CT_TOT);
+      return I2E_ERR_CAPACITY;
+    }
+    A.ainv = ainv.p;
+    A.fin = v.f.p;
+    A.uout = v.u.p;
+    A.nsmooth = nsmooth;
+    hipLaunchKernelGGL(k_ctail<T>, dim3(1), dim3(CT_THR), 0, s, A);
+    return I2E_OK;
+  }
   if (l == nl - 1) {
     hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(nc, 4)), dim3(256), 0, s, nc, nc, ainv.p, v.f.p, v.f.p, v.u.p,
                        v.u.p);

@@ -220,7 +220,8 @@ int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g,
  * pass, "p2g" = ronurecalc's marker scatter, "gsdot" / "gsaxpy" = the
  * Gram-Schmidt projection / update passes of the Krylov solvers,
  * "vsmooth0" / "vsmoothc" = one colour step of the velocity smoother on the
- * finest / a coarse multigrid level, "stokes_apply"; NULL or ""
+ * finest / a coarse multigrid level, "stokes_apply", "precond" = one
+ * whole Stokes preconditioner application (bytes not counted); NULL or ""
  * stops timing), reserves events for up to max_launches launches and clears
  * the counters; i2e_kernel_timer_read returns the launches timed, their
  * summed milliseconds and summed algorithmic HBM bytes.                     */
