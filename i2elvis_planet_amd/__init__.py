@@ -26,7 +26,7 @@ _EXPORTS = [
     "i2e_stokes_apply_dev", "i2e_gravity_apply_dev", "i2e_heat_apply_dev", "i2e_device_ptr", "i2e_time_kernels",
     "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
-    "i2e_kernel_timer", "i2e_kernel_timer_read",
+    "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals",
 ]
 
 _lib = None
@@ -75,6 +75,7 @@ def load_library(path=LIB_PATH):
         "i2e_stokes_precond_dev": ([vp, vp, vp], C.c_int),
         "i2e_vcycle_dev": ([vp, vp, vp], C.c_int),
         "i2e_kernel_timer": ([vp, C.c_char_p, C.c_int], C.c_int),
+        "i2e_stokes_residuals": ([vp, C.POINTER(abi.Resid)], C.c_int),
         "i2e_kernel_timer_read": ([vp, C.POINTER(C.c_int64), dp, dp], C.c_int),
         "i2e_dev_alloc": ([C.c_size_t, C.POINTER(vp)], C.c_int),
         "i2e_dev_free": ([vp], C.c_int),
@@ -233,6 +234,12 @@ class Context:
 
     def vcycle(self, f_dev, u_dev):
         _chk(_lib.i2e_vcycle_dev(self._h, C.c_void_p(f_dev), C.c_void_p(u_dev)), "vcycle")
+
+    def stokes_residuals(self):
+        """viterate's residual diagnostics (movemart.c:1049-1176) as a dict."""
+        r = abi.Resid()
+        _chk(_lib.i2e_stokes_residuals(self._h, C.byref(r)), "stokes_residuals")
+        return {k: getattr(r, k) for k, _ in abi.Resid._fields_}
 
     def kernel_timer(self, kernel, max_launches=65536):
         """Time every launch of `kernel` from now on with HIP events (None stops)."""

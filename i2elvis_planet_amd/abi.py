@@ -96,3 +96,9 @@ class Stats(C.Structure):
 NODE_FIELDS = ("vx", "vy", "pr", "gp", "ga", "exx", "eyy", "exy", "sxx", "syy", "sxy",
                "nu", "nd", "ro", "tk", "cp", "kt", "ht", "et")
 MARKER_FIELDS_F32 = ("markx", "marky", "markk", "markv", "markp", "markpor", "marktmax")
+
+
+class Resid(C.Structure):
+    """i2e_resid: viterate's residual diagnostics (movemart.c:1049-1176)."""
+    _fields_ = [(k, C.c_double) for k in ("stokes", "cont", "bond", "stoknum", "contnum", "bondnum", "vxmin", "vxmax",
+                                          "vymin", "vymax", "prmin", "prmax", "numin", "numax")]

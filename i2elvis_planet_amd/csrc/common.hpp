@@ -93,6 +93,13 @@ I2E_HD void fd2_interp(double x0, double x1, double xi, double &w0, double &w1) 
   w0 = c4 * 1.0 / c3;
 }
 
+// fdweight(n=1, m=1): two-point derivative weights -1/h, 1/h.
+I2E_HD void fd2_deriv(double x0, double x1, double &d0, double &d1) {
+  const double c3 = x1 - x0;
+  d1 = 1.0 / c3;
+  d0 = -1.0 / c3;
+}
+
 I2E_HD double subline(const double *g, int m, int sh) {
   return sh < 0 ? (g[m] + g[m - 1]) / 2.0 : (sh > 0 ? (g[m] + g[m + 1]) / 2.0 : g[m]);
 }

@@ -221,6 +221,129 @@ __global__ void k_bf_out(Grid g, double mu, const int8_t *lmask, const T *q, dou
   if (!lmask[c]) z[3 * m] = -(double)q[c] / mu;
 }
 
+// ---------------------------------------------------------------- viterate diagnostics
+// Residual diagnostics of viterate (movemart.c:1049-1176) on the post-solve
+// fields: per-block partial sums of the squared x/y-Stokes errors in their
+// error form (xstokserr/ystokserr with ynerr=1, movemart.c:51-81, 186-222),
+// continuity errors (conterr, :354-372) and velocity BC errors (x/ybonderr,
+// :442-457), with the value ranges; the host reduces the blocks in order.
+constexpr int RD_B = 256, RD_Q = 14;
+__device__ __forceinline__ double rd_sol(const NodeF &F, int u) {
+  const int m = u / 3;
+  return u % 3 == 0 ? F.pr[m] : (u % 3 == 1 ? F.vx[m] : F.vy[m]);
+}
+__device__ __forceinline__ double rd_bond(const NodeF &F, BcTab bc, int u) {
+  const int b = bc.bondm[u];
+  double leftx = rd_sol(F, u) - bc.bondv[4 * b + 0];
+  for (int n1 = 0; n1 < 3; n1++)
+    if (bc.bondn[3 * b + n1]) leftx -= bc.bondv[4 * b + n1 + 1] * rd_sol(F, bc.bondn[3 * b + n1] - 1);
+  return leftx;
+}
+__global__ __launch_bounds__(RD_B) void k_resid_diag(Grid g, NodeF F, BcTab bc, double timestep, double *part,
+                                                     int *bad) {
+  __shared__ double sh[RD_Q][RD_B];
+  const int X = g.X, Y = g.Y, N = X * Y;
+  const double *gx = g.gx, *gy = g.gy;
+  // 0..5: stoksum, contsum, bondsum, stoknum, contnum, bondnum; 6..13:
+  // vxmin, vxmax, vymin, vymax, prmin, prmax, numin, numax
+  double q[RD_Q] = {0, 0, 0, 0, 0, 0, 1e200, -1e200, 1e200, -1e200, 1e200, -1e200, 1e200, -1e200};
+  for (int m = blockIdx.x * RD_B + threadIdx.x; m < N; m += gridDim.x * RD_B) {
+    const int m1 = m / Y, m2 = m % Y, mc = 3 * m;
+    if (m2 < Y - 1) {
+      q[6] = fmin(q[6], F.vx[m]);
+      q[7] = fmax(q[7], F.vx[m]);
+    }
+    if (m1 < X - 1) {
+      q[8] = fmin(q[8], F.vy[m]);
+      q[9] = fmax(q[9], F.vy[m]);
+    }
+    if (m1 && m2) {
+      double d0, d1, divv = 0;
+      fd2_deriv(gx[m1 - 1], gx[m1], d0, d1);
+      divv += F.vx[(m1 - 1) * Y + (m2 - 1)] * d0;
+      divv += F.vx[m1 * Y + (m2 - 1)] * d1;
+      fd2_deriv(gy[m2 - 1], gy[m2], d0, d1);
+      divv += F.vy[(m1 - 1) * Y + (m2 - 1)] * d0;
+      divv += F.vy[(m1 - 1) * Y + m2] * d1;
+      q[1] += divv * divv;
+      q[4] += 1.0;
+    }
+    if (m2 < Y - 1) {
+      if (!bc.bondm[mc + 1]) {
+        if (m1 < 1 || m1 > X - 2) {
+          atomicAdd(bad, 1);
+        } else {
+          const double xkf = (gx[m1 + 1] - gx[m1 - 1]) / 2.0, ykf = gy[m2 + 1] - gy[m2];
+          const int v0 = m, v1 = v0 + 1, v2 = v0 + Y, v3 = v2 + 1;
+          const double gxk = -(F.gp[v3] - F.gp[v1]) / xkf;
+          const double rightx = -gxk * (F.ro[v0] + F.ro[v1]) / 2.0;
+          double leftx = (F.sxx[v3] - F.sxx[v1]) / xkf;
+          leftx += (F.sxy[v1] - F.sxy[v0]) / ykf;
+          leftx -= (F.pr[v3] - F.pr[v1]) / xkf;
+          leftx -= timestep / 2.0 * gxk *
+                   (F.vx[v0] * (F.ro[v2] + F.ro[v3] - F.ro[v0 - Y] - F.ro[v1 - Y]) / (gx[m1 + 1] - gx[m1 - 1]) / 2.0 +
+                    ((F.vy[v0] + F.vy[v1]) * (gx[m1] - gx[m1 - 1]) +
+                     (F.vy[v0 - Y] + F.vy[v1 - Y]) * (gx[m1 + 1] - gx[m1])) /
+                        (gx[m1 + 1] - gx[m1 - 1]) / 2.0 * (F.ro[v1] - F.ro[v0]) / (gy[m2 + 1] - gy[m2]));
+          const double nueff = (F.nu[v0] + F.nu[v1]) / 2.0, e = (leftx - rightx) / nueff;
+          q[0] += e * e;
+          q[3] += 1.0;
+          q[12] = fmin(q[12], nueff);
+          q[13] = fmax(q[13], nueff);
+          q[11] = fmax(q[11], fmax(F.pr[v1], F.pr[v3]));
+          q[10] = fmin(q[10], fmin(F.pr[v1], F.pr[v3]));
+        }
+      } else if (bc.bondv[4 * bc.bondm[mc + 1] + 1] < 1e+30) {
+        const double e = rd_bond(F, bc, mc + 1);
+        q[2] += e * e;
+        q[5] += 1.0;
+      }
+    }
+    if (m1 < X - 1) {
+      if (!bc.bondm[mc + 2]) {
+        if (m2 < 1 || m2 > Y - 2) {
+          atomicAdd(bad, 1);
+        } else {
+          const double xkf = gx[m1 + 1] - gx[m1], ykf = (gy[m2 + 1] - gy[m2 - 1]) / 2.0;
+          const int v0 = m, v1 = v0 + 1, v2 = v0 + Y, v3 = v2 + 1;
+          const double gyk = -(F.gp[v3] - F.gp[v2]) / ykf;
+          const double righty = -gyk * (F.ro[v0] + F.ro[v2]) / 2.0;
+          double lefty = (F.syy[v3] - F.syy[v2]) / ykf;
+          lefty += (F.sxy[v2] - F.sxy[v0]) / xkf;
+          lefty -= (F.pr[v3] - F.pr[v2]) / ykf;
+          lefty -= timestep / 2.0 * gyk *
+                   (F.vy[v0] * (F.ro[v1] + F.ro[v3] - F.ro[v0 - 1] - F.ro[v2 - 1]) / (gy[m2 + 1] - gy[m2 - 1]) / 2.0 +
+                    ((F.vx[v0] + F.vx[v2]) * (gy[m2] - gy[m2 - 1]) +
+                     (F.vx[v0 - 1] + F.vx[v2 - 1]) * (gy[m2 + 1] - gy[m2])) /
+                        (gy[m2 + 1] - gy[m2 - 1]) / 2.0 * (F.ro[v2] - F.ro[v0]) / (gx[m1 + 1] - gx[m1]));
+          const double nueff = (F.nu[v0] + F.nu[v2]) / 2.0, e = (lefty - righty) / nueff;
+          q[0] += e * e;
+          q[3] += 1.0;
+          q[12] = fmin(q[12], nueff);
+          q[13] = fmax(q[13], nueff);
+          q[11] = fmax(q[11], fmax(F.pr[v2], F.pr[v3]));
+          q[10] = fmin(q[10], fmin(F.pr[v2], F.pr[v3]));
+        }
+      } else if (bc.bondv[4 * bc.bondm[mc + 2] + 1] < 1e+30) {
+        const double e = rd_bond(F, bc, mc + 2);
+        q[2] += e * e;
+        q[5] += 1.0;
+      }
+    }
+  }
+  for (int k = 0; k < RD_Q; k++) sh[k][threadIdx.x] = q[k];
+  __syncthreads();
+  for (int o = RD_B / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < RD_Q; k++) {
+        const double a = sh[k][threadIdx.x], b = sh[k][threadIdx.x + o];
+        sh[k][threadIdx.x] = k < 6 ? a + b : ((k & 1) == 0 ? fmin(a, b) : fmax(a, b));
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < RD_Q) part[(size_t)blockIdx.x * RD_Q + threadIdx.x] = sh[threadIdx.x][0];
+}
+
 // heat preconditioner input/output: f = rho*cp*r on equation rows, r on BC rows
 template <class T>
 __global__ void k_heat_pre(int N, const double *r, const double *sc, const double *ro, const double *cp,
@@ -1187,6 +1310,48 @@ int i2e_copy_to_dev(void *dst_dev, const void *src_host, size_t bytes) {
 int i2e_copy_to_host(void *dst_host, const void *src_dev, size_t bytes) {
   if ((!dst_host || !src_dev) && bytes) return I2E_ERR_ARG;
   if (bytes) HIP_OK(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost));
+  return I2E_OK;
+}
+
+int i2e_stokes_residuals(i2e_ctx *c, i2e_resid *out) {
+  if (!c || !out || !c->setup_done) return I2E_ERR_ARG;
+  const int nb = 120;
+  if (nb * RD_Q + 1 > 4096) return I2E_ERR_CAPACITY;
+  HIP_OK(hipMemsetAsync(c->red.p + nb * RD_Q, 0, sizeof(double), c->s));
+  hipLaunchKernelGGL(k_resid_diag, dim3(nb), dim3(RD_B), 0, c->s, c->grid(), c->F, c->bc, c->timestep, c->red.p,
+                     (int *)(c->red.p + nb * RD_Q));
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(c->pin.p, c->red.p, sizeof(double) * (nb * RD_Q + 1), hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
+  if (*(const int *)(c->pin.p + nb * RD_Q)) {
+    set_error("residual diagnostics: momentum row without BC on the grid edge");
+    return I2E_ERR_UNSUPPORTED;
+  }
+  double q[RD_Q];
+  for (int k = 0; k < RD_Q; k++) q[k] = c->pin.p[k];
+  for (int b = 1; b < nb; b++)
+    for (int k = 0; k < RD_Q; k++) {
+      const double v = c->pin.p[(size_t)b * RD_Q + k];
+      q[k] = k < 6 ? q[k] + v : ((k & 1) == 0 ? std::fmin(q[k], v) : std::fmax(q[k], v));
+    }
+  const double xstpx = c->P.xsize / (double)(c->X - 1), ystpy = c->P.ysize / (double)(c->Y - 1);
+  const double mindx = xstpx <= ystpy ? xstpx : ystpy;
+  double maxvx = q[7] - q[6], maxvy = q[9] - q[8];
+  maxvx = maxvx >= maxvy ? maxvx : maxvy;
+  out->stokes = std::pow(q[0] / q[3], 0.5) / (maxvx / mindx / mindx);
+  out->cont = std::pow(q[1] / q[4], 0.5) / (maxvx / mindx);
+  out->bond = std::pow(q[2] / q[5], 0.5) / maxvx;
+  out->stoknum = q[3];
+  out->contnum = q[4];
+  out->bondnum = q[5];
+  out->vxmin = q[6];
+  out->vxmax = q[7];
+  out->vymin = q[8];
+  out->vymax = q[9];
+  out->prmin = q[10];
+  out->prmax = q[11];
+  out->numin = q[12];
+  out->numax = q[13];
   return I2E_OK;
 }
 

@@ -214,6 +214,20 @@ void *i2e_device_ptr(i2e_ctx *c, const char *name);
 int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g,
                      double *ms_stokes_apply);
 
+/* Residual diagnostics of viterate (movemart.c:1049-1176, printed there
+ * when printmod is set): RMS of the x/y-Stokes errors in their error form
+ * (stresses of the current fields, divided by the mean nodal viscosity) over
+ * (max velocity range / min grid step^2), of the continuity errors over
+ * (range / step) and of the velocity BC rows over the range; row counts and
+ * the velocity, pressure and viscosity ranges the reference prints.  Works
+ * on the context's current fields (call after i2e_viterate).              */
+typedef struct i2e_resid {
+  double stokes, cont, bond;
+  double stoknum, contnum, bondnum;
+  double vxmin, vxmax, vymin, vymax, prmin, prmax, numin, numax;
+} i2e_resid;
+int i2e_stokes_residuals(i2e_ctx *c, i2e_resid *out);
+
 /* Per-launch HIP-event timing of one kernel on the context's stream (no
  * counterpart in the reference), for roofline figures taken inside a normal
  * run: i2e_kernel_timer selects the kernel ("advect" = movemark's marker

@@ -999,6 +999,156 @@ void orc_stokes_post(const i2e_params *p, i2e_host_state *s, const double *x) {
     }
 }
 
+/* Residual diagnostics of viterate (movemart.c:1049-1176): RMS of the
+ * x/y-Stokes errors in their error form (xstokserr/ystokserr with ynerr=1,
+ * movemart.c:51-81, 186-222: stresses from the post-solve fields, divided by
+ * the mean nodal viscosity), of the continuity errors (conterr, ynerr=1,
+ * :354-372) and of the velocity BC rows (x/ybonderr, :442-457), normalised
+ * by the velocity range and the smallest grid step.  out[14] = stokes,
+ * cont, bond, stoknum, contnum, bondnum, vxmin, vxmax, vymin, vymax,
+ * prmin, prmax, numin, numax.                                               */
+static double xstokes_errv(const i2e_host_state *s, int64_t Y, int64_t m1, int64_t m2, double *nue, double *pmax,
+                           double *pmin) {
+  const double *gx = s->gx, *gy = s->gy;
+  double xkf = (gx[m1 + 1] - gx[m1 - 1]) / 2.0, ykf = gy[m2 + 1] - gy[m2];
+  int64_t v0 = m1 * Y + m2, v1 = v0 + 1, v2 = v0 + Y, v3 = v2 + 1;
+  double gxk = -(s->gp[v3] - s->gp[v1]) / xkf;
+  double rightx = -gxk * (s->ro[v0] + s->ro[v1]) / 2.0, leftx, nueff;
+  leftx = (s->sxx[v3] - s->sxx[v1]) / xkf;
+  leftx += (s->sxy[v1] - s->sxy[v0]) / ykf;
+  leftx -= (s->pr[v3] - s->pr[v1]) / xkf;
+  leftx -= s->timestep / 2.0 * gxk *
+           (s->vx[v0] * (s->ro[v2] + s->ro[v3] - s->ro[v0 - Y] - s->ro[v1 - Y]) / (gx[m1 + 1] - gx[m1 - 1]) / 2.0 +
+            ((s->vy[v0] + s->vy[v1]) * (gx[m1] - gx[m1 - 1]) + (s->vy[v0 - Y] + s->vy[v1 - Y]) * (gx[m1 + 1] - gx[m1])) /
+                (gx[m1 + 1] - gx[m1 - 1]) / 2.0 * (s->ro[v1] - s->ro[v0]) / (gy[m2 + 1] - gy[m2]));
+  nueff = (s->nu[v0] + s->nu[v1]) / 2.0;
+  *nue = nueff;
+  *pmax = MAXV(s->pr[v1], s->pr[v3]);
+  *pmin = MINV(s->pr[v1], s->pr[v3]);
+  return (leftx - rightx) / nueff;
+}
+static double ystokes_errv(const i2e_host_state *s, int64_t Y, int64_t m1, int64_t m2, double *nue, double *pmax,
+                           double *pmin) {
+  const double *gx = s->gx, *gy = s->gy;
+  double xkf = gx[m1 + 1] - gx[m1], ykf = (gy[m2 + 1] - gy[m2 - 1]) / 2.0;
+  int64_t v0 = m1 * Y + m2, v1 = v0 + 1, v2 = v0 + Y, v3 = v2 + 1;
+  double gyk = -(s->gp[v3] - s->gp[v2]) / ykf;
+  double righty = -gyk * (s->ro[v0] + s->ro[v2]) / 2.0, lefty, nueff;
+  lefty = (s->syy[v3] - s->syy[v2]) / ykf;
+  lefty += (s->sxy[v2] - s->sxy[v0]) / xkf;
+  lefty -= (s->pr[v3] - s->pr[v2]) / ykf;
+  lefty -= s->timestep / 2.0 * gyk *
+           (s->vy[v0] * (s->ro[v1] + s->ro[v3] - s->ro[v0 - 1] - s->ro[v2 - 1]) / (gy[m2 + 1] - gy[m2 - 1]) / 2.0 +
+            ((s->vx[v0] + s->vx[v2]) * (gy[m2] - gy[m2 - 1]) + (s->vx[v0 - 1] + s->vx[v2 - 1]) * (gy[m2 + 1] - gy[m2])) /
+                (gy[m2 + 1] - gy[m2 - 1]) / 2.0 * (s->ro[v2] - s->ro[v0]) / (gx[m1 + 1] - gx[m1]));
+  nueff = (s->nu[v0] + s->nu[v2]) / 2.0;
+  *nue = nueff;
+  *pmax = MAXV(s->pr[v2], s->pr[v3]);
+  *pmin = MINV(s->pr[v2], s->pr[v3]);
+  return (lefty - righty) / nueff;
+}
+static double cont_errv(const i2e_host_state *s, int64_t Y, int64_t m1, int64_t m2) {
+  double d0, d1, divv = 0;
+  fd2_deriv(s->gx[m1 - 1], s->gx[m1], &d0, &d1);
+  divv += s->vx[(m1 - 1) * Y + (m2 - 1)] * d0;
+  divv += s->vx[m1 * Y + (m2 - 1)] * d1;
+  fd2_deriv(s->gy[m2 - 1], s->gy[m2], &d0, &d1);
+  divv += s->vy[(m1 - 1) * Y + (m2 - 1)] * d0;
+  divv += s->vy[(m1 - 1) * Y + m2] * d1;
+  return divv;
+}
+/* value of Stokes unknown u (0-based, 3m+{0,1,2}) from the node fields */
+static double sol_at(const i2e_host_state *s, int64_t u) {
+  int64_t m = u / 3;
+  return u % 3 == 0 ? s->pr[m] : (u % 3 == 1 ? s->vx[m] : s->vy[m]);
+}
+static double bond_errv(const i2e_host_state *s, int64_t u) {
+  int64_t b = s->bondm[u], n1;
+  double leftx = sol_at(s, u) - s->bondv[4 * b + 0];
+  for (n1 = 0; n1 < 3; n1++)
+    if (s->bondn[3 * b + n1]) leftx -= s->bondv[4 * b + n1 + 1] * sol_at(s, s->bondn[3 * b + n1] - 1);
+  return leftx;
+}
+int orc_stokes_residuals(const i2e_params *p, const i2e_host_state *s, double *out) {
+  int64_t X = p->xnumx, Y = p->ynumy, m1, m2;
+  double minvx = 1e200, maxvx = -1e200, minvy = 1e200, maxvy = -1e200;
+  double stoksum = 0, stoknum = 0, contsum = 0, contnum = 0, bondsum = 0, bondnum = 0;
+  double minnu = 1e200, maxnu = -1e200, minpr = 1e200, maxpr = -1e200, ival, nue, pmx, pmn;
+  double xstpx = p->xsize / (double)(X - 1), ystpy = p->ysize / (double)(Y - 1), mindx;
+  for (m1 = 0; m1 < X; m1++)
+    for (m2 = 0; m2 < Y; m2++) {
+      int64_t m3 = m1 * Y + m2;
+      if (m2 < Y - 1) {
+        minvx = MINV(minvx, s->vx[m3]);
+        maxvx = MAXV(maxvx, s->vx[m3]);
+      }
+      if (m1 < X - 1) {
+        minvy = MINV(minvy, s->vy[m3]);
+        maxvy = MAXV(maxvy, s->vy[m3]);
+      }
+    }
+  out[6] = minvx;
+  out[7] = maxvx;
+  out[8] = minvy;
+  out[9] = maxvy;
+  maxvx -= minvx;
+  maxvy -= minvy;
+  maxvx = MAXV(maxvx, maxvy);
+  mindx = MINV(xstpx, ystpy);
+  for (m1 = 0; m1 < X; m1++)
+    for (m2 = 0; m2 < Y; m2++) {
+      int64_t mc = (m1 * Y + m2) * 3;
+      if (m1 && m2) {
+        ival = cont_errv(s, Y, m1, m2);
+        contsum += ival * ival;
+        contnum += 1.0;
+      }
+      if (m2 < Y - 1) {
+        if (!s->bondm[mc + 1]) {
+          if (m1 < 1 || m1 > X - 2) return -1; /* the error form needs interior x-Stokes rows */
+          ival = xstokes_errv(s, Y, m1, m2, &nue, &pmx, &pmn);
+          stoksum += ival * ival;
+          stoknum += 1.0;
+          minnu = MINV(minnu, nue);
+          maxnu = MAXV(maxnu, nue);
+          maxpr = MAXV(maxpr, pmx);
+          minpr = MINV(minpr, pmn);
+        } else if (s->bondv[4 * s->bondm[mc + 1] + 1] < 1e+30) {
+          ival = bond_errv(s, mc + 1);
+          bondsum += ival * ival;
+          bondnum += 1.0;
+        }
+      }
+      if (m1 < X - 1) {
+        if (!s->bondm[mc + 2]) {
+          if (m2 < 1 || m2 > Y - 2) return -1;
+          ival = ystokes_errv(s, Y, m1, m2, &nue, &pmx, &pmn);
+          stoksum += ival * ival;
+          stoknum += 1.0;
+          minnu = MINV(minnu, nue);
+          maxnu = MAXV(maxnu, nue);
+          maxpr = MAXV(maxpr, pmx);
+          minpr = MINV(minpr, pmn);
+        } else if (s->bondv[4 * s->bondm[mc + 2] + 1] < 1e+30) {
+          ival = bond_errv(s, mc + 2);
+          bondsum += ival * ival;
+          bondnum += 1.0;
+        }
+      }
+    }
+  out[0] = pow(stoksum / stoknum, 0.5) / (maxvx / mindx / mindx);
+  out[1] = pow(contsum / contnum, 0.5) / (maxvx / mindx);
+  out[2] = pow(bondsum / bondnum, 0.5) / maxvx;
+  out[3] = stoknum;
+  out[4] = contnum;
+  out[5] = bondnum;
+  out[10] = minpr;
+  out[11] = maxpr;
+  out[12] = minnu;
+  out[13] = maxnu;
+  return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* heat, heatmart.c:6-155 (heaterr), 162-312 (qx/qycalc), 320-695 (titerate) */
 /* ------------------------------------------------------------------------ */

@@ -57,6 +57,7 @@ def lib():
             ("orc_heat_prepare", [P, S, dp, dp]), ("orc_heat_build", [P, S, dp, Cp]),
             ("orc_tbc_build", [P, S, Cp]), ("orc_heat_markers", [P, S, dp]),
             ("orc_type_counts", [P, S, C.c_int, C.POINTER(C.c_int32)]),
+            ("orc_stokes_residuals", [P, S, dp]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -206,6 +207,10 @@ def solver_kind():
 
 
 # --------------------------------------------------------------------------- step
+RESID_KEYS = ("stokes", "cont", "bond", "stoknum", "contnum", "bondnum", "vxmin", "vxmax", "vymin", "vymax",
+              "prmin", "prmax", "numin", "numax")
+
+
 class Oracle:
     """The reference's step functions on a HostState (i2mart.c:60-153)."""
 
@@ -296,6 +301,13 @@ class Oracle:
             self.movemark()
         self.ronurecalc()
         self.hs.timesum += self.hs.timestep
+
+    def stokes_residuals(self):
+        """viterate's residual diagnostics (movemart.c:1049-1176) on the current fields."""
+        out = np.zeros(14)
+        _check(self.L.orc_stokes_residuals(C.byref(self.p), self._s(), out.ctypes.data_as(C.POINTER(C.c_double))),
+               "stokes_residuals")
+        return dict(zip(RESID_KEYS, out))
 
     def type_counts(self, ntypes=64):
         C_ = (self.p.xnumx - 1) * (self.p.ynumy - 1)

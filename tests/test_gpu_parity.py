@@ -283,3 +283,31 @@ def test_graph_replay_matches_eager_precond(orc, precision):
     assert i0 == i1 and i0 > 10
     for f in ("vx", "vy", "pr"):
         assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def test_residual_diagnostics_parity(orc):
+    """i2e_stokes_residuals against the oracle's restatement of viterate's
+    diagnostics (movemart.c:1049-1176) on the same perturbed solution."""
+    say("case residual diagnostics c1")
+    import ctypes as C
+    p, hs, _ = configs.make("c1")
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    o.viterate()
+    x = np.zeros(3 * hs.N)
+    x[0::3], x[1::3], x[2::3] = hs.pr, hs.vx, hs.vy
+    rng = np.random.default_rng(12)
+    x[1::3] *= 1 + 1e-4 * rng.standard_normal(hs.N)
+    o.L.orc_stokes_post(C.byref(p), o._s(), x.ctypes.data_as(C.POINTER(C.c_double)))
+    ref = o.stokes_residuals()
+    ctx = Context(p, hs, solver=fp64_opts())
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    got = ctx.stokes_residuals()
+    ctx.close()
+    say("gpu", got["stokes"], got["cont"], got["bond"], "oracle", ref["stokes"], ref["cont"], ref["bond"])
+    for k in ("stoknum", "contnum", "bondnum", "vxmin", "vxmax", "vymin", "vymax", "prmin", "prmax", "numin", "numax"):
+        assert got[k] == ref[k], k
+    for k in ("stokes", "cont", "bond"):
+        assert got[k] == pytest.approx(ref[k], rel=1e-10), k

@@ -355,3 +355,50 @@ def test_direct_solvers_agree_on_stokes(orc):
     for k in (1, 2):  # velocities
         assert np.linalg.norm(xp[k::3] - xs[k::3]) <= 1e-9 * np.linalg.norm(xs[k::3])
     assert np.linalg.norm(xp[0::3] - xs[0::3]) <= 1e-9 * np.linalg.norm(xs[0::3])
+
+
+def test_residual_diagnostics_match_assembled_rows(orc):
+    """viterate's error-form residuals (movemart.c:1049-1176), computed from the
+    post-solve stresses, equal the residual of the assembled rows: momentum
+    rows divided by the mean nodal viscosity, velocity BC rows by mukoef.  Two
+    independent code paths (stresses vs CSR coefficients) must agree."""
+    p, hs, _ = configs.make("c1")
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    o.viterate()
+    exact = o.stokes_residuals()
+    assert exact["stokes"] < 1e-10 and exact["bond"] < 1e-12
+    M = o.stokes_matrix()
+    A = M.scipy()
+    x = np.zeros(3 * hs.N)
+    x[0::3], x[1::3], x[2::3] = hs.pr, hs.vx, hs.vy
+    rng = np.random.default_rng(11)
+    x[1::3] *= 1 + 1e-3 * rng.standard_normal(hs.N)
+    x[2::3] *= 1 + 1e-3 * rng.standard_normal(hs.N)
+    o.L.orc_stokes_post(C.byref(p), o._s(), x.ctypes.data_as(C.POINTER(C.c_double)))
+    got = o.stokes_residuals()
+    x[0::3], x[1::3], x[2::3] = hs.pr, hs.vx, hs.vy  # reload conventions (ghosts) as the post step left them
+    r = A @ x - M.b
+    X = Y = p.xnumx
+    m = np.arange(hs.N)
+    m1, m2 = m // Y, m % Y
+    bm = hs.bondm[:3 * hs.N]
+    mu = p.nubeg * p.strmin
+    sx = (m2 < Y - 1) & (bm[3 * m + 1] == 0)
+    sy = (m1 < X - 1) & (bm[3 * m + 2] == 0)
+    nux = (hs.nu[m[sx]] + hs.nu[m[sx] + 1]) / 2
+    nuy = (hs.nu[m[sy]] + hs.nu[m[sy] + Y]) / 2
+    es = np.concatenate([r[3 * m[sx] + 1] / nux, r[3 * m[sy] + 2] / nuy])
+    bx = (m2 < Y - 1) & (bm[3 * m + 1] != 0)
+    by = (m1 < X - 1) & (bm[3 * m + 2] != 0)
+    eb = np.concatenate([r[3 * m[bx] + 1], r[3 * m[by] + 2]]) / mu
+    vxr = hs.vx[m2 < Y - 1]
+    vyr = hs.vy[m1 < X - 1]
+    vrange = max(vxr.max() - vxr.min(), vyr.max() - vyr.min())
+    mindx = min(p.xsize / (X - 1), p.ysize / (Y - 1))
+    assert got["stoknum"] == es.size and got["bondnum"] == eb.size
+    assert got["stokes"] == pytest.approx(np.sqrt(np.mean(es ** 2)) / (vrange / mindx ** 2), rel=1e-9)
+    assert got["bond"] == pytest.approx(np.sqrt(np.mean(eb ** 2)) / vrange, rel=1e-9)
+    assert got["stokes"] > 1e3 * exact["stokes"]
