@@ -145,6 +145,42 @@ def run(steps, warmup, skip_cpu, rk):
     return line
 
 
+def ensemble(nbody, steps, warmup):
+    """Aggregate timesteps/s of an ensemble of independent bodies (radius
+    10-150 km, grids 101^2-301^2, 16 markers/cell): all bodies stepped
+    together (i2e_step_batch, one host thread and stream per body) against
+    the same bodies stepped one after the other on the same GPU."""
+    from i2elvis_planet_amd import Context, configs, default_solver, step_batch
+    bodies = configs.ensemble(nbody)
+    o = default_solver()
+    o.restart = 60
+    log("ensemble of %d bodies generated (nodes %d..%d)" % (nbody, min(b[2]["n"] for b in bodies) ** 2,
+                                                          max(b[2]["n"] for b in bodies) ** 2))
+    ctxs = [Context(p, hs, solver=o) for p, hs, _ in bodies]
+    for c in ctxs:
+        c.ronurecalc()
+    step_batch(ctxs, warmup)
+    for c in ctxs:
+        c.sync()
+    t0 = time.perf_counter()
+    step_batch(ctxs, steps)
+    for c in ctxs:
+        c.sync()
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for c in ctxs:
+        for _ in range(steps):
+            c.step()
+        c.sync()
+    ts = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    return {"metric": "ensemble aggregate timesteps/s", "n_bodies": nbody, "steps": steps, "warmup": warmup,
+            "batched": nbody * steps / tb, "sequential": nbody * steps / ts, "speedup": ts / tb,
+            "config": {"workload": "%d bodies, radius U[10,150] km, grids 101^2-301^2, 4x4 markers/cell, "
+                                   "H0 log-U[1e-8,3e-7], E U[250,350] kJ/mol" % nbody}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,11 +188,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-kernel", default="gsdot", choices=sorted(ROOFLINE_KERNELS))
+    ap.add_argument("--ensemble", type=int, default=0, help="run the ensemble measurement with this many bodies")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs one GPU only: the multi-GPU decomposition of this step is not built "
               "(see the sharding notes); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.ensemble:
+        print(json.dumps(ensemble(a.ensemble, a.steps, a.warmup)), flush=True)
+        return
     line = run(a.steps, a.warmup, a.no_cpu, a.roofline_kernel)
     print(json.dumps(line), flush=True)
 

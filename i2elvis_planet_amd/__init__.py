@@ -26,7 +26,7 @@ _EXPORTS = [
     "i2e_stokes_apply_dev", "i2e_gravity_apply_dev", "i2e_heat_apply_dev", "i2e_device_ptr", "i2e_time_kernels",
     "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
-    "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals",
+    "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
 ]
 
 _lib = None
@@ -76,6 +76,7 @@ def load_library(path=LIB_PATH):
         "i2e_vcycle_dev": ([vp, vp, vp], C.c_int),
         "i2e_kernel_timer": ([vp, C.c_char_p, C.c_int], C.c_int),
         "i2e_stokes_residuals": ([vp, C.POINTER(abi.Resid)], C.c_int),
+        "i2e_step_batch": ([C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "i2e_kernel_timer_read": ([vp, C.POINTER(C.c_int64), dp, dp], C.c_int),
         "i2e_dev_alloc": ([C.c_size_t, C.POINTER(vp)], C.c_int),
         "i2e_dev_free": ([vp], C.c_int),
@@ -257,4 +258,14 @@ class Context:
         return a.value, b.value, c.value
 
 
-__all__ = ["Context", "DeviceArray", "HostState", "Params", "SolverOpts", "Stats", "default_solver", "load_library", "I2EError"]
+def step_batch(contexts, nsteps=1, nthreads=0):
+    """nsteps of step() on every context, bodies overlapping on the GPU
+    (one host thread per body unless nthreads says otherwise)."""
+    L = load_library()
+    n = len(contexts)
+    arr = (C.c_void_p * n)(*[c._h.value for c in contexts])
+    st = (C.c_int * n)()
+    _chk(L.i2e_step_batch(arr, n, nsteps, nthreads, st), "step_batch")
+
+
+__all__ = ["step_batch", "Context", "DeviceArray", "HostState", "Params", "SolverOpts", "Stats", "default_solver", "load_library", "I2EError"]

@@ -49,6 +49,8 @@ CONFIGS = {
 
 
 def make_params(cfg):
+    e_act = cfg.get("e_act", E_ACT)
+    nu0_arrh = 1e20 / math.exp(e_act / (8.314 * ETA_REF_T))  # eta(1000 K) = 1e20 for any E
     p = abi.Params()
     n = cfg["n"]
     p.xnumx = p.ynumy = n
@@ -75,7 +77,7 @@ def make_params(cfg):
     p.tmp_ambient = T_AMBIENT
     p.al2627_init = AL2627_FOR_H0
     p.fe6056_init = 0.0
-    p.h0_scale = 1.0
+    p.h0_scale = cfg.get("h0", H0) / H0
     p.longlived = 0.0
     p.por_init = 0.4 if cfg["porosity"] else 0.0
     p.k_cutoff = 1e6
@@ -104,9 +106,9 @@ def make_params(cfg):
         rock(SILICATE, 3500.0, 1e20)
         rock(IRON, 7800.0, 1e20)
     else:
-        rock(SILICATE, 3500.0, NU0_ARRH, dh=E_ACT)
-        rock(IRON, 7800.0, NU0_ARRH, dh=E_ACT)
-    rock(MOLTEN_SILICATE, 3500.0, NU0_ARRH, dh=E_ACT, n0=1e2)
+        rock(SILICATE, 3500.0, nu0_arrh, dh=e_act)
+        rock(IRON, 7800.0, nu0_arrh, dh=e_act)
+    rock(MOLTEN_SILICATE, 3500.0, nu0_arrh, dh=e_act, n0=1e2)
     for t in (5, 6, 25, 26):
         p.is_silicate[t] = 1
     for t in (7, 8, 9, 10, 17, 18, 19):
@@ -169,9 +171,10 @@ def free_slip_bc(X, Y, t_ambient=T_AMBIENT):
 
 
 def make_state(name, seed=20261203):
-    """Seeded initial state of config `name` (markers, grid, BCs; node fields
-    zero except T = ambient).  Node properties come from a first ronurecalc."""
-    cfg = CONFIGS[name]
+    """Seeded initial state of config `name` (or of a config dict: markers,
+    grid, BCs; node fields zero except T = ambient).  Node properties come
+    from a first ronurecalc."""
+    cfg = CONFIGS[name] if isinstance(name, str) else name
     n, size, mnum, R = cfg["n"], cfg["size"], cfg["mnum"], cfg["radius"]
     X = Y = n
     cells = (X - 1) * (Y - 1)
@@ -241,3 +244,21 @@ def make_state(name, seed=20261203):
 def make(name, seed=20261203):
     cfg = CONFIGS[name]
     return make_params(cfg), make_state(name, seed), cfg
+
+
+def ensemble(nbody=64, seed=20261203, nmin=101, nmax=301):
+    """The batched-ensemble setup: bodies with radius uniform in [10, 150] km,
+    grids scaled with the radius from nmin^2 to nmax^2 nodes (odd sizes, box
+    4R), 4x4 markers per cell, H0 log-uniform in [1e-8, 3e-7] W/kg and
+    activation energy uniform in [250, 350] kJ/mol (Arrhenius, 25 % iron).
+    Returns a list of (params, state, cfg)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 7))
+    out = []
+    for k in range(nbody):
+        R = rng.uniform(10e3, 150e3)
+        n = int(round(nmin + (R - 10e3) / 140e3 * (nmax - nmin))) | 1
+        cfg = dict(n=n, size=4 * R, mnum=4, radius=R, steps=50, rheology="arrhenius", porosity=False,
+                   iron="random", melt=False, nubeg=1e17, nucontr=1e18, densimod=1,
+                   h0=math.exp(rng.uniform(math.log(1e-8), math.log(3e-7))), e_act=rng.uniform(250e3, 350e3))
+        out.append((make_params(cfg), make_state(cfg, seed + 100 * k), cfg))
+    return out

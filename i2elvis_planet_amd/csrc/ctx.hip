@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "krylov.hpp"
@@ -1217,6 +1218,38 @@ int i2e_step(i2e_ctx *c) {
   c->st.ms_markers = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->timesum += c->timestep;
   return I2E_OK;
+}
+
+int i2e_step_batch(i2e_ctx *const *ctxs, int n, int nsteps, int nthreads, int *status) {
+  if (!ctxs || n < 0 || nsteps < 0) return I2E_ERR_ARG;
+  for (int k = 0; k < n; k++)
+    if (!ctxs[k]) return I2E_ERR_ARG;
+  if (nthreads <= 0 || nthreads > n) nthreads = n;
+  std::vector<int> st(n, I2E_OK);
+  std::vector<std::string> msg(n);
+  // body k belongs to host thread k % nthreads for all steps (its own
+  // stream, its own order); different bodies overlap on the GPU
+  auto work = [&](int t) {
+    for (int step = 0; step < nsteps; step++)
+      for (int k = t; k < n; k += nthreads)
+        if (st[k] == I2E_OK) {
+          st[k] = i2e_step(ctxs[k]);
+          if (st[k]) msg[k] = g_err;
+        }
+  };
+  std::vector<std::thread> pool;
+  pool.reserve(nthreads);
+  for (int t = 0; t < nthreads; t++) pool.emplace_back(work, t);
+  for (auto &th : pool) th.join();
+  int first = I2E_OK;
+  for (int k = 0; k < n; k++) {
+    if (status) status[k] = st[k];
+    if (st[k] && first == I2E_OK) {
+      first = st[k];
+      set_error("body %d: %s", k, msg[k].c_str());
+    }
+  }
+  return first;
 }
 
 int i2e_stokes_apply_dev(i2e_ctx *c, const double *x, double *y) {

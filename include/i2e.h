@@ -214,6 +214,13 @@ void *i2e_device_ptr(i2e_ctx *c, const char *name);
 int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g,
                      double *ms_stokes_apply);
 
+/* Ensemble: nsteps of i2e_step on each of n independent bodies, bodies
+ * spread over nthreads host threads (0: one per body) so that their
+ * launches overlap on the GPU.  status[k] (may be NULL) receives body k's
+ * result; the return value is the first failure.  No counterpart in the
+ * reference, which runs one body per process (i2mart.c:60-164).           */
+int i2e_step_batch(i2e_ctx *const *ctxs, int n, int nsteps, int nthreads, int *status);
+
 /* Residual diagnostics of viterate (movemart.c:1049-1176, printed there
  * when printmod is set): RMS of the x/y-Stokes errors in their error form
  * (stresses of the current fields, divided by the mean nodal viscosity) over

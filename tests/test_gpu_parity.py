@@ -311,3 +311,33 @@ def test_residual_diagnostics_parity(orc):
         assert got[k] == ref[k], k
     for k in ("stokes", "cont", "bond"):
         assert got[k] == pytest.approx(ref[k], rel=1e-10), k
+
+
+def test_batch_matches_sequential():
+    """Ensemble stepping (bodies on separate streams driven from separate host
+    threads) gives the results of stepping each body alone."""
+    from i2elvis_planet_amd import step_batch
+    say("case batch vs sequential")
+    bodies = configs.ensemble(4, nmin=41, nmax=71)
+    seq, bat = [], []
+    for p, hs, _ in bodies:
+        ctx = Context(p, hs.copy(), solver=fp64_opts(1e-10))
+        ctx.ronurecalc()
+        ctx.step()
+        ctx.step()
+        seq.append(ctx.download(hs.copy()))
+        ctx.close()
+    ctxs = [Context(p, hs.copy(), solver=fp64_opts(1e-10)) for p, hs, _ in bodies]
+    for c in ctxs:
+        c.ronurecalc()
+    step_batch(ctxs, 2)
+    for c, (p, hs, _) in zip(ctxs, bodies):
+        bat.append(c.download(hs.copy()))
+        c.close()
+    for a, b in zip(seq, bat):
+        assert a.timestep == pytest.approx(b.timestep, rel=1e-12)
+        for f in ("vx", "vy", "pr", "tk"):
+            assert rel(b.__dict__[f], a.__dict__[f]) < 1e-9, f
+        assert np.array_equal(a.markt, b.markt)
+        np.testing.assert_allclose(b.markx, a.markx, rtol=1e-6)
+        np.testing.assert_allclose(b.marky, a.marky, rtol=1e-6)
