@@ -328,6 +328,135 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan_lds(int n, const double *
   for (int idx = t; idx < n * n; idx += nt) inv[idx] = a[(idx / n) * w + n + idx % n];
 }
 
+// In-place Gauss-Jordan inversion with partial pivoting for n too large for
+// LDS: the n x n matrix lives in the registers of one 1024-thread workgroup
+// (thread (g, c) holds rows g*RPG .. g*RPG+RPG-1 of column c), rows and the
+// pivot column are exchanged through LDS, and the column swaps implied by
+// the row pivots are undone when the inverse is written out.
+constexpr int GJR_T = 1024, GJR_R = 32, GJR_N = 192;
+__global__ __launch_bounds__(GJR_T) void k_gauss_jordan_reg(int n, const double *ag, double *inv) {
+  __shared__ double rowb[GJR_N], colb[GJR_N], pv[2 * GJR_N], sval[GJR_T / 32];
+  __shared__ int sidx[GJR_T / 32], piv[GJR_N], pinv[GJR_N], sq;
+  const int t = threadIdx.x, G = GJR_T / n, rpg = (n + G - 1) / G;
+  const int g = t / n, c = t % n;
+  const bool act = g < G;
+  double a[GJR_R];
+#pragma unroll
+  for (int k2 = 0; k2 < GJR_R; k2++) {
+    const int r = g * rpg + k2;
+    a[k2] = (act && k2 < rpg && r < n) ? ag[(size_t)r * 2 * n + c] : 0.0;
+  }
+  for (int k = 0; k < n; k++) {
+    // pivot search down column k
+    if (act && c == k) {
+      double best = -1;
+      int bi = k;
+#pragma unroll
+      for (int k2 = 0; k2 < GJR_R; k2++) {
+        const int r = g * rpg + k2;
+        if (k2 < rpg && r < n && r >= k && fabs(a[k2]) > best) {
+          best = fabs(a[k2]);
+          bi = r;
+        }
+      }
+      sval[g] = best;
+      sidx[g] = bi;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double best = -1;
+      int bi = k;
+      for (int q = 0; q < G; q++)
+        if (sval[q] > best) {
+          best = sval[q];
+          bi = sidx[q];
+        }
+      sq = bi;
+      piv[k] = bi;
+    }
+    __syncthreads();
+    const int q = sq, gk = k / rpg, sk = k % rpg, gq = q / rpg, sk2 = q % rpg;
+    // swap rows k and q (through LDS, one column per thread group member)
+    if (q != k) {
+      if (act && g == gk)
+#pragma unroll
+        for (int k2 = 0; k2 < GJR_R; k2++)
+          if (k2 == sk) pv[c] = a[k2];
+      if (act && g == gq)
+#pragma unroll
+        for (int k2 = 0; k2 < GJR_R; k2++)
+          if (k2 == sk2) pv[GJR_N + c] = a[k2];
+      __syncthreads();
+      if (act && g == gk)
+#pragma unroll
+        for (int k2 = 0; k2 < GJR_R; k2++)
+          if (k2 == sk) a[k2] = pv[GJR_N + c];
+      if (act && g == gq)
+#pragma unroll
+        for (int k2 = 0; k2 < GJR_R; k2++)
+          if (k2 == sk2) a[k2] = pv[c];
+      __syncthreads();
+    }
+    // pivot value
+    if (act && g == gk && c == k)
+#pragma unroll
+      for (int k2 = 0; k2 < GJR_R; k2++)
+        if (k2 == sk) pv[0] = a[k2];
+    __syncthreads();
+    const double p = pv[0];
+    // normalise row k (a[k][k] := 1 first: the column becomes the inverse's)
+    if (act && g == gk)
+#pragma unroll
+      for (int k2 = 0; k2 < GJR_R; k2++)
+        if (k2 == sk) {
+          const double v = (c == k ? 1.0 : a[k2]) / p;
+          a[k2] = v;
+          rowb[c] = v;
+        }
+    // factors of column k, zeroed in place below and above the pivot
+    if (act && c == k)
+#pragma unroll
+      for (int k2 = 0; k2 < GJR_R; k2++) {
+        const int r = g * rpg + k2;
+        if (k2 < rpg && r < n) {
+          colb[r] = r == k ? 0.0 : a[k2];
+          if (r != k) a[k2] = 0.0;
+        }
+      }
+    __syncthreads();
+    if (act) {
+      const double rv = rowb[c];
+#pragma unroll
+      for (int k2 = 0; k2 < GJR_R; k2++) {
+        const int r = g * rpg + k2;
+        if (k2 < rpg && r < n && r != k) a[k2] -= colb[r] * rv;
+      }
+    }
+    __syncthreads();
+  }
+  // undo the column swaps: output column pinv[c] = column c
+  if (t == 0) {
+    for (int j = 0; j < n; j++) pinv[j] = j;  // P
+    for (int k = n - 1; k >= 0; k--) {
+      const int q = piv[k], x = pinv[k];
+      pinv[k] = pinv[q];
+      pinv[q] = x;
+    }
+    // invert P through the (now free) pivot exchange buffer
+    for (int j = 0; j < n; j++) pv[pinv[j]] = (double)j;
+    for (int j = 0; j < n; j++) pinv[j] = (int)pv[j];
+  }
+  __syncthreads();
+  if (act) {
+    const int oc = pinv[c];
+#pragma unroll
+    for (int k2 = 0; k2 < GJR_R; k2++) {
+      const int r = g * rpg + k2;
+      if (k2 < rpg && r < n) inv[(size_t)r * n + oc] = a[k2];
+    }
+  }
+}
+
 static void gauss_jordan(int n, double *aug, double *inv, hipStream_t s);
 
 // Gauss-Jordan inversion with partial pivoting in one workgroup (global memory)
@@ -385,9 +514,17 @@ __global__ void k_gauss_jordan(int n, double *a, T *inv) {
   for (int idx = t; idx < n * n; idx += nt) inv[idx] = T(a[(size_t)(idx / n) * w + n + idx % n]);
 }
 
+static bool gj_reg_fits(int n) {
+  if (n < 1 || n > GJR_N || GJR_T / n < 1) return false;
+  const int G = GJR_T / n;
+  return (n + G - 1) / G <= GJR_R;
+}
+
 static void gauss_jordan(int n, double *aug, double *inv, hipStream_t s) {
   if (n <= GJ_LDS_N)
     hipLaunchKernelGGL(k_gauss_jordan_lds, dim3(1), dim3(1024), 0, s, n, (const double *)aug, inv);
+  else if (gj_reg_fits(n))
+    hipLaunchKernelGGL(k_gauss_jordan_reg, dim3(1), dim3(GJR_T), 0, s, n, (const double *)aug, inv);
   else
     hipLaunchKernelGGL(k_gauss_jordan<double>, dim3(1), dim3(1024), 0, s, n, aug, inv);
 }
@@ -451,31 +588,36 @@ __global__ void k_probe0(VLev<T> L, T *S) {
 // Galerkin coarse stencil R A P (R = normalised transpose of the bilinear
 // prolongation restricted to equation rows, as the residual restriction is);
 // pointwise rows (boundary, ghost) keep their mukoef-scaled form
+// one thread per coarse point, 64 threads per block: the 50 accumulators of
+// a row live in LDS (runtime slot indices would otherwise go to scratch)
+constexpr int GAL_B = 64;
 template <class T>
-__global__ void k_galerkin(StView<T> F, int Xc, int Yc, MapView xn, MapView xm, MapView yn, MapView ym,
-                           const int8_t *ckx, const int8_t *cky, const int *cnbx, const int *cnby, const T *ckkx,
-                           const T *ckky, double mu, T *Sc, int *overflow) {
-  IDX2;
+__global__ __launch_bounds__(GAL_B) void k_galerkin(StView<T> F, int Xc, int Yc, MapView xn, MapView xm, MapView yn,
+                                                    MapView ym, const int8_t *ckx, const int8_t *cky, const int *cnbx,
+                                                    const int *cnby, const T *ckkx, const T *ckky, double mu, T *Sc,
+                                                    int *overflow) {
+  __shared__ double sacc[50 * GAL_B];
+  const int j = blockIdx.x * GAL_B + threadIdx.x, i = blockIdx.y;
+  double *acc = sacc + threadIdx.x;  // slot q at acc[q * GAL_B]
   if (i >= Xc || j >= Yc) return;
   const int Nc = Xc * Yc, Nf = F.X * F.Y, m = i * Yc + j;
   const size_t Rc = 2 * (size_t)Nc, Rf = 2 * (size_t)Nf;
   for (int comp = 1; comp <= 2; comp++) {
     const size_t rc = (size_t)(comp - 1) * Nc + m;
-    double acc[50];
-    for (int q = 0; q < 50; q++) acc[q] = 0.0;
+    for (int q = 0; q < 50; q++) acc[q * GAL_B] = 0.0;
     const int kind = comp == 1 ? ckx[m] : cky[m];
     if (kind != 0) {
-      acc[sslot(2, comp - 1, 0, 0)] = mu;
+      acc[sslot(2, comp - 1, 0, 0) * GAL_B] = mu;
       const int nb = comp == 1 ? cnbx[m] : cnby[m];
       if (kind == 1 && nb >= 0) {
         const double kk = (double)(comp == 1 ? ckkx[m] : ckky[m]);
         const int di = nb / Yc - i, dj = nb % Yc - j;
         if (di >= -2 && di <= 2 && dj >= -2 && dj <= 2)
-          acc[sslot(2, comp - 1, di, dj)] -= mu * kk;
+          acc[sslot(2, comp - 1, di, dj) * GAL_B] -= mu * kk;
         else
           atomicAdd(overflow, 1);
       }
-      for (int q = 0; q < 50; q++) Sc[(size_t)q * Rc + rc] = T(acc[q]);
+      for (int q = 0; q < 50; q++) Sc[(size_t)q * Rc + rc] = T(acc[q * GAL_B]);
       continue;
     }
     const MapView mxr = comp == 1 ? xn : xm, myr = comp == 1 ? ym : yn;
@@ -510,14 +652,14 @@ __global__ void k_galerkin(StView<T> F, int Xc, int Yc, MapView xn, MapView xm, 
                     atomicAdd(overflow, 1);
                     continue;
                   }
-                  acc[sslot(2, cc, di, dj)] += w * sv * pw;
+                  acc[sslot(2, cc, di, dj) * GAL_B] += w * sv * pw;
                 }
             }
         }
       }
     }
     const double inv = wsum > 0 ? 1.0 / wsum : 0.0;
-    for (int q = 0; q < 50; q++) Sc[(size_t)q * Rc + rc] = T(acc[q] * inv);
+    for (int q = 0; q < 50; q++) Sc[(size_t)q * Rc + rc] = T(acc[q * GAL_B] * inv);
   }
 }
 
@@ -929,7 +1071,8 @@ int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, cons
   for (int l = 0; l + 1 < nl; l++) {
     VelLevel<T> &f = L[l], &c = L[l + 1];
     StView<T> fv{f.S.p, f.X, f.Y, f.W, f.kx.p, f.ky.p};
-    hipLaunchKernelGGL(k_galerkin<T>, grd2(c.X, c.Y), blk2(), 0, s, fv, c.X, c.Y, view(f.mxn), view(f.mxm),
+    hipLaunchKernelGGL(k_galerkin<T>, dim3((c.Y + GAL_B - 1) / GAL_B, c.X), dim3(GAL_B), 0, s, fv, c.X, c.Y,
+                       view(f.mxn), view(f.mxm),
                        view(f.myn), view(f.mym), c.kx.p, c.ky.p, c.nbx.p, c.nby.p, c.kkx.p, c.kky.p, mu_, c.S.p,
                        overflow.p);
   }
