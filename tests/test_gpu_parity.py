@@ -341,3 +341,70 @@ def test_batch_matches_sequential():
         assert np.array_equal(a.markt, b.markt)
         np.testing.assert_allclose(b.markx, a.markx, rtol=1e-6)
         np.testing.assert_allclose(b.marky, a.marky, rtol=1e-6)
+
+
+def test_viterate_parity_201_fp64(orc):
+    """viterate at 201^2 (Arrhenius, porosity) against the oracle's direct solve."""
+    say("case viterate c2")
+    p, hs, _ = configs.make("c2")
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    # the 201^2 system stagnates near 5e-12 in FP64; 1e-10 is reached
+    ctx = Context(p, hs, solver=fp64_opts(1e-10))
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.viterate()
+    st = ctx.stats()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o.viterate()
+    say("stokes its", st.stokes_iters, "relres", st.stokes_relres)
+    assert st.stokes_relres <= 1e-10
+    for f in ("gp", "vx", "vy", "pr", "exx", "sxy"):
+        e = rel(getattr(got, f), getattr(hs, f))
+        say(f, e)
+        assert e < 1e-8, f
+
+
+def test_step_501_properties():
+    """One full step of the 501^2 / 6.25M-marker setup: the Stokes solution
+    satisfies the assembled rows to the metric's tolerance when checked
+    independently in FP64 (operator apply on the downloaded fields), the
+    residual diagnostics are finite, markers keep their count and types
+    outside melting, and temperatures stay within physical bounds."""
+    say("case step 501 properties")
+    p, hs, _ = configs.make("c3")
+    ctx = Context(p, hs.copy())
+    ctx.ronurecalc()
+    # the step's Stokes solve, checked before titerate changes the timestep
+    ctx.set_scalars(p.maxtmstep, 0.0)
+    ctx.maxvelstep()
+    ctx.viterate()
+    st = ctx.stats()
+    got = ctx.download(hs.copy())
+    assert st.stokes_relres <= 1e-8
+    # independent residual: b - K x with x the downloaded fields
+    n = 3 * hs.N
+    x = np.zeros(n)
+    x[0::3], x[1::3], x[2::3] = got.pr, got.vx, got.vy
+    xd, yd, bd = DeviceArray(x), DeviceArray(n), DeviceArray(n)
+    ctx.stokes_rhs(bd.ptr)
+    ctx.apply("stokes", xd.ptr, yd.ptr)
+    r = bd.numpy() - yd.numpy()
+    relres = np.linalg.norm(r) / np.linalg.norm(bd.numpy())
+    say("independent relres", relres)
+    assert relres <= 1.5e-8
+    d = ctx.stokes_residuals()
+    assert all(np.isfinite(v) for v in d.values())
+    # rest of the step
+    ctx.set_scalars(p.maxtmstep, 0.0)
+    ctx.maxvelstep()
+    ctx.titerate()
+    ctx.movemark()
+    ctx.ronurecalc()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    assert got.marknum == hs.marknum
+    assert np.array_equal(np.sort(np.unique(got.markt)), np.sort(np.unique(hs.markt)))
+    assert np.all(got.tk >= 149.0) and np.all(got.tk < 2500.0)
