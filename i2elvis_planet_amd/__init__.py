@@ -27,6 +27,7 @@ _EXPORTS = [
     "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
+    "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
 ]
 
 _lib = None
@@ -82,6 +83,12 @@ def load_library(path=LIB_PATH):
         "i2e_dev_free": ([vp], C.c_int),
         "i2e_copy_to_dev": ([vp, vp, C.c_size_t], C.c_int),
         "i2e_copy_to_host": ([vp, vp, C.c_size_t], C.c_int),
+        "i2e_set_params": ([vp, P], C.c_int),
+        "i2e_record": ([vp, C.c_int], C.c_int),
+        "i2e_replay": ([vp, C.c_int], C.c_int),
+        "i2e_peak_t": ([vp, dp], C.c_int),
+        "i2e_adjoint_peak_t": ([vp, C.POINTER(abi.AdjointOut)], C.c_int),
+        "i2e_heat_apply_t_dev": ([vp, vp, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -224,7 +231,7 @@ class Context:
 
     def apply(self, which, x_dev, y_dev):
         f = {"stokes": _lib.i2e_stokes_apply_dev, "gravity": _lib.i2e_gravity_apply_dev,
-             "heat": _lib.i2e_heat_apply_dev}[which]
+             "heat": _lib.i2e_heat_apply_dev, "heat_t": _lib.i2e_heat_apply_t_dev}[which]
         _chk(f(self._h, C.c_void_p(x_dev), C.c_void_p(y_dev)), which + "_apply")
 
     def stokes_rhs(self, b_dev):
@@ -251,6 +258,31 @@ class Context:
         n, ms, b = C.c_int64(), C.c_double(), C.c_double()
         _chk(_lib.i2e_kernel_timer_read(self._h, C.byref(n), C.byref(ms), C.byref(b)), "kernel_timer_read")
         return n.value, ms.value, b.value
+
+    def set_params(self, params):
+        _chk(_lib.i2e_set_params(self._h, C.byref(params)), "set_params")
+        self.p = params
+
+    # adjoint of the peak-temperature objective -------------------------------------
+    def record(self, nsteps):
+        """ronurecalc + nsteps steps from the uploaded state, history kept on the GPU."""
+        _chk(_lib.i2e_record(self._h, nsteps), "record")
+
+    def replay(self, nsteps):
+        """ronurecalc + the thermal part of nsteps recorded steps (recorded positions and dt)."""
+        _chk(_lib.i2e_replay(self._h, nsteps), "replay")
+
+    def peak_t(self):
+        """Mass-weighted mean marktmax of the silicate markers inside the grid."""
+        J = C.c_double()
+        _chk(_lib.i2e_peak_t(self._h, C.byref(J)), "peak_t")
+        return J.value
+
+    def adjoint_peak_t(self):
+        """J and dJ/d(h0_scale, phi0) through the recorded thermal chain (call after record)."""
+        o = abi.AdjointOut()
+        _chk(_lib.i2e_adjoint_peak_t(self._h, C.byref(o)), "adjoint_peak_t")
+        return {k: getattr(o, k) for k, _ in abi.AdjointOut._fields_}
 
     def time_kernels(self, reps=5):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

@@ -102,3 +102,9 @@ class Resid(C.Structure):
     """i2e_resid: viterate's residual diagnostics (movemart.c:1049-1176)."""
     _fields_ = [(k, C.c_double) for k in ("stokes", "cont", "bond", "stoknum", "contnum", "bondnum", "vxmin", "vxmax",
                                           "vymin", "vymax", "prmin", "prmax", "numin", "numax")]
+
+
+class AdjointOut(C.Structure):
+    """i2e_adjoint_out: peak-temperature objective and its gradient."""
+    _fields_ = [("J", C.c_double), ("dJ_dh0", C.c_double), ("dJ_dphi0", C.c_double),
+                ("heat_relres_max", C.c_double), ("steps", C.c_int32), ("heat_iters", C.c_int32)]

@@ -45,6 +45,13 @@ CONFIGS = {
     # 501x501, 600 km, 5x5/cell (6.25M), R=150 km, Arrhenius + linear melting, 400 iron blobs
     "c3": dict(n=501, size=600e3, mnum=5, radius=150e3, steps=200, rheology="arrhenius", porosity=False,
                iron="blobs", melt=True, nubeg=1e2, nucontr=1e21, densimod=1),
+    # adjoint case: the 201x201 setup over 50 steps with the porosity held at
+    # its initial U[0.3,0.5] values (no sintering), no shear heating and no
+    # heating-rate limit on the time step (titerate's dt = maxtkstep/max|dT/dt|
+    # would make dt, and with it every trajectory, depend on the heating)
+    "c4": dict(n=201, size=400e3, mnum=4, radius=100e3, steps=50, rheology="arrhenius", porosity=True,
+               porosity_frozen=True, frictyn=1e30, maxtkstep=1e30, iron="random", melt=False, nubeg=1e17,
+               nucontr=1e18, densimod=1),
 }
 
 
@@ -57,7 +64,7 @@ def make_params(cfg):
     p.mnumx = p.mnumy = cfg["mnum"]
     p.xsize = p.ysize = cfg["size"]
     p.maxxystep = 0.05
-    p.maxtkstep = 1e4
+    p.maxtkstep = cfg.get("maxtkstep", 1e4)
     p.maxtmstep = 2.5e3 * YEAR
     p.nubeg = cfg["nubeg"]
     p.nuend = 1e23
@@ -67,7 +74,7 @@ def make_params(cfg):
     # the same row scale, otherwise the assembled system is badly scaled.
     p.strmin = 1e16 / cfg["nubeg"]
     p.heatdif = 1.0
-    p.frictyn = 1.0
+    p.frictyn = cfg.get("frictyn", 1.0)
     p.adiabyn = 0.0
     p.heatmod = 0
     p.densimod = cfg["densimod"]
@@ -90,7 +97,7 @@ def make_params(cfg):
     p.eta_outer = 1e22
     p.melt_ts, p.melt_tl, p.melt_switch, p.melt_eta, p.melt_hlat = 1360.0, 1850.0, 0.4, 1e2, 4e5
     p.melt_on = 1 if cfg["melt"] else 0
-    p.porosity_on = 1 if cfg["porosity"] else 0
+    p.porosity_on = 1 if cfg["porosity"] and not cfg.get("porosity_frozen") else 0
     p.ntypes = abi.MAXTYPES
 
     def rock(t, ro, nu0, dh=0.0, cp=1000.0, kt=3.0, n0=1e17, n1=1e23):

@@ -64,6 +64,20 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+__device__ __forceinline__ void aadd(double *a, double v) { unsafeAtomicAdd(a, v); }
+
+// segmented sum over the lanes of a wavefront whose segment ids are equal
+// and contiguous: afterwards the first lane of each segment holds its total
+__device__ __forceinline__ double seg_sum(double v, int seg, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double o = __shfl_down(v, off, 64);
+    const int so = __shfl_down(seg, off, 64);
+    if (lane + off < 64 && so == seg) v += o;
+  }
+  return v;
+}
+
 // Grid lines of one level.  Node m = m1*Y + m2.
 struct Grid {
   int X, Y;

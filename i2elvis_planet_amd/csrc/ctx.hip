@@ -14,7 +14,7 @@
 #include <thread>
 #include <vector>
 
-#include "krylov.hpp"
+#include "ctx.hpp"
 
 namespace i2e {
 
@@ -427,47 +427,6 @@ __global__ void k_grav_fields(Grid g, double R, double mu, double *k, double *c,
 
 }  // namespace i2e
 
-using namespace i2e;
-
-struct i2e_ctx {
-  i2e_params P;
-  i2e_params *dP = nullptr;
-  int X = 0, Y = 0, N = 0;
-  int64_t Mmax = 0, M = 0;
-  hipStream_t s = nullptr;
-  DBuf<double> gx, gy, nodes, acc, work, sol1, tk2, red, xs, bs;
-  DBuf<float> mfl;
-  DBuf<int8_t> mt;
-  DBuf<int> bondm, bondn;
-  DBuf<double> bondv;
-  NodeF F;
-  Markers mk;
-  Acc A;
-  BcTab bc;
-  int need_eii = 0;
-  bool setup_done = false;
-  std::vector<GeoLevel> geo;
-  VelMG<float> v32;
-  VelMG<double> v64;
-  CellMG<float> l32;
-  CellMG<double> l64;
-  DBuf<double> bcix, bciy, bLS;  // BFBt: C^-1 on momentum rows, FP64 cell stencil
-  DBuf<int8_t> blmask;
-  ScalMG<float> h32, g32;
-  ScalMG<double> h64, g64;
-  DBuf<double> hk, hc, hdm, gk, gc, gdm, gsc, hsc;
-  DBuf<int8_t> hmask, gmask;
-  Fgmres fs, fh, fgv;
-  i2e_solver_opts o;
-  i2e_stats st;
-  double timestep = 0, timesum = 0, gxkoef = 1.0;
-  HBuf<double> pin;
-  KTimer kt;
-  DBuf<double> rstage, zstage;  // fixed input/output of the captured Stokes preconditioner
-
-  Grid grid() const { return Grid{X, Y, P.xsize, P.ysize, gx.p, gy.p}; }
-  SOp<double> sop() const { return SOp<double>{X, Y, gx.p, gy.p, F.nu, F.nd, F.ro, F.gp, timestep, P.nubeg * P.strmin}; }
-};
 
 extern "C" {
 
@@ -498,8 +457,8 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
     return I2E_ERR_CAPACITY;
   }
   if ((int64_t)p->xnumx * p->ynumy > (int64_t)1 << 28) return I2E_ERR_CAPACITY;
-  if (p->densimod > 1 || p->adiabyn > 0 || p->markmod != 2) {
-    set_error("densimod>1, adiabyn>0 or markmod!=2 not supported");
+  if (p->densimod > 1 || p->adiabyn > 0 || p->markmod != 2 || p->heatmod != 0) {
+    set_error("densimod>1, adiabyn>0, markmod!=2 or heatmod!=0 (sub-stepped heat solve) not supported");
     return I2E_ERR_UNSUPPORTED;
   }
   int ndev = 0;
@@ -562,6 +521,8 @@ int i2e_destroy(i2e_ctx *c) {
   if (c->s) hipStreamSynchronize(c->s);
   for (auto e : c->kt.ev) (void)hipEventDestroy(e);
   c->kt.ev.clear();
+  adj_release(c->adj);
+  c->adj = nullptr;
   c->v32.release();
   c->v64.release();
   c->l32.release();
@@ -603,6 +564,25 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
   bool regeo = c->setup_done && o->coarse_max != c->o.coarse_max;
   c->o = *o;
   if (regeo) c->setup_done = false;
+  return I2E_OK;
+}
+
+int i2e_set_params(i2e_ctx *c, const i2e_params *p) {
+  if (!c || !p) return I2E_ERR_ARG;
+  if (p->xnumx != c->P.xnumx || p->ynumy != c->P.ynumy || p->mnumx != c->P.mnumx || p->mnumy != c->P.mnumy) {
+    set_error("set_params: grid or markers per cell differ from the context's");
+    return I2E_ERR_ARG;
+  }
+  if (p->densimod > 1 || p->adiabyn > 0 || p->markmod != 2 || p->heatmod != 0) {
+    set_error("densimod>1, adiabyn>0, markmod!=2 or heatmod!=0 (sub-stepped heat solve) not supported");
+    return I2E_ERR_UNSUPPORTED;
+  }
+  HIP_OK(hipStreamSynchronize(c->s));
+  HIP_OK(hipMemcpy(c->dP, p, sizeof(i2e_params), hipMemcpyHostToDevice));
+  c->P = *p;
+  c->need_eii = 0;
+  for (int t = 0; t < I2E_MAXTYPES; t++)
+    if (p->rock[t].ss != 0 && p->rock[t].mm != 1.0) c->need_eii = 1;
   return I2E_OK;
 }
 
@@ -1077,6 +1057,7 @@ int i2e_viterate(i2e_ctx *c, int n0) {
 int i2e_titerate(i2e_ctx *c, int n0) {
   (void)n0;
   if (!c || !c->setup_done) return I2E_ERR_ARG;
+  c->heat_solved = 0;
   auto t0 = std::chrono::steady_clock::now();
   const int X = c->X, Y = c->Y, N = c->N;
   Grid g = c->grid();
@@ -1150,6 +1131,7 @@ int i2e_titerate(i2e_ctx *c, int n0) {
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c->s));
+  c->heat_solved = 1;
   c->st.ms_heat = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return I2E_OK;
 }

@@ -1,0 +1,56 @@
+// ctx.hpp -- the context behind the opaque i2e_ctx handle of include/i2e.h,
+// shared by the translation units that implement its entry points.
+#pragma once
+#include "krylov.hpp"
+
+namespace i2e {
+struct AdjHist;               // recorded trajectory and thermal history (adjoint.hip)
+void adj_release(AdjHist *h);
+__global__ void k_copy(int n, const double *a, double *b);
+__global__ void k_heat_fields(int N, const double *ro, const double *cp, double dt, const int *bondm, double *c,
+                              int8_t *mask, double *dm);
+}  // namespace i2e
+
+using namespace i2e;
+
+struct i2e_ctx {
+  i2e_params P;
+  i2e_params *dP = nullptr;
+  int X = 0, Y = 0, N = 0;
+  int64_t Mmax = 0, M = 0;
+  hipStream_t s = nullptr;
+  DBuf<double> gx, gy, nodes, acc, work, sol1, tk2, red, xs, bs;
+  DBuf<float> mfl;
+  DBuf<int8_t> mt;
+  DBuf<int> bondm, bondn;
+  DBuf<double> bondv;
+  NodeF F;
+  Markers mk;
+  Acc A;
+  BcTab bc;
+  int need_eii = 0;
+  bool setup_done = false;
+  std::vector<GeoLevel> geo;
+  VelMG<float> v32;
+  VelMG<double> v64;
+  CellMG<float> l32;
+  CellMG<double> l64;
+  DBuf<double> bcix, bciy, bLS;  // BFBt: C^-1 on momentum rows, FP64 cell stencil
+  DBuf<int8_t> blmask;
+  ScalMG<float> h32, g32;
+  ScalMG<double> h64, g64;
+  DBuf<double> hk, hc, hdm, gk, gc, gdm, gsc, hsc;
+  DBuf<int8_t> hmask, gmask;
+  Fgmres fs, fh, fgv;
+  i2e_solver_opts o;
+  i2e_stats st;
+  double timestep = 0, timesum = 0, gxkoef = 1.0;
+  HBuf<double> pin;
+  KTimer kt;
+  DBuf<double> rstage, zstage;  // fixed input/output of the captured Stokes preconditioner
+  AdjHist *adj = nullptr;       // allocated by i2e_record
+  int heat_solved = 0;          // the last i2e_titerate ran its solve and marker update
+
+  Grid grid() const { return Grid{X, Y, P.xsize, P.ysize, gx.p, gy.p}; }
+  SOp<double> sop() const { return SOp<double>{X, Y, gx.p, gy.p, F.nu, F.nd, F.ro, F.gp, timestep, P.nubeg * P.strmin}; }
+};

@@ -200,26 +200,12 @@ __device__ void porosity_update_d(const i2e_params *p, float &por, float markk, 
   }
 }
 
-__device__ __forceinline__ void aadd(double *a, double v) { unsafeAtomicAdd(a, v); }
-
 // Scatter contributions of one marker: per corner (a,b) of its cell the ten
 // weighted sums of markmart.c:840-911, plus the cell-centre weights of nd.
 struct RonuC {
   double v[4][10];  // ro, et, cp, kt, ht, w0, tk, w1, nu*swt2, swt2 (wnu)
   double nd, wnd;
 };
-
-// segmented sum over the lanes of a wavefront whose segment ids are equal
-// and contiguous: afterwards the first lane of each segment holds its total
-__device__ __forceinline__ double seg_sum(double v, int seg, int lane) {
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double o = __shfl_down(v, off, 64);
-    const int so = __shfl_down(seg, off, 64);
-    if (lane + off < 64 && so == seg) v += o;
-  }
-  return v;
-}
 
 __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
                                                      int64_t M, Acc A, double timesum, double timestep,

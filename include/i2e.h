@@ -250,6 +250,49 @@ int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches);
 int i2e_kernel_timer_read(i2e_ctx *c, int64_t *launches, double *ms_total,
                           double *bytes_total);
 
+/* Replace the run parameters of a context (the reference reads them once
+ * from mode.t3c/init.t3c, loadmart.c:26-90, into globals).  Grid sizes and
+ * markers per cell must match the context's; the change applies from the
+ * next call on.                                                             */
+int i2e_set_params(i2e_ctx *c, const i2e_params *p);
+
+/* ---- adjoint of a peak-temperature objective (no counterpart in the
+ * reference: a sensitivity study of the same time step) --------------------
+ *
+ * J = sum_i m_i marktmax_i / sum_i m_i over silicate markers inside the
+ * grid, m_i = markro(type) (1 - markpor_i).  Parameters: h0_scale (the 26Al
+ * heating multiplier, markmart.c:758-770) and phi0, a uniform shift of every
+ * marker porosity that is positive at the start.
+ *
+ * i2e_record runs i2e_ronurecalc and then nsteps of i2e_step from the
+ * uploaded state, keeping in device memory the marker positions and
+ * temperatures after every ronurecalc and the node fields of every heat
+ * solve.  i2e_replay runs the thermal part of the same steps (ronurecalc,
+ * titerate) from the uploaded state along the recorded positions and time
+ * steps, without Stokes, gravity or advection.  i2e_adjoint_peak_t, called
+ * right after i2e_record, returns J and its gradient through that thermal
+ * chain (titerate heatmart.c:320-695, ronurecalc markmart.c:192-1019) with
+ * positions and time steps held at the recorded ones: the exact gradient of
+ * what i2e_replay computes, and of the full step whenever the flow does not
+ * depend on the parameters.  Requires melt_on = porosity_on = 0 and
+ * temperature/pressure-independent density and conductivity (markaa =
+ * markbb = markkf = markkp = 0); replay also requires frictyn above the run
+ * time (no shear heating).  Device memory: 12 B per marker and 48 B per node
+ * per recorded step.                                                        */
+typedef struct i2e_adjoint_out {
+  double J;
+  double dJ_dh0;           /* dJ / d h0_scale                                */
+  double dJ_dphi0;         /* dJ / d phi0                                    */
+  double heat_relres_max;  /* largest residual of the transposed heat solves */
+  int32_t steps, heat_iters;
+} i2e_adjoint_out;
+int i2e_record(i2e_ctx *c, int nsteps);
+int i2e_replay(i2e_ctx *c, int nsteps);
+int i2e_peak_t(i2e_ctx *c, double *J);
+int i2e_adjoint_peak_t(i2e_ctx *c, i2e_adjoint_out *out);
+/* y = A^T x for the heat rows of i2e_heat_apply_dev (device pointers).     */
+int i2e_heat_apply_t_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
+
 /* Device buffers for callers without a device allocator of their own (no
  * counterpart in the reference): allocation, release and blocking copies,
  * all through the same HIP runtime as the library itself.                   */

@@ -315,3 +315,25 @@ class Oracle:
         _check(self.L.orc_type_counts(C.byref(self.p), self._s(), ntypes, out.ctypes.data_as(C.POINTER(C.c_int32))),
                "type_counts")
         return out.reshape(C_, ntypes)
+
+
+# --------------------------------------------------------------------------- adjoint objective
+def peak_t_objective(p, hs):
+    """Mass-weighted mean peak temperature of the silicate markers inside the
+    grid, the objective of the library's adjoint (no counterpart in the
+    reference): J = sum m_i marktmax_i / sum m_i with the mass of an
+    equal-volume marker m_i = markro (1 - por_i), where por_i is the
+    porosity ronurecalc applies (0 for types 0 and >= 7, clamped to [0, 1],
+    markmart.c:340-347) and a marker counts when it lies strictly inside the
+    box, has markk > 0 and type < 50 (markmart.c:231-236)."""
+    t = hs.markt.astype(np.int64)
+    mm2 = np.where(t >= 100, t - 100, t)
+    x, y, k = hs.markx.astype(np.float64), hs.marky.astype(np.float64), hs.markk
+    sil = np.array([p.is_silicate[i] for i in range(abi.MAXTYPES)], bool)
+    ro = np.array([p.rock[i].ro for i in range(abi.MAXTYPES)])
+    sel = (x > 0) & (y > 0) & (x < p.xsize) & (y < p.ysize) & (k > 0) & (t < 50)
+    sel &= sil[np.clip(mm2, 0, abi.MAXTYPES - 1)]
+    por = np.clip(hs.markpor.astype(np.float64), 0.0, 1.0)
+    por = np.where((mm2 == 0) | (mm2 >= 7), 0.0, por)
+    m = ro[mm2[sel]] * (1.0 - por[sel])
+    return float(np.sum(m * hs.marktmax[sel].astype(np.float64)) / np.sum(m))

@@ -402,3 +402,33 @@ def test_residual_diagnostics_match_assembled_rows(orc):
     assert got["stokes"] == pytest.approx(np.sqrt(np.mean(es ** 2)) / (vrange / mindx ** 2), rel=1e-9)
     assert got["bond"] == pytest.approx(np.sqrt(np.mean(eb ** 2)) / vrange, rel=1e-9)
     assert got["stokes"] > 1e3 * exact["stokes"]
+
+
+def test_peak_t_objective_closed_forms(orc):
+    """The adjoint's objective: mass-weighted mean marktmax of silicate markers
+    inside the box.  Closed forms on two-valued fields: equal porosity cancels
+    from the weights; unequal porosity weights by (1 - phi); iron, air and
+    markers outside the box do not count."""
+    cfg = dict(configs.CONFIGS["c4"], n=31)
+    p, hs = configs.make_params(cfg), configs.make_state(cfg)
+    sil = hs.markt == configs.SILICATE
+    left = hs.markx < cfg["size"] / 2
+    nL, nR = int(np.sum(sil & left)), int(np.sum(sil & ~left))
+    assert nL > 10 and nR > 10
+    hs.marktmax[:] = np.float32(1234.0)
+    assert orc.peak_t_objective(p, hs) == pytest.approx(1234.0, rel=1e-14)
+    a, b = 600.0, 900.0
+    hs.marktmax[:] = np.where(left, np.float32(a), np.float32(b))
+    hs.markpor[sil] = np.float32(0.375)
+    assert orc.peak_t_objective(p, hs) == pytest.approx((nL * a + nR * b) / (nL + nR), rel=1e-13)
+    fL, fR = 0.25, 0.5
+    hs.markpor[sil] = np.where(left[sil], np.float32(fL), np.float32(fR))
+    want = (nL * (1 - fL) * a + nR * (1 - fR) * b) / (nL * (1 - fL) + nR * (1 - fR))
+    assert orc.peak_t_objective(p, hs) == pytest.approx(want, rel=1e-13)
+    hs.marktmax[~sil] = np.float32(5000.0)
+    assert orc.peak_t_objective(p, hs) == pytest.approx(want, rel=1e-13)
+    i = int(np.flatnonzero(sil & left)[0])
+    hs.marktmax[i] = np.float32(1e6)
+    hs.markx[i] = np.float32(-1.0)
+    wantL = ((nL - 1) * (1 - fL) * a + nR * (1 - fR) * b) / ((nL - 1) * (1 - fL) + nR * (1 - fR))
+    assert orc.peak_t_objective(p, hs) == pytest.approx(wantL, rel=1e-13)
