@@ -20,6 +20,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the roofline kernel, measured by two rocprofv3 --pmc
+# runs (FETCH_SIZE, WRITE_SIZE) and scaled by the same counters on a plain
+# 1 GiB copy; written by `bench.py --pmc-summary` (counters cannot be read
+# inside the timed run).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gsdot_501x501_pmc_traffic.md")
 
 
 def log(*a):
@@ -48,14 +53,64 @@ def cpu_baseline(sample_steps=1):
 
 # algorithmic-byte roofline kernels (HBM bound), selectable with --roofline-kernel
 ROOFLINE_KERNELS = {
-    "gsdot": "k_gsdot (Gram-Schmidt projection pass of FGMRES)",
-    "gsaxpy": "k_gsaxpy (Gram-Schmidt update pass of FGMRES)",
+    "gsdot": "k_gsdot (Gram-Schmidt projection pass of the Stokes FGMRES)",
+    "gsaxpy": "k_gsaxpy (Gram-Schmidt update pass of the Stokes FGMRES)",
     "vsmoothc": "k_sjac (coarse-level velocity smoother colour step)",
     "vsmooth0": "k_vgs (finest-level velocity smoother colour step)",
     "stokes_apply": "k_stokes_apply (matrix-free Stokes operator)",
     "advect": "k_movemark (RK4 marker advection)",
     "p2g": "k_ronu_markers (marker-to-node scatter)",
 }
+
+
+def stokes_gs_grid(n_nodes):
+    """Grid size (threads) of a Gram-Schmidt launch of the Stokes FGMRES:
+    ceil(3N / 1024) blocks of 256 threads."""
+    return (3 * n_nodes + 1023) // 1024 * 256
+
+
+def read_traffic(path=TRAFFIC_FILE):
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def pmc_summary(fetch_dir, write_dir, kernel="k_gsdot", n_nodes=501 * 501, copy_bytes=None):
+    """Markdown summary of two rocprofv3 --pmc runs of
+    `bench.py --no-cpu --steps 1 --warmup 0 --copy-calibration 1024`:
+    FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the Stokes-size launches of
+    `kernel`, scaled by what the same counters read for the plain copy."""
+    import csv
+
+    def load(d):
+        rows = list(csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
+        cal = [float(r["Counter_Value"]) * 1024 for r in rows if "k_dev_copy" in r["Kernel_Name"]]
+        grid = stokes_gs_grid(n_nodes)
+        ker = [float(r["Counter_Value"]) * 1024 for r in rows
+               if ("::%s(" % kernel) in r["Kernel_Name"] and int(r["Grid_Size"]) == grid]
+        return cal, ker
+    cf, kf = load(fetch_dir)
+    cw, kw = load(write_dir)
+    if not (cf and cw and kf and kw):
+        raise SystemExit("pmc summary: calibration copy or kernel dispatches missing")
+    nb = copy_bytes
+    fetch_factor = nb / (sum(cf) / len(cf))
+    write_factor = nb / (sum(cw) / len(cw))
+    f_raw, w_raw = sum(kf) / len(kf), sum(kw) / len(kw)
+    traffic = f_raw * fetch_factor + w_raw * write_factor
+    lines = [
+        "calibration copy: %d bytes read + %d written per dispatch; FETCH_SIZE read %.4g B (x%.4f), "
+        "WRITE_SIZE %.4g B (x%.4f), means of %d / %d dispatches" % (nb, nb, sum(cf) / len(cf), fetch_factor,
+                                                                   sum(cw) / len(cw), write_factor, len(cf), len(cw)),
+        "%s, Stokes-size dispatches (grid %d threads): %d / %d; raw FETCH %.4g B, raw WRITE %.4g B per dispatch"
+        % (kernel, stokes_gs_grid(n_nodes), len(kf), len(kw), f_raw, w_raw),
+        "traffic_bytes_per_launch: %.6g" % traffic,
+    ]
+    return "\n".join(lines)
 
 
 def run(steps, warmup, skip_cpu, rk):
@@ -131,7 +186,8 @@ def run(steps, warmup, skip_cpu, rk):
         "kernels_standalone_ms": {"advect_rk4": round(ms_adv, 4), "p2g_scatter": round(ms_p2g, 4),
                                   "stokes_apply_fp64": round(ms_sap, 4)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": ROOFLINE_KERNELS[rk],
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic() if rk == "gsdot" else None,
+                     "kernel": ROOFLINE_KERNELS[rk],
                      "launches": launches, "avg_us": avg_ms * 1e3,
                      "bytes_per_launch": bytes_tot / max(launches, 1)},
         "cpu_baseline": None,
@@ -189,7 +245,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-kernel", default="gsdot", choices=sorted(ROOFLINE_KERNELS))
     ap.add_argument("--ensemble", type=int, default=0, help="run the ensemble measurement with this many bodies")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device three times (PMC calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
     a = ap.parse_args()
+    if a.pmc_summary:
+        print(pmc_summary(*a.pmc_summary, copy_bytes=a.copy_calibration * 2 ** 20 or 2 ** 30))
+        return
+    if a.copy_calibration:
+        from i2elvis_planet_amd import DeviceArray
+        n = a.copy_calibration * 2 ** 20 // 8
+        src, dst = DeviceArray(n), DeviceArray(n)
+        for _ in range(3):
+            dst.copy_from(src)
+        src.free()
+        dst.free()
+        log("calibration copy: 3 x %d bytes" % (8 * n))
     if a.gpus != 1:
         print("bench.py runs one GPU only: the multi-GPU decomposition of this step is not built "
               "(see the sharding notes); use --gpus 1", file=sys.stderr)

@@ -32,6 +32,12 @@ static inline dim3 nb2(int X, int Y) { return dim3((Y + 255) / 256, X); }
 __global__ void k_copy(int n, const double *a, double *b) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) b[k] = a[k];
 }
+// plain 16-byte copy, one read and one write of every element (the
+// reference size for scaling memory-traffic counters)
+__global__ void k_dev_copy(size_t n16, const uint4 *__restrict__ a, uint4 *__restrict__ b) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n16; k += (size_t)gridDim.x * blockDim.x)
+    b[k] = a[k];
+}
 __global__ void k_fill(int n, double v, double *a) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) a[k] = v;
 }
@@ -482,7 +488,7 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
     return s_;
   };
   if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return fail(I2E_ERR_HIP);
-  c->fs.kt = c->fh.kt = c->fgv.kt = &c->kt;
+  c->fs.kt = &c->kt;  // Gram-Schmidt launches are timed for the Stokes solver only
   c->v32.kt = &c->kt;
   c->v64.kt = &c->kt;
   if (hipMalloc((void **)&c->dP, sizeof(i2e_params)) != hipSuccess) return fail(I2E_ERR_HIP);
@@ -1325,6 +1331,16 @@ int i2e_copy_to_dev(void *dst_dev, const void *src_host, size_t bytes) {
 int i2e_copy_to_host(void *dst_host, const void *src_dev, size_t bytes) {
   if ((!dst_host || !src_dev) && bytes) return I2E_ERR_ARG;
   if (bytes) HIP_OK(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost));
+  return I2E_OK;
+}
+
+int i2e_dev_copy(void *dst_dev, const void *src_dev, size_t bytes) {
+  if (bytes % 16 || ((uintptr_t)dst_dev | (uintptr_t)src_dev) % 16) return I2E_ERR_ARG;
+  if (!bytes) return I2E_OK;
+  if (!dst_dev || !src_dev) return I2E_ERR_ARG;
+  hipLaunchKernelGGL(k_dev_copy, dim3(8192), dim3(256), 0, 0, bytes / 16, (const uint4 *)src_dev, (uint4 *)dst_dev);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipDeviceSynchronize());
   return I2E_OK;
 }
 

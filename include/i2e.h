@@ -239,7 +239,7 @@ int i2e_stokes_residuals(i2e_ctx *c, i2e_resid *out);
  * counterpart in the reference), for roofline figures taken inside a normal
  * run: i2e_kernel_timer selects the kernel ("advect" = movemark's marker
  * pass, "p2g" = ronurecalc's marker scatter, "gsdot" / "gsaxpy" = the
- * Gram-Schmidt projection / update passes of the Krylov solvers,
+ * Gram-Schmidt projection / update passes of the Stokes FGMRES,
  * "vsmooth0" / "vsmoothc" = one colour step of the velocity smoother on the
  * finest / a coarse multigrid level, "stokes_apply", "precond" = one
  * whole Stokes preconditioner application (bytes not counted); NULL or ""
@@ -300,6 +300,10 @@ int i2e_dev_alloc(size_t bytes, void **ptr);
 int i2e_dev_free(void *ptr);
 int i2e_copy_to_dev(void *dst_dev, const void *src_host, size_t bytes);
 int i2e_copy_to_host(void *dst_host, const void *src_dev, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (k_dev_copy: one read and
+ * one write per 16-byte element; bytes and both addresses multiples of 16),
+ * the known-size reference for scaling rocprof's FETCH_SIZE / WRITE_SIZE.   */
+int i2e_dev_copy(void *dst_dev, const void *src_dev, size_t bytes);
 
 #ifdef __cplusplus
 }
