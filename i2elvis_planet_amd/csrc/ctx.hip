@@ -453,6 +453,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->coarse_max = 8;
   o->schur = I2E_SCHUR_BFBT;
   o->graphs = 0;
+  o->orth = I2E_ORTH_LAGGED;
   return I2E_OK;
 }
 
@@ -564,11 +565,13 @@ int i2e_destroy(i2e_ctx *c) {
 
 int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
   if (!c || !o) return I2E_ERR_ARG;
-  if (o->restart < 1 || o->restart > 400 || o->maxit < 1 || o->nsmooth < 0 || o->coarse_max < 2 ||
+  if (o->restart < 1 || o->restart > 400 || o->maxit < 1 || o->nsmooth < 0 || o->nsmooth_coarse < 0 ||
+      o->coarse_max < 2 || (o->orth != I2E_ORTH_LAGGED && o->orth != I2E_ORTH_DGKS) ||
       (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS))
     return I2E_ERR_ARG;
   bool regeo = c->setup_done && o->coarse_max != c->o.coarse_max;
   c->o = *o;
+  c->fs.orth = c->fh.orth = c->fgv.orth = o->orth;
   if (regeo) c->setup_done = false;
   return I2E_OK;
 }
@@ -909,9 +912,11 @@ static int stokes_prepare(i2e_ctx *c) {
   hipLaunchKernelGGL(k_stokes_pack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->F.pr, c->F.vx, c->F.vy, c->xs.p);
   ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
   c->v64.nsmooth = c->o.nsmooth;
+  c->v64.nsmooth_c = c->o.nsmooth_coarse;
   if (mixed(c)) {
     ST_OK(c->v32.refresh_from(c->v64, c->s));
     c->v32.nsmooth = c->o.nsmooth;
+    c->v32.nsmooth_c = c->o.nsmooth_coarse;
   }
   if (c->o.schur == I2E_SCHUR_BFBT) {
     const VelLevel<double> &f0 = c->v64.L[0];
@@ -921,9 +926,11 @@ static int stokes_prepare(i2e_ctx *c) {
                        c->bciy.p, c->bLS.p, c->blmask.p);
     ST_OK(c->l64.refresh(c->bLS.p, c->blmask.p, c->s));
     c->l64.nsmooth = c->o.nsmooth;
+    c->l64.nsmooth_c = c->o.nsmooth_coarse;
     if (mixed(c)) {
       ST_OK(c->l32.refresh_from(c->l64, c->s));
       c->l32.nsmooth = c->o.nsmooth;
+      c->l32.nsmooth_c = c->o.nsmooth_coarse;
     }
   }
   return I2E_OK;

@@ -125,6 +125,192 @@ __global__ __launch_bounds__(RB) void k_gsaxpy(int n, int nv, const double *__re
   if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
+// Lagged CGS2 (the second projection of each basis vector is fused into the
+// next iteration's pass, normalisation by the Pythagorean identity; after
+// Swirydowicz, Langou, Ananthan, Yang, Thomas, "Low synchronization
+// Gram-Schmidt and GMRES algorithms", 2020).  One pass forms, per block,
+//   a_i = <V_i, u>, b_i = <V_i, w> (i < nv), <u,u>, <u,w>, <w,w>
+// into part[q*G + blk], q = 0..2nv+2 (a, then b, then the three scalars).
+__global__ __launch_bounds__(RB) void k_gsdot2(int n, int nv, const double *__restrict__ V, size_t ld,
+                                               const double *__restrict__ u, const double *__restrict__ w,
+                                               double *__restrict__ part) {
+  extern __shared__ double sh[];  // [4][2nv + 3]
+  const int G = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6, S = 2 * nv + 3;
+  const size_t base = (size_t)blockIdx.x * CH + threadIdx.x;
+  double ue[EPT], we[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    const size_t k = base + (size_t)e * RB;
+    ue[e] = k < (size_t)n ? u[k] : 0.0;
+    we[e] = k < (size_t)n ? w[k] : 0.0;
+  }
+  int i = 0;
+  for (; i + 2 <= nv; i += 2) {
+    double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const size_t k = base + (size_t)e * RB;
+      if (k < (size_t)n) {
+        const double v0 = V[(size_t)i * ld + k], v1 = V[(size_t)(i + 1) * ld + k];
+        a0 += v0 * ue[e];
+        b0 += v0 * we[e];
+        a1 += v1 * ue[e];
+        b1 += v1 * we[e];
+      }
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    b0 = wave_sum(b0);
+    b1 = wave_sum(b1);
+    if (lane == 0) {
+      sh[wv * S + i] = a0;
+      sh[wv * S + i + 1] = a1;
+      sh[wv * S + nv + i] = b0;
+      sh[wv * S + nv + i + 1] = b1;
+    }
+  }
+  for (; i < nv; i++) {
+    double a0 = 0, b0 = 0;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const size_t k = base + (size_t)e * RB;
+      if (k < (size_t)n) {
+        const double v0 = V[(size_t)i * ld + k];
+        a0 += v0 * ue[e];
+        b0 += v0 * we[e];
+      }
+    }
+    a0 = wave_sum(a0);
+    b0 = wave_sum(b0);
+    if (lane == 0) {
+      sh[wv * S + i] = a0;
+      sh[wv * S + nv + i] = b0;
+    }
+  }
+  double uu = 0, uw = 0, ww = 0;
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    uu += ue[e] * ue[e];
+    uw += ue[e] * we[e];
+    ww += we[e] * we[e];
+  }
+  uu = wave_sum(uu);
+  uw = wave_sum(uw);
+  ww = wave_sum(ww);
+  if (lane == 0) {
+    sh[wv * S + 2 * nv] = uu;
+    sh[wv * S + 2 * nv + 1] = uw;
+    sh[wv * S + 2 * nv + 2] = ww;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < S; q += RB)
+    part[(size_t)q * G + blockIdx.x] = ((sh[q] + sh[S + q]) + sh[2 * S + q]) + sh[3 * S + q];
+}
+
+// Coefficients of the fused update from r = [a(nv), b(nv), uu, uw, ww]:
+//   beta = sqrt(uu - a.a)       (norm of u after its second projection)
+//   h    = (uw - a.b) / beta    (<v, w> for the finished v = (u - V a) / beta)
+//   c    = b - a h / beta       (so that w - V c - (h/beta) u = w - V b - h v)
+// cf = [a(nv), c(nv), 1/beta, h/beta]; out = [beta, h].  beta = 0 flags a
+// breakdown (u in the span of V).
+__global__ void k_gscoef(int nv, const double *__restrict__ r, double *__restrict__ cf, double *__restrict__ out) {
+  __shared__ double sa[RB], sb[RB];
+  double aa = 0, ab = 0;
+  for (int i = threadIdx.x; i < nv; i += RB) {
+    aa += r[i] * r[i];
+    ab += r[i] * r[nv + i];
+  }
+  sa[threadIdx.x] = aa;
+  sb[threadIdx.x] = ab;
+  __syncthreads();
+  for (int o = RB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sa[threadIdx.x] += sa[threadIdx.x + o];
+      sb[threadIdx.x] += sb[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  const double b2 = r[2 * nv] - sa[0];
+  const double beta = b2 > 0 ? sqrt(b2) : 0.0;
+  const double h = beta > 0 ? (r[2 * nv + 1] - sb[0]) / beta : 0.0;
+  const double t = beta > 0 ? h / beta : 0.0;
+  for (int i = threadIdx.x; i < nv; i += RB) {
+    cf[i] = r[i];
+    cf[nv + i] = r[nv + i] - r[i] * t;
+  }
+  if (threadIdx.x == 0) {
+    cf[2 * nv] = beta > 0 ? 1.0 / beta : 0.0;
+    cf[2 * nv + 1] = t;
+    out[0] = beta;
+    out[1] = h;
+  }
+}
+
+// Fused update with cf from k_gscoef: vout = (u - V a) / beta (the finished
+// basis vector), wout = w - V c - (h/beta) u (w projected on the finished
+// basis, once), and part[blk] = <wout, wout> over this block's chunk.
+// vout may alias u; wout may alias w.
+__global__ __launch_bounds__(RB) void k_gsaxpy2(int n, int nv, const double *__restrict__ V, size_t ld,
+                                                const double *__restrict__ cf, const double *u, const double *w,
+                                                double *vout, double *wout, double *__restrict__ part) {
+  __shared__ double sh[RB / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const size_t base = (size_t)blockIdx.x * CH + threadIdx.x;
+  double s1[EPT], s2[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; e++) s1[e] = s2[e] = 0.0;
+  int i = 0;
+  for (; i + 2 <= nv; i += 2) {
+    const double a0 = cf[i], a1 = cf[i + 1], c0 = cf[nv + i], c1 = cf[nv + i + 1];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const size_t k = base + (size_t)e * RB;
+      if (k < (size_t)n) {
+        const double v0 = V[(size_t)i * ld + k], v1 = V[(size_t)(i + 1) * ld + k];
+        s1[e] = (s1[e] + a0 * v0) + a1 * v1;
+        s2[e] = (s2[e] + c0 * v0) + c1 * v1;
+      }
+    }
+  }
+  for (; i < nv; i++) {
+    const double a0 = cf[i], c0 = cf[nv + i];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+      const size_t k = base + (size_t)e * RB;
+      if (k < (size_t)n) {
+        const double v0 = V[(size_t)i * ld + k];
+        s1[e] += a0 * v0;
+        s2[e] += c0 * v0;
+      }
+    }
+  }
+  const double ib = cf[2 * nv], t = cf[2 * nv + 1];
+  double acc = 0;
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    const size_t k = base + (size_t)e * RB;
+    if (k < (size_t)n) {
+      const double uk = u[k], wk = w[k];
+      const double wn = (wk - s2[e]) - t * uk;
+      vout[k] = (uk - s1[e]) * ib;
+      wout[k] = wn;
+      acc += wn * wn;
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) sh[wv] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// v *= 1 / sqrt(*nrm2) (nrm2 on the device; no-op when it is not positive)
+__global__ void k_scale_norm(int n, const double *__restrict__ nrm2, double *__restrict__ v) {
+  const double q = *nrm2;
+  if (!(q > 0)) return;
+  const double f = 1.0 / sqrt(q);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) v[k] *= f;
+}
+
 // out[i] = sum_blk part[i*G + blk] in a fixed order, i < nv (one block per i)
 __global__ void k_reduce_parts(int nv, int G, const double *part, double *out) {
   __shared__ double sh[RB];
@@ -169,9 +355,9 @@ int Fgmres::init(int n_, int m_) {
   ST_OK(V.alloc(ld * (m + 1)));
   ST_OK(Z.alloc(ld * m));
   ST_OK(w.alloc(ld));
-  ST_OK(part.alloc((size_t)gs_blocks(n) * (m + 2)));
-  ST_OK(hd.alloc(4 * (m + 2)));
-  ST_OK(hh.alloc(4 * (m + 2)));
+  ST_OK(part.alloc((size_t)gs_blocks(n) * (2 * m + 4)));
+  ST_OK(hd.alloc(8 * (m + 4)));
+  ST_OK(hh.alloc(8 * (m + 4)));
   return I2E_OK;
 }
 
@@ -204,6 +390,190 @@ int Fgmres::norm(const double *x, double *out, hipStream_t s) {
 
 int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double rtol, int maxit, hipStream_t s,
                   int *iters, double *relres) {
+  return orth == ORTH_DGKS ? solve_dgks(A, M, b, x, rtol, maxit, s, iters, relres)
+                           : solve_lagged(A, M, b, x, rtol, maxit, s, iters, relres);
+}
+
+// One fused lagged-CGS2 step on the pending vector u (slot j of V, scaled to
+// about unit norm) and w = A z_j: finishes u into v_j against V[0..j), and
+// projects w once on V[0..j]; w' goes to wout.  Host receives (after the
+// caller's synchronisation) a(j), b(j), beta, h, <w',w'> at hh[0 ..].
+int Fgmres::lagged_pass(int j, const double *w_, double *wout, hipStream_t s) {
+  const int G = gs_blocks(n);
+  double *u = V.p + (size_t)j * ld;
+  const size_t R = 0, CF = 2 * (size_t)m + 4, OUT = 4 * (size_t)m + 8;  // offsets in hd
+  const double bytes_dot = 8.0 * ((double)j * n + 2.0 * n + (2.0 * j + 3) * G);
+  const double bytes_axpy = 8.0 * ((double)j * n + 4.0 * n + G);
+  if (kt) kt->begin(K_GSDOT, s);
+  hipLaunchKernelGGL(k_gsdot2, dim3(G), dim3(RB), sizeof(double) * 4 * (2 * j + 3), s, n, j, V.p, ld, u, w_,
+                     part.p);
+  if (kt) kt->end(K_GSDOT, s, bytes_dot);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(2 * j + 3), dim3(RB), 0, s, 2 * j + 3, G, part.p, hd.p + R);
+  hipLaunchKernelGGL(k_gscoef, dim3(1), dim3(RB), 0, s, j, hd.p + R, hd.p + CF, hd.p + OUT);
+  if (kt) kt->begin(K_GSAXPY, s);
+  hipLaunchKernelGGL(k_gsaxpy2, dim3(G), dim3(RB), 0, s, n, j, V.p, ld, hd.p + CF, u, w_, u, wout, part.p);
+  if (kt) kt->end(K_GSAXPY, s, bytes_axpy);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(RB), 0, s, 1, G, part.p, hd.p + OUT + 2);
+  // the pending vector of the next step, scaled to unit norm
+  hipLaunchKernelGGL(k_scale_norm, dim3(1024), dim3(256), 0, s, n, hd.p + OUT + 2, wout);
+  HIP_OK(hipMemcpyAsync(hh.p, hd.p + R, sizeof(double) * (2 * j), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(hh.p + 2 * j, hd.p + OUT, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+  return I2E_OK;
+}
+
+// Right-preconditioned FGMRES with lagged CGS2.  Step j preconditions the
+// pending vector u_j (projected once, unit norm), then one pass finishes
+// u_j into v_j and projects w = A z_j once; column j-1 of H becomes final
+// here (A z_{j-1} = V b + h v_{j-1} + sigma_j (beta v_j + V a)), column j
+// one step later.  A step whose estimated residual meets the tolerance
+// finishes its pending vector with one extra projection pass instead of a
+// further preconditioner application.
+int Fgmres::solve_lagged(const Op &A, const Op &M, const double *b, double *x, double rtol, int maxit,
+                         hipStream_t s, int *iters, double *relres) {
+  const unsigned gb = 1024;
+  double bn;
+  ST_OK(norm(b, &bn, s));
+  *iters = 0;
+  *relres = 0;
+  reorth = 0;
+  if (bn == 0) {
+    HIP_OK(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    return I2E_OK;
+  }
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), col(m + 1);
+  const bool verbose = std::getenv("I2E_VERBOSE") != nullptr;
+  int its = 0;
+  double rr = 1;
+  auto rotate_col = [&](double *c, int j) {  // previous rotations on column j (entries 0..j)
+    for (int i = 0; i < j; i++) {
+      const double a = c[i], d = c[i + 1];
+      c[i] = cs[i] * a + sn[i] * d;
+      c[i + 1] = -sn[i] * a + cs[i] * d;
+    }
+  };
+  // finish the pending vector u (slot j+1) against V[0..j] without a new
+  // step: a' and beta' for column j (into colj, entries 0..j+1)
+  auto finish = [&](int j, double sigma, double *colj) -> int {
+    const int nv = j + 1;
+    const int G = gs_blocks(n);
+    double *u = V.p + (size_t)(j + 1) * ld;
+    hipLaunchKernelGGL(k_gsdot, dim3(G), dim3(RB), sizeof(double) * 4 * (nv + 1), s, n, nv, V.p, ld, u, part.p);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nv + 1), dim3(RB), 0, s, nv + 1, G, part.p, hd.p);
+    HIP_OK(hipMemcpyAsync(hh.p, hd.p, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    double aa = 0;
+    for (int i = 0; i < nv; i++) {
+      colj[i] += sigma * hh.p[i];
+      aa += hh.p[i] * hh.p[i];
+    }
+    const double b2 = hh.p[nv] - aa;
+    colj[j + 1] = sigma * (b2 > 0 ? std::sqrt(b2) : 0.0);
+    return I2E_OK;
+  };
+  while (true) {
+    // true residual
+    ST_OK(A(x, w.p));
+    hipLaunchKernelGGL(k_bminus, dim3(gb), dim3(256), 0, s, n, b, w.p);
+    double beta;
+    ST_OK(norm(w.p, &beta, s));
+    rr = beta / bn;
+    if (verbose) fprintf(stderr, "[fgmres n=%d] its %d true relres %.3e (||b|| %.3e)\n", n, its, rr, bn);
+    if (rr <= rtol || its >= maxit || !(beta > 0)) break;
+    hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, s, n, w.p, 1.0 / beta, V.p);
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;          // finished columns
+    double sigma = 0;   // norm of the pending vector before its scaling
+    for (int j = 0; j < m && its < maxit; j++) {
+      double *vj = V.p + (size_t)j * ld, *zj = Z.p + (size_t)j * ld, *un = V.p + (size_t)(j + 1) * ld;
+      ST_OK(M(vj, zj));
+      ST_OK(A(zj, w.p));
+      its++;
+      if (j == 0) {
+        // v_0 is finished: one projection of w, the result is pending u_1
+        ST_OK(gs_pass(1, w.p, 0, s));
+        hipLaunchKernelGGL(k_scale_norm, dim3(1024), dim3(256), 0, s, n, hd.p + 2, w.p);
+        hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, s, n, w.p, 1.0, un);
+        HIP_OK(hipMemcpyAsync(hh.p, hd.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        H[0] = hh.p[0];
+        sigma = std::sqrt(hh.p[2] > 0 ? hh.p[2] : 0.0);
+      } else {
+        ST_OK(lagged_pass(j, w.p, un, s));
+        HIP_OK(hipStreamSynchronize(s));
+        const double *a = hh.p, *bb = hh.p + j;
+        const double bet = hh.p[2 * j], h = hh.p[2 * j + 1], ww = hh.p[2 * j + 2];
+        // column j-1 is final: A z_{j-1} = ... + sigma (beta v_j + V a)
+        for (int i = 0; i < j; i++) H[(size_t)i * m + (j - 1)] += sigma * a[i];
+        H[(size_t)j * m + (j - 1)] = sigma * bet;
+        for (int i = 0; i <= j; i++) col[i] = H[(size_t)i * m + (j - 1)];
+        rotate_col(col.data(), j - 1);
+        {
+          const double p = col[j - 1], q = col[j], den = std::hypot(p, q);
+          cs[j - 1] = den > 0 ? p / den : 1.0;
+          sn[j - 1] = den > 0 ? q / den : 0.0;
+          col[j - 1] = den;
+          col[j] = 0;
+          g[j] = -sn[j - 1] * g[j - 1];
+          g[j - 1] = cs[j - 1] * g[j - 1];
+        }
+        for (int i = 0; i <= j; i++) H[(size_t)i * m + (j - 1)] = col[i];
+        k = j;
+        // column j: V b + h v_j, pending part follows
+        for (int i = 0; i < j; i++) H[(size_t)i * m + j] = bb[i];
+        H[(size_t)j * m + j] = h;
+        sigma = std::sqrt(ww > 0 ? ww : 0.0);
+        if (!(bet > 0)) break;  // u_j was in the span of V: breakdown
+        if (std::fabs(g[j]) / bn <= rtol) break;
+      }
+      // estimated residual with column j finished by the pending norm alone
+      for (int i = 0; i <= j; i++) col[i] = H[(size_t)i * m + j];
+      col[j + 1] = sigma;
+      rotate_col(col.data(), j);
+      const double est = std::fabs(col[j + 1] / std::hypot(col[j], col[j + 1]) * g[j]);
+      if (verbose && (j % 10 == 9)) fprintf(stderr, "[fgmres n=%d]   it %d est %.3e\n", n, its, est / bn);
+      const bool last = (j + 1 == m) || its >= maxit || !(sigma > 0);
+      if (est / bn <= rtol || last) {
+        // finish u_{j+1} with one projection pass and close column j exactly
+        for (int i = 0; i <= j + 1; i++) col[i] = i <= j ? H[(size_t)i * m + j] : 0.0;
+        if (sigma > 0) ST_OK(finish(j, sigma, col.data()));
+        rotate_col(col.data(), j);
+        const double p = col[j], q = col[j + 1], den = std::hypot(p, q);
+        const double c0 = den > 0 ? p / den : 1.0, s0 = den > 0 ? q / den : 0.0;
+        const double gj1 = -s0 * g[j];
+        if (std::fabs(gj1) / bn <= rtol || last) {
+          for (int i = 0; i <= j + 1; i++) H[(size_t)i * m + j] = col[i];
+          H[(size_t)j * m + j] = den;
+          H[(size_t)(j + 1) * m + j] = 0;
+          cs[j] = c0;
+          sn[j] = s0;
+          g[j + 1] = gj1;
+          g[j] = c0 * g[j];
+          k = j + 1;
+          break;
+        }
+        reorth++;  // a finishing pass that did not end the cycle
+      }
+    }
+    // back substitution and x += Z y
+    for (int i = k - 1; i >= 0; i--) {
+      double t = g[i];
+      for (int q = i + 1; q < k; q++) t -= H[(size_t)i * m + q] * y[q];
+      y[i] = H[(size_t)i * m + i] != 0 ? t / H[(size_t)i * m + i] : 0.0;
+    }
+    for (int i = 0; i < k; i++) hh.p[i] = y[i];
+    HIP_OK(hipMemcpyAsync(hd.p, hh.p, sizeof(double) * k, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_multiaxpy, dim3(gb), dim3(256), 0, s, n, k, Z.p, ld, hd.p, 1.0, x);
+    HIP_OK(hipStreamSynchronize(s));
+  }
+  *iters = its;
+  *relres = rr;
+  return (rr <= rtol) ? I2E_OK : I2E_ERR_NOCONV;
+}
+
+int Fgmres::solve_dgks(const Op &A, const Op &M, const double *b, double *x, double rtol, int maxit,
+                       hipStream_t s, int *iters, double *relres) {
   const unsigned gb = 1024;
   double bn;
   ST_OK(norm(b, &bn, s));

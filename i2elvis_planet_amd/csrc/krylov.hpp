@@ -30,13 +30,20 @@ struct Fgmres {
   DBuf<double> V, Z, w, part, hd;
   HBuf<double> hh;
   KTimer *kt = nullptr;  // optional per-launch timing of the Gram-Schmidt kernels
-  int reorth = 0;        // second Gram-Schmidt passes in the last solve
+  int reorth = 0;        // extra Gram-Schmidt passes in the last solve
+  int orth = 0;          // ORTH_LAGGED or ORTH_DGKS
+  enum { ORTH_LAGGED = 0, ORTH_DGKS = 1 };
   int init(int n, int m);
   int gs_pass(int nv, double *w, int off, hipStream_t s);
+  int lagged_pass(int j, const double *w, double *wout, hipStream_t s);
   int norm(const double *x, double *out, hipStream_t s);
   // solves A x = b to ||b - A x|| <= rtol ||b||; x holds the initial guess
   int solve(const Op &A, const Op &M, const double *b, double *x, double rtol, int maxit, hipStream_t s, int *iters,
             double *relres);
+  int solve_lagged(const Op &A, const Op &M, const double *b, double *x, double rtol, int maxit, hipStream_t s,
+                   int *iters, double *relres);
+  int solve_dgks(const Op &A, const Op &M, const double *b, double *x, double rtol, int maxit, hipStream_t s,
+                 int *iters, double *relres);
   void release();
 };
 

@@ -1137,8 +1137,9 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   const double coef = l == 0 ? 6.0 : 100.0;
   const double bytes = sizeof(T) * (6.0 + coef) * v.N + 2.0 * v.N;
   const MapView none{nullptr, nullptr, nullptr, nullptr};
+  const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
   TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr, 0,
-                none, none, none, none, 1, nsmooth, 0};
+                none, none, none, none, 1, ns, 0};
   auto launch = [&](const TileArgs<T> &a) {
     if (kt) kt->begin(kid, s);
     if (l == 0)
@@ -1829,7 +1830,7 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
     A.ainv = ainv.p;
     A.fin = v.f.p;
     A.uout = v.u.p;
-    A.nsmooth = nsmooth;
+    A.nsmooth = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
     hipLaunchKernelGGL(k_ctail<T>, dim3(1), dim3(CT_THR), 0, s, A);
     return I2E_OK;
   }
@@ -1840,7 +1841,8 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
   }
   const dim3 TG((v.Y + TS - 1) / TS, (v.X + TS - 1) / TS), TBK(TB);
   const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
-  CellArgs<T> A{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, nullptr, 0, none, none, 1, nsmooth, 0};
+  const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
+  CellArgs<T> A{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, nullptr, 0, none, none, 1, ns, 0};
   hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, A);
   std::swap(v.u.p, v.u2.p);
   CellLevel<T> &c = L[l + 1];
@@ -1849,7 +1851,7 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
   hipLaunchKernelGGL(k_crestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mx), view(v.my),
                      (const int8_t *)c.mask.p, (const T *)v.r.p, c.f.p);
   ST_OK(vcycle(l + 1, s));
-  CellArgs<T> P{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, nsmooth, 1};
+  CellArgs<T> P{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, ns, 1};
   hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, P);
   std::swap(v.u.p, v.u2.p);
   return I2E_OK;

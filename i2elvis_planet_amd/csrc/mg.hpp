@@ -114,6 +114,7 @@ struct VelMG {
   DBuf<int> overflow;       // Galerkin entries that fell outside the 5x5 window (must stay 0)
   double mu = 0, dt = 0;
   int nsmooth = 2;
+  int nsmooth_c = 0;        // sweeps on the coarser levels (0: nsmooth)
   KTimer *kt = nullptr;     // optional per-launch timing of the smoothers
   int setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,
             const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,
@@ -184,6 +185,7 @@ struct CellMG {
   DBuf<double> dense, ainv;
   DBuf<int> overflow;
   int nsmooth = 2;
+  int nsmooth_c = 0;        // sweeps on the coarser levels (0: nsmooth)
   int setup(const std::vector<double> &x0, const std::vector<double> &y0, int coarse_max);
   // fine stencil (FP64, 9 x N0) and mask given on device; forms the
   // Galerkin hierarchy and the coarsest inverse in FP64

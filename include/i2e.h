@@ -136,6 +136,11 @@ enum { I2E_PREC_FP64 = 0, I2E_PREC_MIXED = 1 };
  * BFBt (two Galerkin-multigrid solves with B diag(A)^-1 B^T around one
  * velocity-block apply) or the viscosity-weighted pressure mass matrix.   */
 enum { I2E_SCHUR_BFBT = 0, I2E_SCHUR_MASS = 1 };
+/* Gram-Schmidt of the FGMRES solvers: lagged CGS2 (each basis vector's second
+ * projection fused into the next iteration's pass: two passes over the basis
+ * per iteration) or classical Gram-Schmidt repeated when the projection
+ * removed most of the vector (Daniel-Gragg-Kaufman-Stewart).               */
+enum { I2E_ORTH_LAGGED = 0, I2E_ORTH_DGKS = 1 };
 
 typedef struct i2e_solver_opts {
   int32_t precision;       /* I2E_PREC_FP64 or I2E_PREC_MIXED                 */
@@ -150,7 +155,9 @@ typedef struct i2e_solver_opts {
   int32_t coarse_max;      /* coarsest grid: cells per direction <= this     */
   int32_t schur;           /* Schur preconditioner: I2E_SCHUR_BFBT or _MASS  */
   int32_t graphs;          /* 1: replay the Stokes preconditioner as a HIP graph */
-  int32_t pad1;
+  int32_t nsmooth_coarse;  /* Stokes multigrids: sweeps below the finest level (0: nsmooth) */
+  int32_t orth;            /* FGMRES orthogonalisation: I2E_ORTH_LAGGED or _DGKS */
+  int32_t pad2;
 } i2e_solver_opts;
 
 typedef struct i2e_stats {

@@ -187,6 +187,31 @@ def test_viterate_parity_fp64(orc):
     assert rel(got.pr, hs.pr) < 1e-8
 
 
+@pytest.mark.parametrize("orth", [0, 1])
+def test_viterate_parity_fp64_orthogonalisation(orc, orth):
+    """Both Gram-Schmidt variants of the FGMRES solvers (lagged CGS2, CGS with
+    DGKS reorthogonalisation) reach the direct solve on the 51^2 case."""
+    say("case viterate c1 orth", orth)
+    p, hs, _ = configs.make("c1")
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    opts = fp64_opts(1e-13)
+    opts.orth = orth
+    ctx = Context(p, hs, solver=opts)
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.viterate()
+    st = ctx.stats()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o.viterate()
+    say("stokes its", st.stokes_iters, "relres", st.stokes_relres)
+    assert st.stokes_relres <= 1e-13
+    for f in ("gp", "vx", "vy", "pr"):
+        assert rel(getattr(got, f), getattr(hs, f)) < 1e-8, f
+
+
 def test_titerate_parity_fp64(orc):
     say("case titerate c1")
     p, hs, _ = configs.make("c1")
