@@ -590,76 +590,103 @@ __global__ void k_probe0(VLev<T> L, T *S) {
 // pointwise rows (boundary, ghost) keep their mukoef-scaled form
 // one thread per coarse point, 64 threads per block: the 50 accumulators of
 // a row live in LDS (runtime slot indices would otherwise go to scratch)
-constexpr int GAL_B = 64;
+// Galerkin product R A P of one coarse level: a block holds GAL_R coarse rows
+// with GAL_S threads each, one per fine row of the row's restriction support
+// (at most RW x RW).  Each thread accumulates its fine row's contributions in
+// its own LDS slots; the GAL_S partial rows are then summed in a fixed order
+// (reproducible).  Coarse pointwise rows get mu on the diagonal (and -mu k on
+// their neighbour), as on the finest level.
+constexpr int GAL_R = 4, GAL_S = RW * RW, GAL_B = GAL_R * GAL_S;
 template <class T>
 __global__ __launch_bounds__(GAL_B) void k_galerkin(StView<T> F, int Xc, int Yc, MapView xn, MapView xm, MapView yn,
                                                     MapView ym, const int8_t *ckx, const int8_t *cky, const int *cnbx,
                                                     const int *cnby, const T *ckkx, const T *ckky, double mu, T *Sc,
                                                     int *overflow) {
   __shared__ double sacc[50 * GAL_B];
-  const int j = blockIdx.x * GAL_B + threadIdx.x, i = blockIdx.y;
+  __shared__ double sw[GAL_B];
+  const int r = threadIdx.x / GAL_S, sub = threadIdx.x % GAL_S;
+  const int j = blockIdx.x * GAL_R + r, i = blockIdx.y;
+  const bool rowok = i < Xc && j < Yc;
   double *acc = sacc + threadIdx.x;  // slot q at acc[q * GAL_B]
-  if (i >= Xc || j >= Yc) return;
-  const int Nc = Xc * Yc, Nf = F.X * F.Y, m = i * Yc + j;
+  const int Nc = Xc * Yc, Nf = F.X * F.Y, m = rowok ? i * Yc + j : 0;
   const size_t Rc = 2 * (size_t)Nc, Rf = 2 * (size_t)Nf;
   for (int comp = 1; comp <= 2; comp++) {
     const size_t rc = (size_t)(comp - 1) * Nc + m;
     for (int q = 0; q < 50; q++) acc[q * GAL_B] = 0.0;
-    const int kind = comp == 1 ? ckx[m] : cky[m];
-    if (kind != 0) {
-      acc[sslot(2, comp - 1, 0, 0) * GAL_B] = mu;
-      const int nb = comp == 1 ? cnbx[m] : cnby[m];
-      if (kind == 1 && nb >= 0) {
-        const double kk = (double)(comp == 1 ? ckkx[m] : ckky[m]);
-        const int di = nb / Yc - i, dj = nb % Yc - j;
-        if (di >= -2 && di <= 2 && dj >= -2 && dj <= 2)
-          acc[sslot(2, comp - 1, di, dj) * GAL_B] -= mu * kk;
-        else
-          atomicAdd(overflow, 1);
-      }
-      for (int q = 0; q < 50; q++) Sc[(size_t)q * Rc + rc] = T(acc[q * GAL_B]);
-      continue;
-    }
-    const MapView mxr = comp == 1 ? xn : xm, myr = comp == 1 ? ym : yn;
-    double wsum = 0;
-    for (int k = mxr.rb[i]; k < mxr.re[i]; k++) {
-      const double wx = wgt(mxr, k, i);
-      if (wx == 0) continue;
-      for (int l = myr.rb[j]; l < myr.re[j]; l++) {
-        const double w = wx * wgt(myr, l, j);
-        if (w == 0) continue;
-        wsum += w;
+    double wpart = 0;
+    const int kind = rowok ? (comp == 1 ? ckx[m] : cky[m]) : -1;
+    if (kind == 0) {
+      const MapView mxr = comp == 1 ? xn : xm, myr = comp == 1 ? ym : yn;
+      const int k = mxr.rb[i] + sub / RW, l = myr.rb[j] + sub % RW;
+      const double w = (k < mxr.re[i] && l < myr.re[j]) ? wgt(mxr, k, i) * wgt(myr, l, j) : 0.0;
+      if (w != 0) {
+        wpart = w;
         const int a = k * F.Y + l;
-        if ((comp == 1 ? F.kx[a] : F.ky[a]) != 0) continue;
-        const size_t rf = (size_t)(comp - 1) * Nf + a;
-        for (int cc = 0; cc < 2; cc++) {
-          const MapView px = cc == 0 ? xn : xm, py = cc == 0 ? ym : yn;
-          for (int dk = -F.W; dk <= F.W; dk++)
-            for (int dl = -F.W; dl <= F.W; dl++) {
-              const double sv = (double)F.S[(size_t)sslot(F.W, cc, dk, dl) * Rf + rf];
-              if (sv == 0) continue;
-              const int kb = k + dk, lb = l + dl;
-              if (kb < 0 || kb >= F.X || lb < 0 || lb >= F.Y) continue;
-              if (cc == 0 ? (lb == F.Y - 1) : (kb == F.X - 1)) continue;  // ghost: no prolongation
-              const int a0 = px.i0[kb], b0 = py.i0[lb];
-              const double wa = px.w[kb], wb = py.w[lb];
-              for (int ia = 0; ia < 2; ia++)
-                for (int ib = 0; ib < 2; ib++) {
-                  const double pw = (ia ? wa : 1.0 - wa) * (ib ? wb : 1.0 - wb);
-                  if (pw == 0) continue;
-                  const int di = a0 + ia - i, dj = b0 + ib - j;
-                  if (di < -2 || di > 2 || dj < -2 || dj > 2) {
-                    atomicAdd(overflow, 1);
-                    continue;
+        if ((comp == 1 ? F.kx[a] : F.ky[a]) == 0) {
+          const size_t rf = (size_t)(comp - 1) * Nf + a;
+          for (int cc = 0; cc < 2; cc++) {
+            const MapView px = cc == 0 ? xn : xm, py = cc == 0 ? ym : yn;
+            for (int dk = -F.W; dk <= F.W; dk++)
+              for (int dl = -F.W; dl <= F.W; dl++) {
+                const double sv = (double)F.S[(size_t)sslot(F.W, cc, dk, dl) * Rf + rf];
+                if (sv == 0) continue;
+                const int kb = k + dk, lb = l + dl;
+                if (kb < 0 || kb >= F.X || lb < 0 || lb >= F.Y) continue;
+                if (cc == 0 ? (lb == F.Y - 1) : (kb == F.X - 1)) continue;  // ghost: no prolongation
+                const int a0 = px.i0[kb], b0 = py.i0[lb];
+                const double wa = px.w[kb], wb = py.w[lb];
+                for (int ia = 0; ia < 2; ia++)
+                  for (int ib = 0; ib < 2; ib++) {
+                    const double pw = (ia ? wa : 1.0 - wa) * (ib ? wb : 1.0 - wb);
+                    if (pw == 0) continue;
+                    const int di = a0 + ia - i, dj = b0 + ib - j;
+                    if (di < -2 || di > 2 || dj < -2 || dj > 2) {
+                      atomicAdd(overflow, 1);
+                      continue;
+                    }
+                    acc[sslot(2, cc, di, dj) * GAL_B] += w * sv * pw;
                   }
-                  acc[sslot(2, cc, di, dj) * GAL_B] += w * sv * pw;
-                }
-            }
+              }
+          }
         }
       }
     }
-    const double inv = wsum > 0 ? 1.0 / wsum : 0.0;
-    for (int q = 0; q < 50; q++) Sc[(size_t)q * Rc + rc] = T(acc[q * GAL_B] * inv);
+    sw[threadIdx.x] = wpart;
+    __syncthreads();
+    if (rowok) {
+      const int base = r * GAL_S;
+      if (kind != 0) {
+        // pointwise row: mu on the diagonal, -mu k on the one neighbour
+        const int nb = comp == 1 ? cnbx[m] : cnby[m];
+        int qn = -1;
+        double kk = 0;
+        if (kind == 1 && nb >= 0) {
+          const int di = nb / Yc - i, dj = nb % Yc - j;
+          if (di >= -2 && di <= 2 && dj >= -2 && dj <= 2) {
+            qn = sslot(2, comp - 1, di, dj);
+            kk = (double)(comp == 1 ? ckkx[m] : ckky[m]);
+          } else if (sub == 0) {
+            atomicAdd(overflow, 1);
+          }
+        }
+        const int qd = sslot(2, comp - 1, 0, 0);
+        for (int q = sub; q < 50; q += GAL_S) {
+          double v = (q == qd ? mu : 0.0);
+          if (q == qn) v -= mu * kk;
+          Sc[(size_t)q * Rc + rc] = T(v);
+        }
+      } else {
+        double wsum = 0;
+        for (int t = 0; t < GAL_S; t++) wsum += sw[base + t];
+        const double inv = wsum > 0 ? 1.0 / wsum : 0.0;
+        for (int q = sub; q < 50; q += GAL_S) {
+          double v = 0;
+          for (int t = 0; t < GAL_S; t++) v += sacc[q * GAL_B + base + t];
+          Sc[(size_t)q * Rc + rc] = T(v * inv);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1071,7 +1098,7 @@ int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, cons
   for (int l = 0; l + 1 < nl; l++) {
     VelLevel<T> &f = L[l], &c = L[l + 1];
     StView<T> fv{f.S.p, f.X, f.Y, f.W, f.kx.p, f.ky.p};
-    hipLaunchKernelGGL(k_galerkin<T>, dim3((c.Y + GAL_B - 1) / GAL_B, c.X), dim3(GAL_B), 0, s, fv, c.X, c.Y,
+    hipLaunchKernelGGL(k_galerkin<T>, dim3((c.Y + GAL_R - 1) / GAL_R, c.X), dim3(GAL_B), 0, s, fv, c.X, c.Y,
                        view(f.mxn), view(f.mxm),
                        view(f.myn), view(f.mym), c.kx.p, c.ky.p, c.nbx.p, c.nby.p, c.kkx.p, c.kky.p, mu_, c.S.p,
                        overflow.p);
