@@ -27,7 +27,7 @@ _EXPORTS = [
     "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
-    "i2e_dev_copy", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
+    "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
 ]
 
 _lib = None
@@ -84,6 +84,9 @@ def load_library(path=LIB_PATH):
         "i2e_copy_to_dev": ([vp, vp, C.c_size_t], C.c_int),
         "i2e_copy_to_host": ([vp, vp, C.c_size_t], C.c_int),
         "i2e_dev_copy": ([vp, vp, C.c_size_t], C.c_int),
+        "i2e_prn_write": ([C.c_char_p, P, S, C.c_int], C.c_int),
+        "i2e_prn_read_header": ([C.c_char_p, C.POINTER(abi.PrnHeader)], C.c_int),
+        "i2e_prn_read": ([C.c_char_p, P, S, C.POINTER(C.c_int)], C.c_int),
         "i2e_set_params": ([vp, P], C.c_int),
         "i2e_record": ([vp, C.c_int], C.c_int),
         "i2e_replay": ([vp, C.c_int], C.c_int),

@@ -110,3 +110,12 @@ class AdjointOut(C.Structure):
     """i2e_adjoint_out: peak-temperature objective and its gradient."""
     _fields_ = [("J", C.c_double), ("dJ_dh0", C.c_double), ("dJ_dphi0", C.c_double),
                 ("heat_relres_max", C.c_double), ("steps", C.c_int32), ("heat_iters", C.c_int32)]
+
+
+class PrnHeader(C.Structure):
+    """i2e_prn_header: sizes and scalars of a binary .prn file."""
+    _fields_ = [("xnumx", C.c_int64), ("ynumy", C.c_int64), ("mnumx", C.c_int64), ("mnumy", C.c_int64),
+                ("marknum", C.c_int64), ("bondnum", C.c_int64), ("rocknum", C.c_int32), ("n0", C.c_int32),
+                ("xsize", C.c_double), ("ysize", C.c_double), ("por_init", C.c_double), ("gxkoef", C.c_double),
+                ("gykoef", C.c_double), ("tmp_ambient", C.c_double), ("al2627_init", C.c_double),
+                ("fe6056_init", C.c_double), ("timesum", C.c_double)]

@@ -14,14 +14,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libi2e.so")
 OBJ = os.path.join(HERE, "_obj")
-SOURCES = ["markers.hip", "ops.hip", "mg.hip", "krylov.hip", "ctx.hip", "adjoint.hip"]
+SOURCES = ["markers.hip", "ops.hip", "mg.hip", "krylov.hip", "ctx.hip", "adjoint.hip", "prn.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 
 
 def _compile(src):
-    obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)

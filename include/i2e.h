@@ -300,6 +300,29 @@ int i2e_adjoint_peak_t(i2e_ctx *c, i2e_adjoint_out *out);
 /* y = A^T x for the heat rows of i2e_heat_apply_dev (device pointers).     */
 int i2e_heat_apply_t_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
 
+/* ---- binary .prn checkpoint (saver/loader of loadmart.c) -----------------
+ * The byte layout of the reference's binary output (saver, loadmart.c:671-809;
+ * loader, loadmart.c:391-540; read by support_files/plot2d.py:157-283), for a
+ * host state: node fields, grid lines and BC values are stored as float32,
+ * as there.  Fields this build does not model are written as zero and
+ * skipped on reading; like the reference, the file holds marknum + 1 marker
+ * records (the last one zero here).  i2e_prn_read_header gives the sizes to
+ * allocate; i2e_prn_read then fills a state of exactly those sizes and the
+ * params the file carries (grid, rock table, 26Al/60Fe, ambient T, GYKOEF),
+ * leaving the others (time-step and solver controls, which the reference
+ * reads from mode.t3c) as the caller set them.  gp and ga are not in the
+ * file and come back zero.                                                  */
+typedef struct i2e_prn_header {
+  int64_t xnumx, ynumy, mnumx, mnumy, marknum, bondnum;
+  int32_t rocknum, n0;
+  double xsize, ysize, por_init, gxkoef, gykoef, tmp_ambient;
+  double al2627_init, fe6056_init;  /* absolute ratios (the file scales them) */
+  double timesum;                   /* seconds (the file holds years)         */
+} i2e_prn_header;
+int i2e_prn_write(const char *path, const i2e_params *p, const i2e_host_state *s, int n0);
+int i2e_prn_read_header(const char *path, i2e_prn_header *h);
+int i2e_prn_read(const char *path, i2e_params *p, i2e_host_state *s, int *n0);
+
 /* Device buffers for callers without a device allocator of their own (no
  * counterpart in the reference): allocation, release and blocking copies,
  * all through the same HIP runtime as the library itself.                   */

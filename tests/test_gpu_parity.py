@@ -433,3 +433,37 @@ def test_step_501_properties():
     assert got.marknum == hs.marknum
     assert np.array_equal(np.sort(np.unique(got.markt)), np.sort(np.unique(hs.markt)))
     assert np.all(got.tk >= 149.0) and np.all(got.tk < 2500.0)
+
+
+def test_restart_from_prn_continues_the_run(tmp_path):
+    """Two steps, the binary .prn checkpoint written and read back into a new
+    context, one more step in both: the restart rounds node fields to float32
+    (as the reference's saver does), so fields agree to 1e-5 and the markers
+    to float32 noise, with identical types."""
+    from i2elvis_planet_amd import prn
+    say("case prn restart c1")
+    p, hs, _ = configs.make("c1")
+    opts = fp64_opts(1e-10)
+    ctx = Context(p, hs, solver=opts)
+    ctx.ronurecalc()
+    for _ in range(2):
+        ctx.step()
+    mid = ctx.download(hs.copy())
+    path = tmp_path / "c1_2.prn"
+    prn.write(path, p, mid, n0=2)
+    q, back, n0 = prn.read(path, params=p)
+    assert n0 == 2 and back.marknum == mid.marknum
+    ctx.step()
+    want = ctx.download(hs.copy())
+    ctx.close()
+    ctx2 = Context(q, back, solver=opts)
+    ctx2.step()
+    got = ctx2.download(back.copy())
+    ctx2.close()
+    for f in ("vx", "vy", "pr", "tk"):
+        e = rel(getattr(got, f), getattr(want, f))
+        say(f, "restart vs continued rel L2 %.2e" % e)
+        assert e < 1e-5, f
+    assert np.array_equal(got.markt, want.markt)
+    assert np.max(np.abs(got.markx - want.markx)) <= 1e-6 * p.xsize
+    assert np.max(np.abs(got.markk - want.markk)) <= 1e-3
