@@ -14,7 +14,8 @@
 
 namespace i2e {
 
-static constexpr int RB = 256;        // threads per block
+static constexpr int RB = 128;        // threads per block (2 wavefronts: measured best for the basis passes)
+static constexpr int NW = RB / 64;    // wavefronts per block
 static constexpr int EPT = 4;        // vector elements per thread in the Gram-Schmidt kernels
 static constexpr int CH = RB * EPT;  // vector elements per block
 
@@ -26,7 +27,7 @@ static inline int gs_blocks(int n) { return (n + CH - 1) / CH; }
 // with cross-lane butterflies, the block's four wavefronts meet in LDS once.
 __global__ __launch_bounds__(RB) void k_gsdot(int n, int nv, const double *__restrict__ V, size_t ld,
                                               const double *__restrict__ w, double *__restrict__ part) {
-  extern __shared__ double sh[];  // [4][nv + 1]
+  extern __shared__ double sh[];  // [NW][nv + 1]
   const int G = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6, S = nv + 1;
   const size_t base = (size_t)blockIdx.x * CH + threadIdx.x;
   double we[EPT];
@@ -70,8 +71,12 @@ __global__ __launch_bounds__(RB) void k_gsdot(int n, int nv, const double *__res
   a = wave_sum(a);
   if (lane == 0) sh[wv * S + nv] = a;
   __syncthreads();
-  for (int i = threadIdx.x; i < S; i += RB)
-    part[(size_t)i * G + blockIdx.x] = ((sh[i] + sh[S + i]) + sh[2 * S + i]) + sh[3 * S + i];
+  for (int i = threadIdx.x; i < S; i += RB) {
+    double t = 0;
+#pragma unroll
+    for (int k = 0; k < NW; k++) t += sh[k * S + i];
+    part[(size_t)i * G + blockIdx.x] = t;
+  }
 }
 
 // w -= sum_i h[i] V_i (i < nv), and part[blk] = <w, w> of the updated w over
@@ -122,7 +127,12 @@ __global__ __launch_bounds__(RB) void k_gsaxpy(int n, int nv, const double *__re
   a = wave_sum(a);
   if (lane == 0) sh[wv] = a;
   __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  if (threadIdx.x == 0) {
+    double t = 0;
+#pragma unroll
+    for (int k = 0; k < NW; k++) t += sh[k];
+    part[blockIdx.x] = t;
+  }
 }
 
 // Lagged CGS2 (the second projection of each basis vector is fused into the
@@ -134,7 +144,7 @@ __global__ __launch_bounds__(RB) void k_gsaxpy(int n, int nv, const double *__re
 __global__ __launch_bounds__(RB) void k_gsdot2(int n, int nv, const double *__restrict__ V, size_t ld,
                                                const double *__restrict__ u, const double *__restrict__ w,
                                                double *__restrict__ part) {
-  extern __shared__ double sh[];  // [4][2nv + 3]
+  extern __shared__ double sh[];  // [NW][2nv + 3]
   const int G = gridDim.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6, S = 2 * nv + 3;
   const size_t base = (size_t)blockIdx.x * CH + threadIdx.x;
   double ue[EPT], we[EPT];
@@ -144,29 +154,29 @@ __global__ __launch_bounds__(RB) void k_gsdot2(int n, int nv, const double *__re
     ue[e] = k < (size_t)n ? u[k] : 0.0;
     we[e] = k < (size_t)n ? w[k] : 0.0;
   }
+  // four basis vectors per round: 16 loads in flight per lane
   int i = 0;
-  for (; i + 2 <= nv; i += 2) {
-    double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+  for (; i + 4 <= nv; i += 4) {
+    double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
       const size_t k = base + (size_t)e * RB;
       if (k < (size_t)n) {
-        const double v0 = V[(size_t)i * ld + k], v1 = V[(size_t)(i + 1) * ld + k];
-        a0 += v0 * ue[e];
-        b0 += v0 * we[e];
-        a1 += v1 * ue[e];
-        b1 += v1 * we[e];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const double v = V[(size_t)(i + q) * ld + k];
+          a[q] += v * ue[e];
+          b[q] += v * we[e];
+        }
       }
     }
-    a0 = wave_sum(a0);
-    a1 = wave_sum(a1);
-    b0 = wave_sum(b0);
-    b1 = wave_sum(b1);
-    if (lane == 0) {
-      sh[wv * S + i] = a0;
-      sh[wv * S + i + 1] = a1;
-      sh[wv * S + nv + i] = b0;
-      sh[wv * S + nv + i + 1] = b1;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double ta = wave_sum(a[q]), tb = wave_sum(b[q]);
+      if (lane == 0) {
+        sh[wv * S + i + q] = ta;
+        sh[wv * S + nv + i + q] = tb;
+      }
     }
   }
   for (; i < nv; i++) {
@@ -203,22 +213,50 @@ __global__ __launch_bounds__(RB) void k_gsdot2(int n, int nv, const double *__re
     sh[wv * S + 2 * nv + 2] = ww;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < S; q += RB)
-    part[(size_t)q * G + blockIdx.x] = ((sh[q] + sh[S + q]) + sh[2 * S + q]) + sh[3 * S + q];
+  for (int q = threadIdx.x; q < S; q += RB) {
+    double t = 0;
+#pragma unroll
+    for (int k = 0; k < NW; k++) t += sh[k * S + q];
+    part[(size_t)q * G + blockIdx.x] = t;
+  }
 }
 
-// Coefficients of the fused update from r = [a(nv), b(nv), uu, uw, ww]:
+// Fixed-order sums of the projection pass's block partials, one block per
+// quantity: r[q] = sum_blk part[q*G + blk], q < 2nv+3, r = [a(nv), b(nv), uu,
+// uw, ww].  The last block to finish then forms the fused update's
+// coefficients:
 //   beta = sqrt(uu - a.a)       (norm of u after its second projection)
 //   h    = (uw - a.b) / beta    (<v, w> for the finished v = (u - V a) / beta)
 //   c    = b - a h / beta       (so that w - V c - (h/beta) u = w - V b - h v)
-// cf = [a(nv), c(nv), 1/beta, h/beta]; out = [beta, h].  beta = 0 flags a
-// breakdown (u in the span of V).
-__global__ void k_gscoef(int nv, const double *__restrict__ r, double *__restrict__ cf, double *__restrict__ out) {
+// cf = [a(nv), c(nv), 1/beta, h/beta]; r[2nv+3] = beta, r[2nv+4] = h.
+// beta = 0 flags a breakdown (u in the span of V).  *done counts finished
+// blocks and is left at zero.
+__global__ __launch_bounds__(RB) void k_reduce_coef(int nv, int G, const double *__restrict__ part, double *r,
+                                                    double *__restrict__ cf, unsigned *done) {
   __shared__ double sa[RB], sb[RB];
+  __shared__ bool last;
+  const int q = blockIdx.x;
+  double sum = 0;
+  for (int b = threadIdx.x; b < G; b += RB) sum += part[(size_t)q * G + b];
+  sa[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = RB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sa[threadIdx.x] += sa[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    r[q] = sa[0];
+    __threadfence();
+    last = atomicAdd(done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile double *rv = r;
   double aa = 0, ab = 0;
   for (int i = threadIdx.x; i < nv; i += RB) {
-    aa += r[i] * r[i];
-    ab += r[i] * r[nv + i];
+    aa += rv[i] * rv[i];
+    ab += rv[i] * rv[nv + i];
   }
   sa[threadIdx.x] = aa;
   sb[threadIdx.x] = ab;
@@ -230,23 +268,24 @@ __global__ void k_gscoef(int nv, const double *__restrict__ r, double *__restric
     }
     __syncthreads();
   }
-  const double b2 = r[2 * nv] - sa[0];
+  const double b2 = rv[2 * nv] - sa[0];
   const double beta = b2 > 0 ? sqrt(b2) : 0.0;
-  const double h = beta > 0 ? (r[2 * nv + 1] - sb[0]) / beta : 0.0;
+  const double h = beta > 0 ? (rv[2 * nv + 1] - sb[0]) / beta : 0.0;
   const double t = beta > 0 ? h / beta : 0.0;
   for (int i = threadIdx.x; i < nv; i += RB) {
-    cf[i] = r[i];
-    cf[nv + i] = r[nv + i] - r[i] * t;
+    cf[i] = rv[i];
+    cf[nv + i] = rv[nv + i] - rv[i] * t;
   }
   if (threadIdx.x == 0) {
     cf[2 * nv] = beta > 0 ? 1.0 / beta : 0.0;
     cf[2 * nv + 1] = t;
-    out[0] = beta;
-    out[1] = h;
+    r[2 * nv + 3] = beta;
+    r[2 * nv + 4] = h;
+    *done = 0;
   }
 }
 
-// Fused update with cf from k_gscoef: vout = (u - V a) / beta (the finished
+// Fused update with cf from k_reduce_coef: vout = (u - V a) / beta (the finished
 // basis vector), wout = w - V c - (h/beta) u (w projected on the finished
 // basis, once), and part[blk] = <wout, wout> over this block's chunk.
 // vout may alias u; wout may alias w.
@@ -260,15 +299,23 @@ __global__ __launch_bounds__(RB) void k_gsaxpy2(int n, int nv, const double *__r
 #pragma unroll
   for (int e = 0; e < EPT; e++) s1[e] = s2[e] = 0.0;
   int i = 0;
-  for (; i + 2 <= nv; i += 2) {
-    const double a0 = cf[i], a1 = cf[i + 1], c0 = cf[nv + i], c1 = cf[nv + i + 1];
+  for (; i + 4 <= nv; i += 4) {
+    double a[4], c[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      a[q] = cf[i + q];
+      c[q] = cf[nv + i + q];
+    }
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
       const size_t k = base + (size_t)e * RB;
       if (k < (size_t)n) {
-        const double v0 = V[(size_t)i * ld + k], v1 = V[(size_t)(i + 1) * ld + k];
-        s1[e] = (s1[e] + a0 * v0) + a1 * v1;
-        s2[e] = (s2[e] + c0 * v0) + c1 * v1;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const double v = V[(size_t)(i + q) * ld + k];
+          s1[e] += a[q] * v;
+          s2[e] += c[q] * v;
+        }
       }
     }
   }
@@ -300,15 +347,39 @@ __global__ __launch_bounds__(RB) void k_gsaxpy2(int n, int nv, const double *__r
   acc = wave_sum(acc);
   if (lane == 0) sh[wv] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  if (threadIdx.x == 0) {
+    double t = 0;
+#pragma unroll
+    for (int k = 0; k < NW; k++) t += sh[k];
+    part[blockIdx.x] = t;
+  }
 }
 
-// v *= 1 / sqrt(*nrm2) (nrm2 on the device; no-op when it is not positive)
-__global__ void k_scale_norm(int n, const double *__restrict__ nrm2, double *__restrict__ v) {
-  const double q = *nrm2;
+// <w',w'> from the update pass's G block partials, summed by every block in
+// the same fixed order (so all blocks agree), then v *= 1/sqrt(<w',w'>) over
+// this block's chunk; block 0 stores the sum in *nrm2.  The scaling keeps
+// the pending vector's size from compounding from step to step.
+__global__ __launch_bounds__(RB) void k_norm_scale(int n, int G, const double *__restrict__ part,
+                                                   double *__restrict__ nrm2, double *__restrict__ v) {
+  __shared__ double sh[RB];
+  double sum = 0;
+  for (int b = threadIdx.x; b < G; b += RB) sum += part[b];
+  sh[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = RB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double q = sh[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nrm2 = q;
   if (!(q > 0)) return;
   const double f = 1.0 / sqrt(q);
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) v[k] *= f;
+  const size_t base = (size_t)blockIdx.x * CH + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < EPT; e++) {
+    const size_t k = base + (size_t)e * RB;
+    if (k < (size_t)n) v[k] *= f;
+  }
 }
 
 // out[i] = sum_blk part[i*G + blk] in a fixed order, i < nv (one block per i)
@@ -337,6 +408,14 @@ __global__ void k_multiaxpy(int n, int nv, const double *__restrict__ V, size_t 
   }
 }
 
+// out = w / sqrt(*q) (w unchanged when *q is not positive)
+__global__ void k_scale_by(int n, const double *__restrict__ q, const double *__restrict__ w,
+                           double *__restrict__ out) {
+  const double qq = *q;
+  const double f = qq > 0 ? 1.0 / sqrt(qq) : 1.0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = w[k] * f;
+}
+
 // v = w * f
 __global__ void k_scale(int n, const double *__restrict__ w, double f, double *__restrict__ v) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) v[k] = w[k] * f;
@@ -358,6 +437,8 @@ int Fgmres::init(int n_, int m_) {
   ST_OK(part.alloc((size_t)gs_blocks(n) * (2 * m + 4)));
   ST_OK(hd.alloc(8 * (m + 4)));
   ST_OK(hh.alloc(8 * (m + 4)));
+  ST_OK(done.alloc(1));
+  HIP_OK(hipMemset(done.p, 0, sizeof(unsigned)));
   return I2E_OK;
 }
 
@@ -368,7 +449,7 @@ int Fgmres::gs_pass(int nv, double *w_, int off, hipStream_t s) {
   const double bytes_dot = 8.0 * ((double)(nv + 1) * n + (double)(nv + 1) * G);
   const double bytes_axpy = 8.0 * ((double)nv * n + 2.0 * n + G);
   if (kt) kt->begin(K_GSDOT, s);
-  hipLaunchKernelGGL(k_gsdot, dim3(G), dim3(RB), sizeof(double) * 4 * (nv + 1), s, n, nv, V.p, ld, w_, part.p);
+  hipLaunchKernelGGL(k_gsdot, dim3(G), dim3(RB), sizeof(double) * NW * (nv + 1), s, n, nv, V.p, ld, w_, part.p);
   if (kt) kt->end(K_GSDOT, s, bytes_dot);
   hipLaunchKernelGGL(k_reduce_parts, dim3(nv + 1), dim3(RB), 0, s, nv + 1, G, part.p, hd.p + off);
   if (nv > 0) {
@@ -394,38 +475,34 @@ int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double r
                            : solve_lagged(A, M, b, x, rtol, maxit, s, iters, relres);
 }
 
-// One fused lagged-CGS2 step on the pending vector u (slot j of V, scaled to
-// about unit norm) and w = A z_j: finishes u into v_j against V[0..j), and
-// projects w once on V[0..j]; w' goes to wout.  Host receives (after the
-// caller's synchronisation) a(j), b(j), beta, h, <w',w'> at hh[0 ..].
+// One fused lagged-CGS2 step on the pending vector u (slot j of V, projected
+// once, unit norm) and w = A z_j: finishes u into v_j against V[0..j), and
+// projects w once on V[0..j]; w' / ||w'|| goes to wout.  After the caller's
+// synchronisation the host holds hh = [a(j), b(j), uu, uw, ww, beta, h, <w',w'>].
 int Fgmres::lagged_pass(int j, const double *w_, double *wout, hipStream_t s) {
   const int G = gs_blocks(n);
   double *u = V.p + (size_t)j * ld;
-  const size_t R = 0, CF = 2 * (size_t)m + 4, OUT = 4 * (size_t)m + 8;  // offsets in hd
+  const size_t R = 0, CF = 2 * (size_t)m + 4;  // offsets in hd
   const double bytes_dot = 8.0 * ((double)j * n + 2.0 * n + (2.0 * j + 3) * G);
   const double bytes_axpy = 8.0 * ((double)j * n + 4.0 * n + G);
   if (kt) kt->begin(K_GSDOT, s);
-  hipLaunchKernelGGL(k_gsdot2, dim3(G), dim3(RB), sizeof(double) * 4 * (2 * j + 3), s, n, j, V.p, ld, u, w_,
+  hipLaunchKernelGGL(k_gsdot2, dim3(G), dim3(RB), sizeof(double) * NW * (2 * j + 3), s, n, j, V.p, ld, u, w_,
                      part.p);
   if (kt) kt->end(K_GSDOT, s, bytes_dot);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(2 * j + 3), dim3(RB), 0, s, 2 * j + 3, G, part.p, hd.p + R);
-  hipLaunchKernelGGL(k_gscoef, dim3(1), dim3(RB), 0, s, j, hd.p + R, hd.p + CF, hd.p + OUT);
+  hipLaunchKernelGGL(k_reduce_coef, dim3(2 * j + 3), dim3(RB), 0, s, j, G, part.p, hd.p + R, hd.p + CF, done.p);
   if (kt) kt->begin(K_GSAXPY, s);
   hipLaunchKernelGGL(k_gsaxpy2, dim3(G), dim3(RB), 0, s, n, j, V.p, ld, hd.p + CF, u, w_, u, wout, part.p);
   if (kt) kt->end(K_GSAXPY, s, bytes_axpy);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(RB), 0, s, 1, G, part.p, hd.p + OUT + 2);
-  // the pending vector of the next step, scaled to unit norm
-  hipLaunchKernelGGL(k_scale_norm, dim3(1024), dim3(256), 0, s, n, hd.p + OUT + 2, wout);
-  HIP_OK(hipMemcpyAsync(hh.p, hd.p + R, sizeof(double) * (2 * j), hipMemcpyDeviceToHost, s));
-  HIP_OK(hipMemcpyAsync(hh.p + 2 * j, hd.p + OUT, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+  hipLaunchKernelGGL(k_norm_scale, dim3(G), dim3(RB), 0, s, n, G, part.p, hd.p + R + 2 * j + 5, wout);
+  HIP_OK(hipMemcpyAsync(hh.p, hd.p + R, sizeof(double) * (2 * j + 6), hipMemcpyDeviceToHost, s));
   return I2E_OK;
 }
 
 // Right-preconditioned FGMRES with lagged CGS2.  Step j preconditions the
-// pending vector u_j (projected once, unit norm), then one pass finishes
-// u_j into v_j and projects w = A z_j once; column j-1 of H becomes final
-// here (A z_{j-1} = V b + h v_{j-1} + sigma_j (beta v_j + V a)), column j
-// one step later.  A step whose estimated residual meets the tolerance
+// pending vector u_j (w' of the previous step, projected once and scaled by
+// 1/sigma_j = 1/||w'||), then one pass finishes u_j into v_j and projects
+// w = A z_j once; column j-1 of H becomes final here (A z_{j-1} = V b +
+// h v_{j-1} + sigma_j (beta v_j + V a)), column j one step later.  A step whose estimated residual meets the tolerance
 // finishes its pending vector with one extra projection pass instead of a
 // further preconditioner application.
 int Fgmres::solve_lagged(const Op &A, const Op &M, const double *b, double *x, double rtol, int maxit,
@@ -457,7 +534,7 @@ int Fgmres::solve_lagged(const Op &A, const Op &M, const double *b, double *x, d
     const int nv = j + 1;
     const int G = gs_blocks(n);
     double *u = V.p + (size_t)(j + 1) * ld;
-    hipLaunchKernelGGL(k_gsdot, dim3(G), dim3(RB), sizeof(double) * 4 * (nv + 1), s, n, nv, V.p, ld, u, part.p);
+    hipLaunchKernelGGL(k_gsdot, dim3(G), dim3(RB), sizeof(double) * NW * (nv + 1), s, n, nv, V.p, ld, u, part.p);
     hipLaunchKernelGGL(k_reduce_parts, dim3(nv + 1), dim3(RB), 0, s, nv + 1, G, part.p, hd.p);
     HIP_OK(hipMemcpyAsync(hh.p, hd.p, sizeof(double) * (nv + 1), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -493,17 +570,19 @@ int Fgmres::solve_lagged(const Op &A, const Op &M, const double *b, double *x, d
       if (j == 0) {
         // v_0 is finished: one projection of w, the result is pending u_1
         ST_OK(gs_pass(1, w.p, 0, s));
-        hipLaunchKernelGGL(k_scale_norm, dim3(1024), dim3(256), 0, s, n, hd.p + 2, w.p);
-        hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, s, n, w.p, 1.0, un);
+        const double *ww0 = hd.p + 2;
+        hipLaunchKernelGGL(k_scale_by, dim3(gb), dim3(256), 0, s, n, ww0, w.p, un);
         HIP_OK(hipMemcpyAsync(hh.p, hd.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+      } else {
+        ST_OK(lagged_pass(j, w.p, un, s));
+      }
+      HIP_OK(hipStreamSynchronize(s));
+      if (j == 0) {
         H[0] = hh.p[0];
         sigma = std::sqrt(hh.p[2] > 0 ? hh.p[2] : 0.0);
       } else {
-        ST_OK(lagged_pass(j, w.p, un, s));
-        HIP_OK(hipStreamSynchronize(s));
         const double *a = hh.p, *bb = hh.p + j;
-        const double bet = hh.p[2 * j], h = hh.p[2 * j + 1], ww = hh.p[2 * j + 2];
+        const double bet = hh.p[2 * j + 3], h = hh.p[2 * j + 4], ww = hh.p[2 * j + 5];
         // column j-1 is final: A z_{j-1} = ... + sigma (beta v_j + V a)
         for (int i = 0; i < j; i++) H[(size_t)i * m + (j - 1)] += sigma * a[i];
         H[(size_t)j * m + (j - 1)] = sigma * bet;
@@ -666,6 +745,7 @@ void Fgmres::release() {
   w.release();
   part.release();
   hd.release();
+  done.release();
   if (hh.p) hipHostFree(hh.p);
   hh.p = nullptr;
   hh.n = 0;

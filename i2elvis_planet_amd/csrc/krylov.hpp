@@ -28,6 +28,7 @@ struct Fgmres {
   int n = 0, m = 0;
   size_t ld = 0;
   DBuf<double> V, Z, w, part, hd;
+  DBuf<unsigned> done;   // finished-block counter of k_reduce_coef (zero between launches)
   HBuf<double> hh;
   KTimer *kt = nullptr;  // optional per-launch timing of the Gram-Schmidt kernels
   int reorth = 0;        // extra Gram-Schmidt passes in the last solve
