@@ -102,7 +102,23 @@ int Map1::build(const std::vector<double> &cf, const std::vector<double> &cc) {
   HIP_OK(hipMemcpy(w.p, hw.data(), nf * sizeof(double), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(rb.p, hrb.data(), nc * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(re.p, hre.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+  this->hrb = hrb;
+  this->hre = hre;
   return I2E_OK;
+}
+
+int Map1::tile_span(int t) const {
+  int worst = 0;
+  for (int c0 = 0; c0 < nc; c0 += t) {
+    int a0 = nf, a1 = 0;
+    for (int c = c0; c < nc && c < c0 + t; c++)
+      if (hre[c] > hrb[c]) {
+        a0 = std::min(a0, hrb[c]);
+        a1 = std::max(a1, hre[c]);
+      }
+    worst = std::max(worst, a1 - a0);
+  }
+  return worst;
 }
 
 __device__ __forceinline__ double wgt(const MapView &m, int k, int I) {
@@ -1585,6 +1601,84 @@ __global__ void k_crestrict(int Xc, int Yc, int Yf, MapView mx, MapView my, cons
   f[m] = cmask[m] ? T(0) : restrict_rw(r, Yf, mx, my, i, j);
 }
 
+// Residual and restriction of one cell level in one kernel: a workgroup
+// takes a CR_T x CR_T tile of coarse cells, forms the fine residuals of the
+// fine range the tile gathers from in LDS (as k_cresid), then restricts from
+// there (as k_crestrict, same operation order: the result is identical).
+// The host checks that every tile's fine range fits CR_F x CR_F.
+constexpr int CR_T = 8, CR_F = 2 * CR_T + 4;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_cresid_restrict(int X, int Y, const T *S, const int8_t *mask, const T *u,
+                                                         const T *f, int Xc, int Yc, MapView mx, MapView my,
+                                                         const int8_t *cmask, T *fc) {
+  __shared__ T rl[CR_F * CR_F];
+  __shared__ int lim[4];
+  const int ci0 = blockIdx.y * CR_T, cj0 = blockIdx.x * CR_T;
+  const int ci1 = min(ci0 + CR_T, Xc), cj1 = min(cj0 + CR_T, Yc);
+  if (threadIdx.x == 0) {
+    int a0 = X, a1 = 0, b0 = Y, b1 = 0;
+    for (int i = ci0; i < ci1; i++)
+      if (mx.re[i] > mx.rb[i]) {
+        a0 = min(a0, mx.rb[i]);
+        a1 = max(a1, mx.re[i]);
+      }
+    for (int j = cj0; j < cj1; j++)
+      if (my.re[j] > my.rb[j]) {
+        b0 = min(b0, my.rb[j]);
+        b1 = max(b1, my.re[j]);
+      }
+    lim[0] = a0;
+    lim[1] = a1 > a0 ? a1 : a0;
+    lim[2] = b0;
+    lim[3] = b1 > b0 ? b1 : b0;
+  }
+  __syncthreads();
+  const int fi0 = lim[0], fj0 = lim[2], FX = lim[1] - lim[0], FY = lim[3] - lim[2];
+  const int N = X * Y;
+  for (int q = threadIdx.x; q < FX * FY; q += blockDim.x) {
+    const int i = fi0 + q / FY, j = fj0 + q % FY, m = i * Y + j;
+    T r = 0;
+    if (!mask[m]) {
+      T acc = 0;
+#pragma unroll
+      for (int di = -1; di <= 1; di++)
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++) {
+          int a = i + di, b = j + dj;
+          a = a < 0 ? 0 : (a >= X ? X - 1 : a);
+          b = b < 0 ? 0 : (b >= Y ? Y - 1 : b);
+          acc += S[(size_t)((di + 1) * 3 + dj + 1) * N + m] * u[a * Y + b];
+        }
+      r = f[m] - acc;
+    }
+    rl[(q / FY) * CR_F + q % FY] = r;
+  }
+  __syncthreads();
+  const int TY = cj1 - cj0, TN = (ci1 - ci0) * TY;
+  for (int q = threadIdx.x; q < TN; q += blockDim.x) {
+    const int i = ci0 + q / TY, j = cj0 + q % TY, m = i * Yc + j;
+    if (cmask[m]) {
+      fc[m] = T(0);
+      continue;
+    }
+    const int k0 = mx.rb[i] - fi0, l0 = my.rb[j] - fj0;
+    double wx[RW], wy[RW];
+#pragma unroll
+    for (int a = 0; a < RW; a++) {
+      wx[a] = mx.rw[i * RW + a];
+      wy[a] = my.rw[j * RW + a];
+    }
+    double sum = 0;
+#pragma unroll
+    for (int a = 0; a < RW; a++)
+#pragma unroll
+      for (int b = 0; b < RW; b++)
+        if (wx[a] != 0 && wy[b] != 0) sum += wx[a] * wy[b] * (double)rl[(k0 + a) * CR_F + l0 + b];
+    fc[m] = T(sum);
+  }
+}
+
 // The small levels of the cell multigrid (from the first with N <= CT_N
 // down to the coarsest) run as one workgroup: iterates, right-hand sides and
 // residuals in LDS, each thread's stencil rows in registers, phases
@@ -1779,6 +1873,7 @@ int CellMG<T>::setup(const std::vector<double> &x0, const std::vector<double> &y
     if (l + 1 < nl) {
       ST_OK(v.mx.build(xs[l], xs[l + 1]));
       ST_OK(v.my.build(ys[l], ys[l + 1]));
+      v.fused = v.mx.tile_span(CR_T) <= CR_F && v.my.tile_span(CR_T) <= CR_F;
     }
   }
   nc = L[nl - 1].N;
@@ -1873,10 +1968,16 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
   hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, A);
   std::swap(v.u.p, v.u2.p);
   CellLevel<T> &c = L[l + 1];
-  hipLaunchKernelGGL(k_cresid<T>, grd2(v.X, v.Y), blk2(), 0, s, v.X, v.Y, (const T *)v.S.p,
-                     (const int8_t *)v.mask.p, (const T *)v.u.p, (const T *)v.f.p, v.r.p);
-  hipLaunchKernelGGL(k_crestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mx), view(v.my),
-                     (const int8_t *)c.mask.p, (const T *)v.r.p, c.f.p);
+  if (v.fused) {
+    hipLaunchKernelGGL(k_cresid_restrict<T>, dim3((c.Y + CR_T - 1) / CR_T, (c.X + CR_T - 1) / CR_T), dim3(256), 0,
+                       s, v.X, v.Y, (const T *)v.S.p, (const int8_t *)v.mask.p, (const T *)v.u.p, (const T *)v.f.p,
+                       c.X, c.Y, view(v.mx), view(v.my), (const int8_t *)c.mask.p, c.f.p);
+  } else {
+    hipLaunchKernelGGL(k_cresid<T>, grd2(v.X, v.Y), blk2(), 0, s, v.X, v.Y, (const T *)v.S.p,
+                       (const int8_t *)v.mask.p, (const T *)v.u.p, (const T *)v.f.p, v.r.p);
+    hipLaunchKernelGGL(k_crestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mx), view(v.my),
+                       (const int8_t *)c.mask.p, (const T *)v.r.p, c.f.p);
+  }
   ST_OK(vcycle(l + 1, s));
   CellArgs<T> P{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, ns, 1};
   hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, P);

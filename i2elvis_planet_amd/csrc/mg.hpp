@@ -57,7 +57,10 @@ struct Map1 {
   DBuf<double> w;    // [nf] weight of i0+1
   DBuf<int> rb, re;  // [nc] fine index range touching coarse index
   DBuf<double> rw;   // [nc*RW] normalised restriction weights of fine indices rb.. (0 past re)
+  std::vector<int> hrb, hre;  // host copies of rb, re
   int build(const std::vector<double> &cf, const std::vector<double> &cc);
+  // widest fine index range that tiles of t consecutive coarse indices gather from
+  int tile_span(int t) const;
 };
 
 struct MapView {
@@ -176,6 +179,7 @@ struct CellLevel {
   DBuf<int8_t> mask;   // 1: identity row
   DBuf<T> u, u2, f, r;
   Map1 mx, my;         // to the next coarser level
+  bool fused = false;  // residual and restriction in one kernel (tile spans fit its LDS tile)
 };
 
 template <class T>
