@@ -19,6 +19,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 SCHUR_BFBT = 0
 SCHUR_MASS = 1
+SCHUR_WBFBT = 2
 ORTH_LAGGED = 0
 ORTH_DGKS = 1
 

@@ -95,14 +95,19 @@ __global__ void k_schur_bc(SOp<double> o, BcTab bc, const double *r, double *z) 
 // (X-1) x (Y-1), cell (i,j) of the node grid at (i-1, j-1).
 
 // C^-1 on momentum rows (0 elsewhere) and the explicit 5-point L on cells
+// C^-1 on the momentum rows: C = diag(A) (scaled BFBt) or, with sqrtw, C =
+// sign(d) sqrt|d| with d = diag(A) ~ eta/h^2: the sqrt(eta) weighting of
+// Rudi, Stadler & Ghattas (SIAM J. Sci. Comput. 39, 2017), robust to large
+// viscosity contrasts (the preconditioner is invariant to scaling C)
 __global__ void k_bf_setup(Grid g, const int *bondm, const int8_t *kx, const int8_t *ky, const double *dx,
-                           const double *dy, double *cix, double *ciy, double *LS, int8_t *lmask) {
+                           const double *dy, double *cix, double *ciy, double *LS, int8_t *lmask, int sqrtw) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   const int X = g.X, Y = g.Y;
   if (j >= Y) return;
   const int m = i * Y + j;
-  cix[m] = kx[m] == 0 ? 1.0 / dx[m] : 0.0;
-  ciy[m] = ky[m] == 0 ? 1.0 / dy[m] : 0.0;
+  auto inv = [&](double d) { return sqrtw ? (d < 0 ? -1.0 : 1.0) / sqrt(fabs(d)) : 1.0 / d; };
+  cix[m] = kx[m] == 0 ? inv(dx[m]) : 0.0;
+  ciy[m] = ky[m] == 0 ? inv(dy[m]) : 0.0;
 }
 __global__ void k_bf_stencil(Grid g, const int *bondm, const double *cix, const double *ciy, double *LS,
                              int8_t *lmask) {
@@ -567,7 +572,7 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
   if (!c || !o) return I2E_ERR_ARG;
   if (o->restart < 1 || o->restart > 400 || o->maxit < 1 || o->nsmooth < 0 || o->nsmooth_coarse < 0 ||
       o->coarse_max < 2 || (o->orth != I2E_ORTH_LAGGED && o->orth != I2E_ORTH_DGKS) ||
-      (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS))
+      (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT))
     return I2E_ERR_ARG;
   bool regeo = c->setup_done && o->coarse_max != c->o.coarse_max;
   c->o = *o;
@@ -918,10 +923,10 @@ static int stokes_prepare(i2e_ctx *c) {
     c->v32.nsmooth = c->o.nsmooth;
     c->v32.nsmooth_c = c->o.nsmooth_coarse;
   }
-  if (c->o.schur == I2E_SCHUR_BFBT) {
+  if (c->o.schur != I2E_SCHUR_MASS) {
     const VelLevel<double> &f0 = c->v64.L[0];
     hipLaunchKernelGGL(k_bf_setup, nb2(X, Y), dim3(256), 0, c->s, c->grid(), c->bondm.p, f0.kx.p, f0.ky.p, f0.dx.p,
-                       f0.dy.p, c->bcix.p, c->bciy.p, c->bLS.p, c->blmask.p);
+                       f0.dy.p, c->bcix.p, c->bciy.p, c->bLS.p, c->blmask.p, c->o.schur == I2E_SCHUR_WBFBT ? 1 : 0);
     hipLaunchKernelGGL(k_bf_stencil, nb2(X - 1, Y - 1), dim3(256), 0, c->s, c->grid(), c->bondm.p, c->bcix.p,
                        c->bciy.p, c->bLS.p, c->blmask.p);
     ST_OK(c->l64.refresh(c->bLS.p, c->blmask.p, c->s));
@@ -968,7 +973,7 @@ static int bfbt_precond(i2e_ctx *c, VelMG<T> &v, CellMG<T> &lm, const double *r,
 static int stokes_precond(i2e_ctx *c, const double *r, double *z) {
   const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
-  if (c->o.schur == I2E_SCHUR_BFBT) return mixed(c) ? bfbt_precond(c, c->v32, c->l32, r, z)
+  if (c->o.schur != I2E_SCHUR_MASS) return mixed(c) ? bfbt_precond(c, c->v32, c->l32, r, z)
                                                      : bfbt_precond(c, c->v64, c->l64, r, z);
   if (mixed(c)) {
     hipLaunchKernelGGL(k_split_v<float>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v32.L[0].fx.p, c->v32.L[0].fy.p);

@@ -133,9 +133,10 @@ typedef struct i2e_host_state {
 /* ---- solver controls ---------------------------------------------------- */
 enum { I2E_PREC_FP64 = 0, I2E_PREC_MIXED = 1 };
 /* Pressure Schur-complement approximation of the Stokes preconditioner:
- * BFBt (two Galerkin-multigrid solves with B diag(A)^-1 B^T around one
- * velocity-block apply) or the viscosity-weighted pressure mass matrix.   */
-enum { I2E_SCHUR_BFBT = 0, I2E_SCHUR_MASS = 1 };
+ * BFBt (two Galerkin-multigrid solves with B C^-1 B^T around one
+ * velocity-block apply; C = diag(A), or with WBFBT its square root, i.e.
+ * sqrt(viscosity) weights) or the viscosity-weighted pressure mass matrix. */
+enum { I2E_SCHUR_BFBT = 0, I2E_SCHUR_MASS = 1, I2E_SCHUR_WBFBT = 2 };
 /* Gram-Schmidt of the FGMRES solvers: lagged CGS2 (each basis vector's second
  * projection fused into the next iteration's pass: two passes over the basis
  * per iteration) or classical Gram-Schmidt repeated when the projection
