@@ -24,7 +24,16 @@ HBM_PEAK_GBS = 8000.0
 # runs (FETCH_SIZE, WRITE_SIZE) and scaled by the same counters on a plain
 # 1 GiB copy; written by `bench.py --pmc-summary` (counters cannot be read
 # inside the timed run).
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gsdot_501x501_pmc_traffic.md")
+TRAFFIC_FILES = {
+    "vsmoothc": os.path.join(ROOT, "profiles", "k_tsmooth_f32_501x501_pmc_traffic.md"),
+    "gsdot": os.path.join(ROOT, "profiles", "k_gsdot2_501x501_pmc_traffic.md"),
+}
+# rocprof kernel-name prefix and, where only some launches count, the grid
+# size (threads) of the launches bench.py times for each roofline kernel
+PMC_MATCH = {
+    "vsmoothc": ("void i2e::k_tsmooth<float, 2>(", None),
+    "gsdot": ("i2e::k_gsdot2(", "stokes"),
+}
 
 
 def log(*a):
@@ -53,46 +62,54 @@ def cpu_baseline(sample_steps=1):
 
 # algorithmic-byte roofline kernels (HBM bound), selectable with --roofline-kernel
 ROOFLINE_KERNELS = {
-    "gsdot": "k_gsdot (Gram-Schmidt projection pass of the Stokes FGMRES)",
-    "gsaxpy": "k_gsaxpy (Gram-Schmidt update pass of the Stokes FGMRES)",
-    "vsmoothc": "k_sjac (coarse-level velocity smoother colour step)",
-    "vsmooth0": "k_vgs (finest-level velocity smoother colour step)",
+    "gsdot": "k_gsdot2 (fused Gram-Schmidt projection pass of the Stokes FGMRES)",
+    "gsaxpy": "k_gsaxpy2 (fused Gram-Schmidt update pass of the Stokes FGMRES)",
+    "vsmoothc": "k_tsmooth<float,2> (tiled velocity smoothing pass, Galerkin levels 251^2..17^2, mixed mode)",
+    "vsmooth0": "k_tsmooth0<float> (tiled velocity smoothing pass, finest level, mixed mode)",
     "stokes_apply": "k_stokes_apply (matrix-free Stokes operator)",
     "advect": "k_movemark (RK4 marker advection)",
     "p2g": "k_ronu_markers (marker-to-node scatter)",
 }
 
 
+# launches sampled by the kernel timer: every 7th coarse smoothing pass (10
+# per Stokes iteration, so the sample cycles through all levels); the others
+# run once per iteration and are all timed
+ROOFLINE_EVERY = {"vsmoothc": 7, "vsmooth0": 3, "precond": 1}
+
+
 def stokes_gs_grid(n_nodes):
     """Grid size (threads) of a Gram-Schmidt launch of the Stokes FGMRES:
-    ceil(3N / 1024) blocks of 256 threads."""
-    return (3 * n_nodes + 1023) // 1024 * 256
+    ceil(3N / 512) blocks of 128 threads."""
+    return (3 * n_nodes + 511) // 512 * 128
 
 
-def read_traffic(path=TRAFFIC_FILE):
+def read_traffic(rk):
     try:
-        for line in open(path):
+        for line in open(TRAFFIC_FILES[rk]):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":")[1])
-    except OSError:
+    except (OSError, KeyError):
         pass
     return None
 
 
-def pmc_summary(fetch_dir, write_dir, kernel="k_gsdot", n_nodes=501 * 501, copy_bytes=None):
+def pmc_summary(fetch_dir, write_dir, rk="vsmoothc", n_nodes=501 * 501, copy_bytes=None):
     """Markdown summary of two rocprofv3 --pmc runs of
     `bench.py --no-cpu --steps 1 --warmup 0 --copy-calibration 1024`:
-    FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the Stokes-size launches of
-    `kernel`, scaled by what the same counters read for the plain copy."""
+    FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the launches of roofline
+    kernel `rk` that bench.py times, scaled by what the same counters read
+    for the plain copy."""
     import csv
+    prefix, only = PMC_MATCH[rk]
 
     def load(d):
         rows = list(csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
         cal = [float(r["Counter_Value"]) * 1024 for r in rows if "k_dev_copy" in r["Kernel_Name"]]
-        grid = stokes_gs_grid(n_nodes)
-        ker = [float(r["Counter_Value"]) * 1024 for r in rows
-               if ("::%s(" % kernel) in r["Kernel_Name"] and int(r["Grid_Size"]) == grid]
+        ker = [float(r["Counter_Value"]) * 1024 for r in rows if r["Kernel_Name"].startswith(prefix) and
+               (only is None or int(r["Grid_Size"]) == stokes_gs_grid(n_nodes))]
         return cal, ker
+    kernel = prefix.rstrip("(")
     cf, kf = load(fetch_dir)
     cw, kw = load(write_dir)
     if not (cf and cw and kf and kw):
@@ -106,8 +123,9 @@ def pmc_summary(fetch_dir, write_dir, kernel="k_gsdot", n_nodes=501 * 501, copy_
         "calibration copy: %d bytes read + %d written per dispatch; FETCH_SIZE read %.4g B (x%.4f), "
         "WRITE_SIZE %.4g B (x%.4f), means of %d / %d dispatches" % (nb, nb, sum(cf) / len(cf), fetch_factor,
                                                                    sum(cw) / len(cw), write_factor, len(cf), len(cw)),
-        "%s, Stokes-size dispatches (grid %d threads): %d / %d; raw FETCH %.4g B, raw WRITE %.4g B per dispatch"
-        % (kernel, stokes_gs_grid(n_nodes), len(kf), len(kw), f_raw, w_raw),
+        "%s%s: %d / %d dispatches; raw FETCH %.4g B, raw WRITE %.4g B per dispatch"
+        % (kernel, " (Stokes-size, grid %d threads)" % stokes_gs_grid(n_nodes) if only else "", len(kf), len(kw),
+           f_raw, w_raw),
         "traffic_bytes_per_launch: %.6g" % traffic,
     ]
     return "\n".join(lines)
@@ -136,7 +154,7 @@ def run(steps, warmup, skip_cpu, rk):
         ctx.upload(snap)
         ctx.sync()
         if mode == "mixed":
-            ctx.kernel_timer(rk, 1 << 17)
+            ctx.kernel_timer(rk, 1 << 17, ROOFLINE_EVERY.get(rk, 1))
         iters = []
         t0 = time.perf_counter()
         for k in range(steps):
@@ -186,8 +204,8 @@ def run(steps, warmup, skip_cpu, rk):
         "kernels_standalone_ms": {"advect_rk4": round(ms_adv, 4), "p2g_scatter": round(ms_p2g, 4),
                                   "stokes_apply_fp64": round(ms_sap, 4)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic() if rk == "gsdot" else None,
-                     "kernel": ROOFLINE_KERNELS[rk],
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(rk),
+                     "kernel": ROOFLINE_KERNELS[rk], "sampled_every": ROOFLINE_EVERY.get(rk, 1),
                      "launches": launches, "avg_us": avg_ms * 1e3,
                      "bytes_per_launch": bytes_tot / max(launches, 1)},
         "cpu_baseline": None,
@@ -243,7 +261,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--roofline-kernel", default="gsdot", choices=sorted(ROOFLINE_KERNELS))
+    ap.add_argument("--roofline-kernel", default="vsmoothc", choices=sorted(ROOFLINE_KERNELS),
+                    help="kernel of the roofline entry (default: the one with the most GPU time)")
     ap.add_argument("--ensemble", type=int, default=0, help="run the ensemble measurement with this many bodies")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times (PMC calibration)")
@@ -251,7 +270,7 @@ def main():
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
     a = ap.parse_args()
     if a.pmc_summary:
-        print(pmc_summary(*a.pmc_summary, copy_bytes=a.copy_calibration * 2 ** 20 or 2 ** 30))
+        print(pmc_summary(*a.pmc_summary, rk=a.roofline_kernel, copy_bytes=a.copy_calibration * 2 ** 20 or 2 ** 30))
         return
     if a.copy_calibration:
         from i2elvis_planet_amd import DeviceArray

@@ -75,7 +75,7 @@ def load_library(path=LIB_PATH):
         "i2e_stokes_rhs_dev": ([vp, vp], C.c_int),
         "i2e_stokes_precond_dev": ([vp, vp, vp], C.c_int),
         "i2e_vcycle_dev": ([vp, vp, vp], C.c_int),
-        "i2e_kernel_timer": ([vp, C.c_char_p, C.c_int], C.c_int),
+        "i2e_kernel_timer": ([vp, C.c_char_p, C.c_int, C.c_int], C.c_int),
         "i2e_stokes_residuals": ([vp, C.POINTER(abi.Resid)], C.c_int),
         "i2e_step_batch": ([C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "i2e_kernel_timer_read": ([vp, C.POINTER(C.c_int64), dp, dp], C.c_int),
@@ -259,9 +259,9 @@ class Context:
         _chk(_lib.i2e_stokes_residuals(self._h, C.byref(r)), "stokes_residuals")
         return {k: getattr(r, k) for k, _ in abi.Resid._fields_}
 
-    def kernel_timer(self, kernel, max_launches=65536):
-        """Time every launch of `kernel` from now on with HIP events (None stops)."""
-        _chk(_lib.i2e_kernel_timer(self._h, (kernel or "").encode(), max_launches), "kernel_timer")
+    def kernel_timer(self, kernel, max_launches=65536, every=1):
+        """Time every `every`-th launch of `kernel` from now on with HIP events (None stops)."""
+        _chk(_lib.i2e_kernel_timer(self._h, (kernel or "").encode(), max_launches, every), "kernel_timer")
 
     def kernel_timer_read(self):
         """(launches, total ms, total algorithmic bytes) since kernel_timer()."""

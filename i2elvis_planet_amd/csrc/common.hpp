@@ -39,18 +39,23 @@ void set_error(const char *fmt, ...);
 // sites bracket the launch with begin/end and state its algorithmic bytes.
 enum KId { K_ADVECT = 0, K_P2G, K_GSDOT, K_GSAXPY, K_VSMOOTH0, K_VSMOOTHC, K_STOKES_APPLY, K_PRECOND, K_COUNT };
 
+// Every `every`-th launch of the target is bracketed (a sample: the event
+// pairs themselves add latency to the stream they are recorded on).
 struct KTimer {
-  int target = -1;
+  int target = -1, every = 1;
   std::vector<hipEvent_t> ev;
   size_t used = 0;
-  int64_t launches = 0;
+  int64_t launches = 0, seen = 0;
   double bytes = 0;
-  bool active(int id) const { return id == target && used + 2 <= ev.size(); }
+  bool open = false;
   void begin(int id, hipStream_t s) {
-    if (active(id)) (void)hipEventRecord(ev[used], s);
+    if (id != target || used + 2 > ev.size()) return;
+    open = (seen++ % every) == 0;
+    if (open) (void)hipEventRecord(ev[used], s);
   }
   void end(int id, hipStream_t s, double b) {
-    if (!active(id)) return;
+    if (id != target || !open) return;
+    open = false;
     (void)hipEventRecord(ev[used + 1], s);
     used += 2;
     launches++;

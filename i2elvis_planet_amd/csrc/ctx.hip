@@ -1398,8 +1398,8 @@ int i2e_stokes_residuals(i2e_ctx *c, i2e_resid *out) {
   return I2E_OK;
 }
 
-int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches) {
-  if (!c || max_launches < 0) return I2E_ERR_ARG;
+int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches, int every) {
+  if (!c || max_launches < 0 || every < 1) return I2E_ERR_ARG;
   static const char *names[K_COUNT] = {"advect", "p2g",          "gsdot",  "gsaxpy",
                                        "vsmooth0", "vsmoothc", "stokes_apply", "precond"};
   int id = -1;
@@ -1419,8 +1419,11 @@ int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches) {
     c->kt.ev.push_back(e);
   }
   c->kt.target = id;
+  c->kt.every = every;
   c->kt.used = 0;
   c->kt.launches = 0;
+  c->kt.seen = 0;
+  c->kt.open = false;
   c->kt.bytes = 0;
   return I2E_OK;
 }

@@ -251,10 +251,12 @@ int i2e_stokes_residuals(i2e_ctx *c, i2e_resid *out);
  * "vsmooth0" / "vsmoothc" = one colour step of the velocity smoother on the
  * finest / a coarse multigrid level, "stokes_apply", "precond" = one
  * whole Stokes preconditioner application (bytes not counted); NULL or ""
- * stops timing), reserves events for up to max_launches launches and clears
- * the counters; i2e_kernel_timer_read returns the launches timed, their
- * summed milliseconds and summed algorithmic HBM bytes.                     */
-int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches);
+ * stops timing), reserves events for up to max_launches timed launches,
+ * times every `every`-th launch of the kernel (1: all; event pairs add
+ * latency to the stream) and clears the counters; i2e_kernel_timer_read
+ * returns the launches timed, their summed milliseconds and summed
+ * algorithmic HBM bytes.                                                    */
+int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches, int every);
 int i2e_kernel_timer_read(i2e_ctx *c, int64_t *launches, double *ms_total,
                           double *bytes_total);
 
