@@ -269,3 +269,25 @@ def test_adjoint_refuses_heating_limited_time_steps():
     with pytest.raises(I2EError, match="heating-rate limit"):
         ctx.replay(4)
     ctx.close()
+
+
+def test_torch_autograd_wrapper():
+    """The adjoint wrapped as a torch.autograd.Function (tests/harness), run in
+    a child process so that torch never shares this process with libi2e:
+    autograd's gradient is the adjoint's, and dJ/dh0 matches a central
+    difference through the same wrapper."""
+    import json
+    import os
+    import subprocess
+    import sys
+    say("case adjoint torch.autograd wrapper (child process)")
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([sys.executable, "-u", os.path.join(here, "harness", "adjoint_autograd.py")],
+                       capture_output=True, text=True, timeout=200)
+    say(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    # the same recorded run (node sums by atomics: equal to round-off)
+    assert abs(out["grad_h0"] - out["direct_h0"]) <= 1e-12 * abs(out["direct_h0"])
+    assert abs(out["grad_phi0"] - out["direct_phi0"]) <= 1e-12 * abs(out["direct_phi0"])
+    assert abs(out["grad_h0"] - out["fd_h0"]) <= 1e-5 * abs(out["fd_h0"])
