@@ -22,6 +22,7 @@ SCHUR_MASS = 1
 SCHUR_WBFBT = 2
 ORTH_LAGGED = 0
 ORTH_DGKS = 1
+FINE_AUTO, FINE_ROWS, FINE_STENCIL = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -83,7 +84,7 @@ class SolverOpts(C.Structure):
         ("stokes_rtol", C.c_double), ("heat_rtol", C.c_double), ("grav_rtol", C.c_double),
         ("schur_scale", C.c_double), ("omega", C.c_double),
         ("coarse_max", C.c_int32), ("schur", C.c_int32),
-        ("graphs", C.c_int32), ("nsmooth_coarse", C.c_int32), ("orth", C.c_int32), ("pad2", C.c_int32),
+        ("graphs", C.c_int32), ("nsmooth_coarse", C.c_int32), ("orth", C.c_int32), ("fine_stencil", C.c_int32),
     ]
 
 

@@ -1156,6 +1156,7 @@ int VelMG<T>::refresh_from(const VelMG<double> &d, hipStream_t s) {
   cast(d.L[0].gp, L[0].gp, N0);
   cast(d.L[0].dx, L[0].dx, N0);
   cast(d.L[0].dy, L[0].dy, N0);
+  cast(d.L[0].S, L[0].S, (size_t)2 * N0 * 18);
   for (int l = 1; l < nl; l++) cast(d.L[l].S, L[l].S, (size_t)2 * L[l].N * 50);
   HIP_OK(hipMemcpyAsync(ainv.p, d.ainv.p, sizeof(double) * (size_t)nc * nc, hipMemcpyDeviceToDevice, s));
   HIP_OK(hipGetLastError());
@@ -1185,8 +1186,10 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
                 none, none, none, none, 1, ns, 0};
   auto launch = [&](const TileArgs<T> &a) {
     if (kt) kt->begin(kid, s);
-    if (l == 0)
+    if (l == 0 && !stencil0)
       hipLaunchKernelGGL(k_tsmooth0<T>, TG, TBK, 0, s, a, o, (const T *)v.dx.p, (const T *)v.dy.p);
+    else if (l == 0)
+      hipLaunchKernelGGL((k_tsmooth<T, 1>), TG, TBK, 0, s, a, (const T *)v.S.p);
     else
       hipLaunchKernelGGL((k_tsmooth<T, 2>), TG, TBK, 0, s, a, (const T *)v.S.p);
     if (kt) kt->end(kid, s, bytes);
@@ -1199,8 +1202,10 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   R.ux = v.ux.p;
   R.uy = v.uy.p;
   R.zero = 0;
-  if (l == 0)
+  if (l == 0 && !stencil0)
     hipLaunchKernelGGL(k_vresid<T>, grd2(v.X, v.Y), blk2(), 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
+  else if (l == 0)
+    hipLaunchKernelGGL((k_tresid<T, 1>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
   else
     hipLaunchKernelGGL((k_tresid<T, 2>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
   VelLevel<T> &c = L[l + 1];

@@ -118,6 +118,7 @@ struct VelMG {
   double mu = 0, dt = 0;
   int nsmooth = 2;
   int nsmooth_c = 0;        // sweeps on the coarser levels (0: nsmooth)
+  int stencil0 = 0;         // finest level smoothed with its probed 3x3 stencil (else matrix-free rows)
   KTimer *kt = nullptr;     // optional per-launch timing of the smoothers
   int setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,
             const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,

@@ -142,6 +142,11 @@ enum { I2E_SCHUR_BFBT = 0, I2E_SCHUR_MASS = 1, I2E_SCHUR_WBFBT = 2 };
  * per iteration) or classical Gram-Schmidt repeated when the projection
  * removed most of the vector (Daniel-Gragg-Kaufman-Stewart).               */
 enum { I2E_ORTH_LAGGED = 0, I2E_ORTH_DGKS = 1 };
+/* Finest velocity level of the Stokes multigrid: matrix-free rows (in
+ * difference form, which keeps FP32 accurate), or its probed 3x3 stencil held
+ * in registers (fewer operations; in FP32 the rounded stencil needs ~6 % more
+ * Krylov iterations).  AUTO: the stencil in FP64 mode, the rows in mixed.  */
+enum { I2E_FINE_AUTO = 0, I2E_FINE_ROWS = 1, I2E_FINE_STENCIL = 2 };
 
 typedef struct i2e_solver_opts {
   int32_t precision;       /* I2E_PREC_FP64 or I2E_PREC_MIXED                 */
@@ -158,7 +163,7 @@ typedef struct i2e_solver_opts {
   int32_t graphs;          /* 1: replay the Stokes preconditioner as a HIP graph */
   int32_t nsmooth_coarse;  /* Stokes multigrids: sweeps below the finest level (0: nsmooth) */
   int32_t orth;            /* FGMRES orthogonalisation: I2E_ORTH_LAGGED or _DGKS */
-  int32_t pad2;
+  int32_t fine_stencil;    /* finest velocity level smoothed with I2E_FINE_*  */
 } i2e_solver_opts;
 
 typedef struct i2e_stats {
