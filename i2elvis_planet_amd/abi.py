@@ -22,7 +22,7 @@ SCHUR_MASS = 1
 SCHUR_WBFBT = 2
 ORTH_LAGGED = 0
 ORTH_DGKS = 1
-FINE_AUTO, FINE_ROWS, FINE_STENCIL = 0, 1, 2
+FINE_AUTO, FINE_ROWS, FINE_STENCIL, FINE_STENCIL64 = 0, 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)

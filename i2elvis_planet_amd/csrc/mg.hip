@@ -819,9 +819,9 @@ __device__ __forceinline__ void tile_load(const TileArgs<T> &A, T *lx, T *ly, in
 
 // S u at a tile point from LDS (out-of-grid halo entries are 0, and so are
 // the coefficients that would multiply them)
-template <class T, int W>
-__device__ __forceinline__ T s_apply_lds(const T *S, size_t R, size_t r, const T *lx, const T *ly, int la, int lb) {
-  T acc = 0;
+template <class T, class TA, int W>
+__device__ __forceinline__ TA s_apply_lds(const TA *S, size_t R, size_t r, const T *lx, const T *ly, int la, int lb) {
+  TA acc = 0;
 #pragma unroll
   for (int cc = 0; cc < 2; cc++) {
     const T *u = cc ? ly : lx;
@@ -833,8 +833,10 @@ __device__ __forceinline__ T s_apply_lds(const T *S, size_t R, size_t r, const T
   return acc;
 }
 
-template <class T, int W>
-__global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const T *S) {
+// TA: type of the stencil and of the row sums (FP64 stencil with FP32
+// iterates keeps the finest level's cancellations exact in mixed mode)
+template <class T, class TA, int W>
+__global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
   __shared__ T lx[TW * TW], ly[TW * TW];
   const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
   const int X = A.X, Y = A.Y, N = X * Y;
@@ -848,16 +850,16 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const T *S) {
   const int kx = ok ? A.kx[m] : -1, ky = ok ? A.ky[m] : -1;
   // both stencil rows of this thread's point stay in registers for the pass
   constexpr int NSL = 2 * (2 * W + 1) * (2 * W + 1);
-  T sx[NSL], sy[NSL];
+  TA sx[NSL], sy[NSL];
 #pragma unroll
   for (int q = 0; q < NSL; q++) {
-    sx[q] = ok ? S[(size_t)q * R + m] : T(0);
-    sy[q] = ok ? S[(size_t)q * R + (size_t)N + m] : T(0);
+    sx[q] = ok ? S[(size_t)q * R + m] : TA(0);
+    sy[q] = ok ? S[(size_t)q * R + (size_t)N + m] : TA(0);
   }
-  const T dx = ok ? sx[sslot(W, 0, 0, 0)] : T(1), dy = ok ? sy[sslot(W, 1, 0, 0)] : T(1);
+  const TA dx = ok ? sx[sslot(W, 0, 0, 0)] : TA(1), dy = ok ? sy[sslot(W, 1, 0, 0)] : TA(1);
   const int la = li / TW, lb = li % TW;
-  auto apply = [&](const T *sv) {
-    T acc = 0;
+  auto apply = [&](const TA *sv) {
+    TA acc = 0;
 #pragma unroll
     for (int cc = 0; cc < 2; cc++) {
       const T *u = cc ? ly : lx;
@@ -874,7 +876,7 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const T *S) {
       const int k = A.rev ? 3 - st : st, comp = 1 + k / 2, col = k & 1;
       const bool act = ok && ((i + j) & 1) == col;
       T nv = 0;
-      if (act) nv = comp == 1 ? lx[li] + (fx - apply(sx)) / dx : ly[li] + (fy - apply(sy)) / dy;
+      if (act) nv = T(comp == 1 ? lx[li] + (fx - apply(sx)) / dx : ly[li] + (fy - apply(sy)) / dy);
       __syncthreads();
       if (act) (comp == 1 ? lx : ly)[li] = nv;
       __syncthreads();
@@ -883,8 +885,8 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const T *S) {
   // re-solve the pointwise rows exactly: their scale (mukoef) dwarfs the
   // momentum rows, and the last colour steps left them inconsistent
   T bx = lx[li], by = ly[li];
-  if (kx > 0) bx += (fx - apply(sx)) / dx;
-  if (ky > 0) by += (fy - apply(sy)) / dy;
+  if (kx > 0) bx = T(bx + (fx - apply(sx)) / dx);
+  if (ky > 0) by = T(by + (fy - apply(sy)) / dy);
   A.ox[m] = bx;
   A.oy[m] = by;
 }
@@ -944,8 +946,8 @@ __global__ __launch_bounds__(TB) void k_tsmooth0(TileArgs<T> A, VLev<T> L, const
 }
 
 // r = f - S u on equation rows, 0 on pointwise rows (u staged through LDS)
-template <class T, int W>
-__global__ __launch_bounds__(TB) void k_tresid(TileArgs<T> A, const T *S, T *rxo, T *ryo) {
+template <class T, class TA, int W>
+__global__ __launch_bounds__(TB) void k_tresid(TileArgs<T> A, const TA *S, T *rxo, T *ryo) {
   __shared__ T lx[TW * TW], ly[TW * TW];
   const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
   const int X = A.X, Y = A.Y, N = X * Y;
@@ -955,8 +957,8 @@ __global__ __launch_bounds__(TB) void k_tresid(TileArgs<T> A, const T *S, T *rxo
   const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
   if (i >= X || j >= Y) return;
   const int m = i * Y + j, la = i - i0, lb = j - j0;
-  rxo[m] = A.kx[m] == 0 ? A.fx[m] - s_apply_lds<T, W>(S, R, m, lx, ly, la, lb) : T(0);
-  ryo[m] = A.ky[m] == 0 ? A.fy[m] - s_apply_lds<T, W>(S, R, (size_t)N + m, lx, ly, la, lb) : T(0);
+  rxo[m] = A.kx[m] == 0 ? T(A.fx[m] - s_apply_lds<T, TA, W>(S, R, m, lx, ly, la, lb)) : T(0);
+  ryo[m] = A.ky[m] == 0 ? T(A.fy[m] - s_apply_lds<T, TA, W>(S, R, (size_t)N + m, lx, ly, la, lb)) : T(0);
 }
 
 // ---------------------------------------------------------------- VelMG host
@@ -1188,10 +1190,12 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
     if (kt) kt->begin(kid, s);
     if (l == 0 && !stencil0)
       hipLaunchKernelGGL(k_tsmooth0<T>, TG, TBK, 0, s, a, o, (const T *)v.dx.p, (const T *)v.dy.p);
+    else if (l == 0 && S0d)
+      hipLaunchKernelGGL((k_tsmooth<T, double, 1>), TG, TBK, 0, s, a, S0d);
     else if (l == 0)
-      hipLaunchKernelGGL((k_tsmooth<T, 1>), TG, TBK, 0, s, a, (const T *)v.S.p);
+      hipLaunchKernelGGL((k_tsmooth<T, T, 1>), TG, TBK, 0, s, a, (const T *)v.S.p);
     else
-      hipLaunchKernelGGL((k_tsmooth<T, 2>), TG, TBK, 0, s, a, (const T *)v.S.p);
+      hipLaunchKernelGGL((k_tsmooth<T, T, 2>), TG, TBK, 0, s, a, (const T *)v.S.p);
     if (kt) kt->end(kid, s, bytes);
     std::swap(v.ux.p, v.ux2.p);
     std::swap(v.uy.p, v.uy2.p);
@@ -1204,10 +1208,12 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   R.zero = 0;
   if (l == 0 && !stencil0)
     hipLaunchKernelGGL(k_vresid<T>, grd2(v.X, v.Y), blk2(), 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
+  else if (l == 0 && S0d)
+    hipLaunchKernelGGL((k_tresid<T, double, 1>), TG, TBK, 0, s, R, S0d, v.rx.p, v.ry.p);
   else if (l == 0)
-    hipLaunchKernelGGL((k_tresid<T, 1>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
+    hipLaunchKernelGGL((k_tresid<T, T, 1>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
   else
-    hipLaunchKernelGGL((k_tresid<T, 2>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
+    hipLaunchKernelGGL((k_tresid<T, T, 2>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
   VelLevel<T> &c = L[l + 1];
   hipLaunchKernelGGL(k_vrestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mxn), view(v.mxm),
                      view(v.myn), view(v.mym), c.kx.p, c.ky.p, v.rx.p, v.ry.p, c.fx.p, c.fy.p);

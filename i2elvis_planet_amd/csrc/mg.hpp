@@ -119,6 +119,7 @@ struct VelMG {
   int nsmooth = 2;
   int nsmooth_c = 0;        // sweeps on the coarser levels (0: nsmooth)
   int stencil0 = 0;         // finest level smoothed with its probed 3x3 stencil (else matrix-free rows)
+  const double *S0d = nullptr;  // that stencil in FP64 (from the FP64 hierarchy) for FP32 iterates
   KTimer *kt = nullptr;     // optional per-launch timing of the smoothers
   int setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,
             const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,

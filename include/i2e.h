@@ -145,8 +145,11 @@ enum { I2E_ORTH_LAGGED = 0, I2E_ORTH_DGKS = 1 };
 /* Finest velocity level of the Stokes multigrid: matrix-free rows (in
  * difference form, which keeps FP32 accurate), or its probed 3x3 stencil held
  * in registers (fewer operations; in FP32 the rounded stencil needs ~6 % more
- * Krylov iterations).  AUTO: the stencil in FP64 mode, the rows in mixed.  */
-enum { I2E_FINE_AUTO = 0, I2E_FINE_ROWS = 1, I2E_FINE_STENCIL = 2 };
+ * Krylov iterations), or -- mixed mode only -- the FP64 stencil applied to the
+ * FP32 iterates with FP64 row sums (STENCIL64: the rows' iteration counts at
+ * fewer operations; same as STENCIL in FP64 mode).  AUTO: STENCIL in FP64
+ * mode, STENCIL64 in mixed.                                                */
+enum { I2E_FINE_AUTO = 0, I2E_FINE_ROWS = 1, I2E_FINE_STENCIL = 2, I2E_FINE_STENCIL64 = 3 };
 
 typedef struct i2e_solver_opts {
   int32_t precision;       /* I2E_PREC_FP64 or I2E_PREC_MIXED                 */

@@ -280,6 +280,34 @@ def test_mixed_precision_error_vs_fp64(orc):
         assert e < 1e-5, f
 
 
+@pytest.mark.parametrize("fine", [1, 2, 3])
+def test_mixed_viterate_fine_level_variants(orc, fine):
+    """Mixed precision with each smoother of the finest velocity level
+    (matrix-free rows, FP32 stencil, FP64 stencil on FP32 iterates) reaches
+    the metric's 1e-8 and the oracle's direct solve at 201^2."""
+    say("case mixed viterate c2 fine_stencil", fine)
+    p, hs, _ = configs.make("c2")
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    opts = default_solver()
+    opts.fine_stencil = fine
+    ctx = Context(p, hs, solver=opts)
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.viterate()
+    st = ctx.stats()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o.viterate()
+    say("stokes its", st.stokes_iters, "relres", st.stokes_relres)
+    assert st.stokes_relres <= 1e-8
+    for f in ("vx", "vy", "pr"):
+        e = rel(getattr(got, f), getattr(hs, f))
+        say(f, e)
+        assert e < 1e-5, f
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_graph_replay_matches_eager_precond(orc, precision):
     """The Stokes preconditioner replayed as a captured HIP graph (once per
