@@ -871,14 +871,18 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
     return acc;
   };
   __syncthreads();
+  // Lanes outside the grid run the colour steps too (zero stencil, unit
+  // diagonal, in-tile LDS reads) and only skip the store: a tile cut by the
+  // grid's last column otherwise leaves a sparse lane mask in every wave,
+  // which measured 2x slower per sweep than the full checkerboard.
   for (int it = 0; it < A.nsmooth; it++)
     for (int st = 0; st < 4; st++) {
       const int k = A.rev ? 3 - st : st, comp = 1 + k / 2, col = k & 1;
-      const bool act = ok && ((i + j) & 1) == col;
+      const bool act = ((i + j) & 1) == col;
       T nv = 0;
       if (act) nv = T(comp == 1 ? lx[li] + (fx - apply(sx)) / dx : ly[li] + (fy - apply(sy)) / dy);
       __syncthreads();
-      if (act) (comp == 1 ? lx : ly)[li] = nv;
+      if (act && ok) (comp == 1 ? lx : ly)[li] = nv;
       __syncthreads();
     }
   if (!ok) return;
@@ -1561,10 +1565,12 @@ __global__ __launch_bounds__(TB) void k_ctsmooth(CellArgs<T> A) {
   for (int q = 0; q < 9; q++) sv[q] = ok ? A.S[(size_t)q * N + m] : T(0);
   const T f = ok ? A.f[m] : T(0), d = ok ? sv[4] : T(1);
   __syncthreads();
+  // out-of-grid lanes compute too and skip the store (dense lane masks; see
+  // the velocity smoother)
   for (int it = 0; it < A.nsmooth; it++)
     for (int st = 0; st < 2; st++) {
       const int col = A.rev ? 1 - st : st;
-      const bool act = ok && ((i + j) & 1) == col;
+      const bool act = ((i + j) & 1) == col;
       T nv = 0;
       if (act) {
         T acc = 0;
@@ -1575,7 +1581,7 @@ __global__ __launch_bounds__(TB) void k_ctsmooth(CellArgs<T> A) {
         nv = lu[li] + (f - acc) / d;
       }
       __syncthreads();
-      if (act) lu[li] = nv;
+      if (act && ok) lu[li] = nv;
       __syncthreads();
     }
   if (ok) A.o[m] = lu[li];
