@@ -2,6 +2,7 @@
 // scalar (heat, gravity) operators.  See mg.hpp for the design.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "mg.hpp"
 
@@ -573,6 +574,10 @@ __device__ __forceinline__ int sslot(int W, int cc, int di, int dj) {
   return cc * w * w + (di + W) * w + (dj + W);
 }
 
+// a*b+c with one rounding (the smoothers' row sums)
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <class T>
 struct StView {
   const T *S;
@@ -857,6 +862,7 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
     sy[q] = ok ? S[(size_t)q * R + (size_t)N + m] : TA(0);
   }
   const TA dx = ok ? sx[sslot(W, 0, 0, 0)] : TA(1), dy = ok ? sy[sslot(W, 1, 0, 0)] : TA(1);
+  const TA rdx = TA(1) / dx, rdy = TA(1) / dy;
   const int la = li / TW, lb = li % TW;
   auto apply = [&](const TA *sv) {
     TA acc = 0;
@@ -866,25 +872,42 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
 #pragma unroll
       for (int di = -W; di <= W; di++)
 #pragma unroll
-        for (int dj = -W; dj <= W; dj++) acc += sv[sslot(W, cc, di, dj)] * u[(la + di) * TW + lb + dj];
+        for (int dj = -W; dj <= W; dj++) acc = fmadd(sv[sslot(W, cc, di, dj)], TA(u[(la + di) * TW + lb + dj]), acc);
     }
     return acc;
   };
-  __syncthreads();
-  // Lanes outside the grid run the colour steps too (zero stencil, unit
-  // diagonal, in-tile LDS reads) and only skip the store: a tile cut by the
-  // grid's last column otherwise leaves a sparse lane mask in every wave,
-  // which measured 2x slower per sweep than the full checkerboard.
-  for (int it = 0; it < A.nsmooth; it++)
-    for (int st = 0; st < 4; st++) {
-      const int k = A.rev ? 3 - st : st, comp = 1 + k / 2, col = k & 1;
-      const bool act = ((i + j) & 1) == col;
+  // two colour steps on one component (compile-time, so that no per-slot
+  // select between the Vx and Vy rows is needed)
+  auto half = [&](auto comp, int c0) {
+    constexpr int C = decltype(comp)::value;
+    T *u = C == 1 ? lx : ly;
+    const TA *sv = C == 1 ? sx : sy;
+    const T f = C == 1 ? fx : fy;
+    const TA rd = C == 1 ? rdx : rdy;
+    for (int c = 0; c < 2; c++) {
+      const bool act = ((i + j) & 1) == (c0 ^ c);
       T nv = 0;
-      if (act) nv = T(comp == 1 ? lx[li] + (fx - apply(sx)) / dx : ly[li] + (fy - apply(sy)) / dy);
+      if (act) nv = T(u[li] + (f - apply(sv)) * rd);
       __syncthreads();
-      if (act && ok) (comp == 1 ? lx : ly)[li] = nv;
+      if (act && ok) u[li] = nv;
       __syncthreads();
     }
+  };
+  __syncthreads();
+  // Colour steps Vx red, Vx black, Vy red, Vy black (reversed after the
+  // coarse-grid correction).  Lanes outside the grid run them too (zero
+  // stencil, unit diagonal, in-tile LDS reads) and only skip the store: a
+  // tile cut by the grid's last column otherwise leaves a sparse lane mask in
+  // every wave, which measured 2x slower per sweep than the full checkerboard.
+  for (int it = 0; it < A.nsmooth; it++) {
+    if (!A.rev) {
+      half(std::integral_constant<int, 1>{}, 0);
+      half(std::integral_constant<int, 2>{}, 0);
+    } else {
+      half(std::integral_constant<int, 2>{}, 1);
+      half(std::integral_constant<int, 1>{}, 1);
+    }
+  }
   if (!ok) return;
   // re-solve the pointwise rows exactly: their scale (mukoef) dwarfs the
   // momentum rows, and the last colour steps left them inconsistent
