@@ -22,7 +22,7 @@ SCHUR_MASS = 1
 SCHUR_WBFBT = 2
 ORTH_LAGGED = 0
 ORTH_DGKS = 1
-FINE_AUTO, FINE_ROWS, FINE_STENCIL, FINE_STENCIL64 = 0, 1, 2, 3
+FINE_AUTO, FINE_ROWS, FINE_STENCIL, FINE_STENCIL64, FINE_DIFF32 = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)

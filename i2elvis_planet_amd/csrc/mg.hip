@@ -606,6 +606,29 @@ __global__ void k_probe0(VLev<T> L, T *S) {
   }
 }
 
+// Finest 3x3 stencil in difference form for FP32 iterates: row r reads
+//   sum_cc [ sum_{k != centre} s_k (u_k - u_centre) + sigma_cc u_centre ]
+// with sigma_cc the FP64 sum of the row's cc-component coefficients, stored
+// in the centre slot.  Rounding the coefficients to FP32 then perturbs the
+// couplings by 1e-7 relative but keeps the near-cancellation of the momentum
+// rows on smooth fields (which plain FP32 coefficients lose).
+template <class T>
+__global__ void k_sdiff(const double *S, int N, T *D) {
+  const size_t R = 2 * (size_t)N;
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  for (int cc = 0; cc < 2; cc++) {
+    double sig = 0;
+    for (int di = -1; di <= 1; di++)
+      for (int dj = -1; dj <= 1; dj++) {
+        const double v = S[(size_t)sslot(1, cc, di, dj) * R + r];
+        sig += v;
+        if (di != 0 || dj != 0) D[(size_t)sslot(1, cc, di, dj) * R + r] = T(v);
+      }
+    D[(size_t)sslot(1, cc, 0, 0) * R + r] = T(sig);
+  }
+}
+
 // Galerkin coarse stencil R A P (R = normalised transpose of the bilinear
 // prolongation restricted to equation rows, as the residual restriction is);
 // pointwise rows (boundary, ghost) keep their mukoef-scaled form
@@ -791,6 +814,7 @@ struct TileArgs {
   int Yc;
   MapView xn, xm, yn, ym;  // this level -> next coarser
   int zero, nsmooth, rev;
+  const T *dx, *dy;  // row diagonals (read by the difference-form stencil only)
 };
 
 template <class T>
@@ -824,23 +848,30 @@ __device__ __forceinline__ void tile_load(const TileArgs<T> &A, T *lx, T *ly, in
 
 // S u at a tile point from LDS (out-of-grid halo entries are 0, and so are
 // the coefficients that would multiply them)
-template <class T, class TA, int W>
+template <class T, class TA, int W, bool DIFF>
 __device__ __forceinline__ TA s_apply_lds(const TA *S, size_t R, size_t r, const T *lx, const T *ly, int la, int lb) {
   TA acc = 0;
 #pragma unroll
   for (int cc = 0; cc < 2; cc++) {
     const T *u = cc ? ly : lx;
+    const TA uc = DIFF ? TA(u[la * TW + lb]) : TA(0);
 #pragma unroll
     for (int di = -W; di <= W; di++)
 #pragma unroll
-      for (int dj = -W; dj <= W; dj++) acc += S[(size_t)sslot(W, cc, di, dj) * R + r] * u[(la + di) * TW + lb + dj];
+      for (int dj = -W; dj <= W; dj++) {
+        const TA s = S[(size_t)sslot(W, cc, di, dj) * R + r];
+        const TA v = TA(u[(la + di) * TW + lb + dj]);
+        acc = fmadd(s, DIFF && (di != 0 || dj != 0) ? v - uc : v, acc);
+      }
   }
   return acc;
 }
 
 // TA: type of the stencil and of the row sums (FP64 stencil with FP32
 // iterates keeps the finest level's cancellations exact in mixed mode)
-template <class T, class TA, int W>
+// DIFF: the finest level's FP32 stencil in difference form (k_sdiff), with
+// the row diagonals read from A.dx / A.dy
+template <class T, class TA, int W, bool DIFF = false>
 __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
   __shared__ T lx[TW * TW], ly[TW * TW];
   const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
@@ -861,7 +892,8 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
     sx[q] = ok ? S[(size_t)q * R + m] : TA(0);
     sy[q] = ok ? S[(size_t)q * R + (size_t)N + m] : TA(0);
   }
-  const TA dx = ok ? sx[sslot(W, 0, 0, 0)] : TA(1), dy = ok ? sy[sslot(W, 1, 0, 0)] : TA(1);
+  const TA dx = ok ? (DIFF ? TA(A.dx[m]) : sx[sslot(W, 0, 0, 0)]) : TA(1);
+  const TA dy = ok ? (DIFF ? TA(A.dy[m]) : sy[sslot(W, 1, 0, 0)]) : TA(1);
   const TA rdx = TA(1) / dx, rdy = TA(1) / dy;
   const int la = li / TW, lb = li % TW;
   auto apply = [&](const TA *sv) {
@@ -869,10 +901,14 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
 #pragma unroll
     for (int cc = 0; cc < 2; cc++) {
       const T *u = cc ? ly : lx;
+      const TA uc = DIFF ? TA(u[la * TW + lb]) : TA(0);
 #pragma unroll
       for (int di = -W; di <= W; di++)
 #pragma unroll
-        for (int dj = -W; dj <= W; dj++) acc = fmadd(sv[sslot(W, cc, di, dj)], TA(u[(la + di) * TW + lb + dj]), acc);
+        for (int dj = -W; dj <= W; dj++) {
+          const TA v = TA(u[(la + di) * TW + lb + dj]);
+          acc = fmadd(sv[sslot(W, cc, di, dj)], DIFF && (di != 0 || dj != 0) ? v - uc : v, acc);
+        }
     }
     return acc;
   };
@@ -973,7 +1009,7 @@ __global__ __launch_bounds__(TB) void k_tsmooth0(TileArgs<T> A, VLev<T> L, const
 }
 
 // r = f - S u on equation rows, 0 on pointwise rows (u staged through LDS)
-template <class T, class TA, int W>
+template <class T, class TA, int W, bool DIFF = false>
 __global__ __launch_bounds__(TB) void k_tresid(TileArgs<T> A, const TA *S, T *rxo, T *ryo) {
   __shared__ T lx[TW * TW], ly[TW * TW];
   const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
@@ -984,8 +1020,8 @@ __global__ __launch_bounds__(TB) void k_tresid(TileArgs<T> A, const TA *S, T *rx
   const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
   if (i >= X || j >= Y) return;
   const int m = i * Y + j, la = i - i0, lb = j - j0;
-  rxo[m] = A.kx[m] == 0 ? T(A.fx[m] - s_apply_lds<T, TA, W>(S, R, m, lx, ly, la, lb)) : T(0);
-  ryo[m] = A.ky[m] == 0 ? T(A.fy[m] - s_apply_lds<T, TA, W>(S, R, (size_t)N + m, lx, ly, la, lb)) : T(0);
+  rxo[m] = A.kx[m] == 0 ? T(A.fx[m] - s_apply_lds<T, TA, W, DIFF>(S, R, m, lx, ly, la, lb)) : T(0);
+  ryo[m] = A.ky[m] == 0 ? T(A.fy[m] - s_apply_lds<T, TA, W, DIFF>(S, R, (size_t)N + m, lx, ly, la, lb)) : T(0);
 }
 
 // ---------------------------------------------------------------- VelMG host
@@ -1185,7 +1221,10 @@ int VelMG<T>::refresh_from(const VelMG<double> &d, hipStream_t s) {
   cast(d.L[0].gp, L[0].gp, N0);
   cast(d.L[0].dx, L[0].dx, N0);
   cast(d.L[0].dy, L[0].dy, N0);
-  cast(d.L[0].S, L[0].S, (size_t)2 * N0 * 18);
+  if (diff0)
+    hipLaunchKernelGGL(k_sdiff<T>, dim3(nblk((size_t)2 * N0, 256)), dim3(256), 0, s, d.L[0].S.p, N0, L[0].S.p);
+  else
+    cast(d.L[0].S, L[0].S, (size_t)2 * N0 * 18);
   for (int l = 1; l < nl; l++) cast(d.L[l].S, L[l].S, (size_t)2 * L[l].N * 50);
   HIP_OK(hipMemcpyAsync(ainv.p, d.ainv.p, sizeof(double) * (size_t)nc * nc, hipMemcpyDeviceToDevice, s));
   HIP_OK(hipGetLastError());
@@ -1212,11 +1251,13 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   const MapView none{nullptr, nullptr, nullptr, nullptr};
   const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
   TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr, 0,
-                none, none, none, none, 1, ns, 0};
+                none, none, none, none, 1, ns, 0, v.dx.p, v.dy.p};
   auto launch = [&](const TileArgs<T> &a) {
     if (kt) kt->begin(kid, s);
     if (l == 0 && !stencil0)
       hipLaunchKernelGGL(k_tsmooth0<T>, TG, TBK, 0, s, a, o, (const T *)v.dx.p, (const T *)v.dy.p);
+    else if (l == 0 && diff0)
+      hipLaunchKernelGGL((k_tsmooth<T, T, 1, true>), TG, TBK, 0, s, a, (const T *)v.S.p);
     else if (l == 0 && S0d)
       hipLaunchKernelGGL((k_tsmooth<T, double, 1>), TG, TBK, 0, s, a, S0d);
     else if (l == 0)
@@ -1235,6 +1276,8 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   R.zero = 0;
   if (l == 0 && !stencil0)
     hipLaunchKernelGGL(k_vresid<T>, grd2(v.X, v.Y), blk2(), 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
+  else if (l == 0 && diff0)
+    hipLaunchKernelGGL((k_tresid<T, T, 1, true>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
   else if (l == 0 && S0d)
     hipLaunchKernelGGL((k_tresid<T, double, 1>), TG, TBK, 0, s, R, S0d, v.rx.p, v.ry.p);
   else if (l == 0)
@@ -1586,7 +1629,7 @@ __global__ __launch_bounds__(TB) void k_ctsmooth(CellArgs<T> A) {
   T sv[9];
 #pragma unroll
   for (int q = 0; q < 9; q++) sv[q] = ok ? A.S[(size_t)q * N + m] : T(0);
-  const T f = ok ? A.f[m] : T(0), d = ok ? sv[4] : T(1);
+  const T f = ok ? A.f[m] : T(0), rd = T(1) / (ok ? sv[4] : T(1));
   __syncthreads();
   // out-of-grid lanes compute too and skip the store (dense lane masks; see
   // the velocity smoother)
@@ -1600,8 +1643,8 @@ __global__ __launch_bounds__(TB) void k_ctsmooth(CellArgs<T> A) {
 #pragma unroll
         for (int di = -1; di <= 1; di++)
 #pragma unroll
-          for (int dj = -1; dj <= 1; dj++) acc += sv[(di + 1) * 3 + dj + 1] * lu[(la + di) * TW + lb + dj];
-        nv = lu[li] + (f - acc) / d;
+          for (int dj = -1; dj <= 1; dj++) acc = fmadd(sv[(di + 1) * 3 + dj + 1], lu[(la + di) * TW + lb + dj], acc);
+        nv = lu[li] + (f - acc) * rd;
       }
       __syncthreads();
       if (act && ok) lu[li] = nv;
@@ -1628,7 +1671,7 @@ __global__ void k_cresid(int X, int Y, const T *S, const int8_t *mask, const T *
       int a = i + di, b = j + dj;
       a = a < 0 ? 0 : (a >= X ? X - 1 : a);
       b = b < 0 ? 0 : (b >= Y ? Y - 1 : b);
-      acc += S[(size_t)((di + 1) * 3 + dj + 1) * N + m] * u[a * Y + b];
+      acc = fmadd(S[(size_t)((di + 1) * 3 + dj + 1) * N + m], u[a * Y + b], acc);
     }
   r[m] = f[m] - acc;
 }
@@ -1688,7 +1731,7 @@ __global__ __launch_bounds__(256) void k_cresid_restrict(int X, int Y, const T *
           int a = i + di, b = j + dj;
           a = a < 0 ? 0 : (a >= X ? X - 1 : a);
           b = b < 0 ? 0 : (b >= Y ? Y - 1 : b);
-          acc += S[(size_t)((di + 1) * 3 + dj + 1) * N + m] * u[a * Y + b];
+          acc = fmadd(S[(size_t)((di + 1) * 3 + dj + 1) * N + m], u[a * Y + b], acc);
         }
       r = f[m] - acc;
     }
@@ -1752,7 +1795,7 @@ __device__ __forceinline__ T ct_apply(const T (&sv)[9], const T *u, int X, int Y
       int a = i + di, b = j + dj;
       a = a < 0 ? 0 : (a >= X ? X - 1 : a);
       b = b < 0 ? 0 : (b >= Y ? Y - 1 : b);
-      acc += sv[(di + 1) * 3 + dj + 1] * u[a * Y + b];
+      acc = fmadd(sv[(di + 1) * 3 + dj + 1], u[a * Y + b], acc);
     }
   return acc;
 }

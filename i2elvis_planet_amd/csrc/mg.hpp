@@ -120,6 +120,7 @@ struct VelMG {
   int nsmooth_c = 0;        // sweeps on the coarser levels (0: nsmooth)
   int stencil0 = 0;         // finest level smoothed with its probed 3x3 stencil (else matrix-free rows)
   const double *S0d = nullptr;  // that stencil in FP64 (from the FP64 hierarchy) for FP32 iterates
+  int diff0 = 0;            // FP32 hierarchy: finest stencil stored in difference form (refresh_from)
   KTimer *kt = nullptr;     // optional per-launch timing of the smoothers
   int setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,
             const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,

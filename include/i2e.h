@@ -147,9 +147,11 @@ enum { I2E_ORTH_LAGGED = 0, I2E_ORTH_DGKS = 1 };
  * in registers (fewer operations; in FP32 the rounded stencil needs ~6 % more
  * Krylov iterations), or -- mixed mode only -- the FP64 stencil applied to the
  * FP32 iterates with FP64 row sums (STENCIL64: the rows' iteration counts at
- * fewer operations; same as STENCIL in FP64 mode).  AUTO: STENCIL in FP64
- * mode, STENCIL64 in mixed.                                                */
-enum { I2E_FINE_AUTO = 0, I2E_FINE_ROWS = 1, I2E_FINE_STENCIL = 2, I2E_FINE_STENCIL64 = 3 };
+ * fewer operations), or -- mixed mode only -- the FP32 stencil in difference
+ * form, sum s_k (u_k - u_centre) + (row sum) u_centre, which keeps the rows'
+ * cancellation at half the bytes of STENCIL64 (DIFF32).  STENCIL64 and DIFF32
+ * are STENCIL in FP64 mode.  AUTO: STENCIL in FP64 mode, DIFF32 in mixed.  */
+enum { I2E_FINE_AUTO = 0, I2E_FINE_ROWS = 1, I2E_FINE_STENCIL = 2, I2E_FINE_STENCIL64 = 3, I2E_FINE_DIFF32 = 4 };
 
 typedef struct i2e_solver_opts {
   int32_t precision;       /* I2E_PREC_FP64 or I2E_PREC_MIXED                 */

@@ -280,11 +280,12 @@ def test_mixed_precision_error_vs_fp64(orc):
         assert e < 1e-5, f
 
 
-@pytest.mark.parametrize("fine", [1, 2, 3])
+@pytest.mark.parametrize("fine", [1, 2, 3, 4])
 def test_mixed_viterate_fine_level_variants(orc, fine):
     """Mixed precision with each smoother of the finest velocity level
-    (matrix-free rows, FP32 stencil, FP64 stencil on FP32 iterates) reaches
-    the metric's 1e-8 and the oracle's direct solve at 201^2."""
+    (matrix-free rows, FP32 stencil, FP64 stencil on FP32 iterates, FP32
+    stencil in difference form) reaches the metric's 1e-8 and the oracle's
+    direct solve at 201^2."""
     say("case mixed viterate c2 fine_stencil", fine)
     p, hs, _ = configs.make("c2")
     o = orc.Oracle(p, hs)
