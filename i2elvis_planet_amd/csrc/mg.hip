@@ -1340,19 +1340,6 @@ struct FUnit {
 };
 
 template <class T>
-__global__ void k_pdiag(POp<T> o, T *d) {
-  IDX2;
-  if (i >= o.X || j >= o.Y) return;
-  d[i * o.Y + j] = prow(o, FUnit<T>{i * o.Y + j, o.Y}, i, j);
-}
-template <class T>
-__global__ void k_pgs(POp<T> o, int color, T *u, const T *f, const T *d) {
-  IDX2;
-  if (i >= o.X || j >= o.Y || ((i + j) & 1) != color) return;
-  const int m = i * o.Y + j;
-  u[m] += (f[m] - prow(o, FSplit<T>{u, o.Y}, i, j)) / d[m];
-}
-template <class T>
 __global__ void k_presid(POp<T> o, const T *u, const T *f, T *r) {
   IDX2;
   if (i >= o.X || j >= o.Y) return;
@@ -1376,18 +1363,6 @@ __global__ void k_prestrict(int Xc, int Yc, int Yf, MapView mx, MapView my, cons
       }
     }
   f[i * Yc + j] = T(ws > 0 ? s / ws : 0.0);
-}
-template <class T>
-__global__ void k_pprolong(int Xf, int Yf, int Yc, MapView mx, MapView my, const int8_t *fmask, const T *c, T *u) {
-  IDX2;
-  if (i >= Xf || j >= Yf) return;
-  const int m = i * Yf + j;
-  if (fmask[m]) return;
-  int a = mx.i0[i], b = my.i0[j];
-  double wa = mx.w[i], wb = my.w[j];
-  double v = (1 - wa) * ((1 - wb) * (double)c[a * Yc + b] + wb * (double)c[a * Yc + b + 1]) +
-             wa * ((1 - wb) * (double)c[(a + 1) * Yc + b] + wb * (double)c[(a + 1) * Yc + b + 1]);
-  u[m] += T(v);
 }
 template <class T>
 __global__ void k_pdense(POp<T> o, int n, double *aug) {
@@ -1445,11 +1420,11 @@ int ScalMG<T>::setup(const std::vector<double> &x0, const std::vector<double> &y
     ST_OK(v.k.alloc(v.N));
     ST_OK(v.c.alloc(v.N));
     ST_OK(v.dm.alloc(v.N));
-    ST_OK(v.diag.alloc(v.N));
     ST_OK(v.mask.alloc(v.N));
     ST_OK(v.u.alloc(v.N));
     ST_OK(v.f.alloc(v.N));
     ST_OK(v.r.alloc(v.N));
+    ST_OK(v.u2.alloc(v.N));
     if (l + 1 < nl) {
       ST_OK(v.mx.build(xs[l], xs[l + 1]));
       ST_OK(v.my.build(ys[l], ys[l + 1]));
@@ -1482,11 +1457,97 @@ int ScalMG<T>::refresh(const double *k, const double *c, const int8_t *mask, con
     hipLaunchKernelGGL(k_pmask_inject<T>, grd2(cc.X, cc.Y), blk2(), 0, s, cc.X, cc.Y, f.Y, sfi[l].p, sfj[l].p,
                        f.mask.p, cc.mask.p, cc.dm.p);
   }
-  for (int l = 0; l < nl; l++) hipLaunchKernelGGL(k_pdiag<T>, grd2(L[l].X, L[l].Y), blk2(), 0, s, op(l), L[l].diag.p);
   hipLaunchKernelGGL(k_pdense<T>, dim3(nblk(nc, 256), nc), dim3(256), 0, s, op(nl - 1), nc, dense.p);
   gauss_jordan(nc, dense.p, ainv.p, s);
   HIP_OK(hipGetLastError());
   return I2E_OK;
+}
+
+template <class T>
+struct PTArgs {
+  POp<T> o;
+  const T *f;
+  const T *u;  // input iterate (not read when zero != 0)
+  T *out;      // output iterate
+  const T *e;  // correction on the next coarser level, added on load (nullptr: none)
+  int Yc;
+  MapView mx, my;
+  int zero, nsmooth, rev;
+};
+
+// Tiled red-black Gauss-Seidel on the scalar 5-point rows (the velocity
+// smoother's tile scheme: one launch per pass, halo frozen at its loaded
+// value, correction added on load).  Each thread forms its row once in
+// difference form, r = c u + sum_n a_n (u - u_n) with a_n = k_face / (h_n h_avg)
+// (prow's arithmetic regrouped), pointwise rows r = dm u.  Lanes outside the
+// grid compute with zero coefficients and skip the store.
+template <class T>
+__global__ __launch_bounds__(TB) void k_ptsmooth(PTArgs<T> A) {
+  __shared__ T lu[TW * TW];
+  const POp<T> &o = A.o;
+  const int X = o.X, Y = o.Y;
+  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
+  for (int q = threadIdx.x; q < TW * TW; q += TB) {
+    const int i = i0 + q / TW, j = j0 + q % TW;
+    T v = 0;
+    if (i >= 0 && i < X && j >= 0 && j < Y) {
+      const int m = i * Y + j;
+      if (!A.zero) v = A.u[m];
+      if (A.e && !o.mask[m]) v += T(prolong_at(A.e, A.Yc, A.mx, A.my, i, j));
+    }
+    lu[q] = v;
+  }
+  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
+  const int li = (i - i0) * TW + (j - j0);
+  const bool ok = i < X && j < Y;
+  const int m = ok ? i * Y + j : 0;
+  T cw = 0, aL = 0, aR = 0, aT = 0, aB = 0, f = 0, rd = 1;
+  if (ok) {
+    f = A.f[m];
+    if (o.mask[m]) {
+      cw = o.dm[m];
+      rd = T(1) / cw;
+    } else {
+      const double kc = (double)o.k[m];
+      double d = (double)o.c[m];
+      if (i > 0 && i < X - 1) {
+        const double hL = o.x[i] - o.x[i - 1], hR = o.x[i + 1] - o.x[i], hx = (hL + hR) / 2.0;
+        const double l = ((double)o.k[m - Y] + kc) / 2.0 / hL / hx, r = ((double)o.k[m + Y] + kc) / 2.0 / hR / hx;
+        aL = T(l);
+        aR = T(r);
+        d += l + r;
+      }
+      if (j > 0 && j < Y - 1) {
+        const double hT = o.y[j] - o.y[j - 1], hB = o.y[j + 1] - o.y[j], hy = (hT + hB) / 2.0;
+        const double t = ((double)o.k[m - 1] + kc) / 2.0 / hT / hy, b = ((double)o.k[m + 1] + kc) / 2.0 / hB / hy;
+        aT = T(t);
+        aB = T(b);
+        d += t + b;
+      }
+      cw = o.c[m];
+      rd = T(1.0 / d);
+    }
+  }
+  __syncthreads();
+  for (int it = 0; it < A.nsmooth; it++)
+    for (int st = 0; st < 2; st++) {
+      const int col = A.rev ? 1 - st : st;
+      const bool act = ((i + j) & 1) == col;
+      T nv = 0;
+      if (act) {
+        const T uc = lu[li];
+        T r = cw * uc;
+        r = fmadd(aL, uc - lu[li - TW], r);
+        r = fmadd(aR, uc - lu[li + TW], r);
+        r = fmadd(aT, uc - lu[li - 1], r);
+        r = fmadd(aB, uc - lu[li + 1], r);
+        nv = uc + (f - r) * rd;
+      }
+      __syncthreads();
+      if (act && ok) lu[li] = nv;
+      __syncthreads();
+    }
+  if (ok) A.out[m] = lu[li];
 }
 
 template <class T>
@@ -1499,26 +1560,30 @@ int ScalMG<T>::vcycle(int l, hipStream_t s) {
     return I2E_OK;
   }
   const dim3 G = grd2(v.X, v.Y), B = blk2();
+  const dim3 TG((v.Y + TS - 1) / TS, (v.X + TS - 1) / TS), TBK(TB);
   const POp<T> o = op(l);
-  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(v.N, 256)), dim3(256), 0, s, v.N, v.u.p);
-  for (int it = 0; it < nsmooth; it++)
-    for (int col = 0; col < 2; col++) hipLaunchKernelGGL(k_pgs<T>, G, B, 0, s, o, col, v.u.p, v.f.p, v.diag.p);
+  const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
+  // pre-smoothing from a zero iterate into u
+  hipLaunchKernelGGL(k_ptsmooth<T>, TG, TBK, 0, s,
+                     PTArgs<T>{o, v.f.p, nullptr, v.u.p, nullptr, 0, none, none, 1, nsmooth, 0});
   hipLaunchKernelGGL(k_presid<T>, G, B, 0, s, o, v.u.p, v.f.p, v.r.p);
   ScalLevel<T> &c = L[l + 1];
   hipLaunchKernelGGL(k_prestrict<T>, grd2(c.X, c.Y), B, 0, s, c.X, c.Y, v.Y, view(v.mx), view(v.my), c.mask.p, v.r.p,
                      c.f.p);
   ST_OK(vcycle(l + 1, s));
-  hipLaunchKernelGGL(k_pprolong<T>, G, B, 0, s, v.X, v.Y, c.Y, view(v.mx), view(v.my), v.mask.p, c.u.p, v.u.p);
-  for (int it = 0; it < nsmooth; it++)
-    for (int col = 1; col >= 0; col--) hipLaunchKernelGGL(k_pgs<T>, G, B, 0, s, o, col, v.u.p, v.f.p, v.diag.p);
+  // coarse-grid correction added on load, post-smoothing into u2, which
+  // becomes u (the V-cycle's result stays in L[l].u)
+  hipLaunchKernelGGL(k_ptsmooth<T>, TG, TBK, 0, s,
+                     PTArgs<T>{o, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, nsmooth, 1});
+  std::swap(v.u.p, v.u2.p);
   return I2E_OK;
 }
 
 template <class T>
 void ScalMG<T>::release() {
   for (auto &v : L) {
-    v.cx.release(); v.cy.release(); v.k.release(); v.c.release(); v.dm.release(); v.diag.release();
-    v.mask.release(); v.u.release(); v.f.release(); v.r.release();
+    v.cx.release(); v.cy.release(); v.k.release(); v.c.release(); v.dm.release();
+    v.mask.release(); v.u.release(); v.f.release(); v.r.release(); v.u2.release();
   }
   for (auto &b : sfi) b.release();
   for (auto &b : sfj) b.release();

@@ -146,10 +146,10 @@ struct ScalLevel {
   int X, Y, N;
   DBuf<double> cx, cy;   // point coordinates (device)
   std::vector<double> hcx, hcy;
-  DBuf<T> k, c, dm, diag;
+  DBuf<T> k, c, dm;
   DBuf<int8_t> mask;
   std::vector<int8_t> hmask;
-  DBuf<T> u, f, r;
+  DBuf<T> u, f, r, u2;  // u2: output of the post-smoothing pass (swapped with u)
   Map1 mx, my;
 };
 
