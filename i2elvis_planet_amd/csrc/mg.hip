@@ -1828,139 +1828,236 @@ __global__ __launch_bounds__(256) void k_cresid_restrict(int X, int Y, const T *
 }
 
 // The small levels of the cell multigrid (from the first with N <= CT_N
-// down to the coarsest) run as one workgroup: iterates, right-hand sides and
-// residuals in LDS, each thread's stencil rows in registers, phases
-// separated by workgroup barriers.  Replaces ~4 launches per level.
-constexpr int CT_N = 1156, CT_THR = 1024, CT_L = 5, CT_TOT = 1600, CT_K = (CT_N + CT_THR - 1) / CT_THR;
+// down to the coarsest) run as one workgroup, phases separated by workgroup
+// barriers.  Replaces ~4 launches per level.  Layout for issue efficiency on
+// one CU:
+//  - each thread's points are taken in colour-sorted order (all red points,
+//    then all black), so in a colour step whole wavefronts are idle instead
+//    of every wavefront running with half its lanes;
+//  - iterates live in LDS with a zero border, so the 3x3 neighbours are
+//    fixed offsets (no clamping); right-hand sides, residuals and row kinds
+//    in LDS unpadded;
+//  - the first tail level's stencil rows stay in registers; the coarsest
+//    inverse, the coarser levels' stencils, every level's row kinds and the
+//    grid transfer maps arrive as one packed LDS image (CellMG::timg, filled
+//    at refresh), copied by a single round of independent 16-byte loads: as
+//    separate copy loops they cost one global round trip each, ~20 us;
+//  - point indices are formed once.
+constexpr int CT_N = 1156, CT_THR = 1024, CT_L = 5, CT_TOT = 1600, CT_PTOT = 2400, CT_IMG = 112 * 1024,
+              CT_K = (CT_N + CT_THR - 1) / CT_THR;
 
 template <class T>
 struct CTLev {
-  int X, Y, N, off;
-  const T *S;
-  const int8_t *mask;
-  MapView mx, my;  // to the next coarser level
+  int X, Y, N;
+  int off, poff;  // LDS offsets of the unpadded arrays (f, r) and of the padded iterate
+  // byte offsets in the image: stencil (levels >= 1), row kinds, and the
+  // maps to the next coarser level (x: i0, w, rb, rw; then y)
+  int s_b, m_b, xi0, xw, xrb, xrw, yi0, yw, yrb, yrw;
+  const T *S;  // stencil in device memory (the first tail level's rows go to registers)
 };
 template <class T>
 struct CTArgs {
-  int nl, nc;
+  int nl, nc, ptot, n16, ainv_b;
   CTLev<T> L[CT_L];
-  const double *ainv;
+  const uint4 *img;
   const T *fin;
   T *uout;
   int nsmooth;
 };
 
-template <class T>
-__device__ __forceinline__ T ct_apply(const T (&sv)[9], const T *u, int X, int Y, int i, int j) {
-  T acc = 0;
-#pragma unroll
-  for (int di = -1; di <= 1; di++)
-#pragma unroll
-    for (int dj = -1; dj <= 1; dj++) {
-      int a = i + di, b = j + dj;
-      a = a < 0 ? 0 : (a >= X ? X - 1 : a);
-      b = b < 0 ? 0 : (b >= Y ? Y - 1 : b);
-      acc = fmadd(sv[(di + 1) * 3 + dj + 1], u[a * Y + b], acc);
-    }
-  return acc;
-}
+struct TailSeg {
+  size_t dst;
+  const void *src;
+  size_t bytes;
+};
 
-// stencil rows of this thread's points on one tail level (global loads)
+// Layout of the tail from level l0: fills A (when given) and the copy list
+// of the image (when given); returns the image size in bytes, or 0 when the
+// tail does not fit the kernel's LDS arrays.
 template <class T>
-__device__ __forceinline__ void ct_rows(const CTLev<T> &L, T (&sv)[CT_K][9]) {
-#pragma unroll
-  for (int k = 0; k < CT_K; k++) {
-    const int p = threadIdx.x + k * CT_THR;
-#pragma unroll
-    for (int q = 0; q < 9; q++) sv[k][q] = p < L.N ? L.S[(size_t)q * L.N + p] : T(q == 4 ? 1 : 0);
+static size_t tail_layout(const CellMG<T> &g, int l0, CTArgs<T> *A, std::vector<TailSeg> *segs) {
+  const int nl = (int)g.L.size() - l0;
+  if (nl > CT_L || g.L[l0].N > CT_N || g.nc > GJ_LDS_N) return 0;
+  size_t b = 0;
+  auto put = [&](const void *src, size_t bytes) {
+    b = (b + 15) / 16 * 16;
+    const size_t o = b;
+    if (segs && bytes) segs->push_back(TailSeg{o, src, bytes});
+    b += bytes;
+    return (int)o;
+  };
+  CTArgs<T> a{};
+  a.nl = nl;
+  a.nc = g.nc;
+  a.ainv_b = put(g.ainv.p, sizeof(double) * (size_t)g.nc * g.nc);
+  int off = 0, poff = 0;
+  for (int q = 0; q < nl; q++) {
+    const CellLevel<T> &w = g.L[l0 + q];
+    CTLev<T> &c = a.L[q];
+    c.X = w.X;
+    c.Y = w.Y;
+    c.N = w.N;
+    c.off = off;
+    c.poff = poff;
+    c.S = w.S.p;
+    off += w.N;
+    poff += (w.X + 2) * (w.Y + 2);
+    c.s_b = q > 0 ? put(w.S.p, sizeof(T) * 9 * (size_t)w.N) : 0;
+    c.m_b = put(w.mask.p, (size_t)w.N);
+    if (q + 1 < nl) {
+      const int Xc = g.L[l0 + q + 1].X, Yc = g.L[l0 + q + 1].Y;
+      c.xi0 = put(w.mx.i0.p, sizeof(int) * w.X);
+      c.xw = put(w.mx.w.p, sizeof(double) * w.X);
+      c.xrb = put(w.mx.rb.p, sizeof(int) * Xc);
+      c.xrw = put(w.mx.rw.p, sizeof(double) * RW * Xc);
+      c.yi0 = put(w.my.i0.p, sizeof(int) * w.Y);
+      c.yw = put(w.my.w.p, sizeof(double) * w.Y);
+      c.yrb = put(w.my.rb.p, sizeof(int) * Yc);
+      c.yrw = put(w.my.rw.p, sizeof(double) * RW * Yc);
+    }
   }
+  b = (b + 15) / 16 * 16;
+  a.ptot = poff;
+  a.n16 = (int)(b / 16);
+  if (b > (size_t)CT_IMG || off > CT_TOT || poff > CT_PTOT) return 0;
+  if (A) *A = a;
+  return b;
 }
 
+// colour-sorted position q (red points (i+j even) first, then black) -> (i, j)
+__device__ __forceinline__ void ct_point(int q, int X, int Y, int &i, int &j) {
+  const int nred = (X * Y + 1) / 2;
+  const bool red = q < nred;
+  const int r = red ? q : q - nred;
+  const int c0 = red ? (Y + 1) / 2 : Y / 2;  // points of this colour in an even row
+  const int pair = r / Y, rem = r % Y;
+  const int odd = rem >= c0 ? 1 : 0;
+  i = 2 * pair + odd;
+  j = 2 * (odd ? rem - c0 : rem) + (red ? (i & 1) : 1 - (i & 1));
+}
+
+// bilinear correction from a padded coarse iterate (prolong_at's weights)
 template <class T>
-__device__ __forceinline__ void ct_smooth(const CTLev<T> &L, const T (&svr)[CT_K][9], bool inreg, T *u, const T *f,
-                                          int nsmooth, bool rev) {
-  for (int it = 0; it < nsmooth; it++)
-    for (int st = 0; st < 2; st++) {
-      const int col = rev ? 1 - st : st;
-      T sv[CT_K][9];
-      if (inreg) {
-#pragma unroll
-        for (int k = 0; k < CT_K; k++)
-#pragma unroll
-          for (int q = 0; q < 9; q++) sv[k][q] = svr[k][q];
-      } else
-        ct_rows(L, sv);
-      T nv[CT_K];
-#pragma unroll
-      for (int k = 0; k < CT_K; k++) {
-        const int p = threadIdx.x + k * CT_THR;
-        nv[k] = 0;
-        if (p < L.N) {
-          const int i = p / L.Y, j = p % L.Y;
-          if (((i + j) & 1) == col)
-            nv[k] = u[L.off + p] + (f[L.off + p] - ct_apply(sv[k], u + L.off, L.X, L.Y, i, j)) / sv[k][4];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < CT_K; k++) {
-        const int p = threadIdx.x + k * CT_THR;
-        if (p < L.N && (((p / L.Y) + (p % L.Y)) & 1) == col) u[L.off + p] = nv[k];
-      }
-      __syncthreads();
-    }
+__device__ __forceinline__ double ct_prolong(const T *e, int Ycp, const MapView &mx, const MapView &my, int i, int j) {
+  const int a = mx.i0[i] + 1, b = my.i0[j] + 1;
+  const double wa = mx.w[i], wb = my.w[j];
+  return (1 - wa) * ((1 - wb) * (double)e[a * Ycp + b] + wb * (double)e[a * Ycp + b + 1]) +
+         wa * ((1 - wb) * (double)e[(a + 1) * Ycp + b] + wb * (double)e[(a + 1) * Ycp + b + 1]);
 }
 
 template <class T>
 __global__ __launch_bounds__(CT_THR) void k_ctail(CTArgs<T> A) {
-  __shared__ T u[CT_TOT], f[CT_TOT], r[CT_TOT];
+  __shared__ uint4 img[CT_IMG / 16];
+  __shared__ T u[CT_PTOT], f[CT_TOT], r[CT_TOT];
   const int t = threadIdx.x;
-  // the first two tail levels keep their rows in registers; smaller ones re-read them
-  constexpr int NR = sizeof(T) == 4 ? 2 : 1;
-  T sv[NR][CT_K][9];
+  const char *ib = (const char *)img;
+  // the image: four independent 16-byte loads per thread per round (one
+  // round up to 64 KB), stored after all of them have been issued
+  for (int base = 0; base < A.n16; base += 4 * CT_THR) {
+    uint4 v[4];
 #pragma unroll
-  for (int l = 0; l < NR; l++) {
-    if (l < A.nl - 1)
-      ct_rows(A.L[l], sv[l]);
-    else
+    for (int k = 0; k < 4; k++) {
+      const int q = base + t + k * CT_THR;
+      v[k] = A.img[q < A.n16 ? q : A.n16 - 1];  // unconditional (in-bounds) loads stay in registers
+    }
+    // lanes past the end rewrite the last element with its own value
 #pragma unroll
-      for (int k = 0; k < CT_K; k++)
-#pragma unroll
-        for (int q = 0; q < 9; q++) sv[l][k][q] = T(q == 4 ? 1 : 0);
+    for (int k = 0; k < 4; k++) {
+      const int q = base + t + k * CT_THR;
+      img[q < A.n16 ? q : A.n16 - 1] = v[k];
+    }
   }
-  for (int p = t; p < A.L[0].N; p += CT_THR) {
-    f[A.L[0].off + p] = A.fin[p];
-    u[A.L[0].off + p] = 0;
-  }
+  for (int q = t; q < A.ptot; q += CT_THR) u[q] = 0;
+  const double *ainv = (const double *)(ib + A.ainv_b);
+  auto msk = [&](const CTLev<T> &L) { return (const int8_t *)(ib + L.m_b); };
+  auto Ss = [&](const CTLev<T> &L) { return (const T *)(ib + L.s_b); };
+  // transfer maps of a level (to the next coarser one), read from LDS
+  auto lmx = [&](const CTLev<T> &L) {
+    return MapView{(const int *)(ib + L.xi0), (const double *)(ib + L.xw), (const int *)(ib + L.xrb), nullptr,
+                   (const double *)(ib + L.xrw)};
+  };
+  auto lmy = [&](const CTLev<T> &L, int) {
+    return MapView{(const int *)(ib + L.yi0), (const double *)(ib + L.yw), (const int *)(ib + L.yrb), nullptr,
+                   (const double *)(ib + L.yrw)};
+  };
+  for (int p = t; p < A.L[0].N; p += CT_THR) f[A.L[0].off + p] = A.fin[p];
+  // this thread's points per level: linear index (-1: none) and padded index
+  int lp[CT_L][CT_K], pp[CT_L][CT_K];
+#pragma unroll
+  for (int l = 0; l < CT_L; l++)
+#pragma unroll
+    for (int k = 0; k < CT_K; k++) {
+      lp[l][k] = -1;
+      pp[l][k] = 0;
+      const int q = t + k * CT_THR;
+      if (l < A.nl && q < A.L[l].N) {
+        int i, j;
+        ct_point(q, A.L[l].X, A.L[l].Y, i, j);
+        lp[l][k] = i * A.L[l].Y + j;
+        pp[l][k] = A.L[l].poff + (i + 1) * (A.L[l].Y + 2) + j + 1;
+      }
+    }
+  T s0[CT_K][9];
+#pragma unroll
+  for (int k = 0; k < CT_K; k++)
+#pragma unroll
+    for (int q = 0; q < 9; q++) s0[k][q] = lp[0][k] >= 0 ? A.L[0].S[(size_t)q * A.L[0].N + lp[0][k]] : T(0);
   __syncthreads();
+  // S u at this thread's point k of level l (l compile-time after unrolling)
+  auto apply = [&](int l, int k) {
+    const CTLev<T> &L = A.L[l];
+    const int Yp = L.Y + 2, pc = pp[l][k];
+    T acc = 0;
+#pragma unroll
+    for (int di = -1; di <= 1; di++)
+#pragma unroll
+      for (int dj = -1; dj <= 1; dj++) {
+        const int q = (di + 1) * 3 + dj + 1;
+        const T sv = l == 0 ? s0[k][q] : Ss(L)[q * L.N + lp[l][k]];
+        acc = fmadd(sv, u[pc + di * Yp + dj], acc);
+      }
+    return acc;
+  };
+  auto diag = [&](int l, int k) { return l == 0 ? s0[k][4] : Ss(A.L[l])[4 * A.L[l].N + lp[l][k]]; };
+  auto smooth = [&](int l, bool rev) {
+    const CTLev<T> &L = A.L[l];
+    const int nred = (L.N + 1) / 2;
+    for (int it = 0; it < A.nsmooth; it++)
+      for (int st = 0; st < 2; st++) {
+        const int col = rev ? 1 - st : st;
+        T nv[CT_K];
+        bool act[CT_K];
+#pragma unroll
+        for (int k = 0; k < CT_K; k++) {
+          const int q = t + k * CT_THR;
+          act[k] = q < L.N && (q >= nred ? 1 : 0) == col;
+          nv[k] = 0;
+          if (act[k]) nv[k] = u[pp[l][k]] + (f[L.off + lp[l][k]] - apply(l, k)) / diag(l, k);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < CT_K; k++)
+          if (act[k]) u[pp[l][k]] = nv[k];
+        __syncthreads();
+      }
+  };
   // down: pre-smooth, residual, restriction
 #pragma unroll
   for (int l = 0; l < CT_L - 1; l++) {
     if (l >= A.nl - 1) break;
     const CTLev<T> &L = A.L[l], &C = A.L[l + 1];
-    const bool inreg = l < NR;
-    ct_smooth(L, sv[l < NR ? l : 0], inreg, u, f, A.nsmooth, false);
-    T sr[CT_K][9];
-    if (inreg) {
+    smooth(l, false);
 #pragma unroll
-      for (int k = 0; k < CT_K; k++)
-#pragma unroll
-        for (int q = 0; q < 9; q++) sr[k][q] = sv[l < NR ? l : 0][k][q];
-    } else
-      ct_rows(L, sr);
-#pragma unroll
-    for (int k = 0; k < CT_K; k++) {
-      const int p = t + k * CT_THR;
-      if (p < L.N) {
-        const int i = p / L.Y, j = p % L.Y;
-        r[L.off + p] = L.mask[p] ? T(0) : f[L.off + p] - ct_apply(sr[k], u + L.off, L.X, L.Y, i, j);
+    for (int k = 0; k < CT_K; k++)
+      if (lp[l][k] >= 0) {
+        const int m = L.off + lp[l][k];
+        r[m] = msk(L)[lp[l][k]] ? T(0) : f[m] - apply(l, k);
       }
-    }
     __syncthreads();
+    const MapView mx = lmx(L), my = lmy(L, C.X);
     for (int p = t; p < C.N; p += CT_THR) {
       const int i = p / C.Y, j = p % C.Y;
-      f[C.off + p] = C.mask[p] ? T(0) : restrict_rw(r + L.off, L.Y, L.mx, L.my, i, j);
-      u[C.off + p] = 0;
+      f[C.off + p] = msk(C)[p] ? T(0) : restrict_rw(r + L.off, L.Y, mx, my, i, j);
     }
     __syncthreads();
   }
@@ -1970,9 +2067,9 @@ __global__ __launch_bounds__(CT_THR) void k_ctail(CTArgs<T> A) {
     const int wv = t >> 6, lane = t & 63, nw = CT_THR / 64;
     for (int row = wv; row < A.nc; row += nw) {
       double acc = 0;
-      for (int c = lane; c < A.nc; c += 64) acc += A.ainv[(size_t)row * A.nc + c] * (double)f[L.off + c];
+      for (int c = lane; c < A.nc; c += 64) acc += ainv[row * A.nc + c] * (double)f[L.off + c];
       acc = wave_sum(acc);
-      if (lane == 0) u[L.off + row] = T(acc);
+      if (lane == 0) u[L.poff + (row / L.Y + 1) * (L.Y + 2) + row % L.Y + 1] = T(acc);
     }
     __syncthreads();
   }
@@ -1981,14 +2078,19 @@ __global__ __launch_bounds__(CT_THR) void k_ctail(CTArgs<T> A) {
   for (int l = CT_L - 2; l >= 0; l--) {
     if (l >= A.nl - 1) continue;
     const CTLev<T> &L = A.L[l], &C = A.L[l + 1];
-    for (int p = t; p < L.N; p += CT_THR) {
-      const int i = p / L.Y, j = p % L.Y;
-      if (!L.mask[p]) u[L.off + p] += T(prolong_at(u + C.off, C.Y, L.mx, L.my, i, j));
-    }
+    const MapView mx = lmx(L), my = lmy(L, C.X);
+#pragma unroll
+    for (int k = 0; k < CT_K; k++)
+      if (lp[l][k] >= 0 && !msk(L)[lp[l][k]]) {
+        const int i = lp[l][k] / L.Y, j = lp[l][k] % L.Y;
+        u[pp[l][k]] += T(ct_prolong(u + C.poff, C.Y + 2, mx, my, i, j));
+      }
     __syncthreads();
-    ct_smooth(L, sv[l < NR ? l : 0], l < NR, u, f, A.nsmooth, true);
+    smooth(l, true);
   }
-  for (int p = t; p < A.L[0].N; p += CT_THR) A.uout[p] = u[A.L[0].off + p];
+#pragma unroll
+  for (int k = 0; k < CT_K; k++)
+    if (lp[0][k] >= 0) A.uout[lp[0][k]] = u[pp[0][k]];
 }
 
 template <class T>
@@ -2029,6 +2131,26 @@ int CellMG<T>::setup(const std::vector<double> &x0, const std::vector<double> &y
   ST_OK(ainv.alloc((size_t)nc * nc));
   ST_OK(overflow.alloc(1));
   HIP_OK(hipMemset(overflow.p, 0, sizeof(int)));
+  // the tail takes over at the first level that fits it
+  tail_l = -1;
+  for (int l = 0; l < nl; l++) {
+    const size_t b = tail_layout(*this, l, (CTArgs<T> *)nullptr, nullptr);
+    if (b) {
+      tail_l = l;
+      ST_OK(timg.alloc(b / 16));
+      break;
+    }
+  }
+  return I2E_OK;
+}
+
+template <class T>
+int CellMG<T>::tail_pack(hipStream_t s) {
+  if (tail_l < 0) return I2E_OK;
+  std::vector<TailSeg> segs;
+  tail_layout(*this, tail_l, (CTArgs<T> *)nullptr, &segs);
+  for (const TailSeg &g : segs)
+    HIP_OK(hipMemcpyAsync((char *)timg.p + g.dst, g.src, g.bytes, hipMemcpyDeviceToDevice, s));
   return I2E_OK;
 }
 
@@ -2050,6 +2172,7 @@ int CellMG<T>::refresh(const double *S0, const int8_t *mask0, hipStream_t s) {
                      L[nl - 1].Y, dense.p);
   gauss_jordan(nc, dense.p, ainv.p, s);
   HIP_OK(hipGetLastError());
+  ST_OK(tail_pack(s));
   int nover = 0;
   HIP_OK(hipMemcpyAsync(&nover, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
@@ -2074,30 +2197,20 @@ int CellMG<T>::refresh_from(const CellMG<double> &d, hipStream_t s) {
   }
   HIP_OK(hipMemcpyAsync(ainv.p, d.ainv.p, sizeof(double) * (size_t)nc * nc, hipMemcpyDeviceToDevice, s));
   HIP_OK(hipGetLastError());
-  return I2E_OK;
+  return tail_pack(s);
 }
 
 template <class T>
 int CellMG<T>::vcycle(int l, hipStream_t s) {
   const int nl = (int)L.size();
   CellLevel<T> &v = L[l];
-  if (v.N <= CT_N && nl - l <= CT_L && nc <= GJ_LDS_N) {
+  if (l == tail_l) {
     CTArgs<T> A{};
-    A.nl = nl - l;
-    A.nc = nc;
-    int off = 0;
-    for (int q = 0; q < A.nl; q++) {
-      CellLevel<T> &w = L[l + q];
-      const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
-      A.L[q] = CTLev<T>{w.X, w.Y, w.N, off, w.S.p, w.mask.p, q + 1 < A.nl ? view(w.mx) : none,
-                        q + 1 < A.nl ? view(w.my) : none};
-      off += w.N;
+    if (!tail_layout(*this, l, &A, nullptr)) {
+      set_error("cell multigrid tail layout changed");
+      return I2E_ERR_ARG;
     }
-    if (off > CT_TOT) {
-      set_error("cell multigrid tail holds %d points (at most %d)", off, CT_TOT);
-      return I2E_ERR_CAPACITY;
-    }
-    A.ainv = ainv.p;
+    A.img = timg.p;
     A.fin = v.f.p;
     A.uout = v.u.p;
     A.nsmooth = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
@@ -2141,6 +2254,7 @@ void CellMG<T>::release() {
   dense.release();
   ainv.release();
   overflow.release();
+  timg.release();
 }
 
 template struct CellMG<float>;

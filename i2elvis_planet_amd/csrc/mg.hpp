@@ -193,7 +193,11 @@ struct CellMG {
   DBuf<int> overflow;
   int nsmooth = 2;
   int nsmooth_c = 0;        // sweeps on the coarser levels (0: nsmooth)
+  int tail_l = -1;          // first level run by the single-workgroup tail kernel (-1: none)
+  DBuf<uint4> timg;         // the tail's LDS image (coarse inverse, stencils, row kinds, maps)
   int setup(const std::vector<double> &x0, const std::vector<double> &y0, int coarse_max);
+  // copies the current coarse inverse, stencils and row kinds into timg
+  int tail_pack(hipStream_t s);
   // fine stencil (FP64, 9 x N0) and mask given on device; forms the
   // Galerkin hierarchy and the coarsest inverse in FP64
   int refresh(const double *S0, const int8_t *mask0, hipStream_t s);
