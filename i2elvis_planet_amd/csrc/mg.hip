@@ -788,7 +788,9 @@ __global__ void k_sresid(const T *S, int X, int Y, const int8_t *kx, const int8_
 }
 
 // ---------------------------------------------------------------- tiled smoothers
-// One workgroup smooths a TS x TS tile of a level in LDS: the tile and a halo
+// One workgroup smooths a TSY x TSX tile of a level in LDS (32 points along
+// the contiguous j direction, so that each wavefront reads whole 128-byte
+// lines of the stencil arrays): the tile and a halo
 // of TH points are loaded once (with the coarse-grid correction added on the
 // way up), nsmooth sweeps of the four colour steps (Vx red, Vx black, Vy red,
 // Vy black; reversed after the coarse-grid correction) run between workgroup
@@ -800,7 +802,20 @@ __global__ void k_sresid(const T *S, int X, int Y, const int8_t *kx, const int8_
 // same-colour points couple; on the finest level they do not, and this is
 // red-black Gauss-Seidel).  Every level smooths with its explicit stencil
 // (finest: 3x3 window probed from the exact rows; coarser: 5x5 Galerkin).
-constexpr int TS = 16, TH = 2, TW = TS + 2 * TH, TB = 256;
+constexpr int TH = 2;
+// tile shape per precision: FP32 levels read whole 128-byte lines with 32
+// points along j (and 32x16 tiles cut fewer couplings than 16x16: 88 vs 90
+// Krylov iterations at 501^2); FP64 keeps 16x16, where the twice heavier
+// colour steps of the larger tile cost more on the few-tile levels than the
+// wider reads save
+template <class T>
+struct TileShape {
+  static constexpr int SX = sizeof(T) == 4 ? 32 : 16, SY = 16, WX = SX + 2 * TH, WY = SY + 2 * TH, B = SX * SY;
+};
+#define TILE_CONSTS(T)                                                                                   \
+  constexpr int TSX = TileShape<T>::SX, TSY = TileShape<T>::SY, TWX = TileShape<T>::WX, TWY = TileShape<T>::WY, \
+                TB = TileShape<T>::B;                                                                     \
+  (void)TSX; (void)TSY; (void)TWX; (void)TWY; (void)TB
 
 template <class T>
 struct TileArgs {
@@ -827,8 +842,9 @@ __device__ __forceinline__ double prolong_at(const T *e, int Yc, const MapView &
 
 template <class T>
 __device__ __forceinline__ void tile_load(const TileArgs<T> &A, T *lx, T *ly, int i0, int j0) {
-  for (int q = threadIdx.x; q < TW * TW; q += TB) {
-    const int i = i0 + q / TW, j = j0 + q % TW;
+  TILE_CONSTS(T);
+  for (int q = threadIdx.x; q < TWX * TWY; q += TB) {
+    const int i = i0 + q / TWX, j = j0 + q % TWX;
     T vx = 0, vy = 0;
     if (i >= 0 && i < A.X && j >= 0 && j < A.Y) {
       const int m = i * A.Y + j;
@@ -850,17 +866,18 @@ __device__ __forceinline__ void tile_load(const TileArgs<T> &A, T *lx, T *ly, in
 // the coefficients that would multiply them)
 template <class T, class TA, int W, bool DIFF>
 __device__ __forceinline__ TA s_apply_lds(const TA *S, size_t R, size_t r, const T *lx, const T *ly, int la, int lb) {
+  TILE_CONSTS(T);
   TA acc = 0;
 #pragma unroll
   for (int cc = 0; cc < 2; cc++) {
     const T *u = cc ? ly : lx;
-    const TA uc = DIFF ? TA(u[la * TW + lb]) : TA(0);
+    const TA uc = DIFF ? TA(u[la * TWX + lb]) : TA(0);
 #pragma unroll
     for (int di = -W; di <= W; di++)
 #pragma unroll
       for (int dj = -W; dj <= W; dj++) {
         const TA s = S[(size_t)sslot(W, cc, di, dj) * R + r];
-        const TA v = TA(u[(la + di) * TW + lb + dj]);
+        const TA v = TA(u[(la + di) * TWX + lb + dj]);
         acc = fmadd(s, DIFF && (di != 0 || dj != 0) ? v - uc : v, acc);
       }
   }
@@ -872,14 +889,15 @@ __device__ __forceinline__ TA s_apply_lds(const TA *S, size_t R, size_t r, const
 // DIFF: the finest level's FP32 stencil in difference form (k_sdiff), with
 // the row diagonals read from A.dx / A.dy
 template <class T, class TA, int W, bool DIFF = false>
-__global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
-  __shared__ T lx[TW * TW], ly[TW * TW];
-  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
+__global__ __launch_bounds__(TileShape<T>::B) void k_tsmooth(TileArgs<T> A, const TA *S) {
+  TILE_CONSTS(T);
+  __shared__ T lx[TWX * TWY], ly[TWX * TWY];
+  const int i0 = blockIdx.y * TSY - TH, j0 = blockIdx.x * TSX - TH;
   const int X = A.X, Y = A.Y, N = X * Y;
   const size_t R = 2 * (size_t)N;
   tile_load(A, lx, ly, i0, j0);
-  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
-  const int li = (i - i0) * TW + (j - j0);
+  const int i = i0 + TH + (int)threadIdx.x / TSX, j = j0 + TH + (int)threadIdx.x % TSX;
+  const int li = (i - i0) * TWX + (j - j0);
   const bool ok = i < X && j < Y;
   const int m = ok ? i * Y + j : 0;
   const T fx = ok ? A.fx[m] : T(0), fy = ok ? A.fy[m] : T(0);
@@ -895,18 +913,18 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
   const TA dx = ok ? (DIFF ? TA(A.dx[m]) : sx[sslot(W, 0, 0, 0)]) : TA(1);
   const TA dy = ok ? (DIFF ? TA(A.dy[m]) : sy[sslot(W, 1, 0, 0)]) : TA(1);
   const TA rdx = TA(1) / dx, rdy = TA(1) / dy;
-  const int la = li / TW, lb = li % TW;
+  const int la = li / TWX, lb = li % TWX;
   auto apply = [&](const TA *sv) {
     TA acc = 0;
 #pragma unroll
     for (int cc = 0; cc < 2; cc++) {
       const T *u = cc ? ly : lx;
-      const TA uc = DIFF ? TA(u[la * TW + lb]) : TA(0);
+      const TA uc = DIFF ? TA(u[la * TWX + lb]) : TA(0);
 #pragma unroll
       for (int di = -W; di <= W; di++)
 #pragma unroll
         for (int dj = -W; dj <= W; dj++) {
-          const TA v = TA(u[(la + di) * TW + lb + dj]);
+          const TA v = TA(u[(la + di) * TWX + lb + dj]);
           acc = fmadd(sv[sslot(W, cc, di, dj)], DIFF && (di != 0 || dj != 0) ? v - uc : v, acc);
         }
     }
@@ -957,10 +975,11 @@ __global__ __launch_bounds__(TB) void k_tsmooth(TileArgs<T> A, const TA *S) {
 // unknowns of the finest level read from the LDS tile (global indices)
 template <class T>
 struct LdsV {
+  static constexpr int TWX = TileShape<T>::WX;
   const T *lx, *ly;
   int i0, j0;
-  __device__ __forceinline__ T vx(int i, int j) const { return lx[(i - i0) * TW + (j - j0)]; }
-  __device__ __forceinline__ T vy(int i, int j) const { return ly[(i - i0) * TW + (j - j0)]; }
+  __device__ __forceinline__ T vx(int i, int j) const { return lx[(i - i0) * TWX + (j - j0)]; }
+  __device__ __forceinline__ T vy(int i, int j) const { return ly[(i - i0) * TWX + (j - j0)]; }
   __device__ __forceinline__ T pp(int, int) const { return T(0); }
 };
 
@@ -968,13 +987,14 @@ struct LdsV {
 // coefficient fields are read instead of an explicit 3x3 stencil, which
 // would be several times the bytes), red-black Gauss-Seidel
 template <class T>
-__global__ __launch_bounds__(TB) void k_tsmooth0(TileArgs<T> A, VLev<T> L, const T *dxa, const T *dya) {
-  __shared__ T lx[TW * TW], ly[TW * TW];
-  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
+__global__ __launch_bounds__(TileShape<T>::B) void k_tsmooth0(TileArgs<T> A, VLev<T> L, const T *dxa, const T *dya) {
+  TILE_CONSTS(T);
+  __shared__ T lx[TWX * TWY], ly[TWX * TWY];
+  const int i0 = blockIdx.y * TSY - TH, j0 = blockIdx.x * TSX - TH;
   const int X = A.X, Y = A.Y;
   tile_load(A, lx, ly, i0, j0);
-  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
-  const int li = (i - i0) * TW + (j - j0);
+  const int i = i0 + TH + (int)threadIdx.x / TSX, j = j0 + TH + (int)threadIdx.x % TSX;
+  const int li = (i - i0) * TWX + (j - j0);
   const bool ok = i < X && j < Y;
   const int m = ok ? i * Y + j : 0;
   const T fx = ok ? A.fx[m] : T(0), fy = ok ? A.fy[m] : T(0);
@@ -1010,14 +1030,15 @@ __global__ __launch_bounds__(TB) void k_tsmooth0(TileArgs<T> A, VLev<T> L, const
 
 // r = f - S u on equation rows, 0 on pointwise rows (u staged through LDS)
 template <class T, class TA, int W, bool DIFF = false>
-__global__ __launch_bounds__(TB) void k_tresid(TileArgs<T> A, const TA *S, T *rxo, T *ryo) {
-  __shared__ T lx[TW * TW], ly[TW * TW];
-  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
+__global__ __launch_bounds__(TileShape<T>::B) void k_tresid(TileArgs<T> A, const TA *S, T *rxo, T *ryo) {
+  TILE_CONSTS(T);
+  __shared__ T lx[TWX * TWY], ly[TWX * TWY];
+  const int i0 = blockIdx.y * TSY - TH, j0 = blockIdx.x * TSX - TH;
   const int X = A.X, Y = A.Y, N = X * Y;
   const size_t R = 2 * (size_t)N;
   tile_load(A, lx, ly, i0, j0);
   __syncthreads();
-  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
+  const int i = i0 + TH + (int)threadIdx.x / TSX, j = j0 + TH + (int)threadIdx.x % TSX;
   if (i >= X || j >= Y) return;
   const int m = i * Y + j, la = i - i0, lb = j - j0;
   rxo[m] = A.kx[m] == 0 ? T(A.fx[m] - s_apply_lds<T, TA, W, DIFF>(S, R, m, lx, ly, la, lb)) : T(0);
@@ -1240,7 +1261,8 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
                        v.uy.p);
     return I2E_OK;
   }
-  const dim3 TG((v.Y + TS - 1) / TS, (v.X + TS - 1) / TS), TBK(TB);
+  TILE_CONSTS(T);
+  const dim3 TG((v.Y + TSX - 1) / TSX, (v.X + TSY - 1) / TSY), TBK(TB);
   const int kid = l == 0 ? K_VSMOOTH0 : K_VSMOOTHC;
   const VLev<T> o = lev(l);
   // algorithmic bytes of one smoothing pass: rhs, input and output iterate
@@ -1482,13 +1504,14 @@ struct PTArgs {
 // (prow's arithmetic regrouped), pointwise rows r = dm u.  Lanes outside the
 // grid compute with zero coefficients and skip the store.
 template <class T>
-__global__ __launch_bounds__(TB) void k_ptsmooth(PTArgs<T> A) {
-  __shared__ T lu[TW * TW];
+__global__ __launch_bounds__(TileShape<T>::B) void k_ptsmooth(PTArgs<T> A) {
+  TILE_CONSTS(T);
+  __shared__ T lu[TWX * TWY];
   const POp<T> &o = A.o;
   const int X = o.X, Y = o.Y;
-  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
-  for (int q = threadIdx.x; q < TW * TW; q += TB) {
-    const int i = i0 + q / TW, j = j0 + q % TW;
+  const int i0 = blockIdx.y * TSY - TH, j0 = blockIdx.x * TSX - TH;
+  for (int q = threadIdx.x; q < TWX * TWY; q += TB) {
+    const int i = i0 + q / TWX, j = j0 + q % TWX;
     T v = 0;
     if (i >= 0 && i < X && j >= 0 && j < Y) {
       const int m = i * Y + j;
@@ -1497,8 +1520,8 @@ __global__ __launch_bounds__(TB) void k_ptsmooth(PTArgs<T> A) {
     }
     lu[q] = v;
   }
-  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
-  const int li = (i - i0) * TW + (j - j0);
+  const int i = i0 + TH + (int)threadIdx.x / TSX, j = j0 + TH + (int)threadIdx.x % TSX;
+  const int li = (i - i0) * TWX + (j - j0);
   const bool ok = i < X && j < Y;
   const int m = ok ? i * Y + j : 0;
   T cw = 0, aL = 0, aR = 0, aT = 0, aB = 0, f = 0, rd = 1;
@@ -1537,8 +1560,8 @@ __global__ __launch_bounds__(TB) void k_ptsmooth(PTArgs<T> A) {
       if (act) {
         const T uc = lu[li];
         T r = cw * uc;
-        r = fmadd(aL, uc - lu[li - TW], r);
-        r = fmadd(aR, uc - lu[li + TW], r);
+        r = fmadd(aL, uc - lu[li - TWX], r);
+        r = fmadd(aR, uc - lu[li + TWX], r);
         r = fmadd(aT, uc - lu[li - 1], r);
         r = fmadd(aB, uc - lu[li + 1], r);
         nv = uc + (f - r) * rd;
@@ -1560,7 +1583,8 @@ int ScalMG<T>::vcycle(int l, hipStream_t s) {
     return I2E_OK;
   }
   const dim3 G = grd2(v.X, v.Y), B = blk2();
-  const dim3 TG((v.Y + TS - 1) / TS, (v.X + TS - 1) / TS), TBK(TB);
+  TILE_CONSTS(T);
+  const dim3 TG((v.Y + TSX - 1) / TSX, (v.X + TSY - 1) / TSY), TBK(TB);
   const POp<T> o = op(l);
   const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
   // pre-smoothing from a zero iterate into u
@@ -1673,12 +1697,13 @@ struct CellArgs {
 // smoother for the tile scheme); the correction from the next coarser level
 // is added on load on the way up
 template <class T>
-__global__ __launch_bounds__(TB) void k_ctsmooth(CellArgs<T> A) {
-  __shared__ T lu[TW * TW];
-  const int i0 = blockIdx.y * TS - TH, j0 = blockIdx.x * TS - TH;
+__global__ __launch_bounds__(TileShape<T>::B) void k_ctsmooth(CellArgs<T> A) {
+  TILE_CONSTS(T);
+  __shared__ T lu[TWX * TWY];
+  const int i0 = blockIdx.y * TSY - TH, j0 = blockIdx.x * TSX - TH;
   const int X = A.X, Y = A.Y, N = X * Y;
-  for (int q = threadIdx.x; q < TW * TW; q += TB) {
-    const int i = i0 + q / TW, j = j0 + q % TW;
+  for (int q = threadIdx.x; q < TWX * TWY; q += TB) {
+    const int i = i0 + q / TWX, j = j0 + q % TWX;
     T v = 0;
     if (i >= 0 && i < X && j >= 0 && j < Y) {
       const int m = i * Y + j;
@@ -1687,8 +1712,8 @@ __global__ __launch_bounds__(TB) void k_ctsmooth(CellArgs<T> A) {
     }
     lu[q] = v;
   }
-  const int i = i0 + TH + (int)threadIdx.x / TS, j = j0 + TH + (int)threadIdx.x % TS;
-  const int la = i - i0, lb = j - j0, li = la * TW + lb;
+  const int i = i0 + TH + (int)threadIdx.x / TSX, j = j0 + TH + (int)threadIdx.x % TSX;
+  const int la = i - i0, lb = j - j0, li = la * TWX + lb;
   const bool ok = i < X && j < Y;
   const int m = ok ? i * Y + j : 0;
   T sv[9];
@@ -1708,7 +1733,7 @@ __global__ __launch_bounds__(TB) void k_ctsmooth(CellArgs<T> A) {
 #pragma unroll
         for (int di = -1; di <= 1; di++)
 #pragma unroll
-          for (int dj = -1; dj <= 1; dj++) acc = fmadd(sv[(di + 1) * 3 + dj + 1], lu[(la + di) * TW + lb + dj], acc);
+          for (int dj = -1; dj <= 1; dj++) acc = fmadd(sv[(di + 1) * 3 + dj + 1], lu[(la + di) * TWX + lb + dj], acc);
         nv = lu[li] + (f - acc) * rd;
       }
       __syncthreads();
@@ -2222,7 +2247,8 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
                        v.u.p);
     return I2E_OK;
   }
-  const dim3 TG((v.Y + TS - 1) / TS, (v.X + TS - 1) / TS), TBK(TB);
+  TILE_CONSTS(T);
+  const dim3 TG((v.Y + TSX - 1) / TSX, (v.X + TSY - 1) / TSY), TBK(TB);
   const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
   const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
   CellArgs<T> A{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, nullptr, 0, none, none, 1, ns, 0};
