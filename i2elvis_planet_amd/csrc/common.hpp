@@ -63,10 +63,32 @@ struct KTimer {
   }
 };
 
-// butterfly sum over the 64 lanes of a wavefront (fixed order: reproducible)
+// v + (v of the lane that the DPP control selects), both 32-bit halves moved
+// by the VALU's data-parallel primitives; rows outside ROW_MASK add 0
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+  return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wavefront, returned to every lane; all lanes
+// must be active.  Fixed order (reproducible): pairs and quads by quad
+// permutes, 8- and 16-lane rows by half-row and row mirrors, then rows
+// chained by row broadcasts 15 and 31, so lane 63 ends with the total.  The
+// whole reduction stays in the VALU (lane shuffles through ds_bpermute cost
+// an LDS round trip per step, which bounded the Gram-Schmidt projection).
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v = dpp_add<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x141, 0xF>(v);  // row_half_mirror
+  v = dpp_add<0x140, 0xF>(v);  // row_mirror
+  v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 __device__ __forceinline__ void aadd(double *a, double v) { unsafeAtomicAdd(a, v); }
