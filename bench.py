@@ -31,7 +31,7 @@ TRAFFIC_FILES = {
 # rocprof kernel-name prefix and, where only some launches count, the grid
 # size (threads) of the launches bench.py times for each roofline kernel
 PMC_MATCH = {
-    "vsmoothc": ("void i2e::k_tsmooth<float, 2>(", None),
+    "vsmoothc": ("void i2e::k_tsmooth<float, float, 2, false>(", None),
     "gsdot": ("i2e::k_gsdot2(", "stokes"),
 }
 
@@ -64,8 +64,10 @@ def cpu_baseline(sample_steps=1):
 ROOFLINE_KERNELS = {
     "gsdot": "k_gsdot2 (fused Gram-Schmidt projection pass of the Stokes FGMRES)",
     "gsaxpy": "k_gsaxpy2 (fused Gram-Schmidt update pass of the Stokes FGMRES)",
-    "vsmoothc": "k_tsmooth<float,2> (tiled velocity smoothing pass, Galerkin levels 251^2..17^2, mixed mode)",
-    "vsmooth0": "k_tsmooth0<float> (tiled velocity smoothing pass, finest level, mixed mode)",
+    "vsmoothc": "k_tsmooth<float,float,2> (tiled velocity smoothing pass, 32x16 tiles, Galerkin levels "
+                "251^2..17^2, mixed mode)",
+    "vsmooth0": "k_tsmooth<float,float,1,diff> (tiled velocity smoothing pass, finest level, FP32 "
+                "difference-form stencil, mixed mode)",
     "stokes_apply": "k_stokes_apply (matrix-free Stokes operator)",
     "advect": "k_movemark (RK4 marker advection)",
     "p2g": "k_ronu_markers (marker-to-node scatter)",

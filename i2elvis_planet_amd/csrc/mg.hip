@@ -1266,10 +1266,22 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   const int kid = l == 0 ? K_VSMOOTH0 : K_VSMOOTHC;
   const VLev<T> o = lev(l);
   // algorithmic bytes of one smoothing pass: rhs, input and output iterate
-  // (two components each), the row kinds, and the coefficients once (finest:
-  // nu, nd, ro, gp and the two diagonals; Galerkin levels: the 50-slot stencil)
-  const double coef = l == 0 ? 6.0 : 100.0;
-  const double bytes = sizeof(T) * (6.0 + coef) * v.N + 2.0 * v.N;
+  // (two components each), the row kinds, and the coefficients once per
+  // point: Galerkin levels 2 x 50 stencil slots; finest level 2 x 18 slots
+  // (FP64 ones for FP32 iterates; in difference form plus the two
+  // diagonals), or for the matrix-free rows nu, nd, ro, gp and the diagonals
+  double coef;
+  if (l > 0)
+    coef = 100.0 * sizeof(T);
+  else if (!stencil0)
+    coef = 6.0 * sizeof(T);
+  else if (diff0)
+    coef = 38.0 * sizeof(T);
+  else if (S0d)
+    coef = 36.0 * sizeof(double);
+  else
+    coef = 36.0 * sizeof(T);
+  const double bytes = (6.0 * sizeof(T) + coef) * v.N + 2.0 * v.N;
   const MapView none{nullptr, nullptr, nullptr, nullptr};
   const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
   TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr, 0,
