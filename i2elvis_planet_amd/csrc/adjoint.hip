@@ -98,15 +98,12 @@ struct Dst {
 };
 template <int NV>
 __device__ __forceinline__ void seg_scatter(int key, double (&v)[NV][4], const Dst<NV> &d, int Y, int lane) {
-  const int kprev = __shfl_up(key, 1, 64);
-  const bool head = lane == 0 || kprev != key;
-  const unsigned long long heads = __ballot(head);
-  const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+  const SegScan sc(key, lane);
 #pragma unroll
   for (int q = 0; q < NV; q++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) v[q][c] = seg_sum(v[q][c], seg, lane);
-  if (head && key >= 0) {
+    for (int c = 0; c < 4; c++) v[q][c] = sc.sum(v[q][c]);
+  if (sc.tail && key >= 0) {
     const int m10 = key / Y, m20 = key % Y;
 #pragma unroll
     for (int c = 0; c < 4; c++) {

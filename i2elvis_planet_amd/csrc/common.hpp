@@ -63,14 +63,19 @@ struct KTimer {
   }
 };
 
-// v + (v of the lane that the DPP control selects), both 32-bit halves moved
-// by the VALU's data-parallel primitives; rows outside ROW_MASK add 0
+// v of the lane that the DPP control selects, both 32-bit halves moved by the
+// VALU's data-parallel primitives; 0 in rows outside ROW_MASK and where the
+// source lane lies outside the lane's row (row shifts)
 template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_add(double v) {
+__device__ __forceinline__ double dpp_get(double v) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xF, false);
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
-  return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double v) {
+  return v + dpp_get<CTRL, ROW_MASK>(v);
 }
 
 // Sum over the 64 lanes of a wavefront, returned to every lane; all lanes
@@ -93,17 +98,42 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __device__ __forceinline__ void aadd(double *a, double v) { unsafeAtomicAdd(a, v); }
 
-// segmented sum over the lanes of a wavefront whose segment ids are equal
-// and contiguous: afterwards the first lane of each segment holds its total
-__device__ __forceinline__ double seg_sum(double v, int seg, int lane) {
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double o = __shfl_down(v, off, 64);
-    const int so = __shfl_down(seg, off, 64);
-    if (lane + off < 64 && so == seg) v += o;
+// Segmented sums over the lanes of a wavefront whose keys are equal on
+// contiguous runs (the scatters' cell index): sum(v) is the inclusive prefix
+// of v within the lane's run, so the run's last lane (tail) holds its total.
+// Hillis-Steele in DPP: row shifts 1, 2, 4, 8, then row broadcasts 15 and 31,
+// each step adding the shifted value only where its source lane lies in the
+// same run (lane - k >= the run's first lane, fixed once per wavefront for
+// every sum).  The step weights are 0/1 factors of an FMA, not a select: a
+// select lets the compiler run the DPP move under the predicate's exec mask,
+// and DPP reads lanes that EXEC disables as out of range (0).  All lanes must
+// be active; the summed values must be finite.
+struct SegScan {
+  double f1, f2, f4, f8, f15, f31;
+  bool tail;
+  __device__ __forceinline__ SegScan(int key, int lane) {
+    const int kprev = __shfl_up(key, 1, 64), knext = __shfl_down(key, 1, 64);
+    const bool head = lane == 0 || kprev != key;
+    tail = lane == 63 || knext != key;
+    const unsigned long long heads = __ballot(head);
+    const int s = 63 - __clzll(heads & ((2ull << lane) - 1ull));  // first lane of the run
+    f1 = lane - 1 >= s ? 1.0 : 0.0;
+    f2 = lane - 2 >= s ? 1.0 : 0.0;
+    f4 = lane - 4 >= s ? 1.0 : 0.0;
+    f8 = lane - 8 >= s ? 1.0 : 0.0;
+    f15 = 16 * (lane >> 4) - 1 >= s ? 1.0 : 0.0;
+    f31 = 31 >= s ? 1.0 : 0.0;
   }
-  return v;
-}
+  __device__ __forceinline__ double sum(double v) const {
+    v = __builtin_fma(dpp_get<0x111, 0xF>(v), f1, v);   // row_shr:1
+    v = __builtin_fma(dpp_get<0x112, 0xF>(v), f2, v);   // row_shr:2
+    v = __builtin_fma(dpp_get<0x114, 0xF>(v), f4, v);   // row_shr:4
+    v = __builtin_fma(dpp_get<0x118, 0xF>(v), f8, v);   // row_shr:8
+    v = __builtin_fma(dpp_get<0x142, 0xA>(v), f15, v);  // row_bcast:15 into rows 1 and 3
+    v = __builtin_fma(dpp_get<0x143, 0xC>(v), f31, v);  // row_bcast:31 into rows 2 and 3
+    return v;
+  }
+};
 
 // Grid lines of one level.  Node m = m1*Y + m2.
 struct Grid {

@@ -355,17 +355,14 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
     }
     // one atomic per run of lanes in the same cell (lanes are cell-sorted in
     // the common case; any order stays correct, just with more runs)
-    const int kprev = __shfl_up(key, 1, 64);
-    const bool head = lane == 0 || kprev != key;
-    const unsigned long long heads = __ballot(head);
-    const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+    const SegScan sc(key, lane);
 #pragma unroll
     for (int c = 0; c < 4; c++)
 #pragma unroll
-      for (int q = 0; q < 10; q++) C.v[c][q] = seg_sum(C.v[c][q], seg, lane);
-    C.nd = seg_sum(C.nd, seg, lane);
-    C.wnd = seg_sum(C.wnd, seg, lane);
-    if (head && key >= 0) {
+      for (int q = 0; q < 10; q++) C.v[c][q] = sc.sum(C.v[c][q]);
+    C.nd = sc.sum(C.nd);
+    C.wnd = sc.sum(C.wnd);
+    if (sc.tail && key >= 0) {
       const int m10 = key / g.Y, m20 = key % g.Y;
       if (C.wnd != 0) {
         const int m3 = (m10 + 1) * g.Y + m20 + 1;
@@ -526,16 +523,13 @@ __global__ __launch_bounds__(256) void k_heat_markers_a(const i2e_params *__rest
     }
     if (!p->heatdif) continue;
     // one atomic per run of lanes in the same cell
-    const int kprev = __shfl_up(key, 1, 64);
-    const bool head = lane == 0 || kprev != key;
-    const unsigned long long heads = __ballot(head);
-    const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+    const SegScan sc(key, lane);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      s0[c] = seg_sum(s0[c], seg, lane);
-      s1[c] = seg_sum(s1[c], seg, lane);
+      s0[c] = sc.sum(s0[c]);
+      s1[c] = sc.sum(s1[c]);
     }
-    if (head && key >= 0) {
+    if (sc.tail && key >= 0) {
       const int m10 = key / g.Y, m20 = key % g.Y;
 #pragma unroll
       for (int a = 0; a < 2; a++)
