@@ -830,6 +830,14 @@ struct TileArgs {
   MapView xn, xm, yn, ym;  // this level -> next coarser
   int zero, nsmooth, rev;
   const T *dx, *dy;  // row diagonals (read by the difference-form stencil only)
+  // pre-smoothing on the way down (k_tsmooth, rfx != nullptr): the
+  // right-hand side is the restriction of the finer level's residual, formed
+  // here (k_vrestrict's weights) and also stored to fxo, fyo for the later
+  // residual and post-smoothing passes
+  const T *rfx = nullptr, *rfy = nullptr;
+  T *fxo = nullptr, *fyo = nullptr;
+  int Yf = 0;
+  MapView fxn{}, fxm{}, fyn{}, fym{};  // finer level -> this one
 };
 
 template <class T>
@@ -888,7 +896,10 @@ __device__ __forceinline__ TA s_apply_lds(const TA *S, size_t R, size_t r, const
 // iterates keeps the finest level's cancellations exact in mixed mode)
 // DIFF: the finest level's FP32 stencil in difference form (k_sdiff), with
 // the row diagonals read from A.dx / A.dy
-template <class T, class TA, int W, bool DIFF = false>
+// RS: pre-smoothing pass that restricts its right-hand side (A.rfx); a
+// template flag, since the restriction code slows the plain pass's schedule
+// even when not taken (13.8 -> 16.0 us for the FP32 Galerkin pass)
+template <class T, class TA, int W, bool DIFF = false, bool RS = false>
 __global__ __launch_bounds__(TileShape<T>::B) void k_tsmooth(TileArgs<T> A, const TA *S) {
   TILE_CONSTS(T);
   __shared__ T lx[TWX * TWY], ly[TWX * TWY];
@@ -900,8 +911,19 @@ __global__ __launch_bounds__(TileShape<T>::B) void k_tsmooth(TileArgs<T> A, cons
   const int li = (i - i0) * TWX + (j - j0);
   const bool ok = i < X && j < Y;
   const int m = ok ? i * Y + j : 0;
-  const T fx = ok ? A.fx[m] : T(0), fy = ok ? A.fy[m] : T(0);
   const int kx = ok ? A.kx[m] : -1, ky = ok ? A.ky[m] : -1;
+  T fx, fy;
+  if constexpr (RS) {
+    fx = kx == 0 && j <= Y - 2 ? restrict_rw(A.rfx, A.Yf, A.fxn, A.fym, i, j) : T(0);
+    fy = ky == 0 && i <= X - 2 ? restrict_rw(A.rfy, A.Yf, A.fxm, A.fyn, i, j) : T(0);
+    if (ok) {
+      A.fxo[m] = fx;
+      A.fyo[m] = fy;
+    }
+  } else {
+    fx = ok ? A.fx[m] : T(0);
+    fy = ok ? A.fy[m] : T(0);
+  }
   // both stencil rows of this thread's point stay in registers for the pass
   constexpr int NSL = 2 * (2 * W + 1) * (2 * W + 1);
   TA sx[NSL], sy[NSL];
@@ -1264,6 +1286,11 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   TILE_CONSTS(T);
   const dim3 TG((v.Y + TSX - 1) / TSX, (v.X + TSY - 1) / TSY), TBK(TB);
   const int kid = l == 0 ? K_VSMOOTH0 : K_VSMOOTHC;
+  // Restriction inside the next level's pre-smoothing pass: in FP64 it adds
+  // 3.8 us to that pass against ~7 us for the separate launch it replaces
+  // (14.07 -> 14.28 steps/s at 501^2); in FP32 (32x16 tiles) it adds 6.4 us
+  // and gains nothing (15.32 -> 15.12)
+  constexpr bool fuse_restrict = sizeof(T) == 8;
   const VLev<T> o = lev(l);
   // algorithmic bytes of one smoothing pass: rhs, input and output iterate
   // (two components each), the row kinds, and the coefficients once per
@@ -1282,10 +1309,28 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   else
     coef = 36.0 * sizeof(T);
   const double bytes = (6.0 * sizeof(T) + coef) * v.N + 2.0 * v.N;
+  // a pre-smoothing pass that restricts its right-hand side also reads the
+  // finer level's two residual components once (it writes the rhs instead
+  // of reading it)
+  const double rbytes = l > 0 ? 2.0 * sizeof(T) * L[l - 1].N : 0.0;
   const MapView none{nullptr, nullptr, nullptr, nullptr};
   const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
   TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr, 0,
                 none, none, none, none, 1, ns, 0, v.dx.p, v.dy.p};
+  if (l > 0 && fuse_restrict) {
+    // this level's right-hand side: restricted from the finer residual
+    // inside the pre-smoothing pass (no separate restriction launch)
+    const VelLevel<T> &f = L[l - 1];
+    A.rfx = f.rx.p;
+    A.rfy = f.ry.p;
+    A.fxo = v.fx.p;
+    A.fyo = v.fy.p;
+    A.Yf = f.Y;
+    A.fxn = view(f.mxn);
+    A.fxm = view(f.mxm);
+    A.fyn = view(f.myn);
+    A.fym = view(f.mym);
+  }
   auto launch = [&](const TileArgs<T> &a) {
     if (kt) kt->begin(kid, s);
     if (l == 0 && !stencil0)
@@ -1296,9 +1341,11 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
       hipLaunchKernelGGL((k_tsmooth<T, double, 1>), TG, TBK, 0, s, a, S0d);
     else if (l == 0)
       hipLaunchKernelGGL((k_tsmooth<T, T, 1>), TG, TBK, 0, s, a, (const T *)v.S.p);
+    else if (a.rfx)
+      hipLaunchKernelGGL((k_tsmooth<T, T, 2, false, true>), TG, TBK, 0, s, a, (const T *)v.S.p);
     else
       hipLaunchKernelGGL((k_tsmooth<T, T, 2>), TG, TBK, 0, s, a, (const T *)v.S.p);
-    if (kt) kt->end(kid, s, bytes);
+    if (kt) kt->end(kid, s, bytes + (a.rfx ? rbytes : 0.0));
     std::swap(v.ux.p, v.ux2.p);
     std::swap(v.uy.p, v.uy2.p);
   };
@@ -1308,6 +1355,7 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   R.ux = v.ux.p;
   R.uy = v.uy.p;
   R.zero = 0;
+  R.rfx = R.rfy = nullptr;
   if (l == 0 && !stencil0)
     hipLaunchKernelGGL(k_vresid<T>, grd2(v.X, v.Y), blk2(), 0, s, o, v.ux.p, v.uy.p, v.fx.p, v.fy.p, v.rx.p, v.ry.p);
   else if (l == 0 && diff0)
@@ -1319,8 +1367,10 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   else
     hipLaunchKernelGGL((k_tresid<T, T, 2>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
   VelLevel<T> &c = L[l + 1];
-  hipLaunchKernelGGL(k_vrestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mxn), view(v.mxm),
-                     view(v.myn), view(v.mym), c.kx.p, c.ky.p, v.rx.p, v.ry.p, c.fx.p, c.fy.p);
+  // the coarsest level has no smoothing pass to restrict into
+  if (l + 1 == nl - 1 || !fuse_restrict)
+    hipLaunchKernelGGL(k_vrestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mxn), view(v.mxm),
+                       view(v.myn), view(v.mym), c.kx.p, c.ky.p, v.rx.p, v.ry.p, c.fx.p, c.fy.p);
   ST_OK(vcycle(l + 1, s));
   TileArgs<T> P = R;
   P.ux = v.ux.p;
