@@ -450,6 +450,8 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->restart = 200;
   o->maxit = 600;
   o->nsmooth = 3;
+  o->nsmooth_coarse = 2;
+  o->nsmooth_cell = 4;
   o->stokes_rtol = 1e-8;
   o->heat_rtol = 1e-10;
   o->grav_rtol = 1e-10;
@@ -571,6 +573,7 @@ int i2e_destroy(i2e_ctx *c) {
 int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
   if (!c || !o) return I2E_ERR_ARG;
   if (o->restart < 1 || o->restart > 400 || o->maxit < 1 || o->nsmooth < 0 || o->nsmooth_coarse < 0 ||
+      o->nsmooth_cell < 0 ||
       o->coarse_max < 2 || (o->orth != I2E_ORTH_LAGGED && o->orth != I2E_ORTH_DGKS) || o->fine_stencil < 0 ||
       o->fine_stencil > I2E_FINE_DIFF32 ||
       (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT))
@@ -936,12 +939,12 @@ static int stokes_prepare(i2e_ctx *c) {
     hipLaunchKernelGGL(k_bf_stencil, nb2(X - 1, Y - 1), dim3(256), 0, c->s, c->grid(), c->bondm.p, c->bcix.p,
                        c->bciy.p, c->bLS.p, c->blmask.p);
     ST_OK(c->l64.refresh(c->bLS.p, c->blmask.p, c->s));
-    c->l64.nsmooth = c->o.nsmooth;
-    c->l64.nsmooth_c = c->o.nsmooth_coarse;
+    c->l64.nsmooth = c->o.nsmooth_cell > 0 ? c->o.nsmooth_cell : c->o.nsmooth;
+    c->l64.nsmooth_c = 0;
     if (mixed(c)) {
       ST_OK(c->l32.refresh_from(c->l64, c->s));
-      c->l32.nsmooth = c->o.nsmooth;
-      c->l32.nsmooth_c = c->o.nsmooth_coarse;
+      c->l32.nsmooth = c->l64.nsmooth;
+      c->l32.nsmooth_c = 0;
     }
   }
   return I2E_OK;

@@ -157,7 +157,8 @@ typedef struct i2e_solver_opts {
   int32_t precision;       /* I2E_PREC_FP64 or I2E_PREC_MIXED                 */
   int32_t restart;         /* FGMRES restart length (1..400, default 200)    */
   int32_t maxit;           /* max outer iterations per solve                 */
-  int32_t nsmooth;         /* smoothing sweeps per level (pre = post)        */
+  int32_t nsmooth;         /* smoothing sweeps (pre = post): finest velocity
+                              level, scalar (heat, gravity) multigrids      */
   double stokes_rtol;      /* ||b-Ax||/||b|| target for Stokes               */
   double heat_rtol;        /* same for heat                                  */
   double grav_rtol;        /* same for gravity                               */
@@ -166,9 +167,14 @@ typedef struct i2e_solver_opts {
   int32_t coarse_max;      /* coarsest grid: cells per direction <= this     */
   int32_t schur;           /* Schur preconditioner: I2E_SCHUR_BFBT or _MASS  */
   int32_t graphs;          /* 1: replay the Stokes preconditioner as a HIP graph */
-  int32_t nsmooth_coarse;  /* Stokes multigrids: sweeps below the finest level (0: nsmooth) */
+  int32_t nsmooth_coarse;  /* velocity multigrid: sweeps below the finest level (0: nsmooth) */
   int32_t orth;            /* FGMRES orthogonalisation: I2E_ORTH_LAGGED or _DGKS */
   int32_t fine_stencil;    /* finest velocity level smoothed with I2E_FINE_*  */
+  int32_t nsmooth_cell;    /* pressure-cell (BFBt) multigrid: sweeps on every
+                              level (0: nsmooth).  Defaults 3 / 2 / 4: the
+                              BFBt Poisson solves gain more from sweeps than
+                              the velocity block (501^2: 82 vs 88 Krylov
+                              iterations against 3 / 3 / 3, 7 % less time) */
 } i2e_solver_opts;
 
 typedef struct i2e_stats {

@@ -405,8 +405,10 @@ def test_viterate_parity_201_fp64(orc):
     o.ronurecalc()
     hs.timestep = p.maxtmstep
     o.maxvelstep()
-    # the 201^2 system stagnates near 5e-12 in FP64; 1e-10 is reached
-    ctx = Context(p, hs, solver=fp64_opts(1e-10))
+    # the 201^2 system stagnates near 5e-12 in FP64; 3e-11 is reached.  A
+    # solve stopped just under 1e-10 left sxy at 1-5e-8 of the direct solve,
+    # depending on the Krylov path: the field bounds need the margin.
+    ctx = Context(p, hs, solver=fp64_opts(3e-11))
     ctx.set_scalars(hs.timestep, hs.timesum)
     ctx.viterate()
     st = ctx.stats()
@@ -414,7 +416,7 @@ def test_viterate_parity_201_fp64(orc):
     ctx.close()
     o.viterate()
     say("stokes its", st.stokes_iters, "relres", st.stokes_relres)
-    assert st.stokes_relres <= 1e-10
+    assert st.stokes_relres <= 3e-11
     for f in ("gp", "vx", "vy", "pr", "exx", "sxy"):
         e = rel(getattr(got, f), getattr(hs, f))
         say(f, e)
