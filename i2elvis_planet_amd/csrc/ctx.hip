@@ -573,7 +573,7 @@ int i2e_destroy(i2e_ctx *c) {
 int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
   if (!c || !o) return I2E_ERR_ARG;
   if (o->restart < 1 || o->restart > 400 || o->maxit < 1 || o->nsmooth < 0 || o->nsmooth_coarse < 0 ||
-      o->nsmooth_cell < 0 ||
+      o->nsmooth_cell < 0 || !(o->omega > 0 && o->omega < 2) ||
       o->coarse_max < 2 || (o->orth != I2E_ORTH_LAGGED && o->orth != I2E_ORTH_DGKS) || o->fine_stencil < 0 ||
       o->fine_stencil > I2E_FINE_DIFF32 ||
       (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT))
@@ -922,6 +922,7 @@ static int stokes_prepare(i2e_ctx *c) {
   ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
   c->v64.nsmooth = c->o.nsmooth;
   c->v64.nsmooth_c = c->o.nsmooth_coarse;
+  c->v64.omega = c->o.omega;
   const int fs = c->o.fine_stencil;
   c->v64.stencil0 = fs != I2E_FINE_ROWS && (fs != I2E_FINE_AUTO || !mixed(c));
   if (mixed(c)) {
@@ -929,6 +930,7 @@ static int stokes_prepare(i2e_ctx *c) {
     ST_OK(c->v32.refresh_from(c->v64, c->s));
     c->v32.nsmooth = c->o.nsmooth;
     c->v32.nsmooth_c = c->o.nsmooth_coarse;
+    c->v32.omega = c->o.omega;
     c->v32.stencil0 = fs != I2E_FINE_ROWS;
     c->v32.S0d = fs == I2E_FINE_STENCIL64 ? c->v64.L[0].S.p : nullptr;
   }

@@ -838,6 +838,7 @@ struct TileArgs {
   T *fxo = nullptr, *fyo = nullptr;
   int Yf = 0;
   MapView fxn{}, fxm{}, fyn{}, fym{};  // finer level -> this one
+  double omega = 1.0;  // damping of the colour steps
 };
 
 template <class T>
@@ -934,7 +935,7 @@ __global__ __launch_bounds__(TileShape<T>::B) void k_tsmooth(TileArgs<T> A, cons
   }
   const TA dx = ok ? (DIFF ? TA(A.dx[m]) : sx[sslot(W, 0, 0, 0)]) : TA(1);
   const TA dy = ok ? (DIFF ? TA(A.dy[m]) : sy[sslot(W, 1, 0, 0)]) : TA(1);
-  const TA rdx = TA(1) / dx, rdy = TA(1) / dy;
+  const TA rdx = TA(A.omega) / dx, rdy = TA(A.omega) / dy;
   const int la = li / TWX, lb = li % TWX;
   auto apply = [&](const TA *sv) {
     TA acc = 0;
@@ -1317,6 +1318,7 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
   TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr, 0,
                 none, none, none, none, 1, ns, 0, v.dx.p, v.dy.p};
+  A.omega = omega;
   if (l > 0 && fuse_restrict) {
     // this level's right-hand side: restricted from the finer residual
     // inside the pre-smoothing pass (no separate restriction launch)
@@ -1557,6 +1559,7 @@ struct PTArgs {
   int Yc;
   MapView mx, my;
   int zero, nsmooth, rev;
+  double omega;  // damping of the colour steps (pointwise rows undamped)
 };
 
 // Tiled red-black Gauss-Seidel on the scalar 5-point rows (the velocity
@@ -1610,7 +1613,7 @@ __global__ __launch_bounds__(TileShape<T>::B) void k_ptsmooth(PTArgs<T> A) {
         d += t + b;
       }
       cw = o.c[m];
-      rd = T(1.0 / d);
+      rd = T(A.omega / d);
     }
   }
   __syncthreads();
@@ -1651,7 +1654,7 @@ int ScalMG<T>::vcycle(int l, hipStream_t s) {
   const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
   // pre-smoothing from a zero iterate into u
   hipLaunchKernelGGL(k_ptsmooth<T>, TG, TBK, 0, s,
-                     PTArgs<T>{o, v.f.p, nullptr, v.u.p, nullptr, 0, none, none, 1, nsmooth, 0});
+                     PTArgs<T>{o, v.f.p, nullptr, v.u.p, nullptr, 0, none, none, 1, nsmooth, 0, omega});
   hipLaunchKernelGGL(k_presid<T>, G, B, 0, s, o, v.u.p, v.f.p, v.r.p);
   ScalLevel<T> &c = L[l + 1];
   hipLaunchKernelGGL(k_prestrict<T>, grd2(c.X, c.Y), B, 0, s, c.X, c.Y, v.Y, view(v.mx), view(v.my), c.mask.p, v.r.p,
@@ -1660,7 +1663,7 @@ int ScalMG<T>::vcycle(int l, hipStream_t s) {
   // coarse-grid correction added on load, post-smoothing into u2, which
   // becomes u (the V-cycle's result stays in L[l].u)
   hipLaunchKernelGGL(k_ptsmooth<T>, TG, TBK, 0, s,
-                     PTArgs<T>{o, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, nsmooth, 1});
+                     PTArgs<T>{o, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, nsmooth, 1, omega});
   std::swap(v.u.p, v.u2.p);
   return I2E_OK;
 }
@@ -1753,6 +1756,7 @@ struct CellArgs {
   int Yc;
   MapView mx, my;
   int zero, nsmooth, rev;
+  double omega;  // damping of the colour steps (identity rows undamped)
 };
 
 // tiled colour-Jacobi smoothing of a 3x3-stencil level (see the velocity
@@ -1781,7 +1785,7 @@ __global__ __launch_bounds__(TileShape<T>::B) void k_ctsmooth(CellArgs<T> A) {
   T sv[9];
 #pragma unroll
   for (int q = 0; q < 9; q++) sv[q] = ok ? A.S[(size_t)q * N + m] : T(0);
-  const T f = ok ? A.f[m] : T(0), rd = T(1) / (ok ? sv[4] : T(1));
+  const T f = ok ? A.f[m] : T(0), rd = (ok && !A.mask[m] ? T(A.omega) : T(1)) / (ok ? sv[4] : T(1));
   __syncthreads();
   // out-of-grid lanes compute too and skip the store (dense lane masks; see
   // the velocity smoother)
@@ -1950,6 +1954,7 @@ struct CTArgs {
   const T *fin;
   T *uout;
   int nsmooth;
+  double omega;
 };
 
 struct TailSeg {
@@ -2119,7 +2124,8 @@ __global__ __launch_bounds__(CT_THR) void k_ctail(CTArgs<T> A) {
           const int q = t + k * CT_THR;
           act[k] = q < L.N && (q >= nred ? 1 : 0) == col;
           nv[k] = 0;
-          if (act[k]) nv[k] = u[pp[l][k]] + (f[L.off + lp[l][k]] - apply(l, k)) / diag(l, k);
+          if (act[k])
+            nv[k] = u[pp[l][k]] + (msk(L)[lp[l][k]] ? T(1) : T(A.omega)) * (f[L.off + lp[l][k]] - apply(l, k)) / diag(l, k);
         }
         __syncthreads();
 #pragma unroll
@@ -2301,6 +2307,7 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
     A.fin = v.f.p;
     A.uout = v.u.p;
     A.nsmooth = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
+    A.omega = omega;
     hipLaunchKernelGGL(k_ctail<T>, dim3(1), dim3(CT_THR), 0, s, A);
     return I2E_OK;
   }
@@ -2313,7 +2320,7 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
   const dim3 TG((v.Y + TSX - 1) / TSX, (v.X + TSY - 1) / TSY), TBK(TB);
   const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
   const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
-  CellArgs<T> A{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, nullptr, 0, none, none, 1, ns, 0};
+  CellArgs<T> A{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, nullptr, 0, none, none, 1, ns, 0, omega};
   hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, A);
   std::swap(v.u.p, v.u2.p);
   CellLevel<T> &c = L[l + 1];
@@ -2328,7 +2335,7 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
                        (const int8_t *)c.mask.p, (const T *)v.r.p, c.f.p);
   }
   ST_OK(vcycle(l + 1, s));
-  CellArgs<T> P{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, ns, 1};
+  CellArgs<T> P{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, ns, 1, omega};
   hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, P);
   std::swap(v.u.p, v.u2.p);
   return I2E_OK;

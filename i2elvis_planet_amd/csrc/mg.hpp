@@ -117,6 +117,7 @@ struct VelMG {
   DBuf<int> overflow;       // Galerkin entries that fell outside the 5x5 window (must stay 0)
   double mu = 0, dt = 0;
   int nsmooth = 2;
+  double omega = 1.0;       // damping of the smoothers' colour steps (exact rows undamped)
   int nsmooth_c = 0;        // sweeps on the coarser levels (0: nsmooth)
   int stencil0 = 0;         // finest level smoothed with its probed 3x3 stencil (else matrix-free rows)
   const double *S0d = nullptr;  // that stencil in FP64 (from the FP64 hierarchy) for FP32 iterates
@@ -161,6 +162,7 @@ struct ScalMG {
   DBuf<double> ainv;  // FP64 in every precision
   std::vector<DBuf<int>> sfi, sfj;
   int nsmooth = 2;
+  double omega = 1.0;       // damping of the smoothers' colour steps (exact rows undamped)
   // coords of the finest points; fi/fj give the coarse point selection
   int setup(const std::vector<double> &x0, const std::vector<double> &y0, int coarse_max);
   // finest-level fields: k, c, mask, dm (FP64 device arrays, length N0)
@@ -192,6 +194,7 @@ struct CellMG {
   DBuf<double> dense, ainv;
   DBuf<int> overflow;
   int nsmooth = 2;
+  double omega = 1.0;       // damping of the smoothers' colour steps (exact rows undamped)
   int nsmooth_c = 0;        // sweeps on the coarser levels (0: nsmooth)
   int tail_l = -1;          // first level run by the single-workgroup tail kernel (-1: none)
   DBuf<uint4> timg;         // the tail's LDS image (coarse inverse, stencils, row kinds, maps)
