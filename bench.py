@@ -31,7 +31,7 @@ TRAFFIC_FILES = {
 # rocprof kernel-name prefix and, where only some launches count, the grid
 # size (threads) of the launches bench.py times for each roofline kernel
 PMC_MATCH = {
-    "vsmoothc": ("void i2e::k_tsmooth<float, float, 2, false>(", None),
+    "vsmoothc": ("void i2e::k_tsmooth<float, float, 2, false, false>(", None),
     "gsdot": ("i2e::k_gsdot2(", "stokes"),
 }
 
