@@ -1756,7 +1756,6 @@ struct CellArgs {
   int Yc;
   MapView mx, my;
   int zero, nsmooth, rev;
-  double omega;  // damping of the colour steps (identity rows undamped)
 };
 
 // tiled colour-Jacobi smoothing of a 3x3-stencil level (see the velocity
@@ -1785,7 +1784,7 @@ __global__ __launch_bounds__(TileShape<T>::B) void k_ctsmooth(CellArgs<T> A) {
   T sv[9];
 #pragma unroll
   for (int q = 0; q < 9; q++) sv[q] = ok ? A.S[(size_t)q * N + m] : T(0);
-  const T f = ok ? A.f[m] : T(0), rd = (ok && !A.mask[m] ? T(A.omega) : T(1)) / (ok ? sv[4] : T(1));
+  const T f = ok ? A.f[m] : T(0), rd = T(1) / (ok ? sv[4] : T(1));
   __syncthreads();
   // out-of-grid lanes compute too and skip the store (dense lane masks; see
   // the velocity smoother)
@@ -1954,7 +1953,6 @@ struct CTArgs {
   const T *fin;
   T *uout;
   int nsmooth;
-  double omega;
 };
 
 struct TailSeg {
@@ -2124,8 +2122,7 @@ __global__ __launch_bounds__(CT_THR) void k_ctail(CTArgs<T> A) {
           const int q = t + k * CT_THR;
           act[k] = q < L.N && (q >= nred ? 1 : 0) == col;
           nv[k] = 0;
-          if (act[k])
-            nv[k] = u[pp[l][k]] + (msk(L)[lp[l][k]] ? T(1) : T(A.omega)) * (f[L.off + lp[l][k]] - apply(l, k)) / diag(l, k);
+          if (act[k]) nv[k] = u[pp[l][k]] + (f[L.off + lp[l][k]] - apply(l, k)) / diag(l, k);
         }
         __syncthreads();
 #pragma unroll
@@ -2307,7 +2304,6 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
     A.fin = v.f.p;
     A.uout = v.u.p;
     A.nsmooth = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
-    A.omega = omega;
     hipLaunchKernelGGL(k_ctail<T>, dim3(1), dim3(CT_THR), 0, s, A);
     return I2E_OK;
   }
@@ -2320,7 +2316,7 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
   const dim3 TG((v.Y + TSX - 1) / TSX, (v.X + TSY - 1) / TSY), TBK(TB);
   const MapView none{nullptr, nullptr, nullptr, nullptr, nullptr};
   const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
-  CellArgs<T> A{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, nullptr, 0, none, none, 1, ns, 0, omega};
+  CellArgs<T> A{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, nullptr, 0, none, none, 1, ns, 0};
   hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, A);
   std::swap(v.u.p, v.u2.p);
   CellLevel<T> &c = L[l + 1];
@@ -2335,7 +2331,7 @@ int CellMG<T>::vcycle(int l, hipStream_t s) {
                        (const int8_t *)c.mask.p, (const T *)v.r.p, c.f.p);
   }
   ST_OK(vcycle(l + 1, s));
-  CellArgs<T> P{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, ns, 1, omega};
+  CellArgs<T> P{v.X, v.Y, v.S.p, v.mask.p, v.f.p, v.u.p, v.u2.p, c.u.p, c.Y, view(v.mx), view(v.my), 0, ns, 1};
   hipLaunchKernelGGL(k_ctsmooth<T>, TG, TBK, 0, s, P);
   std::swap(v.u.p, v.u2.p);
   return I2E_OK;

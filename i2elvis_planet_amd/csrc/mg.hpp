@@ -194,7 +194,6 @@ struct CellMG {
   DBuf<double> dense, ainv;
   DBuf<int> overflow;
   int nsmooth = 2;
-  double omega = 1.0;       // damping of the smoothers' colour steps (exact rows undamped)
   int nsmooth_c = 0;        // sweeps on the coarser levels (0: nsmooth)
   int tail_l = -1;          // first level run by the single-workgroup tail kernel (-1: none)
   DBuf<uint4> timg;         // the tail's LDS image (coarse inverse, stencils, row kinds, maps)
