@@ -23,6 +23,7 @@ SCHUR_WBFBT = 2
 ORTH_LAGGED = 0
 ORTH_DGKS = 1
 FINE_AUTO, FINE_ROWS, FINE_STENCIL, FINE_STENCIL64, FINE_DIFF32 = 0, 1, 2, 3, 4
+GUESS_LAST, GUESS_EXTRAP = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -85,7 +86,7 @@ class SolverOpts(C.Structure):
         ("schur_scale", C.c_double), ("omega", C.c_double),
         ("coarse_max", C.c_int32), ("schur", C.c_int32),
         ("graphs", C.c_int32), ("nsmooth_coarse", C.c_int32), ("orth", C.c_int32), ("fine_stencil", C.c_int32),
-        ("nsmooth_cell", C.c_int32),
+        ("stokes_guess", C.c_int32), ("nsmooth_cell", C.c_int32),
     ]
 
 

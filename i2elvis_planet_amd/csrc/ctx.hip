@@ -32,6 +32,14 @@ static inline dim3 nb2(int X, int Y) { return dim3((Y + 255) / 256, X); }
 __global__ void k_copy(int n, const double *a, double *b) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) b[k] = a[k];
 }
+// x <- x + th (x - xp), xp <- the old x (linear extrapolation in time)
+__global__ void k_extrap(int n, double th, double *x, double *xp) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const double a = x[k];
+    x[k] = a + th * (a - xp[k]);
+    xp[k] = a;
+  }
+}
 // plain 16-byte copy, one read and one write of every element (the
 // reference size for scaling memory-traffic counters)
 __global__ void k_dev_copy(size_t n16, const uint4 *__restrict__ a, uint4 *__restrict__ b) {
@@ -452,6 +460,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->nsmooth = 3;
   o->nsmooth_coarse = 2;
   o->nsmooth_cell = 4;
+  o->stokes_guess = I2E_GUESS_EXTRAP;
   o->stokes_rtol = 1e-8;
   o->heat_rtol = 1e-10;
   o->grav_rtol = 1e-10;
@@ -504,7 +513,7 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
   const int N = c->N;
   if ((st = c->gx.alloc(c->X)) || (st = c->gy.alloc(c->Y)) || (st = c->nodes.alloc((size_t)19 * N)) ||
       (st = c->acc.alloc((size_t)12 * N)) || (st = c->sol1.alloc(N)) || (st = c->tk2.alloc(N)) ||
-      (st = c->red.alloc(4096)) || (st = c->xs.alloc((size_t)3 * N)) || (st = c->bs.alloc((size_t)3 * N)) ||
+      (st = c->red.alloc(4096)) || (st = c->xs.alloc((size_t)3 * N)) || (st = c->xprev.alloc((size_t)3 * N)) || (st = c->bs.alloc((size_t)3 * N)) ||
       (st = c->work.alloc((size_t)3 * N)) || (st = c->bondm.alloc((size_t)4 * N)) || (st = c->pin.alloc(4096)) ||
       (st = c->hk.alloc(N)) || (st = c->hc.alloc(N)) || (st = c->hdm.alloc(N)) || (st = c->hmask.alloc(N)) ||
       (st = c->gk.alloc(N)) || (st = c->gc.alloc(N)) || (st = c->gdm.alloc(N)) || (st = c->gmask.alloc(N)) ||
@@ -574,6 +583,7 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
   if (!c || !o) return I2E_ERR_ARG;
   if (o->restart < 1 || o->restart > 400 || o->maxit < 1 || o->nsmooth < 0 || o->nsmooth_coarse < 0 ||
       o->nsmooth_cell < 0 || !(o->omega > 0 && o->omega < 2) ||
+      (o->stokes_guess != I2E_GUESS_LAST && o->stokes_guess != I2E_GUESS_EXTRAP) ||
       o->coarse_max < 2 || (o->orth != I2E_ORTH_LAGGED && o->orth != I2E_ORTH_DGKS) || o->fine_stencil < 0 ||
       o->fine_stencil > I2E_FINE_DIFF32 ||
       (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT))
@@ -769,6 +779,7 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
 
 int i2e_upload(i2e_ctx *c, const i2e_host_state *s) {
   if (!c || !s || !s->gx || !s->gy) return I2E_ERR_ARG;
+  c->have_prev = 0;  // new fields: no solution history to extrapolate from
   if (s->marknum < 0 || s->marknum > c->Mmax) {
     set_error("marknum %lld exceeds max_markers %lld", (long long)s->marknum, (long long)c->Mmax);
     return I2E_ERR_CAPACITY;
@@ -919,6 +930,14 @@ static int stokes_prepare(i2e_ctx *c) {
   SOp<double> o = c->sop();
   hipLaunchKernelGGL(k_stokes_rhs, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->bs.p);
   hipLaunchKernelGGL(k_stokes_pack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->F.pr, c->F.vx, c->F.vy, c->xs.p);
+  if (c->o.stokes_guess == I2E_GUESS_EXTRAP && c->have_prev && c->dt_prev > 0) {
+    const double th = std::min(1.0, c->timestep / c->dt_prev);
+    hipLaunchKernelGGL(k_extrap, dim3(1024), dim3(256), 0, c->s, 3 * X * Y, th, c->xs.p, c->xprev.p);
+  } else if (c->o.stokes_guess == I2E_GUESS_EXTRAP) {
+    hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, 3 * X * Y, (const double *)c->xs.p, c->xprev.p);
+  }
+  c->have_prev = c->o.stokes_guess == I2E_GUESS_EXTRAP;
+  c->dt_prev = c->timestep;
   ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
   c->v64.nsmooth = c->o.nsmooth;
   c->v64.nsmooth_c = c->o.nsmooth_coarse;

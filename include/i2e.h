@@ -152,6 +152,12 @@ enum { I2E_ORTH_LAGGED = 0, I2E_ORTH_DGKS = 1 };
  * cancellation at half the bytes of STENCIL64 (DIFF32).  STENCIL64 and DIFF32
  * are STENCIL in FP64 mode.  AUTO: STENCIL in FP64 mode, DIFF32 in mixed.  */
 enum { I2E_FINE_AUTO = 0, I2E_FINE_ROWS = 1, I2E_FINE_STENCIL = 2, I2E_FINE_STENCIL64 = 3, I2E_FINE_DIFF32 = 4 };
+/* Initial guess of each Stokes solve: the last step's solution, or its
+ * linear extrapolation in time from the last two solutions,
+ * x_n + (dt / dt_prev) (x_n - x_{n-1}) (the first solve after i2e_upload
+ * starts from the uploaded fields either way).  EXTRAP is the default:
+ * 501^2, 6 steps: 75-77 Krylov iterations against 79-81 (mixed).          */
+enum { I2E_GUESS_LAST = 0, I2E_GUESS_EXTRAP = 1 };
 
 typedef struct i2e_solver_opts {
   int32_t precision;       /* I2E_PREC_FP64 or I2E_PREC_MIXED                 */
@@ -172,6 +178,7 @@ typedef struct i2e_solver_opts {
   int32_t nsmooth_coarse;  /* velocity multigrid: sweeps below the finest level (0: nsmooth) */
   int32_t orth;            /* FGMRES orthogonalisation: I2E_ORTH_LAGGED or _DGKS */
   int32_t fine_stencil;    /* finest velocity level smoothed with I2E_FINE_*  */
+  int32_t stokes_guess;    /* Stokes initial guess: I2E_GUESS_LAST or _EXTRAP */
   int32_t nsmooth_cell;    /* pressure-cell (BFBt) multigrid: sweeps on every
                               level (0: nsmooth).  Defaults 3 / 2 / 4: the
                               BFBt Poisson solves gain more from sweeps than
