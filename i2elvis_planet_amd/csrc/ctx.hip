@@ -231,14 +231,28 @@ __global__ void k_bf_div(Grid g, const int8_t *lmask, const T *wx, const T *wy, 
            ((double)wy[(i - 1) * Y + j] - (double)wy[(i - 1) * Y + j - 1]) / (g.gy[j] - g.gy[j - 1]));
 }
 
-// z_p = -q / mu on E (P boundary rows follow in k_schur_bc)
+// z_p = -q / mu on E, then the P boundary rows as in k_schur_bc, whose
+// pressure neighbours on E take -q / mu directly (one pass instead of two)
 template <class T>
-__global__ void k_bf_out(Grid g, double mu, const int8_t *lmask, const T *q, double *z) {
-  const int J = blockIdx.x * blockDim.x + threadIdx.x, I = blockIdx.y;
-  const int Y = g.Y, Yc = Y - 1;
-  if (J >= Yc) return;
-  const int c = I * Yc + J, m = (I + 1) * Y + J + 1;
-  if (!lmask[c]) z[3 * m] = -(double)q[c] / mu;
+__global__ void k_bf_out_bc(SOp<double> o, BcTab bc, const T *q, const double *r, double *z) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int Y = o.Y, Yc = Y - 1;
+  if (j >= Y || !(i && j)) return;
+  const int m = i * Y + j;
+  const int b = bc.bondm[3 * m];
+  if (!b) {
+    z[3 * m] = -(double)q[(i - 1) * Yc + (j - 1)] / o.mu;
+    return;
+  }
+  double v = r[3 * m] / o.mu;
+  for (int n = 0; n < 3; n++) {
+    const int nb = bc.bondn[3 * b + n];
+    if (!nb) continue;
+    const int u = nb - 1, mu3 = u / 3, iu = mu3 / Y, ju = mu3 - iu * Y;
+    const bool onE = u == 3 * mu3 && iu >= 1 && ju >= 1 && !bc.bondm[u];
+    v += bc.bondv[4 * b + n + 1] * (onE ? -(double)q[(iu - 1) * Yc + (ju - 1)] / o.mu : z[u]);
+  }
+  z[3 * m] = v;
 }
 
 // ---------------------------------------------------------------- viterate diagnostics
@@ -974,13 +988,13 @@ static int stokes_prepare(i2e_ctx *c) {
 // z = P^-1 r, lower block-triangular: velocity V-cycle, then the Schur diagonal
 extern "C++" {
 template <class T>
-static int bfbt_precond(i2e_ctx *c, VelMG<T> &v, CellMG<T> &lm, const double *r, double *z) {
+static int bfbt_precond(i2e_ctx *c, VelMG<T> &v, CellMG<T> &lm, const double *r, double *z, bool presplit) {
   const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
   Grid g = c->grid();
   VelLevel<T> &f0 = v.L[0];
   CellLevel<T> &l0 = lm.L[0];
-  hipLaunchKernelGGL(k_split_v<T>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, f0.fx.p, f0.fy.p);
+  if (!presplit) hipLaunchKernelGGL(k_split_v<T>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, f0.fx.p, f0.fy.p);
   ST_OK(v.vcycle(0, c->s));
   hipLaunchKernelGGL(k_bf_in<T>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, r, (const T *)f0.ux.p,
                      (const T *)f0.uy.p, z, l0.f.p);
@@ -993,26 +1007,29 @@ static int bfbt_precond(i2e_ctx *c, VelMG<T> &v, CellMG<T> &lm, const double *r,
   hipLaunchKernelGGL(k_bf_div<T>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, g, lm.L[0].mask.p, (const T *)f0.ux2.p,
                      (const T *)f0.uy2.p, l0.f.p);
   ST_OK(lm.vcycle(0, c->s));
-  hipLaunchKernelGGL(k_bf_out<T>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, g, o.mu, lm.L[0].mask.p,
-                     (const T *)l0.u.p, z);
-  hipLaunchKernelGGL(k_schur_bc, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, r, z);
+  hipLaunchKernelGGL(k_bf_out_bc<T>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, (const T *)l0.u.p, r, z);
   return I2E_OK;
 }
 }  // extern "C++"
 
-static int stokes_precond(i2e_ctx *c, const double *r, double *z) {
+// presplit: the velocity components of r are already in the finest velocity
+// level's right-hand side (left there by the Krylov projection pass)
+static int stokes_precond(i2e_ctx *c, const double *r, double *z, bool presplit = false) {
   const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
-  if (c->o.schur != I2E_SCHUR_MASS) return mixed(c) ? bfbt_precond(c, c->v32, c->l32, r, z)
-                                                     : bfbt_precond(c, c->v64, c->l64, r, z);
+  if (c->o.schur != I2E_SCHUR_MASS) return mixed(c) ? bfbt_precond(c, c->v32, c->l32, r, z, presplit)
+                                                     : bfbt_precond(c, c->v64, c->l64, r, z, presplit);
   if (mixed(c)) {
-    hipLaunchKernelGGL(k_split_v<float>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v32.L[0].fx.p, c->v32.L[0].fy.p);
+    if (!presplit)
+      hipLaunchKernelGGL(k_split_v<float>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v32.L[0].fx.p,
+                         c->v32.L[0].fy.p);
     ST_OK(c->v32.vcycle(0, c->s));
     hipLaunchKernelGGL(k_schur<float>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->o.schur_scale, r,
                        c->v32.L[0].ux.p, c->v32.L[0].uy.p, z);
   } else {
-    hipLaunchKernelGGL(k_split_v<double>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v64.L[0].fx.p,
-                       c->v64.L[0].fy.p);
+    if (!presplit)
+      hipLaunchKernelGGL(k_split_v<double>, nb2(X, Y), dim3(256), 0, c->s, X, Y, r, c->v64.L[0].fx.p,
+                         c->v64.L[0].fy.p);
     ST_OK(c->v64.vcycle(0, c->s));
     hipLaunchKernelGGL(k_schur<double>, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->o.schur_scale, r,
                        c->v64.L[0].ux.p, c->v64.L[0].uy.p, z);
@@ -1059,10 +1076,21 @@ static int stokes_solve(i2e_ctx *c) {
       HIP_OK(ei);
     }
   }
+  // without the captured preconditioner, each projection pass writes the
+  // velocity components of the pending vector into the finest velocity
+  // level's right-hand side, saving the preconditioner's split pass
+  if (!exec) {
+    c->fs.sx = mixed(c) ? (void *)c->v32.L[0].fx.p : (void *)c->v64.L[0].fx.p;
+    c->fs.sy = mixed(c) ? (void *)c->v32.L[0].fy.p : (void *)c->v64.L[0].fy.p;
+    c->fs.split_f32 = mixed(c);
+  }
+  c->fs.split_of = nullptr;
   auto Mf = [&](const double *r, double *z) -> int {
     c->kt.begin(K_PRECOND, c->s);
+    const bool presplit = r == c->fs.split_of;
+    c->fs.split_of = nullptr;
     if (!exec) {
-      ST_OK(stokes_precond(c, r, z));
+      ST_OK(stokes_precond(c, r, z, presplit));
     } else {
       hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, (int)n3, r, c->rstage.p);
       HIP_OK(hipGraphLaunch(exec, c->s));
@@ -1074,6 +1102,8 @@ static int stokes_solve(i2e_ctx *c) {
   int it = 0;
   double rr = 0;
   int st = c->fs.solve(A, Mf, c->bs.p, c->xs.p, c->o.stokes_rtol, c->o.maxit, c->s, &it, &rr);
+  c->fs.sx = c->fs.sy = nullptr;
+  c->fs.split_of = nullptr;
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   c->st.stokes_iters = it;

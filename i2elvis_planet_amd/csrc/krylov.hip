@@ -357,10 +357,17 @@ __global__ __launch_bounds__(RB) void k_gsaxpy2(int n, int nv, const double *__r
 
 // <w',w'> from the update pass's G block partials, summed by every block in
 // the same fixed order (so all blocks agree), then v *= 1/sqrt(<w',w'>) over
-// this block's chunk; block 0 stores the sum in *nrm2.  The scaling keeps
-// the pending vector's size from compounding from step to step.
+// this block's chunk; block 0 stores the sum in *nrm2 and, with hout, the
+// pass's nc coefficients coef[0..nc) and the sum into hout[0..nc] (host
+// memory, read after the stream's synchronisation: no separate copy).  The
+// scaling keeps the pending vector's size from compounding from step to
+// step.  With sx, the scaled velocity unknowns 3m+1 and 3m+2 also go to
+// sx[m] and sy[m] (the next preconditioner's right-hand side).
+template <class S>
 __global__ __launch_bounds__(RB) void k_norm_scale(int n, int G, const double *__restrict__ part,
-                                                   double *__restrict__ nrm2, double *__restrict__ v) {
+                                                   double *__restrict__ nrm2, double *__restrict__ v, int nc,
+                                                   const double *__restrict__ coef, double *__restrict__ hout,
+                                                   S *__restrict__ sx, S *__restrict__ sy) {
   __shared__ double sh[RB];
   double sum = 0;
   for (int b = threadIdx.x; b < G; b += RB) sum += part[b];
@@ -371,14 +378,27 @@ __global__ __launch_bounds__(RB) void k_norm_scale(int n, int G, const double *_
     __syncthreads();
   }
   const double q = sh[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) *nrm2 = q;
-  if (!(q > 0)) return;
-  const double f = 1.0 / sqrt(q);
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) *nrm2 = q;
+    if (hout) {
+      for (int t = threadIdx.x; t < nc; t += RB) hout[t] = coef[t];
+      if (threadIdx.x == 0) hout[nc] = q;
+    }
+  }
+  const double f = q > 0 ? 1.0 / sqrt(q) : 1.0;
   const size_t base = (size_t)blockIdx.x * CH + threadIdx.x;
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
     const size_t k = base + (size_t)e * RB;
-    if (k < (size_t)n) v[k] *= f;
+    if (k < (size_t)n) {
+      const double x = v[k] * f;
+      v[k] = x;
+      if (sx) {
+        const size_t mm = k / 3, cmp = k - 3 * mm;
+        if (cmp == 1) sx[mm] = S(x);
+        if (cmp == 2) sy[mm] = S(x);
+      }
+    }
   }
 }
 
@@ -437,6 +457,7 @@ int Fgmres::init(int n_, int m_) {
   ST_OK(part.alloc((size_t)gs_blocks(n) * (2 * m + 4)));
   ST_OK(hd.alloc(8 * (m + 4)));
   ST_OK(hh.alloc(8 * (m + 4)));
+  ST_OK(hc.alloc(8 * (m + 4), hipHostMallocCoherent));
   ST_OK(done.alloc(1));
   HIP_OK(hipMemset(done.p, 0, sizeof(unsigned)));
   return I2E_OK;
@@ -478,7 +499,7 @@ int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double r
 // One fused lagged-CGS2 step on the pending vector u (slot j of V, projected
 // once, unit norm) and w = A z_j: finishes u into v_j against V[0..j), and
 // projects w once on V[0..j]; w' / ||w'|| goes to wout.  After the caller's
-// synchronisation the host holds hh = [a(j), b(j), uu, uw, ww, beta, h, <w',w'>].
+// synchronisation the host holds hc = [a(j), b(j), uu, uw, ww, beta, h, <w',w'>].
 int Fgmres::lagged_pass(int j, const double *w_, double *wout, hipStream_t s) {
   const int G = gs_blocks(n);
   double *u = V.p + (size_t)j * ld;
@@ -493,8 +514,13 @@ int Fgmres::lagged_pass(int j, const double *w_, double *wout, hipStream_t s) {
   if (kt) kt->begin(K_GSAXPY, s);
   hipLaunchKernelGGL(k_gsaxpy2, dim3(G), dim3(RB), 0, s, n, j, V.p, ld, hd.p + CF, u, w_, u, wout, part.p);
   if (kt) kt->end(K_GSAXPY, s, bytes_axpy);
-  hipLaunchKernelGGL(k_norm_scale, dim3(G), dim3(RB), 0, s, n, G, part.p, hd.p + R + 2 * j + 5, wout);
-  HIP_OK(hipMemcpyAsync(hh.p, hd.p + R, sizeof(double) * (2 * j + 6), hipMemcpyDeviceToHost, s));
+  if (split_f32 || !sx)
+    hipLaunchKernelGGL(k_norm_scale<float>, dim3(G), dim3(RB), 0, s, n, G, part.p, hd.p + R + 2 * j + 5, wout,
+                       2 * j + 5, hd.p + R, hc.p, (float *)sx, (float *)sy);
+  else
+    hipLaunchKernelGGL(k_norm_scale<double>, dim3(G), dim3(RB), 0, s, n, G, part.p, hd.p + R + 2 * j + 5, wout,
+                       2 * j + 5, hd.p + R, hc.p, (double *)sx, (double *)sy);
+  split_of = sx ? wout : nullptr;
   return I2E_OK;
 }
 
@@ -581,8 +607,8 @@ int Fgmres::solve_lagged(const Op &A, const Op &M, const double *b, double *x, d
         H[0] = hh.p[0];
         sigma = std::sqrt(hh.p[2] > 0 ? hh.p[2] : 0.0);
       } else {
-        const double *a = hh.p, *bb = hh.p + j;
-        const double bet = hh.p[2 * j + 3], h = hh.p[2 * j + 4], ww = hh.p[2 * j + 5];
+        const double *a = hc.p, *bb = hc.p + j;
+        const double bet = hc.p[2 * j + 3], h = hc.p[2 * j + 4], ww = hc.p[2 * j + 5];
         // column j-1 is final: A z_{j-1} = ... + sigma (beta v_j + V a)
         for (int i = 0; i < j; i++) H[(size_t)i * m + (j - 1)] += sigma * a[i];
         H[(size_t)j * m + (j - 1)] = sigma * bet;
@@ -749,6 +775,10 @@ void Fgmres::release() {
   if (hh.p) hipHostFree(hh.p);
   hh.p = nullptr;
   hh.n = 0;
+  if (hc.p) hipHostFree(hc.p);
+  hc.p = nullptr;
+  hc.n = 0;
+  split_of = nullptr;
 }
 
 }  // namespace i2e

@@ -12,12 +12,12 @@ template <class T>
 struct HBuf {
   T *p = nullptr;
   size_t n = 0;
-  int alloc(size_t count) {
+  int alloc(size_t count, unsigned flags = hipHostMallocDefault) {
     if (count <= n && p) return I2E_OK;
     if (p) (void)hipHostFree(p);
     p = nullptr;
     n = 0;
-    HIP_OK(hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault));
+    HIP_OK(hipHostMalloc((void **)&p, count * sizeof(T), flags));
     n = count;
     return I2E_OK;
   }
@@ -30,6 +30,14 @@ struct Fgmres {
   DBuf<double> V, Z, w, part, hd;
   DBuf<unsigned> done;   // finished-block counter of k_reduce_coef (zero between launches)
   HBuf<double> hh;
+  HBuf<double> hc;       // coherent: the projection pass's last kernel stores its coefficients here
+  // Optional split of each pending vector that a projection pass leaves
+  // (the next preconditioner input): unknown k = 3 m + 1 into sx[m], 3 m + 2
+  // into sy[m], as float (split_f32) or double.  split_of is the vector whose
+  // split sx/sy hold; the preconditioner clears it when it takes the split.
+  void *sx = nullptr, *sy = nullptr;
+  bool split_f32 = false;
+  const double *split_of = nullptr;
   KTimer *kt = nullptr;  // optional per-launch timing of the Gram-Schmidt kernels
   int reorth = 0;        // extra Gram-Schmidt passes in the last solve
   int orth = 0;          // ORTH_LAGGED or ORTH_DGKS
