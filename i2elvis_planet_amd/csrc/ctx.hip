@@ -519,6 +519,7 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
     return s_;
   };
   if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return fail(I2E_ERR_HIP);
+  if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) return fail(I2E_ERR_HIP);
   c->fs.kt = &c->kt;  // Gram-Schmidt launches are timed for the Stokes solver only
   c->v32.kt = &c->kt;
   c->v64.kt = &c->kt;
@@ -588,6 +589,8 @@ int i2e_destroy(i2e_ctx *c) {
   c->gmask.release();
   if (c->pin.p) hipHostFree(c->pin.p);
   if (c->dP) hipFree(c->dP);
+  if (c->s2) hipStreamSynchronize(c->s2);
+  if (c->s2) hipStreamDestroy(c->s2);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
   return I2E_OK;
@@ -952,7 +955,7 @@ static int stokes_prepare(i2e_ctx *c) {
   }
   c->have_prev = c->o.stokes_guess == I2E_GUESS_EXTRAP;
   c->dt_prev = c->timestep;
-  ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s));
+  ST_OK(c->v64.refresh(c->F.nu, c->F.nd, c->F.ro, c->F.gp, o.mu, o.dt, c->s, c->s2));
   c->v64.nsmooth = c->o.nsmooth;
   c->v64.nsmooth_c = c->o.nsmooth_coarse;
   c->v64.omega = c->o.omega;
@@ -982,6 +985,8 @@ static int stokes_prepare(i2e_ctx *c) {
       c->l32.nsmooth_c = 0;
     }
   }
+  // the velocity coarsest inverse (second stream) is needed from here on
+  ST_OK(c->v64.join(c->s));
   return I2E_OK;
 }
 

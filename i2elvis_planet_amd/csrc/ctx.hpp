@@ -19,6 +19,7 @@ struct i2e_ctx {
   int X = 0, Y = 0, N = 0;
   int64_t Mmax = 0, M = 0;
   hipStream_t s = nullptr;
+  hipStream_t s2 = nullptr;  // second stream: the velocity coarsest inversion overlaps the Stokes set-up
   DBuf<double> gx, gy, nodes, acc, work, sol1, tk2, red, xs, bs;
   DBuf<float> mfl;
   DBuf<int8_t> mt;

@@ -291,55 +291,67 @@ __global__ void k_vdense(VLev<T> L, int n, double *aug) {
 }
 
 // Gauss-Jordan inversion with partial pivoting in one workgroup, on the
-// n x 2n augmented matrix held in LDS (n <= GJ_LDS_N)
+// n x 2n augmented matrix held in LDS (n <= GJ_LDS_N).  Three barriers per
+// pivot: (1) column p is staged and the pivot found by a lane-shuffle
+// maximum over the first two wavefronts (largest |a|, lowest row on ties);
+// (2) rows p and q are exchanged with the new row p divided by the pivot;
+// (3) every other row r subtracts (its staged column-p entry) x row p, which
+// also leaves exact zeros in column p (f - f * 1).
 constexpr int GJ_LDS_N = 90;
 __global__ __launch_bounds__(1024) void k_gauss_jordan_lds(int n, const double *ag, double *inv) {
   __shared__ double a[GJ_LDS_N * 2 * GJ_LDS_N];
-  __shared__ double sv[1024];
-  __shared__ int si[1024];
+  __shared__ double colp[GJ_LDS_N], sv[2], sd[2];
+  __shared__ int si[2];
   const int t = threadIdx.x, nt = blockDim.x, w = 2 * n;
   for (int idx = t; idx < n * w; idx += nt) a[idx] = ag[idx];
+  // elimination layout: thread t updates column t % w of rows t / w + k * rpp
+  const int rpp = nt / w, ce = t % w, re = t / w;
   __syncthreads();
   for (int p = 0; p < n; p++) {
-    double best = -1;
-    int bi = p;
-    for (int r = p + t; r < n; r += nt) {
-      const double v = fabs(a[r * w + p]);
-      if (v > best) {
-        best = v;
+    if (t < n) colp[t] = a[t * w + p];
+    if (t < 128) {
+      const int r = p + t;
+      double best = -1, val = 0;
+      int bi = n;
+      if (r < n) {
+        val = a[r * w + p];
+        best = fabs(val);
         bi = r;
       }
-    }
-    sv[t] = best;
-    si[t] = bi;
-    __syncthreads();
-    for (int o = nt / 2; o > 0; o >>= 1) {
-      if (t < o && sv[t + o] > sv[t]) {
-        sv[t] = sv[t + o];
-        si[t] = si[t + o];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64), ov = __shfl_xor(val, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          val = ov;
+          bi = oi;
+        }
       }
-      __syncthreads();
-    }
-    const int q = si[0];
-    if (q != p)
-      for (int c = t; c < w; c += nt) {
-        const double x = a[p * w + c];
-        a[p * w + c] = a[q * w + c];
-        a[q * w + c] = x;
+      if ((t & 63) == 0) {
+        sv[t >> 6] = best;
+        sd[t >> 6] = val;
+        si[t >> 6] = bi;
       }
-    __syncthreads();
-    const double d = a[p * w + p];
-    __syncthreads();
-    for (int c = t; c < w; c += nt) a[p * w + c] /= d;
-    __syncthreads();
-    for (int idx = t; idx < n * w; idx += nt) {
-      const int r = idx / w, c = idx % w;
-      if (r == p || c == p) continue;
-      a[r * w + c] -= a[r * w + p] * a[p * w + c];
     }
     __syncthreads();
-    for (int r = t; r < n; r += nt)
-      if (r != p) a[r * w + p] = 0.0;
+    const bool w1 = sv[1] > sv[0];  // rows of the second wavefront lie below: ties stay in the first
+    const int q = w1 ? si[1] : si[0];
+    const double d = w1 ? sd[1] : sd[0];
+    if (t < w) {
+      const double x = a[p * w + t], y = a[q * w + t];
+      a[p * w + t] = y / d;
+      if (q != p) a[q * w + t] = x;
+    }
+    __syncthreads();
+    if (re < rpp) {
+      const double pr = a[p * w + ce];
+      for (int r = re; r < n; r += rpp) {
+        if (r == p) continue;
+        const double f = r == q ? colp[p] : colp[r];
+        a[r * w + ce] -= f * pr;
+      }
+    }
     __syncthreads();
   }
   for (int idx = t; idx < n * n; idx += nt) inv[idx] = a[(idx / n) * w + n + idx % n];
@@ -1208,7 +1220,8 @@ int VelMG<T>::setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &k
 
 template <class T>
 int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, const double *gp, double mu_, double dt_,
-                      hipStream_t s) {
+                      hipStream_t s, hipStream_t side_) {
+  ST_OK(join(s));  // a previous inversion still running (refresh without a join)
   mu = mu_;
   dt = dt_;
   const int nl = (int)L.size();
@@ -1234,7 +1247,19 @@ int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, cons
                        dense.p);
   else
     hipLaunchKernelGGL(k_vdense<T>, dim3(nblk(n, 64), n), dim3(64), 0, s, lev(0), n, dense.p);
-  gauss_jordan(n, dense.p, ainv.p, s);
+  if (side_) {
+    // the inversion (one workgroup, ~1 ms at 162 unknowns) overlaps the
+    // caller's further set-up on s until join()
+    if (!ev_fork) HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    if (!ev_join) HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(ev_fork, s));
+    HIP_OK(hipStreamWaitEvent(side_, ev_fork, 0));
+    gauss_jordan(n, dense.p, ainv.p, side_);
+    side = side_;
+  } else {
+    side = nullptr;
+    gauss_jordan(n, dense.p, ainv.p, s);
+  }
   HIP_OK(hipGetLastError());
   int nover = 0;
   HIP_OK(hipMemcpyAsync(&nover, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1270,8 +1295,19 @@ int VelMG<T>::refresh_from(const VelMG<double> &d, hipStream_t s) {
   else
     cast(d.L[0].S, L[0].S, (size_t)2 * N0 * 18);
   for (int l = 1; l < nl; l++) cast(d.L[l].S, L[l].S, (size_t)2 * L[l].N * 50);
-  HIP_OK(hipMemcpyAsync(ainv.p, d.ainv.p, sizeof(double) * (size_t)nc * nc, hipMemcpyDeviceToDevice, s));
+  // behind d's inversion when that runs on d's second stream (joined with it)
+  HIP_OK(hipMemcpyAsync(ainv.p, d.ainv.p, sizeof(double) * (size_t)nc * nc, hipMemcpyDeviceToDevice,
+                        d.side ? d.side : s));
   HIP_OK(hipGetLastError());
+  return I2E_OK;
+}
+
+template <class T>
+int VelMG<T>::join(hipStream_t s) {
+  if (!side) return I2E_OK;
+  HIP_OK(hipEventRecord(ev_join, side));
+  HIP_OK(hipStreamWaitEvent(s, ev_join, 0));
+  side = nullptr;
   return I2E_OK;
 }
 
@@ -1404,6 +1440,10 @@ void VelMG<T>::release() {
   for (auto &b : gfj) b.release();
   dense.release();
   ainv.release();
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  ev_fork = ev_join = nullptr;
+  side = nullptr;
 }
 
 // ---------------------------------------------------------------- scalar MG

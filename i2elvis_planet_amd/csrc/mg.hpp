@@ -129,7 +129,13 @@ struct VelMG {
   // restrict fields from the finest level (FP64 node arrays) and refresh
   // diagonals and the coarse inverse
   int refresh(const double *nu, const double *nd, const double *ro, const double *gp, double mu, double dt,
-              hipStream_t s);
+              hipStream_t s, hipStream_t side = nullptr);
+  // with a side stream, refresh leaves the coarsest inversion running there
+  // (and refresh_from of an FP32 copy queues its inverse copy behind it);
+  // join(s) makes s wait for both before the first V-cycle
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int join(hipStream_t s);
   // the same hierarchy rounded from an FP64 one on the same geometry: the
   // Galerkin products and the coarsest inverse are formed in FP64 (products
   // of rounded fine-level entries lose the near-cancellation that carries
