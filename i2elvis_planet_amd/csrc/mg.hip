@@ -558,6 +558,12 @@ static void gauss_jordan(int n, double *aug, double *inv, hipStream_t s) {
     hipLaunchKernelGGL(k_gauss_jordan<double>, dim3(1), dim3(1024), 0, s, n, aug, inv);
 }
 
+// b = a^T (n x n)
+__global__ void k_transpose(int n, const double *a, double *b) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+  if (c < n) b[(size_t)c * n + r] = a[(size_t)r * n + c];
+}
+
 // u = Ainv f with f = [f0 (n0 entries), f1]
 template <class T>
 __global__ void k_dense_solve(int n, int n0, const double *ainv, const T *f0, const T *f1, T *u0, T *u1) {
@@ -1080,6 +1086,137 @@ __global__ __launch_bounds__(TileShape<T>::B) void k_tresid(TileArgs<T> A, const
   ryo[m] = A.ky[m] == 0 ? T(A.fy[m] - s_apply_lds<T, TA, W, DIFF>(S, R, (size_t)N + m, lx, ly, la, lb)) : T(0);
 }
 
+// The two coarsest velocity levels in one workgroup: ns colour sweeps on the
+// second-coarsest level (the whole level as one tile in LDS, zero outside
+// the grid), its residual restricted to the coarsest level, the coarsest
+// solve by the dense inverse, the prolongated correction and ns reversed
+// sweeps, the pointwise rows re-solved after each smoothing.  Each part does
+// the arithmetic of k_tsmooth, k_tresid and k_vrestrict; one tile instead of
+// several only drops the frozen halo between tiles, and the dense product
+// sums each row in column order.  Four
+// launches per V-cycle fewer.
+constexpr int VT_S = 17, VT_P = VT_S + 2 * TH, VT_B = 320, VT_C = 81;
+template <class T>
+struct VTail {
+  int X, Y, Xc, Yc, ns;
+  const int8_t *kx, *ky, *ckx, *cky;
+  const T *S, *fx, *fy;
+  MapView xn, xm, yn, ym;  // this level -> coarsest
+  const double *ainvT;     // transposed inverse, (2 Xc Yc)^2
+  T *ox, *oy;
+  double omega;
+};
+template <class T>
+__global__ __launch_bounds__(VT_B) void k_vtail(VTail<T> A) {
+  __shared__ T lx[VT_P * VT_P], ly[VT_P * VT_P], rx[VT_S * VT_S], ry[VT_S * VT_S], fc[2 * VT_C], uc[2 * VT_C];
+  const int t = threadIdx.x, X = A.X, Y = A.Y, N = X * Y;
+  const size_t R = 2 * (size_t)N;
+  for (int q = t; q < VT_P * VT_P; q += VT_B) {
+    lx[q] = T(0);
+    ly[q] = T(0);
+  }
+  // lanes past the level run the colour steps at (0,0) with a zero stencil
+  // and never store
+  const bool ok = t < N;
+  const int i = ok ? t / Y : 0, j = ok ? t % Y : 0, m = i * Y + j;
+  const int la = i + TH, lb = j + TH, li = la * VT_P + lb;
+  const int kx = ok ? A.kx[m] : -1, ky = ok ? A.ky[m] : -1;
+  const T fx = ok ? A.fx[m] : T(0), fy = ok ? A.fy[m] : T(0);
+  constexpr int NSL = 50;
+  T sx[NSL], sy[NSL];
+#pragma unroll
+  for (int q = 0; q < NSL; q++) {
+    sx[q] = ok ? A.S[(size_t)q * R + m] : T(0);
+    sy[q] = ok ? A.S[(size_t)q * R + (size_t)N + m] : T(0);
+  }
+  const T dx = ok ? sx[sslot(2, 0, 0, 0)] : T(1), dy = ok ? sy[sslot(2, 1, 0, 0)] : T(1);
+  const T rdx = T(A.omega) / dx, rdy = T(A.omega) / dy;
+  auto apply = [&](const T *sv) {
+    T acc = 0;
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++) {
+      const T *u = cc ? ly : lx;
+#pragma unroll
+      for (int di = -2; di <= 2; di++)
+#pragma unroll
+        for (int dj = -2; dj <= 2; dj++) acc = fmadd(sv[sslot(2, cc, di, dj)], u[(la + di) * VT_P + lb + dj], acc);
+    }
+    return acc;
+  };
+  auto half = [&](auto comp, int c0) {
+    constexpr int C = decltype(comp)::value;
+    T *u = C == 1 ? lx : ly;
+    const T *sv = C == 1 ? sx : sy;
+    const T f = C == 1 ? fx : fy;
+    const T rd = C == 1 ? rdx : rdy;
+    for (int c = 0; c < 2; c++) {
+      const bool act = ((i + j) & 1) == (c0 ^ c);
+      T nv = 0;
+      if (act) nv = T(u[li] + (f - apply(sv)) * rd);
+      __syncthreads();
+      if (act && ok) u[li] = nv;
+      __syncthreads();
+    }
+  };
+  // pointwise rows re-solved exactly from the smoothed neighbours
+  auto resolve = [&](T &bx, T &by) {
+    bx = lx[li];
+    by = ly[li];
+    if (kx > 0) bx = T(bx + (fx - apply(sx)) / dx);
+    if (ky > 0) by = T(by + (fy - apply(sy)) / dy);
+  };
+  __syncthreads();
+  for (int it = 0; it < A.ns; it++) {
+    half(std::integral_constant<int, 1>{}, 0);
+    half(std::integral_constant<int, 2>{}, 0);
+  }
+  T bx, by;
+  resolve(bx, by);
+  __syncthreads();
+  if (ok) {
+    lx[li] = bx;
+    ly[li] = by;
+  }
+  __syncthreads();
+  if (ok) {
+    rx[m] = kx == 0 ? T(fx - apply(sx)) : T(0);
+    ry[m] = ky == 0 ? T(fy - apply(sy)) : T(0);
+  }
+  __syncthreads();
+  const int Nc = A.Xc * A.Yc, n = 2 * Nc;
+  for (int q = t; q < Nc; q += VT_B) {
+    const int ic = q / A.Yc, jc = q % A.Yc;
+    fc[q] = (A.ckx[q] == 0 && jc <= A.Yc - 2) ? restrict_rw((const T *)rx, Y, A.xn, A.ym, ic, jc) : T(0);
+    fc[Nc + q] = (A.cky[q] == 0 && ic <= A.Xc - 2) ? restrict_rw((const T *)ry, Y, A.xm, A.yn, ic, jc) : T(0);
+  }
+  __syncthreads();
+  // one thread per row, columns of the transposed inverse (coalesced, the
+  // loads independent of the running sum)
+  for (int r = t; r < n; r += VT_B) {
+    double sum = 0;
+#pragma unroll 8
+    for (int c = 0; c < n; c++) sum += A.ainvT[(size_t)c * n + r] * (double)fc[c];
+    uc[r] = T(sum);
+  }
+  __syncthreads();
+  if (ok) {
+    T vx = lx[li], vy = ly[li];
+    if (kx != 2 && j <= Y - 2) vx += T(prolong_at((const T *)uc, A.Yc, A.xn, A.ym, i, j));
+    if (ky != 2 && i <= X - 2) vy += T(prolong_at((const T *)(uc + Nc), A.Yc, A.xm, A.yn, i, j));
+    lx[li] = vx;
+    ly[li] = vy;
+  }
+  __syncthreads();
+  for (int it = 0; it < A.ns; it++) {
+    half(std::integral_constant<int, 2>{}, 1);
+    half(std::integral_constant<int, 1>{}, 1);
+  }
+  resolve(bx, by);
+  if (!ok) return;
+  A.ox[m] = bx;
+  A.oy[m] = by;
+}
+
 // ---------------------------------------------------------------- VelMG host
 template <class T>
 VLev<T> VelMG<T>::lev(int l) const {
@@ -1213,6 +1350,7 @@ int VelMG<T>::setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &k
   nc = 2 * L[nl - 1].N;
   ST_OK(dense.alloc((size_t)nc * 2 * nc));
   ST_OK(ainv.alloc((size_t)nc * nc));
+  ST_OK(ainvT.alloc((size_t)nc * nc));
   ST_OK(overflow.alloc(1));
   HIP_OK(hipMemset(overflow.p, 0, sizeof(int)));
   return I2E_OK;
@@ -1255,10 +1393,12 @@ int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, cons
     HIP_OK(hipEventRecord(ev_fork, s));
     HIP_OK(hipStreamWaitEvent(side_, ev_fork, 0));
     gauss_jordan(n, dense.p, ainv.p, side_);
+    hipLaunchKernelGGL(k_transpose, dim3(nblk(n, 256), n), dim3(256), 0, side_, n, (const double *)ainv.p, ainvT.p);
     side = side_;
   } else {
     side = nullptr;
     gauss_jordan(n, dense.p, ainv.p, s);
+    hipLaunchKernelGGL(k_transpose, dim3(nblk(n, 256), n), dim3(256), 0, s, n, (const double *)ainv.p, ainvT.p);
   }
   HIP_OK(hipGetLastError());
   int nover = 0;
@@ -1298,8 +1438,21 @@ int VelMG<T>::refresh_from(const VelMG<double> &d, hipStream_t s) {
   // behind d's inversion when that runs on d's second stream (joined with it)
   HIP_OK(hipMemcpyAsync(ainv.p, d.ainv.p, sizeof(double) * (size_t)nc * nc, hipMemcpyDeviceToDevice,
                         d.side ? d.side : s));
+  HIP_OK(hipMemcpyAsync(ainvT.p, d.ainvT.p, sizeof(double) * (size_t)nc * nc, hipMemcpyDeviceToDevice,
+                        d.side ? d.side : s));
   HIP_OK(hipGetLastError());
   return I2E_OK;
+}
+
+// the two coarsest levels in one k_vtail launch: second-coarsest level l
+// with the 5x5 Galerkin stencil, small enough for one workgroup; FP32 only
+// (the FP64 instance needs every register and spills: 95 us against ~45 us
+// for the launches it replaces at 17^2)
+template <class T>
+bool VelMG<T>::tail_at(int l) const {
+  const int nl = (int)L.size();
+  return sizeof(T) == 4 && l >= 1 && l == nl - 2 && L[l].W == 2 && L[l].X <= VT_S && L[l].Y <= VT_S && L[l].N <= VT_B &&
+         L[nl - 1].N <= VT_C;
 }
 
 template <class T>
@@ -1318,6 +1471,15 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   if (l == nl - 1) {
     hipLaunchKernelGGL(k_dense_solve<T>, dim3(nblk(nc, 4)), dim3(256), 0, s, nc, v.N, ainv.p, v.fx.p, v.fy.p, v.ux.p,
                        v.uy.p);
+    return I2E_OK;
+  }
+  const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
+  if (tail_at(l)) {
+    const VelLevel<T> &c = L[nl - 1];
+    const VTail<T> a{v.X,      v.Y,      c.X,        c.Y,        ns,         v.kx.p,     v.ky.p,   c.kx.p,
+                     c.ky.p,   v.S.p,    v.fx.p,     v.fy.p,     view(v.mxn), view(v.mxm), view(v.myn),
+                     view(v.mym), ainvT.p, v.ux.p,    v.uy.p,     omega};
+    hipLaunchKernelGGL(k_vtail<T>, dim3(1), dim3(VT_B), 0, s, a);
     return I2E_OK;
   }
   TILE_CONSTS(T);
@@ -1351,7 +1513,6 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   // of reading it)
   const double rbytes = l > 0 ? 2.0 * sizeof(T) * L[l - 1].N : 0.0;
   const MapView none{nullptr, nullptr, nullptr, nullptr};
-  const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
   TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr, 0,
                 none, none, none, none, 1, ns, 0, v.dx.p, v.dy.p};
   A.omega = omega;
@@ -1405,8 +1566,9 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   else
     hipLaunchKernelGGL((k_tresid<T, T, 2>), TG, TBK, 0, s, R, (const T *)v.S.p, v.rx.p, v.ry.p);
   VelLevel<T> &c = L[l + 1];
-  // the coarsest level has no smoothing pass to restrict into
-  if (l + 1 == nl - 1 || !fuse_restrict)
+  // the coarsest level (and the two-level tail) has no smoothing pass to
+  // restrict into
+  if (l + 1 == nl - 1 || !fuse_restrict || tail_at(l + 1))
     hipLaunchKernelGGL(k_vrestrict<T>, grd2(c.X, c.Y), blk2(), 0, s, c.X, c.Y, v.Y, view(v.mxn), view(v.mxm),
                        view(v.myn), view(v.mym), c.kx.p, c.ky.p, v.rx.p, v.ry.p, c.fx.p, c.fy.p);
   ST_OK(vcycle(l + 1, s));
@@ -1440,6 +1602,7 @@ void VelMG<T>::release() {
   for (auto &b : gfj) b.release();
   dense.release();
   ainv.release();
+  ainvT.release();
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   ev_fork = ev_join = nullptr;

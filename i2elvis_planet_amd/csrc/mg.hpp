@@ -113,6 +113,7 @@ struct VelMG {
   int nc = 0;               // coarsest unknown count (2 N)
   DBuf<double> dense;       // n x 2n work
   DBuf<double> ainv;        // n x n, FP64 in every precision (the coarsest solve stays exact)
+  DBuf<double> ainvT;       // its transpose (column c at c*n: one thread per row reads coalesced)
   std::vector<DBuf<int>> gfi, gfj;  // fine line index of each coarse line, per level
   DBuf<int> overflow;       // Galerkin entries that fell outside the 5x5 window (must stay 0)
   double mu = 0, dt = 0;
@@ -136,6 +137,7 @@ struct VelMG {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int join(hipStream_t s);
+  bool tail_at(int l) const;
   // the same hierarchy rounded from an FP64 one on the same geometry: the
   // Galerkin products and the coarsest inverse are formed in FP64 (products
   // of rounded fine-level entries lose the near-cancellation that carries
