@@ -223,18 +223,10 @@ __global__ __launch_bounds__(RB) void k_gsdot2(int n, int nv, const double *__re
 
 // Fixed-order sums of the projection pass's block partials, one block per
 // quantity: r[q] = sum_blk part[q*G + blk], q < 2nv+3, r = [a(nv), b(nv), uu,
-// uw, ww].  The last block to finish then forms the fused update's
-// coefficients:
-//   beta = sqrt(uu - a.a)       (norm of u after its second projection)
-//   h    = (uw - a.b) / beta    (<v, w> for the finished v = (u - V a) / beta)
-//   c    = b - a h / beta       (so that w - V c - (h/beta) u = w - V b - h v)
-// cf = [a(nv), c(nv), 1/beta, h/beta]; r[2nv+3] = beta, r[2nv+4] = h.
-// beta = 0 flags a breakdown (u in the span of V).  *done counts finished
-// blocks and is left at zero.
-__global__ __launch_bounds__(RB) void k_reduce_coef(int nv, int G, const double *__restrict__ part, double *r,
-                                                    double *__restrict__ cf, unsigned *done) {
-  __shared__ double sa[RB], sb[RB];
-  __shared__ bool last;
+// uw, ww].  The update kernel forms its coefficients from r (no grid-wide
+// completion count here: a device-scope fence per block writes back the L2).
+__global__ __launch_bounds__(RB) void k_reduce_coef(int G, const double *__restrict__ part, double *__restrict__ r) {
+  __shared__ double sa[RB];
   const int q = blockIdx.x;
   double sum = 0;
   for (int b = threadIdx.x; b < G; b += RB) sum += part[(size_t)q * G + b];
@@ -244,54 +236,48 @@ __global__ __launch_bounds__(RB) void k_reduce_coef(int nv, int G, const double 
     if (threadIdx.x < o) sa[threadIdx.x] += sa[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    r[q] = sa[0];
-    __threadfence();
-    last = atomicAdd(done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const volatile double *rv = r;
-  double aa = 0, ab = 0;
-  for (int i = threadIdx.x; i < nv; i += RB) {
-    aa += rv[i] * rv[i];
-    ab += rv[i] * rv[nv + i];
-  }
-  sa[threadIdx.x] = aa;
-  sb[threadIdx.x] = ab;
-  __syncthreads();
-  for (int o = RB / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      sa[threadIdx.x] += sa[threadIdx.x + o];
-      sb[threadIdx.x] += sb[threadIdx.x + o];
-    }
-    __syncthreads();
-  }
-  const double b2 = rv[2 * nv] - sa[0];
-  const double beta = b2 > 0 ? sqrt(b2) : 0.0;
-  const double h = beta > 0 ? (rv[2 * nv + 1] - sb[0]) / beta : 0.0;
-  const double t = beta > 0 ? h / beta : 0.0;
-  for (int i = threadIdx.x; i < nv; i += RB) {
-    cf[i] = rv[i];
-    cf[nv + i] = rv[nv + i] - rv[i] * t;
-  }
-  if (threadIdx.x == 0) {
-    cf[2 * nv] = beta > 0 ? 1.0 / beta : 0.0;
-    cf[2 * nv + 1] = t;
-    r[2 * nv + 3] = beta;
-    r[2 * nv + 4] = h;
-    *done = 0;
-  }
+  if (threadIdx.x == 0) r[q] = sa[0];
 }
 
-// Fused update with cf from k_reduce_coef: vout = (u - V a) / beta (the finished
+// Fused update.  Every block first forms the coefficients from the summed
+// quantities r = [a(nv), b(nv), uu, uw, ww] (fixed order, so all blocks agree):
+//   beta = sqrt(uu - a.a)       (norm of u after its second projection)
+//   h    = (uw - a.b) / beta    (<v, w> for the finished v = (u - V a) / beta)
+//   c    = b - a h / beta       (so that w - V c - (h/beta) u = w - V b - h v)
+// and block 0 stores r[2nv+3] = beta, r[2nv+4] = h (beta = 0 flags a
+// breakdown: u in the span of V).  Then vout = (u - V a) / beta (the finished
 // basis vector), wout = w - V c - (h/beta) u (w projected on the finished
 // basis, once), and part[blk] = <wout, wout> over this block's chunk.
 // vout may alias u; wout may alias w.
-__global__ __launch_bounds__(RB) void k_gsaxpy2(int n, int nv, const double *__restrict__ V, size_t ld,
-                                                const double *__restrict__ cf, const double *u, const double *w,
-                                                double *vout, double *wout, double *__restrict__ part) {
+__global__ __launch_bounds__(RB) void k_gsaxpy2(int n, int nv, const double *__restrict__ V, size_t ld, double *r,
+                                                const double *u, const double *w, double *vout, double *wout,
+                                                double *__restrict__ part) {
+  __shared__ double sa[RB], sb[RB];
+  {
+    double aa = 0, ab = 0;
+    for (int i = threadIdx.x; i < nv; i += RB) {
+      aa += r[i] * r[i];
+      ab += r[i] * r[nv + i];
+    }
+    sa[threadIdx.x] = aa;
+    sb[threadIdx.x] = ab;
+    __syncthreads();
+    for (int o = RB / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) {
+        sa[threadIdx.x] += sa[threadIdx.x + o];
+        sb[threadIdx.x] += sb[threadIdx.x + o];
+      }
+      __syncthreads();
+    }
+  }
+  const double b2 = r[2 * nv] - sa[0];
+  const double beta = b2 > 0 ? sqrt(b2) : 0.0;
+  const double hh = beta > 0 ? (r[2 * nv + 1] - sb[0]) / beta : 0.0;
+  const double t = beta > 0 ? hh / beta : 0.0, ib = beta > 0 ? 1.0 / beta : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    r[2 * nv + 3] = beta;
+    r[2 * nv + 4] = hh;
+  }
   __shared__ double sh[RB / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const size_t base = (size_t)blockIdx.x * CH + threadIdx.x;
@@ -303,8 +289,8 @@ __global__ __launch_bounds__(RB) void k_gsaxpy2(int n, int nv, const double *__r
     double a[4], c[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      a[q] = cf[i + q];
-      c[q] = cf[nv + i + q];
+      a[q] = r[i + q];
+      c[q] = r[nv + i + q] - r[i + q] * t;
     }
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
@@ -320,7 +306,7 @@ __global__ __launch_bounds__(RB) void k_gsaxpy2(int n, int nv, const double *__r
     }
   }
   for (; i < nv; i++) {
-    const double a0 = cf[i], c0 = cf[nv + i];
+    const double a0 = r[i], c0 = r[nv + i] - r[i] * t;
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
       const size_t k = base + (size_t)e * RB;
@@ -331,7 +317,6 @@ __global__ __launch_bounds__(RB) void k_gsaxpy2(int n, int nv, const double *__r
       }
     }
   }
-  const double ib = cf[2 * nv], t = cf[2 * nv + 1];
   double acc = 0;
 #pragma unroll
   for (int e = 0; e < EPT; e++) {
@@ -458,8 +443,6 @@ int Fgmres::init(int n_, int m_) {
   ST_OK(hd.alloc(8 * (m + 4)));
   ST_OK(hh.alloc(8 * (m + 4)));
   ST_OK(hc.alloc(8 * (m + 4), hipHostMallocCoherent));
-  ST_OK(done.alloc(1));
-  HIP_OK(hipMemset(done.p, 0, sizeof(unsigned)));
   return I2E_OK;
 }
 
@@ -503,16 +486,16 @@ int Fgmres::solve(const Op &A, const Op &M, const double *b, double *x, double r
 int Fgmres::lagged_pass(int j, const double *w_, double *wout, hipStream_t s) {
   const int G = gs_blocks(n);
   double *u = V.p + (size_t)j * ld;
-  const size_t R = 0, CF = 2 * (size_t)m + 4;  // offsets in hd
+  const size_t R = 0;  // offset in hd
   const double bytes_dot = 8.0 * ((double)j * n + 2.0 * n + (2.0 * j + 3) * G);
   const double bytes_axpy = 8.0 * ((double)j * n + 4.0 * n + G);
   if (kt) kt->begin(K_GSDOT, s);
   hipLaunchKernelGGL(k_gsdot2, dim3(G), dim3(RB), sizeof(double) * NW * (2 * j + 3), s, n, j, V.p, ld, u, w_,
                      part.p);
   if (kt) kt->end(K_GSDOT, s, bytes_dot);
-  hipLaunchKernelGGL(k_reduce_coef, dim3(2 * j + 3), dim3(RB), 0, s, j, G, part.p, hd.p + R, hd.p + CF, done.p);
+  hipLaunchKernelGGL(k_reduce_coef, dim3(2 * j + 3), dim3(RB), 0, s, G, part.p, hd.p + R);
   if (kt) kt->begin(K_GSAXPY, s);
-  hipLaunchKernelGGL(k_gsaxpy2, dim3(G), dim3(RB), 0, s, n, j, V.p, ld, hd.p + CF, u, w_, u, wout, part.p);
+  hipLaunchKernelGGL(k_gsaxpy2, dim3(G), dim3(RB), 0, s, n, j, V.p, ld, hd.p + R, u, w_, u, wout, part.p);
   if (kt) kt->end(K_GSAXPY, s, bytes_axpy);
   if (split_f32 || !sx)
     hipLaunchKernelGGL(k_norm_scale<float>, dim3(G), dim3(RB), 0, s, n, G, part.p, hd.p + R + 2 * j + 5, wout,
@@ -771,7 +754,6 @@ void Fgmres::release() {
   w.release();
   part.release();
   hd.release();
-  done.release();
   if (hh.p) hipHostFree(hh.p);
   hh.p = nullptr;
   hh.n = 0;

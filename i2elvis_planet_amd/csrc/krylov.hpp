@@ -28,7 +28,6 @@ struct Fgmres {
   int n = 0, m = 0;
   size_t ld = 0;
   DBuf<double> V, Z, w, part, hd;
-  DBuf<unsigned> done;   // finished-block counter of k_reduce_coef (zero between launches)
   HBuf<double> hh;
   HBuf<double> hc;       // coherent: the projection pass's last kernel stores its coefficients here
   // Optional split of each pending vector that a projection pass leaves
