@@ -359,13 +359,19 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan_lds(int n, const double *
 
 // In-place Gauss-Jordan inversion with partial pivoting for n too large for
 // LDS: the n x n matrix lives in the registers of one 1024-thread workgroup
-// (thread (g, c) holds rows g*RPG .. g*RPG+RPG-1 of column c), rows and the
-// pivot column are exchanged through LDS, and the column swaps implied by
-// the row pivots are undone when the inverse is written out.
+// (thread (g, c) holds rows g*RPG .. g*RPG+RPG-1 of column c).  Two barriers
+// per pivot k: (A) the threads of column k stage it (double-buffered) and
+// search their rows for the pivot, the threads of row k stage that row;
+// (B) every thread picks the pivot q (largest |a|, lowest row on ties), the
+// holders of row q form the new row k = row q / pivot into LDS and take the
+// old row k in its place; (C) the holders of row k take the new row and all
+// other rows subtract (their staged column-k entry) x row k, the column k
+// entries becoming the inverse's (0 - f / pivot).  The column swaps implied
+// by the row pivots are undone when the inverse is written out.
 constexpr int GJR_T = 1024, GJR_R = 32, GJR_N = 192;
 __global__ __launch_bounds__(GJR_T) void k_gauss_jordan_reg(int n, const double *ag, double *inv) {
-  __shared__ double rowb[GJR_N], colb[GJR_N], pv[2 * GJR_N], sval[GJR_T / 32];
-  __shared__ int sidx[GJR_T / 32], piv[GJR_N], pinv[GJR_N], sq;
+  __shared__ double rowb[GJR_N], rowx[GJR_N], colb[2][GJR_N], pv[2 * GJR_N], sval[GJR_T / 32], sdv[GJR_T / 32];
+  __shared__ int sidx[GJR_T / 32], piv[GJR_N], pinv[GJR_N];
   const int t = threadIdx.x, G = GJR_T / n, rpg = (n + G - 1) / G;
   const int g = t / n, c = t % n;
   const bool act = g < G;
@@ -376,93 +382,69 @@ __global__ __launch_bounds__(GJR_T) void k_gauss_jordan_reg(int n, const double 
     a[k2] = (act && k2 < rpg && r < n) ? ag[(size_t)r * 2 * n + c] : 0.0;
   }
   for (int k = 0; k < n; k++) {
-    // pivot search down column k
+    const int gk = k / rpg, sk = k % rpg, bf = k & 1;
+    // (A) stage column k and search it; stage row k
     if (act && c == k) {
-      double best = -1;
+      double best = -1, bv = 0;
       int bi = k;
-#pragma unroll
-      for (int k2 = 0; k2 < GJR_R; k2++) {
-        const int r = g * rpg + k2;
-        if (k2 < rpg && r < n && r >= k && fabs(a[k2]) > best) {
-          best = fabs(a[k2]);
-          bi = r;
-        }
-      }
-      sval[g] = best;
-      sidx[g] = bi;
-    }
-    __syncthreads();
-    if (t == 0) {
-      double best = -1;
-      int bi = k;
-      for (int q = 0; q < G; q++)
-        if (sval[q] > best) {
-          best = sval[q];
-          bi = sidx[q];
-        }
-      sq = bi;
-      piv[k] = bi;
-    }
-    __syncthreads();
-    const int q = sq, gk = k / rpg, sk = k % rpg, gq = q / rpg, sk2 = q % rpg;
-    // swap rows k and q (through LDS, one column per thread group member)
-    if (q != k) {
-      if (act && g == gk)
-#pragma unroll
-        for (int k2 = 0; k2 < GJR_R; k2++)
-          if (k2 == sk) pv[c] = a[k2];
-      if (act && g == gq)
-#pragma unroll
-        for (int k2 = 0; k2 < GJR_R; k2++)
-          if (k2 == sk2) pv[GJR_N + c] = a[k2];
-      __syncthreads();
-      if (act && g == gk)
-#pragma unroll
-        for (int k2 = 0; k2 < GJR_R; k2++)
-          if (k2 == sk) a[k2] = pv[GJR_N + c];
-      if (act && g == gq)
-#pragma unroll
-        for (int k2 = 0; k2 < GJR_R; k2++)
-          if (k2 == sk2) a[k2] = pv[c];
-      __syncthreads();
-    }
-    // pivot value
-    if (act && g == gk && c == k)
-#pragma unroll
-      for (int k2 = 0; k2 < GJR_R; k2++)
-        if (k2 == sk) pv[0] = a[k2];
-    __syncthreads();
-    const double p = pv[0];
-    // normalise row k (a[k][k] := 1 first: the column becomes the inverse's)
-    if (act && g == gk)
-#pragma unroll
-      for (int k2 = 0; k2 < GJR_R; k2++)
-        if (k2 == sk) {
-          const double v = (c == k ? 1.0 : a[k2]) / p;
-          a[k2] = v;
-          rowb[c] = v;
-        }
-    // factors of column k, zeroed in place below and above the pivot
-    if (act && c == k)
 #pragma unroll
       for (int k2 = 0; k2 < GJR_R; k2++) {
         const int r = g * rpg + k2;
         if (k2 < rpg && r < n) {
-          colb[r] = r == k ? 0.0 : a[k2];
-          if (r != k) a[k2] = 0.0;
+          colb[bf][r] = a[k2];
+          if (r >= k && fabs(a[k2]) > best) {
+            best = fabs(a[k2]);
+            bv = a[k2];
+            bi = r;
+          }
         }
       }
+      sval[g] = best;
+      sdv[g] = bv;
+      sidx[g] = bi;
+    }
+    if (act && g == gk)
+#pragma unroll
+      for (int k2 = 0; k2 < GJR_R; k2++)
+        if (k2 == sk) rowx[c] = a[k2];
     __syncthreads();
+    // (B) pivot row q: row k := row q / pivot (into rowb), row q := old row k
+    double best = -1, d = 0;
+    int q = k;
+    for (int gg = 0; gg < G; gg++)
+      if (sval[gg] > best) {
+        best = sval[gg];
+        d = sdv[gg];
+        q = sidx[gg];
+      }
+    if (t == 0) piv[k] = q;
+    const int gq = q / rpg, sq = q % rpg;
+    if (act && g == gq)
+#pragma unroll
+      for (int k2 = 0; k2 < GJR_R; k2++)
+        if (k2 == sq) {
+          rowb[c] = (c == k ? 1.0 : a[k2]) / d;
+          if (q != k) a[k2] = rowx[c];
+        }
+    __syncthreads();
+    // (C) row k from rowb; elimination of column k from every other row
     if (act) {
       const double rv = rowb[c];
+      if (g == gk)
+#pragma unroll
+        for (int k2 = 0; k2 < GJR_R; k2++)
+          if (k2 == sk) a[k2] = rv;
 #pragma unroll
       for (int k2 = 0; k2 < GJR_R; k2++) {
         const int r = g * rpg + k2;
-        if (k2 < rpg && r < n && r != k) a[k2] -= colb[r] * rv;
+        if (k2 < rpg && r < n && r != k) {
+          const double f = r == q ? colb[bf][k] : colb[bf][r];
+          a[k2] = c == k ? 0.0 - f * rv : a[k2] - f * rv;
+        }
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
   // undo the column swaps: output column pinv[c] = column c
   if (t == 0) {
     for (int j = 0; j < n; j++) pinv[j] = j;  // P
