@@ -195,8 +195,12 @@ typedef struct i2e_stats {
 
 typedef struct i2e_ctx i2e_ctx;
 
-/* Context: one HIP stream, device copies of every field.  max_markers bounds
- * marker storage (markers can only leave through the out-of-grid path).     */
+/* Context: one HIP stream for all work, plus a second one on which the
+ * Stokes set-up inverts the coarsest velocity matrix while the rest of the
+ * set-up proceeds (the first stream waits for it before the solve, so
+ * synchronising the first stream covers both); device copies of every
+ * field.  max_markers bounds marker storage (markers can only leave through
+ * the out-of-grid path).                                                    */
 int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out);
 int i2e_destroy(i2e_ctx *c);
 int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o);
