@@ -200,6 +200,8 @@ def run(steps, warmup, skip_cpu, rk):
         "stokes_iters_mixed": [i[0] for i in it],
         "stokes_iters_fp64": [i[0] for i in res["fp64"][2]],
         "stokes_relres_mixed": [i[1] for i in it],
+        "heat_iters_mixed": [i[3] for i in it],
+        "grav_iters_mixed": [i[2] for i in it],
         "phase_ms_mixed": {"stokes": float(np.mean([i[4] for i in it])), "heat": float(np.mean([i[5] for i in it])),
                            "gravity": float(np.mean([i[6] for i in it])),
                            "markers": float(np.mean([i[7] for i in it]))},
