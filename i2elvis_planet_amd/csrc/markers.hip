@@ -202,10 +202,6 @@ __device__ void porosity_update_d(const i2e_params *p, float &por, float markk, 
 
 // Scatter contributions of one marker: per corner (a,b) of its cell the ten
 // weighted sums of markmart.c:840-911, plus the cell-centre weights of nd.
-struct RonuC {
-  double v[4][10];  // ro, et, cp, kt, ht, w0, tk, w1, nu*swt2, swt2 (wnu)
-  double nd, wnd;
-};
 
 __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
                                                      int64_t M, Acc A, double timesum, double timestep,
@@ -216,12 +212,10 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < M; base += stride) {
     const int64_t mm1 = base + threadIdx.x;
     bool act = mm1 < M;
-    RonuC C;
-#pragma unroll
-    for (int c = 0; c < 4; c++)
-#pragma unroll
-      for (int q = 0; q < 10; q++) C.v[c][q] = 0.0;
-    C.nd = C.wnd = 0.0;
+    // the marker's properties and its four scatter weights (0 off the
+    // active markers); the per-corner products are formed at the scatter
+    double s_ro = 0, s_bb = 0, s_cp = 0, s_kt = 0, s_ht = 0, s_tk = 0, s_nu = 0;
+    double wx0 = 0, wx1 = 0, wy0 = 0, wy1 = 0, c_nd = 0, c_wnd = 0;
     int key = -1;
     if (act) {
       int t = mk.t[mm1];
@@ -325,70 +319,70 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
           double ddx = 1.0 - dabs(w.wx0 - 0.5), ddy = 1.0 - dabs(w.wy0 - 0.5);
           if (ddx > 0.5 && ddy > 0.5) {
             double swt2 = 1.0 * 4.0 * (ddx - 0.5) * (ddy - 0.5);
-            C.nd = mnu * swt2;
-            C.wnd = swt2;
+            c_nd = mnu * swt2;
+            c_wnd = swt2;
           }
         }
-    #pragma unroll
-        for (int a = 0; a < 2; a++)
-    #pragma unroll
-          for (int b = 0; b < 2; b++) {
-            double *v = C.v[a * 2 + b];
-            double ddx = a ? w.wx1 : w.wx0, ddy = b ? w.wy1 : w.wy0;
-            double swt = 1.0 * ddx * ddy;
-            if (ddx > 0.5 && ddy > 0.5) {
-              double swt2 = 1.0 * 4.0 * (ddx - 0.5) * (ddy - 0.5);
-              v[8] = mnu * swt2;
-              v[9] = swt2;
-            }
-            v[0] = mro * swt;
-            v[1] = mbb * swt;
-            v[2] = mcp * swt;
-            v[3] = mkt * swt;
-            v[4] = mht * swt;
-            v[5] = swt;
-            v[6] = mtk * swt;
-            v[7] = swt;
-          }
-
+        wx0 = w.wx0;
+        wx1 = w.wx1;
+        wy0 = w.wy0;
+        wy1 = w.wy1;
+        s_ro = mro;
+        s_bb = mbb;
+        s_cp = mcp;
+        s_kt = mkt;
+        s_ht = mht;
+        s_tk = mtk;
+        s_nu = mnu;
       }
     }
     // one atomic per run of lanes in the same cell (lanes are cell-sorted in
-    // the common case; any order stays correct, just with more runs)
+    // the common case; any order stays correct, just with more runs).  One
+    // corner at a time: its ten products are formed, summed over the run and
+    // added, so only one corner's values are live (the kernel's registers
+    // bound its occupancy).
     const SegScan sc(key, lane);
-#pragma unroll
-    for (int c = 0; c < 4; c++)
-#pragma unroll
-      for (int q = 0; q < 10; q++) C.v[c][q] = sc.sum(C.v[c][q]);
-    C.nd = sc.sum(C.nd);
-    C.wnd = sc.sum(C.wnd);
-    if (sc.tail && key >= 0) {
-      const int m10 = key / g.Y, m20 = key % g.Y;
-      if (C.wnd != 0) {
+    const bool put = sc.tail && key >= 0;
+    const int m10 = put ? key / g.Y : 0, m20 = put ? key % g.Y : 0;
+    {
+      const double nd = sc.sum(c_nd), wnd = sc.sum(c_wnd);
+      if (put && wnd != 0) {
         const int m3 = (m10 + 1) * g.Y + m20 + 1;
-        aadd(&A.nd0[m3], C.nd);
-        aadd(&A.wnd[m3], C.wnd);
+        aadd(&A.nd0[m3], nd);
+        aadd(&A.wnd[m3], wnd);
       }
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-          const int m3 = (m10 + a) * g.Y + m20 + b;
-          const double *v = C.v[a * 2 + b];
-          if (v[9] != 0) {
-            aadd(&A.nu0[m3], v[8]);
-            aadd(&A.wnu[m3], v[9]);
-          }
-          aadd(&A.ro0[m3], v[0]);
-          aadd(&A.et0[m3], v[1]);
-          aadd(&A.cp0[m3], v[2]);
-          aadd(&A.kt0[m3], v[3]);
-          aadd(&A.ht0[m3], v[4]);
-          aadd(&A.w0[m3], v[5]);
-          aadd(&A.tk0[m3], v[6]);
-          aadd(&A.w1[m3], v[7]);
-        }
     }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        const double ddx = a ? wx1 : wx0, ddy = b ? wy1 : wy0;
+        const double swt = 1.0 * ddx * ddy;
+        double v8 = 0, v9 = 0;
+        if (ddx > 0.5 && ddy > 0.5) {
+          const double swt2 = 1.0 * 4.0 * (ddx - 0.5) * (ddy - 0.5);
+          v8 = s_nu * swt2;
+          v9 = swt2;
+        }
+        const double r0 = sc.sum(s_ro * swt), r1 = sc.sum(s_bb * swt), r2 = sc.sum(s_cp * swt);
+        const double r3 = sc.sum(s_kt * swt), r4 = sc.sum(s_ht * swt), r5 = sc.sum(swt);
+        const double r6 = sc.sum(s_tk * swt), r8 = sc.sum(v8), r9 = sc.sum(v9);
+        if (put) {
+          const int m3 = (m10 + a) * g.Y + m20 + b;
+          if (r9 != 0) {
+            aadd(&A.nu0[m3], r8);
+            aadd(&A.wnu[m3], r9);
+          }
+          aadd(&A.ro0[m3], r0);
+          aadd(&A.et0[m3], r1);
+          aadd(&A.cp0[m3], r2);
+          aadd(&A.kt0[m3], r3);
+          aadd(&A.ht0[m3], r4);
+          aadd(&A.w0[m3], r5);
+          aadd(&A.tk0[m3], r6);
+          aadd(&A.w1[m3], r5);  // the second weight sum equals the first
+        }
+      }
   }
 }
 
