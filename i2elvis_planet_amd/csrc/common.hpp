@@ -143,8 +143,22 @@ struct Grid {
 };
 
 // m1serch / m2serch (markmart.c:1051-1088): bisection over grid lines,
-// clamped to n-2.
+// clamped to n-2, i.e. the largest m <= n-2 with g[m] <= x (0 below g[0]).
+// Inside the grid the same m is reached from the cell that the mean spacing
+// predicts, stepping down while g[m] > x and up while g[m+1] <= x (grid
+// lines strictly increasing): one or two loads on a near-uniform grid
+// instead of a chain of log2(n) dependent ones.  Positions outside [g[0],
+// g[n-1]) and NaN take the bisection itself.
 I2E_HD int serch(const double *g, int n, double x) {
+  const double g0 = g[0], gn = g[n - 1];
+  const double f = (x - g0) / (gn - g0) * (double)(n - 1);
+  if (x >= g0 && x < gn && f >= 0 && f < (double)(n - 1)) {
+    int m = (int)f;
+    if (m > n - 2) m = n - 2;
+    while (m > 0 && g[m] > x) m--;
+    while (m < n - 2 && g[m + 1] <= x) m++;
+    return m;
+  }
   int m, m0 = 0, m1 = n - 1;
   do {
     m = (m0 + m1) / 2;
