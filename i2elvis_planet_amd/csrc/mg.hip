@@ -403,10 +403,12 @@ __global__ __launch_bounds__(GJR_T) void k_gauss_jordan_reg(int n, const double 
       sdv[g] = bv;
       sidx[g] = bi;
     }
-    if (act && g == gk)
+    if (act && g == gk) {
+      double x = 0;
 #pragma unroll
-      for (int k2 = 0; k2 < GJR_R; k2++)
-        if (k2 == sk) rowx[c] = a[k2];
+      for (int k2 = 0; k2 < GJR_R; k2++) x = k2 == sk ? a[k2] : x;
+      rowx[c] = x;
+    }
     __syncthreads();
     // (B) pivot row q: row k := row q / pivot (into rowb), row q := old row k
     double best = -1, d = 0;
@@ -419,21 +421,25 @@ __global__ __launch_bounds__(GJR_T) void k_gauss_jordan_reg(int n, const double 
       }
     if (t == 0) piv[k] = q;
     const int gq = q / rpg, sq = q % rpg;
-    if (act && g == gq)
+    if (act && g == gq) {
+      // slot sq by selects (no per-slot branch), one division
+      double y = 0;
 #pragma unroll
-      for (int k2 = 0; k2 < GJR_R; k2++)
-        if (k2 == sq) {
-          rowb[c] = (c == k ? 1.0 : a[k2]) / d;
-          if (q != k) a[k2] = rowx[c];
-        }
+      for (int k2 = 0; k2 < GJR_R; k2++) y = k2 == sq ? a[k2] : y;
+      rowb[c] = (c == k ? 1.0 : y) / d;
+      if (q != k) {
+        const double x = rowx[c];
+#pragma unroll
+        for (int k2 = 0; k2 < GJR_R; k2++) a[k2] = k2 == sq ? x : a[k2];
+      }
+    }
     __syncthreads();
     // (C) row k from rowb; elimination of column k from every other row
     if (act) {
       const double rv = rowb[c];
       if (g == gk)
 #pragma unroll
-        for (int k2 = 0; k2 < GJR_R; k2++)
-          if (k2 == sk) a[k2] = rv;
+        for (int k2 = 0; k2 < GJR_R; k2++) a[k2] = k2 == sk ? rv : a[k2];
 #pragma unroll
       for (int k2 = 0; k2 < GJR_R; k2++) {
         const int r = g * rpg + k2;
