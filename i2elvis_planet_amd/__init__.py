@@ -28,6 +28,7 @@ _EXPORTS = [
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
     "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
+    "i2e_type_counts",
 ]
 
 _lib = None
@@ -93,6 +94,7 @@ def load_library(path=LIB_PATH):
         "i2e_peak_t": ([vp, dp], C.c_int),
         "i2e_adjoint_peak_t": ([vp, C.POINTER(abi.AdjointOut)], C.c_int),
         "i2e_heat_apply_t_dev": ([vp, vp, vp], C.c_int),
+        "i2e_type_counts": ([vp, C.c_int, C.POINTER(C.c_int32)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -293,6 +295,13 @@ class Context:
         o = abi.AdjointOut()
         _chk(_lib.i2e_adjoint_peak_t(self._h, C.byref(o)), "adjoint_peak_t")
         return {k: getattr(o, k) for k, _ in abi.AdjointOut._fields_}
+
+    def type_counts(self, ntypes=64):
+        """Per-cell marker-type histogram of the device markers, shape (cells, ntypes)."""
+        X, Y = self.p.xnumx, self.p.ynumy
+        out = np.zeros((X - 1) * (Y - 1) * ntypes, np.int32)
+        _chk(_lib.i2e_type_counts(self._h, ntypes, out.ctypes.data_as(C.POINTER(C.c_int32))), "type_counts")
+        return out.reshape((X - 1) * (Y - 1), ntypes)
 
     def time_kernels(self, reps=5):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

@@ -218,7 +218,7 @@ def make_state(name, seed=20261203):
     if cfg["iron"] == "random":
         t[inside & (rng2.random(M) < 0.25)] = IRON
     else:
-        sigma, nblob = 2e3, 400
+        sigma, nblob = 2e3, cfg.get("nblob", 400)
         rb = (R - 3 * sigma) * np.sqrt(rng2.random(nblob))
         th = 2 * np.pi * rng2.random(nblob)
         bx, by = size / 2 + rb * np.cos(th), size / 2 + rb * np.sin(th)
@@ -269,3 +269,34 @@ def ensemble(nbody=64, seed=20261203, nmin=101, nmax=301):
                    h0=math.exp(rng.uniform(math.log(1e-8), math.log(3e-7))), e_act=rng.uniform(250e3, 350e3))
         out.append((make_params(cfg), make_state(cfg, seed + 100 * k), cfg))
     return out
+
+
+def heat_state(hs, cfg, t_core=1800.0, t_surface=T_AMBIENT):
+    """The molten regime of the 501^2 setup, seeded directly: marker
+    temperatures follow T(r) = t_surface + (t_core - t_surface)(1 - (r/R)^2)
+    inside the disk (R the body radius) and stay ambient outside.  With the
+    linear melting of 1360-1850 K the core inside r = 0.385 R (t_core 1800 K)
+    exceeds the 0.4 melt fraction (eta -> 1e2 Pa s), a partially molten shell
+    follows, and the cold rim stays at the 1e23 Pa s clip: the 1e21
+    viscosity contrast of the run's hot half (the 26Al heating raises the
+    interior by ~12 K per 2500-yr step, so the solidus is crossed after ~100
+    steps).  marktmax is raised to match; node fields follow from the next
+    ronurecalc.  Modifies and returns hs."""
+    size, R = cfg["size"], cfg["radius"]
+    cx = hs.markx.astype(np.float64) - size / 2.0
+    cy = hs.marky.astype(np.float64) - size / 2.0
+    r2 = (cx * cx + cy * cy) / (R * R)
+    T = np.where(r2 < 1.0, t_surface + (t_core - t_surface) * (1.0 - r2), T_AMBIENT)
+    hs.markk[:] = T.astype(np.float32)
+    hs.marktmax[:] = np.maximum(hs.marktmax, hs.markk)
+    return hs
+
+
+def make_hot(name="c3", seed=20261203, t_core=1800.0, **over):
+    """make(name) with the molten-core temperature field of heat_state; keyword
+    arguments override the config (e.g. n, size, radius for smaller grids)."""
+    cfg = dict(CONFIGS[name], **over)
+    p = make_params(cfg)
+    hs = make_state(cfg, seed)
+    heat_state(hs, cfg, t_core=t_core)
+    return p, hs, cfg

@@ -578,8 +578,9 @@ int i2e_destroy(i2e_ctx *c) {
   }
   DBuf<double> *d[] = {&c->gx, &c->gy, &c->nodes, &c->acc, &c->work, &c->sol1, &c->tk2, &c->red, &c->xs,
                        &c->bs, &c->bondv, &c->hk, &c->hc, &c->hdm, &c->gk, &c->gc, &c->gdm, &c->gsc, &c->hsc,
-                       &c->bcix, &c->bciy, &c->bLS, &c->rstage, &c->zstage};
+                       &c->bcix, &c->bciy, &c->bLS, &c->xprev};
   c->blmask.release();
+  c->tcnt.release();
   for (auto *b : d) b->release();
   c->mfl.release();
   c->mt.release();
@@ -603,9 +604,29 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
       (o->stokes_guess != I2E_GUESS_LAST && o->stokes_guess != I2E_GUESS_EXTRAP) ||
       o->coarse_max < 2 || (o->orth != I2E_ORTH_LAGGED && o->orth != I2E_ORTH_DGKS) || o->fine_stencil < 0 ||
       o->fine_stencil > I2E_FINE_DIFF32 ||
-      (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT))
+      (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT) ||
+      (o->precision != I2E_PREC_FP64 && o->precision != I2E_PREC_MIXED) || (o->graphs != 0 && o->graphs != 1)) {
+    set_error("set_solver: option out of range");
     return I2E_ERR_ARG;
+  }
+  // Replaying the Stokes preconditioner as a captured HIP graph is switched
+  // off: one profiled 501^2 run (kernel trace on) ended in a host SIGSEGV
+  // inside hipGraphLaunch whose cause is not established, and the replay
+  // measured slower than eager launches (66.7 against 64.4 ms per step).
+  if (o->graphs) {
+    set_error("set_solver: graph replay of the Stokes preconditioner is not supported");
+    return I2E_ERR_UNSUPPORTED;
+  }
   bool regeo = c->setup_done && o->coarse_max != c->o.coarse_max;
+  const bool rekrylov = c->setup_done && o->restart != c->o.restart;
+  if (rekrylov) {
+    // the Krylov bases are sized by the restart length: resize them now
+    // (after upload, setup_solvers does not run again)
+    HIP_OK(hipStreamSynchronize(c->s));
+    ST_OK(c->fs.init(3 * c->N, o->restart));
+    ST_OK(c->fh.init(c->N, o->restart));
+    ST_OK(c->fgv.init(c->N, o->restart));
+  }
   c->o = *o;
   c->fs.orth = c->fh.orth = c->fgv.orth = o->orth;
   if (regeo) c->setup_done = false;
@@ -639,6 +660,21 @@ int i2e_get_stats(i2e_ctx *c, i2e_stats *st) {
 
 int i2e_sync(i2e_ctx *c) {
   if (!c) return I2E_ERR_ARG;
+  HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_type_counts(i2e_ctx *c, int ntypes, int32_t *counts) {
+  if (!c || !counts || ntypes < 1 || ntypes > I2E_MAXTYPES) return I2E_ERR_ARG;
+  const size_t n = (size_t)(c->X - 1) * (c->Y - 1) * ntypes;
+  ST_OK(c->tcnt.alloc(n));
+  HIP_OK(hipMemsetAsync(c->tcnt.p, 0, n * sizeof(int), c->s));
+  if (c->M) {
+    unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+    hipLaunchKernelGGL(k_type_counts, dim3(gb), dim3(256), 0, c->s, c->grid(), c->mk, c->M, ntypes, c->tcnt.p);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipMemcpyAsync(counts, c->tcnt.p, n * sizeof(int), hipMemcpyDeviceToHost, c->s));
   HIP_OK(hipStreamSynchronize(c->s));
   return I2E_OK;
 }
@@ -773,8 +809,6 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
     ST_OK(c->bciy.alloc(N));
     ST_OK(c->bLS.alloc(9 * Nc));
     ST_OK(c->blmask.alloc(Nc));
-    ST_OK(c->rstage.alloc((size_t)3 * N));
-    ST_OK(c->zstage.alloc((size_t)3 * N));
   }
   ST_OK(c->fs.init(3 * N, c->o.restart));
   ST_OK(c->fh.init(N, c->o.restart));
@@ -1056,51 +1090,18 @@ static int stokes_solve(i2e_ctx *c) {
     c->kt.end(K_STOKES_APPLY, c->s, bytes);
     return I2E_OK;
   };
-  // The preconditioner (~100 small launches) is captured once per solve
-  // (its kernel arguments carry this step's dt) and replayed per iteration
-  // between two copies through fixed staging vectors.  No memset is captured.
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  const size_t n3 = (size_t)3 * c->N;
-  if (c->o.graphs) {
-    KTimer *k64 = c->v64.kt, *k32 = c->v32.kt;
-    c->v64.kt = c->v32.kt = nullptr;  // per-launch events are not captured
-    HIP_OK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-    const int sp = stokes_precond(c, c->rstage.p, c->zstage.p);
-    const hipError_t ec = hipStreamEndCapture(c->s, &graph);
-    c->v64.kt = k64;
-    c->v32.kt = k32;
-    if (sp) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return sp;
-    }
-    HIP_OK(ec);
-    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ei != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      HIP_OK(ei);
-    }
-  }
-  // without the captured preconditioner, each projection pass writes the
-  // velocity components of the pending vector into the finest velocity
-  // level's right-hand side, saving the preconditioner's split pass
-  if (!exec) {
-    c->fs.sx = mixed(c) ? (void *)c->v32.L[0].fx.p : (void *)c->v64.L[0].fx.p;
-    c->fs.sy = mixed(c) ? (void *)c->v32.L[0].fy.p : (void *)c->v64.L[0].fy.p;
-    c->fs.split_f32 = mixed(c);
-  }
+  // each projection pass writes the velocity components of the pending
+  // vector into the finest velocity level's right-hand side, saving the
+  // preconditioner's split pass
+  c->fs.sx = mixed(c) ? (void *)c->v32.L[0].fx.p : (void *)c->v64.L[0].fx.p;
+  c->fs.sy = mixed(c) ? (void *)c->v32.L[0].fy.p : (void *)c->v64.L[0].fy.p;
+  c->fs.split_f32 = mixed(c);
   c->fs.split_of = nullptr;
   auto Mf = [&](const double *r, double *z) -> int {
     c->kt.begin(K_PRECOND, c->s);
     const bool presplit = r == c->fs.split_of;
     c->fs.split_of = nullptr;
-    if (!exec) {
-      ST_OK(stokes_precond(c, r, z, presplit));
-    } else {
-      hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, (int)n3, r, c->rstage.p);
-      HIP_OK(hipGraphLaunch(exec, c->s));
-      hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, c->s, (int)n3, (const double *)c->zstage.p, z);
-    }
+    ST_OK(stokes_precond(c, r, z, presplit));
     c->kt.end(K_PRECOND, c->s, 0.0);
     return I2E_OK;
   };
@@ -1109,8 +1110,6 @@ static int stokes_solve(i2e_ctx *c) {
   int st = c->fs.solve(A, Mf, c->bs.p, c->xs.p, c->o.stokes_rtol, c->o.maxit, c->s, &it, &rr);
   c->fs.sx = c->fs.sy = nullptr;
   c->fs.split_of = nullptr;
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
   c->st.stokes_iters = it;
   c->st.stokes_relres = rr;
   if (st) {

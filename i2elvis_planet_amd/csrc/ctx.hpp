@@ -48,12 +48,12 @@ struct i2e_ctx {
   double timestep = 0, timesum = 0, gxkoef = 1.0;
   HBuf<double> pin;
   KTimer kt;
-  DBuf<double> rstage, zstage;  // fixed input/output of the captured Stokes preconditioner
   AdjHist *adj = nullptr;       // allocated by i2e_record
   int heat_solved = 0;          // the last i2e_titerate ran its solve and marker update
   DBuf<double> xprev;           // Stokes solution of the step before the last (extrapolated guess)
   int have_prev = 0;            // xprev holds a solution of this run (reset by i2e_upload)
   double dt_prev = 0;           // timestep of the last Stokes solve
+  DBuf<int> tcnt;               // per-cell marker-type counts (i2e_type_counts)
 
   Grid grid() const { return Grid{X, Y, P.xsize, P.ysize, gx.p, gy.p}; }
   SOp<double> sop() const { return SOp<double>{X, Y, gx.p, gy.p, F.nu, F.nd, F.ro, F.gp, timestep, P.nubeg * P.strmin}; }

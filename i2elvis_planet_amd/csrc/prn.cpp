@@ -278,7 +278,17 @@ int i2e_prn_read(const char *path, i2e_params *p, i2e_host_state *s, int *n0) {
   s->gxkoef = h.gxkoef;
   s->timesum = h.timesum;
   if (n0) *n0 = h.n0;
-  r.skip(4L * h.rocknum);  // markim
+  // markim: immobile rock types are never advected and are left out of the
+  // temperature sums (markmart.c:27, 905); this build moves every marker, so
+  // a file that marks any type immobile is refused rather than run as mobile
+  for (int t = 0; t < h.rocknum; t++) {
+    const int32_t im = r.v<int32_t>();
+    if (r.ok && im != 0) {
+      fclose(r.f);
+      i2e::set_error("prn: rock type %d is immobile (markim = %d); immobile markers are not supported", t, im);
+      return I2E_ERR_UNSUPPORTED;
+    }
+  }
   for (int k = 0; k < NDROCK; k++)
     for (int t = 0; t < h.rocknum; t++) set_rock_value(p->rock[t], k, r.v<double>());
   for (int64_t m = 0; m < N && r.ok; m++) {

@@ -174,7 +174,8 @@ typedef struct i2e_solver_opts {
                               and 1.2 each took 1-4 more iterations at 501^2) */
   int32_t coarse_max;      /* coarsest grid: cells per direction <= this     */
   int32_t schur;           /* Schur preconditioner: I2E_SCHUR_BFBT or _MASS  */
-  int32_t graphs;          /* 1: replay the Stokes preconditioner as a HIP graph */
+  int32_t graphs;          /* must be 0: graph replay of the Stokes
+                              preconditioner is refused (I2E_ERR_UNSUPPORTED) */
   int32_t nsmooth_coarse;  /* velocity multigrid: sweeps below the finest level (0: nsmooth) */
   int32_t orth;            /* FGMRES orthogonalisation: I2E_ORTH_LAGGED or _DGKS */
   int32_t fine_stencil;    /* finest velocity level smoothed with I2E_FINE_*  */
@@ -220,6 +221,13 @@ int i2e_ronurecalc(i2e_ctx *c);                          /* markmart.c:192  */
  * titerate, movemark, ronurecalc, timesum += timestep).                     */
 int i2e_step(i2e_ctx *c);
 int i2e_get_scalars(i2e_ctx *c, double *timestep, double *timesum, double *gxkoef);
+/* Per-cell marker-type histogram of the device-resident markers:
+ * counts[(c1*(ynumy-1) + c2)*ntypes + t] = number of markers of type t in
+ * cell (c1, c2), the cell being (m1serch, m2serch) of the marker position
+ * (markmart.c:1051-1088) for markers strictly inside the box; host array of
+ * (xnumx-1)*(ynumy-1)*ntypes ints.  The quantity the reference's scatter
+ * (markmart.c:840-911) distributes per cell, exported for bit-exact checks. */
+int i2e_type_counts(i2e_ctx *c, int ntypes, int32_t *counts);
 int i2e_set_scalars(i2e_ctx *c, double timestep, double timesum);
 int i2e_get_stats(i2e_ctx *c, i2e_stats *st);
 /* Synchronise the context's stream. */
