@@ -215,6 +215,22 @@ static void allintere(const G *g, const i2e_host_state *s, double x, double y, E
   e->ga = isum(g, &w, s->ga);
 }
 
+/* allintere at one point for the tests: out8 = {exy, sxy, exx, sxx, eyy,
+ * syy, pr, ga}                                                               */
+void orc_allintere(const i2e_params *p, const i2e_host_state *s, double x, double y, double *out8) {
+  G g = mkG(p, s);
+  Ev e;
+  allintere(&g, s, x, y, &e);
+  out8[0] = e.exy;
+  out8[1] = e.sxy;
+  out8[2] = e.exx;
+  out8[3] = e.sxx;
+  out8[4] = e.eyy;
+  out8[5] = e.syy;
+  out8[6] = e.pr;
+  out8[7] = e.ga;
+}
+
 /* ------------------------------------------------------------------------ */
 /* rheology, density (markmart.c:1855-2063, 1024-1046)                       */
 /* ------------------------------------------------------------------------ */

@@ -64,6 +64,7 @@ def lib():
             f.restype = C.c_int
         for name, args in [("orc_gravity_post", [P, S, dp]), ("orc_stokes_post", [P, S, dp]),
                            ("orc_allinterv", [P, S, C.c_double, C.c_double, dp]),
+                           ("orc_allintere", [P, S, C.c_double, C.c_double, dp]),
                            ("orc_csr_matvec", [Cp, dp, dp])]:
             f = getattr(L, name)
             f.argtypes = args

@@ -1,0 +1,402 @@
+"""Closed-form and invariant pins of the oracle's marker closures and row
+terms that no reference fixture covers: allintere's two sub-grids, the marker
+temperature update with subgrid diffusion (heatmart.c:515-689), the linear
+melt closure (markmart.c:2178-2220, 2599-2613), dencalc's T/P terms
+(markmart.c:1024-1046), the porous conductivity (markmart.c:614-638), the
+shear-heating source (heatmart.c:74-87), Henke cold pressing and sintering
+(markmart.c:363-468) and the Duretz stabilisation rows (movemart.c:116-125,
+259-270).  Each expectation is computed here from the mathematics, not by
+calling the oracle's own formula.  CPU only."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from i2elvis_planet_amd import configs
+
+dp = C.POINTER(C.c_double)
+
+
+def small(n=13, size=48e3, mnum=2, seed=5, base="c1", **kw):
+    cfg = dict(configs.CONFIGS[base], n=n, size=size, mnum=mnum, radius=size / 4, **kw)
+    return configs.make_params(cfg), configs.make_state(cfg, seed), cfg
+
+
+def pure_nodes(hs, t):
+    """Nodes all of whose four adjacent cells hold only markers of type t (the
+    scatter reaches a node from the markers of its four cells)."""
+    X, Y = hs.xnumx, hs.ynumy
+    i = np.clip(np.searchsorted(hs.gx, hs.markx.astype(np.float64), side="right") - 1, 0, X - 2)
+    j = np.clip(np.searchsorted(hs.gy, hs.marky.astype(np.float64), side="right") - 1, 0, Y - 2)
+    bad = np.zeros((X - 1, Y - 1), bool)
+    np.logical_or.at(bad, (i, j), hs.markt != t)
+    ok = np.zeros((X, Y), bool)
+    ok[1:-1, 1:-1] = ~(bad[:-1, :-1] | bad[1:, :-1] | bad[:-1, 1:] | bad[1:, 1:])
+    return ok.ravel()
+
+
+# ------------------------------------------------------------------ allintere
+def test_allintere_reproduces_affine_fields_on_both_subgrids(orc):
+    """Basic-node fields (exy, sxy) and cell-centre fields (exx, sxx, eyy,
+    syy, pr, ga) that are affine in x, y are reproduced at any point of the
+    box, edges and corners included (the clamped stencils extrapolate);
+    ghost cells (index 0) are never read (they hold NaN here), and points
+    outside the box evaluate at the clamped position."""
+    p, hs, _ = small(n=11, size=40e3)
+    X, Y = hs.xnumx, hs.ynumy
+    gx, gy = hs.gx, hs.gy
+    xm = np.r_[np.nan, (gx[:-1] + gx[1:]) / 2]
+    ym = np.r_[np.nan, (gy[:-1] + gy[1:]) / 2]
+    coef = np.random.default_rng(3).standard_normal((8, 3))
+    f = lambda k, x, y: coef[k, 0] + coef[k, 1] * x / 1e4 + coef[k, 2] * y / 1e4
+    node = [hs.exy, hs.sxy]
+    cell = [hs.exx, hs.sxx, hs.eyy, hs.syy, hs.pr, hs.ga]
+    for k, a in enumerate(node):
+        a[:] = f(k, gx[:, None], gy[None, :]).ravel()
+    for k, a in enumerate(cell):
+        v = f(k + 2, xm[:, None], ym[None, :])
+        a[:] = v.ravel()  # NaN in row 0 and column 0 (ghost cells)
+    L = orc.lib()
+    cs = hs.cstruct()
+    out = np.zeros(8)
+    rng = np.random.default_rng(4)
+    pts = list(zip(rng.uniform(0, p.xsize, 200), rng.uniform(0, p.ysize, 200)))
+    pts += [(0.0, 0.0), (p.xsize, p.ysize), (0.0, p.ysize), (1.0, 17e3), (p.xsize - 1.0, 3e3), (gx[3], gy[5])]
+    for x, y in pts:
+        L.orc_allintere(C.byref(p), C.byref(cs), x, y, out.ctypes.data_as(dp))
+        want = [f(k, x, y) for k in range(8)]
+        assert np.all(np.isfinite(out))
+        assert np.allclose(out, want, rtol=1e-12, atol=1e-12), (x, y)
+    for (x, y), (cx, cy) in [((-5e3, 7e3), (0.0, 7e3)), ((50e3, -1e3), (p.xsize, 0.0))]:
+        L.orc_allintere(C.byref(p), C.byref(cs), x, y, out.ctypes.data_as(dp))
+        assert np.allclose(out, [f(k, cx, cy) for k in range(8)], rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------------ marker temperature update
+def heat_markers(orc, p, hs, tk2):
+    tk2 = np.ascontiguousarray(tk2, np.float64)
+    cs = hs.cstruct()
+    assert orc.lib().orc_heat_markers(C.byref(p), C.byref(cs), tk2.ctypes.data_as(dp)) == 0
+
+
+def uniform_heat_state(p, hs, T_old=500.0, T_new=512.5, k=3.0, ro=3000.0, cp=1000.0):
+    hs.kt[:], hs.ro[:], hs.cp[:] = k, ro, cp
+    hs.tk[:] = T_new
+    return np.full(hs.N, T_old)
+
+
+def park_outside(hs, keep):
+    """Markers not in `keep` leave the box (the update skips them)."""
+    out = np.ones(hs.marknum, bool)
+    out[keep] = False
+    hs.markx[out] = np.float32(-1.0)
+
+
+def test_marker_update_one_marker_keeps_only_the_nodal_change(orc):
+    """A lone marker's subgrid correction is removed again by the
+    antidiffusion pass (its node holds only its own dtk1): T' = T + (e2 - e3)."""
+    p, hs, _ = small()
+    tk2 = uniform_heat_state(p, hs)
+    hs.timestep = 3e12
+    park_outside(hs, [0])
+    hs.markx[0], hs.marky[0] = np.float32(13.7e3), np.float32(21.1e3)
+    hs.markk[0] = np.float32(731.0)
+    heat_markers(orc, p, hs, tk2)
+    assert hs.markk[0] == pytest.approx(731.0 + 12.5, abs=1e-4)
+
+
+def test_marker_update_new_marker_takes_the_node_temperature(orc):
+    """markk <= 0 (a marker that has just entered) starts from e3, so a lone
+    one ends at e2 (heatmart.c:548-551)."""
+    p, hs, _ = small()
+    tk2 = uniform_heat_state(p, hs)
+    hs.timestep = 3e12
+    park_outside(hs, [0])
+    hs.markx[0], hs.marky[0] = np.float32(9.3e3), np.float32(30.2e3)
+    hs.markk[0] = np.float32(0.0)
+    heat_markers(orc, p, hs, tk2)
+    assert hs.markk[0] == pytest.approx(512.5, abs=1e-4)
+
+
+@pytest.mark.parametrize("dt", [1e11, 3e12, 1e20])
+def test_marker_update_two_colocated_markers_relax_by_the_closed_form(orc, dt):
+    """Two markers at one point, temperatures Ta, Tb, uniform node fields: each
+    relaxes towards e3 by f = 1 - exp(-k/(rho cp) 2 (1/dx^2 + 1/dy^2) dt)
+    (the exponent clamped at -150), the antidiffusion removes the mean of the
+    two changes, so Ta' = Ta - (Ta - Tb) f / 2 + (e2 - e3)."""
+    p, hs, _ = small()
+    k, ro, cp = 3.0, 3000.0, 1000.0
+    tk2 = uniform_heat_state(p, hs, k=k, ro=ro, cp=cp)
+    hs.timestep = dt
+    park_outside(hs, [0, 1])
+    for i in (0, 1):
+        hs.markx[i], hs.marky[i] = np.float32(17.3e3), np.float32(25.9e3)
+    Ta, Tb = 700.0, 420.0
+    hs.markk[0], hs.markk[1] = np.float32(Ta), np.float32(Tb)
+    heat_markers(orc, p, hs, tk2)
+    h = hs.gx[1] - hs.gx[0]
+    D = min(k / (ro * cp) * 2 * (2 / h ** 2) * dt, 150.0)
+    f = 1 - math.exp(-D)
+    assert hs.markk[0] == pytest.approx(Ta - (Ta - Tb) * f / 2 + 12.5, abs=2e-4)
+    assert hs.markk[1] == pytest.approx(Tb - (Tb - Ta) * f / 2 + 12.5, abs=2e-4)
+
+
+def test_marker_update_antidiffusion_conserves_the_subgrid_sum(orc):
+    """With many markers and nonuniform fields, the subgrid changes minus the
+    antidiffusion corrections sum to zero over the markers: sum_i (T'_i - T_i
+    - (e2_i - e3_i)) = 0 (the nodal averages of dtk1 are subtracted with the
+    same weights that formed them).  The state is shifted by a uniform 10 K
+    (e2 - e3 = 10 for every marker)."""
+    p, hs, _ = small(n=17, size=64e3, mnum=3)
+    rng = np.random.default_rng(8)
+    hs.kt[:] = 2 + rng.random(hs.N)
+    hs.ro[:] = 3000 + 500 * rng.random(hs.N)
+    hs.cp[:] = 1000 + 200 * rng.random(hs.N)
+    tk2 = 300 + 400 * rng.random(hs.N)
+    hs.tk[:] = tk2 + 10.0
+    hs.markk[:] = (300 + 400 * rng.random(hs.marknum)).astype(np.float32)
+    hs.markt[:] = configs.SILICATE
+    hs.timestep = 2e12
+    T0 = hs.markk.astype(np.float64).copy()
+    heat_markers(orc, p, hs, tk2)
+    change = hs.markk.astype(np.float64) - T0 - 10.0
+    assert np.abs(change).max() > 1.0
+    assert abs(change.sum()) <= 1e-5 * np.abs(change).sum() + 1e-4 * hs.marknum ** 0.5
+
+
+# ------------------------------------------------------------------ ronurecalc closures
+def ronu(orc, p, hs):
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    return o
+
+
+def melt_state(T, start_type=configs.SILICATE):
+    cfg = dict(configs.CONFIGS["c3"], n=21, size=60e3, mnum=3, radius=20e3, iron="random")
+    p, hs = configs.make_params(cfg), configs.make_state(cfg, 11)
+    inside = hs.markt != configs.AIR
+    hs.markt[inside] = start_type
+    hs.markk[inside] = np.float32(T)
+    hs.markv[:] = 0.0  # viscosity from viscalc at this temperature
+    p.rock[configs.MOLTEN_SILICATE].ro = 2900.0
+    p.rock[configs.MOLTEN_SILICATE].kt = 1.5
+    p.porous[configs.SILICATE] = 0
+    return p, hs
+
+
+@pytest.mark.parametrize("T", [1605.0, 1409.0])
+def test_melt_closure_at_uniform_temperature(orc, T):
+    """Linear melting between Ts = 1360 K and Tl = 1850 K at a uniform
+    temperature: melt fraction xm = (T - Ts)/(Tl - Ts); every silicate marker
+    becomes type 26; density and conductivity mix linearly with the molten
+    type's (2900 kg/m^3, 1.5 W/m/K); cp gains the latent heat 4e5 J/kg over
+    the interval; above xm = 0.4 the viscosity is the melt's 1e2 Pa s, below
+    it the solid Arrhenius value."""
+    p, hs = melt_state(T)
+    sil = hs.markt == configs.SILICATE
+    ronu(orc, p, hs)
+    xm = (T - 1360.0) / (1850.0 - 1360.0)
+    assert np.all(hs.markt[sil] == configs.MOLTEN_SILICATE)
+    pure = pure_nodes(hs, configs.MOLTEN_SILICATE)
+    assert pure.sum() >= 20
+    assert np.allclose(hs.ro[pure], 3500 * (1 - xm) + 2900 * xm, rtol=1e-13)
+    assert np.allclose(hs.kt[pure], 3.0 * (1 - xm) + 1.5 * xm, rtol=1e-13)
+    assert np.allclose(hs.cp[pure], 1000.0 + 4e5 / 490.0, rtol=1e-9)
+    if xm > 0.4:
+        eta = 1e2
+    else:
+        eta = min(max(configs.NU0_ARRH * math.exp(configs.E_ACT / (8.314 * T)), 1e17), 1e23)
+    assert np.allclose(hs.nu[pure], eta, rtol=1e-12)
+
+
+def test_melt_closure_switches_back_below_the_solidus(orc):
+    """Below the solidus a molten marker returns to its solid type.  The step
+    that switches it still takes the pre-switch type's constants (the
+    reference's tdbasecalc evaluates the type it was called with and switches
+    afterwards, markmart.c:2180-2183), the next one the solid type's."""
+    p, hs = melt_state(1300.0, start_type=configs.MOLTEN_SILICATE)
+    mol = hs.markt == configs.MOLTEN_SILICATE
+    ronu(orc, p, hs)
+    assert np.all(hs.markt[mol] == configs.SILICATE)
+    pure = pure_nodes(hs, configs.SILICATE)
+    assert pure.sum() >= 20
+    assert np.allclose(hs.ro[pure], 2900.0, rtol=1e-14)
+    assert np.allclose(hs.cp[pure], 1000.0, rtol=1e-14)
+    ronu(orc, p, hs)
+    assert np.allclose(hs.ro[pure], 3500.0, rtol=1e-14)
+    assert np.allclose(hs.kt[pure], 3.0, rtol=1e-14)
+
+
+def test_dencalc_temperature_and_pressure_terms(orc):
+    """rho = ro (1 - bb (T - 298.15)) (1 + aa (P - 1) 1e-3) (1 - phi) with P in
+    bar from the interpolated pressure, and et = bb / (1 - bb (T - 298.15))."""
+    p, hs, _ = small(base="c2", n=15, size=56e3, mnum=3)
+    p.porosity_on = 0
+    sil = hs.markt != configs.AIR
+    hs.markt[sil] = configs.SILICATE
+    aa, bb, T, P, phi = 0.01, 3e-5, 800.0, 1e8, 0.2
+    r = p.rock[configs.SILICATE]
+    r.aa, r.bb = aa, bb
+    hs.markk[sil] = np.float32(T)
+    hs.markpor[sil] = np.float32(phi)
+    hs.pr[:] = P
+    ronu(orc, p, hs)
+    pure = pure_nodes(hs, configs.SILICATE)
+    assert pure.sum() >= 10
+    mpb = P * 1e-5
+    phi32 = float(np.float32(phi))
+    T32 = float(np.float32(T))
+    want = 3500.0 * (1 - bb * (T32 - 298.15)) * (1 + aa * (mpb - 1) * 1e-3) * (1 - phi32)
+    assert np.allclose(hs.ro[pure], want, rtol=1e-13)
+    assert np.allclose(hs.et[pure], bb / (1 - bb * (T32 - 298.15)), rtol=1e-13)
+
+
+@pytest.mark.parametrize("phi", [0.1, 0.3, 0.45])
+def test_porous_conductivity_closed_forms(orc, phi):
+    """k(phi) = k0 exp(-phi/0.08) below 0.2, k0 exp(-1.2 - phi/0.167) above 0.4,
+    and their 4-norm (a^4 + b^4)^(1/4) in between (k0 = 3 W/m/K)."""
+    p, hs, _ = small(base="c2", n=15, size=56e3, mnum=3)
+    p.porosity_on = 0
+    sil = hs.markt != configs.AIR
+    hs.markt[sil] = configs.SILICATE
+    hs.markk[sil] = np.float32(300.0)
+    hs.markpor[sil] = np.float32(phi)
+    ronu(orc, p, hs)
+    pure = pure_nodes(hs, configs.SILICATE)
+    f = float(np.float32(phi))
+    a, b = 3.0 * math.exp(-f / 0.08), 3.0 * math.exp(-1.2 - f / 0.167)
+    want = a if f < 0.2 else (b if f > 0.4 else (a ** 4 + b ** 4) ** 0.25)
+    assert np.allclose(hs.kt[pure], want, rtol=1e-13)
+
+
+def henke_lim(mpb):
+    return 0.42 + 0.46 / ((mpb / 0.13) ** 1.72 + 1.0)
+
+
+@pytest.mark.parametrize("P,phi0", [(1e5, 0.5), (3e6, 0.45), (3e6, 0.30), (5e7, 0.60)])
+def test_henke_cold_pressing_below_700K(orc, P, phi0):
+    """Below 700 K porosity is capped by the cold-press limit
+    0.42 + 0.46/((P/0.13 bar)^1.72 + 1) and by 0.85."""
+    p, hs, _ = small(base="c2", n=15, size=56e3, mnum=3)
+    sil = hs.markt == configs.SILICATE
+    hs.markk[sil] = np.float32(500.0)
+    hs.markpor[sil] = np.float32(phi0)
+    hs.pr[:] = P
+    ronu(orc, p, hs)
+    want = np.float32(min(float(np.float32(phi0)), henke_lim(P * 1e-5), 0.85))
+    assert np.all(np.abs(hs.markpor[sil] - want) <= 1e-7)
+
+
+def test_henke_sintering_matches_an_independent_root_solve(orc):
+    """At and above 700 K the grain size solves Henke et al.'s relation
+    g^3 + g0^3 - 3 g^2 g0 q^(1/3) + g0^3 q = 0, q = (1 - phi_orig)/(1 - phi),
+    for the root above g0; the porosity then drops by
+    a (1 - phi) s^1.5 / (100 g)^3 exp(-E/(R T)) dt (times 10 phi below 0.1),
+    s = 2 sqrt(3)(g^2 - c^2) P / (pi c^2), c^2 = g^2 - g0^2 q^(2/3).  The
+    root comes from numpy's companion-matrix solve, not Newton's method."""
+    p, hs, _ = small(base="c2", n=15, size=56e3, mnum=3)
+    sil = hs.markt == configs.SILICATE
+    T, P, dt = 750.0, 2e7, 7.9e10
+    hs.timestep = dt
+    hs.markk[sil] = np.float32(T)
+    phis = np.linspace(0.05, 0.38, sil.sum()).astype(np.float32)
+    hs.markpor[sil] = phis
+    hs.pr[:] = P
+    ronu(orc, p, hs)
+    got = hs.markpor[sil]
+    mpb = P * 1e-5
+    phi_orig = min(henke_lim(mpb), p.por_init, 0.85)
+    g0, a, E, Rg = 1e-6, 4e-5, 85000.0, 1.9872
+    checked = 0
+    for phi, out in zip(phis[::7].astype(np.float64), got[::7]):
+        q = (1 - phi_orig) / (1 - phi)
+        c3 = q ** (1.0 / 3.0)
+        roots = np.roots([1.0, -3 * g0 * c3, 0.0, g0 ** 3 * (1 + q)])
+        real = sorted(r.real for r in roots if abs(r.imag) < 1e-9 * abs(r) and r.real > g0)
+        # for q < 1 the cubic has two roots above g0; the grain size is the
+        # nearer one (the other leaves the porosity almost unchanged here)
+        g = real[0]
+        c2 = g * g - g0 * g0 * q ** (2.0 / 3.0)
+        s = 2 * math.sqrt(3) * (g * g - c2) * mpb / (math.pi * c2)
+        d = a * (1 - phi) * s ** 1.5 / (100 * g) ** 3 * math.exp(-E / (Rg * T)) * dt
+        if phi <= 0.1:
+            d *= 10 * phi
+        want = phi - d if d > 0 else phi
+        assert out == pytest.approx(want, rel=2e-6, abs=1e-7), phi
+        assert abs(want - phi) > 1e-3
+        checked += 1
+    assert checked >= 20
+
+
+# ------------------------------------------------------------------ shear heating
+def test_shear_heating_source_for_uniform_fields(orc):
+    """sol1 = H/(rho cp) + (2 exy sxy + exx sxx + eyy syy)/(rho cp) once
+    timesum > frictyn and rho > 1e3; without either, H/(rho cp) alone."""
+    p, hs, _ = small(n=15, size=56e3)
+    exy, sxy, exx, sxx, eyy, syy = 1e-14, 2e6, -3e-14, 5e6, 2e-14, -4e6
+    for name, v in zip(("exy", "sxy", "exx", "sxx", "eyy", "syy"), (exy, sxy, exx, sxx, eyy, syy)):
+        getattr(hs, name)[:] = v
+    H, ro, cp = 3e-4, 3200.0, 1100.0
+    hs.ht[:], hs.ro[:], hs.cp[:], hs.kt[:] = H, ro, cp, 3.0
+    hs.tk[:] = 500.0
+    hs.timestep = 1e10
+    N = hs.N
+    interior = hs.bondm[3 * N:] == 0
+
+    def sol1_for(timesum, rho):
+        hs.timesum = timesum
+        hs.ro[:] = rho
+        s1 = np.zeros(N)
+        t = C.c_double()
+        assert orc.lib().orc_heat_prepare(C.byref(p), C.byref(hs.cstruct()), s1.ctypes.data_as(dp), C.byref(t)) == 0
+        return s1[interior]
+
+    shear = 2 * exy * sxy + exx * sxx + eyy * syy
+    assert np.allclose(sol1_for(10.0, ro), (H + shear) / (ro * cp), rtol=1e-13)
+    assert np.allclose(sol1_for(0.5, ro), H / (ro * cp), rtol=1e-14)    # timesum <= frictyn
+    assert np.allclose(sol1_for(10.0, 900.0), H / (900.0 * cp), rtol=1e-14)  # rho <= 1e3
+
+
+# ------------------------------------------------------------------ Duretz stabilisation
+def test_stabilisation_rows_on_a_linear_density_field(orc):
+    """rho = a + b x + c y and gp = g0 x + g1 y on a uniform grid: the rows'
+    dt-dependent part (A(dt) - A(0)) is -(dt/2) gX (b vx + c mean(vy)) in the
+    x rows, gX = -g0, and -(dt/2) gY (c vy + b mean(vx)) in the y rows,
+    gY = -g1, each mean over the four neighbouring values."""
+    p, hs, _ = small(n=15, size=56e3)
+    X, Y = hs.xnumx, hs.ynumy
+    b, c, g0, g1 = 2.0e-3, -3.0e-3, 0.25, -0.4
+    gx, gy = np.repeat(hs.gx, Y), np.tile(hs.gy, X)
+    hs.ro[:] = 3000.0 + b * gx + c * gy
+    xm = np.r_[0.0, (hs.gx[:-1] + hs.gx[1:]) / 2]
+    ym = np.r_[0.0, (hs.gy[:-1] + hs.gy[1:]) / 2]
+    hs.gp[:] = (g0 * xm[:, None] + g1 * ym[None, :]).ravel()
+    hs.nu[:] = hs.nd[:] = 1e20
+    o = orc.Oracle(p, hs)
+    hs.timestep = 0.0
+    A0 = o.stokes_matrix().scipy().tocsr()
+    dt = 1e13
+    hs.timestep = dt
+    A1 = o.stokes_matrix().scipy().tocsr()
+    D = (A1 - A0).tocsr()
+    bm = hs.bondm
+    nx = ny = 0
+    for i in range(2, X - 2):
+        for j in range(2, Y - 2):
+            m = i * Y + j
+            if not bm[3 * m + 1]:
+                r = 3 * m + 1
+                assert D[r, r] == pytest.approx(-dt / 2 * (-g0) * b, rel=1e-9)
+                for mv in (m, m + 1, m - Y, m + 1 - Y):
+                    assert D[r, 3 * mv + 2] == pytest.approx(-dt / 2 * (-g0) * c / 4, rel=1e-9)
+                assert D[r].nnz == 5
+                nx += 1
+            if not bm[3 * m + 2]:
+                r = 3 * m + 2
+                assert D[r, r] == pytest.approx(-dt / 2 * (-g1) * c, rel=1e-9)
+                for mv in (m, m + Y, m - 1, m + Y - 1):
+                    assert D[r, 3 * mv + 1] == pytest.approx(-dt / 2 * (-g1) * b / 4, rel=1e-9)
+                assert D[r].nnz == 5
+                ny += 1
+    assert nx > 50 and ny > 50
