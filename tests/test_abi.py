@@ -61,3 +61,21 @@ def test_create_without_device_fails_loudly():
         return
     ctx.close()
     pytest.skip("a GPU is visible")
+
+
+def test_dropin_source_defines_every_function_its_header_declares(tmp_path):
+    """include/i2e_dropin.h declares the reference-named step functions
+    (headmart.c:321-366); csrc/dropin.c, compiled on its own, defines each of
+    them (the globals it binds stay undefined until the caller links them)."""
+    hdr = open(os.path.join(ROOT, "include", "i2e_dropin.h")).read()
+    names = sorted(set(re.findall(r"^\s*void\s+\*?\s*([a-z_0-9]+)\s*\(", hdr, re.M)))
+    assert {"maxvelstep", "viterate", "titerate", "movemark", "ronurecalc"} <= set(names)
+    obj = tmp_path / "dropin.o"
+    subprocess.check_call(["gcc", "-c", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-o",
+                           str(obj), os.path.join(ROOT, "i2elvis_planet_amd", "csrc", "dropin.c")])
+    syms = subprocess.check_output(["nm", str(obj)], text=True).split("\n")
+    defined = {l.split()[-1] for l in syms if len(l.split()) == 3 and l.split()[1] == "T"}
+    undefined = {l.split()[-1] for l in syms if len(l.split()) == 2 and l.split()[0] == "U"}
+    for n in names:
+        assert n in defined, n
+    assert {"vx", "markx", "bondm", "timestep", "marknum"} <= undefined
