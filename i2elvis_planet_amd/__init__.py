@@ -28,7 +28,7 @@ _EXPORTS = [
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
     "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
-    "i2e_type_counts",
+    "i2e_type_counts", "i2e_maxvelstep_noalloc",
 ]
 
 _lib = None
@@ -95,6 +95,8 @@ def load_library(path=LIB_PATH):
         "i2e_adjoint_peak_t": ([vp, C.POINTER(abi.AdjointOut)], C.c_int),
         "i2e_heat_apply_t_dev": ([vp, vp, vp], C.c_int),
         "i2e_type_counts": ([vp, C.c_int, C.POINTER(C.c_int32)], C.c_int),
+        "i2e_maxvelstep_noalloc": ([vp, vp, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, vp, vp],
+                                   C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

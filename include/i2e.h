@@ -363,6 +363,22 @@ int i2e_prn_write(const char *path, const i2e_params *p, const i2e_host_state *s
 int i2e_prn_read_header(const char *path, i2e_prn_header *h);
 int i2e_prn_read(const char *path, i2e_params *p, i2e_host_state *s, int *n0);
 
+/* maxvelstep (movemart.c:1225-1262) on caller-owned device arrays, for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops.  No counterpart in the reference.  vx, vy: N = xnumx
+ * * ynumy FP64 node fields (node m = m1*ynumy + m2); *timestep_dev (one FP64
+ * on the device) holds the current time step and receives
+ * min(dt, maxxystep*dx/max|vx|, maxxystep*dy/max|vy|), dx = xsize/(xnumx-1),
+ * dy = ysize/(ynumy-1), bit-identical to i2e_maxvelstep.  Launches three
+ * kernels on `stream` (a hipStream_t; NULL = the null stream) and returns
+ * without waiting; allocates nothing and calls no hipMalloc/hipFree/memset.
+ * Its scratch (the two maxima, 32 bytes) is a device array of the library,
+ * one per GPU, kept until the process ends: calls that overlap on one GPU
+ * race on it and must be ordered by the caller (one stream, or events).
+ * Grids above 16385 per direction or 2^28 nodes: I2E_ERR_CAPACITY.       */
+int i2e_maxvelstep_noalloc(const double *vx_dev, const double *vy_dev, int64_t xnumx, int64_t ynumy,
+                           double xsize, double ysize, double maxxystep, double *timestep_dev, void *stream);
+
 /* Device buffers for callers without a device allocator of their own (no
  * counterpart in the reference): allocation, release and blocking copies,
  * all through the same HIP runtime as the library itself.                   */
