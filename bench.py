@@ -12,6 +12,7 @@ mixed/FP64 ratio.  Both solve Stokes to ||b - Ax|| / ||b|| <= 1e-8.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,12 +28,16 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILES = {
     "vsmoothc": os.path.join(ROOT, "profiles", "k_tsmooth_f32_501x501_pmc_traffic.md"),
     "gsdot": os.path.join(ROOT, "profiles", "k_gsdot2_501x501_pmc_traffic.md"),
+    "p2g": os.path.join(ROOT, "profiles", "k_ronu_markers_501x501_pmc_traffic.md"),
+    "advect": os.path.join(ROOT, "profiles", "k_movemark_501x501_pmc_traffic.md"),
 }
 # rocprof kernel-name prefix and, where only some launches count, the grid
 # size (threads) of the launches bench.py times for each roofline kernel
 PMC_MATCH = {
     "vsmoothc": ("void i2e::k_tsmooth<float, float, 2, false, false>(", None),
     "gsdot": ("i2e::k_gsdot2(", "stokes"),
+    "p2g": ("i2e::k_ronu_markers(", None),
+    "advect": ("i2e::k_movemark(", None),
 }
 
 
@@ -40,24 +45,79 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sample_steps=1):
-    """One full step of the CPU oracle on the same 501x501 / 6.25M-marker
-    input (serial C marker loops and node assembly, PARDISO direct solves
-    as the reference factorises every step), timed on this host's cores."""
+def physical_cores():
+    """Physical cores of this host from lscpu (sockets x cores per socket)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        f = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
+        return int(f["Socket(s)"].strip()) * int(f["Core(s) per socket"].strip())
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def oracle_step_phases(orc):
+    """One oracle step (i2mart.c:60-153 order) with wall-clock seconds per
+    phase: gravity, Stokes assembly + direct solve, heat, advection,
+    marker-to-node."""
+    hs, p = orc.hs, orc.p
+    t = {}
+    c = time.perf_counter
+    hs.timestep = p.maxtmstep
+    orc.maxvelstep()
+    a = c()
+    orc.gravity()
+    t["gravity"] = c() - a
+    a = c()
+    M = orc.stokes_matrix()
+    x = orc.solve_matrix(M)
+    orc.stokes_post(x)
+    t["stokes"] = c() - a
+    hs.timestep = p.maxtmstep
+    orc.maxvelstep()
+    a = c()
+    if hs.timestep:
+        orc.titerate()
+    t["heat"] = c() - a
+    a = c()
+    if hs.timestep:
+        orc.movemark()
+    t["advection"] = c() - a
+    a = c()
+    orc.ronurecalc()
+    t["marker_to_node"] = c() - a
+    hs.timesum += hs.timestep
+    return t
+
+
+def cpu_baseline(name="c3"):
+    """The CPU oracle (serial C marker loops and node assembly, PARDISO
+    direct solves factorised every step as the reference does) on the same
+    501x501 / 6.25M-marker input: one full step with the sparse solves on
+    all physical cores, one on 2 and one on 1 thread (the reference's
+    production settings), each from the same state, split by phase."""
     from i2elvis_planet_amd import configs
     from oracle import oracle
-    p, hs, _ = configs.make("c3")
-    orc = oracle.Oracle(p, hs)
-    orc.ronurecalc()
-    t = time.time()
-    for _ in range(sample_steps):
-        orc.step()
-    dt = time.time() - t
-    threads = int(os.environ.get("MKL_NUM_THREADS", os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    return {"value": sample_steps / dt, "unit": "timesteps/s", "cores": threads, "kind": "port",
-            "sample": "%d full step(s) of the 501x501 / 6.25M-marker setup after its initial ronurecalc "
-                      "(%.1f s): marker loops and assembly serial C, sparse solves %s with %d threads"
-                      % (sample_steps, dt, oracle.solver_kind(), threads)}
+    p, hs, _ = configs.make(name)
+    oracle.Oracle(p, hs).ronurecalc()
+    cores = physical_cores()
+    runs = {}
+    for nt in (cores, 2, 1):
+        if nt in runs:
+            continue
+        oracle.set_solver_threads(nt)
+        orc = oracle.Oracle(p, hs.copy())
+        a = time.perf_counter()
+        ph = oracle_step_phases(orc)
+        runs[nt] = (time.perf_counter() - a, ph)
+        log("cpu oracle step, %d thread(s): %.1f s %s" % (nt, runs[nt][0], {k: round(v, 2) for k, v in ph.items()}))
+    dt, ph = runs[cores]
+    return {"value": 1.0 / dt, "unit": "timesteps/s", "cores": cores, "kind": "port",
+            "sample": "1 full step of the 501x501 / 6.25M-marker setup after its initial ronurecalc: marker loops "
+                      "and assembly serial C, sparse solves %s with %d threads (%.1f s)"
+                      % (oracle.solver_kind(), cores, dt),
+            "s_per_step_by_threads": {str(k): round(v[0], 3) for k, v in sorted(runs.items())},
+            "phase_s": {k: round(v, 3) for k, v in ph.items()}}
 
 
 # algorithmic-byte roofline kernels (HBM bound), selectable with --roofline-kernel
@@ -70,7 +130,8 @@ ROOFLINE_KERNELS = {
                 "difference-form stencil, mixed mode)",
     "stokes_apply": "k_stokes_apply (matrix-free Stokes operator)",
     "advect": "k_movemark (RK4 marker advection)",
-    "p2g": "k_ronu_markers (marker-to-node scatter)",
+    "p2g": "k_ronu_markers + k_ronu_nodes (marker-to-node scatter and normalisation)",
+    "heatmark": "k_heat_markers_a + k_heat_nodes_norm + k_heat_markers_b (marker temperature update)",
 }
 
 
@@ -133,11 +194,133 @@ def pmc_summary(fetch_dir, write_dir, rk="vsmoothc", n_nodes=501 * 501, copy_byt
     return "\n".join(lines)
 
 
-def run(steps, warmup, skip_cpu, rk):
+KERNEL_TABLE = ("advect", "p2g", "heatmark", "stokes_apply", "gsdot", "gsaxpy", "vsmooth0", "vsmoothc")
+
+
+def timed_steps(ctx, o, snap, mode, steps, timer=None):
+    """Upload snap, run `steps` steps in `mode` (mixed / fp64); returns
+    (seconds, per-step stats, timer reading or None)."""
+    o.precision = 1 if mode == "mixed" else 0
+    ctx.set_solver(o)
+    ctx.upload(snap)
+    ctx.sync()
+    if timer:
+        ctx.kernel_timer(timer[0], 1 << 17, timer[1])
+    iters = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.step()
+        st = ctx.stats()
+        iters.append((st.stokes_iters, st.stokes_relres, st.grav_iters, st.heat_iters, st.ms_stokes, st.ms_heat,
+                      st.ms_grav, st.ms_markers))
+    ctx.sync()
+    el = time.perf_counter() - t0
+    reading = None
+    if timer:
+        reading = ctx.kernel_timer_read()
+        ctx.kernel_timer(None, 0)
+    return el, iters, reading
+
+
+def kernel_table(ctx, o, snap):
+    """Every kernel unit with an algorithmic byte count, timed on each launch
+    with HIP events over one mixed step (outside the timed region; an event
+    pair adds ~2.5 us to each timed launch)."""
+    out = {}
+    for k in KERNEL_TABLE:
+        _, _, (n, ms, alg, rep) = timed_steps(ctx, o, snap, "mixed", 1, (k, 1))
+        if not n:
+            continue
+        gbs = alg / (ms * 1e-3) / 1e9
+        out[k] = {"launches": n, "avg_us": round(ms * 1e3 / n, 2), "alg_bytes_per_launch": round(alg / n),
+                  "alg_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                  "repr_bytes_per_launch": round(rep / n), "repr_GBs": round(rep / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
+def hot_attempt(ctx, o, hs_cold, cfg):
+    """The molten regime (core at 1800 K, melt above 0.4 at 1e2 Pa s against
+    the 1e23 clip): one Stokes solve attempted per precision, capped at 300
+    Krylov iterations; reports what the solver reached."""
+    from i2elvis_planet_amd import I2EError, configs
+    hot = configs.heat_state(hs_cold.copy(), cfg, t_core=1800.0)
+    out = {}
+    for mode in ("mixed", "fp64"):
+        o.precision = 1 if mode == "mixed" else 0
+        o2 = type(o).from_buffer_copy(o)
+        o2.maxit = 300
+        ctx.set_solver(o2)
+        ctx.upload(hot)
+        ctx.ronurecalc()
+        st0 = ctx.stats()
+        ctx.set_scalars(hot.timestep or 7.8894e10, hot.timesum)
+        ctx.maxvelstep()
+        t0 = time.perf_counter()
+        try:
+            ctx.viterate()
+            conv = True
+        except I2EError:
+            conv = False
+        ctx.sync()
+        st = ctx.stats()
+        rr = st.stokes_relres if math.isfinite(st.stokes_relres) else None  # NaN: a non-finite iterate
+        out[mode] = {"converged": conv, "stokes_iters": st.stokes_iters, "stokes_relres": rr,
+                     "ms": round((time.perf_counter() - t0) * 1e3, 1), "eta_min": st0.numin, "eta_max": st0.numax}
+        log("hot %s: converged %s, %d its, relres %r" % (mode, conv, st.stokes_iters, rr))
+    ctx.set_solver(o)
+    return out
+
+
+def valu_summary(pmc_dir, rk):
+    """VALU activity of roofline kernel `rk` from one rocprofv3 --pmc run of
+    SQ_ACTIVE_INST_VALU SQ_INSTS_VALU SQ_WAVE_CYCLES GRBM_GUI_ACTIVE: VALU
+    busy = SQ_ACTIVE_INST_VALU * 4 / (SIMDs) / GRBM_GUI_ACTIVE (4 SIMDs per CU,
+    256 CUs), per dispatch, averaged."""
+    import csv
+    prefix, _ = PMC_MATCH[rk]
+    rows = [r for r in csv.DictReader(open(os.path.join(pmc_dir, "run_counter_collection.csv")))
+            if r["Kernel_Name"].startswith(prefix)]
+    if not rows:
+        raise SystemExit("valu summary: no %s dispatches" % prefix)
+    per = {}
+    for r in rows:
+        per.setdefault(r["Dispatch_Id"], {})[r["Counter_Name"]] = float(r["Counter_Value"])
+    simds = 4 * 256
+    busy, ipw = [], []
+    for d in per.values():
+        if d.get("GRBM_GUI_ACTIVE") and "SQ_ACTIVE_INST_VALU" in d:
+            busy.append(100.0 * d["SQ_ACTIVE_INST_VALU"] * 4 / simds / d["GRBM_GUI_ACTIVE"])
+        if d.get("SQ_WAVE_CYCLES") and "SQ_INSTS_VALU" in d:
+            ipw.append(d["SQ_INSTS_VALU"] / d["SQ_WAVE_CYCLES"])
+    return "\n".join([
+        "%s: %d dispatches" % (prefix.rstrip("("), len(per)),
+        "VALU busy (SQ_ACTIVE_INST_VALU*4/SIMDs/GRBM_GUI_ACTIVE): %.1f %%" % (sum(busy) / max(len(busy), 1)),
+        "VALU instructions per wave cycle (SQ_INSTS_VALU/SQ_WAVE_CYCLES): %.3f" % (sum(ipw) / max(len(ipw), 1)),
+    ])
+
+
+def kernel_stats_table(stats_csv, top=30):
+    """Markdown table of a rocprofv3 --stats kernel summary (run_kernel_stats.csv)."""
+    import csv
+    rows = list(csv.DictReader(open(stats_csv)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:top]:
+        name = r["Name"].replace("void ", "").replace("i2e::", "").split("(")[0]
+        out.append("| %s | %s | %.1f | %.1f | %.1f | %.1f | %.2f |" % (
+            name, r["Calls"], float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3,
+            float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, 100.0 * float(r["TotalDurationNs"]) / tot))
+    out.append("")
+    out.append("All kernels: %.1f ms over %d kernel names." % (tot / 1e6, len(rows)))
+    return "\n".join(out)
+
+
+def run(steps, warmup, skip_cpu, rk, minimal=False):
     import numpy as np
     from i2elvis_planet_amd import Context, configs, default_solver
 
     p, hs, cfg = configs.make("c3")
+    cold = hs.copy()
     log("setup generated: %d markers" % hs.marknum)
     o = default_solver()
     o.precision = 1
@@ -151,32 +334,34 @@ def run(steps, warmup, skip_cpu, rk):
     res = {}
     roof = None
     for mode in ("mixed", "fp64"):
-        o.precision = 1 if mode == "mixed" else 0
-        ctx.set_solver(o)
-        ctx.upload(snap)
-        ctx.sync()
-        if mode == "mixed":
-            ctx.kernel_timer(rk, 1 << 17, ROOFLINE_EVERY.get(rk, 1))
-        iters = []
-        t0 = time.perf_counter()
-        for k in range(steps):
-            ctx.step()
-            st = ctx.stats()
-            iters.append((st.stokes_iters, st.stokes_relres, st.grav_iters, st.heat_iters, st.ms_stokes, st.ms_heat,
-                          st.ms_grav, st.ms_markers))
-        ctx.sync()
-        el = time.perf_counter() - t0
-        if mode == "mixed":
-            roof = ctx.kernel_timer_read()
-            ctx.kernel_timer(None, 0)
+        timer = (rk, ROOFLINE_EVERY.get(rk, 1)) if mode == "mixed" else None
+        el, iters, reading = timed_steps(ctx, o, snap, mode, steps, timer)
+        if reading:
+            roof = reading
         res[mode] = (steps / el, el * 1e3 / steps, iters)
         log("%s: %.3f steps/s, %.1f ms/step; stokes its %s" % (mode, steps / el, el * 1e3 / steps,
                                                                 [i[0] for i in iters]))
-    # standalone kernel timings (HIP events, outside the timed region)
+    # outside the timed region: per-kernel figures, the pre-melt regime with
+    # a hot interior, the molten regime, standalone kernel timings
+    kt, warm_res, hot = {}, None, None
+    if not minimal:
+        kt = kernel_table(ctx, o, snap)
+        warm = configs.heat_state(cold.copy(), cfg, t_core=1340.0)
+        ctx.upload(warm)
+        ctx.ronurecalc()
+        ctx.sync()
+        warm_snap = ctx.download(warm.copy())
+        warm_res = {}
+        for mode in ("mixed", "fp64"):
+            el, iters, _ = timed_steps(ctx, o, warm_snap, mode, steps)
+            warm_res[mode] = (steps / el, [i[0] for i in iters], [i[1] for i in iters])
+            log("warm %s: %.3f steps/s; stokes its %s" % (mode, steps / el, warm_res[mode][1]))
+        hot = hot_attempt(ctx, o, cold, cfg)
+    ctx.upload(snap)
     ms_adv, ms_p2g, ms_sap = ctx.time_kernels(5)
     mixed_sps, mixed_ms, it = res["mixed"]
     fp64_sps = res["fp64"][0]
-    launches, ms_tot, bytes_tot = roof
+    launches, ms_tot, bytes_tot, repr_tot = roof
     avg_ms = ms_tot / max(launches, 1)
     achieved = bytes_tot / max(ms_tot, 1e-30) * 1e3 / 1e9
     line = {
@@ -205,13 +390,22 @@ def run(steps, warmup, skip_cpu, rk):
         "phase_ms_mixed": {"stokes": float(np.mean([i[4] for i in it])), "heat": float(np.mean([i[5] for i in it])),
                            "gravity": float(np.mean([i[6] for i in it])),
                            "markers": float(np.mean([i[7] for i in it]))},
+        "warm_core_1340K": None if warm_res is None else {
+            "mixed": warm_res["mixed"][0], "fp64": warm_res["fp64"][0],
+            "mixed_over_fp64": warm_res["mixed"][0] / warm_res["fp64"][0],
+            "stokes_iters_mixed": warm_res["mixed"][1], "stokes_iters_fp64": warm_res["fp64"][1],
+            "stokes_relres_mixed": warm_res["mixed"][2]},
+        "hot_core_1800K": hot,
+        "kernels": kt,
         "kernels_standalone_ms": {"advect_rk4": round(ms_adv, 4), "p2g_scatter": round(ms_p2g, 4),
                                   "stokes_apply_fp64": round(ms_sap, 4)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(rk),
                      "kernel": ROOFLINE_KERNELS[rk], "sampled_every": ROOFLINE_EVERY.get(rk, 1),
                      "launches": launches, "avg_us": avg_ms * 1e3,
-                     "bytes_per_launch": bytes_tot / max(launches, 1)},
+                     "bytes_per_launch": bytes_tot / max(launches, 1),
+                     "repr_bytes_per_launch": repr_tot / max(launches, 1),
+                     "repr_GBs": repr_tot / max(ms_tot, 1e-30) * 1e3 / 1e9},
         "cpu_baseline": None,
     }
     ctx.close()
@@ -265,6 +459,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--minimal", action="store_true",
+                    help="only the timed steps and the roofline kernel (for counter runs)")
     ap.add_argument("--roofline-kernel", default="vsmoothc", choices=sorted(ROOFLINE_KERNELS),
                     help="kernel of the roofline entry (default: the one with the most GPU time)")
     ap.add_argument("--ensemble", type=int, default=0, help="run the ensemble measurement with this many bodies")
@@ -272,9 +468,19 @@ def main():
                     help="first copy MIB MiB device-to-device three times (PMC calibration)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
+    ap.add_argument("--valu-summary", metavar="PMC_DIR",
+                    help="summarise a rocprofv3 --pmc run of the VALU counters for --roofline-kernel and exit")
+    ap.add_argument("--stats-table", metavar="KERNEL_STATS_CSV",
+                    help="print a rocprofv3 --stats kernel summary as a markdown table and exit")
     a = ap.parse_args()
     if a.pmc_summary:
         print(pmc_summary(*a.pmc_summary, rk=a.roofline_kernel, copy_bytes=a.copy_calibration * 2 ** 20 or 2 ** 30))
+        return
+    if a.valu_summary:
+        print(valu_summary(a.valu_summary, a.roofline_kernel))
+        return
+    if a.stats_table:
+        print(kernel_stats_table(a.stats_table))
         return
     if a.copy_calibration:
         from i2elvis_planet_amd import DeviceArray
@@ -292,7 +498,7 @@ def main():
     if a.ensemble:
         print(json.dumps(ensemble(a.ensemble, a.steps, a.warmup)), flush=True)
         return
-    line = run(a.steps, a.warmup, a.no_cpu, a.roofline_kernel)
+    line = run(a.steps, a.warmup, a.no_cpu, a.roofline_kernel, a.minimal)
     print(json.dumps(line), flush=True)
 
 

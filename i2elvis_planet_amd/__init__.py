@@ -28,7 +28,8 @@ _EXPORTS = [
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
     "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
-    "i2e_type_counts", "i2e_maxvelstep_noalloc",
+    "i2e_type_counts", "i2e_maxvelstep_noalloc", "i2e_kernel_timer_repr", "i2e_set_marker_sort",
+    "i2e_sort_markers",
 ]
 
 _lib = None
@@ -80,6 +81,9 @@ def load_library(path=LIB_PATH):
         "i2e_stokes_residuals": ([vp, C.POINTER(abi.Resid)], C.c_int),
         "i2e_step_batch": ([C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "i2e_kernel_timer_read": ([vp, C.POINTER(C.c_int64), dp, dp], C.c_int),
+        "i2e_kernel_timer_repr": ([vp, dp], C.c_int),
+        "i2e_set_marker_sort": ([vp, C.c_int], C.c_int),
+        "i2e_sort_markers": ([vp], C.c_int),
         "i2e_dev_alloc": ([C.c_size_t, C.POINTER(vp)], C.c_int),
         "i2e_dev_free": ([vp], C.c_int),
         "i2e_copy_to_dev": ([vp, vp, C.c_size_t], C.c_int),
@@ -268,10 +272,12 @@ class Context:
         _chk(_lib.i2e_kernel_timer(self._h, (kernel or "").encode(), max_launches, every), "kernel_timer")
 
     def kernel_timer_read(self):
-        """(launches, total ms, total algorithmic bytes) since kernel_timer()."""
-        n, ms, b = C.c_int64(), C.c_double(), C.c_double()
+        """(launches, total ms, total algorithmic bytes, total representation
+        bytes) since kernel_timer()."""
+        n, ms, b, r = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
         _chk(_lib.i2e_kernel_timer_read(self._h, C.byref(n), C.byref(ms), C.byref(b)), "kernel_timer_read")
-        return n.value, ms.value, b.value
+        _chk(_lib.i2e_kernel_timer_repr(self._h, C.byref(r)), "kernel_timer_repr")
+        return n.value, ms.value, b.value, r.value
 
     def set_params(self, params):
         _chk(_lib.i2e_set_params(self._h, C.byref(params)), "set_params")
@@ -297,6 +303,14 @@ class Context:
         o = abi.AdjointOut()
         _chk(_lib.i2e_adjoint_peak_t(self._h, C.byref(o)), "adjoint_peak_t")
         return {k: getattr(o, k) for k, _ in abi.AdjointOut._fields_}
+
+    def set_marker_sort(self, every):
+        """Sort the markers by cell every `every` steps of step() (0: never)."""
+        _chk(_lib.i2e_set_marker_sort(self._h, int(every)), "set_marker_sort")
+
+    def sort_markers(self):
+        """Group the device markers by cell now (download keeps the caller's order)."""
+        _chk(_lib.i2e_sort_markers(self._h), "sort_markers")
 
     def type_counts(self, ntypes=64):
         """Per-cell marker-type histogram of the device markers, shape (cells, ntypes)."""

@@ -36,8 +36,24 @@ void set_error(const char *fmt, ...);
 
 // Per-launch HIP-event timing of one kernel class on its launch stream, for
 // roofline figures measured inside a normal run.  Host-side only: launch
-// sites bracket the launch with begin/end and state its algorithmic bytes.
-enum KId { K_ADVECT = 0, K_P2G, K_GSDOT, K_GSAXPY, K_VSMOOTH0, K_VSMOOTHC, K_STOKES_APPLY, K_PRECOND, K_COUNT };
+// sites bracket the launch (or a group of launches forming one unit of work)
+// with begin/end and state two byte counts: the algorithmic bytes of the
+// unit (each distinct element of the data it must touch counted once: 4 B
+// per float32 marker field, 1 B per type, 8 B per FP64 node value) and the
+// representation bytes of this implementation (e.g. the explicit Galerkin
+// stencils a smoothing pass reads in place of matrix-free rows).
+enum KId {
+  K_ADVECT = 0,
+  K_P2G,
+  K_GSDOT,
+  K_GSAXPY,
+  K_VSMOOTH0,
+  K_VSMOOTHC,
+  K_STOKES_APPLY,
+  K_PRECOND,
+  K_HEATMARK,
+  K_COUNT
+};
 
 // Every `every`-th launch of the target is bracketed (a sample: the event
 // pairs themselves add latency to the stream they are recorded on).
@@ -46,20 +62,21 @@ struct KTimer {
   std::vector<hipEvent_t> ev;
   size_t used = 0;
   int64_t launches = 0, seen = 0;
-  double bytes = 0;
+  double bytes = 0, repr = 0;
   bool open = false;
   void begin(int id, hipStream_t s) {
     if (id != target || used + 2 > ev.size()) return;
     open = (seen++ % every) == 0;
     if (open) (void)hipEventRecord(ev[used], s);
   }
-  void end(int id, hipStream_t s, double b) {
+  void end(int id, hipStream_t s, double alg, double rep) {
     if (id != target || !open) return;
     open = false;
     (void)hipEventRecord(ev[used + 1], s);
     used += 2;
     launches++;
-    bytes += b;
+    bytes += alg;
+    repr += rep;
   }
 };
 

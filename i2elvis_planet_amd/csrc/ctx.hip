@@ -50,6 +50,11 @@ __global__ void k_fill(int n, double v, double *a) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) a[k] = v;
 }
 
+// the SoA view of a marker storage of capacity MM (7 float fields + type)
+static Markers marker_view(float *mf, int8_t *t, size_t MM) {
+  return Markers{mf, mf + MM, mf + 2 * MM, mf + 3 * MM, mf + 4 * MM, mf + 5 * MM, mf + 6 * MM, t};
+}
+
 // Stokes preconditioner pieces (lower block-triangular):
 //   z_v = MG(r_v);  z_p = -sc * eta_cell / mu * (r_p - mu * div z_v)  on continuity rows,
 //   z_p = r_p / mu + sum k z_p(nbr) on P boundary rows, r_p / mu on ghost rows
@@ -544,7 +549,7 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
   for (int k = 0; k < 19; k++) *f[k] = n0 + (size_t)k * N;
   float *mf = c->mfl.p;
   size_t MM = (size_t)max_markers;
-  c->mk = Markers{mf, mf + MM, mf + 2 * MM, mf + 3 * MM, mf + 4 * MM, mf + 5 * MM, mf + 6 * MM, c->mt.p};
+  c->mk = marker_view(mf, c->mt.p, MM);
   double *a0 = c->acc.p;
   c->A = Acc{a0, a0 + N, a0 + 2 * (size_t)N, a0 + 3 * (size_t)N, a0 + 4 * (size_t)N, a0 + 5 * (size_t)N,
              a0 + 6 * (size_t)N, a0 + 7 * (size_t)N, a0 + 8 * (size_t)N, a0 + 9 * (size_t)N, a0 + 10 * (size_t)N,
@@ -581,6 +586,10 @@ int i2e_destroy(i2e_ctx *c) {
                        &c->bcix, &c->bciy, &c->bLS, &c->xprev};
   c->blmask.release();
   c->tcnt.release();
+  c->mfl2.release();
+  c->mt2.release();
+  DBuf<int> *ib[] = {&c->morig, &c->morig2, &c->mkey, &c->mcnt, &c->moff, &c->mbsum};
+  for (auto *b : ib) b->release();
   for (auto *b : d) b->release();
   c->mfl.release();
   c->mt.release();
@@ -704,6 +713,7 @@ void *i2e_device_ptr(i2e_ctx *c, const char *name) {
   for (int k = 0; k < 7; k++)
     if (!strcmp(name, mn[k])) return c->mfl.p + (size_t)k * c->Mmax;
   if (!strcmp(name, "markt")) return c->mt.p;
+  if (!strcmp(name, "markorig")) return c->permuted ? c->morig.p : nullptr;
   if (!strcmp(name, "gx")) return c->gx.p;
   if (!strcmp(name, "gy")) return c->gy.p;
   return nullptr;
@@ -845,6 +855,8 @@ int i2e_upload(i2e_ctx *c, const i2e_host_state *s) {
     HIP_OK(hipMemcpy(c->nodes.p + (size_t)k * N, src[k], sizeof(double) * N, hipMemcpyHostToDevice));
   }
   c->M = s->marknum;
+  c->permuted = 0;  // slots in the caller's order again
+  c->since_sort = 0;
   const float *ms[7] = {s->markx, s->marky, s->markk, s->markv, s->markp, s->markpor, s->marktmax};
   for (int k = 0; k < 7; k++)
     if (c->M) HIP_OK(hipMemcpy(c->mfl.p + (size_t)k * c->Mmax, ms[k], sizeof(float) * c->M, hipMemcpyHostToDevice));
@@ -881,10 +893,22 @@ int i2e_download(i2e_ctx *c, i2e_host_state *s) {
   for (int k = 0; k < 19; k++)
     if (dst[k]) HIP_OK(hipMemcpy(dst[k], c->nodes.p + (size_t)k * N, sizeof(double) * N, hipMemcpyDeviceToHost));
   float *ms[7] = {s->markx, s->marky, s->markk, s->markv, s->markp, s->markpor, s->marktmax};
+  // markers in the caller's order: undo the cell sort into the spare storage
+  const float *mf = c->mfl.p;
+  const int8_t *mtp = c->mt.p;
+  if (c->permuted && c->M) {
+    unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+    hipLaunchKernelGGL(k_munperm, dim3(gb), dim3(256), 0, c->s, c->mk, marker_view(c->mfl2.p, c->mt2.p, c->Mmax),
+                       c->M, (const int *)c->morig.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->s));
+    mf = c->mfl2.p;
+    mtp = c->mt2.p;
+  }
   for (int k = 0; k < 7; k++)
     if (ms[k] && c->M)
-      HIP_OK(hipMemcpy(ms[k], c->mfl.p + (size_t)k * c->Mmax, sizeof(float) * c->M, hipMemcpyDeviceToHost));
-  if (s->markt && c->M) HIP_OK(hipMemcpy(s->markt, c->mt.p, c->M, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(ms[k], mf + (size_t)k * c->Mmax, sizeof(float) * c->M, hipMemcpyDeviceToHost));
+  if (s->markt && c->M) HIP_OK(hipMemcpy(s->markt, mtp, c->M, hipMemcpyDeviceToHost));
   s->marknum = c->M;
   s->timestep = c->timestep;
   s->timesum = c->timesum;
@@ -1081,13 +1105,14 @@ static int stokes_solve(i2e_ctx *c) {
   const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
   ST_OK(stokes_prepare(c));
-  // algorithmic bytes: x and y (3 FP64 unknowns per node), nu, nd, ro, gp
-  // (FP64 per node) and the 4 BC row numbers (int32) per node
-  const double bytes = (24.0 + 24.0 + 32.0 + 16.0) * (double)c->N;
+  // algorithmic bytes: read vx, vy, p, the two viscosities, rho and gp,
+  // write the three residual components, 8 B each per node; this
+  // implementation also reads the 4 BC row numbers (int32) per node
+  const double alg = 80.0 * (double)c->N, bytes = alg + 16.0 * (double)c->N;
   auto A = [&](const double *x, double *y) -> int {
     c->kt.begin(K_STOKES_APPLY, c->s);
     hipLaunchKernelGGL(k_stokes_apply, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, x, y);
-    c->kt.end(K_STOKES_APPLY, c->s, bytes);
+    c->kt.end(K_STOKES_APPLY, c->s, alg, bytes);
     return I2E_OK;
   };
   // each projection pass writes the velocity components of the pending
@@ -1102,7 +1127,7 @@ static int stokes_solve(i2e_ctx *c) {
     const bool presplit = r == c->fs.split_of;
     c->fs.split_of = nullptr;
     ST_OK(stokes_precond(c, r, z, presplit));
-    c->kt.end(K_PRECOND, c->s, 0.0);
+    c->kt.end(K_PRECOND, c->s, 0.0, 0.0);
     return I2E_OK;
   };
   int it = 0;
@@ -1135,6 +1160,28 @@ int i2e_viterate(i2e_ctx *c, int n0) {
   c->st.ms_stokes = std::chrono::duration<double, std::milli>(t2 - t1).count();
   return I2E_OK;
 }
+
+// Algorithmic bytes of the marker passes, each distinct element once:
+// advection reads x, y, type, T and writes x, y, eta (25 B per marker) and
+// reads vx, vy (16 B per node); this implementation also reads and writes P
+// at markers and gathers the cell pressure (+ the strain rates where a
+// power-law rheology needs them).
+static double advect_bytes(const i2e_ctx *c) { return 25.0 * (double)c->M + 16.0 * (double)c->N; }
+static double advect_repr(const i2e_ctx *c) {
+  return 29.0 * (double)c->M + (24.0 + 24.0 * c->need_eii) * (double)c->N;
+}
+// scatter + normalisation: reads x, y, type, T, porosity, eta and writes
+// porosity, Tmax, type (30 B per marker); writes the 8 node fields (64 B per
+// node).  Implemented: 38 B per marker, the 12 FP64 accumulators written
+// and read back, node P and T gathered, 8 fields written.
+static double p2g_bytes(const i2e_ctx *c) { return 30.0 * (double)c->M + 64.0 * (double)c->N; }
+static double p2g_repr(const i2e_ctx *c) {
+  return 38.0 * (double)c->M + (2.0 * 96.0 + 16.0 + 64.0 + 24.0 * c->need_eii) * (double)c->N;
+}
+// marker temperature update, both passes: x, y, type read twice and T
+// read and written (34 B per marker); node T before and after, k, rho, cp
+// read and the two subgrid sums written and read (56 B per node)
+static double heatmark_bytes(const i2e_ctx *c) { return 34.0 * (double)c->M + 56.0 * (double)c->N; }
 
 int i2e_titerate(i2e_ctx *c, int n0) {
   (void)n0;
@@ -1204,12 +1251,14 @@ int i2e_titerate(i2e_ctx *c, int n0) {
   HIP_OK(hipMemsetAsync(s1, 0, sizeof(double) * N, c->s));
   if (c->M) {
     unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+    c->kt.begin(K_HEATMARK, c->s);
     hipLaunchKernelGGL(k_heat_markers_a, dim3(gb), dim3(256), 0, c->s, c->dP, g, c->F, c->tk2.p, c->mk, c->M, dt, s0,
                        s1);
     if (c->P.heatdif) {
       hipLaunchKernelGGL(k_heat_nodes_norm, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, s0, s1);
       hipLaunchKernelGGL(k_heat_markers_b, dim3(gb), dim3(256), 0, c->s, g, c->mk, c->M, s1);
     }
+    c->kt.end(K_HEATMARK, c->s, heatmark_bytes(c), heatmark_bytes(c));
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c->s));
@@ -1222,15 +1271,7 @@ int i2e_titerate(i2e_ctx *c, int n0) {
 // or write once, every node field they gather once.
 // movemark: reads type (1 B), x, y, T (12 B); writes P, eta, x, y (16 B);
 // gathers vx, vy, P (+ exx, eyy, exy for strain-rate-dependent viscosity).
-static double advect_bytes(const i2e_ctx *c) {
-  return 29.0 * (double)c->M + (24.0 + 24.0 * c->need_eii) * (double)c->N;
-}
-// ronurecalc marker pass: reads x, y, T, eta, P, porosity, Tmax, type
-// (29 B), writes type, porosity, Tmax (9 B); 12 FP64 node accumulators
-// written once, node P, T (+ strain rates) gathered.
-static double p2g_bytes(const i2e_ctx *c) {
-  return 38.0 * (double)c->M + (96.0 + 16.0 + 24.0 * c->need_eii) * (double)c->N;
-}
+
 
 int i2e_movemark(i2e_ctx *c) {
   if (!c) return I2E_ERR_ARG;
@@ -1239,7 +1280,7 @@ int i2e_movemark(i2e_ctx *c) {
   c->kt.begin(K_ADVECT, c->s);
   hipLaunchKernelGGL(k_movemark, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->timestep,
                      c->need_eii);
-  c->kt.end(K_ADVECT, c->s, advect_bytes(c));
+  c->kt.end(K_ADVECT, c->s, advect_bytes(c), advect_repr(c));
   HIP_OK(hipGetLastError());
   return I2E_OK;
 }
@@ -1250,14 +1291,15 @@ int i2e_ronurecalc(i2e_ctx *c) {
   HIP_OK(hipMemsetAsync(c->acc.p, 0, sizeof(double) * 12 * N, c->s));
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->s, 1, 1e300, c->red.p + 16);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->s, 1, 0.0, c->red.p + 17);
+  c->kt.begin(K_P2G, c->s);
   if (c->M) {
     unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
-    c->kt.begin(K_P2G, c->s);
     hipLaunchKernelGGL(k_ronu_markers, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->A,
-                       c->timesum, c->timestep, c->need_eii);
-    c->kt.end(K_P2G, c->s, p2g_bytes(c));
+                       heat_per_kg(c->P, c->timesum), c->timestep, c->need_eii);
   }
-  hipLaunchKernelGGL(k_ronu_nodes, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->dP, c->F, c->A, N, c->red.p + 16);
+  hipLaunchKernelGGL(k_ronu_nodes, dim3(std::min(nblk(N, 256), 256u)), dim3(256), 0, c->s, c->dP, c->F, c->A, N,
+                     c->red.p + 16);
+  c->kt.end(K_P2G, c->s, p2g_bytes(c), p2g_repr(c));
   hipLaunchKernelGGL(k_contrast, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->dP, c->F, N, c->red.p + 16);
   hipLaunchKernelGGL(k_tbc, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->F, c->bc, N);
   HIP_OK(hipGetLastError());
@@ -1268,8 +1310,49 @@ int i2e_ronurecalc(i2e_ctx *c) {
   return I2E_OK;
 }
 
+int i2e_sort_markers(i2e_ctx *c) {
+  if (!c) return I2E_ERR_ARG;
+  if (!c->M) return I2E_OK;
+  const size_t MM = (size_t)c->Mmax;
+  const int C = (c->X - 1) * (c->Y - 1), n = C + 1, nb = (n + SB - 1) / SB;
+  ST_OK(c->mfl2.alloc(7 * MM));
+  ST_OK(c->mt2.alloc(MM));
+  ST_OK(c->morig.alloc(MM));
+  ST_OK(c->morig2.alloc(MM));
+  ST_OK(c->mkey.alloc(MM));
+  ST_OK(c->mcnt.alloc(n));
+  ST_OK(c->moff.alloc(n));
+  ST_OK(c->mbsum.alloc(nb));
+  const unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+  HIP_OK(hipMemsetAsync(c->mcnt.p, 0, sizeof(int) * n, c->s));
+  hipLaunchKernelGGL(k_mkey, dim3(gb), dim3(256), 0, c->s, c->grid(), c->mk, c->M, C, c->mkey.p, c->mcnt.p);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, c->s, n, (const int *)c->mcnt.p, c->moff.p, c->mbsum.p);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, c->s, nb, c->mbsum.p);
+  hipLaunchKernelGGL(k_scan_add, dim3(nblk(n, 256)), dim3(256), 0, c->s, n, (const int *)c->mbsum.p, c->moff.p);
+  hipLaunchKernelGGL(k_mscatter, dim3(gb), dim3(256), 0, c->s, c->mk, marker_view(c->mfl2.p, c->mt2.p, MM), c->M,
+                     (const int *)c->mkey.p, c->moff.p, c->permuted ? (const int *)c->morig.p : nullptr,
+                     c->morig2.p);
+  HIP_OK(hipGetLastError());
+  std::swap(c->mfl, c->mfl2);
+  std::swap(c->mt, c->mt2);
+  std::swap(c->morig, c->morig2);
+  c->mk = marker_view(c->mfl.p, c->mt.p, MM);
+  c->permuted = 1;
+  c->since_sort = 0;
+  return I2E_OK;
+}
+
+int i2e_set_marker_sort(i2e_ctx *c, int every) {
+  if (!c || every < 0) return I2E_ERR_ARG;
+  c->sort_every = every;
+  return I2E_OK;
+}
+
 int i2e_step(i2e_ctx *c) {
   if (!c) return I2E_ERR_ARG;
+  // keep the markers grouped by cell (never while a trajectory is being
+  // recorded: the record keeps per-slot histories)
+  if (c->sort_every > 0 && !c->adj && c->M && ++c->since_sort >= c->sort_every) ST_OK(i2e_sort_markers(c));
   c->timestep = c->P.maxtmstep;
   ST_OK(i2e_maxvelstep(c, nullptr));
   ST_OK(i2e_viterate(c, 0));
@@ -1464,8 +1547,8 @@ int i2e_stokes_residuals(i2e_ctx *c, i2e_resid *out) {
 
 int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches, int every) {
   if (!c || max_launches < 0 || every < 1) return I2E_ERR_ARG;
-  static const char *names[K_COUNT] = {"advect", "p2g",          "gsdot",  "gsaxpy",
-                                       "vsmooth0", "vsmoothc", "stokes_apply", "precond"};
+  static const char *names[K_COUNT] = {"advect",   "p2g",          "gsdot",   "gsaxpy",  "vsmooth0",
+                                       "vsmoothc", "stokes_apply", "precond", "heatmark"};
   int id = -1;
   if (kernel && kernel[0]) {
     for (int k = 0; k < K_COUNT; k++)
@@ -1489,6 +1572,7 @@ int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches, int every
   c->kt.seen = 0;
   c->kt.open = false;
   c->kt.bytes = 0;
+  c->kt.repr = 0;
   return I2E_OK;
 }
 
@@ -1504,6 +1588,12 @@ int i2e_kernel_timer_read(i2e_ctx *c, int64_t *launches, double *ms_total, doubl
   if (launches) *launches = c->kt.launches;
   if (ms_total) *ms_total = tot;
   if (bytes_total) *bytes_total = c->kt.bytes;
+  return I2E_OK;
+}
+
+int i2e_kernel_timer_repr(i2e_ctx *c, double *repr_total) {
+  if (!c || !repr_total) return I2E_ERR_ARG;
+  *repr_total = c->kt.repr;
   return I2E_OK;
 }
 
@@ -1539,7 +1629,7 @@ int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g, do
     HIP_OK(hipMemsetAsync(c->acc.p, 0, sizeof(double) * 12 * c->N, c->s));
     HIP_OK(hipEventRecord(e0, c->s));
     hipLaunchKernelGGL(k_ronu_markers, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->A,
-                       c->timesum, c->timestep, c->need_eii);
+                       heat_per_kg(c->P, c->timesum), c->timestep, c->need_eii);
     HIP_OK(hipEventRecord(e1, c->s));
     HIP_OK(hipEventSynchronize(e1));
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));

@@ -54,6 +54,14 @@ struct i2e_ctx {
   int have_prev = 0;            // xprev holds a solution of this run (reset by i2e_upload)
   double dt_prev = 0;           // timestep of the last Stokes solve
   DBuf<int> tcnt;               // per-cell marker-type counts (i2e_type_counts)
+  // cell sort of the markers (sort.hip): second marker storage for the
+  // permutation, the caller's index of each slot, keys, counts/offsets
+  DBuf<float> mfl2;
+  DBuf<int8_t> mt2;
+  DBuf<int> morig, morig2, mkey, mcnt, moff, mbsum;
+  int permuted = 0;             // slots are not in the caller's order (morig maps them back)
+  int sort_every = 10;          // i2e_step sorts the markers every this many steps (0: never)
+  int since_sort = 0;
 
   Grid grid() const { return Grid{X, Y, P.xsize, P.ysize, gx.p, gy.p}; }
   SOp<double> sop() const { return SOp<double>{X, Y, gx.p, gy.p, F.nu, F.nd, F.ro, F.gp, timestep, P.nubeg * P.strmin}; }

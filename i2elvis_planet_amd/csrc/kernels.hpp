@@ -31,9 +31,16 @@ struct BcTab {
   const double *bondv;
 };
 
+// radiogenic heating per kg of a step (markmart.c:758-785): silicate (26Al
+// and, with longlived, 40K/235U/238U/232Th) and iron (60Fe, 40K)
+struct HeatK {
+  double sil, fe;
+};
+HeatK heat_per_kg(const i2e_params &p, double timesum);
+
 // markers.hip
 __global__ void k_movemark(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, double timestep, int need_eii);
-__global__ void k_ronu_markers(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, Acc A, double timesum,
+__global__ void k_ronu_markers(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, Acc A, HeatK hk,
                                double timestep, int need_eii);
 __global__ void k_ronu_nodes(const i2e_params *p, NodeF F, Acc A, int N, double *mm);
 __global__ void k_contrast(const i2e_params *p, NodeF F, int N, const double *mm);
@@ -43,5 +50,15 @@ __global__ void k_heat_markers_a(const i2e_params *p, Grid g, NodeF F, const dou
 __global__ void k_heat_nodes_norm(int N, const double *sol0, double *sol1);
 __global__ void k_heat_markers_b(Grid g, Markers mk, int64_t M, const double *sol1);
 __global__ void k_type_counts(Grid g, Markers mk, int64_t M, int ntypes, int *cnt);
+
+// sort.hip (exclusive scans run in blocks of SB elements: 256 threads x 4)
+constexpr int SB = 1024;
+__global__ void k_mkey(Grid g, Markers mk, int64_t M, int C, int *key, int *cnt);
+__global__ void k_scan_blocks(int n, const int *in, int *out, int *bsum);
+__global__ void k_scan_sums(int nb, int *bsum);
+__global__ void k_scan_add(int n, const int *bsum, int *out);
+__global__ void k_mscatter(Markers src, Markers dst, int64_t M, const int *key, int *off, const int *orig,
+                           int *orig2);
+__global__ void k_munperm(Markers src, Markers dst, int64_t M, const int *orig);
 
 }  // namespace i2e

@@ -77,7 +77,10 @@ __device__ __forceinline__ void clampxy(const Grid &g, double &x, double &y) {
     y = g.ysize;
 }
 
-__device__ void allintere_d(const Grid &g, const NodeF &F, double x, double y, Ev &e, bool full) {
+// (|g| at the marker is only read by the soft-turbulence conductivity, which
+// this build does not model: with_ga = false skips that gather)
+__device__ void allintere_d(const Grid &g, const NodeF &F, double x, double y, Ev &e, bool full,
+                            bool with_ga = false) {
   W4 w;
   clampxy(g, x, y);
   e.m10 = serch(g.gx, g.X, x);
@@ -95,7 +98,7 @@ __device__ void allintere_d(const Grid &g, const NodeF &F, double x, double y, E
     e.syy = isum(w, g.Y, F.syy);
   }
   e.pr = isum(w, g.Y, F.pr);
-  e.ga = isum(w, g.Y, F.ga);
+  e.ga = with_ga ? isum(w, g.Y, F.ga) : 0.0;
 }
 
 // allinterv, markmart.c:1378-1453
@@ -148,6 +151,22 @@ __global__ __launch_bounds__(256) void k_movemark(const i2e_params *__restrict__
 }
 
 // ---------------------------------------------------------------- ronurecalc
+HeatK heat_per_kg(const i2e_params &p, double timesum) {
+  HeatK h{0.0, 0.0};
+  const double timesumA = timesum / (365.250 * 24.000 * 3600.000);
+  const double init_al = 1.89716e+23 * p.al2627_init * 4.999e-13 / (3.264e+13);
+  double ll = 0;
+  if (p.longlived != 0)
+    ll = 0.40689 * 1.430e-11 * exp(-timesumA * 4.9867e-10) + 2.990e-12 * exp(-timesumA * 9.8458e-10) +
+         1.600e-12 * exp(-timesumA * 1.5541e-10) + 1.000e-12 * exp(-timesumA * 4.9511e-11);
+  h.sil = init_al * exp(-timesumA * 9.6673e-7) * p.h0_scale + ll;
+  const double init_fe = 1.971e+24 * p.fe6056_init * 4.3451e-13 / (1.183e+14);
+  double ka = 0;
+  if (p.longlived != 0) ka = 0.090625 * 1.430e-11 * exp(-timesumA * 4.9867e-10);
+  h.fe = init_fe * exp(-timesumA * 2.6660e-7) + ka;
+  return h;
+}
+
 // Henke et al. cold pressing / sintering (markmart.c:363-468); the Newton
 // restarts the reference repeats without bound stop after 64 here.
 __device__ void porosity_update_d(const i2e_params *p, float &por, float markk, double mpb, double mtk,
@@ -204,8 +223,7 @@ __device__ void porosity_update_d(const i2e_params *p, float &por, float markk, 
 // weighted sums of markmart.c:840-911, plus the cell-centre weights of nd.
 
 __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
-                                                     int64_t M, Acc A, double timesum, double timestep,
-                                                     int need_eii) {
+                                                     int64_t M, Acc A, HeatK hk, double timestep, int need_eii) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int lane = threadIdx.x & 63;
   // whole wavefronts iterate together: the scatter reduces across lanes
@@ -238,7 +256,7 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
           mnu = viscalc_d(p, mtk, mpb, t < 20 ? mm2 : mm2 - 20, eii);
         }
         double celdx = fx - g.xsize / 2.0, celdy = fy - g.ysize / 2.0;
-        celdy = pow(celdx * celdx + celdy * celdy, 0.5);
+        celdy = sqrt(celdx * celdx + celdy * celdy);
         if (celdy > g.xsize / 2.0 * p->radmax_frac) mnu = p->eta_outer;
         float por = mk.por[mm1];
         if (mm2 == 0 || mm2 >= 7) por = 0.000f;
@@ -297,20 +315,13 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
         }
         if (mkt <= 0.0010) mkt = 0.0010;
         if (mkt >= p->k_cutoff) mkt = p->k_cutoff;
+        // radiogenic heating per kg is the same for every marker of a step
+        // (markmart.c:758-785): formed once on the host, times the density
         double mht = 0;
-        double timesumA = timesum / (365.250 * 24.000 * 3600.000);
-        if (p->is_silicate[mm2]) {
-          double init_al = 1.89716e+23 * p->al2627_init * 4.999e-13 / (3.264e+13);
-          double ll = 0;
-          if (p->longlived != 0)
-            ll = 0.40689 * 1.430e-11 * exp(-timesumA * 4.9867e-10) + 2.990e-12 * exp(-timesumA * 9.8458e-10) +
-                 1.600e-12 * exp(-timesumA * 1.5541e-10) + 1.000e-12 * exp(-timesumA * 4.9511e-11);
-          mht = (init_al * exp(-timesumA * 9.6673e-7) * p->h0_scale + ll) * mro;
-        } else if (p->is_iron[mm2]) {
-          double init_fe = 1.971e+24 * p->fe6056_init * 4.3451e-13 / (1.183e+14), ka = 0;
-          if (p->longlived != 0) ka = 0.090625 * 1.430e-11 * exp(-timesumA * 4.9867e-10);
-          mht = (init_fe * exp(-timesumA * 2.6660e-7) + ka) * mro;
-        }
+        if (p->is_silicate[mm2])
+          mht = hk.sil * mro;
+        else if (p->is_iron[mm2])
+          mht = hk.fe * mro;
         // scatter weights (markmart.c:840-911)
         W4 w;
         nodewt2(g.gx, g.gy, e.m10, e.m20, x, y, 0, 0, w);
@@ -380,7 +391,6 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
           aadd(&A.ht0[m3], r4);
           aadd(&A.w0[m3], r5);
           aadd(&A.tk0[m3], r6);
-          aadd(&A.w1[m3], r5);  // the second weight sum equals the first
         }
       }
   }
@@ -395,11 +405,14 @@ __device__ __forceinline__ void amax_pos(double *a, double v) {
 }
 
 // node normalisation, markmart.c:917-994; mm[0]=numin, mm[1]=numax
-// (initialised to +huge / 0 by the caller)
-__global__ void k_ronu_nodes(const i2e_params *__restrict__ p, NodeF F, Acc A, int N, double *mm) {
-  int m3 = blockIdx.x * blockDim.x + threadIdx.x;
+// (initialised to +huge / 0 by the caller).  Grid-stride over the nodes with
+// a fixed, small grid: the viscosity range is reduced in the block and
+// combined with one atomic pair per block (one pair per wavefront on the
+// same two addresses serialised to ~95 us at 501^2).
+__global__ __launch_bounds__(256) void k_ronu_nodes(const i2e_params *__restrict__ p, NodeF F, Acc A, int N,
+                                                   double *mm) {
   double lmin = 1e300, lmax = 0;
-  if (m3 < N) {
+  for (int m3 = blockIdx.x * blockDim.x + threadIdx.x; m3 < N; m3 += gridDim.x * blockDim.x) {
     if (A.wnu[m3]) {
       double v = A.nu0[m3] / A.wnu[m3];
       if (v < p->nubeg) v = p->nubeg;
@@ -423,15 +436,25 @@ __global__ void k_ronu_nodes(const i2e_params *__restrict__ p, NodeF F, Acc A, i
       F.cp[m3] = A.cp0[m3] / w0;
       F.kt[m3] = A.kt0[m3] / w0;
       F.ht[m3] = A.ht0[m3] / w0;
-      if (A.w1[m3]) F.tk[m3] = A.tk0[m3] / A.w1[m3];
+      F.tk[m3] = A.tk0[m3] / w0;  // T's weight sum is the same w0 (markmart.c:905-911)
     }
   }
-  // wavefront reduction then one atomic per wavefront
   for (int o = 32; o > 0; o >>= 1) {
     lmin = fmin(lmin, __shfl_xor(lmin, o));
     lmax = fmax(lmax, __shfl_xor(lmax, o));
   }
+  __shared__ double smin[4], smax[4];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    smin[w] = lmin;
+    smax[w] = lmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+      lmin = fmin(lmin, smin[k]);
+      lmax = fmax(lmax, smax[k]);
+    }
     if (lmin < 1e300) amin_pos(&mm[0], lmin);
     if (lmax > 0) amax_pos(&mm[1], lmax);
   }

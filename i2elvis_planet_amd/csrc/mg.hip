@@ -1500,6 +1500,10 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   // finer level's two residual components once (it writes the rhs instead
   // of reading it)
   const double rbytes = l > 0 ? 2.0 * sizeof(T) * L[l - 1].N : 0.0;
+  // algorithmic bytes of a smoothing pass on matrix-free rows: ~40 B per
+  // point and sweep in FP32 (iterate and right-hand side of both components,
+  // the viscosities and density it needs), 80 in FP64
+  const double alg = 10.0 * sizeof(T) * (double)v.N * ns;
   const MapView none{nullptr, nullptr, nullptr, nullptr};
   TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr, 0,
                 none, none, none, none, 1, ns, 0, v.dx.p, v.dy.p};
@@ -1532,7 +1536,7 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
       hipLaunchKernelGGL((k_tsmooth<T, T, 2, false, true>), TG, TBK, 0, s, a, (const T *)v.S.p);
     else
       hipLaunchKernelGGL((k_tsmooth<T, T, 2>), TG, TBK, 0, s, a, (const T *)v.S.p);
-    if (kt) kt->end(kid, s, bytes + (a.rfx ? rbytes : 0.0));
+    if (kt) kt->end(kid, s, alg, bytes + (a.rfx ? rbytes : 0.0));
     std::swap(v.ux.p, v.ux2.p);
     std::swap(v.uy.p, v.uy2.p);
   };

@@ -251,8 +251,21 @@ int i2e_stokes_rhs_dev(i2e_ctx *c, double *b_dev);
 int i2e_stokes_precond_dev(i2e_ctx *c, const double *r_dev, double *z_dev);
 int i2e_vcycle_dev(i2e_ctx *c, const double *f_dev, double *u_dev);
 /* Device pointers of the context's node/marker arrays, by reference name
- * ("vx", "tk", "markx", ...); NULL if unknown.                              */
+ * ("vx", "tk", "markx", ...); NULL if unknown.  Marker arrays are in the
+ * context's slot order, which the cell sort below permutes; "markorig"
+ * gives the caller's index of every slot (int32; NULL while the slots are
+ * in the caller's order).                                                   */
 void *i2e_device_ptr(i2e_ctx *c, const char *name);
+/* Cell sort of the device markers (no counterpart in the reference, which
+ * keeps its markers in creation order): a counting sort on the cell key
+ * m1serch*(ynumy-1) + m2serch, markers outside the box last, so that the
+ * marker kernels read whole cells per wavefront.  It changes no result
+ * beyond the summation order of the scatter; i2e_download returns markers
+ * in the caller's order and i2e_upload restores it.  i2e_step sorts every
+ * `every` steps (default 10; 0 = never; suspended while i2e_record runs);
+ * i2e_sort_markers sorts now.  Extra device memory: 33 B per marker slot. */
+int i2e_set_marker_sort(i2e_ctx *c, int every);
+int i2e_sort_markers(i2e_ctx *c);
 /* Advection kernel alone, timed with HIP events on the context's stream:
  * runs movemark's marker loop `reps` times from the same start positions
  * and returns the mean kernel milliseconds.                                 */
@@ -296,6 +309,14 @@ int i2e_stokes_residuals(i2e_ctx *c, i2e_resid *out);
 int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches, int every);
 int i2e_kernel_timer_read(i2e_ctx *c, int64_t *launches, double *ms_total,
                           double *bytes_total);
+/* The representation bytes of the same launches: what this implementation
+ * reads and writes to do the work (e.g. the explicit Galerkin stencils of
+ * the coarse smoothing passes, 4 x 106 B per point, against the ~40 B per
+ * point and sweep of the matrix-free algorithm the timer's algorithmic
+ * bytes count).  Timer names: also "heatmark" = the marker temperature
+ * update's three launches (titerate, heatmart.c:515-689) as one unit; "p2g"
+ * covers the scatter and the node normalisation launch.                   */
+int i2e_kernel_timer_repr(i2e_ctx *c, double *repr_total);
 
 /* Replace the run parameters of a context (the reference reads them once
  * from mode.t3c/init.t3c, loadmart.c:26-90, into globals).  Grid sizes and

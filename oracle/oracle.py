@@ -203,6 +203,17 @@ def lu_solve(A, b, refine=3):
     return best
 
 
+def set_solver_threads(n):
+    """Threads of the PARDISO solves (the standalone MKL runtime's own
+    setting; the C parts of the oracle are serial)."""
+    mkl = _mkl_lib()
+    if mkl is not None:
+        try:
+            mkl.MKL_Set_Num_Threads(C.c_int(int(n)))
+        except AttributeError:
+            pass
+
+
 def solver_kind():
     return "pardiso" if (_use_pardiso() and _mkl_lib() is not None) else "superlu"
 
@@ -247,12 +258,21 @@ class Oracle:
         _check(self.L.orc_stokes_build(C.byref(self.p), self._s(), C.byref(M.c)), "stokes_build")
         return M
 
+    def solve_matrix(self, M):
+        return solve(M)
+
+    def stokes_post(self, x):
+        """P, vx, vy from the Stokes solution x, then strain rates and stresses
+        (movemart.c:979-1045)."""
+        x = np.ascontiguousarray(x, np.float64)
+        self.L.orc_stokes_post(C.byref(self.p), self._s(), x.ctypes.data_as(C.POINTER(C.c_double)))
+
     def viterate(self):
         self.gravity()
         M = self.stokes_matrix()
         x = solve(M)
         self.last["stokes"] = (M, x)
-        self.L.orc_stokes_post(C.byref(self.p), self._s(), x.ctypes.data_as(C.POINTER(C.c_double)))
+        self.stokes_post(x)
 
     def titerate(self):
         N = self.hs.N

@@ -242,12 +242,13 @@ def test_titerate_parity_fp64(orc):
     np.testing.assert_allclose(got.markk, ref.markk, rtol=1e-6)
 
 
-@pytest.mark.parametrize("name,nsteps", [("c1", 3)])
-def test_full_step_parity_fp64(orc, name, nsteps):
-    say("case steps", name, nsteps)
+@pytest.mark.parametrize("name,nsteps,sort_every", [("c1", 3, 10), ("c1", 3, 1)])
+def test_full_step_parity_fp64(orc, name, nsteps, sort_every):
+    say("case steps", name, nsteps, "sort every", sort_every)
     p, hs, _ = configs.make(name)
     ref = hs.copy()
     ctx = Context(p, hs, solver=fp64_opts(1e-13))
+    ctx.set_marker_sort(sort_every)
     ctx.ronurecalc()
     o = orc.Oracle(p, ref)
     o.ronurecalc()
@@ -262,6 +263,50 @@ def test_full_step_parity_fp64(orc, name, nsteps):
         assert np.array_equal(got.markt, ref.markt)
         assert np.array_equal(o.type_counts(), ctx.type_counts())
     ctx.close()
+
+
+def test_marker_sort_is_invisible_at_the_boundary(orc):
+    """The cell sort permutes the device markers only: after sorting, the
+    device order is nondecreasing in the cell key, download returns the
+    caller's order bit for bit, and advection from the sorted slots gives
+    per marker exactly what it gives unsorted."""
+    import ctypes as C
+    from i2elvis_planet_amd import load_library
+    say("case marker sort")
+    p, hs, _ = configs.make("c1")
+    rotation_field(hs, noise=0.3)
+    rng = np.random.default_rng(9)
+    perm = rng.permutation(hs.marknum)  # start from a shuffled order
+    for f in ("markx", "marky", "markk", "markv", "markp", "markpor", "marktmax", "markt"):
+        getattr(hs, f)[:] = getattr(hs, f)[perm]
+    hs.timestep = 0.05 * (hs.gx[1] - hs.gx[0]) / np.abs(hs.vx).max()
+    outs = []
+    for sort in (False, True):
+        ctx = Context(p, hs.copy(), solver=fp64_opts())
+        ctx.set_scalars(hs.timestep, 0.0)
+        if sort:
+            ctx.sort_markers()
+            L = load_library()
+            xs = np.empty(hs.marknum, np.float32)
+            ys = np.empty(hs.marknum, np.float32)
+            orig = np.empty(hs.marknum, np.int32)
+            for a, name in ((xs, "markx"), (ys, "marky"), (orig, "markorig")):
+                assert L.i2e_copy_to_host(a.ctypes.data_as(C.c_void_p), C.c_void_p(ctx.device_ptr(name)),
+                                          a.nbytes) == 0
+            i = np.clip(np.searchsorted(hs.gx, xs.astype(np.float64), side="right") - 1, 0, hs.xnumx - 2)
+            j = np.clip(np.searchsorted(hs.gy, ys.astype(np.float64), side="right") - 1, 0, hs.ynumy - 2)
+            key = i * (hs.ynumy - 1) + j
+            assert np.all(np.diff(key) >= 0)
+            assert np.array_equal(np.sort(orig), np.arange(hs.marknum))
+            assert np.array_equal(xs, hs.markx[orig])
+            back = ctx.download(hs.copy())
+            assert np.array_equal(back.markx, hs.markx) and np.array_equal(back.markt, hs.markt)
+        ctx.movemark()
+        outs.append(ctx.download(hs.copy()))
+        ctx.close()
+    a, b = outs
+    for f in ("markx", "marky", "markk", "markv", "markp", "markpor", "marktmax", "markt"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
 
 
 def test_type_counts_after_step_501(orc):
