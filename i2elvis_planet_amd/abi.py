@@ -116,6 +116,13 @@ class AdjointOut(C.Structure):
                 ("heat_relres_max", C.c_double), ("steps", C.c_int32), ("heat_iters", C.c_int32)]
 
 
+class InitInfo(C.Structure):
+    """i2e_init_info: sizes of the state an init.t3c file generates."""
+    _fields_ = [("xnumx", C.c_int64), ("ynumy", C.c_int64), ("mnumx", C.c_int64), ("mnumy", C.c_int64),
+                ("marknum", C.c_int64), ("bondnum", C.c_int64), ("rocknum", C.c_int32), ("nonstab", C.c_int32),
+                ("timesum", C.c_double)]
+
+
 class PrnHeader(C.Structure):
     """i2e_prn_header: sizes and scalars of a binary .prn file."""
     _fields_ = [("xnumx", C.c_int64), ("ynumy", C.c_int64), ("mnumx", C.c_int64), ("mnumy", C.c_int64),

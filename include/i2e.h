@@ -400,6 +400,29 @@ int i2e_prn_read(const char *path, i2e_params *p, i2e_host_state *s, int *n0);
 int i2e_maxvelstep_noalloc(const double *vx_dev, const double *vy_dev, int64_t xnumx, int64_t ynumy,
                            double xsize, double ysize, double maxxystep, double *timestep_dev, void *stream);
 
+/* ---- initial state from an init.t3c file (in2mart.c:23-1453) -------------
+ * The state a reference run starts from, built with in2mart's semantics:
+ * grid lines (with X/Y grid-line definitions), the marker lattice and its
+ * rand() jitter (the C library's sequence without srand, as the reference
+ * has it), the rock table, the boundary-condition tables (regular boxes,
+ * irregular rectangles and circular sectors for Vx, Vy, P, T; continuity
+ * coefficients 'c'; cell marker sets), rock bodies on the markers,
+ * the initial temperature field (box, exponential, circular, cooling-age,
+ * transition and impact distributions), porosity and the marker
+ * temperatures interpolated from it.  Not handled (I2E_ERR_UNSUPPORTED):
+ * markers loaded from a file and immobile rock types.  i2e_init_read gives
+ * the sizes; i2e_init_generate fills a state of exactly those sizes and
+ * the params the file holds (grid, rock table, 26Al/60Fe, ambient T, GYKOEF,
+ * por_init).  Node fields are FP64 as in2mart holds them (its binary save
+ * rounds them to float32: write and read back a .prn for that).          */
+typedef struct i2e_init_info {
+  int64_t xnumx, ynumy, mnumx, mnumy, marknum, bondnum;
+  int32_t rocknum, nonstab;
+  double timesum;                   /* seconds                                */
+} i2e_init_info;
+int i2e_init_read(const char *init_path, i2e_init_info *info);
+int i2e_init_generate(const char *init_path, i2e_params *p, i2e_host_state *s);
+
 /* Device buffers for callers without a device allocator of their own (no
  * counterpart in the reference): allocation, release and blocking copies,
  * all through the same HIP runtime as the library itself.                   */

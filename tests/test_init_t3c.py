@@ -1,0 +1,246 @@
+"""The initial state from an init.t3c file (i2e_init_read/i2e_init_generate,
+in2mart.c:23-1453 semantics), on the reference's own init.t3c (kept as a
+fixture under tests/golden/) and on small files written here.  CPU only,
+except the last test, which steps the generated state on the GPU against
+the oracle."""
+import ctypes as C
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from i2elvis_planet_amd import I2EError, configs, init_t3c
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_INIT = os.path.join(HERE, "golden", "init_reference.t3c")
+
+HEADER = """/GRID
+{X}-xnumx
+{Y}-ynumy
+{mx}-mnumx
+{my}-mnumy
+{size}-xsize
+{size}-ysize
+0.3-gamma
+0.3-memfe
+0.3-memsi
+{por}-por_init
+0-growth
+100.0-gr_init
+1-znumz
+1-corr2d3d
+0-pinit
+1.0-GXKOEF
+{R}-GYKOEF
+150-tmp_ambient
+1.5e6-timeexit
+5.250-al2627
+1.150-fe6056
+{tsum}-timesum
+{nonstab}
+0
+out.prn b
+/ROCKS
+      0   1e+19 1e+19 0 5e+29 1.00E+19 0 0 0 1.00E+00 0.0 0 0 0 0 0 1.0 1.00E+00 0 0 3.00E+03 2.00E+03 0 0 0
+      6   1e+17 1e+23 0 2.9e+09 5.10E+20 0 0 0 1.00E+00 1.00 1e+08 1e+08 0.3 0.3 0.0 1.0 3.50E+03 3.00E-05 3.00E-04 1.00E+03 3.00E+00 0 0 k1.8e-07
+      7   1e+17 1e+23 0 1e+12 1.26E+11 0 0 0 1.00E+00 1.00 1e+08 1e+08 0 0 0 1.0 7.54E+03 1.00E-05 3.00E-04 1.00E+03 3.00E+00 0 0 k1.8e-07
+~
+"""
+
+
+def write(tmp_path, body, name="init.t3c", X=11, Y=11, mx=3, my=3, size=100000.0, por=0.0, R=30000.0, tsum=0.0,
+          nonstab="0"):
+    path = tmp_path / name
+    path.write_text(HEADER.format(X=X, Y=Y, mx=mx, my=my, size=size, por=por, R=R, tsum=tsum, nonstab=nonstab) + body)
+    return str(path)
+
+
+FREE_SLIP = """/P
+P 3 3 1 1 0 0 0 0
+~
+"""
+
+
+def test_reference_init_gives_the_free_slip_tables_and_the_disk():
+    """The reference's init.t3c: 251^2 nodes, 4x4 markers per cell (1M), the
+    boundary tables of init.t3c:49-86 row for row (the same conditions as
+    configs.free_slip_bc, whose rows are numbered in another order), the
+    silicate disk of radius 100 km in sticky air, 150 K everywhere."""
+    info = init_t3c.read_info(REF_INIT)
+    assert (info.xnumx, info.ynumy, info.mnumx, info.mnumy) == (251, 251, 4, 4)
+    assert info.marknum == 16 * 250 * 250 and info.nonstab == 100
+    assert info.timesum == pytest.approx(1e6 * 3.15576e7)
+    p, hs = init_t3c.generate(REF_INIT)
+    assert (p.xsize, p.gykoef, p.tmp_ambient) == (500000.0, 249000.0, 150.0)
+    assert p.al2627_init == pytest.approx(5.25e-5) and p.ntypes == 27
+    assert p.rock[6].ro == 3500.0 and p.rock[6].mm == 4.0 and p.rock[7].ro == 7540.0
+
+    bm, bn, bv = configs.free_slip_bc(251, 251)
+
+    def rows(bondm, bondn, bondv):
+        bondn, bondv = bondn.reshape(-1, 3), bondv.reshape(-1, 4)
+        return {int(u): (bondv[bondm[u], 0], tuple((int(bondn[bondm[u], k]), bondv[bondm[u], k + 1])
+                                                   for k in range(3) if bondn[bondm[u], k]))
+                for u in np.nonzero(bondm)[0]}
+    assert rows(hs.bondm, hs.bondn, hs.bondv) == rows(bm, bn, bv)
+    assert hs.bondn.shape == bn.shape
+    # the disk: circular body of type 6, centre (0.5, 0.5), radius 0.2 of xsize
+    x, y = hs.markx.astype(np.float64) / p.xsize, hs.marky.astype(np.float64) / p.ysize
+    inside = (x - 0.5) ** 2 + (y - 0.5) ** 2 <= 0.2 ** 2
+    assert np.array_equal(hs.markt == 6, inside)
+    assert np.all(hs.markt[~inside] == 0)
+    assert np.allclose(hs.tk, 150.0, rtol=1e-15)
+    box = (hs.markx > 0) & (hs.marky > 0) & (hs.markx < p.xsize) & (hs.marky < p.ysize)
+    assert np.allclose(hs.markk[box], 150.0, rtol=1e-6) and np.all(hs.markk[~box] == 0)
+    assert np.array_equal(hs.marktmax[hs.markt == 6], hs.markk[hs.markt == 6])
+
+
+def test_jitter_matches_the_c_library_rand(tmp_path):
+    """Random nonstability: every marker moves by (rand() % (2n+1) - n)/n
+    times the jitter amplitude, x then y, marker after marker, from the C
+    library's unseeded sequence (in2mart.c:188-197), in float32."""
+    path = write(tmp_path, FREE_SLIP + "~\n~\n", nonstab="7-nonstab 0.5-dx 0.25-dy")
+    p, hs = init_t3c.generate(path)
+    libc = C.CDLL("libc.so.6")
+    libc.srand(1)
+    X, m, size = 11, 3, 100000.0
+    xstp = size / (X - 1)
+    dx = xstp / m
+    xn = 0.5 * size / (X - 1) / m
+    yn = 0.25 * size / (X - 1) / m
+    want_x, want_y = [], []
+    for m1 in range(X - 1):
+        for m2 in range(X - 1):
+            x = m1 * xstp + dx / 2.0
+            for _ in range(m):
+                y = m2 * xstp + dx / 2.0
+                for _ in range(m):
+                    want_x.append(np.float32(x))
+                    want_y.append(np.float32(y))
+                    y += dx
+                x += dx
+    want_x, want_y = np.array(want_x, np.float32), np.array(want_y, np.float32)
+    for i in range(want_x.size):
+        for arr, amp in ((want_x, xn), (want_y, yn)):
+            k = libc.rand() % 15 - 7
+            arr[i] = arr[i] + (np.float32(k) / np.float32(7)) * np.float32(amp)
+    assert np.array_equal(hs.markx, want_x)
+    assert np.array_equal(hs.marky, want_y)
+    assert not np.array_equal(hs.markx, np.round(hs.markx, -3))
+
+
+def test_grid_lines_bodies_and_temperature_boxes(tmp_path):
+    """X/Y grid-line definitions (linear and geometric steps), a rectangle
+    of iron, an incremental circle, markers added in a rectangle, and the
+    box, cooling-age and circular temperature distributions, each against
+    its closed form."""
+    body = """/P
+P 3 3 1 1 0 0 0 0
+/X_lines:_linear_steps_8_to_12_km_over_lines_1..5,_then_geometric
+X 1 5 0 0 8000 12000 0 0 0 0
+X 6 10 0 0 5000 0 10000 0 0 0
+~
+/BODIES
+0 7 0.2 0.2 0.2 0.4 0.4 0.2 0.4 0.4
+4 1 0.5 0.5 0.5 0.5 0.1 0 0 360
+1 6 0.0 0.0 0.0 0.1 0.1 0.0 0.1 0.1
+~
+/T
+0 0 0 0 1 1 0 1 1 100 100 300 300
+4 0.6 0.0 0.6 1.0 1.0 0.0 1.0 1.0 200 1000 200 1000 1e6 1e6 1e-6
+~
+"""
+    path = write(tmp_path, body)
+    p, hs = init_t3c.generate(path)
+    gx = hs.gx
+    steps = np.diff(gx)
+    # linear ramp of the step over lines 1..5: step(m) = 8000 + 4000 (m-1)/4
+    for m in range(1, 6):
+        assert steps[m - 1] == pytest.approx(8000 + 4000 * (m - 1) / 4, rel=1e-12)
+    # geometric over lines 6..10: step(m) = 5000 (10000/5000)^((m-6)/4),
+    # except the last line, reset to xsize
+    for m in range(6, 10):
+        assert steps[m - 1] == pytest.approx(5000 * 2.0 ** ((m - 6) / 4), rel=1e-12)
+    assert gx[-1] == 100000.0 and np.allclose(np.diff(hs.gy), 10000.0)
+    x = hs.markx.astype(np.float64) / p.xsize
+    y = hs.marky.astype(np.float64) / p.ysize
+    n_lattice = 10 * 10 * 9
+    assert hs.marknum == n_lattice + 2 * 2 * 9  # cells 0..1 x 0..1 added by the last body
+    lat = slice(0, n_lattice)
+    rect = (x[lat] >= 0.2) & (x[lat] <= 0.4) & (y[lat] >= 0.2) & (y[lat] <= 0.4)
+    circ = (x[lat] - 0.5) ** 2 + (y[lat] - 0.5) ** 2 <= 0.1 ** 2
+    want = np.where(rect, 7, 0) + np.where(circ, 1, 0)
+    assert np.array_equal(hs.markt[lat], want)
+    assert np.all(hs.markt[n_lattice:] == 6)
+    # T: linear in x from 100 to 300 K, then the cooling-age profile for x >= 0.6
+    X, Y = hs.xnumx, hs.ynumy
+    tk = hs.tk.reshape(X, Y)
+    for i in range(X):
+        for j in range(Y):
+            xi, yj = gx[i] / p.xsize, hs.gy[j] / p.ysize
+            if xi >= 0.6:
+                xr = (xi - 0.6) / 0.4
+                arg = hs.gy[j] / 2.0 / math.sqrt(1e-6 * 1e6 * 3.15576e7)
+                t = (1000 + 0 * xr) - (1 - math.erf(arg)) * (800 * (1 - xr) + 800 * xr)
+            else:
+                t = 100 * (1 - xi) + 300 * xi
+            assert tk[i, j] == pytest.approx(t, rel=1e-12, abs=1e-9), (i, j)
+
+
+def test_unsupported_inputs_are_refused(tmp_path):
+    imm = write(tmp_path, FREE_SLIP + "~\n~\n", name="imm.t3c").replace("imm", "imm")
+    txt = open(imm).read().replace("      7   1e+17", "      i7   1e+17")
+    open(imm, "w").write(txt)
+    with pytest.raises(I2EError, match="status 5.*immobile"):
+        init_t3c.read_info(imm)
+    mf = tmp_path / "mf.t3c"
+    mf.write_text(open(write(tmp_path, FREE_SLIP + "~\n~\n", name="x.t3c")).read().replace("\n0\nout.prn b",
+                                                                                            "\nold.prn b\nout.prn b"))
+    with pytest.raises(I2EError, match="status 5.*from a file"):
+        init_t3c.read_info(str(mf))
+    bad = tmp_path / "bad.t3c"
+    bad.write_text(open(write(tmp_path, "Q 1 1 1 1 0 0 0 0\n~\n~\n~\n", name="q.t3c")).read())
+    with pytest.raises(I2EError, match="unknown boundary parameter"):
+        init_t3c.read_info(str(bad))
+
+
+@pytest.mark.gpu
+def test_reference_init_steps_like_the_oracle(orc, tmp_path):
+    """The reference's init.t3c state, through the .prn round trip in2mart's
+    saver and i2mart's loader make (float32 node fields and grid lines), one
+    FP64 step on the GPU against the oracle's: power-law silicate (markss,
+    markmm set), so the strain-rate branch of the viscosity runs."""
+    from i2elvis_planet_amd import Context, default_solver, prn
+    print("case init.t3c 251^2 step", file=sys.stderr, flush=True)
+    base = configs.make_params(dict(configs.CONFIGS["c2"], n=251, size=500e3, radius=249e3))
+    p, hs = init_t3c.generate(REF_INIT, params=base)
+    path = tmp_path / "start.prn"
+    prn.write(path, p, hs, n0=0)
+    q, st, _ = prn.read(path, params=p)
+    ref = st.copy()
+    o = default_solver()
+    o.precision = 0
+    o.stokes_rtol = 1e-12
+    o.heat_rtol = 1e-13
+    o.grav_rtol = 1e-13
+    ctx = Context(q, st, solver=o)
+    ctx.ronurecalc()
+    ctx.step()
+    got = ctx.download(st.copy())
+    ctx.close()
+    orc_ = orc.Oracle(q, ref)
+    orc_.ronurecalc()
+    orc_.step()
+    # sticky air at 1e19 Pa s around a disk at the 1e23 clip: the system is
+    # conditioned so that two direct LU solves (COLAMD / natural ordering)
+    # differ by 1e-9 in v, and an iterate at ||b - Ax|| <= 1e-12 ||b|| (the
+    # FP64 Krylov floor) by up to ~1e-7; P and T are far better determined
+    tol = {"vx": 3e-7, "vy": 3e-7, "pr": 1e-8, "tk": 1e-10}
+    assert got.timestep == pytest.approx(ref.timestep, rel=3e-7)
+    for f in ("vx", "vy", "pr", "tk"):
+        e = np.linalg.norm(getattr(got, f) - getattr(ref, f)) / np.linalg.norm(getattr(ref, f))
+        print(f, "rel L2 %.2e" % e, file=sys.stderr)
+        assert e < tol[f], f
+    assert np.array_equal(got.markt, ref.markt)
