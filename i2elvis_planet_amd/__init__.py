@@ -29,7 +29,7 @@ _EXPORTS = [
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
     "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
     "i2e_type_counts", "i2e_maxvelstep_noalloc", "i2e_kernel_timer_repr", "i2e_set_marker_sort",
-    "i2e_sort_markers", "i2e_init_read", "i2e_init_generate",
+    "i2e_sort_markers", "i2e_init_read", "i2e_init_generate", "i2e_mode_read",
 ]
 
 _lib = None
@@ -86,6 +86,7 @@ def load_library(path=LIB_PATH):
         "i2e_sort_markers": ([vp], C.c_int),
         "i2e_init_read": ([C.c_char_p, C.POINTER(abi.InitInfo)], C.c_int),
         "i2e_init_generate": ([C.c_char_p, P, S], C.c_int),
+        "i2e_mode_read": ([C.c_char_p, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
         "i2e_dev_alloc": ([C.c_size_t, C.POINTER(vp)], C.c_int),
         "i2e_dev_free": ([vp], C.c_int),
         "i2e_copy_to_dev": ([vp, vp, C.c_size_t], C.c_int),

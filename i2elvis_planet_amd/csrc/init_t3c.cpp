@@ -843,4 +843,83 @@ int i2e_init_generate(const char *init_path, i2e_params *p, i2e_host_state *s) {
   return I2E_OK;
 }
 
+int i2e_mode_read(const char *mode_path, int file_index, i2e_params *p, int *cyc0max) {
+  if (!mode_path || !p || file_index < 0) return I2E_ERR_ARG;
+  Tokens t;
+  if (!t.load(mode_path)) {
+    i2e::set_error("mode: cannot read %s", mode_path);
+    return I2E_ERR_ARG;
+  }
+  std::string s;
+  bool ok = true;
+  auto tok = [&]() {
+    if (!t.next(s)) ok = false;
+    return ok ? s : std::string("0");
+  };
+  // loadconf (loadmart.c:2-105): the input file and its type, then one line
+  // per output file (name, type, cyc0max, maxxystep, maxtkstep, maxtmstep in
+  // years) up to '~', then the GENERAL, STOKES and HEAT parameters
+  tok();
+  tok();
+  int n = 0, found = 0;
+  for (std::string name = tok(); ok && name[0] != '~'; name = tok(), n++) {
+    tok();
+    const int cyc = atoi(tok().c_str());
+    const double xy = atof(tok().c_str()), tkst = atof(tok().c_str()), tm = atof(tok().c_str()) * 3.15576e+7;
+    if (n == file_index) {
+      found = 1;
+      if (cyc0max) *cyc0max = cyc;
+      p->maxxystep = xy;
+      p->maxtkstep = tkst;
+      p->maxtmstep = tm;
+    }
+  }
+  tok();  // printmod
+  tok();  // timedir
+  tok();  // movemod
+  tok();  // tempmod
+  const int markmod = atoi(tok().c_str());
+  tok();  // ratemod
+  tok();  // gridmod
+  tok();  // intermod
+  tok();  // intermod1
+  const int outgrid = atoi(tok().c_str()), densimod = atoi(tok().c_str());
+  tok();  // DIVVMIN
+  tok();  // STOKSMIN
+  tok();  // stoksmod
+  tok();  // presmod
+  tok();  // stoksfd
+  const double nubeg = atof(tok().c_str()), nuend = atof(tok().c_str()), nucontr = atof(tok().c_str());
+  tok();  // hidry
+  tok();  // hidrl
+  const double strmin = atof(tok().c_str());
+  tok();  // strmax
+  tok();  // viscmod
+  tok();  // HEATMIN
+  const int heatmod = atoi(tok().c_str());
+  tok();  // heatfd
+  const double heatdif = atof(tok().c_str());
+  const double frictyn = atof(tok().c_str()) * 3.15576e+7, adiabyn = atof(tok().c_str()) * 3.15576e+7;
+  if (!ok) {
+    i2e::set_error("mode: %s ends early", mode_path);
+    return I2E_ERR_ARG;
+  }
+  if (!found) {
+    i2e::set_error("mode: no output file %d (the file lists %d)", file_index, n);
+    return I2E_ERR_ARG;
+  }
+  p->markmod = markmod;
+  p->outgrid = outgrid;
+  p->densimod = densimod;
+  p->nubeg = nubeg;
+  p->nuend = nuend;
+  p->nucontr = nucontr;
+  p->strmin = strmin;
+  p->heatmod = heatmod;
+  p->heatdif = heatdif;
+  p->frictyn = frictyn;
+  p->adiabyn = adiabyn;
+  return I2E_OK;
+}
+
 }  // extern "C"

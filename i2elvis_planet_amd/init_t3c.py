@@ -39,3 +39,13 @@ def generate(path, params=None):
     _chk(_lib().i2e_init_generate(str(path).encode(), C.byref(p), C.byref(cs)), "i2e_init_generate")
     hs.pull_scalars(cs)
     return p, hs
+
+
+def read_mode(path, file_index=0, params=None):
+    """Run controls of a mode.t3c file for output file `file_index` (time-step
+    limits, rheology bounds, heat options); returns (params, cyc0max)."""
+    from . import _chk
+    p = copy.deepcopy(params) if params is not None else abi.Params()
+    n = C.c_int(0)
+    _chk(_lib().i2e_mode_read(str(path).encode(), int(file_index), C.byref(p), C.byref(n)), "i2e_mode_read")
+    return p, n.value

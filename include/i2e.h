@@ -422,6 +422,13 @@ typedef struct i2e_init_info {
 } i2e_init_info;
 int i2e_init_read(const char *init_path, i2e_init_info *info);
 int i2e_init_generate(const char *init_path, i2e_params *p, i2e_host_state *s);
+/* The run controls of a mode.t3c file (loadconf, loadmart.c:2-105) into p:
+ * from output-file line `file_index` (0-based) the time-step limits
+ * maxxystep, maxtkstep, maxtmstep (years in the file, seconds in p) and its
+ * cycle count (*cyc0max, may be NULL); markmod, outgrid, densimod; nubeg,
+ * nuend, nucontr, strmin; heatmod, heatdif, frictyn and adiabyn (years in
+ * the file).  Other fields of p are left as they are.                     */
+int i2e_mode_read(const char *mode_path, int file_index, i2e_params *p, int *cyc0max);
 
 /* Device buffers for callers without a device allocator of their own (no
  * counterpart in the reference): allocation, release and blocking copies,

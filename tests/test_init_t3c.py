@@ -206,6 +206,71 @@ def test_unsupported_inputs_are_refused(tmp_path):
         init_t3c.read_info(str(bad))
 
 
+MODE = """/LOADFILE_TYPE
+dd_0.prn b
+/SAVEFILE_TYPE_cyc0max_maxxystep_maxtkstep_maxtmstep
+dd_1.prn b 10 0.00001 1.00E+04 2.50E+03
+dd_2.prn b 51 0.05 2.00E+09 1.00E+04
+~
+/GENERAL_PARAMETERS
+10000-printmod
+1-timedir
+1-movemod
+1-tempmod
+2-markmod
+0-ratemod
+1-gridmod
+0-intermod
+0-intermod1
+2-outgrid
+1-densimod
+/STOKES_CONTINUITY_PARAMETERS
+1e-04-DIVVMIN
+1e-06-STOKSMIN
+1-stoksmod
+0-presmod
+0-stoksfd
+1e+17-nubeg
+1e+26-nuend
+1e+18-nucontr
+5e+3-hidry
+6e-1-hidrl
+1e-01-strmin
+1e-01-strmax
+0-viscmod
+/HEAT_TRANSFER_PARAMETERS
+1e-7-HEATMIN
+0-heatmod
+0-heatfd
+1.0-heatdif
+1-frictyn
+0-adiabyn
+1095001-core_form_time
+1-si_melting
+1-fe_melting
+1-ice_melting
+"""
+
+
+def test_mode_file_run_controls(tmp_path):
+    """mode.t3c in loadconf's layout (loadmart.c:2-105): the time-step limits
+    of the chosen output file (maxtmstep and frictyn in years in the file),
+    rheology bounds and heat options."""
+    path = tmp_path / "mode.t3c"
+    path.write_text(MODE)
+    yr = 3.15576e7
+    p, cyc = init_t3c.read_mode(path, 1)
+    assert cyc == 51 and (p.maxxystep, p.maxtkstep) == (0.05, 2e9)
+    assert p.maxtmstep == pytest.approx(1e4 * yr)
+    assert (p.markmod, p.outgrid, p.densimod, p.heatmod) == (2, 2, 1, 0)
+    assert (p.nubeg, p.nuend, p.nucontr, p.strmin, p.heatdif) == (1e17, 1e26, 1e18, 0.1, 1.0)
+    assert p.frictyn == pytest.approx(yr) and p.adiabyn == 0.0
+    p0, cyc0 = init_t3c.read_mode(path, 0)
+    assert cyc0 == 10 and p0.maxxystep == 1e-5 and p0.maxtmstep == pytest.approx(2.5e3 * yr)
+    with pytest.raises(I2EError, match="no output file 2"):
+        init_t3c.read_mode(path, 2)
+
+
 @pytest.mark.gpu
 def test_reference_init_steps_like_the_oracle(orc, tmp_path):
     """The reference's init.t3c state, through the .prn round trip in2mart's
