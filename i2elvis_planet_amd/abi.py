@@ -61,6 +61,8 @@ class Params(C.Structure):
         ("is_iron", C.c_uint8 * MAXTYPES),
         ("porous", C.c_uint8 * MAXTYPES),
         ("meltable", C.c_uint8 * MAXTYPES),
+        ("core_form_time", C.c_double), ("fe_ts", C.c_double), ("fe_tl", C.c_double),
+        ("soft_turb", C.c_int32), ("magnet", C.c_int32), ("dynamo", C.c_int32), ("pad1", C.c_int32),
     ]
 
 
@@ -76,6 +78,7 @@ class HostStateC(C.Structure):
         ("markt", _i8p),
         ("bondm", _i64p), ("bondn", _i64p), ("bondv", _dp), ("bondnum", C.c_int64),
         ("timestep", C.c_double), ("timesum", C.c_double), ("gxkoef", C.c_double),
+        ("markmg_old", _i8p), ("markmg_time", _dp),
     ]
 
 

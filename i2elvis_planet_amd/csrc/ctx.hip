@@ -50,9 +50,13 @@ __global__ void k_fill(int n, double v, double *a) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) a[k] = v;
 }
 
+// slot of the reduction buffer holding marker 0's temperature for the
+// soft-turbulence conductivity (clear of the partial sums below it)
+constexpr int kTrefSlot = 4000;
+
 // the SoA view of a marker storage of capacity MM (7 float fields + type)
-static Markers marker_view(float *mf, int8_t *t, size_t MM) {
-  return Markers{mf, mf + MM, mf + 2 * MM, mf + 3 * MM, mf + 4 * MM, mf + 5 * MM, mf + 6 * MM, t};
+static Markers marker_view(float *mf, int8_t *t, int8_t *mg, double *mgt, size_t MM) {
+  return Markers{mf, mf + MM, mf + 2 * MM, mf + 3 * MM, mf + 4 * MM, mf + 5 * MM, mf + 6 * MM, t, mg, mgt};
 }
 
 // Stokes preconditioner pieces (lower block-triangular):
@@ -540,7 +544,9 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
       (st = c->gsc.alloc(N)) || (st = c->hsc.alloc(N)))
     return fail(st);
   if (max_markers > 0) {
-    if ((st = c->mfl.alloc((size_t)7 * max_markers)) || (st = c->mt.alloc(max_markers))) return fail(st);
+    if ((st = c->mfl.alloc((size_t)7 * max_markers)) || (st = c->mt.alloc(max_markers)) ||
+        (st = c->mg.alloc(max_markers)) || (st = c->mgt.alloc(max_markers)))
+      return fail(st);
   }
   double *n0 = c->nodes.p;
   double **f[19] = {&c->F.vx, &c->F.vy, &c->F.pr, &c->F.gp, &c->F.ga, &c->F.exx, &c->F.eyy,
@@ -549,7 +555,7 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
   for (int k = 0; k < 19; k++) *f[k] = n0 + (size_t)k * N;
   float *mf = c->mfl.p;
   size_t MM = (size_t)max_markers;
-  c->mk = marker_view(mf, c->mt.p, MM);
+  c->mk = marker_view(mf, c->mt.p, c->mg.p, c->mgt.p, MM);
   double *a0 = c->acc.p;
   c->A = Acc{a0, a0 + N, a0 + 2 * (size_t)N, a0 + 3 * (size_t)N, a0 + 4 * (size_t)N, a0 + 5 * (size_t)N,
              a0 + 6 * (size_t)N, a0 + 7 * (size_t)N, a0 + 8 * (size_t)N, a0 + 9 * (size_t)N, a0 + 10 * (size_t)N,
@@ -588,7 +594,11 @@ int i2e_destroy(i2e_ctx *c) {
   c->tcnt.release();
   c->mfl2.release();
   c->mt2.release();
-  DBuf<int> *ib[] = {&c->morig, &c->morig2, &c->mkey, &c->mcnt, &c->moff, &c->mbsum};
+  DBuf<int> *ib[] = {&c->morig, &c->morig2, &c->mkey, &c->mcnt, &c->moff, &c->mbsum, &c->mslot0};
+  c->mg.release();
+  c->mg2.release();
+  c->mgt.release();
+  c->mgt2.release();
   for (auto *b : ib) b->release();
   for (auto *b : d) b->release();
   c->mfl.release();
@@ -861,6 +871,18 @@ int i2e_upload(i2e_ctx *c, const i2e_host_state *s) {
   for (int k = 0; k < 7; k++)
     if (c->M) HIP_OK(hipMemcpy(c->mfl.p + (size_t)k * c->Mmax, ms[k], sizeof(float) * c->M, hipMemcpyHostToDevice));
   if (c->M) HIP_OK(hipMemcpy(c->mt.p, s->markt, c->M, hipMemcpyHostToDevice));
+  // magnetisation: the caller's, or the initial 0 / -1 Ma of in2mart.c:1408-1413
+  if (c->M && s->markmg_old)
+    HIP_OK(hipMemcpy(c->mg.p, s->markmg_old, c->M, hipMemcpyHostToDevice));
+  else if (c->M)
+    HIP_OK(hipMemset(c->mg.p, 0, c->M));
+  if (c->M && s->markmg_time) {
+    HIP_OK(hipMemcpy(c->mgt.p, s->markmg_time, sizeof(double) * c->M, hipMemcpyHostToDevice));
+  } else if (c->M) {
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->s, (int)c->M, -1.0, c->mgt.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->s));
+  }
   // BC tables (int64 -> int32)
   if (s->bondnum < 1 || s->bondnum > (int64_t)1 << 30) return I2E_ERR_ARG;
   std::vector<int> bm(4 * (size_t)N), bn(3 * (size_t)s->bondnum);
@@ -895,20 +917,27 @@ int i2e_download(i2e_ctx *c, i2e_host_state *s) {
   float *ms[7] = {s->markx, s->marky, s->markk, s->markv, s->markp, s->markpor, s->marktmax};
   // markers in the caller's order: undo the cell sort into the spare storage
   const float *mf = c->mfl.p;
-  const int8_t *mtp = c->mt.p;
+  const int8_t *mtp = c->mt.p, *mgp = c->mg.p;
+  const double *mgtp = c->mgt.p;
   if (c->permuted && c->M) {
     unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
-    hipLaunchKernelGGL(k_munperm, dim3(gb), dim3(256), 0, c->s, c->mk, marker_view(c->mfl2.p, c->mt2.p, c->Mmax),
-                       c->M, (const int *)c->morig.p);
+    hipLaunchKernelGGL(k_munperm, dim3(gb), dim3(256), 0, c->s, c->mk,
+                       marker_view(c->mfl2.p, c->mt2.p, c->mg2.p, c->mgt2.p, c->Mmax), c->M,
+                       (const int *)c->morig.p);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(c->s));
     mf = c->mfl2.p;
     mtp = c->mt2.p;
+    mgp = c->mg2.p;
+    mgtp = c->mgt2.p;
   }
   for (int k = 0; k < 7; k++)
     if (ms[k] && c->M)
       HIP_OK(hipMemcpy(ms[k], mf + (size_t)k * c->Mmax, sizeof(float) * c->M, hipMemcpyDeviceToHost));
   if (s->markt && c->M) HIP_OK(hipMemcpy(s->markt, mtp, c->M, hipMemcpyDeviceToHost));
+  if (s->markmg_old && c->M) HIP_OK(hipMemcpy(s->markmg_old, mgp, c->M, hipMemcpyDeviceToHost));
+  if (s->markmg_time && c->M)
+    HIP_OK(hipMemcpy(s->markmg_time, mgtp, sizeof(double) * c->M, hipMemcpyDeviceToHost));
   s->marknum = c->M;
   s->timestep = c->timestep;
   s->timesum = c->timesum;
@@ -1302,8 +1331,12 @@ int i2e_ronurecalc(i2e_ctx *c) {
   c->kt.begin(K_P2G, c->s);
   if (c->M) {
     unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+    if (c->P.soft_turb)
+      hipLaunchKernelGGL(k_tref, dim3(1), dim3(64), 0, c->s, c->dP, c->grid(), c->mk,
+                         c->permuted ? (const int *)c->mslot0.p : nullptr, c->red.p + kTrefSlot);
     hipLaunchKernelGGL(k_ronu_markers, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->A,
-                       heat_per_kg(c->P, c->timesum), c->timestep, c->need_eii);
+                       heat_per_kg(c->P, c->timesum), c->timestep, c->timesum, c->need_eii,
+                       (const double *)c->red.p + kTrefSlot);
   }
   hipLaunchKernelGGL(k_ronu_nodes, dim3(std::min(nblk(N, 256), 256u)), dim3(256), 0, c->s, c->dP, c->F, c->A, N,
                      c->red.p + 16);
@@ -1325,6 +1358,9 @@ int i2e_sort_markers(i2e_ctx *c) {
   const int C = (c->X - 1) * (c->Y - 1), n = C + 1, nb = (n + SB - 1) / SB;
   ST_OK(c->mfl2.alloc(7 * MM));
   ST_OK(c->mt2.alloc(MM));
+  ST_OK(c->mg2.alloc(MM));
+  ST_OK(c->mgt2.alloc(MM));
+  ST_OK(c->mslot0.alloc(1));
   ST_OK(c->morig.alloc(MM));
   ST_OK(c->morig2.alloc(MM));
   ST_OK(c->mkey.alloc(MM));
@@ -1337,14 +1373,16 @@ int i2e_sort_markers(i2e_ctx *c) {
   hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, c->s, n, (const int *)c->mcnt.p, c->moff.p, c->mbsum.p);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, c->s, nb, c->mbsum.p);
   hipLaunchKernelGGL(k_scan_add, dim3(nblk(n, 256)), dim3(256), 0, c->s, n, (const int *)c->mbsum.p, c->moff.p);
-  hipLaunchKernelGGL(k_mscatter, dim3(gb), dim3(256), 0, c->s, c->mk, marker_view(c->mfl2.p, c->mt2.p, MM), c->M,
-                     (const int *)c->mkey.p, c->moff.p, c->permuted ? (const int *)c->morig.p : nullptr,
-                     c->morig2.p);
+  hipLaunchKernelGGL(k_mscatter, dim3(gb), dim3(256), 0, c->s, c->mk,
+                     marker_view(c->mfl2.p, c->mt2.p, c->mg2.p, c->mgt2.p, MM), c->M, (const int *)c->mkey.p,
+                     c->moff.p, c->permuted ? (const int *)c->morig.p : nullptr, c->morig2.p, c->mslot0.p);
   HIP_OK(hipGetLastError());
   std::swap(c->mfl, c->mfl2);
   std::swap(c->mt, c->mt2);
+  std::swap(c->mg, c->mg2);
+  std::swap(c->mgt, c->mgt2);
   std::swap(c->morig, c->morig2);
-  c->mk = marker_view(c->mfl.p, c->mt.p, MM);
+  c->mk = marker_view(c->mfl.p, c->mt.p, c->mg.p, c->mgt.p, MM);
   c->permuted = 1;
   c->since_sort = 0;
   return I2E_OK;
@@ -1635,8 +1673,11 @@ int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g, do
   for (int r = 0; r < reps; r++) {
     HIP_OK(hipMemsetAsync(c->acc.p, 0, sizeof(double) * 12 * c->N, c->s));
     HIP_OK(hipEventRecord(e0, c->s));
+    hipLaunchKernelGGL(k_tref, dim3(1), dim3(64), 0, c->s, c->dP, c->grid(), c->mk,
+                       c->permuted ? (const int *)c->mslot0.p : nullptr, c->red.p + kTrefSlot);
     hipLaunchKernelGGL(k_ronu_markers, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->A,
-                       heat_per_kg(c->P, c->timesum), c->timestep, c->need_eii);
+                       heat_per_kg(c->P, c->timesum), c->timestep, c->timesum, c->need_eii,
+                       (const double *)c->red.p + kTrefSlot);
     HIP_OK(hipEventRecord(e1, c->s));
     HIP_OK(hipEventSynchronize(e1));
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));

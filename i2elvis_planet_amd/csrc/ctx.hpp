@@ -59,6 +59,10 @@ struct i2e_ctx {
   DBuf<float> mfl2;
   DBuf<int8_t> mt2;
   DBuf<int> morig, morig2, mkey, mcnt, moff, mbsum;
+  DBuf<int> mslot0;             // slot of the caller's marker 0 (while permuted)
+  // magnetisation per marker slot (markmg_old, markmg_time) and their spares
+  DBuf<int8_t> mg, mg2;
+  DBuf<double> mgt, mgt2;
   int permuted = 0;             // slots are not in the caller's order (morig maps them back)
   int sort_every = 10;          // i2e_step sorts the markers every this many steps (0: never)
   int since_sort = 0;

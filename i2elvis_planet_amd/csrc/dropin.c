@@ -15,7 +15,8 @@
 extern double gx[], gy[], vx[], vy[], pr[], gp[], ga[], exx[], eyy[], exy[], sxx[], syy[], sxy[];
 extern double nu[], nd[], ro[], tk[], cp[], kt[], ht[], et[];
 extern float markx[], marky[], markk[], markv[], markp[], markpor[], marktmax[];
-extern char markt[];
+extern char markt[], markmg_old[];
+extern double markmg_time[];
 extern long int bondm[], bondn[][3];
 extern double bondv[][4];
 extern long int xnumx, ynumy, mnumx, mnumy, marknum, bondnum;
@@ -26,6 +27,8 @@ extern double xsize, ysize, GXKOEF, GYKOEF, tmp_ambient, al2627_init, fe6056_ini
 extern double maxxystep, maxtkstep, maxtmstep, timestep, timesum;
 extern double nubeg, nuend, nucontr, strmin, heatdif, frictyn, adiabyn, por_init, k_cutoff;
 extern int heatmod, densimod, outgrid, markmod;
+extern double core_form_time;
+extern int dynamo;
 
 static i2e_ctx *g_ctx = NULL;
 static int64_t g_cap = 0, g_X = 0, g_Y = 0, g_mx = 0, g_my = 0;
@@ -48,7 +51,12 @@ static void die(const char *fn, int st) {
  * code: radmax = 0.95 xsize/2 with 1e22 Pa s beyond (markmart.c:268, 340),
  * the long-lived isotopes (markmart.c:772-782), porosity closures for the
  * solid silicate types 5 and 6 (markmart.c:347-360, 612-648), 26Al in types
- * 5, 6, 25, 26 and 60Fe in 7-10, 17-19 (markmart.c:760, 789).             */
+ * 5, 6, 25, 26 and 60Fe in 7-10, 17-19 (markmart.c:760, 789), the
+ * soft-turbulence conductivity after core_form_time (markmart.c:508-610)
+ * and the magnetisation of solid silicates (markmart.c:651-744).  The
+ * reference's iron solidus and liquidus depend on pressure (markmart.c:
+ * 2519-2560); the shim leaves fe_tl = 0, iron solid, unless the hook sets
+ * them.                                                                   */
 static void build_params(i2e_params *p) {
   int t;
   memset(p, 0, sizeof *p);
@@ -84,6 +92,10 @@ static void build_params(i2e_params *p) {
   p->eta_outer = 1.0e22;
   p->melt_on = 0;
   p->porosity_on = 1;
+  p->soft_turb = 1;
+  p->core_form_time = core_form_time;
+  p->magnet = 1;
+  p->dynamo = dynamo;
   p->ntypes = rocknum < I2E_MAXTYPES ? rocknum : I2E_MAXTYPES;
   for (t = 0; t < p->ntypes; t++) {
     i2e_rock *r = &p->rock[t];
@@ -140,6 +152,8 @@ static void bind_state(i2e_host_state *s) {
   s->markpor = markpor;
   s->marktmax = marktmax;
   s->markt = (int8_t *)markt;
+  s->markmg_old = (int8_t *)markmg_old;
+  s->markmg_time = markmg_time;
   s->bondm = (int64_t *)bondm;
   s->bondn = (int64_t *)&bondn[0][0];
   s->bondv = &bondv[0][0];

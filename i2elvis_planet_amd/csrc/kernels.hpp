@@ -16,6 +16,8 @@ struct NodeF {
 struct Markers {
   float *x, *y, *k, *v, *p, *por, *tmax;
   int8_t *t;
+  int8_t *mg;   // magnetisation state (markmg_old)
+  double *mgt;  // its time, Ma (markmg_time)
 };
 
 // ronurecalc accumulators (ro0..tk0, sol0/sol1 and the viscosity weights)
@@ -41,7 +43,8 @@ HeatK heat_per_kg(const i2e_params &p, double timesum);
 // markers.hip
 __global__ void k_movemark(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, double timestep, int need_eii);
 __global__ void k_ronu_markers(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, Acc A, HeatK hk,
-                               double timestep, int need_eii);
+                               double timestep, double timesum, int need_eii, const double *tref);
+__global__ void k_tref(const i2e_params *p, Grid g, Markers mk, const int *slot0, double *tref);
 __global__ void k_ronu_nodes(const i2e_params *p, NodeF F, Acc A, int N, double *mm);
 __global__ void k_contrast(const i2e_params *p, NodeF F, int N, const double *mm);
 __global__ void k_tbc(NodeF F, BcTab bc, int N);
@@ -58,7 +61,7 @@ __global__ void k_scan_blocks(int n, const int *in, int *out, int *bsum);
 __global__ void k_scan_sums(int nb, int *bsum);
 __global__ void k_scan_add(int n, const int *bsum, int *out);
 __global__ void k_mscatter(Markers src, Markers dst, int64_t M, const int *key, int *off, const int *orig,
-                           int *orig2);
+                           int *orig2, int *slot0);
 __global__ void k_munperm(Markers src, Markers dst, int64_t M, const int *orig);
 
 }  // namespace i2e

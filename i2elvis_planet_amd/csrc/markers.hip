@@ -222,8 +222,38 @@ __device__ void porosity_update_d(const i2e_params *p, float &por, float markk, 
 // Scatter contributions of one marker: per corner (a,b) of its cell the ten
 // weighted sums of markmart.c:840-911, plus the cell-centre weights of nd.
 
+// soft-turbulence conductivity (markmart.c:503-610): the heat flux of soft
+// turbulence, q = 0.089 k^(2/3) |dT|^(4/3) (rho^2 beta g cp / eta)^(1/3),
+// with the fluid's viscosity eta_f, and the conductivity that gives the same
+// q with the numerical viscosity mnu; kt itself when q is tiny.  dT is the
+// float difference of the marker temperature and marker 0's.
+__device__ double soft_turb_k(double kt, float dTf, double mro, double bb, double mga, double mcp, double eta_f,
+                              double mnu) {
+  const double heat_flux = 0.089 * pow(kt, (2.000 / 3.000)) * pow((double)(dTf >= 0 ? dTf : -dTf), (4.000 / 3.000)) *
+                           pow((mro * mro * bb * mga * mcp / eta_f), (1.000 / 3.000));
+  if (heat_flux <= 0.0001) return kt;
+  return pow((heat_flux / 0.089), (3.000 / 2.000)) / (pow((double)dTf, 2.000) * mro) *
+         pow((bb * mga * mcp / mnu), (-1.000 / 2.000));
+}
+
+// the temperature of the caller's marker 0 as the marker loop of ronurecalc
+// sees it after that marker's own turn (sticky air reset to tmp_ambient,
+// markmart.c:296-309): the reference temperature of the soft-turbulence dT
+__global__ void k_tref(const i2e_params *__restrict__ p, Grid g, Markers mk, const int *slot0, double *tref) {
+  if (blockIdx.x || threadIdx.x) return;
+  const int i = slot0 ? *slot0 : 0;
+  const int t = mk.t[i];
+  const int mm2 = t >= 100 ? t - 100 : t;
+  float fk = mk.k[i];
+  const float fx = mk.x[i], fy = mk.y[i];
+  if (fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize && fk > 0 && t < 50 && mm2 < 2)
+    fk = (float)p->tmp_ambient;
+  *tref = (double)fk;
+}
+
 __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
-                                                     int64_t M, Acc A, HeatK hk, double timestep, int need_eii) {
+                                                     int64_t M, Acc A, HeatK hk, double timestep, double timesum,
+                                                     int need_eii, const double *__restrict__ tref) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int lane = threadIdx.x & 63;
   // whole wavefronts iterate together: the scatter reduces across lanes
@@ -243,7 +273,7 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
       if (act) {
         const double x = (double)fx, y = (double)fy;
         Ev e;
-        allintere_d(g, F, x, y, e, need_eii != 0);
+        allintere_d(g, F, x, y, e, need_eii != 0, p->soft_turb != 0);
         double mpb = e.pr * 1.0000e-5, mtk = (double)fk;
         if (mm2 < 2) {
           fk = (float)p->tmp_ambient;
@@ -277,10 +307,12 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
         }
         double mcp = r.cp;
         double mkt = (r.kt + r.kf / (mtk + 77.0)) * exp(r.kp * mpb);
+        double xmelt = 0;  // the marker's own silicate melt fraction
         if (p->melt_on && p->meltable[mm2 < 20 ? mm2 : mm2 - 20]) {
           double xm = (mtk - p->melt_ts) / (p->melt_tl - p->melt_ts);
           if (xm < 0.0) xm = 0.0;
           if (xm > 1.0) xm = 1.0;
+          xmelt = xm;
           int tnew = t;
           if (xm > 0 && tnew < 20) tnew += 20;
           if (xm <= 0 && tnew > 20) tnew -= 20;
@@ -306,6 +338,26 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
             }
           }
         }
+        if (p->soft_turb && timesum > p->core_form_time) {
+          const float dTf = fk - (float)*tref;
+          if (mm2 == 25 || mm2 == 26) {
+            // crystal mush between 40 and 60 % melt, melt above
+            double eta_f = -1.0;
+            if (xmelt >= 0.4000 && xmelt <= 0.6000) eta_f = r.n0 * exp((-172.690) * (xmelt - 0.400));
+            if (xmelt > 0.6000) eta_f = 100.0000;
+            if (eta_f > 0) mkt = soft_turb_k(r.kt, dTf, mro, r.bb, e.ga, mcp, eta_f, mnu);
+          }
+          if (p->is_iron[mm2]) {
+            double xfe = 0;
+            if (p->fe_tl > 0) {
+              xfe = (mtk - p->fe_ts) / (p->fe_tl - p->fe_ts);
+              if (xfe < 0.000) xfe = 0.0000;
+              if (xfe > 1.000) xfe = 1.0000;
+            }
+            // solid iron 1e12 Pa s, molten iron 1e-2 Pa s
+            mkt = soft_turb_k(r.kt, dTf, mro, r.bb, e.ga, mcp, xfe < 1.0000 ? 1.0000e12 : 0.010, mnu);
+          }
+        }
         if (p->porous[mm2]) {
           double kt0 = r.kt, pr0 = (double)por;
           if (por < 0.200f) mkt = kt0 * exp(-pr0 / 0.080);
@@ -315,6 +367,52 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
         }
         if (mkt <= 0.0010) mkt = 0.0010;
         if (mkt >= p->k_cutoff) mkt = p->k_cutoff;
+        if (p->magnet) {
+          // Curie temperatures of magnetite (851.16 K) and hematite (948.16 K);
+          // states 0 none, 1 hematite, 2 both, 3 cannot magnetise
+          const int old = mk.mg[mm1];
+          const bool sil = mm2 == 5 || mm2 == 6;
+          int mag = 3;
+          double mag_time = mk.mgt[mm1];
+          const double now = timesum / (3600.000 * 24.000 * 365.250 * 1.000e+6);
+          if (p->dynamo == 0) {
+            if (sil) {
+              if (mtk <= 851.16 && old == 2)
+                mag = 2;
+              else if ((mtk > 851.16 && mtk <= 948.16) && old == 2)
+                mag = 1;
+              else if (mtk <= 851.16 && old == 1)
+                mag = 1;
+              else if ((mtk > 851.16 && mtk <= 948.16) && old == 1)
+                mag = 1;
+              else if (mtk > 948.16) {
+                mag = 0;
+                mag_time = -1.000;
+              }
+            } else {
+              mag = 3;
+              mag_time = -1.000;
+            }
+          } else if (p->dynamo == 1) {
+            if (sil) {
+              if (mtk <= 851.16) {
+                mag = 2;
+                if (old <= 1 || old == 3) mag_time = now;
+              } else if (mtk > 851.16 && mtk <= 948.16) {
+                mag = 1;
+                if (old == 0 || old == 3) mag_time = now;
+              } else {
+                mag = 0;
+                mag_time = -1.000;
+              }
+            } else {
+              mag = 3;
+              mag_time = -1.000;
+            }
+          }
+          mk.mg[mm1] = (int8_t)mag;
+          mk.mgt[mm1] = mag_time;
+        }
         // radiogenic heating per kg is the same for every marker of a step
         // (markmart.c:758-785): formed once on the host, times the density
         double mht = 0;

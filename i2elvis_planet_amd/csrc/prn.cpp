@@ -217,7 +217,15 @@ int i2e_prn_write(const char *path, const i2e_params *p, const i2e_host_state *s
                           in ? s->markv[i] : 0.0f, 0.0f,                     in ? s->markpor[i] : 0.0f,
                           0.0f,                    in ? s->marktmax[i] : 0.0f, 0.0f};
     w.put(rec, sizeof rec);
-    const int8_t tail[3] = {0, 0, in ? s->markt[i] : (int8_t)0};
+    // magnetisation state and its time in Ma, each one byte as the saver
+    // casts them (loadmart.c:805-806)
+    int8_t mg = 0, mgt = 0;
+    if (in && s->markmg_old) mg = s->markmg_old[i];
+    if (in && s->markmg_time) {
+      const double t = s->markmg_time[i];
+      mgt = (int8_t)(t >= 127.0 ? 127 : t <= -128.0 ? -128 : (int)t);
+    }
+    const int8_t tail[3] = {mg, mgt, in ? s->markt[i] : (int8_t)0};
     w.put(tail, 3);
   }
   const bool ok = w.ok;
@@ -328,6 +336,10 @@ int i2e_prn_read(const char *path, i2e_params *p, i2e_host_state *s, int *n0) {
     s->marktmax[i] = rec[7];
     s->markt[i] = tail[2];
     s->markp[i] = 0.0f;
+    // the bytes as written (the reference's binary loader assigns the
+    // accretion time it just read to both, loadmart.c:533-534)
+    if (s->markmg_old) s->markmg_old[i] = tail[0];
+    if (s->markmg_time) s->markmg_time[i] = tail[1];
   }
   r.skip(39);  // the record one past the last marker
   const bool ok = r.ok;

@@ -91,8 +91,9 @@ __global__ void k_scan_add(int n, const int *bsum, int *out) {
 // move every marker to its slot in key order (off: the exclusive scan of the
 // counts, advanced atomically; the order inside one cell is the order the
 // atomics happen to take).  orig2[slot] = the caller's index of the marker.
+// slot0 receives the slot of the caller's marker 0.
 __global__ void k_mscatter(Markers src, Markers dst, int64_t M, const int *key, int *off, const int *orig,
-                           int *orig2) {
+                           int *orig2, int *slot0) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
     const int d = atomicAdd(&off[key[i]], 1);
     dst.x[d] = src.x[i];
@@ -103,7 +104,11 @@ __global__ void k_mscatter(Markers src, Markers dst, int64_t M, const int *key, 
     dst.por[d] = src.por[i];
     dst.tmax[d] = src.tmax[i];
     dst.t[d] = src.t[i];
-    orig2[d] = orig ? orig[i] : (int)i;
+    dst.mg[d] = src.mg[i];
+    dst.mgt[d] = src.mgt[i];
+    const int o = orig ? orig[i] : (int)i;
+    orig2[d] = o;
+    if (o == 0) *slot0 = d;
   }
 }
 
@@ -119,6 +124,8 @@ __global__ void k_munperm(Markers src, Markers dst, int64_t M, const int *orig) 
     dst.por[j] = src.por[i];
     dst.tmax[j] = src.tmax[i];
     dst.t[j] = src.t[i];
+    dst.mg[j] = src.mg[i];
+    dst.mgt[j] = src.mgt[i];
   }
 }
 

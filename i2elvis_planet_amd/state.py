@@ -23,6 +23,9 @@ class HostState:
         for f in abi.MARKER_FIELDS_F32:
             setattr(self, f, np.zeros(marknum, np.float32))
         self.markt = np.zeros(marknum, np.int8)
+        # magnetisation state and time (markmart.c:651-744; in2mart.c sets 0 and -1)
+        self.markmg_old = np.zeros(marknum, np.int8)
+        self.markmg_time = np.full(marknum, -1.0)
         self.bondm = np.zeros(4 * N, np.int64)
         self.bondn = np.zeros((bondnum, 3), np.int64)
         self.bondv = np.zeros((bondnum, 4))
@@ -60,6 +63,9 @@ class HostState:
             setattr(s, f, a.ctypes.data_as(fp))
         assert self.markt.dtype == np.int8
         s.markt = self.markt.ctypes.data_as(C.POINTER(C.c_int8))
+        assert self.markmg_old.dtype == np.int8 and self.markmg_time.dtype == np.float64
+        s.markmg_old = self.markmg_old.ctypes.data_as(C.POINTER(C.c_int8))
+        s.markmg_time = self.markmg_time.ctypes.data_as(dp)
         s.marknum = self.marknum
         self.bondn = np.ascontiguousarray(self.bondn, np.int64)
         self.bondv = np.ascontiguousarray(self.bondv, np.float64)

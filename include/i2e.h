@@ -106,6 +106,21 @@ typedef struct i2e_params {
   uint8_t is_iron[I2E_MAXTYPES];      /* 60Fe carrier (7-10,17-19)            */
   uint8_t porous[I2E_MAXTYPES];       /* porosity closures (5,6)              */
   uint8_t meltable[I2E_MAXTYPES];     /* melt switch applies                  */
+  /* soft-turbulence conductivity (markmart.c:503-610): once timesum >
+   * core_form_time, molten silicate (types 25, 26) with melt fraction >= 0.4
+   * and iron (7-10, 17-19) get the conductivity that makes the
+   * soft-turbulence heat flux with the numerical viscosity equal to the one
+   * with the fluid's (crystal mush, 1e2, 1e12 or 1e-2 Pa s).  The melt
+   * fractions are the marker's own (linear silicate closure above; iron
+   * between fe_ts and fe_tl, 0 if fe_tl is 0).                            */
+  double core_form_time, fe_ts, fe_tl;
+  int32_t soft_turb;
+  /* magnetisation of solid silicates (markmart.c:651-744): markmg_old and
+   * markmg_time of the host state follow the Curie temperatures of
+   * magnetite (851.16 K) and hematite (948.16 K); dynamo = 1 while the
+   * planetary dynamo runs (decided by core.c in the reference).           */
+  int32_t magnet, dynamo;
+  int32_t pad1;
 } i2e_params;
 
 /* ---- host state: the reference globals as raw pointers ------------------ */
@@ -128,6 +143,10 @@ typedef struct i2e_host_state {
   int64_t bondnum;
   /* scalars                                                                 */
   double timestep, timesum, gxkoef;
+  /* magnetisation per marker (headmart.c markmg_old / markmg_time; NULL: not
+   * kept by the caller, the device starts them at 0 and -1 Ma)              */
+  int8_t *markmg_old;
+  double *markmg_time;
 } i2e_host_state;
 
 /* ---- solver controls ---------------------------------------------------- */

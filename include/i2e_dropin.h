@@ -27,7 +27,7 @@
  *   double gx[], gy[], vx[], vy[], pr[], gp[], ga[], exx[], eyy[], exy[],
  *          sxx[], syy[], sxy[], nu[], nd[], ro[], tk[], cp[], kt[], ht[], et[];
  *   float  markx[], marky[], markk[], markv[], markp[], markpor[], marktmax[];
- *   char   markt[];
+ *   char   markt[], markmg_old[];   double markmg_time[];
  *   long   bondm[], bondn[][3];  double bondv[][4];
  *   long   xnumx, ynumy, mnumx, mnumy, marknum, bondnum;
  *   double marknu[], markdh[], markdv[], markss[], markmm[], markn0[], markn1[],
@@ -36,7 +36,7 @@
  *   double xsize, ysize, GXKOEF, GYKOEF, tmp_ambient, al2627_init, fe6056_init,
  *          maxxystep, maxtkstep, maxtmstep, timestep, timesum, nubeg, nuend,
  *          nucontr, strmin, heatdif, frictyn, adiabyn, por_init, k_cutoff;
- *   int    heatmod, densimod, outgrid, markmod;
+ *   int    heatmod, densimod, outgrid, markmod, dynamo;   double core_form_time;
  *
  * Errors: like the reference (gausmart.c:271-304, exit on a failed solve),
  * a failing call prints the library's message to stderr and exits the

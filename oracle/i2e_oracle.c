@@ -432,9 +432,19 @@ typedef struct Acc {
   double *ro0, *et0, *cp0, *kt0, *ht0, *tk0, *nu0, *nd0, *w0, *w1, *wnu, *wnd;
 } Acc;
 
+/* conductivity from the soft-turbulence heat flux (markmart.c:520-535) */
+static double soft_turb_k(double kt, float dTf, double mro, double bb, double mga, double mcp, double eta_f,
+                          double mnu) {
+  double heat_flux = 0.089 * pow(kt, (2.000 / 3.000)) * pow((double)ABSV(dTf), (4.000 / 3.000)) *
+                     pow((mro * mro * bb * mga * mcp / eta_f), (1.000 / 3.000));
+  if (heat_flux <= 0.0001) return kt;
+  return pow((heat_flux / 0.089), (3.000 / 2.000)) / (pow((double)dTf, 2.000) * mro) *
+         pow((bb * mga * mcp / mnu), (-1.000 / 2.000));
+}
+
 static void marker_props_scatter(const i2e_params *p, i2e_host_state *s, const G *g, int64_t mm1, Acc *A) {
   int mm2 = (int)s->markt[mm1];
-  double mpb, mtk, mnu, celdx, celdy, mro, mbb, mcp, mkt, mht = 0, radmax, timesumA;
+  double mpb, mtk, mnu, celdx, celdy, mro, mbb, mcp, mkt, mht = 0, radmax, timesumA, xmelt;
   Ev e;
   W4 w;
   if (mm2 >= 100) mm2 -= 100;
@@ -466,11 +476,13 @@ static void marker_props_scatter(const i2e_params *p, i2e_host_state *s, const G
   mcp = p->rock[mm2].cp;
   mkt = (p->rock[mm2].kt + p->rock[mm2].kf / (mtk + 77.0)) * exp(p->rock[mm2].kp * mpb);
   /* melting closure in place of tdbasecalc (markmart.c:2178-2220) */
+  xmelt = 0;
   if (p->melt_on && p->meltable[mm2 < 20 ? mm2 : mm2 - 20]) {
     double xm = (mtk - p->melt_ts) / (p->melt_tl - p->melt_ts), xp, xq;
     int tnew;
     if (xm < 0.0) xm = 0.0;
     if (xm > 1.0) xm = 1.0;
+    xmelt = xm;
     tnew = s->markt[mm1];
     if (xm > 0 && tnew < 20) tnew += 20;
     if (xm <= 0 && tnew > 20) tnew -= 20;
@@ -498,6 +510,28 @@ static void marker_props_scatter(const i2e_params *p, i2e_host_state *s, const G
       }
     }
   }
+  /* soft-turbulence conductivity, markmart.c:503-610.  The melt fractions
+   * are the marker's own (the reference reads eps[21]/eps[50] as the last
+   * melt-fraction call left them); dT against marker 0's temperature. */
+  if (p->soft_turb && s->timesum > p->core_form_time) {
+    const float dTf = s->markk[mm1] - s->markk[0];
+    const i2e_rock *r = &p->rock[mm2];
+    if (mm2 == 25 || mm2 == 26) {
+      double eta_f = -1.0;
+      if (xmelt >= 0.4000 && xmelt <= 0.6000) eta_f = r->n0 * exp((-172.690) * (xmelt - 0.400));
+      if (xmelt > 0.6000) eta_f = 100.0000;
+      if (eta_f > 0) mkt = soft_turb_k(r->kt, dTf, mro, r->bb, e.ga, mcp, eta_f, mnu);
+    }
+    if (mm2 == 7 || mm2 == 8 || mm2 == 9 || mm2 == 10 || mm2 == 17 || mm2 == 18 || mm2 == 19) {
+      double xfe = 0;
+      if (p->fe_tl > 0) {
+        xfe = (mtk - p->fe_ts) / (p->fe_tl - p->fe_ts);
+        if (xfe < 0.000) xfe = 0.0000;
+        if (xfe > 1.000) xfe = 1.0000;
+      }
+      mkt = soft_turb_k(r->kt, dTf, mro, r->bb, e.ga, mcp, xfe < 1.0000 ? 1.0000e12 : 0.010, mnu);
+    }
+  }
   /* porosity-dependent conductivity, markmart.c:612-638 */
   if (p->porous[mm2]) {
     double por_ref1 = 0.080, por_ref2 = 0.167, a_fact = -1.200, kt0 = p->rock[mm2].kt, pr0 = s->markpor[mm1];
@@ -508,6 +542,50 @@ static void marker_props_scatter(const i2e_params *p, i2e_host_state *s, const G
   }
   if (mkt <= 0.0010) mkt = 0.0010;
   if (mkt >= p->k_cutoff) mkt = p->k_cutoff;
+  /* magnetisation of solid silicates, markmart.c:651-744 (a state the
+   * rules do not assign keeps the marker's own time; the reference keeps
+   * the previous marker's) */
+  if (p->magnet && s->markmg_old && s->markmg_time) {
+    int mag = 3, old = s->markmg_old[mm1], sil = (mm2 == 5 || mm2 == 6);
+    double mag_time = s->markmg_time[mm1];
+    if (p->dynamo == 0) {
+      if (sil) {
+        if (mtk <= 851.16 && old == 2)
+          mag = 2;
+        else if ((mtk > 851.16 && mtk <= 948.16) && old == 2)
+          mag = 1;
+        else if (mtk <= 851.16 && old == 1)
+          mag = 1;
+        else if ((mtk > 851.16 && mtk <= 948.16) && old == 1)
+          mag = 1;
+        else if (mtk > 948.16) {
+          mag = 0;
+          mag_time = -1.000;
+        }
+      } else {
+        mag = 3;
+        mag_time = -1.000;
+      }
+    } else if (p->dynamo == 1) {
+      if (sil) {
+        if (mtk <= 851.16) {
+          mag = 2;
+          if (old <= 1 || old == 3) mag_time = s->timesum / (3600.000 * 24.000 * 365.250 * 1.000e+6);
+        } else if (mtk > 851.16 && mtk <= 948.16) {
+          mag = 1;
+          if (old == 0 || old == 3) mag_time = s->timesum / (3600.000 * 24.000 * 365.250 * 1.000e+6);
+        } else {
+          mag = 0;
+          mag_time = -1.000;
+        }
+      } else {
+        mag = 3;
+        mag_time = -1.000;
+      }
+    }
+    s->markmg_old[mm1] = (int8_t)mag;
+    s->markmg_time[mm1] = mag_time;
+  }
   /* radiogenic heating, markmart.c:747-831 */
   timesumA = s->timesum / (365.250 * 24.000 * 3600.000);
   if (mm2 == 0) mht = 0.0000e-10;

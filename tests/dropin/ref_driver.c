@@ -31,7 +31,8 @@ double exx[DMAXNOD], eyy[DMAXNOD], exy[DMAXNOD], sxx[DMAXNOD], syy[DMAXNOD], sxy
 double nu[DMAXNOD], nd[DMAXNOD], ro[DMAXNOD], tk[DMAXNOD], cp[DMAXNOD], kt[DMAXNOD], ht[DMAXNOD], et[DMAXNOD];
 float markx[DMAXMRK], marky[DMAXMRK], markk[DMAXMRK], markv[DMAXMRK], markp[DMAXMRK], markpor[DMAXMRK],
     marktmax[DMAXMRK];
-char markt[DMAXMRK];
+char markt[DMAXMRK], markmg_old[DMAXMRK];
+double markmg_time[DMAXMRK];
 long int bondm[DMAXBON], bondn[DMAXBON][3];
 double bondv[DMAXBON][4];
 long int xnumx, ynumy, mnumx, mnumy, marknum, bondnum;
@@ -44,16 +45,21 @@ double xsize, ysize, GXKOEF, GYKOEF, tmp_ambient, al2627_init, fe6056_init;
 double maxxystep, maxtkstep, maxtmstep, timestep, timesum;
 double nubeg, nuend, nucontr, strmin, heatdif, frictyn, adiabyn, por_init, k_cutoff;
 int heatmod, densimod, outgrid, markmod;
+double core_form_time;
+int dynamo;
 
 static double radmax_frac = 0.95, longlived = 1.0, h0_scale = 1.0, stokes_rtol = 1e-10, heat_rtol = 1e-12,
               grav_rtol = 1e-12;
 static int porosity_on = 1;
+static double fe_ts = 0, fe_tl = 0;
 
 static void hook(i2e_params *p, i2e_solver_opts *o) {
   p->radmax_frac = radmax_frac;
   p->longlived = longlived;
   p->h0_scale = h0_scale;
   p->porosity_on = porosity_on;
+  p->fe_ts = fe_ts;
+  p->fe_tl = fe_tl;
   o->stokes_rtol = stokes_rtol;
   o->heat_rtol = heat_rtol;
   o->grav_rtol = grav_rtol;
@@ -74,7 +80,8 @@ static void read_mode(const char *path) {
     }
     D(maxxystep) D(maxtkstep) D(maxtmstep) D(nubeg) D(nuend) D(nucontr) D(strmin) D(heatdif) D(frictyn)
     D(adiabyn) D(por_init) D(k_cutoff) D(timestep) D(radmax_frac) D(longlived) D(h0_scale) D(stokes_rtol)
-    D(heat_rtol) D(grav_rtol) I(heatmod) I(densimod) I(outgrid) I(markmod) I(porosity_on)
+    D(heat_rtol) D(grav_rtol) I(heatmod) I(densimod) I(outgrid) I(markmod) I(porosity_on) D(core_form_time)
+    I(dynamo) D(fe_ts) D(fe_tl)
     else {
       fprintf(stderr, "unknown mode parameter %s\n", name);
       exit(21);
@@ -107,6 +114,7 @@ static void load(const char *path) {
   s.marknum = h.marknum;
   s.markx = markx, s.marky = marky, s.markk = markk, s.markv = markv, s.markp = markp;
   s.markpor = markpor, s.marktmax = marktmax, s.markt = (int8_t *)markt;
+  s.markmg_old = (int8_t *)markmg_old, s.markmg_time = markmg_time;
   s.bondm = (int64_t *)bondm, s.bondn = (int64_t *)&bondn[0][0], s.bondv = &bondv[0][0];
   s.bondnum = h.bondnum;
   if (i2e_prn_read(path, &p, &s, &n0)) {
@@ -168,6 +176,8 @@ int main(int argc, char **argv) {
   fwrite(marky, sizeof(float), marknum, f);
   fwrite(markk, sizeof(float), marknum, f);
   fwrite(markt, 1, marknum, f);
+  fwrite(markmg_old, 1, marknum, f);
+  fwrite(markmg_time, sizeof(double), marknum, f);
   fwrite(&marknum, sizeof marknum, 1, f);
   fwrite(&timestep, sizeof timestep, 1, f);
   fwrite(&timesum, sizeof timesum, 1, f);

@@ -32,6 +32,8 @@ MODE_KEYS = ("maxxystep", "maxtkstep", "maxtmstep", "nubeg", "nuend", "nucontr",
 
 def write_case(tmp_path, name="c1", rtol=(1e-13, 1e-13, 1e-13)):
     p, hs, _ = configs.make(name)
+    for t in (6, 7, 26):
+        p.rock[t].bb = 3e-5  # thermal expansion drives the soft-turbulence flux
     path = tmp_path / "state.prn"
     prn.write(path, p, hs, n0=0)
     q, back, _ = prn.read(path, params=p)
@@ -41,6 +43,11 @@ def write_case(tmp_path, name="c1", rtol=(1e-13, 1e-13, 1e-13)):
     lines += ["radmax_frac %r" % p.radmax_frac, "longlived %r" % p.longlived, "h0_scale %r" % p.h0_scale,
               "porosity_on %d" % p.porosity_on, "stokes_rtol %r" % rtol[0], "heat_rtol %r" % rtol[1],
               "grav_rtol %r" % rtol[2]]
+    # the soft-turbulence conductivity and the magnetisation run as in the
+    # reference (the shim switches both on); the oracle gets the same
+    q.soft_turb, q.magnet, q.core_form_time, q.dynamo = 1, 1, 0.0, 1
+    q.fe_ts, q.fe_tl = 400.0, 450.0
+    lines += ["core_form_time 0", "dynamo 1", "fe_ts 400", "fe_tl 450"]
     mode.write_text("\n".join(lines) + "\n")
     return q, back, path, mode
 
@@ -57,6 +64,10 @@ def read_out(path, N, M):
         o += 4 * M
     f["markt"] = np.frombuffer(b, np.int8, M, o)
     o += M
+    f["markmg_old"] = np.frombuffer(b, np.int8, M, o)
+    o += M
+    f["markmg_time"] = np.frombuffer(b, np.float64, M, o)
+    o += 8 * M
     f["marknum"], f["timestep"], f["timesum"], f["gxkoef"] = struct.unpack_from("<q3d", b, o)
     assert o + 32 == len(b)
     return f
@@ -104,6 +115,8 @@ def test_dropin_steps_match_the_oracle(orc, tmp_path):
         print(k, "rel L2 %.2e" % e, file=sys.stderr)
         assert e < 1e-8, k
     assert np.array_equal(got["markt"], ref.markt)
+    assert np.array_equal(got["markmg_old"], ref.markmg_old)
+    assert np.array_equal(got["markmg_time"], ref.markmg_time)
     g = ref.copy()
     g.markx[:], g.marky[:], g.markt[:] = got["markx"], got["marky"], got["markt"]
     assert np.array_equal(orc.Oracle(q, g).type_counts(), o.type_counts())

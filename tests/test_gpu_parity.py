@@ -116,6 +116,63 @@ def test_ronurecalc_parity(orc, name, kw):
     np.testing.assert_allclose(got.marktmax, ref.marktmax, rtol=0)
 
 
+def shuffle_markers(hs, rng):
+    perm = rng.permutation(hs.marknum)
+    for f in ("markx", "marky", "markk", "markv", "markp", "markpor", "marktmax", "markt", "markmg_old",
+              "markmg_time"):
+        a = getattr(hs, f)
+        a[:] = a[perm]
+
+
+@pytest.mark.parametrize("dynamo", [0, 1])
+def test_ronurecalc_soft_turbulence_and_magnetisation(orc, dynamo):
+    """ronurecalc with the soft-turbulence conductivity (markmart.c:508-610)
+    and the magnetisation states (markmart.c:651-744) on: mixed silicate,
+    molten and iron types over 150-1900 K (both Curie temperatures, both melt
+    windows), random previous states and times, markers shuffled and then
+    sorted by cell on the device (the temperature difference is taken
+    against the caller's marker 0 wherever the sort put it)."""
+    say("case ronurecalc soft_turb magnet dynamo", dynamo)
+    p, hs, _ = small_cfg("c3", n=101, size=120e3, radius=30e3)
+    p.soft_turb, p.core_form_time, p.magnet, p.dynamo = 1, 0.0, 1, dynamo
+    p.fe_ts, p.fe_tl = 1400.0, 1450.0
+    p.k_cutoff = 1e12
+    p.meltable[5] = 1
+    for t in (5, 6, 7, 17, 25, 26):
+        p.rock[t].bb = 3e-5
+    rng = np.random.default_rng(9)
+    inside = hs.markt != configs.AIR
+    pick = rng.choice(np.array([5, 6, 7, 17, 25, 26], np.int8), size=hs.marknum)
+    hs.markt[inside] = pick[inside]
+    hs.markk[:] = rng.uniform(150, 1900, hs.marknum).astype(np.float32)
+    hs.markmg_old[:] = rng.integers(0, 4, hs.marknum)
+    hs.markmg_time[:] = rng.uniform(0, 5, hs.marknum)
+    hs.pr[:] = 1e7 * rng.random(hs.N)
+    hs.ga[:] = rng.random(hs.N)
+    hs.timestep = 7.9e10
+    hs.timesum = 3e12
+    shuffle_markers(hs, rng)
+    ref = hs.copy()
+    ctx = Context(p, hs, solver=fp64_opts())
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.sort_markers()
+    ctx.ronurecalc()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    orc.Oracle(p, ref).ronurecalc()
+    assert np.array_equal(got.markt, ref.markt)
+    assert np.array_equal(got.markk, ref.markk)
+    assert np.array_equal(got.markmg_old, ref.markmg_old)
+    assert np.array_equal(got.markmg_time, ref.markmg_time)
+    for f in ("nu", "nd", "ro", "cp", "kt", "ht", "tk", "et"):
+        assert rel(getattr(got, f), getattr(ref, f)) < 1e-10, f
+    # the model changed the conductivity of molten and iron nodes
+    p.soft_turb = 0
+    off = hs.copy()
+    orc.Oracle(p, off).ronurecalc()
+    assert rel(off.kt, ref.kt) > 1e-3
+
+
 # ------------------------------------------------------------------ operators
 def _dev(x):
     return DeviceArray(x)

@@ -400,3 +400,114 @@ def test_stabilisation_rows_on_a_linear_density_field(orc):
                 assert D[r].nnz == 5
                 ny += 1
     assert nx > 50 and ny > 50
+
+
+# ------------------------------------------------------------------ soft turbulence, magnetisation
+def soft_state(T, start_type, bb=3e-5, ga=0.7, melt_eta=1e4):
+    p, hs = melt_state(T, start_type=start_type)
+    p.soft_turb = 1
+    p.core_form_time = 0.0
+    p.melt_eta = melt_eta
+    for t in (start_type, configs.MOLTEN_SILICATE):
+        p.rock[t].bb = bb
+    hs.ga[:] = ga
+    hs.timesum = 1e10
+    return p, hs
+
+
+@pytest.mark.parametrize("T,eta_f", [(1800.0, 100.0), (1605.0, 1e2 * math.exp(-172.690 * 0.1))])
+def test_soft_turbulence_matches_the_fluid_heat_flux(orc, T, eta_f):
+    """The soft-turbulence conductivity keeps the heat flux
+    q = 0.089 k^(2/3) |dT|^(4/3) (rho^2 beta g cp / eta)^(1/3) of the fluid
+    (eta_f: 1e2 Pa s above 60 % melt, the crystal mush n0 exp(-172.69 (x -
+    0.4)) between 40 and 60 %) when evaluated with the numerical viscosity:
+    solving q(k, eta_num) = q(k0, eta_f) for k gives k = k0 sqrt(eta_num /
+    eta_f), whatever dT, rho, beta, g and cp are.  eta_num is the melt's
+    1e4 Pa s here, k0 the molten type's 1.5 W/m/K."""
+    p, hs = soft_state(T, configs.MOLTEN_SILICATE)
+    ronu(orc, p, hs)
+    pure = pure_nodes(hs, configs.MOLTEN_SILICATE)
+    assert pure.sum() >= 20
+    assert np.allclose(hs.nu[pure], 1e4, rtol=1e-13)
+    assert np.allclose(hs.kt[pure], 1.5 * math.sqrt(1e4 / eta_f), rtol=1e-12)
+
+
+def test_soft_turbulence_limits(orc):
+    """No thermal expansion means no flux (q <= 1e-4 W/m^2): k is the molten
+    type's own 1.5 W/m/K instead of the melt closure's linear mix; below 40 %
+    melt, before core_form_time, or with the model off, the mix stays."""
+    T = 1800.0
+    xm = (T - 1360.0) / 490.0
+    mix = 3.0 * (1 - xm) + 1.5 * xm
+    for bb, cft, on, want in ((0.0, 0.0, 1, 1.5), (3e-5, 1e11, 1, mix), (3e-5, 0.0, 0, mix)):
+        p, hs = soft_state(T, configs.MOLTEN_SILICATE, bb=bb)
+        p.core_form_time, p.soft_turb = cft, on
+        ronu(orc, p, hs)
+        pure = pure_nodes(hs, configs.MOLTEN_SILICATE)
+        assert np.allclose(hs.kt[pure], want, rtol=1e-13), (bb, cft, on)
+    p, hs = soft_state(1409.0, configs.MOLTEN_SILICATE)
+    ronu(orc, p, hs)
+    xm = (1409.0 - 1360.0) / 490.0
+    pure = pure_nodes(hs, configs.MOLTEN_SILICATE)
+    assert np.allclose(hs.kt[pure], 3.0 * (1 - xm) + 1.5 * xm, rtol=1e-13)
+
+
+@pytest.mark.parametrize("T,eta_f", [(1500.0, 0.01), (1420.0, 1e12)])
+def test_soft_turbulence_of_iron(orc, T, eta_f):
+    """Iron takes 1e-2 Pa s when fully molten (melt fraction between fe_ts =
+    1400 K and fe_tl = 1450 K reaching 1) and 1e12 Pa s otherwise, again
+    k = k0 sqrt(eta_num / eta_f) with the iron's Arrhenius viscosity."""
+    p, hs = soft_state(T, configs.IRON)
+    p.fe_ts, p.fe_tl = 1400.0, 1450.0
+    p.k_cutoff = 1e12  # 3 sqrt(1e17 / 1e-2) = 9.5e9 W/m/K
+    ronu(orc, p, hs)
+    pure = pure_nodes(hs, configs.IRON)
+    assert pure.sum() >= 20
+    eta = min(max(configs.NU0_ARRH * math.exp(configs.E_ACT / (8.314 * T)), 1e17), 1e23)  # the type's n0, n1
+    assert np.allclose(hs.nu[pure], eta, rtol=1e-12)
+    assert np.allclose(hs.kt[pure], 3.0 * math.sqrt(eta / eta_f), rtol=1e-12)
+
+
+# (band, old state) -> new state, with the dynamo off and on: the Curie
+# temperatures of magnetite (851.16 K) and hematite (948.16 K) split solid
+# silicate temperatures into bands 0, 1, 2.  With the dynamo off a state can
+# only be kept or lose magnetite; with it on, the band decides.
+MAG_OFF = {(0, 2): 2, (1, 2): 1, (0, 1): 1, (1, 1): 1, (0, 0): 3, (1, 0): 3, (0, 3): 3, (1, 3): 3}
+MAG_NEW_TIME_ON = {0: (0, 1, 3), 1: (0, 3)}  # band -> old states that start a new magnetisation
+
+
+@pytest.mark.parametrize("dynamo", [0, 1])
+def test_magnetisation_state_table(orc, dynamo):
+    """markmg_old/markmg_time of solid silicates after one ronurecalc, for
+    every (temperature band, previous state) pair, checked against the table
+    above; other types always end in state 3 with time -1, and a state the
+    rules leave alone keeps the marker's own time."""
+    p, hs = melt_state(800.0)
+    p.magnet, p.dynamo = 1, dynamo
+    rng = np.random.default_rng(8)
+    M = hs.marknum
+    sil = hs.markt == configs.SILICATE
+    iron = rng.random(M) < 0.15
+    hs.markt[sil & iron] = configs.IRON
+    Tb = np.array([800.0, 900.0, 1000.0])
+    band = rng.integers(0, 3, M)
+    hs.markk[sil] = Tb[band[sil]].astype(np.float32)
+    old = rng.integers(0, 4, M).astype(np.int8)
+    t_old = rng.uniform(0.1, 5.0, M)
+    hs.markmg_old[:], hs.markmg_time[:] = old, t_old
+    hs.timesum = 4.2e13
+    now = 4.2e13 / (3600.0 * 24.0 * 365.25 * 1e6)
+    types0 = hs.markt.copy()
+    ronu(orc, p, hs)
+    for i in range(M):
+        if types0[i] not in (5, 6):
+            assert (hs.markmg_old[i], hs.markmg_time[i]) == (3, -1.0)
+            continue
+        b, o = int(band[i]), int(old[i])
+        if b == 2:
+            want = (0, -1.0)
+        elif dynamo == 0:
+            want = (MAG_OFF[(b, o)], t_old[i])
+        else:
+            want = (2 - b, now if o in MAG_NEW_TIME_ON[b] else t_old[i])
+        assert (hs.markmg_old[i], hs.markmg_time[i]) == want, (i, b, o)

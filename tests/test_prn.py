@@ -152,3 +152,21 @@ def test_immobile_rock_type_is_refused(tmp_path):
     bad.write_bytes(bytes(b))
     with pytest.raises(Exception, match="status 5.*immobile"):
         prn.read(bad, params=p)
+
+
+def test_magnetisation_bytes(tmp_path):
+    """markmg_old and markmg_time ride in the two bytes before the type, each
+    cast to char as the saver does (loadmart.c:805-806): the state exactly,
+    the time in Ma truncated toward zero and clamped to a byte."""
+    p, hs = state_with_fields(seed=4)
+    rng = np.random.default_rng(2)
+    hs.markmg_old[:] = rng.integers(0, 4, hs.marknum)
+    hs.markmg_time[:] = rng.uniform(-1.0, 5.0, hs.marknum)
+    hs.markmg_time[:3] = (-1.0, 300.0, 2.99)
+    path = tmp_path / "t.prn"
+    prn.write(path, p, hs, n0=1)
+    _, got, _ = prn.read(path, params=p)
+    assert np.array_equal(got.markmg_old, hs.markmg_old)
+    assert np.array_equal(got.markmg_time, np.clip(np.trunc(hs.markmg_time), -128, 127))
+    assert tuple(got.markmg_time[:3]) == (-1.0, 127.0, 2.0)
+
