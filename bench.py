@@ -274,8 +274,10 @@ def hot_attempt(ctx, o, hs_cold, cfg):
 def valu_summary(pmc_dir, rk):
     """VALU activity of roofline kernel `rk` from one rocprofv3 --pmc run of
     SQ_ACTIVE_INST_VALU SQ_INSTS_VALU SQ_WAVE_CYCLES GRBM_GUI_ACTIVE: VALU
-    busy = SQ_ACTIVE_INST_VALU * 4 / (SIMDs) / GRBM_GUI_ACTIVE (4 SIMDs per CU,
-    256 CUs), per dispatch, averaged."""
+    busy = SQ_ACTIVE_INST_VALU * 4 / SIMDs / (GRBM_GUI_ACTIVE / XCDs), per
+    dispatch, averaged (4 SIMDs per CU, 256 CUs; SQ_ACTIVE_INST_VALU counts
+    quad-cycles; GRBM_GUI_ACTIVE is summed over the 8 XCDs: 8.3x the
+    kernel's duration in 2.4 GHz cycles)."""
     import csv
     prefix, _ = PMC_MATCH[rk]
     rows = [r for r in csv.DictReader(open(os.path.join(pmc_dir, "run_counter_collection.csv")))
@@ -289,12 +291,12 @@ def valu_summary(pmc_dir, rk):
     busy, ipw = [], []
     for d in per.values():
         if d.get("GRBM_GUI_ACTIVE") and "SQ_ACTIVE_INST_VALU" in d:
-            busy.append(100.0 * d["SQ_ACTIVE_INST_VALU"] * 4 / simds / d["GRBM_GUI_ACTIVE"])
+            busy.append(100.0 * d["SQ_ACTIVE_INST_VALU"] * 4 / simds / (d["GRBM_GUI_ACTIVE"] / 8))
         if d.get("SQ_WAVE_CYCLES") and "SQ_INSTS_VALU" in d:
             ipw.append(d["SQ_INSTS_VALU"] / d["SQ_WAVE_CYCLES"])
     return "\n".join([
         "%s: %d dispatches" % (prefix.rstrip("("), len(per)),
-        "VALU busy (SQ_ACTIVE_INST_VALU*4/SIMDs/GRBM_GUI_ACTIVE): %.1f %%" % (sum(busy) / max(len(busy), 1)),
+        "VALU busy (SQ_ACTIVE_INST_VALU*4/SIMDs/(GRBM_GUI_ACTIVE/8)): %.1f %%" % (sum(busy) / max(len(busy), 1)),
         "VALU instructions per wave cycle (SQ_INSTS_VALU/SQ_WAVE_CYCLES): %.3f" % (sum(ipw) / max(len(ipw), 1)),
     ])
 

@@ -866,7 +866,7 @@ int i2e_upload(i2e_ctx *c, const i2e_host_state *s) {
   }
   c->M = s->marknum;
   c->permuted = 0;  // slots in the caller's order again
-  c->since_sort = 0;
+  c->since_sort = c->sort_every;  // the next step sorts
   const float *ms[7] = {s->markx, s->marky, s->markk, s->markv, s->markp, s->markpor, s->marktmax};
   for (int k = 0; k < 7; k++)
     if (c->M) HIP_OK(hipMemcpy(c->mfl.p + (size_t)k * c->Mmax, ms[k], sizeof(float) * c->M, hipMemcpyHostToDevice));
@@ -1312,6 +1312,22 @@ static void launch_movemark(i2e_ctx *c) {
                      c->need_eii);
 }
 
+// marker properties and their node sums: per tile of the last cell sort
+// (node sums staged in LDS) once the markers are sorted, else grid-stride
+static void launch_ronu_markers(i2e_ctx *c) {
+  const HeatK hk = heat_per_kg(c->P, c->timesum);
+  const double *tref = (const double *)c->red.p + kTrefSlot;
+  if (c->permuted) {
+    hipLaunchKernelGGL(k_ronu_markers<true>, dim3(tile_count(c->X, c->Y)), dim3(256), 0, c->s, c->dP, c->grid(),
+                       c->F, c->mk, c->M, c->A, hk, c->timestep, c->timesum, c->need_eii, tref,
+                       (const int *)c->moff.p, tiles_per_col(c->Y));
+  } else {
+    const unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
+    hipLaunchKernelGGL(k_ronu_markers<false>, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M,
+                       c->A, hk, c->timestep, c->timesum, c->need_eii, tref, nullptr, 0);
+  }
+}
+
 int i2e_movemark(i2e_ctx *c) {
   if (!c) return I2E_ERR_ARG;
   if (!c->M) return I2E_OK;
@@ -1330,13 +1346,10 @@ int i2e_ronurecalc(i2e_ctx *c) {
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->s, 1, 0.0, c->red.p + 17);
   c->kt.begin(K_P2G, c->s);
   if (c->M) {
-    unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
     if (c->P.soft_turb)
       hipLaunchKernelGGL(k_tref, dim3(1), dim3(64), 0, c->s, c->dP, c->grid(), c->mk,
                          c->permuted ? (const int *)c->mslot0.p : nullptr, c->red.p + kTrefSlot);
-    hipLaunchKernelGGL(k_ronu_markers, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->A,
-                       heat_per_kg(c->P, c->timesum), c->timestep, c->timesum, c->need_eii,
-                       (const double *)c->red.p + kTrefSlot);
+    launch_ronu_markers(c);
   }
   hipLaunchKernelGGL(k_ronu_nodes, dim3(std::min(nblk(N, 256), 256u)), dim3(256), 0, c->s, c->dP, c->F, c->A, N,
                      c->red.p + 16);
@@ -1355,7 +1368,7 @@ int i2e_sort_markers(i2e_ctx *c) {
   if (!c) return I2E_ERR_ARG;
   if (!c->M) return I2E_OK;
   const size_t MM = (size_t)c->Mmax;
-  const int C = (c->X - 1) * (c->Y - 1), n = C + 1, nb = (n + SB - 1) / SB;
+  const int C = tile_count(c->X, c->Y) * kTile * kTile, n = C + 1, nb = (n + SB - 1) / SB;
   ST_OK(c->mfl2.alloc(7 * MM));
   ST_OK(c->mt2.alloc(MM));
   ST_OK(c->mg2.alloc(MM));
@@ -1657,7 +1670,6 @@ int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g, do
   HIP_OK(hipMemcpyAsync(save.p + MM, c->mk.y, sizeof(float) * c->M, hipMemcpyDeviceToDevice, c->s));
   HIP_OK(hipMemcpyAsync(save.p + 2 * MM, c->mk.k, sizeof(float) * c->M, hipMemcpyDeviceToDevice, c->s));
   double tadv = 0;
-  unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
   for (int r = 0; r < reps; r++) {
     HIP_OK(hipEventRecord(e0, c->s));
     launch_movemark(c);
@@ -1675,9 +1687,7 @@ int i2e_time_kernels(i2e_ctx *c, int reps, double *ms_advect, double *ms_p2g, do
     HIP_OK(hipEventRecord(e0, c->s));
     hipLaunchKernelGGL(k_tref, dim3(1), dim3(64), 0, c->s, c->dP, c->grid(), c->mk,
                        c->permuted ? (const int *)c->mslot0.p : nullptr, c->red.p + kTrefSlot);
-    hipLaunchKernelGGL(k_ronu_markers, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->A,
-                       heat_per_kg(c->P, c->timesum), c->timestep, c->timesum, c->need_eii,
-                       (const double *)c->red.p + kTrefSlot);
+    launch_ronu_markers(c);
     HIP_OK(hipEventRecord(e1, c->s));
     HIP_OK(hipEventSynchronize(e1));
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));

@@ -6,13 +6,16 @@
 namespace i2e {
 
 // node fields (reference names, headmart.c:87-99)
+// node fields; the context allocates them as one block, N apart in this
+// order (k_ronu_markers relies on it)
 struct NodeF {
   double *vx, *vy, *pr, *gp, *ga;
   double *exx, *eyy, *exy, *sxx, *syy, *sxy;
   double *nu, *nd, *ro, *tk, *cp, *kt, *ht, *et;
 };
 
-// marker SoA (headmart.c:122-127)
+// marker SoA (headmart.c:122-127); the float fields are one block, one
+// capacity apart in this order (k_ronu_markers relies on it)
 struct Markers {
   float *x, *y, *k, *v, *p, *por, *tmax;
   int8_t *t;
@@ -20,7 +23,8 @@ struct Markers {
   double *mgt;  // its time, Ma (markmg_time)
 };
 
-// ronurecalc accumulators (ro0..tk0, sol0/sol1 and the viscosity weights)
+// ronurecalc accumulators (ro0..tk0, sol0/sol1 and the viscosity weights),
+// one block, N apart in this order (k_ronu_markers relies on it)
 struct Acc {
   double *ro0, *et0, *cp0, *kt0, *ht0, *tk0, *nu0, *nd0, *w0, *w1, *wnu, *wnd;
 };
@@ -42,8 +46,12 @@ HeatK heat_per_kg(const i2e_params &p, double timesum);
 
 // markers.hip
 __global__ void k_movemark(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, double timestep, int need_eii);
+// TILED: one block per marker tile of the last cell sort (toff: the sort's
+// key offsets, ntj tiles per column of tiles), node sums staged in LDS
+template <bool TILED>
 __global__ void k_ronu_markers(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, Acc A, HeatK hk,
-                               double timestep, double timesum, int need_eii, const double *tref);
+                               double timestep, double timesum, int need_eii, const double *tref, const int *toff,
+                               int ntj);
 __global__ void k_tref(const i2e_params *p, Grid g, Markers mk, const int *slot0, double *tref);
 __global__ void k_ronu_nodes(const i2e_params *p, NodeF F, Acc A, int N, double *mm);
 __global__ void k_contrast(const i2e_params *p, NodeF F, int N, const double *mm);
@@ -56,6 +64,18 @@ __global__ void k_type_counts(Grid g, Markers mk, int64_t M, int ntypes, int *cn
 
 // sort.hip (exclusive scans run in blocks of SB elements: 256 threads x 4)
 constexpr int SB = 1024;
+// The sort key is tile-major: kTile x kTile cells form a tile, tiles are
+// numbered column-major (ntj per column), cells inside a tile column-major.
+// A tile's markers are then one contiguous slot range, and the scatter of
+// one tile touches (kTile+1)^2 nodes.
+constexpr int kTile = 8;
+__host__ __device__ __forceinline__ int tiles_per_col(int Y) { return (Y - 1 + kTile - 1) / kTile; }
+__host__ __device__ __forceinline__ int tile_count(int X, int Y) {
+  return ((X - 1 + kTile - 1) / kTile) * tiles_per_col(Y);
+}
+__device__ __forceinline__ int tile_key(int ci, int cj, int ntj) {
+  return (((ci / kTile) * ntj + cj / kTile) * kTile + ci % kTile) * kTile + cj % kTile;
+}
 __global__ void k_mkey(Grid g, Markers mk, int64_t M, int C, int *key, int *cnt);
 __global__ void k_scan_blocks(int n, const int *in, int *out, int *bsum);
 __global__ void k_scan_sums(int nb, int *bsum);

@@ -4,11 +4,19 @@
 // projection (tkrecalc, heatmart.c:758-826) and the marker temperature update
 // after the implicit heat solve (titerate, heatmart.c:515-689).
 //
-// Markers stay in the cell-major order the initial lattice gives them
-// (in2mart.c:153-182) and move at most maxxystep cells per step, so a
-// wavefront touches a handful of neighbouring cells: node reads hit L2, and
-// scatters use hardware FP64 atomics (global_atomic_add_f64).
+// Markers are kept grouped by cell and tile (the cell sort, sort.hip) and
+// move at most maxxystep cells per step, so a wavefront touches a handful of
+// neighbouring cells: node reads hit L2, and the scatters reduce same-cell
+// lanes in registers before an FP64 atomic add (to LDS for the tiled
+// ronurecalc scatter, to HBM otherwise).
 #include "kernels.hpp"
+
+// Waves per SIMD the ronurecalc marker kernel is compiled for: 3 caps it at
+// 168 VGPRs (8 spilled) and measured 12-15 % faster than 2 (211 VGPRs) at
+// 6.25M markers; 4 spills 54 and is slower again.
+#ifndef I2E_RONU_WAVES
+#define I2E_RONU_WAVES 3
+#endif
 
 namespace i2e {
 
@@ -60,6 +68,12 @@ __device__ double viscalc_d(const i2e_params *p, double mtk, double mpb, int mm2
   return nueff;
 }
 
+// exp(kp P), the pressure factor of the conductivity (markmart.c:486):
+// exactly 1 when kp = 0 and P is finite, without the exponential
+__device__ __forceinline__ double exp_kp(double kp, double mpb) {
+  return (kp == 0 && isfinite(mpb)) ? 1.0 : exp(kp * mpb);
+}
+
 // allintere, markmart.c:1458-1543: strain rate, stress, P and |g| at (x,y)
 struct Ev {
   double exy, sxy, exx, sxx, eyy, syy, pr, ga;
@@ -77,8 +91,8 @@ __device__ __forceinline__ void clampxy(const Grid &g, double &x, double &y) {
     y = g.ysize;
 }
 
-// (|g| at the marker is only read by the soft-turbulence conductivity, which
-// this build does not model: with_ga = false skips that gather)
+// (|g| at the marker is only read by the soft-turbulence conductivity:
+// with_ga = false skips that gather)
 __device__ void allintere_d(const Grid &g, const NodeF &F, double x, double y, Ev &e, bool full,
                             bool with_ga = false) {
   W4 w;
@@ -251,15 +265,73 @@ __global__ void k_tref(const i2e_params *__restrict__ p, Grid g, Markers mk, con
   *tref = (double)fk;
 }
 
-__global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
+// Node sums per run of same-cell lanes (segmented sums over the wavefront),
+// added into the twelve accumulators: straight to HBM with atomics
+// (untiled), or -- TILED, one block per marker tile of the last sort -- into
+// an LDS copy of the tile's (kTile+1)^2 nodes, flushed with one atomic per
+// node and field at the end.  Markers that left their tile since the sort
+// take the atomics to HBM; any order stays correct.
+template <bool TILED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(I2E_RONU_WAVES))) void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
                                                      int64_t M, Acc A, HeatK hk, double timestep, double timesum,
-                                                     int need_eii, const double *__restrict__ tref) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+                                                     int need_eii, const double *__restrict__ tref,
+                                                     const int *__restrict__ toff, int ntj) {
+  constexpr int TN = kTile + 1, TA = TN * TN, NF = 11;
+  // LDS node sums of the tile: nu0 wnu ro0 et0 cp0 kt0 ht0 w0 tk0 nd0 wnd
+  __shared__ double tacc[TILED ? NF * TA : 1];
   const int lane = threadIdx.x & 63;
+  int ti0 = 0, tj0 = 0;
+  int64_t lo, hi, stride;
+  if (TILED) {
+    const int kk = kTile * kTile, T = blockIdx.x;
+    lo = T ? toff[T * kk - 1] : 0;
+    hi = toff[T * kk + kk - 1];
+    stride = blockDim.x;
+    ti0 = (T / ntj) * kTile;
+    tj0 = (T % ntj) * kTile;
+    for (int k = threadIdx.x; k < NF * TA; k += blockDim.x) tacc[k] = 0;
+    __syncthreads();
+  } else {
+    lo = (int64_t)blockIdx.x * blockDim.x;
+    hi = M;
+    stride = (int64_t)gridDim.x * blockDim.x;
+  }
+  {
+    // Every field pointer the kernel uses, derived from the three block
+    // bases the context allocates (node fields and accumulators N apart in
+    // NodeF / Acc order, marker float fields one capacity apart): passed as
+    // separate kernel arguments they outgrow the scalar registers and spill
+    // into vector-register lanes, one readlane per use.
+    const size_t N = (size_t)g.X * g.Y, MS = mk.y - mk.x;
+    double *const n0 = F.vx, *const a0 = A.ro0;
+    F.pr = n0 + 2 * N;
+    F.ga = n0 + 4 * N;
+    F.exx = n0 + 5 * N;
+    F.eyy = n0 + 6 * N;
+    F.exy = n0 + 7 * N;
+    F.sxx = n0 + 8 * N;
+    F.syy = n0 + 9 * N;
+    F.sxy = n0 + 10 * N;
+    A.et0 = a0 + N;
+    A.cp0 = a0 + 2 * N;
+    A.kt0 = a0 + 3 * N;
+    A.ht0 = a0 + 4 * N;
+    A.tk0 = a0 + 5 * N;
+    A.nu0 = a0 + 6 * N;
+    A.nd0 = a0 + 7 * N;
+    A.w0 = a0 + 8 * N;
+    A.wnu = a0 + 10 * N;
+    A.wnd = a0 + 11 * N;
+    mk.k = mk.x + 2 * MS;
+    mk.v = mk.x + 3 * MS;
+    mk.p = mk.x + 4 * MS;
+    mk.por = mk.x + 5 * MS;
+    mk.tmax = mk.x + 6 * MS;
+  }
   // whole wavefronts iterate together: the scatter reduces across lanes
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < M; base += stride) {
+  for (int64_t base = lo; base < hi; base += stride) {
     const int64_t mm1 = base + threadIdx.x;
-    bool act = mm1 < M;
+    bool act = mm1 < hi;
     // the marker's properties and its four scatter weights (0 off the
     // active markers); the per-corner products are formed at the scatter
     double s_ro = 0, s_bb = 0, s_cp = 0, s_kt = 0, s_ht = 0, s_tk = 0, s_nu = 0;
@@ -306,7 +378,7 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
           mro = r.ro * (1.0 - r.bb * (mtk - 298.15)) * (1.0 + r.aa * (mpb - 1.0) * 1e-3) * (1.000 - (double)por);
         }
         double mcp = r.cp;
-        double mkt = (r.kt + r.kf / (mtk + 77.0)) * exp(r.kp * mpb);
+        double mkt = (r.kt + r.kf / (mtk + 77.0)) * exp_kp(r.kp, mpb);
         double xmelt = 0;  // the marker's own silicate melt fraction
         if (p->melt_on && p->meltable[mm2 < 20 ? mm2 : mm2 - 20]) {
           double xm = (mtk - p->melt_ts) / (p->melt_tl - p->melt_ts);
@@ -327,8 +399,8 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
               if (nueff > p->nuend) nueff = p->nuend;
               mnu = nueff;
             }
-            mkt = ((rm.kt + rm.kf / (mtk + 77.0)) * exp(rm.kp * mpb)) * xm +
-                  ((rs.kt + rs.kf / (mtk + 77.0)) * exp(rs.kp * mpb)) * (1.0 - xm);
+            mkt = ((rm.kt + rm.kf / (mtk + 77.0)) * exp_kp(rm.kp, mpb)) * xm +
+                  ((rs.kt + rs.kf / (mtk + 77.0)) * exp_kp(rs.kp, mpb)) * (1.0 - xm);
             if (xm > 0 && xm < 1.0) {
               double xp = (mtk + 1.0 - p->melt_ts) / (p->melt_tl - p->melt_ts);
               double xq = (mtk - 1.0 - p->melt_ts) / (p->melt_tl - p->melt_ts);
@@ -453,12 +525,18 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
     const SegScan sc(key, lane);
     const bool put = sc.tail && key >= 0;
     const int m10 = put ? key / g.Y : 0, m20 = put ? key % g.Y : 0;
+    const bool in_tile = TILED && (unsigned)(m10 - ti0) < (unsigned)kTile && (unsigned)(m20 - tj0) < (unsigned)kTile;
+    auto add = [&](int f, double *acc, int a, int b, double v) {
+      if (in_tile)
+        atomicAdd(&tacc[f * TA + (m10 - ti0 + a) * TN + (m20 - tj0 + b)], v);
+      else
+        aadd(&acc[(m10 + a) * g.Y + m20 + b], v);
+    };
     {
       const double nd = sc.sum(c_nd), wnd = sc.sum(c_wnd);
       if (put && wnd != 0) {
-        const int m3 = (m10 + 1) * g.Y + m20 + 1;
-        aadd(&A.nd0[m3], nd);
-        aadd(&A.wnd[m3], wnd);
+        add(9, A.nd0, 1, 1, nd);
+        add(10, A.wnd, 1, 1, wnd);
       }
     }
 #pragma unroll
@@ -477,22 +555,38 @@ __global__ __launch_bounds__(256) void k_ronu_markers(const i2e_params *__restri
         const double r3 = sc.sum(s_kt * swt), r4 = sc.sum(s_ht * swt), r5 = sc.sum(swt);
         const double r6 = sc.sum(s_tk * swt), r8 = sc.sum(v8), r9 = sc.sum(v9);
         if (put) {
-          const int m3 = (m10 + a) * g.Y + m20 + b;
           if (r9 != 0) {
-            aadd(&A.nu0[m3], r8);
-            aadd(&A.wnu[m3], r9);
+            add(0, A.nu0, a, b, r8);
+            add(1, A.wnu, a, b, r9);
           }
-          aadd(&A.ro0[m3], r0);
-          aadd(&A.et0[m3], r1);
-          aadd(&A.cp0[m3], r2);
-          aadd(&A.kt0[m3], r3);
-          aadd(&A.ht0[m3], r4);
-          aadd(&A.w0[m3], r5);
-          aadd(&A.tk0[m3], r6);
+          add(2, A.ro0, a, b, r0);
+          add(3, A.et0, a, b, r1);
+          add(4, A.cp0, a, b, r2);
+          add(5, A.kt0, a, b, r3);
+          add(6, A.ht0, a, b, r4);
+          add(7, A.w0, a, b, r5);
+          add(8, A.tk0, a, b, r6);
         }
       }
   }
+  if (TILED) {
+    __syncthreads();
+    double *const dst[NF] = {A.nu0, A.wnu, A.ro0, A.et0, A.cp0, A.kt0, A.ht0, A.w0, A.tk0, A.nd0, A.wnd};
+    for (int k = threadIdx.x; k < TA; k += blockDim.x) {
+      const int i = ti0 + k / TN, j = tj0 + k % TN;
+      if (i >= g.X || j >= g.Y) continue;
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        const double v = tacc[f * TA + k];
+        if (v != 0) aadd(&dst[f][i * g.Y + j], v);
+      }
+    }
+  }
 }
+template __global__ void k_ronu_markers<false>(const i2e_params *, Grid, NodeF, Markers, int64_t, Acc, HeatK, double,
+                                               double, int, const double *, const int *, int);
+template __global__ void k_ronu_markers<true>(const i2e_params *, Grid, NodeF, Markers, int64_t, Acc, HeatK, double,
+                                              double, int, const double *, const int *, int);
 
 // positive doubles order like their bit patterns
 __device__ __forceinline__ void amin_pos(double *a, double v) {

@@ -3,24 +3,26 @@
 //
 // The reference never reorders its markers, and neither does the boundary:
 // i2e_download returns them in the caller's order.  Inside a context the
-// markers are kept grouped by cell (key = m1serch * (ynumy-1) + m2serch,
-// markmart.c:1051-1088; markers outside the box last), so that a wavefront's
-// markers share a few cells: coalesced marker reads, node gathers that hit
-// the same lines, and long same-cell runs in the scatter's segmented sums.
-// The initial lattice (in2mart.c:153-182) is already in this order; markers
-// drift by at most maxxystep cells per step, so the order decays slowly and
-// a sort every few steps restores it.
+// markers are kept grouped by cell, cells grouped in kTile x kTile tiles
+// (tile_key of m1serch, m2serch, markmart.c:1051-1088; markers outside the
+// box last), so that a wavefront's markers share a few cells -- coalesced
+// marker reads, node gathers that hit the same lines, long same-cell runs in
+// the scatter's segmented sums -- and a block's markers share one tile,
+// whose node sums it can stage in LDS.  Markers drift by at most maxxystep
+// cells per step, so the order decays slowly and a sort every few steps
+// restores it.
 #include "kernels.hpp"
 
 namespace i2e {
 
-// cell key of every marker and the per-key counts (cnt zeroed by the caller)
+// sort key of every marker and the per-key counts (cnt zeroed by the caller)
 __global__ void k_mkey(Grid g, Markers mk, int64_t M, int C, int *key, int *cnt) {
+  const int ntj = tiles_per_col(g.Y);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
     const float fx = mk.x[i], fy = mk.y[i];
     int k = C;
     if (fx >= 0 && fy >= 0 && (double)fx <= g.xsize && (double)fy <= g.ysize)
-      k = serch(g.gx, g.X, (double)fx) * (g.Y - 1) + serch(g.gy, g.Y, (double)fy);
+      k = tile_key(serch(g.gx, g.X, (double)fx), serch(g.gy, g.Y, (double)fy), ntj);
     key[i] = k;
     atomicAdd(&cnt[k], 1);
   }

@@ -324,7 +324,7 @@ def test_full_step_parity_fp64(orc, name, nsteps, sort_every):
 
 def test_marker_sort_is_invisible_at_the_boundary(orc):
     """The cell sort permutes the device markers only: after sorting, the
-    device order is nondecreasing in the cell key, download returns the
+    device order is nondecreasing in the tile-major cell key, download returns the
     caller's order bit for bit, and advection from the sorted slots gives
     per marker exactly what it gives unsorted."""
     import ctypes as C
@@ -352,7 +352,10 @@ def test_marker_sort_is_invisible_at_the_boundary(orc):
                                           a.nbytes) == 0
             i = np.clip(np.searchsorted(hs.gx, xs.astype(np.float64), side="right") - 1, 0, hs.xnumx - 2)
             j = np.clip(np.searchsorted(hs.gy, ys.astype(np.float64), side="right") - 1, 0, hs.ynumy - 2)
-            key = i * (hs.ynumy - 1) + j
+            # tile-major key: 8 x 8 cells per tile, tiles and the cells in
+            # a tile column-major
+            T, ntj = 8, (hs.ynumy - 1 + 7) // 8
+            key = (((i // T) * ntj + j // T) * T + i % T) * T + j % T
             assert np.all(np.diff(key) >= 0)
             assert np.array_equal(np.sort(orig), np.arange(hs.marknum))
             assert np.array_equal(xs, hs.markx[orig])
