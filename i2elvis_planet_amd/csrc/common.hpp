@@ -5,6 +5,7 @@
 // makes marker arithmetic bit-identical to the serial C code when both sides
 // are compiled with -ffp-contract=off.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -78,7 +79,33 @@ struct KTimer {
     bytes += alg;
     repr += rep;
   }
+  // a single-kernel unit: the events come back to be given to the launch
+  // itself (hipExtLaunchKernelGGL), which stamps them from the dispatch's
+  // own begin and end -- the kernel's duration as a kernel trace reports
+  // it, without the ~3 us an event pair around the launch adds
+  bool take(int id, double alg, double rep, hipEvent_t *e0, hipEvent_t *e1) {
+    if (id != target || used + 2 > ev.size()) return false;
+    if ((seen++ % every) != 0) return false;
+    *e0 = ev[used];
+    *e1 = ev[used + 1];
+    used += 2;
+    launches++;
+    bytes += alg;
+    repr += rep;
+    return true;
+  }
 };
+
+// launch KERNEL on STREAM, timed by the KTimer* KT as unit ID when it samples
+// this launch (ALG / REP: its algorithmic and representation bytes)
+#define KT_LAUNCH(KT, ID, ALG, REP, KERNEL, GRID, BLOCK, SHM, STREAM, ...)                              \
+  do {                                                                                                  \
+    hipEvent_t kt_e0_, kt_e1_;                                                                          \
+    if ((KT) && (KT)->take((ID), (ALG), (REP), &kt_e0_, &kt_e1_))                                       \
+      hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, STREAM, kt_e0_, kt_e1_, 0, __VA_ARGS__);          \
+    else                                                                                                \
+      hipLaunchKernelGGL(KERNEL, GRID, BLOCK, SHM, STREAM, __VA_ARGS__);                                \
+  } while (0)
 
 // v of the lane that the DPP control selects, both 32-bit halves moved by the
 // VALU's data-parallel primitives; 0 in rows outside ROW_MASK and where the

@@ -1139,9 +1139,7 @@ static int stokes_solve(i2e_ctx *c) {
   // implementation also reads the 4 BC row numbers (int32) per node
   const double alg = 80.0 * (double)c->N, bytes = alg + 16.0 * (double)c->N;
   auto A = [&](const double *x, double *y) -> int {
-    c->kt.begin(K_STOKES_APPLY, c->s);
-    hipLaunchKernelGGL(k_stokes_apply, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, x, y);
-    c->kt.end(K_STOKES_APPLY, c->s, alg, bytes);
+    KT_LAUNCH(&c->kt, K_STOKES_APPLY, alg, bytes, k_stokes_apply, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, x, y);
     return I2E_OK;
   };
   // each projection pass writes the velocity components of the pending
@@ -1306,10 +1304,10 @@ int i2e_titerate(i2e_ctx *c, int n0) {
 // block's cells (velocities, cell pressure, grid lines) in LDS measured
 // slower (0.51-0.71 ms against 0.47 ms at 6.25M markers): the stage
 // gathers hit L2 already, and the cost is the dependent search/weight chain.
-static void launch_movemark(i2e_ctx *c) {
+static void launch_movemark(i2e_ctx *c, bool timed = false) {
   const unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
-  hipLaunchKernelGGL(k_movemark, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->timestep,
-                     c->need_eii);
+  KT_LAUNCH(timed ? &c->kt : nullptr, K_ADVECT, advect_bytes(c), advect_repr(c), k_movemark, dim3(gb), dim3(256), 0,
+            c->s, c->dP, c->grid(), c->F, c->mk, c->M, c->timestep, c->need_eii);
 }
 
 // marker properties and their node sums: per tile of the last cell sort
@@ -1331,9 +1329,7 @@ static void launch_ronu_markers(i2e_ctx *c) {
 int i2e_movemark(i2e_ctx *c) {
   if (!c) return I2E_ERR_ARG;
   if (!c->M) return I2E_OK;
-  c->kt.begin(K_ADVECT, c->s);
-  launch_movemark(c);
-  c->kt.end(K_ADVECT, c->s, advect_bytes(c), advect_repr(c));
+  launch_movemark(c, true);
   HIP_OK(hipGetLastError());
   return I2E_OK;
 }

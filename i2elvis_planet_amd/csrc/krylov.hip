@@ -452,14 +452,12 @@ int Fgmres::gs_pass(int nv, double *w_, int off, hipStream_t s) {
   const int G = gs_blocks(n);
   const double bytes_dot = 8.0 * ((double)(nv + 1) * n + (double)(nv + 1) * G);
   const double bytes_axpy = 8.0 * ((double)nv * n + 2.0 * n + G);
-  if (kt) kt->begin(K_GSDOT, s);
-  hipLaunchKernelGGL(k_gsdot, dim3(G), dim3(RB), sizeof(double) * NW * (nv + 1), s, n, nv, V.p, ld, w_, part.p);
-  if (kt) kt->end(K_GSDOT, s, bytes_dot, bytes_dot);
+  KT_LAUNCH(kt, K_GSDOT, bytes_dot, bytes_dot, k_gsdot, dim3(G), dim3(RB), sizeof(double) * NW * (nv + 1), s, n,
+            nv, V.p, ld, w_, part.p);
   hipLaunchKernelGGL(k_reduce_parts, dim3(nv + 1), dim3(RB), 0, s, nv + 1, G, part.p, hd.p + off);
   if (nv > 0) {
-    if (kt) kt->begin(K_GSAXPY, s);
-    hipLaunchKernelGGL(k_gsaxpy, dim3(G), dim3(RB), 0, s, n, nv, V.p, ld, hd.p + off, w_, part.p);
-    if (kt) kt->end(K_GSAXPY, s, bytes_axpy, bytes_axpy);
+    KT_LAUNCH(kt, K_GSAXPY, bytes_axpy, bytes_axpy, k_gsaxpy, dim3(G), dim3(RB), 0, s, n, nv, V.p, ld,
+              hd.p + off, w_, part.p);
     hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(RB), 0, s, 1, G, part.p, hd.p + off + nv + 1);
   }
   return I2E_OK;
@@ -489,14 +487,11 @@ int Fgmres::lagged_pass(int j, const double *w_, double *wout, hipStream_t s) {
   const size_t R = 0;  // offset in hd
   const double bytes_dot = 8.0 * ((double)j * n + 2.0 * n + (2.0 * j + 3) * G);
   const double bytes_axpy = 8.0 * ((double)j * n + 4.0 * n + G);
-  if (kt) kt->begin(K_GSDOT, s);
-  hipLaunchKernelGGL(k_gsdot2, dim3(G), dim3(RB), sizeof(double) * NW * (2 * j + 3), s, n, j, V.p, ld, u, w_,
-                     part.p);
-  if (kt) kt->end(K_GSDOT, s, bytes_dot, bytes_dot);
+  KT_LAUNCH(kt, K_GSDOT, bytes_dot, bytes_dot, k_gsdot2, dim3(G), dim3(RB), sizeof(double) * NW * (2 * j + 3), s,
+            n, j, V.p, ld, u, w_, part.p);
   hipLaunchKernelGGL(k_reduce_coef, dim3(2 * j + 3), dim3(RB), 0, s, G, part.p, hd.p + R);
-  if (kt) kt->begin(K_GSAXPY, s);
-  hipLaunchKernelGGL(k_gsaxpy2, dim3(G), dim3(RB), 0, s, n, j, V.p, ld, hd.p + R, u, w_, u, wout, part.p);
-  if (kt) kt->end(K_GSAXPY, s, bytes_axpy, bytes_axpy);
+  KT_LAUNCH(kt, K_GSAXPY, bytes_axpy, bytes_axpy, k_gsaxpy2, dim3(G), dim3(RB), 0, s, n, j, V.p, ld,
+            hd.p + R, u, w_, u, wout, part.p);
   if (split_f32 || !sx)
     hipLaunchKernelGGL(k_norm_scale<float>, dim3(G), dim3(RB), 0, s, n, G, part.p, hd.p + R + 2 * j + 5, wout,
                        2 * j + 5, hd.p + R, hc.p, (float *)sx, (float *)sy);

@@ -1523,20 +1523,19 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
     A.fym = view(f.mym);
   }
   auto launch = [&](const TileArgs<T> &a) {
-    if (kt) kt->begin(kid, s);
+    const double rep = bytes + (a.rfx ? rbytes : 0.0);
     if (l == 0 && !stencil0)
-      hipLaunchKernelGGL(k_tsmooth0<T>, TG, TBK, 0, s, a, o, (const T *)v.dx.p, (const T *)v.dy.p);
+      KT_LAUNCH(kt, kid, alg, rep, k_tsmooth0<T>, TG, TBK, 0, s, a, o, (const T *)v.dx.p, (const T *)v.dy.p);
     else if (l == 0 && diff0)
-      hipLaunchKernelGGL((k_tsmooth<T, T, 1, true>), TG, TBK, 0, s, a, (const T *)v.S.p);
+      KT_LAUNCH(kt, kid, alg, rep, (k_tsmooth<T, T, 1, true>), TG, TBK, 0, s, a, (const T *)v.S.p);
     else if (l == 0 && S0d)
-      hipLaunchKernelGGL((k_tsmooth<T, double, 1>), TG, TBK, 0, s, a, S0d);
+      KT_LAUNCH(kt, kid, alg, rep, (k_tsmooth<T, double, 1>), TG, TBK, 0, s, a, S0d);
     else if (l == 0)
-      hipLaunchKernelGGL((k_tsmooth<T, T, 1>), TG, TBK, 0, s, a, (const T *)v.S.p);
+      KT_LAUNCH(kt, kid, alg, rep, (k_tsmooth<T, T, 1>), TG, TBK, 0, s, a, (const T *)v.S.p);
     else if (a.rfx)
-      hipLaunchKernelGGL((k_tsmooth<T, T, 2, false, true>), TG, TBK, 0, s, a, (const T *)v.S.p);
+      KT_LAUNCH(kt, kid, alg, rep, (k_tsmooth<T, T, 2, false, true>), TG, TBK, 0, s, a, (const T *)v.S.p);
     else
-      hipLaunchKernelGGL((k_tsmooth<T, T, 2>), TG, TBK, 0, s, a, (const T *)v.S.p);
-    if (kt) kt->end(kid, s, alg, bytes + (a.rfx ? rbytes : 0.0));
+      KT_LAUNCH(kt, kid, alg, rep, (k_tsmooth<T, T, 2>), TG, TBK, 0, s, a, (const T *)v.S.p);
     std::swap(v.ux.p, v.ux2.p);
     std::swap(v.uy.p, v.uy2.p);
   };
