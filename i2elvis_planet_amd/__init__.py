@@ -29,7 +29,7 @@ _EXPORTS = [
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
     "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
     "i2e_type_counts", "i2e_maxvelstep_noalloc", "i2e_kernel_timer_repr", "i2e_set_marker_sort",
-    "i2e_sort_markers", "i2e_init_read", "i2e_init_generate", "i2e_mode_read",
+    "i2e_sort_markers", "i2e_init_read", "i2e_init_generate", "i2e_mode_read", "i2e_tuned_shapes",
 ]
 
 _lib = None
@@ -84,6 +84,8 @@ def load_library(path=LIB_PATH):
         "i2e_kernel_timer_repr": ([vp, dp], C.c_int),
         "i2e_set_marker_sort": ([vp, C.c_int], C.c_int),
         "i2e_sort_markers": ([vp], C.c_int),
+        "i2e_tuned_shapes": ([vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                              C.POINTER(C.c_int)], C.c_int),
         "i2e_init_read": ([C.c_char_p, C.POINTER(abi.InitInfo)], C.c_int),
         "i2e_init_generate": ([C.c_char_p, P, S], C.c_int),
         "i2e_mode_read": ([C.c_char_p, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
@@ -310,6 +312,18 @@ class Context:
     def set_marker_sort(self, every):
         """Sort the markers by cell every `every` steps of step() (0: never)."""
         _chk(_lib.i2e_set_marker_sort(self._h, int(every)), "set_marker_sort")
+
+    def tuned_shapes(self):
+        """[(level nodes along x, tile width, tile height)] of the coarse
+        velocity levels smoothed by tiles (the autotuner's choice)."""
+        n = C.c_int()
+        _chk(_lib.i2e_tuned_shapes(self._h, 0, None, None, None, C.byref(n)), "tuned_shapes")
+        k = n.value
+        lv, sx, sy = (np.zeros(k, np.int32) for _ in range(3))
+        i32 = C.POINTER(C.c_int32)
+        _chk(_lib.i2e_tuned_shapes(self._h, k, lv.ctypes.data_as(i32), sx.ctypes.data_as(i32),
+                                   sy.ctypes.data_as(i32), C.byref(n)), "tuned_shapes")
+        return [(int(a), int(b), int(c)) for a, b, c in zip(lv, sx, sy)]
 
     def sort_markers(self):
         """Group the device markers by cell now (download keeps the caller's order)."""

@@ -484,6 +484,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->nsmooth_coarse = 2;
   o->nsmooth_cell = 4;
   o->stokes_guess = I2E_GUESS_EXTRAP;
+  o->autotune = 0;
   o->stokes_rtol = 1e-8;
   o->heat_rtol = 1e-10;
   o->grav_rtol = 1e-10;
@@ -624,7 +625,8 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
       o->coarse_max < 2 || (o->orth != I2E_ORTH_LAGGED && o->orth != I2E_ORTH_DGKS) || o->fine_stencil < 0 ||
       o->fine_stencil > I2E_FINE_DIFF32 ||
       (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT) ||
-      (o->precision != I2E_PREC_FP64 && o->precision != I2E_PREC_MIXED) || (o->graphs != 0 && o->graphs != 1)) {
+      (o->precision != I2E_PREC_FP64 && o->precision != I2E_PREC_MIXED) || (o->graphs != 0 && o->graphs != 1) ||
+      (o->autotune != 0 && o->autotune != 1)) {
     set_error("set_solver: option out of range");
     return I2E_ERR_ARG;
   }
@@ -645,6 +647,10 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
     ST_OK(c->fs.init(3 * c->N, o->restart));
     ST_OK(c->fh.init(c->N, o->restart));
     ST_OK(c->fgv.init(c->N, o->restart));
+  }
+  if (!o->autotune) {  // back to the default tile shapes
+    c->v32.shape.clear();
+    c->v64.shape.clear();
   }
   c->o = *o;
   c->fs.orth = c->fh.orth = c->fgv.orth = o->orth;
@@ -1074,6 +1080,12 @@ static int stokes_prepare(i2e_ctx *c) {
   }
   // the velocity coarsest inverse (second stream) is needed from here on
   ST_OK(c->v64.join(c->s));
+  // tile shapes of the Galerkin levels, tuned on this hierarchy's data the
+  // first time (and looked up by level size after that)
+  if (c->o.autotune) {
+    if (mixed(c) && c->v32.shape.empty()) ST_OK(c->v32.tune(c->s));
+    if (!mixed(c) && c->v64.shape.empty()) ST_OK(c->v64.tune(c->s));
+  }
   return I2E_OK;
 }
 
@@ -1394,6 +1406,30 @@ int i2e_sort_markers(i2e_ctx *c) {
   c->mk = marker_view(c->mfl.p, c->mt.p, c->mg.p, c->mgt.p, MM);
   c->permuted = 1;
   c->since_sort = 0;
+  return I2E_OK;
+}
+
+int i2e_tuned_shapes(i2e_ctx *c, int nmax, int32_t *levels, int32_t *sx, int32_t *sy, int *n) {
+  if (!c || nmax < 0 || !n) return I2E_ERR_ARG;
+  const bool m = mixed(c);
+  const std::vector<int> &sh = m ? c->v32.shape : c->v64.shape;
+  const int nl = m ? (int)c->v32.L.size() : (int)c->v64.L.size();
+  int k = 0;
+  for (int l = 1; l < nl - 1 && l < (int)sh.size(); l++) {
+    if (m ? c->v32.tail_at(l) : c->v64.tail_at(l)) continue;
+    if (k < nmax) {
+      int x = 0, y = 0;
+      if (m)
+        tile_shape<float>(sh[l], &x, &y);
+      else
+        tile_shape<double>(sh[l], &x, &y);
+      if (levels) levels[k] = m ? c->v32.L[l].X : c->v64.L[l].X;
+      if (sx) sx[k] = x;
+      if (sy) sy[k] = y;
+    }
+    k++;
+  }
+  *n = k;
   return I2E_OK;
 }
 

@@ -1,10 +1,16 @@
 // mg.hip -- geometric multigrid for the Stokes velocity block and for the
 // scalar (heat, gravity) operators.  See mg.hpp for the design.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <type_traits>
 
 #include "mg.hpp"
+
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace i2e {
 
@@ -822,6 +828,24 @@ struct TileShape {
   constexpr int TSX = TileShape<T>::SX, TSY = TileShape<T>::SY, TWX = TileShape<T>::WX, TWY = TileShape<T>::WY, \
                 TB = TileShape<T>::B;                                                                     \
   (void)TSX; (void)TSY; (void)TWX; (void)TWY; (void)TB
+#define TILE_CONSTS_XY(SX_, SY_)                                                                          \
+  constexpr int TSX = SX_, TSY = SY_, TWX = SX_ + 2 * TH, TWY = SY_ + 2 * TH, TB = SX_ * SY_;            \
+  (void)TSX; (void)TSY; (void)TWX; (void)TWY; (void)TB
+
+// Tile shapes the autotuner chooses among for the Galerkin-level smoothing
+// pass (VelMG::tune); candidate 0 is TileShape's.  Every shape keeps whole
+// 128-byte lines along j for FP32 (32 or 64 points) or FP64 (16 or 32).
+constexpr int kShapes = 4;
+template <class T, int K>
+struct ShapeCand;
+template <> struct ShapeCand<float, 0> { static constexpr int SX = 32, SY = 16; };
+template <> struct ShapeCand<float, 1> { static constexpr int SX = 32, SY = 8; };
+template <> struct ShapeCand<float, 2> { static constexpr int SX = 64, SY = 8; };
+template <> struct ShapeCand<float, 3> { static constexpr int SX = 32, SY = 32; };
+template <> struct ShapeCand<double, 0> { static constexpr int SX = 16, SY = 16; };
+template <> struct ShapeCand<double, 1> { static constexpr int SX = 16, SY = 8; };
+template <> struct ShapeCand<double, 2> { static constexpr int SX = 32, SY = 8; };
+template <> struct ShapeCand<double, 3> { static constexpr int SX = 32, SY = 16; };
 
 template <class T>
 struct TileArgs {
@@ -855,9 +879,9 @@ __device__ __forceinline__ double prolong_at(const T *e, int Yc, const MapView &
          wa * ((1 - wb) * (double)e[(a + 1) * Yc + b] + wb * (double)e[(a + 1) * Yc + b + 1]);
 }
 
-template <class T>
+template <class T, int SX = TileShape<T>::SX, int SY = TileShape<T>::SY>
 __device__ __forceinline__ void tile_load(const TileArgs<T> &A, T *lx, T *ly, int i0, int j0) {
-  TILE_CONSTS(T);
+  TILE_CONSTS_XY(SX, SY);
   for (int q = threadIdx.x; q < TWX * TWY; q += TB) {
     const int i = i0 + q / TWX, j = j0 + q % TWX;
     T vx = 0, vy = 0;
@@ -906,14 +930,16 @@ __device__ __forceinline__ TA s_apply_lds(const TA *S, size_t R, size_t r, const
 // RS: pre-smoothing pass that restricts its right-hand side (A.rfx); a
 // template flag, since the restriction code slows the plain pass's schedule
 // even when not taken (13.8 -> 16.0 us for the FP32 Galerkin pass)
-template <class T, class TA, int W, bool DIFF = false, bool RS = false>
-__global__ __launch_bounds__(TileShape<T>::B) void k_tsmooth(TileArgs<T> A, const TA *S) {
-  TILE_CONSTS(T);
+// SX, SY: the tile shape (TileShape's unless the autotuner chose another)
+template <class T, class TA, int W, bool DIFF = false, bool RS = false, int SX = TileShape<T>::SX,
+          int SY = TileShape<T>::SY>
+__global__ __launch_bounds__(SX * SY) void k_tsmooth(TileArgs<T> A, const TA *S) {
+  TILE_CONSTS_XY(SX, SY);
   __shared__ T lx[TWX * TWY], ly[TWX * TWY];
   const int i0 = blockIdx.y * TSY - TH, j0 = blockIdx.x * TSX - TH;
   const int X = A.X, Y = A.Y, N = X * Y;
   const size_t R = 2 * (size_t)N;
-  tile_load(A, lx, ly, i0, j0);
+  tile_load<T, SX, SY>(A, lx, ly, i0, j0);
   const int i = i0 + TH + (int)threadIdx.x / TSX, j = j0 + TH + (int)threadIdx.x % TSX;
   const int li = (i - i0) * TWX + (j - j0);
   const bool ok = i < X && j < Y;
@@ -1237,6 +1263,7 @@ template <class T>
 int VelMG<T>::setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,
                     const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,
                     const std::vector<double> &kky0) {
+  shape.clear();
   geo = &g;
   const int nl = (int)g.size();
   L.clear();
@@ -1453,6 +1480,100 @@ int VelMG<T>::join(hipStream_t s) {
 }
 
 template <class T>
+void tile_shape(int k, int *sx, int *sy) {
+  switch (k) {
+    case 1: *sx = ShapeCand<T, 1>::SX, *sy = ShapeCand<T, 1>::SY; break;
+    case 2: *sx = ShapeCand<T, 2>::SX, *sy = ShapeCand<T, 2>::SY; break;
+    case 3: *sx = ShapeCand<T, 3>::SX, *sy = ShapeCand<T, 3>::SY; break;
+    default: *sx = ShapeCand<T, 0>::SX, *sy = ShapeCand<T, 0>::SY; break;
+  }
+}
+template void tile_shape<float>(int, int *, int *);
+template void tile_shape<double>(int, int *, int *);
+
+// Galerkin-level smoothing pass with tile shape candidate K
+template <class T, bool RS, int K>
+static void launch_tw2(const TileArgs<T> &a, const T *S, hipStream_t s, KTimer *kt, int kid, double alg,
+                       double rep) {
+  constexpr int SX = ShapeCand<T, K>::SX, SY = ShapeCand<T, K>::SY;
+  const dim3 TG((a.Y + SX - 1) / SX, (a.X + SY - 1) / SY), TBK(SX * SY);
+  KT_LAUNCH(kt, kid, alg, rep, (k_tsmooth<T, T, 2, false, RS, SX, SY>), TG, TBK, 0, s, a, S);
+}
+template <class T, bool RS>
+static void launch_tw2(int k, const TileArgs<T> &a, const T *S, hipStream_t s, KTimer *kt, int kid, double alg,
+                       double rep) {
+  switch (k) {
+    case 1: launch_tw2<T, RS, 1>(a, S, s, kt, kid, alg, rep); break;
+    case 2: launch_tw2<T, RS, 2>(a, S, s, kt, kid, alg, rep); break;
+    case 3: launch_tw2<T, RS, 3>(a, S, s, kt, kid, alg, rep); break;
+    default: launch_tw2<T, RS, 0>(a, S, s, kt, kid, alg, rep); break;
+  }
+}
+
+// tuned shape per (precision, level size, sweeps), shared by every context
+// of the process (an ensemble tunes each level size once)
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int>, int> g_tuned;
+
+template <class T>
+int VelMG<T>::tune(hipStream_t s, int reps) {
+  const int nl = (int)L.size();
+  shape.assign(nl, 0);
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  for (int l = 1; l < nl - 1; l++) {
+    if (tail_at(l)) continue;
+    VelLevel<T> &v = L[l];
+    const int ns = nsmooth_c <= 0 ? nsmooth : nsmooth_c;
+    const auto key = std::make_tuple((int)sizeof(T), v.X, v.Y, ns);
+    {
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      auto it = g_tuned.find(key);
+      if (it != g_tuned.end()) {
+        shape[l] = it->second;
+        continue;
+      }
+    }
+    // the pre-smoothing pass from the level's current iterate into its
+    // second buffers (nothing the V-cycle reads is written)
+    const MapView none{nullptr, nullptr, nullptr, nullptr};
+    TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr, 0,
+                  none, none, none, none, 0, ns, 0, v.dx.p, v.dy.p};
+    A.omega = omega;
+    double best = 1e300;
+    int bk = 0;
+    HIP_OK(hipGetLastError());
+    for (int k = 0; k < kShapes; k++) {
+      for (int w = 0; w < 2; w++) launch_tw2<T, false>(k, A, v.S.p, s, nullptr, 0, 0, 0);
+      HIP_OK(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++) launch_tw2<T, false>(k, A, v.S.p, s, nullptr, 0, 0, 0);
+      HIP_OK(hipEventRecord(e1, s));
+      HIP_OK(hipEventSynchronize(e1));
+      // a shape the device cannot launch (registers, LDS) is no candidate
+      const hipError_t le = hipGetLastError();
+      if (getenv("I2E_TUNE_VERBOSE"))
+        fprintf(stderr, "tune: %s level %dx%d shape %d: %s\n", sizeof(T) == 4 ? "fp32" : "fp64", v.X, v.Y, k,
+                hipGetErrorString(le));
+      if (le != hipSuccess) continue;
+      float ms = 0;
+      HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+      if (getenv("I2E_TUNE_VERBOSE")) fprintf(stderr, "tune:   %.4f ms per pass\n", ms / reps);
+      if (ms < best) {
+        best = ms;
+        bk = k;
+      }
+    }
+    shape[l] = bk;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tuned[key] = bk;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return I2E_OK;
+}
+
+template <class T>
 int VelMG<T>::vcycle(int l, hipStream_t s) {
   const int nl = (int)L.size();
   VelLevel<T> &v = L[l];
@@ -1533,9 +1654,9 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
     else if (l == 0)
       KT_LAUNCH(kt, kid, alg, rep, (k_tsmooth<T, T, 1>), TG, TBK, 0, s, a, (const T *)v.S.p);
     else if (a.rfx)
-      KT_LAUNCH(kt, kid, alg, rep, (k_tsmooth<T, T, 2, false, true>), TG, TBK, 0, s, a, (const T *)v.S.p);
+      launch_tw2<T, true>(l < (int)shape.size() ? shape[l] : 0, a, (const T *)v.S.p, s, kt, kid, alg, rep);
     else
-      KT_LAUNCH(kt, kid, alg, rep, (k_tsmooth<T, T, 2>), TG, TBK, 0, s, a, (const T *)v.S.p);
+      launch_tw2<T, false>(l < (int)shape.size() ? shape[l] : 0, a, (const T *)v.S.p, s, kt, kid, alg, rep);
     std::swap(v.ux.p, v.ux2.p);
     std::swap(v.uy.p, v.uy2.p);
   };

@@ -106,6 +106,10 @@ struct VelLevel {
   std::vector<T> hkkx, hkky;
 };
 
+// (width along j, height along i) of tile shape candidate k
+template <class T>
+void tile_shape(int k, int *sx, int *sy);
+
 template <class T>
 struct VelMG {
   std::vector<VelLevel<T>> L;
@@ -145,6 +149,12 @@ struct VelMG {
   int refresh_from(const VelMG<double> &d, hipStream_t s);
   // z = M^-1 r for the velocity block; f/u live in L[0].fx/fy and L[0].ux/uy
   int vcycle(int l, hipStream_t s);
+  // Tile shape of each Galerkin level's smoothing pass (candidate index,
+  // 0 = the default): tune() times every candidate on the level's current
+  // data (reps launches each, best kept) unless a level of that size was
+  // tuned before in this process
+  std::vector<int> shape;
+  int tune(hipStream_t s, int reps = 10);
   VLev<T> lev(int l) const;
   void release();
 };

@@ -204,6 +204,14 @@ typedef struct i2e_solver_opts {
                               BFBt Poisson solves gain more from sweeps than
                               the velocity block (501^2: 82 vs 88 Krylov
                               iterations against 3 / 3 / 3, 7 % less time) */
+  int32_t autotune;        /* 1: the tile shape of each coarse velocity level's
+                              smoothing pass is timed among four candidates
+                              at the first Stokes set-up and the fastest kept
+                              (per level size, for the whole process; the
+                              tiles change the block Gauss-Seidel partition,
+                              so iteration counts can differ slightly);
+                              0 (default): the fixed shapes                 */
+  int32_t pad_opts;
 } i2e_solver_opts;
 
 typedef struct i2e_stats {
@@ -284,6 +292,12 @@ void *i2e_device_ptr(i2e_ctx *c, const char *name);
  * `every` steps (default 10; 0 = never; suspended while i2e_record runs);
  * i2e_sort_markers sorts now.  Extra device memory: 33 B per marker slot. */
 int i2e_set_marker_sort(i2e_ctx *c, int every);
+/* The tile shapes the autotuner chose (autotune = 1), one entry per coarse
+ * velocity level smoothed by tiles, finest first: the level's nodes along x
+ * in levels[], the tile's width (along y) and height in sx[], sy[] (arrays
+ * of nmax, any may be NULL); *n = the number of such levels.  No
+ * counterpart in the reference (its solves are PARDISO's).                 */
+int i2e_tuned_shapes(i2e_ctx *c, int nmax, int32_t *levels, int32_t *sx, int32_t *sy, int *n);
 int i2e_sort_markers(i2e_ctx *c);
 /* Advection kernel alone, timed with HIP events on the context's stream:
  * runs movemark's marker loop `reps` times from the same start positions
