@@ -34,7 +34,7 @@ TRAFFIC_FILES = {
 # rocprof kernel-name prefix and, where only some launches count, the grid
 # size (threads) of the launches bench.py times for each roofline kernel
 PMC_MATCH = {
-    "vsmoothc": ("void i2e::k_tsmooth<float, float, 2, false, false>(", None),
+    "vsmoothc": ("void i2e::k_tsmooth<float, float, 2, false, false, 32, 16>(", None),
     "gsdot": ("i2e::k_gsdot2(", "stokes"),
     "p2g": ("void i2e::k_ronu_markers<true>(", None),
     "advect": ("i2e::k_movemark(", None),
@@ -121,9 +121,9 @@ def cpu_baseline(name="c3"):
         oracle.SOLVER = solver0
     dt, ph = runs[cores]
     return {"value": 1.0 / dt, "unit": "timesteps/s", "cores": cores, "kind": "port",
-            "sample": "1 full step of the 501x501 / 6.25M-marker setup after its initial ronurecalc: marker loops "
-                      "and assembly serial C, sparse solves PARDISO with %d threads, its answer taken unchecked "
-                      "as the reference takes it (%.1f s)" % (cores, dt),
+            "sample": "1 full step of the %s setup after its initial ronurecalc: marker loops and assembly "
+                      "serial C, sparse solves PARDISO with %d threads, its answer taken unchecked as the "
+                      "reference takes it (%.1f s)" % (WORKLOADS[name][1].split(";")[0], cores, dt),
             "s_per_step_by_threads": {str(k): round(v[0], 3) for k, v in sorted(runs.items())},
             "phase_s": {k: round(v, 3) for k, v in ph.items()}}
 
@@ -327,11 +327,23 @@ def kernel_stats_table(stats_csv, top=30):
     return "\n".join(out)
 
 
-def run(steps, warmup, skip_cpu, rk, minimal=False):
+WORKLOADS = {
+    "c3": ("timesteps/s @501\u00b2, 6.3M markers: mixed vs FP64 GPU at Stokes rel.resid 1e-8",
+           "501x501 nodes, 25 markers/cell (6.25M), R=150 km, Arrhenius 300 kJ/mol, linear melting 1360-1850 K, "
+           "400 iron blobs; one i2mart step"),
+    "c2": ("timesteps/s @201\u00b2, 640k markers: mixed vs FP64 GPU at Stokes rel.resid 1e-8",
+           "201x201 nodes, 16 markers/cell (640k), R=100 km, Arrhenius 300 kJ/mol, porosity U[0.3,0.5]; "
+           "one i2mart step"),
+    "c1": ("timesteps/s @51\u00b2, 40k markers: mixed vs FP64 GPU at Stokes rel.resid 1e-8",
+           "51x51 nodes, 16 markers/cell (40k), R=50 km, constant viscosity, 25 % iron; one i2mart step"),
+}
+
+
+def run(steps, warmup, skip_cpu, rk, minimal=False, name="c3"):
     import numpy as np
     from i2elvis_planet_amd import Context, configs, default_solver
 
-    p, hs, cfg = configs.make("c3")
+    p, hs, cfg = configs.make(name)
     cold = hs.copy()
     log("setup generated: %d markers" % hs.marknum)
     o = default_solver()
@@ -356,7 +368,7 @@ def run(steps, warmup, skip_cpu, rk, minimal=False):
     # outside the timed region: per-kernel figures, the pre-melt regime with
     # a hot interior, the molten regime, standalone kernel timings
     kt, warm_res, hot = {}, None, None
-    if not minimal:
+    if not minimal and name == "c3":
         kt = kernel_table(ctx, o, snap)
         warm = configs.heat_state(cold.copy(), cfg, t_core=1340.0)
         ctx.upload(warm)
@@ -378,7 +390,7 @@ def run(steps, warmup, skip_cpu, rk, minimal=False):
     avg_ms = ms_tot / max(launches, 1)
     achieved = bytes_tot / max(ms_tot, 1e-30) * 1e3 / 1e9
     line = {
-        "metric": "timesteps/s @501\u00b2, 6.3M markers: mixed vs FP64 GPU at Stokes rel.resid 1e-8",
+        "metric": WORKLOADS[name][0],
         "value": mixed_sps,
         "unit": "timesteps/s",
         "n_gpus": 1,
@@ -390,9 +402,7 @@ def run(steps, warmup, skip_cpu, rk, minimal=False):
         "vs_baseline": None,
         "dtype": "f64 (f32 multigrid preconditioner)",
         "data": "synthetic",
-        "config": {"workload": "501x501 nodes, 25 markers/cell (6.25M), R=150 km, Arrhenius 300 kJ/mol, "
-                               "linear melting 1360-1850 K, 400 iron blobs; one i2mart step",
-                   "stokes_rtol": 1e-8, "precision": "mixed"},
+        "config": {"workload": WORKLOADS[name][1], "stokes_rtol": 1e-8, "precision": "mixed"},
         "fp64_value": fp64_sps,
         "mixed_over_fp64": mixed_sps / fp64_sps,
         "stokes_iters_mixed": [i[0] for i in it],
@@ -424,7 +434,7 @@ def run(steps, warmup, skip_cpu, rk, minimal=False):
     ctx.close()
     if not skip_cpu:
         try:
-            line["cpu_baseline"] = cpu_baseline()
+            line["cpu_baseline"] = cpu_baseline(name)
         except Exception as e:  # reported, never fatal to the GPU number
             log("cpu baseline failed: %r" % (e,))
     return line
@@ -496,6 +506,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--minimal", action="store_true",
                     help="only the timed steps and the roofline kernel (for counter runs)")
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS),
+                    help="setup of the timed steps: c3 (501^2, the metric's), c2 (201^2) or c1 (51^2)")
     ap.add_argument("--roofline-kernel", default="vsmoothc", choices=sorted(ROOFLINE_KERNELS),
                     help="kernel of the roofline entry (default: the one with the most GPU time)")
     ap.add_argument("--ensemble", type=int, default=0, help="run the ensemble measurement with this many bodies")
@@ -533,7 +545,7 @@ def main():
     if a.ensemble:
         print(json.dumps(ensemble(a.ensemble, a.steps, a.warmup)), flush=True)
         return
-    line = run(a.steps, a.warmup, a.no_cpu, a.roofline_kernel, a.minimal)
+    line = run(a.steps, a.warmup, a.no_cpu, a.roofline_kernel, a.minimal, a.workload)
     print(json.dumps(line), flush=True)
 
 
