@@ -29,7 +29,7 @@ _EXPORTS = [
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
     "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
     "i2e_type_counts", "i2e_maxvelstep_noalloc", "i2e_kernel_timer_repr", "i2e_set_marker_sort",
-    "i2e_sort_markers", "i2e_init_read", "i2e_init_generate", "i2e_mode_read", "i2e_tuned_shapes",
+    "i2e_sort_markers", "i2e_init_read", "i2e_init_generate", "i2e_mode_read", "i2e_tuned_shapes", "i2e_hydration",
 ]
 
 _lib = None
@@ -86,6 +86,7 @@ def load_library(path=LIB_PATH):
         "i2e_sort_markers": ([vp], C.c_int),
         "i2e_tuned_shapes": ([vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                               C.POINTER(C.c_int)], C.c_int),
+        "i2e_hydration": ([vp, C.c_double, C.c_double, C.c_int, C.POINTER(abi.BodyStats)], C.c_int),
         "i2e_init_read": ([C.c_char_p, C.POINTER(abi.InitInfo)], C.c_int),
         "i2e_init_generate": ([C.c_char_p, P, S], C.c_int),
         "i2e_mode_read": ([C.c_char_p, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
@@ -312,6 +313,16 @@ class Context:
     def set_marker_sort(self, every):
         """Sort the markers by cell every `every` steps of step() (0: never)."""
         _chk(_lib.i2e_set_marker_sort(self._h, int(every)), "set_marker_sort")
+
+    def hydration(self, t_sol=1393.811, t_liq=1973.0, hydrate=True):
+        """impact()'s per-step hydration and body statistics on the device
+        state (impact.c:3022-3228): switches rock+ice / hydrous silicate
+        marker types by their peak temperature and returns the counts and
+        fractions (abi.BodyStats).  t_sol, t_liq: impact()'s peridotite
+        solidus and liquidus; hydrate: growth_model == 0 && por_init <= 0.001."""
+        o = abi.BodyStats()
+        _chk(_lib.i2e_hydration(self._h, float(t_sol), float(t_liq), int(bool(hydrate)), C.byref(o)), "hydration")
+        return o
 
     def tuned_shapes(self):
         """[(level nodes along x, tile width, tile height)] of the coarse

@@ -133,3 +133,14 @@ class PrnHeader(C.Structure):
                 ("xsize", C.c_double), ("ysize", C.c_double), ("por_init", C.c_double), ("gxkoef", C.c_double),
                 ("gykoef", C.c_double), ("tmp_ambient", C.c_double), ("al2627_init", C.c_double),
                 ("fe6056_init", C.c_double), ("timesum", C.c_double)]
+
+
+class BodyStats(C.Structure):
+    """i2e_body_stats (include/i2e.h): impact()'s hydration and body
+    statistics (impact.c:3022-3228)."""
+    _fields_ = [(n, C.c_double) for n in (
+        "sol_frac", "liq_frac", "hydrous_frac", "primitive_frac", "n2co2_frac", "cocl_frac", "h2o_frac",
+        "phyllo1_frac", "phyllo2_frac", "phyllo3_frac", "phyllo4_frac", "perco_frac", "melt1_frac", "melt2_frac",
+        "maxtk", "t_max_body", "meantk", "t_mean_body")] + [
+        ("count_toohot", C.c_int64), ("count_tot", C.c_int64), ("count_body_nodes", C.c_int64)]
+

@@ -593,6 +593,8 @@ int i2e_destroy(i2e_ctx *c) {
                        &c->bcix, &c->bciy, &c->bLS, &c->xprev};
   c->blmask.release();
   c->tcnt.release();
+  c->hacc.release();
+  c->hdacc.release();
   c->mfl2.release();
   c->mt2.release();
   DBuf<int> *ib[] = {&c->morig, &c->morig2, &c->mkey, &c->mcnt, &c->moff, &c->mbsum, &c->mslot0};
@@ -701,6 +703,47 @@ int i2e_type_counts(i2e_ctx *c, int ntypes, int32_t *counts) {
   }
   HIP_OK(hipMemcpyAsync(counts, c->tcnt.p, n * sizeof(int), hipMemcpyDeviceToHost, c->s));
   HIP_OK(hipStreamSynchronize(c->s));
+  return I2E_OK;
+}
+
+int i2e_hydration(i2e_ctx *c, double t_sol, double t_liq, int hydrate, i2e_body_stats *out) {
+  if (!c || !out) return I2E_ERR_ARG;
+  ST_OK(c->hacc.alloc(18));
+  ST_OK(c->hdacc.alloc(3));
+  HIP_OK(hipMemsetAsync(c->hacc.p, 0, 18 * sizeof(unsigned long long), c->s));
+  HIP_OK(hipMemsetAsync(c->hdacc.p, 0, 3 * sizeof(double), c->s));
+  const double tamb = c->P.tmp_ambient;
+  hipLaunchKernelGGL(k_hyd_nodes, dim3(std::min(nblk(c->N, 256), 256u)), dim3(256), 0, c->s, c->N,
+                     (const double *)c->F.tk, tamb, c->hacc.p, c->hdacc.p);
+  if (c->M) {
+    const unsigned gb = std::min(nblk(c->M, 256), 2048u);
+    hipLaunchKernelGGL(k_hyd_markers, dim3(gb), dim3(256), 0, c->s, c->mk, c->M, t_sol, t_liq, hydrate,
+                       (const double *)c->hdacc.p + 2, tamb, c->hacc.p, c->hdacc.p);
+  }
+  HIP_OK(hipGetLastError());
+  unsigned long long h[18];
+  double d[3];
+  HIP_OK(hipMemcpyAsync(h, c->hacc.p, sizeof h, hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipMemcpyAsync(d, c->hdacc.p, sizeof d, hipMemcpyDeviceToHost, c->s));
+  HIP_OK(hipStreamSynchronize(c->s));
+  // counts in the order of k_hyd_markers: sol liq hydrous primitive n2co2
+  // cocl h2o phyllo1-4 perco melt1 melt2 tot toohot
+  const double tot = (double)h[14];
+  double *fr[14] = {&out->sol_frac,     &out->liq_frac,     &out->hydrous_frac, &out->primitive_frac,
+                    &out->n2co2_frac,   &out->cocl_frac,    &out->h2o_frac,     &out->phyllo1_frac,
+                    &out->phyllo2_frac, &out->phyllo3_frac, &out->phyllo4_frac, &out->perco_frac,
+                    &out->melt1_frac,   &out->melt2_frac};
+  for (int k = 0; k < 14; k++) *fr[k] = (double)h[k] / tot;  // 0/0 when nothing counts, as impact.c:3208-3222
+  out->count_tot = (int64_t)h[14];
+  out->count_toohot = (int64_t)h[15];
+  out->count_body_nodes = (int64_t)h[17];
+  out->maxtk = d[2];
+  out->meantk = d[1] / (double)h[17];
+  out->t_mean_body = d[0] / tot;
+  float tm;
+  const unsigned tb = (unsigned)h[16];
+  memcpy(&tm, &tb, sizeof tm);
+  out->t_max_body = h[16] ? (double)tm : tamb;
   return I2E_OK;
 }
 

@@ -54,6 +54,8 @@ struct i2e_ctx {
   int have_prev = 0;            // xprev holds a solution of this run (reset by i2e_upload)
   double dt_prev = 0;           // timestep of the last Stokes solve
   DBuf<int> tcnt;               // per-cell marker-type counts (i2e_type_counts)
+  DBuf<unsigned long long> hacc;  // body-statistics counters (i2e_hydration)
+  DBuf<double> hdacc;           // its sums and the largest node temperature
   // cell sort of the markers (sort.hip): second marker storage for the
   // permutation, the caller's index of each slot, keys, counts/offsets
   DBuf<float> mfl2;

@@ -61,6 +61,13 @@ __global__ void k_heat_markers_a(const i2e_params *p, Grid g, NodeF F, const dou
 __global__ void k_heat_nodes_norm(int N, const double *sol0, double *sol1);
 __global__ void k_heat_markers_b(Grid g, Markers mk, int64_t M, const double *sol1);
 __global__ void k_type_counts(Grid g, Markers mk, int64_t M, int ntypes, int *cnt);
+// hydration and body statistics (impact.c:3022-3228): acc[0..16) marker
+// counters, acc[16] bits of the largest silicate marker T above the ambient,
+// acc[17] body node count; dacc[0] sum of silicate marker T, dacc[1] sum of
+// body node T, dacc[2] largest node T
+__global__ void k_hyd_nodes(int N, const double *tk, double tamb, unsigned long long *acc, double *dacc);
+__global__ void k_hyd_markers(Markers mk, int64_t M, double t_sol, double t_liq, int hydrate, const double *maxtk,
+                              double tamb, unsigned long long *acc, double *dacc);
 
 // sort.hip (exclusive scans run in blocks of SB elements: 256 threads x 4)
 constexpr int SB = 1024;

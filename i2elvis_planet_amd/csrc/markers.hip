@@ -781,6 +781,95 @@ __global__ void k_heat_markers_b(Grid g, Markers mk, int64_t M, const double *__
 }
 
 // per-cell marker-type histogram (cell = m1serch,m2serch of the position)
+// ---------------------------------------------------------------- body statistics
+// The node pass (impact.c:3064-3076): the largest node temperature (at least
+// the ambient) and the sum and count of the body's nodes (tk != ambient).
+__global__ __launch_bounds__(256) void k_hyd_nodes(int N, const double *__restrict__ tk, double tamb,
+                                                  unsigned long long *acc, double *dacc) {
+  double mx = tamb, sm = 0;
+  unsigned cnt = 0;
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < N; m += gridDim.x * blockDim.x) {
+    const double t = tk[m];
+    mx = fmax(mx, t);
+    if (t != tamb) {
+      sm += t;
+      cnt++;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o));
+    sm += __shfl_xor(sm, o);
+    cnt += __shfl_xor(cnt, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (cnt) {
+      atomicAdd(&acc[17], (unsigned long long)cnt);
+      aadd(&dacc[1], sm);
+    }
+    // node temperatures are positive: their bit patterns order like them
+    atomicMax((unsigned long long *)&dacc[2], (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+// The marker pass (impact.c:3078-3206): the rheology switch between rock+ice
+// (5) and hydrous silicate (6) by the peak temperature, then the counts of
+// the silicate markers (5, 6, 25, 26) by type and peak temperature.
+__global__ __launch_bounds__(256) void k_hyd_markers(Markers mk, int64_t M, double t_sol, double t_liq, int hydrate,
+                                                    const double *__restrict__ maxtk, double tamb,
+                                                    unsigned long long *acc, double *dacc) {
+  enum { SOL, LIQ, HYDROUS, PRIMITIVE, N2CO2, COCL, H2O, PH1, PH2, PH3, PH4, PERCO, MELT1, MELT2, TOT, TOOHOT, NC };
+  unsigned c[NC];
+#pragma unroll
+  for (int k = 0; k < NC; k++) c[k] = 0;
+  unsigned tmaxb = 0;  // bits of the largest silicate marker T above the ambient (positive floats)
+  double tsum = 0;
+  const double mxt = *maxtk;
+  if (hydrate) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
+      int t = mk.t[i];
+      const double tm = (double)mk.tmax[i];
+      if (t == 5 && tm >= 273.15 && tm < 1223.15) t = 6;
+      if (t == 6 && tm >= 1223.15) t = 5;
+      if (t != mk.t[i]) mk.t[i] = (int8_t)t;
+      if (t == 5 || t == 6) c[SOL]++;
+      if (t == 25 || t == 26) c[LIQ]++;
+      if (t == 5 || t == 6 || t == 25 || t == 26) {
+        c[PRIMITIVE] += tm < 273.15;
+        c[HYDROUS] += tm >= 273.15 && tm < 1223.15;
+        c[N2CO2] += tm < 1073.15;
+        c[COCL] += tm < (0.10 * (t_liq - t_sol) + t_sol);
+        c[H2O] += tm < 1223.15;
+        c[PH1] += tm >= 273.15 && tm < 773.15;
+        c[PH2] += tm >= 773.15 && tm < 873.15;
+        c[PH3] += tm >= 873.15 && tm < 973.15;
+        c[PH4] += tm >= 973.15 && tm < 1223.15;
+        c[PERCO] += tm >= 1273.15;
+        c[MELT1] += tm >= t_sol;
+        c[MELT2] += tm >= (0.40 * (t_liq - t_sol) + t_sol);
+        c[TOT]++;
+        const float k = mk.k[i];
+        if ((double)k > tamb && __float_as_uint(k) > tmaxb) tmaxb = __float_as_uint(k);
+        tsum += (double)k;
+        if ((double)k / mxt > 1.1) c[TOOHOT]++;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NC; k++)
+    for (int o = 32; o > 0; o >>= 1) c[k] += __shfl_xor(c[k], o);
+  for (int o = 32; o > 0; o >>= 1) {
+    tsum += __shfl_xor(tsum, o);
+    tmaxb = max(tmaxb, (unsigned)__shfl_xor((int)tmaxb, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+      if (c[k]) atomicAdd(&acc[k], (unsigned long long)c[k]);
+    if (tmaxb) atomicMax(&acc[16], (unsigned long long)tmaxb);
+    if (tsum != 0) aadd(&dacc[0], tsum);
+  }
+}
+
 __global__ void k_type_counts(Grid g, Markers mk, int64_t M, int ntypes, int *cnt) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
     float fx = mk.x[i], fy = mk.y[i];

@@ -292,6 +292,26 @@ void *i2e_device_ptr(i2e_ctx *c, const char *name);
  * `every` steps (default 10; 0 = never; suspended while i2e_record runs);
  * i2e_sort_markers sorts now.  Extra device memory: 33 B per marker slot. */
 int i2e_set_marker_sort(i2e_ctx *c, int every);
+/* The per-step hydration and body-statistics sweep of impact()
+ * (impact.c:3022-3228) on the device-resident state.  Nodes: maxtk = max of
+ * tmp_ambient and every tk, meantk = mean tk over the nodes whose tk differs
+ * from tmp_ambient.  Markers, when `hydrate` is nonzero (the reference's
+ * growth_model == 0 && por_init <= 0.001): type 5 becomes 6 once marktmax
+ * reaches 273.15 K (below 1223.15 K) and 6 becomes 5 at 1223.15 K; then the
+ * silicate markers (types 5, 6, 25, 26) are counted by their current type
+ * and by marktmax against the classification thresholds, with the body's
+ * maximum and mean marker temperature and the markers hotter than 1.1 maxtk.
+ * t_sol and t_liq are impact()'s peridotite solidus and liquidus (1393.811
+ * or 1416.193 K and 1973 K, impact.c:188-198).  Fractions are counts over
+ * count_tot (0/0 when no marker counts, as in the reference).            */
+typedef struct i2e_body_stats {
+  double sol_frac, liq_frac, hydrous_frac, primitive_frac, n2co2_frac, cocl_frac, h2o_frac;
+  double phyllo1_frac, phyllo2_frac, phyllo3_frac, phyllo4_frac, perco_frac, melt1_frac, melt2_frac;
+  double maxtk, t_max_body, meantk, t_mean_body;
+  int64_t count_toohot, count_tot, count_body_nodes;
+} i2e_body_stats;
+int i2e_hydration(i2e_ctx *c, double t_sol, double t_liq, int hydrate, i2e_body_stats *out);
+
 /* The tile shapes the autotuner chose (autotune = 1), one entry per coarse
  * velocity level smoothed by tiles, finest first: the level's nodes along x
  * in levels[], the tile's width (along y) and height in sx[], sy[] (arrays

@@ -721,6 +721,75 @@ int orc_type_counts(const i2e_params *p, const i2e_host_state *s, int ntypes, in
   return I2E_OK;
 }
 
+/* hydration and body statistics, impact.c:3022-3228: the node pass
+ * (:3064-3076), the rheology switch and the classification of every silicate
+ * marker by its current type and peak temperature (:3078-3206), the fractions
+ * (:3208-3223), in the reference's order                                    */
+int orc_hydration(const i2e_params *p, i2e_host_state *s, double t_sol, double t_liq, int hydrate,
+                  i2e_body_stats *o) {
+  int64_t m1, mm1, N = p->xnumx * p->ynumy;
+  int64_t count_toohot = 0, count_body_nodes = 0, count_tot = 0, count_sol = 0, count_liq = 0, count_hydrous = 0,
+          count_primitive = 0, count_n2co2 = 0, count_cocl = 0, count_h2o = 0, count_phyllo1 = 0, count_phyllo2 = 0,
+          count_phyllo3 = 0, count_phyllo4 = 0, count_perco = 0, count_melt1 = 0, count_melt2 = 0;
+  double maxtk = p->tmp_ambient, meantk = 0.0, t_max_body = p->tmp_ambient, t_mean_body = 0.0;
+  for (m1 = 0; m1 < N; m1++) {
+    maxtk = MAXV(maxtk, s->tk[m1]);
+    if (s->tk[m1] != p->tmp_ambient) {
+      meantk += s->tk[m1];
+      count_body_nodes += 1;
+    }
+  }
+  meantk = meantk / (double)(count_body_nodes);
+  if (hydrate) {
+    for (mm1 = 0; mm1 < s->marknum; mm1++) {
+      if (s->markt[mm1] == 5 && s->marktmax[mm1] >= 273.15 && s->marktmax[mm1] < 1223.15) s->markt[mm1] = 6;
+      if (s->markt[mm1] == 6 && s->marktmax[mm1] >= 1223.15) s->markt[mm1] = 5;
+      if (s->markt[mm1] == 5 || s->markt[mm1] == 6) count_sol += 1;
+      if (s->markt[mm1] == 25 || s->markt[mm1] == 26) count_liq += 1;
+      if (s->markt[mm1] == 5 || s->markt[mm1] == 6 || s->markt[mm1] == 25 || s->markt[mm1] == 26) {
+        if (s->marktmax[mm1] < 273.15) count_primitive += 1;
+        if (s->marktmax[mm1] >= 273.15 && s->marktmax[mm1] < 1223.15) count_hydrous += 1;
+        if (s->marktmax[mm1] < 1073.15) count_n2co2 += 1;
+        if (s->marktmax[mm1] < (0.10 * (t_liq - t_sol) + t_sol)) count_cocl += 1;
+        if (s->marktmax[mm1] < 1223.15) count_h2o += 1;
+        if (s->marktmax[mm1] >= 273.15 && s->marktmax[mm1] < 773.15) count_phyllo1 += 1;
+        if (s->marktmax[mm1] >= 773.15 && s->marktmax[mm1] < 873.15) count_phyllo2 += 1;
+        if (s->marktmax[mm1] >= 873.15 && s->marktmax[mm1] < 973.15) count_phyllo3 += 1;
+        if (s->marktmax[mm1] >= 973.15 && s->marktmax[mm1] < 1223.15) count_phyllo4 += 1;
+        if (s->marktmax[mm1] >= 1273.15) count_perco += 1;
+        if (s->marktmax[mm1] >= t_sol) count_melt1 += 1;
+        if (s->marktmax[mm1] >= (0.40 * (t_liq - t_sol) + t_sol)) count_melt2 += 1;
+        count_tot += 1;
+        if (s->markk[mm1] > t_max_body) t_max_body = s->markk[mm1];
+        t_mean_body += s->markk[mm1];
+        if (s->markk[mm1] / maxtk > 1.1) count_toohot += 1;
+      }
+    }
+  }
+  o->sol_frac = (double)(count_sol) / (double)(count_tot);
+  o->liq_frac = (double)(count_liq) / (double)(count_tot);
+  o->hydrous_frac = (double)(count_hydrous) / (double)(count_tot);
+  o->primitive_frac = (double)(count_primitive) / (double)(count_tot);
+  o->n2co2_frac = (double)(count_n2co2) / (double)(count_tot);
+  o->cocl_frac = (double)(count_cocl) / (double)(count_tot);
+  o->h2o_frac = (double)(count_h2o) / (double)(count_tot);
+  o->phyllo1_frac = (double)(count_phyllo1) / (double)(count_tot);
+  o->phyllo2_frac = (double)(count_phyllo2) / (double)(count_tot);
+  o->phyllo3_frac = (double)(count_phyllo3) / (double)(count_tot);
+  o->phyllo4_frac = (double)(count_phyllo4) / (double)(count_tot);
+  o->perco_frac = (double)(count_perco) / (double)(count_tot);
+  o->melt1_frac = (double)(count_melt1) / (double)(count_tot);
+  o->melt2_frac = (double)(count_melt2) / (double)(count_tot);
+  o->t_mean_body = t_mean_body / (double)(count_tot);
+  o->maxtk = maxtk;
+  o->meantk = meantk;
+  o->t_max_body = t_max_body;
+  o->count_toohot = count_toohot;
+  o->count_tot = count_tot;
+  o->count_body_nodes = count_body_nodes;
+  return I2E_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* gausmat4 row builder, gausmart.c:320-403                                  */
 /* ------------------------------------------------------------------------ */

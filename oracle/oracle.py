@@ -58,6 +58,7 @@ def lib():
             ("orc_tbc_build", [P, S, Cp]), ("orc_heat_markers", [P, S, dp]),
             ("orc_type_counts", [P, S, C.c_int, C.POINTER(C.c_int32)]),
             ("orc_stokes_residuals", [P, S, dp]),
+            ("orc_hydration", [P, S, C.c_double, C.c_double, C.c_int, C.POINTER(abi.BodyStats)]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -344,6 +345,15 @@ class Oracle:
         _check(self.L.orc_type_counts(C.byref(self.p), self._s(), ntypes, out.ctypes.data_as(C.POINTER(C.c_int32))),
                "type_counts")
         return out.reshape(C_, ntypes)
+
+    def hydration(self, t_sol=1393.811, t_liq=1973.0, hydrate=True):
+        """impact()'s hydration and body statistics (impact.c:3022-3228);
+        switches marker types in place, returns an abi.BodyStats."""
+        o = abi.BodyStats()
+        _check(self.L.orc_hydration(C.byref(self.p), self._s(), t_sol, t_liq, int(bool(hydrate)), C.byref(o)),
+               "hydration")
+        self._pull()
+        return o
 
 
 # --------------------------------------------------------------------------- adjoint objective

@@ -173,6 +173,43 @@ def test_ronurecalc_soft_turbulence_and_magnetisation(orc, dynamo):
     assert rel(off.kt, ref.kt) > 1e-3
 
 
+@pytest.mark.parametrize("hydrate", [True, False])
+def test_hydration_statistics_match_the_oracle(orc, hydrate):
+    """impact()'s hydration and body-statistics sweep (impact.c:3022-3228) on
+    the device state at 101^2 with every silicate type and peak temperatures
+    across all thresholds, markers tile-sorted on the device: switched types,
+    counts, fractions, maximum temperatures exact; the mean temperatures to
+    1e-14 (sums in another order)."""
+    say("case hydration", hydrate)
+    p, hs, _ = small_cfg("c2", n=101, size=160e3, radius=40e3)
+    rng = np.random.default_rng(12)
+    hs.markt[:] = rng.choice(np.array([0, 5, 6, 7, 25, 26], np.int8), hs.marknum)
+    hs.marktmax[:] = rng.uniform(100, 1900, hs.marknum).astype(np.float32)
+    hs.markk[:] = rng.uniform(150, 1800, hs.marknum).astype(np.float32)
+    hs.tk[:] = np.where(rng.random(hs.N) < 0.4, p.tmp_ambient, rng.uniform(150, 1700, hs.N))
+    ref = hs.copy()
+    ctx = Context(p, hs, solver=fp64_opts())
+    ctx.sort_markers()
+    g = ctx.hydration(t_sol=1416.193, t_liq=1973.0, hydrate=hydrate)
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o = orc.Oracle(p, ref).hydration(t_sol=1416.193, t_liq=1973.0, hydrate=hydrate)
+    assert np.array_equal(got.markt, ref.markt)
+    for k in ("count_tot", "count_toohot", "count_body_nodes"):
+        assert getattr(g, k) == getattr(o, k), k
+    for k in ("sol", "liq", "hydrous", "primitive", "n2co2", "cocl", "h2o", "phyllo1", "phyllo2", "phyllo3",
+              "phyllo4", "perco", "melt1", "melt2"):
+        a, b = getattr(g, k + "_frac"), getattr(o, k + "_frac")
+        assert a == b or (math.isnan(a) and math.isnan(b)), k
+    assert g.maxtk == o.maxtk and g.t_max_body == o.t_max_body
+    assert g.meantk == pytest.approx(o.meantk, rel=1e-14)
+    if hydrate:
+        assert o.count_tot > 1000
+        assert g.t_mean_body == pytest.approx(o.t_mean_body, rel=1e-14)
+    else:
+        assert math.isnan(g.t_mean_body) and math.isnan(o.t_mean_body)
+
+
 # ------------------------------------------------------------------ operators
 def _dev(x):
     return DeviceArray(x)

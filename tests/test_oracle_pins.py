@@ -511,3 +511,64 @@ def test_magnetisation_state_table(orc, dynamo):
         else:
             want = (2 - b, now if o in MAG_NEW_TIME_ON[b] else t_old[i])
         assert (hs.markmg_old[i], hs.markmg_time[i]) == want, (i, b, o)
+
+
+# ------------------------------------------------------------------ body statistics
+def hydration_state(seed=3):
+    p, hs, _ = small(n=21, size=60e3, mnum=3, seed=seed)
+    rng = np.random.default_rng(seed)
+    M = hs.marknum
+    hs.markt[:] = rng.choice(np.array([0, 5, 6, 7, 25, 26], np.int8), M)
+    # peak temperatures on and around every threshold, and in between
+    edges = np.array([273.15, 773.15, 873.15, 973.15, 1073.15, 1223.15, 1273.15, 1393.811, 1451.7299, 1625.4866])
+    tm = np.concatenate([edges, np.nextafter(edges.astype(np.float32), np.float32(0)), rng.uniform(100, 1900, M)])
+    hs.marktmax[:] = tm[:M].astype(np.float32)
+    hs.markk[:] = rng.uniform(150, 1800, M).astype(np.float32)
+    hs.tk[:] = np.where(rng.random(hs.N) < 0.4, p.tmp_ambient, rng.uniform(150, 1700, hs.N))
+    return p, hs
+
+
+def test_hydration_classification_and_invariants(orc):
+    """impact()'s sweep (impact.c:3022-3228) on a state whose peak
+    temperatures sit on, just below and between every threshold: the type
+    switch (5 -> 6 at 273.15 K below 1223.15 K, 6 -> 5 from 1223.15 K), each
+    count against the thresholds, and the partitions the classes form
+    (solid + molten = hydrous + primitive + dehydrated = all; the four
+    phyllosilicate classes = hydrous; H2O retained = primitive + hydrous)."""
+    p, hs = hydration_state()
+    t0, tm = hs.markt.copy(), hs.marktmax.astype(np.float64)
+    o = orc.Oracle(p, hs).hydration(t_sol=1393.811, t_liq=1973.0)
+    t = t0.copy()
+    t[(t == 5) & (tm >= 273.15) & (tm < 1223.15)] = 6
+    t[(t == 6) & (tm >= 1223.15)] = 5
+    assert np.array_equal(hs.markt, t)
+    sil = np.isin(t, (5, 6, 25, 26))
+    x = tm[sil]
+    n = sil.sum()
+    want = {"sol": np.isin(t, (5, 6)).sum(), "liq": np.isin(t, (25, 26)).sum(), "primitive": (x < 273.15).sum(),
+            "hydrous": ((x >= 273.15) & (x < 1223.15)).sum(), "n2co2": (x < 1073.15).sum(),
+            "cocl": (x < 1393.811 + 0.1 * (1973.0 - 1393.811)).sum(), "h2o": (x < 1223.15).sum(),
+            "phyllo1": ((x >= 273.15) & (x < 773.15)).sum(), "phyllo2": ((x >= 773.15) & (x < 873.15)).sum(),
+            "phyllo3": ((x >= 873.15) & (x < 973.15)).sum(), "phyllo4": ((x >= 973.15) & (x < 1223.15)).sum(),
+            "perco": (x >= 1273.15).sum(), "melt1": (x >= 1393.811).sum(),
+            "melt2": (x >= 1393.811 + 0.4 * (1973.0 - 1393.811)).sum()}
+    assert o.count_tot == n > 100
+    for k, v in want.items():
+        assert getattr(o, k + "_frac") == v / n, k
+    f = {k: round(getattr(o, k + "_frac") * n) for k in want}
+    assert f["sol"] + f["liq"] == n
+    assert f["phyllo1"] + f["phyllo2"] + f["phyllo3"] + f["phyllo4"] == f["hydrous"]
+    assert f["primitive"] + f["hydrous"] == f["h2o"]
+    mk = hs.markk[sil].astype(np.float64)
+    body = hs.tk != p.tmp_ambient
+    assert o.maxtk == max(p.tmp_ambient, hs.tk.max())
+    assert o.count_body_nodes == body.sum()
+    assert o.meantk == pytest.approx(hs.tk[body].mean(), rel=1e-14)
+    assert o.t_max_body == max(p.tmp_ambient, mk.max())
+    assert o.t_mean_body == pytest.approx(mk.mean(), rel=1e-15)
+    assert o.count_toohot == (mk / o.maxtk > 1.1).sum()
+    # growth_model != 0 or initial porosity: no switch, 0/0 fractions
+    q, hq = hydration_state()
+    t1 = hq.markt.copy()
+    o2 = orc.Oracle(q, hq).hydration(hydrate=False)
+    assert np.array_equal(hq.markt, t1) and o2.count_tot == 0 and math.isnan(o2.sol_frac)
