@@ -649,6 +649,57 @@ def test_viterate_parity_201_fp64(orc):
         assert e < 1e-8, f
 
 
+@pytest.mark.parametrize("melt_eta,dt", [(1e15, None), (1e13, 1e7)])
+def test_viterate_high_contrast_fp64(orc, melt_eta, dt):
+    """The molten 501^2 geometry at 101^2 (same grid step and blob density,
+    core at 1800 K) with the melt viscosity raised until the GPU solve
+    converges: contrast 1e8 at dt = maxtmstep, 1e10 at dt = 1e7 s (at 1e10
+    and dt = maxtmstep the stabilisation term dominates the melt rows and the
+    solve diverges; from 1e14 it diverges at every dt).  FP64 to the metric's
+    1e-8 against the oracle's rows solved directly after Ruiz equilibration
+    (the unscaled LU is not accurate enough at these contrasts): vx, vy, P
+    within 1e-6 relative L2."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    say("case high contrast melt_eta", melt_eta, "dt", dt)
+    p, hs, cfg = configs.make_hot("c3", n=101, size=120e3, radius=30e3, nblob=16)
+    p.melt_eta = melt_eta
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = dt or p.maxtmstep
+    ctx = Context(p, hs.copy(), solver=fp64_opts(1e-8))
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.viterate()
+    st = ctx.stats()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o.gravity()
+    M = o.stokes_matrix()
+    A = M.scipy().tocsr()
+    r = np.ones(A.shape[0])
+    c = np.ones(A.shape[1])
+    B = A.copy()
+    for _ in range(10):
+        rs = np.sqrt(abs(B).max(axis=1).toarray().ravel())
+        cs = np.sqrt(abs(B).max(axis=0).toarray().ravel())
+        rs[rs == 0] = 1
+        cs[cs == 0] = 1
+        B = sp.diags(1 / rs) @ B @ sp.diags(1 / cs)
+        r /= rs
+        c /= cs
+    lu = spla.splu(B.tocsc())
+    y = lu.solve(r * M.b)
+    for _ in range(2):
+        y = y + lu.solve(r * M.b - B @ y)
+    o.stokes_post(c * y)
+    say("stokes its", st.stokes_iters, "relres", st.stokes_relres)
+    assert st.stokes_relres <= 1e-8
+    for f in ("vx", "vy", "pr"):
+        e = rel(getattr(got, f), getattr(hs, f))
+        say(f, e)
+        assert e < 1e-6, f
+
+
 def test_step_501_properties():
     """One full step of the 501^2 / 6.25M-marker setup: the Stokes solution
     satisfies the assembled rows to the metric's tolerance when checked
