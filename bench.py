@@ -357,14 +357,23 @@ def run(steps, warmup, skip_cpu, rk, minimal=False, name="c3"):
     snap = ctx.download(hs.copy())
     res = {}
     roof = None
+    ends = {}
     for mode in ("mixed", "fp64"):
         timer = (rk, ROOFLINE_EVERY.get(rk, 1)) if mode == "mixed" else None
         el, iters, reading = timed_steps(ctx, o, snap, mode, steps, timer)
         if reading:
             roof = reading
         res[mode] = (steps / el, el * 1e3 / steps, iters)
+        ends[mode] = ctx.download(snap.copy())  # after the timed region
         log("%s: %.3f steps/s, %.1f ms/step; stokes its %s" % (mode, steps / el, el * 1e3 / steps,
                                                                 [i[0] for i in iters]))
+    # each field of the mixed run against the FP64 run after the same steps
+    # from the same state (relative L2; both solve Stokes to 1e-8)
+    field_err = {}
+    for f in ("vx", "vy", "pr", "tk", "gp"):
+        a, b = getattr(ends["mixed"], f), getattr(ends["fp64"], f)
+        field_err[f] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    log("mixed vs fp64 after %d steps: %s" % (steps, {k: "%.1e" % v for k, v in field_err.items()}))
     # outside the timed region: per-kernel figures, the pre-melt regime with
     # a hot interior, the molten regime, standalone kernel timings
     kt, warm_res, hot = {}, None, None
@@ -408,6 +417,7 @@ def run(steps, warmup, skip_cpu, rk, minimal=False, name="c3"):
         "stokes_iters_mixed": [i[0] for i in it],
         "stokes_iters_fp64": [i[0] for i in res["fp64"][2]],
         "stokes_relres_mixed": [i[1] for i in it],
+        "mixed_vs_fp64_rel_l2": field_err,
         "heat_iters_mixed": [i[3] for i in it],
         "grav_iters_mixed": [i[2] for i in it],
         "phase_ms_mixed": {"stokes": float(np.mean([i[4] for i in it])), "heat": float(np.mean([i[5] for i in it])),

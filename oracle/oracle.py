@@ -7,7 +7,9 @@ solves per step, which it does like the reference: a direct solve of the
 assembled system (gausmart.c:174-318).  With Intel MKL's runtime present the
 solve is PARDISO with the reference's iparm settings (mtype 11, minimum
 degree, MPS scaling, 1e-13 pivot perturbation, <= 20 refinement steps);
-otherwise scipy's SuperLU.
+otherwise scipy's SuperLU.  A PARDISO answer whose relative residual in the
+assembled rows exceeds 1e-10 is replaced by SuperLU's (set SOLVER to
+"pardiso" to take its answers unchecked, as the reference does).
 """
 import ctypes as C
 import os

@@ -359,6 +359,33 @@ def test_full_step_parity_fp64(orc, name, nsteps, sort_every):
     ctx.close()
 
 
+def test_full_step_parity_201_fp64(orc):
+    """One full FP64 step of the 201^2 / 640k-marker setup (Arrhenius,
+    porosity and sintering, shear heating) against the oracle's step: dt,
+    velocity, pressure, temperature and gravity potential, marker types and
+    per-cell type counts.  The 201^2 Stokes system stalls near 5e-12 in FP64:
+    solved to 3e-11."""
+    say("case step c2")
+    p, hs, _ = configs.make("c2")
+    ref = hs.copy()
+    ctx = Context(p, hs, solver=fp64_opts(3e-11))
+    ctx.ronurecalc()
+    ctx.step()
+    got = ctx.download(hs.copy())
+    counts = ctx.type_counts()
+    ctx.close()
+    o = orc.Oracle(p, ref)
+    o.ronurecalc()
+    o.step()
+    say("dt", got.timestep, ref.timestep, "vx", rel(got.vx, ref.vx), "tk", rel(got.tk, ref.tk))
+    assert got.timestep == pytest.approx(ref.timestep, rel=1e-8)
+    for f in ("vx", "vy", "pr", "tk", "gp"):
+        assert rel(getattr(got, f), getattr(ref, f)) < 1e-8, f
+    assert np.array_equal(got.markt, ref.markt)
+    np.testing.assert_allclose(got.markpor, ref.markpor, rtol=1e-6, atol=1e-7)
+    assert np.array_equal(counts, o.type_counts())
+
+
 def test_marker_sort_is_invisible_at_the_boundary(orc):
     """The cell sort permutes the device markers only: after sorting, the
     device order is nondecreasing in the tile-major cell key, download returns the
