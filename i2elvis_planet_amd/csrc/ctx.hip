@@ -1334,8 +1334,13 @@ int i2e_titerate(i2e_ctx *c, int n0) {
   if (c->M) {
     unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
     c->kt.begin(K_HEATMARK, c->s);
-    hipLaunchKernelGGL(k_heat_markers_a, dim3(gb), dim3(256), 0, c->s, c->dP, g, c->F, c->tk2.p, c->mk, c->M, dt, s0,
-                       s1);
+    if (c->permuted)
+      hipLaunchKernelGGL(k_heat_markers_a<true>, dim3(tile_count(c->X, c->Y)), dim3(256), 0, c->s, c->dP, g, c->F,
+                         (const double *)c->tk2.p, c->mk, c->M, dt, s0, s1, (const int *)c->moff.p,
+                         tiles_per_col(c->Y));
+    else
+      hipLaunchKernelGGL(k_heat_markers_a<false>, dim3(gb), dim3(256), 0, c->s, c->dP, g, c->F,
+                         (const double *)c->tk2.p, c->mk, c->M, dt, s0, s1, (const int *)nullptr, 0);
     if (c->P.heatdif) {
       hipLaunchKernelGGL(k_heat_nodes_norm, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, s0, s1);
       hipLaunchKernelGGL(k_heat_markers_b, dim3(gb), dim3(256), 0, c->s, g, c->mk, c->M, s1);

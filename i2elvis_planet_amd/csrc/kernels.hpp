@@ -56,8 +56,9 @@ __global__ void k_tref(const i2e_params *p, Grid g, Markers mk, const int *slot0
 __global__ void k_ronu_nodes(const i2e_params *p, NodeF F, Acc A, int N, double *mm);
 __global__ void k_contrast(const i2e_params *p, NodeF F, int N, const double *mm);
 __global__ void k_tbc(NodeF F, BcTab bc, int N);
+template <bool TILED>
 __global__ void k_heat_markers_a(const i2e_params *p, Grid g, NodeF F, const double *tk2, Markers mk, int64_t M,
-                                 double timestep, double *sol0, double *sol1);
+                                 double timestep, double *sol0, double *sol1, const int *toff, int ntj);
 __global__ void k_heat_nodes_norm(int N, const double *sol0, double *sol1);
 __global__ void k_heat_markers_b(Grid g, Markers mk, int64_t M, const double *sol1);
 __global__ void k_type_counts(Grid g, Markers mk, int64_t M, int ntypes, int *cnt);

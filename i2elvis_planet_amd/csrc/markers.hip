@@ -688,17 +688,38 @@ __global__ void k_tbc(NodeF F, BcTab bc, int N) {
 
 // ---------------------------------------------------------------- marker T
 // heatmart.c:529-627: subgrid diffusion + scatter of the diffusion change
+// TILED: one block per marker tile of the last cell sort, the two node sums
+// staged in LDS as in k_ronu_markers
+template <bool TILED>
 __global__ __launch_bounds__(256) void k_heat_markers_a(const i2e_params *__restrict__ p, Grid g, NodeF F,
                                                        const double *__restrict__ tk2, Markers mk, int64_t M,
-                                                       double timestep, double *sol0, double *sol1) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+                                                       double timestep, double *sol0, double *sol1,
+                                                       const int *__restrict__ toff, int ntj) {
+  constexpr int TN = kTile + 1, TA = TN * TN;
+  __shared__ double tacc[TILED ? 2 * TA : 1];
   const int lane = threadIdx.x & 63;
+  int ti0 = 0, tj0 = 0;
+  int64_t lo, hi, stride;
+  if (TILED) {
+    const int kk = kTile * kTile, T = blockIdx.x;
+    lo = T ? toff[T * kk - 1] : 0;
+    hi = toff[T * kk + kk - 1];
+    stride = blockDim.x;
+    ti0 = (T / ntj) * kTile;
+    tj0 = (T % ntj) * kTile;
+    for (int k = threadIdx.x; k < 2 * TA; k += blockDim.x) tacc[k] = 0;
+    __syncthreads();
+  } else {
+    lo = (int64_t)blockIdx.x * blockDim.x;
+    hi = M;
+    stride = (int64_t)gridDim.x * blockDim.x;
+  }
   // whole wavefronts iterate together: the scatter reduces across lanes
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < M; base += stride) {
+  for (int64_t base = lo; base < hi; base += stride) {
     const int64_t m3 = base + threadIdx.x;
     double s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
     int key = -1;
-    if (m3 < M) {
+    if (m3 < hi) {
       float fx = mk.x[m3], fy = mk.y[m3];
       if (fx > 0 && fy > 0 && (double)fx < g.xsize && (double)fy < g.ysize && mk.t[m3] < 50) {
         double x = fx, y = fy;
@@ -740,17 +761,39 @@ __global__ __launch_bounds__(256) void k_heat_markers_a(const i2e_params *__rest
     }
     if (sc.tail && key >= 0) {
       const int m10 = key / g.Y, m20 = key % g.Y;
+      const bool in_tile =
+          TILED && (unsigned)(m10 - ti0) < (unsigned)kTile && (unsigned)(m20 - tj0) < (unsigned)kTile;
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
-          const int m4 = (m10 + a) * g.Y + m20 + b;
-          aadd(&sol0[m4], s0[a * 2 + b]);
-          aadd(&sol1[m4], s1[a * 2 + b]);
+          if (in_tile) {
+            const int l = (m10 - ti0 + a) * TN + (m20 - tj0 + b);
+            atomicAdd(&tacc[l], s0[a * 2 + b]);
+            atomicAdd(&tacc[TA + l], s1[a * 2 + b]);
+          } else {
+            const int m4 = (m10 + a) * g.Y + m20 + b;
+            aadd(&sol0[m4], s0[a * 2 + b]);
+            aadd(&sol1[m4], s1[a * 2 + b]);
+          }
         }
     }
   }
+  if (TILED && p->heatdif) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < TA; k += blockDim.x) {
+      const int i = ti0 + k / TN, j = tj0 + k % TN;
+      if (i >= g.X || j >= g.Y) continue;
+      const double a = tacc[k], b = tacc[TA + k];
+      if (a != 0) aadd(&sol0[i * g.Y + j], a);
+      if (b != 0) aadd(&sol1[i * g.Y + j], b);
+    }
+  }
 }
+template __global__ void k_heat_markers_a<false>(const i2e_params *, Grid, NodeF, const double *, Markers, int64_t,
+                                                 double, double *, double *, const int *, int);
+template __global__ void k_heat_markers_a<true>(const i2e_params *, Grid, NodeF, const double *, Markers, int64_t,
+                                                double, double *, double *, const int *, int);
 
 __global__ void k_heat_nodes_norm(int N, const double *sol0, double *sol1) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
