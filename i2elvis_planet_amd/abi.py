@@ -63,6 +63,7 @@ class Params(C.Structure):
         ("meltable", C.c_uint8 * MAXTYPES),
         ("core_form_time", C.c_double), ("fe_ts", C.c_double), ("fe_tl", C.c_double),
         ("soft_turb", C.c_int32), ("magnet", C.c_int32), ("dynamo", C.c_int32), ("pad1", C.c_int32),
+        ("immobile", C.c_uint8 * MAXTYPES),
     ]
 
 

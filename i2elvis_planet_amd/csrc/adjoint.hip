@@ -628,10 +628,16 @@ static inline unsigned mgrid(int64_t M) { return nblk(M, 256) < 65535u ? nblk(M,
 // pressure-independent density and conductivity, no melting, no sintering.
 static int adj_supported(const i2e_ctx *c) {
   const i2e_params &P = c->P;
-  if (P.melt_on || P.porosity_on || P.adiabyn > 0) {
-    set_error("adjoint: melting, porosity evolution and adiabatic heating are not differentiated");
+  if (P.melt_on || P.porosity_on || P.adiabyn > 0 || P.soft_turb || P.magnet) {
+    set_error("adjoint: melting, porosity evolution, adiabatic heating and the soft-turbulence conductivity "
+              "are not differentiated");
     return I2E_ERR_UNSUPPORTED;
   }
+  for (int t = 0; t < I2E_MAXTYPES; t++)
+    if (P.immobile[t]) {
+      set_error("adjoint: immobile rock types are not differentiated");
+      return I2E_ERR_UNSUPPORTED;
+    }
   for (int t = 0; t < I2E_MAXTYPES; t++) {
     const i2e_rock &r = P.rock[t];
     if (r.aa != 0 || r.bb != 0 || r.kf != 0 || r.kp != 0) {

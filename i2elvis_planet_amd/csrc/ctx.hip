@@ -1372,18 +1372,32 @@ static void launch_movemark(i2e_ctx *c, bool timed = false) {
 
 // marker properties and their node sums: per tile of the last cell sort
 // (node sums staged in LDS) once the markers are sorted, else grid-stride
-static void launch_ronu_markers(i2e_ctx *c) {
+extern "C++" {
+template <bool IMM>
+static void launch_ronu_markers_t(i2e_ctx *c) {
   const HeatK hk = heat_per_kg(c->P, c->timesum);
   const double *tref = (const double *)c->red.p + kTrefSlot;
   if (c->permuted) {
-    hipLaunchKernelGGL(k_ronu_markers<true>, dim3(tile_count(c->X, c->Y)), dim3(256), 0, c->s, c->dP, c->grid(),
-                       c->F, c->mk, c->M, c->A, hk, c->timestep, c->timesum, c->need_eii, tref,
+    hipLaunchKernelGGL((k_ronu_markers<true, IMM>), dim3(tile_count(c->X, c->Y)), dim3(256), 0, c->s, c->dP,
+                       c->grid(), c->F, c->mk, c->M, c->A, hk, c->timestep, c->timesum, c->need_eii, tref,
                        (const int *)c->moff.p, tiles_per_col(c->Y));
   } else {
     const unsigned gb = nblk(c->M, 256) < 65535u ? nblk(c->M, 256) : 65535u;
-    hipLaunchKernelGGL(k_ronu_markers<false>, dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk, c->M,
-                       c->A, hk, c->timestep, c->timesum, c->need_eii, tref, nullptr, 0);
+    hipLaunchKernelGGL((k_ronu_markers<false, IMM>), dim3(gb), dim3(256), 0, c->s, c->dP, c->grid(), c->F, c->mk,
+                       c->M, c->A, hk, c->timestep, c->timesum, c->need_eii, tref, (const int *)nullptr, 0);
   }
+}
+}  // extern "C++"
+static bool any_immobile(const i2e_params &p) {
+  for (int t = 0; t < I2E_MAXTYPES; t++)
+    if (p.immobile[t]) return true;
+  return false;
+}
+static void launch_ronu_markers(i2e_ctx *c) {
+  if (any_immobile(c->P))
+    launch_ronu_markers_t<true>(c);
+  else
+    launch_ronu_markers_t<false>(c);
 }
 
 int i2e_movemark(i2e_ctx *c) {
@@ -1408,7 +1422,7 @@ int i2e_ronurecalc(i2e_ctx *c) {
     launch_ronu_markers(c);
   }
   hipLaunchKernelGGL(k_ronu_nodes, dim3(std::min(nblk(N, 256), 256u)), dim3(256), 0, c->s, c->dP, c->F, c->A, N,
-                     c->red.p + 16);
+                     c->red.p + 16, any_immobile(c->P) ? 1 : 0);
   c->kt.end(K_P2G, c->s, p2g_bytes(c), p2g_repr(c));
   hipLaunchKernelGGL(k_contrast, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->dP, c->F, N, c->red.p + 16);
   hipLaunchKernelGGL(k_tbc, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->F, c->bc, N);

@@ -113,6 +113,7 @@ static void build_params(i2e_params *p) {
     r->kt = markkt[t];
     r->kf = markkf[t];
     r->kp = markkp[t];
+    p->immobile[t] = markim[t] != 0;
   }
   p->is_silicate[5] = p->is_silicate[6] = p->is_silicate[25] = p->is_silicate[26] = 1;
   for (t = 7; t <= 10; t++) p->is_iron[t] = 1;
@@ -169,17 +170,12 @@ static void enter(const char *fn) {
   i2e_params p;
   i2e_solver_opts o;
   i2e_host_state s;
-  int st, t;
+  int st;
   if (sizeof(long int) != sizeof(int64_t)) die(fn, I2E_ERR_UNSUPPORTED);
   if (rocknum > I2E_MAXTYPES) {
     fprintf(stderr, "%s: %d rock types, at most %d supported\n", fn, rocknum, I2E_MAXTYPES);
     exit(I2E_ERR_CAPACITY);
   }
-  for (t = 0; t < rocknum; t++)
-    if (markim[t]) {
-      fprintf(stderr, "%s: rock type %d is immobile (markim); immobile markers are not supported\n", fn, t);
-      exit(I2E_ERR_UNSUPPORTED);
-    }
   build_params(&p);
   i2e_default_solver(&o);
   o.precision = I2E_PREC_FP64;

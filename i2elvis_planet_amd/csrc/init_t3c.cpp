@@ -93,7 +93,7 @@ struct Gen {
   std::vector<double> bondv;          // 4 per row
   int rocknum = 0;
   i2e_rock rock[I2E_MAXTYPES];
-  int markim_any = 0;
+  uint8_t markim[I2E_MAXTYPES] = {};  // immobile rock types ('i' prefix)
   GlibcRand rnd;
   Tokens tk_in;
   std::string err;
@@ -244,14 +244,10 @@ bool Gen::rocks() {
   memset(rock, 0, sizeof rock);
   if (!tok(s)) return false;
   while (s[0] != '~') {
-    int n1;
-    if (s[0] == 'i') {
-      n1 = atoi(s.c_str() + 1);
-      markim_any = 1;
-    } else {
-      n1 = atoi(s.c_str());
-    }
+    const bool im = s[0] == 'i';
+    const int n1 = atoi(s.c_str() + (im ? 1 : 0));
     if (n1 < 0 || n1 >= I2E_MAXTYPES) return fail("init: rock type " + s + " outside 0..63");
+    markim[n1] = im ? 1 : 0;
     i2e_rock &r = rock[n1];
     r.n0 = dtok();
     r.n1 = dtok();
@@ -274,7 +270,6 @@ bool Gen::rocks() {
     if (!tok(s)) return false;
     rocknum = rocknum > n1 + 1 ? rocknum : n1 + 1;
   }
-  if (markim_any) return fail("init: immobile rock types (i prefix) are not supported");
   return err.empty();
 }
 
@@ -816,7 +811,10 @@ int i2e_init_generate(const char *init_path, i2e_params *p, i2e_host_state *s) {
   p->fe6056_init = g.fe6056;
   p->por_init = g.por_init;
   p->ntypes = g.rocknum;
-  for (int t = 0; t < I2E_MAXTYPES; t++) p->rock[t] = g.rock[t];
+  for (int t = 0; t < I2E_MAXTYPES; t++) {
+    p->rock[t] = g.rock[t];
+    p->immobile[t] = g.markim[t];
+  }
   memcpy(s->gx, g.gx.data(), sizeof(double) * g.X);
   memcpy(s->gy, g.gy.data(), sizeof(double) * g.Y);
   double *zero[] = {s->pr, s->gp, s->ga, s->exx, s->eyy, s->exy, s->sxx, s->syy, s->sxy,

@@ -48,12 +48,12 @@ HeatK heat_per_kg(const i2e_params &p, double timesum);
 __global__ void k_movemark(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, double timestep, int need_eii);
 // TILED: one block per marker tile of the last cell sort (toff: the sort's
 // key offsets, ntj tiles per column of tiles), node sums staged in LDS
-template <bool TILED>
+template <bool TILED, bool IMM>
 __global__ void k_ronu_markers(const i2e_params *p, Grid g, NodeF F, Markers mk, int64_t M, Acc A, HeatK hk,
                                double timestep, double timesum, int need_eii, const double *tref, const int *toff,
                                int ntj);
 __global__ void k_tref(const i2e_params *p, Grid g, Markers mk, const int *slot0, double *tref);
-__global__ void k_ronu_nodes(const i2e_params *p, NodeF F, Acc A, int N, double *mm);
+__global__ void k_ronu_nodes(const i2e_params *p, NodeF F, Acc A, int N, double *mm, int imm);
 __global__ void k_contrast(const i2e_params *p, NodeF F, int N, const double *mm);
 __global__ void k_tbc(NodeF F, BcTab bc, int N);
 template <bool TILED>

@@ -137,6 +137,7 @@ __global__ __launch_bounds__(256) void k_movemark(const i2e_params *__restrict__
     int mm2 = t >= 100 ? t - 100 : t;
     float fx = mk.x[mm1], fy = mk.y[mm1];
     if (!((fx >= 0 && fy >= 0 && (double)fx <= g.xsize && (double)fy <= g.ysize) || p->outgrid != 1)) continue;
+    if ((unsigned)mm2 < (unsigned)I2E_MAXTYPES && p->immobile[mm2]) continue;  // markmart.c:27
     Ev e;
     allintere_d(g, F, (double)fx, (double)fy, e, need_eii != 0);
     double mpb = e.pr * 1e-5, mtk = (double)mk.k[mm1];
@@ -271,13 +272,15 @@ __global__ void k_tref(const i2e_params *__restrict__ p, Grid g, Markers mk, con
 // an LDS copy of the tile's (kTile+1)^2 nodes, flushed with one atomic per
 // node and field at the end.  Markers that left their tile since the sort
 // take the atomics to HBM; any order stays correct.
-template <bool TILED>
+// IMM: some rock type is immobile: its markers' temperatures stay out of the
+// node temperature, which then has a weight sum of its own (w1)
+template <bool TILED, bool IMM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(I2E_RONU_WAVES))) void k_ronu_markers(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
                                                      int64_t M, Acc A, HeatK hk, double timestep, double timesum,
                                                      int need_eii, const double *__restrict__ tref,
                                                      const int *__restrict__ toff, int ntj) {
-  constexpr int TN = kTile + 1, TA = TN * TN, NF = 11;
-  // LDS node sums of the tile: nu0 wnu ro0 et0 cp0 kt0 ht0 w0 tk0 nd0 wnd
+  constexpr int TN = kTile + 1, TA = TN * TN, NF = IMM ? 12 : 11;
+  // LDS node sums of the tile: nu0 wnu ro0 et0 cp0 kt0 ht0 w0 tk0 nd0 wnd (w1)
   __shared__ double tacc[TILED ? NF * TA : 1];
   const int lane = threadIdx.x & 63;
   int ti0 = 0, tj0 = 0;
@@ -320,6 +323,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(I2E_RONU_WA
     A.nu0 = a0 + 6 * N;
     A.nd0 = a0 + 7 * N;
     A.w0 = a0 + 8 * N;
+    A.w1 = a0 + 9 * N;
     A.wnu = a0 + 10 * N;
     A.wnd = a0 + 11 * N;
     mk.k = mk.x + 2 * MS;
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(I2E_RONU_WA
     bool act = mm1 < hi;
     // the marker's properties and its four scatter weights (0 off the
     // active markers); the per-corner products are formed at the scatter
-    double s_ro = 0, s_bb = 0, s_cp = 0, s_kt = 0, s_ht = 0, s_tk = 0, s_nu = 0;
+    double s_ro = 0, s_bb = 0, s_cp = 0, s_kt = 0, s_ht = 0, s_tk = 0, s_nu = 0, s_mob = 0;
     double wx0 = 0, wx1 = 0, wy0 = 0, wy1 = 0, c_nd = 0, c_wnd = 0;
     int key = -1;
     if (act) {
@@ -515,6 +519,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(I2E_RONU_WA
         s_ht = mht;
         s_tk = mtk;
         s_nu = mnu;
+        if (IMM) {  // markmart.c:905: immobile markers carry no temperature
+          s_mob = p->immobile[mm2] ? 0.0 : 1.0;
+          s_tk *= s_mob;
+        }
       }
     }
     // one atomic per run of lanes in the same cell (lanes are cell-sorted in
@@ -554,6 +562,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(I2E_RONU_WA
         const double r0 = sc.sum(s_ro * swt), r1 = sc.sum(s_bb * swt), r2 = sc.sum(s_cp * swt);
         const double r3 = sc.sum(s_kt * swt), r4 = sc.sum(s_ht * swt), r5 = sc.sum(swt);
         const double r6 = sc.sum(s_tk * swt), r8 = sc.sum(v8), r9 = sc.sum(v9);
+        const double r10 = IMM ? sc.sum(s_mob * swt) : 0.0;
         if (put) {
           if (r9 != 0) {
             add(0, A.nu0, a, b, r8);
@@ -566,12 +575,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(I2E_RONU_WA
           add(6, A.ht0, a, b, r4);
           add(7, A.w0, a, b, r5);
           add(8, A.tk0, a, b, r6);
+          if (IMM) add(11, A.w1, a, b, r10);
         }
       }
   }
   if (TILED) {
     __syncthreads();
-    double *const dst[NF] = {A.nu0, A.wnu, A.ro0, A.et0, A.cp0, A.kt0, A.ht0, A.w0, A.tk0, A.nd0, A.wnd};
+    double *const dst[12] = {A.nu0, A.wnu, A.ro0, A.et0, A.cp0, A.kt0, A.ht0, A.w0, A.tk0, A.nd0, A.wnd, A.w1};
     for (int k = threadIdx.x; k < TA; k += blockDim.x) {
       const int i = ti0 + k / TN, j = tj0 + k % TN;
       if (i >= g.X || j >= g.Y) continue;
@@ -583,10 +593,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(I2E_RONU_WA
     }
   }
 }
-template __global__ void k_ronu_markers<false>(const i2e_params *, Grid, NodeF, Markers, int64_t, Acc, HeatK, double,
-                                               double, int, const double *, const int *, int);
-template __global__ void k_ronu_markers<true>(const i2e_params *, Grid, NodeF, Markers, int64_t, Acc, HeatK, double,
-                                              double, int, const double *, const int *, int);
+#define I2E_RONU_INST(TL, IM)                                                                               \
+  template __global__ void k_ronu_markers<TL, IM>(const i2e_params *, Grid, NodeF, Markers, int64_t, Acc, HeatK, \
+                                                  double, double, int, const double *, const int *, int);
+I2E_RONU_INST(false, false)
+I2E_RONU_INST(true, false)
+I2E_RONU_INST(false, true)
+I2E_RONU_INST(true, true)
+#undef I2E_RONU_INST
 
 // positive doubles order like their bit patterns
 __device__ __forceinline__ void amin_pos(double *a, double v) {
@@ -602,7 +616,7 @@ __device__ __forceinline__ void amax_pos(double *a, double v) {
 // combined with one atomic pair per block (one pair per wavefront on the
 // same two addresses serialised to ~95 us at 501^2).
 __global__ __launch_bounds__(256) void k_ronu_nodes(const i2e_params *__restrict__ p, NodeF F, Acc A, int N,
-                                                   double *mm) {
+                                                   double *mm, int imm) {
   double lmin = 1e300, lmax = 0;
   for (int m3 = blockIdx.x * blockDim.x + threadIdx.x; m3 < N; m3 += gridDim.x * blockDim.x) {
     if (A.wnu[m3]) {
@@ -628,7 +642,11 @@ __global__ __launch_bounds__(256) void k_ronu_nodes(const i2e_params *__restrict
       F.cp[m3] = A.cp0[m3] / w0;
       F.kt[m3] = A.kt0[m3] / w0;
       F.ht[m3] = A.ht0[m3] / w0;
-      F.tk[m3] = A.tk0[m3] / w0;  // T's weight sum is the same w0 (markmart.c:905-911)
+      // T's weight sum is w0 unless some rock type is immobile (markmart.c:905-911)
+      if (!imm)
+        F.tk[m3] = A.tk0[m3] / w0;
+      else if (A.w1[m3])
+        F.tk[m3] = A.tk0[m3] / A.w1[m3];
     }
   }
   for (int o = 32; o > 0; o >>= 1) {

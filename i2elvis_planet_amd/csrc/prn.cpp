@@ -189,7 +189,7 @@ int i2e_prn_write(const char *path, const i2e_params *p, const i2e_host_state *s
   w.v<int64_t>(s->bondnum);
   w.v<int32_t>(n0);
   w.v<double>(s->timesum / YEAR);
-  for (int t = 0; t < rocknum; t++) w.v<int32_t>(0);  // markim
+  for (int t = 0; t < rocknum; t++) w.v<int32_t>(p->immobile[t] ? 1 : 0);  // markim
   for (int k = 0; k < NDROCK; k++)
     for (int t = 0; t < rocknum; t++) w.v<double>(rock_value(p->rock[t], k));
   for (int64_t m = 0; m < N && w.ok; m++) {
@@ -287,15 +287,11 @@ int i2e_prn_read(const char *path, i2e_params *p, i2e_host_state *s, int *n0) {
   s->timesum = h.timesum;
   if (n0) *n0 = h.n0;
   // markim: immobile rock types are never advected and are left out of the
-  // temperature sums (markmart.c:27, 905); this build moves every marker, so
-  // a file that marks any type immobile is refused rather than run as mobile
+  // temperature sums (markmart.c:27, 905)
+  for (int t = 0; t < I2E_MAXTYPES; t++) p->immobile[t] = 0;
   for (int t = 0; t < h.rocknum; t++) {
     const int32_t im = r.v<int32_t>();
-    if (r.ok && im != 0) {
-      fclose(r.f);
-      i2e::set_error("prn: rock type %d is immobile (markim = %d); immobile markers are not supported", t, im);
-      return I2E_ERR_UNSUPPORTED;
-    }
+    if (t < I2E_MAXTYPES) p->immobile[t] = im != 0;
   }
   for (int k = 0; k < NDROCK; k++)
     for (int t = 0; t < h.rocknum; t++) set_rock_value(p->rock[t], k, r.v<double>());

@@ -121,6 +121,10 @@ typedef struct i2e_params {
    * planetary dynamo runs (decided by core.c in the reference).           */
   int32_t magnet, dynamo;
   int32_t pad1;
+  /* immobile rock types (markim): their markers are never advected
+   * (markmart.c:27) and are left out of the temperature scatter
+   * (markmart.c:905)                                                       */
+  uint8_t immobile[I2E_MAXTYPES];
 } i2e_params;
 
 /* ---- host state: the reference globals as raw pointers ------------------ */

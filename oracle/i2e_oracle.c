@@ -333,6 +333,7 @@ static void move_one(const i2e_params *p, const i2e_host_state *s, const G *g, i
          (double)(s->marky[mm1]) <= g->ysize) ||
         p->outgrid != 1))
     return;
+  if (mm2 >= 0 && mm2 < I2E_MAXTYPES && p->immobile[mm2]) return; /* markmart.c:27 */
   allintere(g, s, (double)(s->markx[mm1]), (double)(s->marky[mm1]), &e);
   mpb = e.pr * 1e-5;
   mtk = (double)(s->markk[mm1]);
@@ -631,8 +632,10 @@ static void marker_props_scatter(const i2e_params *p, i2e_host_state *s, const G
         A->kt0[m3] += mkt * swt;
         A->ht0[m3] += mht * swt;
         A->w0[m3] += swt;
-        A->tk0[m3] += mtk * swt;
-        A->w1[m3] += swt;
+        if (!p->immobile[mm2]) { /* markmart.c:905 */
+          A->tk0[m3] += mtk * swt;
+          A->w1[m3] += swt;
+        }
       }
   }
 }

@@ -210,6 +210,42 @@ def test_hydration_statistics_match_the_oracle(orc, hydrate):
         assert math.isnan(g.t_mean_body) and math.isnan(o.t_mean_body)
 
 
+@pytest.mark.parametrize("sort", [False, True])
+def test_immobile_types_match_the_oracle(orc, sort):
+    """Immobile rock types (markim): movemark leaves their markers where they
+    are, bit-exact, and ronurecalc keeps their temperatures out of the node
+    temperature (the scatter's IMM variant with its own weight sum), on the
+    grid-stride and the tiled scatter."""
+    say("case immobile, sorted", sort)
+    p, hs, _ = small_cfg("c2", n=81, size=160e3, radius=40e3)
+    p.immobile[configs.IRON] = 1
+    rotation_field(hs, noise=0.3)
+    rng = np.random.default_rng(15)
+    hs.markk[:] = rng.uniform(150, 1500, hs.marknum).astype(np.float32)
+    hs.pr[:] = 1e6 * rng.random(hs.N)
+    hs.timestep = 0.05 * (hs.gx[1] - hs.gx[0]) / np.abs(hs.vx).max()
+    hs.timesum = 3e12
+    ref = hs.copy()
+    ctx = Context(p, hs, solver=fp64_opts())
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    if sort:
+        ctx.sort_markers()
+    ctx.movemark()
+    ctx.ronurecalc()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o = orc.Oracle(p, ref)
+    o.movemark()
+    o.ronurecalc()
+    iron = ref.markt == configs.IRON
+    assert iron.sum() > 100
+    assert np.array_equal(got.markx, ref.markx) and np.array_equal(got.marky, ref.marky)
+    assert np.array_equal(got.markx[iron], hs.markx[iron])
+    assert np.array_equal(got.markt, ref.markt) and np.array_equal(got.markk, ref.markk)
+    for f in ("nu", "nd", "ro", "cp", "kt", "ht", "tk", "et"):
+        assert rel(getattr(got, f), getattr(ref, f)) < 1e-10, f
+
+
 # ------------------------------------------------------------------ operators
 def _dev(x):
     return DeviceArray(x)

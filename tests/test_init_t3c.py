@@ -189,12 +189,18 @@ X 6 10 0 0 5000 0 10000 0 0 0
             assert tk[i, j] == pytest.approx(t, rel=1e-12, abs=1e-9), (i, j)
 
 
+def test_immobile_rock_types(tmp_path):
+    """An 'i' before a rock number (in2mart.c:221-231) marks the type
+    immobile in the params' markim table; the others stay mobile."""
+    imm = write(tmp_path, FREE_SLIP + "~\n~\n", name="imm.t3c")
+    txt = open(imm).read()
+    assert "      7   1e+17" in txt
+    open(imm, "w").write(txt.replace("      7   1e+17", "      i7   1e+17"))
+    p, hs = init_t3c.generate(imm)[:2]
+    assert p.immobile[7] == 1 and sum(p.immobile) == 1
+
+
 def test_unsupported_inputs_are_refused(tmp_path):
-    imm = write(tmp_path, FREE_SLIP + "~\n~\n", name="imm.t3c").replace("imm", "imm")
-    txt = open(imm).read().replace("      7   1e+17", "      i7   1e+17")
-    open(imm, "w").write(txt)
-    with pytest.raises(I2EError, match="status 5.*immobile"):
-        init_t3c.read_info(imm)
     mf = tmp_path / "mf.t3c"
     mf.write_text(open(write(tmp_path, FREE_SLIP + "~\n~\n", name="x.t3c")).read().replace("\n0\nout.prn b",
                                                                                             "\nold.prn b\nout.prn b"))

@@ -572,3 +572,51 @@ def test_hydration_classification_and_invariants(orc):
     t1 = hq.markt.copy()
     o2 = orc.Oracle(q, hq).hydration(hydrate=False)
     assert np.array_equal(hq.markt, t1) and o2.count_tot == 0 and math.isnan(o2.sol_frac)
+
+
+# ------------------------------------------------------------------ immobile rock types
+def test_immobile_types_stay_put_and_carry_no_temperature(orc):
+    """markim (markmart.c:27, 905): markers of an immobile type are not
+    advected (position, pressure and viscosity untouched), and the node
+    temperature is the weighted mean over the mobile markers only -- with
+    every mobile marker at 300 K and the immobile ones at 900 K, every node
+    that sees a mobile marker gets 300 K, while the density still mixes both."""
+    p, hs, _ = small(n=15, size=56e3, mnum=3)
+    p.immobile[configs.IRON] = 1
+    rng = np.random.default_rng(6)
+    iron = hs.markt == configs.IRON
+    assert iron.sum() > 20
+    hs.markt[~iron] = configs.SILICATE  # no sticky air, whose T is reset to the ambient
+    hs.markk[:] = np.where(iron, 900.0, 300.0).astype(np.float32)
+    # a velocity field and a time step that move every mobile marker
+    hs.vx[:] = 1e-9 * (1 + rng.random(hs.N))
+    hs.vy[:] = -1e-9 * (1 + rng.random(hs.N))
+    hs.timestep = 1e11
+    before = hs.copy()
+    orc.Oracle(p, hs).movemark()
+    assert np.array_equal(hs.markx[iron], before.markx[iron])
+    assert np.array_equal(hs.marky[iron], before.marky[iron])
+    assert np.array_equal(hs.markp[iron], before.markp[iron])
+    inside = (before.markx > 0) & (before.markx < p.xsize) & (before.marky > 0) & (before.marky < p.ysize)
+    assert np.all(hs.markx[~iron & inside] != before.markx[~iron & inside])
+    hs = before.copy()
+    hs.markv[:] = 0.0
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    # nodes whose four cells hold a mobile marker: T from the mobile ones only
+    X, Y = hs.xnumx, hs.ynumy
+    i = np.clip(np.searchsorted(hs.gx, hs.markx.astype(np.float64), side="right") - 1, 0, X - 2)
+    j = np.clip(np.searchsorted(hs.gy, hs.marky.astype(np.float64), side="right") - 1, 0, Y - 2)
+    mob = np.zeros((X - 1, Y - 1), bool)
+    np.logical_or.at(mob, (i[~iron], j[~iron]), True)
+    ok = np.zeros((X, Y), bool)
+    ok[1:-1, 1:-1] = mob[:-1, :-1] & mob[1:, :-1] & mob[:-1, 1:] & mob[1:, 1:]
+    ok = ok.ravel()
+    assert ok.sum() > 50
+    np.testing.assert_allclose(hs.tk[ok], 300.0, rtol=1e-14)
+    # the density at a node that sees iron still carries the iron (7800 vs 3500 and 1)
+    has_iron = np.zeros((X - 1, Y - 1), bool)
+    np.logical_or.at(has_iron, (i[iron], j[iron]), True)
+    hi = np.zeros((X, Y), bool)
+    hi[1:-1, 1:-1] = has_iron[:-1, :-1] | has_iron[1:, :-1]
+    assert np.any(hs.ro[(hi.ravel()) & ok] > 3600.0)

@@ -134,24 +134,23 @@ def test_reader_rejects_other_sizes(tmp_path):
         prn.read(short, params=p)
 
 
-def test_immobile_rock_type_is_refused(tmp_path):
-    """A file whose markim table marks a rock type immobile (markmart.c:27
-    skips those markers, markmart.c:905 leaves them out of the T sums) is
-    refused with I2E_ERR_UNSUPPORTED instead of being run as mobile."""
+def test_immobile_rock_types_round_trip(tmp_path):
+    """markim follows the header as one int per rock type (loadmart.c:443,
+    715): the params' immobile table is written there and read back."""
     p, hs = state_with_fields(seed=5)
+    p.immobile[6] = 1
     path = tmp_path / "t.prn"
     prn.write(path, p, hs, n0=1)
-    b = bytearray(path.read_bytes())
+    b = path.read_bytes()
     # markim follows the header: 164 bytes, 6 doubles, rocknum (4), bondnum
     # (8), n0 (4), timesum (8)
     off = 164 + 48 + 4 + 8 + 4 + 8
     assert struct.unpack_from("<i", b, off - 24)[0] == p.ntypes
-    assert struct.unpack_from("<%di" % p.ntypes, b, off) == (0,) * p.ntypes
-    struct.pack_into("<i", b, off + 4 * 6, 1)
-    bad = tmp_path / "immobile.prn"
-    bad.write_bytes(bytes(b))
-    with pytest.raises(Exception, match="status 5.*immobile"):
-        prn.read(bad, params=p)
+    want = [0] * p.ntypes
+    want[6] = 1
+    assert list(struct.unpack_from("<%di" % p.ntypes, b, off)) == want
+    q, _, _ = prn.read(path, params=p)
+    assert list(q.immobile)[:p.ntypes] == want
 
 
 def test_magnetisation_bytes(tmp_path):
