@@ -812,30 +812,56 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
           k = 0;
           continue;
         }
-        if (s->bondv[4 * b + 1] > 1e30) {
-          set_error("continuity-type velocity BC (movemart.c:863) not supported");
-          return I2E_ERR_UNSUPPORTED;
-        }
         k = 1;
-        int cnt = 0;
+        const int64_t U = 3 * (int64_t)m + comp;
+        if (s->bondv[4 * b + 1] > 1e30) {
+          // continuity-type (movemart.c:863-879): the continuity row of the
+          // cell the first neighbour names, whose pressure carries a BC; the
+          // reference exits on anything else.  The multigrid takes the row as
+          // a fixed value (the Krylov solve applies the exact row).
+          const int64_t u0 = s->bondn[3 * b];
+          int dm1 = 0, dm2 = 0;
+          if (u0 > 0) cont_shift(u0 - 1 - U, Y, dm1, dm2);
+          const int64_t m3 = u0 > 0 ? (u0 - 1 - U) / 3 : -1, pu = u0 - 1 - comp;
+          if (u0 <= 0 || m3 < 0 || m3 > Y + 1 || i + dm1 < 1 || j + dm2 < 1 || i + dm1 > X - 1 || j + dm2 > Y - 1 ||
+              pu < 0 || pu >= 3 * (int64_t)N || !s->bondm[pu]) {
+            set_error("inconsistent continuity-type velocity BC of unknown %lld (movemart.c:863-879)", (long long)U);
+            return I2E_ERR_ARG;
+          }
+          // the multigrid keeps the row's same-component term: the other
+          // velocity of that cell's divergence, weight 1 (V_U = V_other
+          // less the transverse terms, which it drops); none if the unknown
+          // is not part of that divergence
+          const int a = i + dm1, bb = j + dm2;
+          const int self = i * Y + j;
+          const int n0 = comp == 1 ? (a - 1) * Y + bb - 1 : (a - 1) * Y + bb - 1;
+          const int n1 = comp == 1 ? a * Y + bb - 1 : (a - 1) * Y + bb;
+          nb = self == n0 ? n1 : (self == n1 ? n0 : -1);
+          kk = nb >= 0 ? 1.0 : 0.0;
+          continue;
+        }
+        // Any linear row V = c + sum k V_n (movemart.c:433-539): the
+        // operator applies it exactly; the multigrid's pointwise re-solve
+        // keeps the strongest neighbour of the same component next to the
+        // node (one neighbour, the free-slip and mirror rows, is exact)
+        double best = 0;
         for (int q = 0; q < 3; q++) {
           int64_t u = s->bondn[3 * b + q];
           if (!u) continue;
-          cnt++;
-          if ((u - 1) % 3 != comp || u - 1 >= 3 * (int64_t)N) {
-            set_error("velocity BC of unknown %d couples another component", 3 * m + comp);
-            return I2E_ERR_UNSUPPORTED;
+          if (u - 1 < 0 || u - 1 >= 3 * (int64_t)N) {
+            set_error("velocity BC of unknown %lld references unknown %lld outside the grid", (long long)U,
+                      (long long)(u - 1));
+            return I2E_ERR_ARG;
           }
-          nb = (int)((u - 1) / 3);
-          kk = s->bondv[4 * b + q + 1];
-          if (std::abs(nb / Y - i) > 1 || std::abs(nb % Y - j) > 1) {
-            set_error("velocity BC of unknown %d references a non-adjacent node", 3 * m + comp);
-            return I2E_ERR_UNSUPPORTED;
+          if ((u - 1) % 3 != comp) continue;
+          const int cand = (int)((u - 1) / 3);
+          if (std::abs(cand / Y - i) > 1 || std::abs(cand % Y - j) > 1) continue;
+          const double kq = s->bondv[4 * b + q + 1];
+          if (std::fabs(kq) > best) {
+            best = std::fabs(kq);
+            nb = cand;
+            kk = kq;
           }
-        }
-        if (cnt > 1) {
-          set_error("velocity BC with more than one neighbour not supported by the multigrid");
-          return I2E_ERR_UNSUPPORTED;
         }
       }
     }

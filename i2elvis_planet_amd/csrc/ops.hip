@@ -22,17 +22,28 @@ __global__ void k_stokes_apply(SOp<double> o, BcTab bc, const double *__restrict
     }
     return o.mu * s;
   };
+  // a velocity BC row, or (coefficient above 1e30, movemart.c:863-879) the
+  // continuity row of the cell its first neighbour names
+  auto vbc = [&](int u) {
+    const int b = bc.bondm[u];
+    if (bc.bondv[4 * b + 1] > 1e30) {
+      int dm1, dm2;
+      cont_shift(bc.bondn[3 * b] - 1 - u, Y, dm1, dm2);
+      return o.mu * cont(o, v, i + dm1, j + dm2);
+    }
+    return bcrow(u);
+  };
   double r0, r1, r2;
   if (i && j)
     r0 = bc.bondm[3 * m] ? bcrow(3 * m) : o.mu * cont(o, v, i, j);
   else
     r0 = o.mu * x[3 * m];
   if (j < Y - 1)
-    r1 = bc.bondm[3 * m + 1] ? bcrow(3 * m + 1) : xmom(o, v, i, j, true);
+    r1 = bc.bondm[3 * m + 1] ? vbc(3 * m + 1) : xmom(o, v, i, j, true);
   else
     r1 = o.mu * x[3 * m + 1];
   if (i < X - 1)
-    r2 = bc.bondm[3 * m + 2] ? bcrow(3 * m + 2) : ymom(o, v, i, j, true);
+    r2 = bc.bondm[3 * m + 2] ? vbc(3 * m + 2) : ymom(o, v, i, j, true);
   else
     r2 = o.mu * x[3 * m + 2];
   y[3 * m] = r0;
@@ -50,7 +61,7 @@ __global__ void k_stokes_rhs(SOp<double> o, BcTab bc, double *__restrict__ b) {
   if (j < Y - 1) {
     int bb = bc.bondm[3 * m + 1];
     if (bb)
-      r1 = bc.bondv[4 * bb] * o.mu;
+      r1 = bc.bondv[4 * bb + 1] > 1e30 ? 0.0 : bc.bondv[4 * bb] * o.mu;  // continuity-type: 0
     else {
       double xkf = (o.gx[i + 1] - o.gx[i - 1]) / 2.0;
       double GX = -(o.gp[m + Y + 1] - o.gp[m + 1]) / xkf;
@@ -60,7 +71,7 @@ __global__ void k_stokes_rhs(SOp<double> o, BcTab bc, double *__restrict__ b) {
   if (i < X - 1) {
     int bb = bc.bondm[3 * m + 2];
     if (bb)
-      r2 = bc.bondv[4 * bb] * o.mu;
+      r2 = bc.bondv[4 * bb + 1] > 1e30 ? 0.0 : bc.bondv[4 * bb] * o.mu;
     else {
       double ykf = (o.gy[j + 1] - o.gy[j - 1]) / 2.0;
       double GY = -(o.gp[m + Y + 1] - o.gp[m + Y]) / ykf;

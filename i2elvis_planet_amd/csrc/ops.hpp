@@ -5,6 +5,16 @@
 
 namespace i2e {
 
+// continuity-type velocity BC (movemart.c:863-879): d = (first neighbour's
+// unknown) - (this unknown), 0-based; the row is the continuity equation of
+// node (m1 + dm1, m2 + dm2).  C integer division, as the reference's.
+__host__ __device__ inline void cont_shift(int64_t d, int64_t Y, int &dm1, int &dm2) {
+  const int64_t m3 = d / 3;
+  dm1 = m3 >= Y;
+  dm2 = m3 == 1 || m3 > Y;
+}
+
+
 struct GravGeo {
   double R, gxkoef, mu;
 };

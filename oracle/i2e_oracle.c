@@ -16,8 +16,7 @@
  *
  * Scope restrictions (each returns I2E_ERR_UNSUPPORTED rather than guessing):
  * intermod = intermod1 = 0, stoksfd = heatfd = 0 (mode.t3c:6011-6036),
- * densimod in {0,1}, adiabyn = 0, no immobile marker types, no
- * continuity-type velocity BCs (movemart.c:863-879).
+ * densimod in {0,1}, adiabyn = 0.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1068,6 +1067,24 @@ static void cont_row(const i2e_params *p, const i2e_host_state *s, int64_t m1, i
   radd(r, ((m1 - 1) * Y + m2) * 3 + 2, d1);
 }
 
+/* continuity-type velocity BC of unknown U (movemart.c:863-879): the
+ * continuity row (conterr) of node (m1 + dm1, m2 + dm2), the shift read from
+ * the first neighbour's unknown; the reference exits on an inconsistent
+ * table, this returns I2E_ERR_ARG                                           */
+static int cont_bc_row(const i2e_params *p, const i2e_host_state *s, int64_t m1, int64_t m2, int64_t U, int comp,
+                       Row *r) {
+  int64_t X = p->xnumx, Y = p->ynumy, b = s->bondm[U], u0 = s->bondn[3 * b], m3, dm1 = 0, dm2 = 0;
+  if (u0 <= 0) return I2E_ERR_ARG;
+  m3 = (u0 - 1 - U) / 3;
+  if (m3 >= Y) dm1 = 1;
+  if (m3 == 1 || m3 > Y) dm2 = 1;
+  if (m3 < 0 || m3 > Y + 1 || !(m1 + dm1) || !(m2 + dm2) || m1 + dm1 > X - 1 || m2 + dm2 > Y - 1 ||
+      u0 - 1 - comp < 0 || u0 - 1 - comp >= 3 * X * Y || !s->bondm[u0 - 1 - comp])
+    return I2E_ERR_ARG;
+  cont_row(p, s, m1 + dm1, m2 + dm2, r);
+  return I2E_OK;
+}
+
 int orc_stokes_build(const i2e_params *p, const i2e_host_state *s, orc_csr *A) {
   int64_t X = p->xnumx, Y = p->ynumy, m1, m2, mcmax;
   double mukoef = p->nubeg * p->strmin;
@@ -1096,8 +1113,11 @@ int orc_stokes_build(const i2e_params *p, const i2e_host_state *s, orc_csr *A) {
         if (!s->bondm[mcmax + 1])
           xstokes_row(p, s, m1, m2, &r);
         else {
-          if (s->bondv[s->bondm[mcmax + 1] * 4 + 1] > 1e+30) return I2E_ERR_UNSUPPORTED;
-          bcrow(s, mcmax + 1, mcmax + 1, 0, &r);
+          if (s->bondv[s->bondm[mcmax + 1] * 4 + 1] > 1e+30) {
+            if ((st = cont_bc_row(p, s, m1, m2, mcmax + 1, 1, &r))) return st;
+          } else {
+            bcrow(s, mcmax + 1, mcmax + 1, 0, &r);
+          }
           rscale(&r, mukoef);
         }
       } else {
@@ -1111,8 +1131,11 @@ int orc_stokes_build(const i2e_params *p, const i2e_host_state *s, orc_csr *A) {
         if (!s->bondm[mcmax + 2])
           ystokes_row(p, s, m1, m2, &r);
         else {
-          if (s->bondv[s->bondm[mcmax + 2] * 4 + 1] > 1e+30) return I2E_ERR_UNSUPPORTED;
-          bcrow(s, mcmax + 2, mcmax + 2, 0, &r);
+          if (s->bondv[s->bondm[mcmax + 2] * 4 + 1] > 1e+30) {
+            if ((st = cont_bc_row(p, s, m1, m2, mcmax + 2, 2, &r))) return st;
+          } else {
+            bcrow(s, mcmax + 2, mcmax + 2, 0, &r);
+          }
           rscale(&r, mukoef);
         }
       } else {

@@ -763,6 +763,40 @@ def test_viterate_high_contrast_fp64(orc, melt_eta, dt):
         assert e < 1e-6, f
 
 
+@pytest.mark.parametrize("kind", ["general", "continuity", "both"])
+def test_viterate_general_bc_rows(orc, kind):
+    """Velocity BC rows beyond free slip at 51^2: several neighbours, a
+    neighbour of the other component or two nodes in (the operator applies
+    them exactly, the multigrid keeps the strongest adjacent neighbour), and
+    continuity-type rows of an open right wall (movemart.c:863-879), against
+    the oracle's direct solve: vx, vy, P to 1e-8."""
+    import sys as _s
+    from harness.bc import continuity_rows, general_rows
+    say("case viterate bc", kind)
+    p, hs, _ = configs.make("c1")
+    if kind in ("general", "both"):
+        general_rows(hs)
+    if kind in ("continuity", "both"):
+        continuity_rows(hs)
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    ctx = Context(p, hs.copy(), solver=fp64_opts(1e-12))
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.viterate()
+    st = ctx.stats()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o.viterate()
+    say("stokes its", st.stokes_iters, "relres", st.stokes_relres)
+    assert st.stokes_relres <= 1e-12
+    for f in ("vx", "vy", "pr"):
+        e = rel(getattr(got, f), getattr(hs, f))
+        say(f, e)
+        assert e < 1e-8, f
+
+
 def test_step_501_properties():
     """One full step of the 501^2 / 6.25M-marker setup: the Stokes solution
     satisfies the assembled rows to the metric's tolerance when checked

@@ -620,3 +620,34 @@ def test_immobile_types_stay_put_and_carry_no_temperature(orc):
     hi = np.zeros((X, Y), bool)
     hi[1:-1, 1:-1] = has_iron[:-1, :-1] | has_iron[1:, :-1]
     assert np.any(hs.ro[(hi.ravel()) & ok] > 3600.0)
+
+
+# ------------------------------------------------------------------ general velocity BC rows
+def test_general_and_continuity_bc_rows(orc):
+    """The assembled Stokes rows of general BC rows are mukoef (V - sum k
+    V_n) with the table's coefficients, and a continuity-type row
+    (coefficient above 1e30) is mukoef times the two-point divergence of the
+    cell its first neighbour names: -1/hx, +1/hx on that cell's Vx, -1/hy,
+    +1/hy on its Vy, right-hand side 0."""
+    from harness.bc import continuity_rows, general_rows
+    p, hs, _ = small(n=13, size=48e3, mnum=2)
+    gen = general_rows(hs)
+    con = continuity_rows(hs)
+    A = orc.Oracle(p, hs).stokes_matrix()
+    S = A.scipy().tocsr()
+    mu = p.nubeg * p.strmin
+    Y = hs.ynumy
+    for u, nb in gen:
+        want = {u: mu}
+        for v, k in nb:
+            want[v] = want.get(v, 0.0) - mu * k
+        row = S.getrow(u)
+        assert dict(zip(row.indices, row.data)) == pytest.approx(want, rel=1e-15)
+        assert A.b[u] == 0.0
+    for u, a, b in con:
+        hx, hy = hs.gx[a] - hs.gx[a - 1], hs.gy[b] - hs.gy[b - 1]
+        want = {3 * ((a - 1) * Y + b - 1) + 1: -mu / hx, 3 * (a * Y + b - 1) + 1: mu / hx,
+                3 * ((a - 1) * Y + b - 1) + 2: -mu / hy, 3 * ((a - 1) * Y + b) + 2: mu / hy}
+        row = S.getrow(u)
+        assert dict(zip(row.indices, row.data)) == pytest.approx(want, rel=1e-14)
+        assert A.b[u] == 0.0
