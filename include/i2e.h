@@ -467,10 +467,10 @@ int i2e_maxvelstep_noalloc(const double *vx_dev, const double *vy_dev, int64_t x
  * the initial temperature field (box, exponential, circular, cooling-age,
  * transition and impact distributions), porosity and the marker
  * temperatures interpolated from it.  Not handled (I2E_ERR_UNSUPPORTED):
- * markers loaded from a file and immobile rock types.  i2e_init_read gives
- * the sizes; i2e_init_generate fills a state of exactly those sizes and
- * the params the file holds (grid, rock table, 26Al/60Fe, ambient T, GYKOEF,
- * por_init).  Node fields are FP64 as in2mart holds them (its binary save
+ * markers loaded from a file and rock bodies flagged immobile.
+ * i2e_init_read gives the sizes; i2e_init_generate fills a state of exactly
+ * those sizes and the params the file holds (grid, rock table with its
+ * immobile types, 26Al/60Fe, ambient T, GYKOEF, por_init).  Node fields are FP64 as in2mart holds them (its binary save
  * rounds them to float32: write and read back a .prn for that).          */
 typedef struct i2e_init_info {
   int64_t xnumx, ynumy, mnumx, mnumy, marknum, bondnum;
