@@ -633,6 +633,10 @@ static int adj_supported(const i2e_ctx *c) {
               "are not differentiated");
     return I2E_ERR_UNSUPPORTED;
   }
+  if (c->tbc_passes > 1) {
+    set_error("adjoint: chained temperature BC rows are not differentiated");
+    return I2E_ERR_UNSUPPORTED;
+  }
   for (int t = 0; t < I2E_MAXTYPES; t++)
     if (P.immobile[t]) {
       set_error("adjoint: immobile rock types are not differentiated");

@@ -865,8 +865,14 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
         }
       }
     }
-  // heat equation rows need the 3x3 neighbourhood (heaterr, heatmart.c:16-23);
-  // temperature BC rows must reference free nodes (explicit projection)
+  // heat equation rows need the 3x3 neighbourhood (heaterr, heatmart.c:16-23).
+  // tkrecalc's rows T = c + sum k T_n (heatmart.c:758-826, solved by
+  // gausmat3) are projected explicitly: a row whose neighbours are BC rows
+  // themselves needs as many passes as the longest chain of such rows;
+  // rows that reference each other in a cycle are refused.
+  std::vector<int> depth(N, 0);  // 0 unvisited, -1 on the current path
+  std::vector<std::pair<int, int>> stack;
+  int passes = 1;
   for (int m = 0; m < N; m++) {
     int64_t b = s->bondm[3 * N + m];
     if (!b) {
@@ -881,12 +887,49 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
       int64_t u = s->bondn[3 * b + q];
       if (!u) continue;
       int64_t nm = u - 1 - 3 * (int64_t)N;
-      if (nm < 0 || nm >= N || s->bondm[3 * N + nm]) {
-        set_error("temperature BC of node %d references another BC node", m);
-        return I2E_ERR_UNSUPPORTED;
+      if (nm < 0 || nm >= N) {
+        set_error("temperature BC of node %d references unknown %lld, not a temperature", m, (long long)(u - 1));
+        return I2E_ERR_ARG;
       }
     }
   }
+  // depth of every BC row: 1 + the deepest BC row among its neighbours
+  for (int m0 = 0; m0 < N; m0++) {
+    if (!s->bondm[3 * N + m0] || depth[m0] > 0) continue;
+    stack.assign(1, {m0, 0});
+    depth[m0] = -1;
+    while (!stack.empty()) {
+      auto &top = stack.back();
+      const int m = top.first;
+      const int64_t b = s->bondm[3 * N + m];
+      if (top.second < 3) {
+        const int64_t u = s->bondn[3 * b + top.second++];
+        if (!u) continue;
+        const int nm = (int)(u - 1 - 3 * (int64_t)N);
+        if (!s->bondm[3 * N + nm]) continue;
+        if (depth[nm] == -1) {
+          set_error("temperature BC rows of nodes %d and %d reference each other in a cycle", m, nm);
+          return I2E_ERR_UNSUPPORTED;
+        }
+        if (depth[nm] == 0) {
+          depth[nm] = -1;
+          stack.push_back({nm, 0});
+        }
+        continue;
+      }
+      int d = 1;
+      for (int q = 0; q < 3; q++) {
+        const int64_t u = s->bondn[3 * b + q];
+        if (!u) continue;
+        const int nm = (int)(u - 1 - 3 * (int64_t)N);
+        if (s->bondm[3 * N + nm]) d = std::max(d, depth[nm] + 1);
+      }
+      depth[m] = d;
+      passes = std::max(passes, d);
+      stack.pop_back();
+    }
+  }
+  c->tbc_passes = passes;
   ST_OK(c->v32.setup(c->geo, kx, ky, nbx, nby, kkx, kky));
   ST_OK(c->v64.setup(c->geo, kx, ky, nbx, nby, kkx, kky));
   std::vector<double> hx(s->gx, s->gx + X), hy(s->gy, s->gy + Y), cxs, cys;
@@ -1451,7 +1494,8 @@ int i2e_ronurecalc(i2e_ctx *c) {
                      c->red.p + 16, any_immobile(c->P) ? 1 : 0);
   c->kt.end(K_P2G, c->s, p2g_bytes(c), p2g_repr(c));
   hipLaunchKernelGGL(k_contrast, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->dP, c->F, N, c->red.p + 16);
-  hipLaunchKernelGGL(k_tbc, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->F, c->bc, N);
+  for (int k = 0; k < c->tbc_passes; k++)  // one pass per level of chained rows
+    hipLaunchKernelGGL(k_tbc, dim3(nblk(N, 256)), dim3(256), 0, c->s, c->F, c->bc, N);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(c->pin.p + 16, c->red.p + 16, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
   HIP_OK(hipStreamSynchronize(c->s));

@@ -66,6 +66,7 @@ struct i2e_ctx {
   DBuf<int8_t> mg, mg2;
   DBuf<double> mgt, mgt2;
   int permuted = 0;             // slots are not in the caller's order (morig maps them back)
+  int tbc_passes = 1;            // tkrecalc projection passes (longest chain of temperature BC rows)
   int sort_every = 10;          // i2e_step sorts the markers every this many steps (0: never)
   int since_sort = 0;
 

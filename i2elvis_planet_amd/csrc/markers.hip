@@ -689,8 +689,10 @@ __global__ void k_contrast(const i2e_params *__restrict__ p, NodeF F, int N, con
   F.nd[m3] = b;
 }
 
-// tkrecalc (heatmart.c:758-826) for BC rows whose neighbours are free nodes:
-// T = const + sum k T_nbr; free nodes keep their value.
+// tkrecalc (heatmart.c:758-826): T = const + sum k T_nbr on BC rows, free
+// nodes keep their value.  A row whose neighbours are BC rows is exact once
+// they are: the caller runs one pass per level of such chains (a neighbour
+// read while another lane writes it gives its old or its new value).
 __global__ void k_tbc(NodeF F, BcTab bc, int N) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= N) return;

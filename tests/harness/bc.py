@@ -50,3 +50,26 @@ def continuity_rows(hs):
         _append(hs, u, 0.0, [(3 * node(x, m2 + 1) + 1, 1.1e30)])
         rows.append((u, x, m2 + 1))
     return rows
+
+
+def chained_temperature_rows(hs):
+    """An insulating left wall, T(0, m2) = T(1, m2), whose corner rows point
+    at wall rows in turn, T(0, 0) = T(0, 1) and T(0, y) = T(0, y - 1): chains
+    of two temperature BC rows (tkrecalc solves them with gausmat3,
+    heatmart.c:758-826)."""
+    X, Y = hs.xnumx, hs.ynumy
+    N = X * Y
+    node = lambda a, b: a * Y + b
+    for m2 in range(1, Y - 1):
+        _append(hs, 3 * N + node(0, m2), 0.0, [(3 * N + node(1, m2), 1.0)])
+    _append(hs, 3 * N + node(0, 0), 0.0, [(3 * N + node(0, 1), 1.0)])
+    _append(hs, 3 * N + node(0, Y - 1), 0.0, [(3 * N + node(0, Y - 2), 1.0)])
+
+
+def cyclic_temperature_rows(hs):
+    """Two temperature BC rows that reference each other."""
+    X, Y = hs.xnumx, hs.ynumy
+    N = X * Y
+    a, b = 3 * N + 0 * Y + 0, 3 * N + 1 * Y + 0
+    _append(hs, a, 0.0, [(b, 0.5)])
+    _append(hs, b, 0.0, [(a, 0.5)])

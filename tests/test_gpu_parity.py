@@ -797,6 +797,40 @@ def test_viterate_general_bc_rows(orc, kind):
         assert e < 1e-8, f
 
 
+def test_chained_temperature_bc_rows(orc):
+    """Temperature BC rows that point at other BC rows (an insulating wall
+    with its corners pointing at the wall): ronurecalc's projection matches
+    the oracle's direct solve of tkrecalc's system, and a full FP64 step at
+    51^2 the oracle's step; rows that reference each other in a cycle are
+    refused."""
+    from i2elvis_planet_amd import I2EError
+    from harness.bc import chained_temperature_rows, cyclic_temperature_rows
+    say("case chained temperature BC")
+    p, hs, _ = configs.make("c1")
+    chained_temperature_rows(hs)
+    rng = np.random.default_rng(21)
+    hs.markk[:] = rng.uniform(150, 900, hs.marknum).astype(np.float32)
+    ref = hs.copy()
+    ctx = Context(p, hs, solver=fp64_opts(1e-13))
+    ctx.ronurecalc()
+    got = ctx.download(hs.copy())
+    o = orc.Oracle(p, ref)
+    o.ronurecalc()
+    assert rel(got.tk, ref.tk) < 1e-12
+    N, Y = hs.N, hs.ynumy
+    assert got.tk[0] == got.tk[1] == got.tk[Y + 1]  # corner = wall = inner node
+    ctx.step()
+    o.step()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    for f in ("vx", "vy", "pr", "tk"):
+        assert rel(getattr(got, f), getattr(ref, f)) < 1e-8, f
+    p2, h2, _ = configs.make("c1")
+    cyclic_temperature_rows(h2)
+    with pytest.raises(I2EError, match="status 5.*cycle"):
+        Context(p2, h2, solver=fp64_opts())
+
+
 def test_step_501_properties():
     """One full step of the 501^2 / 6.25M-marker setup: the Stokes solution
     satisfies the assembled rows to the metric's tolerance when checked
