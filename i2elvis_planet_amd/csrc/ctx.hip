@@ -504,8 +504,8 @@ int i2e_create(const i2e_params *p, int64_t max_markers, i2e_ctx **out) {
     return I2E_ERR_CAPACITY;
   }
   if ((int64_t)p->xnumx * p->ynumy > (int64_t)1 << 28) return I2E_ERR_CAPACITY;
-  if (p->densimod > 1 || p->adiabyn > 0 || p->markmod != 2 || p->heatmod != 0) {
-    set_error("densimod>1, adiabyn>0, markmod!=2 or heatmod!=0 (sub-stepped heat solve) not supported");
+  if (p->densimod > 1 || p->adiabyn > 0 || p->heatmod != 0) {
+    set_error("densimod>1, adiabyn>0 or heatmod!=0 (sub-stepped heat solve) not supported");
     return I2E_ERR_UNSUPPORTED;
   }
   int ndev = 0;
@@ -666,8 +666,8 @@ int i2e_set_params(i2e_ctx *c, const i2e_params *p) {
     set_error("set_params: grid or markers per cell differ from the context's");
     return I2E_ERR_ARG;
   }
-  if (p->densimod > 1 || p->adiabyn > 0 || p->markmod != 2 || p->heatmod != 0) {
-    set_error("densimod>1, adiabyn>0, markmod!=2 or heatmod!=0 (sub-stepped heat solve) not supported");
+  if (p->densimod > 1 || p->adiabyn > 0 || p->heatmod != 0) {
+    set_error("densimod>1, adiabyn>0 or heatmod!=0 (sub-stepped heat solve) not supported");
     return I2E_ERR_UNSUPPORTED;
   }
   HIP_OK(hipStreamSynchronize(c->s));

@@ -148,11 +148,14 @@ __global__ __launch_bounds__(256) void k_movemark(const i2e_params *__restrict__
     }
     double x0 = (double)fx, y0 = (double)fy, vx1, vy1, vx2, vy2, vx3, vy3, vx4, vy4;
     allinterv_d(g, F.vx, F.vy, x0, y0, vx1, vy1);
-    allinterv_d(g, F.vx, F.vy, x0 + vx1 * timestep / 2.0, y0 + vy1 * timestep / 2.0, vx2, vy2);
-    allinterv_d(g, F.vx, F.vy, x0 + vx2 * timestep / 2.0, y0 + vy2 * timestep / 2.0, vx3, vy3);
-    allinterv_d(g, F.vx, F.vy, x0 + vx3 * timestep, y0 + vy3 * timestep, vx4, vy4);
-    double vx0 = (vx1 + 2.0 * vx2 + 2.0 * vx3 + vx4) / 6.0;
-    double vy0 = (vy1 + 2.0 * vy2 + 2.0 * vy3 + vy4) / 6.0;
+    double vx0 = vx1, vy0 = vy1;  // markmod 1: Euler (markmart.c:74-79)
+    if (p->markmod != 1) {        // otherwise RK4 (markmart.c:80-97)
+      allinterv_d(g, F.vx, F.vy, x0 + vx1 * timestep / 2.0, y0 + vy1 * timestep / 2.0, vx2, vy2);
+      allinterv_d(g, F.vx, F.vy, x0 + vx2 * timestep / 2.0, y0 + vy2 * timestep / 2.0, vx3, vy3);
+      allinterv_d(g, F.vx, F.vy, x0 + vx3 * timestep, y0 + vy3 * timestep, vx4, vy4);
+      vx0 = (vx1 + 2.0 * vx2 + 2.0 * vx3 + vx4) / 6.0;
+      vy0 = (vy1 + 2.0 * vy2 + 2.0 * vy3 + vy4) / 6.0;
+    }
     if (p->outgrid == 2) {
       if (fx < 0 || (double)fx > g.xsize) vy0 = 0;
       if (fy < 0 || (double)fy > g.ysize) vx0 = 0;

@@ -78,7 +78,7 @@ typedef struct i2e_params {
   /* HEAT_TRANSFER_PARAMETERS                                                */
   double heatdif, frictyn, adiabyn;
   int32_t heatmod, densimod;
-  int32_t outgrid, markmod;   /* markmod 2 = RK4 (movemark, markmart.c:77)  */
+  int32_t outgrid, markmod;   /* markmod 1 = Euler, other = RK4 (markmart.c:74) */
   /* planet                                                                  */
   double gykoef;              /* GYKOEF: planet radius for gravity, m       */
   double tmp_ambient;         /* sticky-air temperature, K                  */

@@ -344,12 +344,16 @@ static void move_one(const i2e_params *p, const i2e_host_state *s, const G *g, i
   }
   x0 = (double)(s->markx[mm1]);
   y0 = (double)(s->marky[mm1]);
-  allinterv(g, s->vx, s->vy, x0, y0, &vx1, &vy1);
-  allinterv(g, s->vx, s->vy, x0 + vx1 * timestep / 2.0, y0 + vy1 * timestep / 2.0, &vx2, &vy2);
-  allinterv(g, s->vx, s->vy, x0 + vx2 * timestep / 2.0, y0 + vy2 * timestep / 2.0, &vx3, &vy3);
-  allinterv(g, s->vx, s->vy, x0 + vx3 * timestep, y0 + vy3 * timestep, &vx4, &vy4);
-  vx0 = (vx1 + 2.0 * vx2 + 2.0 * vx3 + vx4) / 6.0;
-  vy0 = (vy1 + 2.0 * vy2 + 2.0 * vy3 + vy4) / 6.0;
+  if (p->markmod == 1) { /* Euler, markmart.c:74-79 */
+    allinterv(g, s->vx, s->vy, x0, y0, &vx0, &vy0);
+  } else { /* RK4, markmart.c:80-97 */
+    allinterv(g, s->vx, s->vy, x0, y0, &vx1, &vy1);
+    allinterv(g, s->vx, s->vy, x0 + vx1 * timestep / 2.0, y0 + vy1 * timestep / 2.0, &vx2, &vy2);
+    allinterv(g, s->vx, s->vy, x0 + vx2 * timestep / 2.0, y0 + vy2 * timestep / 2.0, &vx3, &vy3);
+    allinterv(g, s->vx, s->vy, x0 + vx3 * timestep, y0 + vy3 * timestep, &vx4, &vy4);
+    vx0 = (vx1 + 2.0 * vx2 + 2.0 * vx3 + vx4) / 6.0;
+    vy0 = (vy1 + 2.0 * vy2 + 2.0 * vy3 + vy4) / 6.0;
+  }
   if (p->outgrid == 2) {
     if (s->markx[mm1] < 0 || (double)(s->markx[mm1]) > g->xsize) vy0 = 0;
     if (s->marky[mm1] < 0 || (double)(s->marky[mm1]) > g->ysize) vx0 = 0;
@@ -364,7 +368,6 @@ static void move_one(const i2e_params *p, const i2e_host_state *s, const G *g, i
 int orc_movemark(const i2e_params *p, i2e_host_state *s) {
   G g = mkG(p, s);
   int64_t mm1;
-  if (p->markmod != 2) return I2E_ERR_UNSUPPORTED;
   for (mm1 = 0; mm1 < s->marknum; mm1++) move_one(p, s, &g, mm1, s->timestep);
   return I2E_OK;
 }

@@ -58,10 +58,11 @@ def rotation_field(hs, om=2e-13, noise=0.0, seed=0):
 
 
 # ------------------------------------------------------------------ markers
-@pytest.mark.parametrize("name", ["c1", "c3"])
-def test_movemark_bitexact(orc, name):
-    say("case movemark", name)
+@pytest.mark.parametrize("name,markmod", [("c1", 2), ("c3", 2), ("c1", 1)])
+def test_movemark_bitexact(orc, name, markmod):
+    say("case movemark", name, "markmod", markmod)
     p, hs, _ = configs.make(name)
+    p.markmod = markmod  # 1: Euler, 2: RK4 (markmart.c:74-97)
     rotation_field(hs, noise=0.3)
     rng = np.random.default_rng(3)
     hs.pr[:] = 1e6 * rng.random(hs.N)

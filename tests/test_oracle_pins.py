@@ -651,3 +651,40 @@ def test_general_and_continuity_bc_rows(orc):
         row = S.getrow(u)
         assert dict(zip(row.indices, row.data)) == pytest.approx(want, rel=1e-14)
         assert A.b[u] == 0.0
+
+
+def test_euler_advection_is_one_velocity_evaluation(orc):
+    """markmod = 1 (markmart.c:74-79): x += dt vx(x), one interpolation; a
+    uniform flow gives the same float32 update as RK4, a rotation differs by
+    the scheme's O(dt^2) error."""
+    p, hs, _ = small(n=13, size=48e3, mnum=2)
+    X, Y = hs.xnumx, hs.ynumy
+    hs.vx[:] = 2e-9
+    hs.vy[:] = -1e-9
+    hs.timestep = 3e11
+    a, b = hs.copy(), hs.copy()
+    p.markmod = 1
+    orc.Oracle(p, a).movemark()
+    p.markmod = 2
+    orc.Oracle(p, b).movemark()
+    inside = (hs.markx > 0) & (hs.markx < p.xsize) & (hs.marky > 0) & (hs.marky < p.ysize)
+    want_x = (hs.markx + np.float32(3e11 * 2e-9)).astype(np.float32)
+    assert np.array_equal(a.markx[inside], want_x[inside]) and np.array_equal(a.markx, b.markx)
+    # solid-body rotation: Euler leaves the circle by ~ (om dt)^2 r / 2
+    c = p.xsize / 2
+    gx, gy = hs.gx, hs.gy
+    om = 1e-12
+    xm, ym = (gx[:-1] + gx[1:]) / 2, (gy[:-1] + gy[1:]) / 2
+    vx = np.zeros((X, Y))
+    vy = np.zeros((X, Y))
+    vx[:, :Y - 1] = -om * (ym[None, :] - c)
+    vy[:X - 1, :] = om * (xm[:, None] - c)
+    hs.vx[:], hs.vy[:] = vx.ravel(), vy.ravel()
+    hs.timestep = 1e10
+    e = hs.copy()
+    p.markmod = 1
+    orc.Oracle(p, e).movemark()
+    r0 = np.hypot(hs.markx.astype(np.float64) - c, hs.marky.astype(np.float64) - c)
+    r1 = np.hypot(e.markx.astype(np.float64) - c, e.marky.astype(np.float64) - c)
+    sel = inside & (r0 > 4e3) & (r0 < 16e3)
+    np.testing.assert_allclose(r1[sel] - r0[sel], 0.5 * (om * 1e10) ** 2 * r0[sel], rtol=0.05, atol=2e-3)
