@@ -27,7 +27,7 @@ _EXPORTS = [
     "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
-    "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev",
+    "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev", "i2e_sink_rate", "i2e_adjoint_sink_rate", "i2e_stokes_apply_t_dev",
     "i2e_type_counts", "i2e_maxvelstep_noalloc", "i2e_kernel_timer_repr", "i2e_set_marker_sort",
     "i2e_sort_markers", "i2e_init_read", "i2e_init_generate", "i2e_mode_read", "i2e_tuned_shapes", "i2e_hydration",
 ]
@@ -103,6 +103,9 @@ def load_library(path=LIB_PATH):
         "i2e_replay": ([vp, C.c_int], C.c_int),
         "i2e_peak_t": ([vp, dp], C.c_int),
         "i2e_adjoint_peak_t": ([vp, C.POINTER(abi.AdjointOut)], C.c_int),
+        "i2e_sink_rate": ([vp, C.POINTER(C.c_double)], C.c_int),
+        "i2e_adjoint_sink_rate": ([vp, C.POINTER(abi.FlowAdjointOut)], C.c_int),
+        "i2e_stokes_apply_t_dev": ([vp, vp, vp], C.c_int),
         "i2e_heat_apply_t_dev": ([vp, vp, vp], C.c_int),
         "i2e_type_counts": ([vp, C.c_int, C.POINTER(C.c_int32)], C.c_int),
         "i2e_maxvelstep_noalloc": ([vp, vp, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double, vp, vp],
@@ -255,7 +258,8 @@ class Context:
 
     def apply(self, which, x_dev, y_dev):
         f = {"stokes": _lib.i2e_stokes_apply_dev, "gravity": _lib.i2e_gravity_apply_dev,
-             "heat": _lib.i2e_heat_apply_dev, "heat_t": _lib.i2e_heat_apply_t_dev}[which]
+             "heat": _lib.i2e_heat_apply_dev, "heat_t": _lib.i2e_heat_apply_t_dev,
+             "stokes_t": _lib.i2e_stokes_apply_t_dev}[which]
         _chk(f(self._h, C.c_void_p(x_dev), C.c_void_p(y_dev)), which + "_apply")
 
     def stokes_rhs(self, b_dev):
@@ -309,6 +313,20 @@ class Context:
         o = abi.AdjointOut()
         _chk(_lib.i2e_adjoint_peak_t(self._h, C.byref(o)), "adjoint_peak_t")
         return {k: getattr(o, k) for k, _ in abi.AdjointOut._fields_}
+
+    # flow adjoint: one Stokes solve and the advection that reads it ---------------
+    def sink_rate(self):
+        """Mass-weighted mean outward RK4 velocity of the iron markers (m/s)."""
+        J = C.c_double()
+        _chk(_lib.i2e_sink_rate(self._h, C.byref(J)), "sink_rate")
+        return J.value
+
+    def adjoint_sink_rate(self):
+        """sink_rate and its derivative in a common factor on every activation
+        energy, through the last Stokes solve (call right after viterate)."""
+        o = abi.FlowAdjointOut()
+        _chk(_lib.i2e_adjoint_sink_rate(self._h, C.byref(o)), "adjoint_sink_rate")
+        return {k: getattr(o, k) for k, _ in abi.FlowAdjointOut._fields_}
 
     def set_marker_sort(self, every):
         """Sort the markers by cell every `every` steps of step() (0: never)."""

@@ -120,6 +120,12 @@ class AdjointOut(C.Structure):
                 ("heat_relres_max", C.c_double), ("steps", C.c_int32), ("heat_iters", C.c_int32)]
 
 
+class FlowAdjointOut(C.Structure):
+    """i2e_flow_adjoint_out: iron sinking rate and its activation-energy derivative."""
+    _fields_ = [("J", C.c_double), ("dJ_dEscale", C.c_double), ("stokes_relres", C.c_double),
+                ("stokes_iters", C.c_int64)]
+
+
 class InitInfo(C.Structure):
     """i2e_init_info: sizes of the state an init.t3c file generates."""
     _fields_ = [("xnumx", C.c_int64), ("ynumy", C.c_int64), ("mnumx", C.c_int64), ("mnumy", C.c_int64),

@@ -145,7 +145,6 @@ __device__ __forceinline__ bool objective_marker(const i2e_params *p, const Grid
   return true;
 }
 
-constexpr int RB = 256, RG = 1024;  // deterministic reductions: RG blocks of RB threads, fixed order
 
 __device__ __forceinline__ void block_store2(double a, double b, double *out) {
   __shared__ double sa[RB / 64], sb[RB / 64];

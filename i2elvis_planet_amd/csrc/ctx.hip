@@ -590,8 +590,10 @@ int i2e_destroy(i2e_ctx *c) {
   }
   DBuf<double> *d[] = {&c->gx, &c->gy, &c->nodes, &c->acc, &c->work, &c->sol1, &c->tk2, &c->red, &c->xs,
                        &c->bs, &c->bondv, &c->hk, &c->hc, &c->hdm, &c->gk, &c->gc, &c->gdm, &c->gsc, &c->hsc,
-                       &c->bcix, &c->bciy, &c->bLS, &c->xprev};
+                       &c->bcix, &c->bciy, &c->bLS, &c->xprev, &c->fadj};
   c->blmask.release();
+  c->fflag.release();
+  c->fmask.release();
   c->tcnt.release();
   c->hacc.release();
   c->hdacc.release();
@@ -662,6 +664,7 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
 
 int i2e_set_params(i2e_ctx *c, const i2e_params *p) {
   if (!c || !p) return I2E_ERR_ARG;
+  c->stokes_valid = 0;
   if (p->xnumx != c->P.xnumx || p->ynumy != c->P.ynumy || p->mnumx != c->P.mnumx || p->mnumy != c->P.mnumy) {
     set_error("set_params: grid or markers per cell differ from the context's");
     return I2E_ERR_ARG;
@@ -969,6 +972,7 @@ static int setup_solvers(i2e_ctx *c, const i2e_host_state *s) {
 int i2e_upload(i2e_ctx *c, const i2e_host_state *s) {
   if (!c || !s || !s->gx || !s->gy) return I2E_ERR_ARG;
   c->have_prev = 0;  // new fields: no solution history to extrapolate from
+  c->stokes_valid = 0;
   if (s->marknum < 0 || s->marknum > c->Mmax) {
     set_error("marknum %lld exceeds max_markers %lld", (long long)s->marknum, (long long)c->Mmax);
     return I2E_ERR_CAPACITY;
@@ -1230,7 +1234,7 @@ static int bfbt_precond(i2e_ctx *c, VelMG<T> &v, CellMG<T> &lm, const double *r,
 
 // presplit: the velocity components of r are already in the finest velocity
 // level's right-hand side (left there by the Krylov projection pass)
-static int stokes_precond(i2e_ctx *c, const double *r, double *z, bool presplit = false) {
+extern "C++" int stokes_precond(i2e_ctx *c, const double *r, double *z, bool presplit) {
   const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
   if (c->o.schur != I2E_SCHUR_MASS) return mixed(c) ? bfbt_precond(c, c->v32, c->l32, r, z, presplit)
@@ -1257,6 +1261,7 @@ static int stokes_precond(i2e_ctx *c, const double *r, double *z, bool presplit 
 static int stokes_solve(i2e_ctx *c) {
   const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
+  c->stokes_valid = 0;
   ST_OK(stokes_prepare(c));
   // algorithmic bytes: read vx, vy, p, the two viscosities, rho and gp,
   // write the three residual components, 8 B each per node; this
@@ -1294,6 +1299,8 @@ static int stokes_solve(i2e_ctx *c) {
   }
   hipLaunchKernelGGL(k_stokes_unpack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->xs.p, c->F.pr, c->F.vx, c->F.vy);
   hipLaunchKernelGGL(k_strain, nb2(X, Y), dim3(256), 0, c->s, c->grid(), c->F);
+  c->stokes_valid = 1;
+  c->stokes_dt = o.dt;
   return I2E_OK;
 }
 
@@ -1338,6 +1345,7 @@ int i2e_titerate(i2e_ctx *c, int n0) {
   (void)n0;
   if (!c || !c->setup_done) return I2E_ERR_ARG;
   c->heat_solved = 0;
+  c->stokes_valid = 0;  // marker temperatures change
   auto t0 = std::chrono::steady_clock::now();
   const int X = c->X, Y = c->Y, N = c->N;
   Grid g = c->grid();
@@ -1471,6 +1479,7 @@ static void launch_ronu_markers(i2e_ctx *c) {
 
 int i2e_movemark(i2e_ctx *c) {
   if (!c) return I2E_ERR_ARG;
+  c->stokes_valid = 0;
   if (!c->M) return I2E_OK;
   launch_movemark(c, true);
   HIP_OK(hipGetLastError());
@@ -1479,6 +1488,7 @@ int i2e_movemark(i2e_ctx *c) {
 
 int i2e_ronurecalc(i2e_ctx *c) {
   if (!c) return I2E_ERR_ARG;
+  c->stokes_valid = 0;
   const int N = c->N;
   HIP_OK(hipMemsetAsync(c->acc.p, 0, sizeof(double) * 12 * N, c->s));
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->s, 1, 1e300, c->red.p + 16);

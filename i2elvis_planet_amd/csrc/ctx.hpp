@@ -9,6 +9,11 @@ void adj_release(AdjHist *h);
 __global__ void k_copy(int n, const double *a, double *b);
 __global__ void k_heat_fields(int N, const double *ro, const double *cp, double dt, const int *bondm, double *c,
                               int8_t *mask, double *dm);
+// deterministic two-sum reduction of adjoint.hip: k_sum_part over RG blocks
+// of RB threads, then k_sum_final (fixed order)
+constexpr int RB = 256, RG = 1024;
+__global__ void k_sum_part(int64_t n, const double *a, const double *b, double *part);
+__global__ void k_sum_final(const double *part, double *out);
 }  // namespace i2e
 
 using namespace i2e;
@@ -69,7 +74,19 @@ struct i2e_ctx {
   int tbc_passes = 1;            // tkrecalc projection passes (longest chain of temperature BC rows)
   int sort_every = 10;          // i2e_step sorts the markers every this many steps (0: never)
   int since_sort = 0;
+  // the node velocities, pressure and viscosities are the last Stokes
+  // solve's, with the markers that set them unchanged since (the flow
+  // adjoint differentiates that solve); stokes_dt is its stabilisation dt
+  int stokes_valid = 0;
+  double stokes_dt = 0;
+  DBuf<double> fadj;            // flow adjoint scratch (flow_adjoint.hip)
+  DBuf<int> fflag;
+  DBuf<int8_t> fmask;
 
   Grid grid() const { return Grid{X, Y, P.xsize, P.ysize, gx.p, gy.p}; }
   SOp<double> sop() const { return SOp<double>{X, Y, gx.p, gy.p, F.nu, F.nd, F.ro, F.gp, timestep, P.nubeg * P.strmin}; }
 };
+
+// the Stokes preconditioner of the current solve (ctx.hip); presplit: the
+// velocity part of r is already in the finest level's right-hand side
+int stokes_precond(i2e_ctx *c, const double *r, double *z, bool presplit = false);

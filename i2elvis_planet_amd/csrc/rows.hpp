@@ -44,10 +44,14 @@ struct VUnit {  // unit vector e_(comp, idx): extracts one matrix column
   __device__ __forceinline__ T pp(int i, int j) const { return (comp == 0 && i * Y + j == idx) ? T(1) : T(0); }
 };
 
+// The row functions take the operator O by type: SOp<T> for the solvers, or
+// a caller's struct with the same members whose field members return a
+// dual number (the adjoint's field derivatives).
+
 // sigma_xy at basic node (i,j) (sxycalc, movemart.c:1515-1649, stencil
 // clamped at the edges exactly as the reference clamps it)
-template <class T, class A>
-__device__ __forceinline__ T sxy_at(const SOp<T> &o, const A &v, int i, int j) {
+template <class T, class O, class A>
+__device__ __forceinline__ T sxy_at_g(const O &o, const A &v, int i, int j) {
   T a = 0, b = 0;
   int jmin = j - 1 < 0 ? 0 : j - 1, jmax = j > o.Y - 2 ? o.Y - 2 : j;
   if (jmax > jmin) {
@@ -63,15 +67,15 @@ __device__ __forceinline__ T sxy_at(const SOp<T> &o, const A &v, int i, int j) {
 }
 
 // x-Stokes row at Vx(i,j), i in [1,X-2], j in [0,Y-2] (xstokserr)
-template <class T, class A>
-__device__ __forceinline__ T xmom(const SOp<T> &o, const A &v, int i, int j, bool withP) {
+template <class T, class O, class A>
+__device__ __forceinline__ T xmom_g(const O &o, const A &v, int i, int j, bool withP) {
   const int Y = o.Y;
   const double *gx = o.gx, *gy = o.gy;
   const T xkf = T((gx[i + 1] - gx[i - 1]) / 2.0), ykf = T(gy[j + 1] - gy[j]);
   const int v0 = i * Y + j, v1 = v0 + 1, v2 = v0 + Y, v3 = v2 + 1;
   T sL = T(2) * o.nd[v1] * (v.vx(i, j) - v.vx(i - 1, j)) / T(gx[i] - gx[i - 1]);
   T sR = T(2) * o.nd[v3] * (v.vx(i + 1, j) - v.vx(i, j)) / T(gx[i + 1] - gx[i]);
-  T sT = sxy_at(o, v, i, j), sB = sxy_at(o, v, i, j + 1);
+  T sT = sxy_at_g<T>(o, v, i, j), sB = sxy_at_g<T>(o, v, i, j + 1);
   T r = (sR - sL) / xkf + (sB - sT) / ykf;
   if (withP) r -= (v.pp(i + 1, j + 1) - v.pp(i, j + 1)) / xkf;
   // -gx dt/2 (vx drho/dx + vy drho/dy), movemart.c:116-125
@@ -87,15 +91,15 @@ __device__ __forceinline__ T xmom(const SOp<T> &o, const A &v, int i, int j, boo
 }
 
 // y-Stokes row at Vy(i,j), i in [0,X-2], j in [1,Y-2] (ystokserr)
-template <class T, class A>
-__device__ __forceinline__ T ymom(const SOp<T> &o, const A &v, int i, int j, bool withP) {
+template <class T, class O, class A>
+__device__ __forceinline__ T ymom_g(const O &o, const A &v, int i, int j, bool withP) {
   const int Y = o.Y;
   const double *gx = o.gx, *gy = o.gy;
   const T xkf = T(gx[i + 1] - gx[i]), ykf = T((gy[j + 1] - gy[j - 1]) / 2.0);
   const int v0 = i * Y + j, v1 = v0 + 1, v2 = v0 + Y, v3 = v2 + 1;
   T sT = T(2) * o.nd[v2] * (v.vy(i, j) - v.vy(i, j - 1)) / T(gy[j] - gy[j - 1]);
   T sB = T(2) * o.nd[v3] * (v.vy(i, j + 1) - v.vy(i, j)) / T(gy[j + 1] - gy[j]);
-  T sLx = sxy_at(o, v, i, j), sRx = sxy_at(o, v, i + 1, j);
+  T sLx = sxy_at_g<T>(o, v, i, j), sRx = sxy_at_g<T>(o, v, i + 1, j);
   T r = (sB - sT) / ykf + (sRx - sLx) / xkf;
   if (withP) r -= (v.pp(i + 1, j + 1) - v.pp(i + 1, j)) / ykf;
   T GY = -(o.gp[v3] - o.gp[v2]) / ykf;
@@ -110,10 +114,27 @@ __device__ __forceinline__ T ymom(const SOp<T> &o, const A &v, int i, int j, boo
 }
 
 // continuity at cell (i,j), i,j >= 1, unscaled (conterr)
-template <class T, class A>
-__device__ __forceinline__ T cont(const SOp<T> &o, const A &v, int i, int j) {
+template <class T, class O, class A>
+__device__ __forceinline__ T cont_g(const O &o, const A &v, int i, int j) {
   return (v.vx(i, j - 1) - v.vx(i - 1, j - 1)) / T(o.gx[i] - o.gx[i - 1]) +
          (v.vy(i - 1, j) - v.vy(i - 1, j - 1)) / T(o.gy[j] - o.gy[j - 1]);
+}
+
+template <class T, class A>
+__device__ __forceinline__ T sxy_at(const SOp<T> &o, const A &v, int i, int j) {
+  return sxy_at_g<T>(o, v, i, j);
+}
+template <class T, class A>
+__device__ __forceinline__ T xmom(const SOp<T> &o, const A &v, int i, int j, bool withP) {
+  return xmom_g<T>(o, v, i, j, withP);
+}
+template <class T, class A>
+__device__ __forceinline__ T ymom(const SOp<T> &o, const A &v, int i, int j, bool withP) {
+  return ymom_g<T>(o, v, i, j, withP);
+}
+template <class T, class A>
+__device__ __forceinline__ T cont(const SOp<T> &o, const A &v, int i, int j) {
+  return cont_g<T>(o, v, i, j);
 }
 
 // ---------------------------------------------------------------------------

@@ -418,6 +418,37 @@ int i2e_adjoint_peak_t(i2e_ctx *c, i2e_adjoint_out *out);
 /* y = A^T x for the heat rows of i2e_heat_apply_dev (device pointers).     */
 int i2e_heat_apply_t_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
 
+/* ---- flow adjoint: one Stokes solve and the advection that reads it ------
+ * No counterpart in the reference: a sensitivity of the iron sinking rate
+ *   J = sum_i m_i (v_i . r_i) / sum_i m_i
+ * over the iron markers inside the grid (markk > 0, is_iron, mobile), m_i =
+ * markro of the type, r_i the unit vector from the box centre, v_i the
+ * velocity movemark gives marker i (RK4, or Euler for markmod 1;
+ * markmart.c:74-97) from the node velocities of the last viterate with the
+ * context's current timestep.  i2e_sink_rate returns J.
+ * i2e_adjoint_sink_rate, called after i2e_viterate (and i2e_maxvelstep) and
+ * before anything that changes the markers or node fields, returns J and
+ * dJ/ds, s a common factor on every rock's activation energy markdh at
+ * s = 1, through: the reversed RK4 interpolation (allinterv, markmart.c:
+ * 1378-1453), the transposed Stokes solve K^T mu = dJ/dx (rows of
+ * movemart.c:4-426) by FGMRES with viterate's preconditioner, the viscosity
+ * scatter and node normalisation of ronurecalc (markmart.c:840-1009) and
+ * viscalc's Arrhenius branch (markmart.c:1855-2063).  Marker positions and
+ * temperatures, density, gravity and the time steps are held.  Refuses melt
+ * switching, activation volumes, negative activation energies, power-law
+ * rheologies (I2E_ERR_UNSUPPORTED) and node viscosities set by the contrast
+ * limit nucontr.                                                            */
+typedef struct i2e_flow_adjoint_out {
+  double J;              /* m/s (positive outward)                           */
+  double dJ_dEscale;     /* dJ / ds                                          */
+  double stokes_relres;  /* residual of the transposed Stokes solve          */
+  int64_t stokes_iters;  /* its FGMRES iterations                           */
+} i2e_flow_adjoint_out;
+int i2e_sink_rate(i2e_ctx *c, double *J);
+int i2e_adjoint_sink_rate(i2e_ctx *c, i2e_flow_adjoint_out *out);
+/* y = K^T x for the Stokes rows of i2e_stokes_apply_dev (device pointers).  */
+int i2e_stokes_apply_t_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
+
 /* ---- binary .prn checkpoint (saver/loader of loadmart.c) -----------------
  * The byte layout of the reference's binary output (saver, loadmart.c:671-809;
  * loader, loadmart.c:391-540; read by support_files/plot2d.py:157-283), for a

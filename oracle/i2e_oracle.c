@@ -372,6 +372,42 @@ int orc_movemark(const i2e_params *p, i2e_host_state *s) {
   return I2E_OK;
 }
 
+/* The flow adjoint's objective (no reference counterpart): mass-weighted
+ * mean outward velocity of the mobile iron markers inside the grid, each
+ * moving with the velocity move_one would give it (RK4, or Euler for
+ * markmod 1; markmart.c:74-97), m = markro of its type.                     */
+int orc_sink_rate(const i2e_params *p, const i2e_host_state *s, double *J) {
+  G g = mkG(p, s);
+  double smv = 0, sm = 0, dt = s->timestep;
+  int64_t mm1;
+  for (mm1 = 0; mm1 < s->marknum; mm1++) {
+    int t = (int)s->markt[mm1], mm2 = t >= 100 ? t - 100 : t;
+    double x0 = (double)s->markx[mm1], y0 = (double)s->marky[mm1], vx0, vy0, dx, dy, r;
+    if (!(s->markx[mm1] > 0 && s->marky[mm1] > 0 && x0 < g.xsize && y0 < g.ysize && s->markk[mm1] > 0 && t < 50))
+      continue;
+    if (mm2 < 0 || mm2 >= I2E_MAXTYPES || !p->is_iron[mm2] || p->immobile[mm2]) continue;
+    dx = x0 - g.xsize / 2.0;
+    dy = y0 - g.ysize / 2.0;
+    r = sqrt(dx * dx + dy * dy);
+    if (!(r > 0)) continue;
+    if (p->markmod == 1) {
+      allinterv(&g, s->vx, s->vy, x0, y0, &vx0, &vy0);
+    } else {
+      double vx1, vy1, vx2, vy2, vx3, vy3, vx4, vy4;
+      allinterv(&g, s->vx, s->vy, x0, y0, &vx1, &vy1);
+      allinterv(&g, s->vx, s->vy, x0 + vx1 * dt / 2.0, y0 + vy1 * dt / 2.0, &vx2, &vy2);
+      allinterv(&g, s->vx, s->vy, x0 + vx2 * dt / 2.0, y0 + vy2 * dt / 2.0, &vx3, &vy3);
+      allinterv(&g, s->vx, s->vy, x0 + vx3 * dt, y0 + vy3 * dt, &vx4, &vy4);
+      vx0 = (vx1 + 2.0 * vx2 + 2.0 * vx3 + vx4) / 6.0;
+      vy0 = (vy1 + 2.0 * vy2 + 2.0 * vy3 + vy4) / 6.0;
+    }
+    smv += p->rock[mm2].ro * (vx0 * dx / r + vy0 * dy / r);
+    sm += p->rock[mm2].ro;
+  }
+  *J = sm > 0 ? smv / sm : 0.0;
+  return I2E_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* ronurecalc, markmart.c:192-1019                                           */
 /* ------------------------------------------------------------------------ */

@@ -54,7 +54,7 @@ def lib():
         Cp = C.POINTER(Csr)
         dp = C.POINTER(C.c_double)
         for name, args in [
-            ("orc_maxvelstep", [P, S, dp]), ("orc_movemark", [P, S]), ("orc_ronurecalc", [P, S, dp]),
+            ("orc_maxvelstep", [P, S, dp]), ("orc_movemark", [P, S]), ("orc_sink_rate", [P, S, dp]), ("orc_ronurecalc", [P, S, dp]),
             ("orc_gravity_build", [P, S, Cp]), ("orc_stokes_build", [P, S, Cp]),
             ("orc_heat_prepare", [P, S, dp, dp]), ("orc_heat_build", [P, S, dp, Cp]),
             ("orc_tbc_build", [P, S, Cp]), ("orc_heat_markers", [P, S, dp]),
@@ -308,6 +308,14 @@ class Oracle:
 
     def movemark(self):
         _check(self.L.orc_movemark(C.byref(self.p), self._s()), "movemark")
+
+    def sink_rate(self):
+        """Mass-weighted mean outward velocity of the iron markers, each moving
+        with the velocity movemark would give it this step (the library's flow
+        adjoint objective; no reference counterpart)."""
+        J = C.c_double()
+        _check(self.L.orc_sink_rate(C.byref(self.p), self._s(), C.byref(J)), "sink_rate")
+        return J.value
 
     def ronurecalc(self):
         mm = np.zeros(2)

@@ -688,3 +688,40 @@ def test_euler_advection_is_one_velocity_evaluation(orc):
     r1 = np.hypot(e.markx.astype(np.float64) - c, e.marky.astype(np.float64) - c)
     sel = inside & (r0 > 4e3) & (r0 < 16e3)
     np.testing.assert_allclose(r1[sel] - r0[sel], 0.5 * (om * 1e10) ** 2 * r0[sel], rtol=0.05, atol=2e-3)
+
+
+def test_sink_rate_of_a_radial_linear_flow(orc):
+    """The flow adjoint's objective on v = c (x - xc, y - yc): bilinear
+    interpolation reproduces the affine field on both staggered sub-grids,
+    so every iron marker moves with the RK4 velocity of dr/dt = c r,
+    c r (1 + c dt/2 + (c dt)^2/6 + (c dt)^3/24), or c r for Euler, and J is
+    that factor times c times the density-weighted mean distance of the iron
+    markers from the centre.  Other types do not count."""
+    p, hs, _ = small(n=15, size=60e3, mnum=3)
+    X, Y = hs.xnumx, hs.ynumy
+    c0 = p.xsize / 2
+    gx, gy = hs.gx, hs.gy
+    xm, ym = (gx[:-1] + gx[1:]) / 2, (gy[:-1] + gy[1:]) / 2
+    cr = -2e-13
+    vx = np.zeros((X, Y))
+    vy = np.zeros((X, Y))
+    vx[:, :Y - 1] = cr * (gx[:, None] - c0)
+    vy[:X - 1, :] = cr * (gy[None, :] - c0)
+    hs.vx[:], hs.vy[:] = vx.ravel(), vy.ravel()
+    hs.timestep = 4e11
+    rng = np.random.default_rng(3)
+    iron = rng.random(hs.marknum) < 0.3
+    hs.markt[:] = np.where(iron, 7, 6).astype(np.int8)
+    hs.markk[:] = 300.0
+    x, y = hs.markx.astype(np.float64), hs.marky.astype(np.float64)
+    inside = (x > 0) & (y > 0) & (x < p.xsize) & (y < p.ysize) & iron
+    rad = np.hypot(x - c0, y - c0)[inside]
+    h = cr * hs.timestep
+    for mod, fac in ((2, 1 + h / 2 + h * h / 6 + h ** 3 / 24), (1, 1.0)):
+        p.markmod = mod
+        J = orc.Oracle(p, hs.copy()).sink_rate()
+        want = cr * fac * rad.mean()  # one density: the weighted mean is the mean
+        assert abs(J - want) <= 1e-12 * abs(want), (mod, J, want)
+    # a single silicate marker does not count; nor does an iron marker at the centre
+    hs.markt[:] = 6
+    assert orc.Oracle(p, hs.copy()).sink_rate() == 0.0
