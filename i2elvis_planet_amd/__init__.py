@@ -27,7 +27,7 @@ _EXPORTS = [
     "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
-    "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_heat_apply_t_dev", "i2e_sink_rate", "i2e_adjoint_sink_rate", "i2e_stokes_apply_t_dev",
+    "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_adjoint_peak_t_coupled", "i2e_adjoint_marker_positions", "i2e_heat_apply_t_dev", "i2e_sink_rate", "i2e_adjoint_sink_rate", "i2e_stokes_apply_t_dev",
     "i2e_type_counts", "i2e_maxvelstep_noalloc", "i2e_kernel_timer_repr", "i2e_set_marker_sort",
     "i2e_sort_markers", "i2e_init_read", "i2e_init_generate", "i2e_mode_read", "i2e_tuned_shapes", "i2e_hydration",
 ]
@@ -103,6 +103,8 @@ def load_library(path=LIB_PATH):
         "i2e_replay": ([vp, C.c_int], C.c_int),
         "i2e_peak_t": ([vp, dp], C.c_int),
         "i2e_adjoint_peak_t": ([vp, C.POINTER(abi.AdjointOut)], C.c_int),
+        "i2e_adjoint_peak_t_coupled": ([vp, C.POINTER(abi.CoupledAdjointOut)], C.c_int),
+        "i2e_adjoint_marker_positions": ([vp, vp, vp], C.c_int),
         "i2e_sink_rate": ([vp, C.POINTER(C.c_double)], C.c_int),
         "i2e_adjoint_sink_rate": ([vp, C.POINTER(abi.FlowAdjointOut)], C.c_int),
         "i2e_stokes_apply_t_dev": ([vp, vp, vp], C.c_int),
@@ -313,6 +315,22 @@ class Context:
         o = abi.AdjointOut()
         _chk(_lib.i2e_adjoint_peak_t(self._h, C.byref(o)), "adjoint_peak_t")
         return {k: getattr(o, k) for k, _ in abi.AdjointOut._fields_}
+
+    def adjoint_peak_t_coupled(self):
+        """J and dJ/d(h0_scale, phi0, activation-energy factor) through the
+        whole recorded step, flow and advection included (call after record;
+        the context's node fields are left at the first recorded step's)."""
+        o = abi.CoupledAdjointOut()
+        _chk(_lib.i2e_adjoint_peak_t_coupled(self._h, C.byref(o)), "adjoint_peak_t_coupled")
+        return {k: getattr(o, k) for k, _ in abi.CoupledAdjointOut._fields_ if k != "pad"}
+
+    def adjoint_marker_positions(self, n):
+        """dJ/dx, dJ/dy of the n initial marker positions (caller's order)
+        from the last adjoint_peak_t_coupled."""
+        gx = np.zeros(n)
+        gy = np.zeros(n)
+        _chk(_lib.i2e_adjoint_marker_positions(self._h, gx.ctypes.data, gy.ctypes.data), "adjoint_marker_positions")
+        return gx, gy
 
     # flow adjoint: one Stokes solve and the advection that reads it ---------------
     def sink_rate(self):

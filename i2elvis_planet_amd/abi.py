@@ -120,6 +120,15 @@ class AdjointOut(C.Structure):
                 ("heat_relres_max", C.c_double), ("steps", C.c_int32), ("heat_iters", C.c_int32)]
 
 
+class CoupledAdjointOut(C.Structure):
+    """i2e_coupled_adjoint_out: peak-temperature objective and its gradient
+    through the whole step (flow included)."""
+    _fields_ = [("J", C.c_double), ("dJ_dh0", C.c_double), ("dJ_dphi0", C.c_double), ("dJ_dEscale", C.c_double),
+                ("heat_relres_max", C.c_double), ("stokes_relres_max", C.c_double), ("grav_relres_max", C.c_double),
+                ("heat_iters", C.c_int64), ("stokes_iters", C.c_int64), ("grav_iters", C.c_int64),
+                ("steps", C.c_int32), ("pad", C.c_int32)]
+
+
 class FlowAdjointOut(C.Structure):
     """i2e_flow_adjoint_out: iron sinking rate and its activation-energy derivative."""
     _fields_ = [("J", C.c_double), ("dJ_dEscale", C.c_double), ("stokes_relres", C.c_double),

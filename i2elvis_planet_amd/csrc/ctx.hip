@@ -1150,8 +1150,46 @@ static int gravity_solve(i2e_ctx *c) {
   return I2E_OK;
 }
 
+// G x = b for the gravity rows of the context's grid (b in the rows' own
+// scaling, overwritten by its row-scaled form), from x = 0: the adjoint's
+// transposed gravity solve on uniform grids, where the interior rows are
+// symmetric
+extern "C++" int gravity_solve_rhs(i2e_ctx *c, double *b, double *x, int *its, double *rr) {
+  const int X = c->X, Y = c->Y, N = c->N;
+  Grid g = c->grid();
+  const double xstpx = c->P.xsize / (double)(X - 1), ystpy = c->P.ysize / (double)(Y - 1);
+  GravGeo q{c->P.gykoef, c->gxkoef, 2.0 / xstpx / xstpx + 2 / ystpy / ystpy};
+  const double *sc = c->gsc.p;
+  hipLaunchKernelGGL(k_scale_vec, dim3(1024), dim3(256), 0, c->s, N, sc, b);
+  HIP_OK(hipMemsetAsync(x, 0, sizeof(double) * N, c->s));
+  GravPre gp{X, Y, q.mu};
+  auto A = [&](const double *u, double *y) -> int {
+    hipLaunchKernelGGL(k_grav_apply, nb2(X, Y), dim3(256), 0, c->s, g, q, u, y);
+    hipLaunchKernelGGL(k_scale_vec, dim3(1024), dim3(256), 0, c->s, N, sc, y);
+    return I2E_OK;
+  };
+  auto Mf = [&](const double *r, double *z) -> int {
+    if (mixed(c)) {
+      hipLaunchKernelGGL(k_grav_pre<float>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, gp, r, sc, c->gmask.p,
+                         c->g32.L[0].f.p);
+      ST_OK(c->g32.vcycle(0, c->s));
+      hipLaunchKernelGGL(k_grav_post<float>, nb2(X, Y), dim3(256), 0, c->s, gp, r, sc, c->g32.L[0].u.p, z);
+    } else {
+      hipLaunchKernelGGL(k_grav_pre<double>, nb2(X - 1, Y - 1), dim3(256), 0, c->s, gp, r, sc, c->gmask.p,
+                         c->g64.L[0].f.p);
+      ST_OK(c->g64.vcycle(0, c->s));
+      hipLaunchKernelGGL(k_grav_post<double>, nb2(X, Y), dim3(256), 0, c->s, gp, r, sc, c->g64.L[0].u.p, z);
+    }
+    return I2E_OK;
+  };
+  c->g32.nsmooth = c->g64.nsmooth = c->o.nsmooth;
+  int st = c->fgv.solve(A, Mf, b, x, c->o.grav_rtol, c->o.maxit, c->s, its, rr);
+  if (st) set_error("adjoint gravity solve: %d iterations, relative residual %.3e > %.1e", *its, *rr, c->o.grav_rtol);
+  return st;
+}
+
 // Stokes: right-hand side, initial guess and multigrid refresh for this step
-static int stokes_prepare(i2e_ctx *c) {
+extern "C++" int stokes_prepare(i2e_ctx *c) {
   const int X = c->X, Y = c->Y;
   SOp<double> o = c->sop();
   hipLaunchKernelGGL(k_stokes_rhs, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, c->bs.p);

@@ -90,3 +90,8 @@ struct i2e_ctx {
 // the Stokes preconditioner of the current solve (ctx.hip); presplit: the
 // velocity part of r is already in the finest level's right-hand side
 int stokes_precond(i2e_ctx *c, const double *r, double *z, bool presplit = false);
+// the Stokes set-up of i2e_viterate for the context's current fields and
+// timestep: right-hand side, initial guess, multigrid hierarchies (ctx.hip)
+int stokes_prepare(i2e_ctx *c);
+// G x = b for the gravity rows, from x = 0 (b overwritten; ctx.hip)
+int gravity_solve_rhs(i2e_ctx *c, double *b, double *x, int *its, double *rr);

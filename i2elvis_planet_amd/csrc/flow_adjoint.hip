@@ -37,9 +37,7 @@
 // caller).
 #include <string.h>
 
-#include "ctx.hpp"
-#include "kernels.hpp"
-#include "ops.hpp"
+#include "adj_dev.hpp"
 #include "viscalc.hpp"
 
 namespace i2e {
@@ -283,53 +281,6 @@ __global__ void k_unfold(int n, const int8_t *bm, double mu, const double *r, co
   if (u < n && bm[u]) z[u] = (r[u] - t[u]) / mu;
 }
 
-// ------------------------------------------------------------------ RK4 and its reverse
-// allinterv at (x, y) with the stencils and weights kept for the reverse
-struct Interp {
-  W4 wx, wy;           // vx and vy sub-grid stencils
-  double cx, cy;       // 1 where clampxy left the coordinate as it was
-  double ivx, ivy;     // interpolated velocity
-};
-__device__ __forceinline__ void interp_at(const Grid &g, const double *vx, const double *vy, double x, double y,
-                                          Interp &I) {
-  I.cx = (x >= 0 && x <= g.xsize) ? 1.0 : 0.0;
-  I.cy = (y >= 0 && y <= g.ysize) ? 1.0 : 0.0;
-  if (x < 0)
-    x = 0;
-  else if (x > g.xsize)
-    x = g.xsize;
-  if (y < 0)
-    y = 0;
-  else if (y > g.ysize)
-    y = g.ysize;
-  const int m10 = serch(g.gx, g.X, x), m20 = serch(g.gy, g.Y, y);
-  nodewt2(g.gx, g.gy, m10, vx_j0(g.gy, g.Y, m20, y), x, y, 0, +1, I.wx);
-  I.ivx = isum(I.wx, g.Y, vx);
-  nodewt2(g.gx, g.gy, vy_i0(g.gx, g.X, m10, x), m20, x, y, +1, 0, I.wy);
-  I.ivy = isum(I.wy, g.Y, vy);
-}
-// d(ivx, ivy)/d(x, y) transposed onto (bx, by): the bilinear weights are
-// linear in each coordinate inside the stencil's cell
-__device__ __forceinline__ void interp_grad_t(const Grid &g, const W4 &w, int ynx, int yny, const double *f,
-                                              double bar, double cx, double cy, double &px, double &py) {
-  const int Y = g.Y, m = w.i0 * Y + w.j0;
-  const double hx = subline(g.gx, w.i0 + 1, ynx) - subline(g.gx, w.i0, ynx);
-  const double hy = subline(g.gy, w.j0 + 1, yny) - subline(g.gy, w.j0, yny);
-  const double f00 = f[m], f01 = f[m + 1], f10 = f[m + Y], f11 = f[m + Y + 1];
-  const double dfx = (w.wy0 * (f10 - f00) + w.wy1 * (f11 - f01)) / hx;
-  const double dfy = (w.wx0 * (f01 - f00) + w.wx1 * (f11 - f10)) / hy;
-  px += bar * dfx * cx;
-  py += bar * dfy * cy;
-}
-__device__ __forceinline__ void scatter_w(const W4 &w, int Y, double bar, double *dst, int comp) {
-  if (bar == 0) return;
-  const int m = w.i0 * Y + w.j0;
-  aadd(&dst[3 * m + comp], w.wx0 * w.wy0 * bar);
-  aadd(&dst[3 * (m + 1) + comp], w.wx0 * w.wy1 * bar);
-  aadd(&dst[3 * (m + Y) + comp], w.wx1 * w.wy0 * bar);
-  aadd(&dst[3 * (m + Y + 1) + comp], w.wx1 * w.wy1 * bar);
-}
-
 // the markers of the objective (inside, markk > 0, active type, iron, mobile)
 __device__ __forceinline__ bool sink_marker(const i2e_params *p, const Grid &g, float fx, float fy, float fk, int t,
                                             double &m, double &rx, double &ry) {
@@ -342,26 +293,6 @@ __device__ __forceinline__ bool sink_marker(const i2e_params *p, const Grid &g, 
   rx = dx / r;
   ry = dy / r;
   return true;
-}
-
-struct Rk4 {
-  Interp s[4];
-  double vx, vy;
-};
-// movemark's velocity (k_movemark): RK4 stages, or one evaluation for Euler
-__device__ __forceinline__ void rk4(const i2e_params *p, const Grid &g, const double *vx, const double *vy,
-                                    double x0, double y0, double dt, Rk4 &R) {
-  interp_at(g, vx, vy, x0, y0, R.s[0]);
-  if (p->markmod == 1) {
-    R.vx = R.s[0].ivx;
-    R.vy = R.s[0].ivy;
-    return;
-  }
-  interp_at(g, vx, vy, x0 + R.s[0].ivx * dt / 2.0, y0 + R.s[0].ivy * dt / 2.0, R.s[1]);
-  interp_at(g, vx, vy, x0 + R.s[1].ivx * dt / 2.0, y0 + R.s[1].ivy * dt / 2.0, R.s[2]);
-  interp_at(g, vx, vy, x0 + R.s[2].ivx * dt, y0 + R.s[2].ivy * dt, R.s[3]);
-  R.vx = (R.s[0].ivx + 2.0 * R.s[1].ivx + 2.0 * R.s[2].ivx + R.s[3].ivx) / 6.0;
-  R.vy = (R.s[0].ivy + 2.0 * R.s[1].ivy + 2.0 * R.s[2].ivy + R.s[3].ivy) / 6.0;
 }
 
 __global__ __launch_bounds__(RB) void k_sink_part(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk,
@@ -463,33 +394,6 @@ __device__ __forceinline__ bool visc_marker(const i2e_params *p, const Grid &g, 
   return true;
 }
 
-// the two tent weights of the marker (nearest basic node, cell centre)
-struct Tent {
-  int nn, nc;      // node indices, -1 none
-  double wn, wc;
-};
-__device__ __forceinline__ Tent tent(const Grid &g, double x, double y) {
-  const int m10 = serch(g.gx, g.X, x), m20 = serch(g.gy, g.Y, y);
-  W4 w;
-  nodewt2(g.gx, g.gy, m10, m20, x, y, 0, 0, w);
-  Tent T{-1, -1, 0.0, 0.0};
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++) {
-      const double ddx = a ? w.wx1 : w.wx0, ddy = b ? w.wy1 : w.wy0;
-      if (ddx > 0.5 && ddy > 0.5) {
-        T.nn = (m10 + a) * g.Y + m20 + b;
-        T.wn = 4.0 * (ddx - 0.5) * (ddy - 0.5);
-      }
-    }
-  const double ddx = 1.0 - dabs(w.wx0 - 0.5), ddy = 1.0 - dabs(w.wy0 - 0.5);
-  if (ddx > 0.5 && ddy > 0.5) {
-    T.nc = (m10 + 1) * g.Y + m20 + 1;
-    T.wc = 4.0 * (ddx - 0.5) * (ddy - 0.5);
-  }
-  return T;
-}
-
-// S = {sum w eta, sum w} for nu (basic nodes) and nd (cell centres)
 __global__ void k_visc_sums(const i2e_params *__restrict__ p, Grid g, NodeF F, Markers mk, int64_t M, double *S) {
   const int N = g.X * g.Y;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x) {
@@ -587,6 +491,65 @@ static int sink_sums(i2e_ctx *c, double *smv, double *sm) {
   return I2E_OK;
 }
 
+// K^T mu = g for the Stokes rows of the context's current fields and
+// set-up (stokes_dt the stabilisation's dt); g is overwritten.  scratch:
+// three vectors of 3N.
+int i2e::stokes_solve_t(i2e_ctx *c, double *gv, double *mu, double *scratch, int *its_out, double *rr_out) {
+  const int X = c->X, Y = c->Y, N = c->N;
+  const size_t n3 = 3 * (size_t)N;
+  ST_OK(c->fmask.alloc(n3));
+  int8_t *bm = c->fmask.p;
+  double *tmp = scratch, *tb = scratch + n3, *tc = scratch + 2 * n3;
+  hipLaunchKernelGGL(k_bc_mask, dim3(nblk((long)n3, 256)), dim3(256), 0, c->s, c->bc, X, Y, bm);
+  HIP_OK(hipMemsetAsync(mu, 0, sizeof(double) * n3, c->s));
+  SOp<double> o = c->sop();
+  o.dt = c->stokes_dt;
+  // K^T mu = g, solved as (P^-1 K^T P) nu = P^-1 g, mu = P nu: where the
+  // viscous block is symmetric P^-1 K^T P = K, so the forward preconditioner
+  // applies unchanged and the residual is measured with the forward rows'
+  // scaling (continuity rows times mukoef)
+  auto A = [&](const double *x, double *y) -> int {
+    hipLaunchKernelGGL(k_copy_scale_p, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, 1.0 / o.mu, x, tmp);
+    hipLaunchKernelGGL(k_stokes_apply_t, nb2f(X, Y), dim3(256), 0, c->s, o, c->bc, (const double *)tmp, y);
+    hipLaunchKernelGGL(k_stokes_apply_t_bc, dim3(nblk((long)n3, 256)), dim3(256), 0, c->s, c->bc, X, Y, o.mu,
+                       (const double *)tmp, y);
+    hipLaunchKernelGGL(k_scale_p, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, o.mu, y);
+    return I2E_OK;
+  };
+  // preconditioner of C = P^-1 K^T P: eliminate the BC rows, whose
+  // transposes couple a ghost unknown to the interior rows with weight mu
+  // (C_BB = mu I), apply the forward preconditioner to the interior
+  // (reduced operator C_II - C_IB C_BI / mu, the transpose of the reduced
+  // forward operator up to P), and back-substitute:
+  //   r'_I = r_I - C_IB r_B / mu,  z_I = M r'_I,  z_B = (r_B - C_BI z_I) / mu
+  const unsigned gb = nblk((long)n3, 256);
+  auto Mf = [&](const double *r, double *z) -> int {
+    hipLaunchKernelGGL(k_mask_keep, dim3(gb), dim3(256), 0, c->s, (int)n3, bm, 1, r, tb);
+    ST_OK(A(tb, tc));
+    hipLaunchKernelGGL(k_fold, dim3(gb), dim3(256), 0, c->s, (int)n3, bm, o.mu, r, (const double *)tc, tb);
+    ST_OK(stokes_precond(c, tb, z, false));
+    hipLaunchKernelGGL(k_mask_keep, dim3(gb), dim3(256), 0, c->s, (int)n3, bm, 0, (const double *)z, tb);
+    ST_OK(A(tb, tc));
+    hipLaunchKernelGGL(k_unfold, dim3(gb), dim3(256), 0, c->s, (int)n3, bm, o.mu, r, (const double *)tc, z);
+    return I2E_OK;
+  };
+  hipLaunchKernelGGL(k_scale_p, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, o.mu, gv);
+  c->fs.sx = c->fs.sy = nullptr;
+  c->fs.split_of = nullptr;
+  int its = 0;
+  double rr = 0;
+  int st = c->fs.solve(A, Mf, gv, mu, c->o.stokes_rtol, c->o.maxit, c->s, &its, &rr);
+  *its_out = its;
+  *rr_out = rr;
+  if (st) {
+    set_error("adjoint: transposed Stokes solve, %d iterations, relative residual %.3e > %.1e", its, rr,
+              c->o.stokes_rtol);
+    return st;
+  }
+  hipLaunchKernelGGL(k_scale_p, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, 1.0 / o.mu, mu);
+  return I2E_OK;
+}
+
 extern "C" {
 
 int i2e_stokes_apply_t_dev(i2e_ctx *c, const double *x, double *y) {
@@ -634,65 +597,23 @@ int i2e_adjoint_sink_rate(i2e_ctx *c, i2e_flow_adjoint_out *out) {
   const size_t n3 = 3 * (size_t)N;
   ST_OK(c->fadj.alloc(2 * RG + 2 + 6 * n3 + 8 * (size_t)N + (M ? M : 1)));
   ST_OK(c->fflag.alloc(1));
-  ST_OK(c->fmask.alloc(n3));
-  double *part = c->fadj.p, *gv = part + 2 * RG + 2, *mu = gv + n3, *xs = mu + n3, *tmp = xs + n3, *tb = tmp + n3,
-         *tc = tb + n3, *lnu = tc + n3, *lnd = lnu + N, *lro = lnd + N, *lgp = lro + N, *S = lgp + N,
-         *gs = S + 4 * (size_t)N;
+  double *part = c->fadj.p, *gv = part + 2 * RG + 2, *mu = gv + n3, *xs = mu + n3, *tmp = xs + n3,
+         *lnu = tmp + 3 * n3, *lnd = lnu + N, *lro = lnd + N, *lgp = lro + N, *S = lgp + N, *gs = S + 4 * (size_t)N;
   DBuf<int> &flag = c->fflag;
-  int8_t *bm = c->fmask.p;
-  hipLaunchKernelGGL(k_bc_mask, dim3(nblk((long)n3, 256)), dim3(256), 0, c->s, c->bc, X, Y, bm);
   hipLaunchKernelGGL(k_stokes_pack, nb2f(X, Y), dim3(256), 0, c->s, X, Y, c->F.pr, c->F.vx, c->F.vy, xs);
   HIP_OK(hipMemsetAsync(gv, 0, sizeof(double) * n3, c->s));
-  HIP_OK(hipMemsetAsync(mu, 0, sizeof(double) * n3, c->s));
   HIP_OK(hipMemsetAsync(S, 0, sizeof(double) * 4 * (size_t)N, c->s));
   HIP_OK(hipMemsetAsync(flag.p, 0, sizeof(int), c->s));
   const unsigned gm = mgridf(M ? M : 1);
   if (M) hipLaunchKernelGGL(k_sink_adj, dim3(gm), dim3(256), 0, c->s, c->dP, g, c->F, c->mk, M, c->timestep, sm, gv);
-  // K^T mu = g, solved as (P^-1 K^T P) nu = P^-1 g, mu = P nu: where the
-  // viscous block is symmetric P^-1 K^T P = K, so the forward preconditioner
-  // applies unchanged and the residual is measured with the forward rows'
-  // scaling (continuity rows times mukoef)
-  SOp<double> o = c->sop();
-  o.dt = c->stokes_dt;
-  auto A = [&](const double *x, double *y) -> int {
-    hipLaunchKernelGGL(k_copy_scale_p, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, 1.0 / o.mu, x, tmp);
-    hipLaunchKernelGGL(k_stokes_apply_t, nb2f(X, Y), dim3(256), 0, c->s, o, c->bc, (const double *)tmp, y);
-    hipLaunchKernelGGL(k_stokes_apply_t_bc, dim3(nblk((long)n3, 256)), dim3(256), 0, c->s, c->bc, X, Y, o.mu,
-                       (const double *)tmp, y);
-    hipLaunchKernelGGL(k_scale_p, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, o.mu, y);
-    return I2E_OK;
-  };
-  // preconditioner of C = P^-1 K^T P: eliminate the BC rows, whose
-  // transposes couple a ghost unknown to the interior rows with weight mu
-  // (C_BB = mu I), apply the forward preconditioner to the interior
-  // (reduced operator C_II - C_IB C_BI / mu, the transpose of the reduced
-  // forward operator up to P), and back-substitute:
-  //   r'_I = r_I - C_IB r_B / mu,  z_I = M r'_I,  z_B = (r_B - C_BI z_I) / mu
-  const unsigned gb = nblk((long)n3, 256);
-  auto Mf = [&](const double *r, double *z) -> int {
-    hipLaunchKernelGGL(k_mask_keep, dim3(gb), dim3(256), 0, c->s, (int)n3, bm, 1, r, tb);
-    ST_OK(A(tb, tc));
-    hipLaunchKernelGGL(k_fold, dim3(gb), dim3(256), 0, c->s, (int)n3, bm, o.mu, r, (const double *)tc, tb);
-    ST_OK(stokes_precond(c, tb, z, false));
-    hipLaunchKernelGGL(k_mask_keep, dim3(gb), dim3(256), 0, c->s, (int)n3, bm, 0, (const double *)z, tb);
-    ST_OK(A(tb, tc));
-    hipLaunchKernelGGL(k_unfold, dim3(gb), dim3(256), 0, c->s, (int)n3, bm, o.mu, r, (const double *)tc, z);
-    return I2E_OK;
-  };
-  hipLaunchKernelGGL(k_scale_p, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, o.mu, gv);
-  c->fs.sx = c->fs.sy = nullptr;
-  c->fs.split_of = nullptr;
   int its = 0;
   double rr = 0;
-  int st = c->fs.solve(A, Mf, gv, mu, c->o.stokes_rtol, c->o.maxit, c->s, &its, &rr);
+  const int st = stokes_solve_t(c, gv, mu, tmp, &its, &rr);
   out->stokes_iters = its;
   out->stokes_relres = rr;
-  if (st) {
-    set_error("flow adjoint: transposed Stokes solve, %d iterations, relative residual %.3e > %.1e", its, rr,
-              c->o.stokes_rtol);
-    return st;
-  }
-  hipLaunchKernelGGL(k_scale_p, dim3(nblk(N, 256)), dim3(256), 0, c->s, N, 1.0 / o.mu, mu);
+  if (st) return st;
+  SOp<double> o = c->sop();
+  o.dt = c->stokes_dt;
   hipLaunchKernelGGL(k_stokes_field_adj, nb2f(X, Y), dim3(256), 0, c->s, o, c->bc, (const double *)xs,
                      (const double *)mu, lnu, lnd, lro, lgp);
   if (M) hipLaunchKernelGGL(k_visc_sums, dim3(gm), dim3(256), 0, c->s, c->dP, g, c->F, c->mk, M, S);

@@ -190,10 +190,6 @@ __global__ void k_grav_mean(Grid g, const double *ro, double R, double *part) {
   }
 }
 
-__device__ __forceinline__ double grav_dist(const Grid &g, int i, int j) {
-  double dx = (g.gx[i - 1] + g.gx[i] - g.xsize) / 2.0, dy = (g.gy[j - 1] + g.gy[j] - g.ysize) / 2.0;
-  return pow(dx * dx + dy * dy, 0.5);
-}
 
 // gravity rows (movemart.c:661-749) applied to x (N array over nodes)
 __global__ void k_grav_apply(Grid g, GravGeo q, const double *__restrict__ x, double *__restrict__ y) {

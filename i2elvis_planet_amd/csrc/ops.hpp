@@ -18,6 +18,11 @@ __host__ __device__ inline void cont_shift(int64_t d, int64_t Y, int &dm1, int &
 struct GravGeo {
   double R, gxkoef, mu;
 };
+// distance of cell (i, j)'s centre from the box centre (movemart.c:636-650)
+__device__ __forceinline__ double grav_dist(const Grid &g, int i, int j) {
+  double dx = (g.gx[i - 1] + g.gx[i] - g.xsize) / 2.0, dy = (g.gy[j - 1] + g.gy[j] - g.ysize) / 2.0;
+  return pow(dx * dx + dy * dy, 0.5);
+}
 
 __global__ void k_stokes_apply(SOp<double> o, BcTab bc, const double *x, double *y);
 __global__ void k_stokes_rhs(SOp<double> o, BcTab bc, double *b);

@@ -418,6 +418,38 @@ int i2e_adjoint_peak_t(i2e_ctx *c, i2e_adjoint_out *out);
 /* y = A^T x for the heat rows of i2e_heat_apply_dev (device pointers).     */
 int i2e_heat_apply_t_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
 
+/* The same objective J through the whole recorded step (no counterpart in
+ * the reference): i2e_adjoint_peak_t_coupled, called right after
+ * i2e_record, returns J and its gradient with respect to h0_scale, phi0 and
+ * s, a common factor on every rock's activation energy markdh (at s = 1),
+ * with the flow differentiated too: the marker positions move with the
+ * Stokes velocities (RK4 or Euler, markmart.c:74-115, reversed from the
+ * positions recorded at every step), the velocities solve the Stokes rows
+ * (movemart.c:4-426; transposed solve with viterate's preconditioner) whose
+ * viscosities, density and gravity potential (movemart.c:622-785,
+ * transposed gravity solve) come from ronurecalc's scatter at those
+ * positions (markmart.c:840-1009, weights differentiated in the positions),
+ * and the viscosities from viscalc's Arrhenius branch of the marker
+ * temperatures (markmart.c:1959-1977).  Time steps are held: records whose
+ * dt maxvelstep or titerate's heating-rate limit shortened are refused
+ * (raise maxxystep and maxtkstep), as are non-uniform grids, shear heating
+ * and the cases i2e_adjoint_peak_t and the flow adjoint refuse.  Device
+ * memory: the record's 12 B per marker and 48 B per node per step plus 4 B
+ * per marker and 56 B per node per step.  The call leaves the context's
+ * node fields at those of the first recorded step (upload before stepping
+ * again).                                                                   */
+typedef struct i2e_coupled_adjoint_out {
+  double J;
+  double dJ_dh0, dJ_dphi0, dJ_dEscale;
+  double heat_relres_max, stokes_relres_max, grav_relres_max;
+  int64_t heat_iters, stokes_iters, grav_iters;
+  int32_t steps, pad;
+} i2e_coupled_adjoint_out;
+int i2e_adjoint_peak_t_coupled(i2e_ctx *c, i2e_coupled_adjoint_out *out);
+/* dJ/d(initial marker positions) from the last i2e_adjoint_peak_t_coupled
+ * of the current record: host arrays of marknum doubles, caller's order.    */
+int i2e_adjoint_marker_positions(i2e_ctx *c, double *dJ_dx, double *dJ_dy);
+
 /* ---- flow adjoint: one Stokes solve and the advection that reads it ------
  * No counterpart in the reference: a sensitivity of the iron sinking rate
  *   J = sum_i m_i (v_i . r_i) / sum_i m_i

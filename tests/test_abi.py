@@ -30,18 +30,19 @@ def test_struct_layouts_match_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "i2e.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(i2e_params),"
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(i2e_params),"
         " sizeof(i2e_host_state), sizeof(i2e_solver_opts), sizeof(i2e_stats), sizeof(i2e_rock),"
         " offsetof(i2e_params, rock), offsetof(i2e_host_state, bondm), sizeof(i2e_resid),"
         " sizeof(i2e_adjoint_out), offsetof(i2e_adjoint_out, steps), sizeof(i2e_init_info),"
-        " sizeof(i2e_body_stats), offsetof(i2e_body_stats, count_toohot)); return 0;}\n")
+        " sizeof(i2e_body_stats), offsetof(i2e_body_stats, count_toohot), sizeof(i2e_coupled_adjoint_out),"
+        " offsetof(i2e_coupled_adjoint_out, steps)); return 0;}\n")
     exe = tmp_path / "sz"
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)])
     got = [int(v) for v in subprocess.check_output([str(exe)]).split()]
     want = [C.sizeof(abi.Params), C.sizeof(abi.HostStateC), C.sizeof(abi.SolverOpts), C.sizeof(abi.Stats),
             C.sizeof(abi.Rock), abi.Params.rock.offset, abi.HostStateC.bondm.offset, C.sizeof(abi.Resid),
             C.sizeof(abi.AdjointOut), abi.AdjointOut.steps.offset, C.sizeof(abi.InitInfo), C.sizeof(abi.BodyStats),
-            abi.BodyStats.count_toohot.offset]
+            abi.BodyStats.count_toohot.offset, C.sizeof(abi.CoupledAdjointOut), abi.CoupledAdjointOut.steps.offset]
     assert got == want
 
 
