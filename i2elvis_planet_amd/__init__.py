@@ -27,7 +27,7 @@ _EXPORTS = [
     "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
     "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
-    "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_adjoint_peak_t_coupled", "i2e_adjoint_marker_positions", "i2e_heat_apply_t_dev", "i2e_sink_rate", "i2e_adjoint_sink_rate", "i2e_stokes_apply_t_dev",
+    "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_adjoint_peak_t_coupled", "i2e_adjoint_marker_positions", "i2e_visual", "i2e_visual_write", "i2e_markersave", "i2e_grain", "i2e_heat_apply_t_dev", "i2e_sink_rate", "i2e_adjoint_sink_rate", "i2e_stokes_apply_t_dev",
     "i2e_type_counts", "i2e_maxvelstep_noalloc", "i2e_kernel_timer_repr", "i2e_set_marker_sort",
     "i2e_sort_markers", "i2e_init_read", "i2e_init_generate", "i2e_mode_read", "i2e_tuned_shapes", "i2e_hydration",
 ]
@@ -37,6 +37,24 @@ _lib = None
 
 class I2EError(RuntimeError):
     pass
+
+
+def visual_write(path, spec, timesum, out):
+    """saver()'s text file for a grid from Context.visual (loadjmart.c:681, 1130-1165)."""
+    load_library()
+    a = np.ascontiguousarray(out)
+    _chk(_lib.i2e_visual_write(os.fsencode(path), C.byref(spec), float(timesum),
+                               None if spec.type == 5 else a.ctypes.data, a.ctypes.data if spec.type == 5 else None),
+         "visual_write")
+
+
+def markersave(directory, params, hs, melt=None):
+    """markersave()'s four marker text files (loadjmart.c:3-53)."""
+    load_library()
+    cs = hs.cstruct()
+    m = None if melt is None else np.ascontiguousarray(melt, np.float64)
+    _chk(_lib.i2e_markersave(os.fsencode(directory), C.byref(params), C.byref(cs),
+                             None if m is None else m.ctypes.data), "markersave")
 
 
 def load_library(path=LIB_PATH):
@@ -105,6 +123,10 @@ def load_library(path=LIB_PATH):
         "i2e_adjoint_peak_t": ([vp, C.POINTER(abi.AdjointOut)], C.c_int),
         "i2e_adjoint_peak_t_coupled": ([vp, C.POINTER(abi.CoupledAdjointOut)], C.c_int),
         "i2e_adjoint_marker_positions": ([vp, vp, vp], C.c_int),
+        "i2e_visual": ([vp, C.POINTER(abi.VisualSpec), vp, vp, vp], C.c_int),
+        "i2e_visual_write": ([C.c_char_p, C.POINTER(abi.VisualSpec), C.c_double, vp, vp], C.c_int),
+        "i2e_markersave": ([C.c_char_p, C.POINTER(abi.Params), C.POINTER(abi.HostStateC), vp], C.c_int),
+        "i2e_grain": ([vp, C.POINTER(abi.GrainParams), vp], C.c_int),
         "i2e_sink_rate": ([vp, C.POINTER(C.c_double)], C.c_int),
         "i2e_adjoint_sink_rate": ([vp, C.POINTER(abi.FlowAdjointOut)], C.c_int),
         "i2e_stokes_apply_t_dev": ([vp, vp, vp], C.c_int),
@@ -207,6 +229,7 @@ class Context:
     def upload(self, hs):
         self._cs = hs.cstruct()
         _chk(_lib.i2e_upload(self._h, C.byref(self._cs)), "i2e_upload")
+        self.M = int(hs.marknum)
 
     def download(self, hs):
         cs = hs.cstruct()
@@ -331,6 +354,25 @@ class Context:
         gy = np.zeros(n)
         _chk(_lib.i2e_adjoint_marker_positions(self._h, gx.ctypes.data, gy.ctypes.data), "adjoint_marker_positions")
         return gx, gy
+
+    # post-processing: i2jmart's visualisation grid ---------------------------------
+    def visual(self, spec):
+        """One output of i2jmart's saver on the device (loadjmart.c:619-1197):
+        returns (values float32, or rock types int32 for type 5; melt fraction
+        per marker)."""
+        n = spec.xresol * spec.yresol
+        out = np.zeros(n, np.int32 if spec.type == 5 else np.float32)
+        melt = np.zeros(max(self.M, 1))
+        vp_val = None if spec.type == 5 else out.ctypes.data
+        vp_rock = out.ctypes.data if spec.type == 5 else None
+        _chk(_lib.i2e_visual(self._h, C.byref(spec), vp_val, vp_rock, melt.ctypes.data), "visual")
+        return out, melt[:self.M]
+
+    def grain(self, gp, markgr):
+        """grain() on the device (grain.c:4-156): markgr (float32, one per
+        marker, caller's order) updated in place; marker types may change."""
+        assert markgr.dtype == np.float32 and markgr.flags.c_contiguous and markgr.size == self.M
+        _chk(_lib.i2e_grain(self._h, C.byref(gp), markgr.ctypes.data), "grain")
 
     # flow adjoint: one Stokes solve and the advection that reads it ---------------
     def sink_rate(self):

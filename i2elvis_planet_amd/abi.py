@@ -151,6 +151,20 @@ class PrnHeader(C.Structure):
                 ("fe6056_init", C.c_double), ("timesum", C.c_double)]
 
 
+class VisualSpec(C.Structure):
+    """i2e_visual_spec: one output of i2jmart's saver (type = amir.t3c's letter
+    code, the window as fractions of the box, the pixel resolution)."""
+    _fields_ = [("type", C.c_int32), ("gridmod", C.c_int32), ("xresol", C.c_int32), ("yresol", C.c_int32),
+                ("xbeg", C.c_double), ("xend", C.c_double), ("ybeg", C.c_double), ("yend", C.c_double)]
+
+
+class GrainParams(C.Structure):
+    """i2e_grain_params: grain() controls (growth_model, gr_init in
+    micrometres, start_time and impact_time[1] in Ma after CAIs)."""
+    _fields_ = [("growth_model", C.c_int32), ("pad", C.c_int32), ("gr_init", C.c_double),
+                ("start_time_ma", C.c_double), ("impact_time_ma", C.c_double)]
+
+
 class BodyStats(C.Structure):
     """i2e_body_stats (include/i2e.h): impact()'s hydration and body
     statistics (impact.c:3022-3228)."""

@@ -481,6 +481,66 @@ int i2e_adjoint_sink_rate(i2e_ctx *c, i2e_flow_adjoint_out *out);
 /* y = K^T x for the Stokes rows of i2e_stokes_apply_dev (device pointers).  */
 int i2e_stokes_apply_t_dev(i2e_ctx *c, const double *x_dev, double *y_dev);
 
+/* ---- post-processing: the visualisation grid of i2jmart's saver ---------
+ * i2jmart (i2jmart.c, loadjmart.c) turns a .prn file into a regular grid of
+ * one field for plotting (Amira).  i2e_visual computes that grid on the
+ * device from the context's state, as saver() does (loadjmart.c:619-1197,
+ * one x-chunk): markers inside the window (every markstp-th, markt < 50) add
+ * their value and bilinear weight to the four surrounding pixels in float32,
+ * in marker order; a pixel with weight divides, one without interpolates the
+ * node fields at its position (allinter1, loadjmart.c:1327-1520).  type
+ * (amir.t3c's letter): 1 T (K), 2 log10 viscosity, 4 log10 shear heating,
+ * 5 rock types (the type of largest summed weight per pixel, in type order,
+ * empty pixels filled from neighbours within gridmod pixels), 6 density,
+ * 7 conductivity, 8 heat generation (the rock table's markht is not carried:
+ * shear heating only on markers, plus the node ht field in empty pixels),
+ * 12 vorticity / strain-rate ratio (spncalc, loadjmart.c:1523-1617).  Types
+ * 3 (the reference leaves the marker value unset), 9-11 (seismic velocities,
+ * never computed there) and densimod 2 are refused (I2E_ERR_UNSUPPORTED).
+ * xbeg..yend are fractions of the box (clamped to [0, 1]); xresol, yresol
+ * >= 2, xresol * yresol * 2 <= 40000000 (the reference's val0[]).  Outputs,
+ * each xresol * yresol long, pixel (m1, m2) at m1 * yresol + m2: val (types
+ * != 5) or rock (type 5; -1 where none).  melt (optional, marknum doubles):
+ * the melt fraction saver() computes per visited marker for markersave()
+ * (0 elsewhere).  Markers with markk < 1 take the interpolated node T and
+ * raise their marktmax as saver() does; the context's markers are left as
+ * they are.                                                                 */
+typedef struct i2e_visual_spec {
+  int32_t type, gridmod;
+  int32_t xresol, yresol;
+  double xbeg, xend, ybeg, yend;
+} i2e_visual_spec;
+int i2e_visual(i2e_ctx *c, const i2e_visual_spec *spec, float *val, int32_t *rock, double *melt);
+/* The text file saver() writes (loadjmart.c:681, 1130-1165): the time in
+ * years and the resolution, then every pixel in m1-outer order as
+ * " % 5.4e", or for type 5 the rock types with runs of more than three equal
+ * values along y written as " -2 count type".                                */
+int i2e_visual_write(const char *path, const i2e_visual_spec *spec, double timesum, const float *val,
+                     const int32_t *rock);
+/* markersave (loadjmart.c:3-53): marker_melt.txt, marker_comp.txt,
+ * marker_magnet.txt and marker_magnet_time.txt in directory dir, one line
+ * per marker (x / xsize, y / ysize, value), marknum + 1 lines as the
+ * reference writes them (the last marker record is zero).  melt may be NULL
+ * (zeros).                                                                  */
+int i2e_markersave(const char *dir, const i2e_params *p, const i2e_host_state *s, const double *melt);
+
+/* grain() (grain.c:4-156), called by main() after each output cycle
+ * (i2mart.c:178): olivine grain growth per marker, d <- (d^n + k0
+ * exp(-E/(R T)) dt)^(1/n) with the context's last timestep, for olivine in
+ * molten FeS (types 5, 25) after the first impact and pure olivine (6, 26)
+ * after the impact (growth_model 1) or the model start (2); with growth
+ * models 1 and 2 and t + dt in [impact, impact + 0.45 Ma], markers in the
+ * mantle sector 100-200 km from the centre at 45 +- 10 degrees become
+ * pallasite (types 6 -> 5, 26 -> 25) with the initial grain size.  markgr:
+ * the marknum grain sizes (markgr of headmart.c:124, micrometres) in the
+ * caller's order, updated in place; marker types change in the context.
+ * impact_time_ma: impact_time[1] of the impact history (Ma after CAIs).    */
+typedef struct i2e_grain_params {
+  int32_t growth_model, pad;
+  double gr_init, start_time_ma, impact_time_ma;
+} i2e_grain_params;
+int i2e_grain(i2e_ctx *c, const i2e_grain_params *gp, float *markgr);
+
 /* ---- binary .prn checkpoint (saver/loader of loadmart.c) -----------------
  * The byte layout of the reference's binary output (saver, loadmart.c:671-809;
  * loader, loadmart.c:391-540; read by support_files/plot2d.py:157-283), for a

@@ -1597,3 +1597,350 @@ void orc_csr_matvec(const orc_csr *A, const double *x, double *y) {
     y[i] = s;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* post-processing: the visualisation grid of i2jmart's saver               */
+/* ------------------------------------------------------------------------ */
+/* allinter1, loadjmart.c:1327-1520 (intermod 0): the values saver() reads */
+typedef struct E1 {
+  double t, nu, ro, kt, ht;            /* eps[2], eps[16..19]: basic-node cell */
+  double exy, sxy, exysxy, esp;        /* eps[4], eps[5], eps[13], eps[27]     */
+  double exx, sxx, eyy, syy, pr, exxsxx, eyysyy; /* eps[6..10], eps[14], [15] */
+} E1;
+static void allinter1(const G *g, const i2e_host_state *s, const double *esp, double x, double y, E1 *e) {
+  W4 w;
+  int64_t m10, m20;
+  if (x < 0) x = 0; else if (x > g->xsize) x = g->xsize;
+  if (y < 0) y = 0; else if (y > g->ysize) y = g->ysize;
+  m10 = serch(g->gx, g->X, x);
+  m20 = serch(g->gy, g->Y, y);
+  nodewt2(g, m10, m20, x, y, 0, 0, &w);
+  e->t = isum(g, &w, s->tk);
+  e->nu = isum(g, &w, s->nu);
+  e->ro = isum(g, &w, s->ro);
+  e->kt = isum(g, &w, s->kt);
+  e->ht = isum(g, &w, s->ht);
+  nodewt2(g, bn_i0(m10, g->X), bn_i0(m20, g->Y), x, y, 0, 0, &w);
+  e->exy = isum(g, &w, s->exy);
+  e->sxy = isum(g, &w, s->sxy);
+  e->exysxy = isum2(g, &w, s->exy, s->sxy);
+  e->esp = isum(g, &w, esp);
+  nodewt2(g, ce_i0(g->gx, m10, x, g->X), ce_i0(g->gy, m20, y, g->Y), x, y, -1, -1, &w);
+  e->exx = isum(g, &w, s->exx);
+  e->sxx = isum(g, &w, s->sxx);
+  e->eyy = isum(g, &w, s->eyy);
+  e->syy = isum(g, &w, s->syy);
+  e->pr = isum(g, &w, s->pr);
+  e->exxsxx = isum2(g, &w, s->exx, s->sxx);
+  e->eyysyy = isum2(g, &w, s->eyy, s->syy);
+}
+
+/* spncalc, loadjmart.c:1523-1617 (stoksfd 0): the rotation rate at an
+ * interior basic node, (dvy/dx - dvx/dy) / 2 with two-point weights        */
+static double spncalc(const G *g, const i2e_host_state *s, int64_t m1, int64_t m2) {
+  const int64_t Y = g->Y;
+  int64_t m2min = m2 - 1 < 0 ? 0 : m2 - 1, m2max = m2 > Y - 2 ? Y - 2 : m2;
+  int64_t m1min = m1 - 1 < 0 ? 0 : m1 - 1, m1max = m1 > g->X - 2 ? g->X - 2 : m1;
+  double d0, d1, a = 0, b = 0;
+  fd2_deriv((g->gy[m2min] + g->gy[m2min + 1]) / 2.0, (g->gy[m2max] + g->gy[m2max + 1]) / 2.0, &d0, &d1);
+  a += s->vx[m1 * Y + m2min] * d0 / 2.0;
+  a += s->vx[m1 * Y + m2max] * d1 / 2.0;
+  fd2_deriv((g->gx[m1min] + g->gx[m1min + 1]) / 2.0, (g->gx[m1max] + g->gx[m1max + 1]) / 2.0, &d0, &d1);
+  b += s->vy[m1min * Y + m2] * d0 / 2.0;
+  b += s->vy[m1max * Y + m2] * d1 / 2.0;
+  return b - a;
+}
+
+/* saver(), loadjmart.c:619-1197, for one output and one x-chunk: val (float,
+ * xresol*yresol) for spec type != 5, rock (int32) for type 5; melt (marknum,
+ * may be NULL) as markersave() reads it.  The markers of s are modified as
+ * saver() modifies them (markk < 1 reset, marktmax).                        */
+int orc_visual(const i2e_params *p, i2e_host_state *s, const i2e_visual_spec *sp, float *val, int32_t *rock,
+               double *melt) {
+  const G g = mkG(p, s);
+  const int type = sp->type;
+  const int64_t M = s->marknum, X = g.X, Y = g.Y;
+  int64_t xresol = sp->xresol, yresol = sp->yresol, nresol = xresol * yresol, markstp, m1, m2, m3, m4;
+  double xb = sp->xbeg, xe = sp->xend, yb = sp->ybeg, ye = sp->yend;
+  double xminx, xmaxx, xresx, yminy, ymaxy, yresy, xminx0, yminy0, ival = 0, ea, na;
+  float *val0;
+  int64_t *lin0;
+  double *esp;
+  if (type == 3 || (type >= 9 && type <= 11) || type < 1 || type > 12 || p->densimod == 2) return I2E_ERR_UNSUPPORTED;
+  if (xresol < 2 || yresol < 2 || nresol * 2 > 40000000) return I2E_ERR_ARG;
+  if (xb < 0) xb = 0;
+  if (xe > 1.0) xe = 1.0;
+  if (yb < 0) yb = 0;
+  if (ye > 1.0) ye = 1.0;
+  xminx0 = xminx = xb * g.xsize;
+  xmaxx = xe * g.xsize;
+  xresx = (xmaxx - xminx) / (double)(xresol - 1);
+  yminy0 = yminy = yb * g.ysize;
+  ymaxy = ye * g.ysize;
+  yresy = (ymaxy - yminy) / (double)(yresol - 1);
+  yminy -= yresy;
+  if (yminy < 0) yminy = 0;
+  ymaxy += yresy;
+  if (ymaxy > g.ysize) ymaxy = g.ysize;
+  /* one chunk: xresolsum = 0 (loadjmart.c:701-708) */
+  xminx -= xresx;
+  if (xminx < 0) xminx = 0;
+  xmaxx = xb * g.xsize + xresx * (double)xresol;
+  if (xmaxx > g.xsize) xmaxx = g.xsize;
+  ival = 0.1 * (double)M / (double)nresol;
+  markstp = (int64_t)ival;
+  if (markstp < 1) markstp = 1;
+  val0 = (float *)calloc(2 * nresol, sizeof(float));
+  lin0 = (int64_t *)calloc(2 * nresol, sizeof(int64_t));
+  esp = (double *)calloc(X * Y, sizeof(double));
+  if (!val0 || !lin0 || !esp) return I2E_ERR_ARG;
+  for (m1 = 1; m1 < X - 1; m1++)
+    for (m2 = 1; m2 < Y - 1; m2++) esp[m1 * Y + m2] = spncalc(&g, s, m1, m2);
+  if (melt) memset(melt, 0, sizeof(double) * M);
+#define INWIN(k) ((double)s->markx[k] > xminx && (double)s->markx[k] < xmaxx && (double)s->marky[k] > yminy && \
+                  (double)s->marky[k] < ymaxy)
+#define POS(k)                                                    \
+  do {                                                            \
+    ea = ((double)(s->markx[k]) - xminx0) / xresx + 1.0;          \
+    m1 = (int64_t)(ea);                                           \
+    if (m1 < 0) m1 = 0;                                           \
+    if (m1 > xresol) m1 = xresol;                                 \
+    ea -= (double)(m1);                                           \
+    na = ((double)(s->marky[k]) - yminy0) / yresy + 1.0;          \
+    m2 = (int64_t)(na);                                           \
+    if (m2 < 0) m2 = 0;                                           \
+    if (m2 > yresol) m2 = yresol;                                 \
+    na -= (double)(m2);                                           \
+  } while (0)
+  if (type == 5) {
+    int cmin = 1000, cmax = 0, ccur, yn;
+    for (m3 = 0; m3 < M; m3 += markstp)
+      if (INWIN(m3) && s->markt[m3] < 50) {
+        if (cmin > s->markt[m3]) cmin = s->markt[m3];
+        if (cmax < s->markt[m3]) cmax = s->markt[m3];
+      }
+    for (ccur = cmin; ccur <= cmax; ccur++) {
+      for (m3 = 0; m3 < M; m3 += markstp)
+        if (INWIN(m3) && s->markt[m3] == ccur) {
+          POS(m3);
+          m4 = nresol + (m1 - 1) * yresol + (m2 - 1);
+          if (m1 > 0 && m2 > 0) val0[m4] += (float)((1.0 - ea) * (1.0 - na));
+          if (m1 > 0 && m2 < yresol) val0[m4 + 1] += (float)((1.0 - ea) * na);
+          if (m1 < xresol && m2 > 0) val0[m4 + yresol] += (float)(ea * (1.0 - na));
+          if (m1 < xresol && m2 < yresol) val0[m4 + yresol + 1] += (float)(ea * na);
+        }
+      for (m1 = 0; m1 < nresol; m1++)
+        if (val0[nresol + m1] > val0[m1]) {
+          val0[m1] = val0[nresol + m1];
+          lin0[m1] = ccur;
+          val0[nresol + m1] = 0;
+        }
+    }
+    yn = 0;
+    for (m1 = 0; m1 < nresol; m1++)
+      if (!val0[m1]) {
+        lin0[m1] = -1;
+        yn = 1;
+      }
+    if (sp->gridmod && yn) {
+      int64_t m5, m6, m7, r;
+      for (m1 = 0; m1 < xresol; m1++)
+        for (m2 = 0; m2 < yresol; m2++) {
+          m3 = m1 * yresol + m2;
+          if (lin0[m3] != -1) continue;
+          r = 1;
+          yn = 0;
+          do {
+            cmin = 1000, cmax = 0;
+            for (m5 = m1 - r; m5 <= m1 + r; m5++)
+              for (m6 = m2 - r; m6 <= m2 + r; m6++)
+                if (m5 >= 0 && m5 < xresol && m6 >= 0 && m6 < yresol) {
+                  m7 = m5 * yresol + m6;
+                  if (lin0[m7] != -1) {
+                    yn = 1;
+                    if (cmin > lin0[m7]) cmin = (int)lin0[m7];
+                    if (cmax < lin0[m7]) cmax = (int)lin0[m7];
+                  }
+                }
+            r++;
+          } while (!yn && r <= sp->gridmod);
+          if (!yn) continue;
+          for (ccur = cmin; ccur <= cmax; ccur++) {
+            for (m5 = m1 - r + 1; m5 < m1 + r; m5++)
+              for (m6 = m2 - r + 1; m6 < m2 + r; m6++)
+                if (m5 >= 0 && m5 < xresol && m6 >= 0 && m6 < yresol) {
+                  m7 = m5 * yresol + m6;
+                  if (lin0[m7] == ccur) {
+                    ea = ABSV(((double)(m5 - m1)) / (double)(r));
+                    na = ABSV(((double)(m6 - m2)) / (double)(r));
+                    val0[nresol + m3] += (float)((1.0 - ea) * (1.0 - na)) * val0[m7];
+                  }
+                }
+            if (val0[nresol + m3] > val0[m3]) {
+              val0[m3] = val0[nresol + m3];
+              lin0[nresol + m3] = ccur;
+              val0[nresol + m3] = 0;
+            }
+          }
+        }
+      for (m1 = 0; m1 < nresol; m1++)
+        if (lin0[m1] == -1 && val0[m1]) {
+          lin0[m1] = lin0[nresol + m1];
+          lin0[nresol + m1] = 0;
+        }
+    }
+    for (m1 = 0; m1 < nresol; m1++) rock[m1] = (int32_t)lin0[m1];
+  } else {
+    for (m3 = 0; m3 < M; m3 += markstp)
+      if (INWIN(m3) && s->markt[m3] < 50) {
+        E1 e;
+        double mpb, mtk, mnu, mro, mbb, mcp, mkt, mht;
+        int mm2;
+        allinter1(&g, s, esp, (double)s->markx[m3], (double)s->marky[m3], &e);
+        mpb = e.pr * 1e-5;
+        if (s->markk[m3] < 1.0) s->markk[m3] = (float)(e.t);
+        mtk = (double)(s->markk[m3]);
+        if (s->markt[m3] != 0)
+          if (s->marktmax[m3] < s->markk[m3]) s->marktmax[m3] = (float)(s->markk[m3]);
+        mm2 = s->markt[m3];
+        if (melt) {
+          double xm = 0;
+          const int base = mm2 < 20 ? mm2 : mm2 - 20;
+          if (p->melt_on && base >= 0 && base < I2E_MAXTYPES && p->meltable[base]) {
+            xm = (mtk - p->melt_ts) / (p->melt_tl - p->melt_ts);
+            if (xm < 0.0) xm = 0.0;
+            if (xm > 1.0) xm = 1.0;
+          }
+          melt[m3] = xm;
+        }
+        mnu = s->markv[m3];
+        if (mnu < p->nubeg) mnu = p->nubeg;
+        if (mnu > p->nuend) mnu = p->nuend;
+        mro = dencalc(p, mtk, mpb, (double)(s->markpor[m3]), mm2, &mbb);
+        mcp = p->rock[mm2].cp;
+        mkt = (p->rock[mm2].kt + p->rock[mm2].kf / (mtk + 77.0)) * exp(p->rock[mm2].kp * mpb);
+        if (p->rock[mm2].kt < 0)
+          mkt = -p->rock[mm2].kt / (1.0 + p->rock[mm2].kf * (mtk - p->rock[mm2].kp) / p->rock[mm2].kp);
+        mht = 0.0; /* markht: not carried */
+        (void)mcp;
+        switch (type) {
+          case 1: ival = mtk; break;
+          case 2: ival = log(mnu) / log(10.0); break;
+          case 4:
+            ival = 2.0 * e.exysxy + e.exxsxx + e.eyysyy;
+            if (ival < 1e-30) ival = 1e-30;
+            ival = log(ival) / log(10.0);
+            break;
+          case 6: ival = mro; break;
+          case 7: ival = mkt; break;
+          case 8: ival = mht + 2.0 * e.exysxy + e.exxsxx + e.eyysyy; break;
+          case 12:
+            ival = pow(e.exy * e.exy + 0.5 * (e.exx * e.exx + e.eyy * e.eyy), 0.5);
+            if (ival < 1e-30) ival = 1e-30;
+            ival = e.esp / ival;
+            break;
+        }
+        POS(m3);
+        m4 = (m1 - 1) * yresol + (m2 - 1);
+        if (m1 > 0 && m2 > 0) {
+          val0[m4] += (float)(ival * (1.0 - ea) * (1.0 - na));
+          val0[nresol + m4] += (float)((1.0 - ea) * (1.0 - na));
+        }
+        if (m1 > 0 && m2 < yresol) {
+          val0[m4 + 1] += (float)(ival * (1.0 - ea) * na);
+          val0[nresol + m4 + 1] += (float)((1.0 - ea) * na);
+        }
+        if (m1 < xresol && m2 > 0) {
+          val0[m4 + yresol] += (float)(ival * ea * (1.0 - na));
+          val0[nresol + m4 + yresol] += (float)(ea * (1.0 - na));
+        }
+        if (m1 < xresol && m2 < yresol) {
+          val0[m4 + yresol + 1] += (float)(ival * ea * na);
+          val0[nresol + m4 + yresol + 1] += (float)(ea * na);
+        }
+      }
+    for (m1 = 0; m1 < xresol; m1++)
+      for (m2 = 0; m2 < yresol; m2++) {
+        m4 = m1 * yresol + m2;
+        if (val0[nresol + m4]) {
+          val0[m4] /= val0[nresol + m4];
+        } else {
+          E1 e;
+          allinter1(&g, s, esp, xminx0 + xresx * ((double)(m1)), yminy0 + yresy * ((double)(m2)), &e);
+          switch (type) {
+            case 1: val0[m4] = (float)(e.t); break;
+            case 2: val0[m4] = (float)(log(e.nu) / log(10.0)); break;
+            case 4:
+              val0[m4] = (float)(2.0 * e.exysxy + e.exxsxx + e.eyysyy);
+              if (val0[m4] < 1e-30) val0[m4] = 1e-30;
+              val0[m4] = (float)(log(val0[m4]) / log(10.0));
+              break;
+            case 6: val0[m4] = (float)(e.ro); break;
+            case 7: val0[m4] = (float)(e.kt); break;
+            case 8: val0[m4] = (float)(e.ht + 2.0 * e.exysxy + e.exxsxx + e.eyysyy); break;
+            case 12:
+              ival = pow(e.exy * e.exy + 0.5 * (e.exx * e.exx + e.eyy * e.eyy), 0.5);
+              if (ival < 1e-30) ival = 1e-30;
+              val0[m4] = e.esp / ival;
+              break;
+          }
+        }
+      }
+    memcpy(val, val0, sizeof(float) * nresol);
+  }
+#undef INWIN
+#undef POS
+  free(val0);
+  free(lin0);
+  free(esp);
+  return I2E_OK;
+}
+
+/* grain(), grain.c:4-156: olivine grain growth and the pallasite emplacement
+ * after the first impact, every marker in index order; markgr updated in
+ * place, types of s changed                                                 */
+int orc_grain(const i2e_params *p, i2e_host_state *s, const i2e_grain_params *gp, double timesum, double timestep,
+              float *markgr) {
+  double n_grain = 3.70, E_grain = 1.0098e+5, k0_grain = 3.2033, n_oliv = 2.250, E_oliv = 5.71674e+5,
+         k0_oliv = 1.06593e+16, pival = 3.141592654, radius_core = 1.000e+5, radius_planet = 2.000e+5,
+         ref_angle = 45.000, dev_angle = 10.000, time_window = 0.45;
+  double curr_angle, distancee, dist_x, dist_y, a, b, c;
+  const int growth_model = gp->growth_model;
+  int64_t mm1;
+  for (mm1 = 0; mm1 < s->marknum; mm1++) {
+    if ((growth_model == 1 || growth_model == 2) &&
+        ((timesum + timestep) >= (gp->impact_time_ma * 1.000e+6 * 365.250 * 24.000 * 3600.000)) &&
+        ((timesum + timestep) <= ((gp->impact_time_ma + time_window) * 1.000e+6 * 365.250 * 24.000 * 3600.000))) {
+      dist_x = s->markx[mm1] - (p->xsize / 2.000);
+      dist_y = s->marky[mm1] - (p->ysize / 2.000);
+      distancee = pow((dist_x * dist_x + dist_y * dist_y), 0.500);
+      if (distancee >= radius_core && distancee <= radius_planet && dist_x > 0.000 && dist_y < 0.000) {
+        curr_angle = asin(dist_x / distancee) / pival * 180.000;
+        if (curr_angle >= ref_angle - dev_angle && curr_angle <= ref_angle + dev_angle) {
+          if (s->markt[mm1] == 6) s->markt[mm1] = 5;
+          if (s->markt[mm1] == 26) s->markt[mm1] = 25;
+          markgr[mm1] = (float)(gp->gr_init);
+        }
+      }
+    }
+    if ((timesum + timestep) >= (gp->impact_time_ma * 1.000e+6 * 365.250 * 24.000 * 3600.000)) {
+      if ((s->markt[mm1] == 5 && s->markk[mm1] >= 1073.000) || (s->markt[mm1] == 25 && s->markk[mm1] < 2103.000)) {
+        a = pow(((double)(markgr[mm1])), n_grain);
+        b = k0_grain * exp(-E_grain / (8.3145 * s->markk[mm1])) * timestep;
+        c = pow((a + b), (1.000 / n_grain));
+        markgr[mm1] = (float)(c);
+      }
+    }
+    if ((growth_model == 1 && (timesum + timestep) >= (gp->impact_time_ma * 1.000e+6 * 365.250 * 24.000 * 3600.000)) ||
+        (growth_model == 2 && (timesum + timestep) >= (gp->start_time_ma * 1.000e+6 * 365.250 * 24.000 * 3600.000))) {
+      if (s->markt[mm1] == 6 || (s->markt[mm1] == 26 && s->markk[mm1] < 2103.000)) {
+        a = pow(((double)(markgr[mm1])), n_oliv);
+        b = k0_oliv * exp(-E_oliv / (8.3145 * s->markk[mm1])) * timestep;
+        c = pow((a + b), (1.000 / n_oliv));
+        markgr[mm1] = (float)(c);
+      }
+      if ((s->markt[mm1] == 25 || s->markt[mm1] == 26) && s->markk[mm1] >= 2103.000) markgr[mm1] = (float)(gp->gr_init);
+    }
+  }
+  return I2E_OK;
+}

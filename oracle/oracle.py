@@ -61,6 +61,8 @@ def lib():
             ("orc_type_counts", [P, S, C.c_int, C.POINTER(C.c_int32)]),
             ("orc_stokes_residuals", [P, S, dp]),
             ("orc_hydration", [P, S, C.c_double, C.c_double, C.c_int, C.POINTER(abi.BodyStats)]),
+            ("orc_visual", [P, S, C.POINTER(abi.VisualSpec), C.c_void_p, C.c_void_p, C.c_void_p]),
+            ("orc_grain", [P, S, C.POINTER(abi.GrainParams), C.c_double, C.c_double, C.c_void_p]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -364,6 +366,26 @@ class Oracle:
                "hydration")
         self._pull()
         return o
+
+
+    def visual(self, spec):
+        """i2jmart's saver for one output (loadjmart.c:619-1197): (val or
+        rock, melt); modifies the markers as saver() does."""
+        n = spec.xresol * spec.yresol
+        val = np.zeros(n, np.float32)
+        rock = np.zeros(n, np.int32)
+        melt = np.zeros(max(int(self.hs.marknum), 1))
+        _check(self.L.orc_visual(C.byref(self.p), self._s(), C.byref(spec), val.ctypes.data, rock.ctypes.data,
+                                 melt.ctypes.data), "visual")
+        return (rock if spec.type == 5 else val), melt[:int(self.hs.marknum)]
+
+
+    def grain(self, gp, markgr):
+        """grain() (grain.c:4-156) with the state's timesum and timestep:
+        markgr (float32, marknum) and the marker types updated in place."""
+        assert markgr.dtype == np.float32 and markgr.flags.c_contiguous
+        _check(self.L.orc_grain(C.byref(self.p), self._s(), C.byref(gp), self.hs.timesum, self.hs.timestep,
+                                markgr.ctypes.data), "grain")
 
 
 # --------------------------------------------------------------------------- adjoint objective
