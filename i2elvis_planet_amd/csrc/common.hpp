@@ -53,6 +53,7 @@ enum KId {
   K_STOKES_APPLY,
   K_PRECOND,
   K_HEATMARK,
+  K_VCOARSE,
   K_COUNT
 };
 

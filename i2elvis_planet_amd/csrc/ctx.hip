@@ -485,6 +485,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->nsmooth_cell = 4;
   o->stokes_guess = I2E_GUESS_EXTRAP;
   o->autotune = 0;
+  o->coarse_coop = 0;
   o->stokes_rtol = 1e-8;
   o->heat_rtol = 1e-10;
   o->grav_rtol = 1e-10;
@@ -630,7 +631,7 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
       o->fine_stencil > I2E_FINE_DIFF32 ||
       (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT) ||
       (o->precision != I2E_PREC_FP64 && o->precision != I2E_PREC_MIXED) || (o->graphs != 0 && o->graphs != 1) ||
-      (o->autotune != 0 && o->autotune != 1)) {
+      (o->autotune != 0 && o->autotune != 1) || (o->coarse_coop != 0 && o->coarse_coop != 1)) {
     set_error("set_solver: option out of range");
     return I2E_ERR_ARG;
   }
@@ -1214,6 +1215,7 @@ extern "C++" int stokes_prepare(i2e_ctx *c) {
     c->v32.nsmooth = c->o.nsmooth;
     c->v32.nsmooth_c = c->o.nsmooth_coarse;
     c->v32.omega = c->o.omega;
+    c->v32.coop = c->o.coarse_coop;
     c->v32.stencil0 = fs != I2E_FINE_ROWS;
     c->v32.S0d = fs == I2E_FINE_STENCIL64 ? c->v64.L[0].S.p : nullptr;
   }
@@ -1819,7 +1821,7 @@ int i2e_stokes_residuals(i2e_ctx *c, i2e_resid *out) {
 int i2e_kernel_timer(i2e_ctx *c, const char *kernel, int max_launches, int every) {
   if (!c || max_launches < 0 || every < 1) return I2E_ERR_ARG;
   static const char *names[K_COUNT] = {"advect",   "p2g",          "gsdot",   "gsaxpy",  "vsmooth0",
-                                       "vsmoothc", "stokes_apply", "precond", "heatmark"};
+                                       "vsmoothc", "stokes_apply", "precond", "heatmark", "vcoarse"};
   int id = -1;
   if (kernel && kernel[0]) {
     for (int k = 0; k < K_COUNT; k++)

@@ -3,12 +3,14 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <cmath>
 #include <type_traits>
 
 #include "mg.hpp"
 
 #include <map>
+#include <hip/hip_cooperative_groups.h>
 #include <mutex>
 #include <tuple>
 
@@ -1231,6 +1233,273 @@ __global__ __launch_bounds__(VT_B) void k_vtail(VTail<T> A) {
   A.oy[m] = by;
 }
 
+// ---------------------------------------------------------------- coarse V-cycle, one launch
+// The FP32 levels from lc down to the two-level tail in one cooperative
+// launch: every workgroup owns one 32x16 tile of one level (or, the last
+// one, the tail), loads that tile's two stencil rows per point into
+// registers once, and keeps them for the pre-smoothing pass, the residual
+// and the post-smoothing pass; the phases (pre-smoothing and residual per
+// level downwards, the tail, post-smoothing upwards) are separated by
+// grid-wide barriers, which every workgroup reaches.  Each phase does the
+// arithmetic of k_tsmooth (the restricting variant for every level after
+// the first), k_tresid and k_vtail on the same tiles, so the V-cycle equals
+// the launch sequence's bit for bit; it replaces 4 launches per level and
+// the tail's with one, and the stencils are read once instead of three
+// times.  The arguments (several KB) live in device memory.
+constexpr int VC_MAXL = 4, VC_B = 512;
+template <class T>
+struct VCoarse {
+  int nl;                    // tiled levels before the tail
+  int b0[VC_MAXL + 1];       // first block of each level; b0[nl]: the tail's block
+  int ntj[VC_MAXL];          // tiles along j of each level
+  // pre-smoothing (from zero; restricting its right-hand side after the
+  // first level), residual of its output, post-smoothing
+  TileArgs<T> pre[VC_MAXL], res[VC_MAXL], post[VC_MAXL];
+  const T *S[VC_MAXL];
+  T *rx[VC_MAXL], *ry[VC_MAXL];  // residual of each level's pre-smoothed iterate
+  VTail<T> tail;                 // its fx, fy: restricted here from rx[nl - 1]
+  T *tfx, *tfy;
+  int Yf;                        // Y of level nl - 1
+  MapView xn, xm, yn, ym;        // level nl - 1 -> the tail level
+};
+
+template <class T>
+__global__ __launch_bounds__(VC_B) void k_vcoarse(const VCoarse<T> *__restrict__ Vp) {
+  namespace cg = cooperative_groups;
+  const VCoarse<T> &V = *Vp;
+  cg::grid_group grid = cg::this_grid();
+  constexpr int SX = 32, SY = 16, TWX = SX + 2 * TH, TWY = SY + 2 * TH, NSL = 50;
+  static_assert(SX * SY == VC_B, "one thread per tile point");
+  __shared__ T lx[TWX * TWY], ly[TWX * TWY];
+  __shared__ T trx[VT_S * VT_S], try_[VT_S * VT_S], tfc[2 * VT_C], tuc[2 * VT_C];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  int lev = V.nl;  // this block's level (nl: the tail)
+  for (int k = 0; k < V.nl; k++)
+    if (b >= V.b0[k] && b < V.b0[k + 1]) lev = k;
+  const bool tiled = lev < V.nl;
+  // the tile of this block, its point, and the point's stencil rows
+  int i0 = 0, j0 = 0, i = 0, j = 0, li = 0, m = 0, kx = -1, ky = -1;
+  bool ok = false;
+  T sx[NSL], sy[NSL], fx = 0, fy = 0, dx = 1, dy = 1;
+  if (tiled) {
+    const TileArgs<T> &A = V.pre[lev];
+    const int t = b - V.b0[lev];
+    i0 = (t / V.ntj[lev]) * SY - TH;
+    j0 = (t % V.ntj[lev]) * SX - TH;
+    i = i0 + TH + tid / SX;
+    j = j0 + TH + tid % SX;
+    li = (i - i0) * TWX + (j - j0);
+    ok = i < A.X && j < A.Y;
+    m = ok ? i * A.Y + j : 0;
+    kx = ok ? A.kx[m] : -1;
+    ky = ok ? A.ky[m] : -1;
+    const size_t R = 2 * (size_t)A.X * A.Y, N = (size_t)A.X * A.Y;
+#pragma unroll
+    for (int q = 0; q < NSL; q++) {
+      sx[q] = ok ? V.S[lev][(size_t)q * R + m] : T(0);
+      sy[q] = ok ? V.S[lev][(size_t)q * R + N + m] : T(0);
+    }
+    dx = ok ? sx[sslot(2, 0, 0, 0)] : T(1);
+    dy = ok ? sy[sslot(2, 1, 0, 0)] : T(1);
+  }
+  const int la = i - i0, lb = j - j0;
+  auto apply = [&](const T *sv) {
+    T acc = 0;
+#pragma unroll
+    for (int cc = 0; cc < 2; cc++) {
+      const T *u = cc ? ly : lx;
+#pragma unroll
+      for (int di = -2; di <= 2; di++)
+#pragma unroll
+        for (int dj = -2; dj <= 2; dj++) acc = fmadd(sv[sslot(2, cc, di, dj)], u[(la + di) * TWX + lb + dj], acc);
+    }
+    return acc;
+  };
+  // one smoothing pass of k_tsmooth<T, T, 2, false, RS, 32, 16> on this tile
+  auto pass = [&](const TileArgs<T> &A, bool RS) {
+    tile_load<T, SX, SY>(A, lx, ly, i0, j0);
+    if (RS) {
+      fx = kx == 0 && j <= A.Y - 2 ? restrict_rw(A.rfx, A.Yf, A.fxn, A.fym, i, j) : T(0);
+      fy = ky == 0 && i <= A.X - 2 ? restrict_rw(A.rfy, A.Yf, A.fxm, A.fyn, i, j) : T(0);
+      if (ok) {
+        A.fxo[m] = fx;
+        A.fyo[m] = fy;
+      }
+    } else {
+      fx = ok ? A.fx[m] : T(0);
+      fy = ok ? A.fy[m] : T(0);
+    }
+    const T rdx = T(A.omega) / dx, rdy = T(A.omega) / dy;
+    auto half = [&](auto comp, int c0) {
+      constexpr int C = decltype(comp)::value;
+      T *u = C == 1 ? lx : ly;
+      const T *sv = C == 1 ? sx : sy;
+      const T f = C == 1 ? fx : fy;
+      const T rd = C == 1 ? rdx : rdy;
+      for (int c = 0; c < 2; c++) {
+        const bool act = ((i + j) & 1) == (c0 ^ c);
+        T nv = 0;
+        if (act) nv = T(u[li] + (f - apply(sv)) * rd);
+        __syncthreads();
+        if (act && ok) u[li] = nv;
+        __syncthreads();
+      }
+    };
+    __syncthreads();
+    for (int it = 0; it < A.nsmooth; it++) {
+      if (!A.rev) {
+        half(std::integral_constant<int, 1>{}, 0);
+        half(std::integral_constant<int, 2>{}, 0);
+      } else {
+        half(std::integral_constant<int, 2>{}, 1);
+        half(std::integral_constant<int, 1>{}, 1);
+      }
+    }
+    if (ok) {
+      T bx = lx[li], by = ly[li];
+      if (kx > 0) bx = T(bx + (fx - apply(sx)) / dx);
+      if (ky > 0) by = T(by + (fy - apply(sy)) / dy);
+      A.ox[m] = bx;
+      A.oy[m] = by;
+    }
+    __syncthreads();
+  };
+  // k_tresid<T, T, 2> on this tile: r = f - S u of the pre-smoothed iterate
+  auto resid = [&](const TileArgs<T> &Rg, T *rxo, T *ryo) {
+    tile_load<T, SX, SY>(Rg, lx, ly, i0, j0);
+    __syncthreads();
+    if (ok) {
+      rxo[m] = kx == 0 ? T(fx - apply(sx)) : T(0);
+      ryo[m] = ky == 0 ? T(fy - apply(sy)) : T(0);
+    }
+    __syncthreads();
+  };
+  // downwards: pre-smoothing and residual, level by level
+  for (int k = 0; k < V.nl; k++) {
+    if (lev == k) pass(V.pre[k], k > 0);
+    grid.sync();
+    if (lev == k) resid(V.res[k], V.rx[k], V.ry[k]);
+    grid.sync();
+  }
+  // the tail (k_vtail), its right-hand side restricted from the last level
+  if (!tiled) {
+    const VTail<T> &A = V.tail;
+    const int t = tid, X = A.X, Y = A.Y, N = X * Y;
+    const size_t R = 2 * (size_t)N;
+    constexpr int P = VT_P;
+    for (int q = t; q < P * P; q += VC_B) {
+      lx[q] = T(0);
+      ly[q] = T(0);
+    }
+    const bool tok = t < N;
+    const int ti = tok ? t / Y : 0, tj = tok ? t % Y : 0, tm = ti * Y + tj;
+    const int ta = ti + TH, tb = tj + TH, tli = ta * P + tb;
+    const int tkx = tok ? A.kx[tm] : -1, tky = tok ? A.ky[tm] : -1;
+    // k_vrestrict into this level's right-hand side
+    const T tfx = tok ? ((tkx == 0 && tj <= Y - 2) ? restrict_rw(V.rx[V.nl - 1], V.Yf, V.xn, V.ym, ti, tj) : T(0)) : T(0);
+    const T tfy = tok ? ((tky == 0 && ti <= X - 2) ? restrict_rw(V.ry[V.nl - 1], V.Yf, V.xm, V.yn, ti, tj) : T(0)) : T(0);
+    if (tok) {
+      V.tfx[tm] = tfx;
+      V.tfy[tm] = tfy;
+    }
+#pragma unroll
+    for (int q = 0; q < NSL; q++) {
+      sx[q] = tok ? A.S[(size_t)q * R + tm] : T(0);
+      sy[q] = tok ? A.S[(size_t)q * R + (size_t)N + tm] : T(0);
+    }
+    const T tdx = tok ? sx[sslot(2, 0, 0, 0)] : T(1), tdy = tok ? sy[sslot(2, 1, 0, 0)] : T(1);
+    const T rdx = T(A.omega) / tdx, rdy = T(A.omega) / tdy;
+    auto tapply = [&](const T *sv) {
+      T acc = 0;
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++) {
+        const T *u = cc ? ly : lx;
+#pragma unroll
+        for (int di = -2; di <= 2; di++)
+#pragma unroll
+          for (int dj = -2; dj <= 2; dj++) acc = fmadd(sv[sslot(2, cc, di, dj)], u[(ta + di) * P + tb + dj], acc);
+      }
+      return acc;
+    };
+    auto thalf = [&](auto comp, int c0) {
+      constexpr int C = decltype(comp)::value;
+      T *u = C == 1 ? lx : ly;
+      const T *sv = C == 1 ? sx : sy;
+      const T f = C == 1 ? tfx : tfy;
+      const T rd = C == 1 ? rdx : rdy;
+      for (int c = 0; c < 2; c++) {
+        const bool act = ((ti + tj) & 1) == (c0 ^ c);
+        T nv = 0;
+        if (act) nv = T(u[tli] + (f - tapply(sv)) * rd);
+        __syncthreads();
+        if (act && tok) u[tli] = nv;
+        __syncthreads();
+      }
+    };
+    auto resolve = [&](T &bx, T &by) {
+      bx = lx[tli];
+      by = ly[tli];
+      if (tkx > 0) bx = T(bx + (tfx - tapply(sx)) / tdx);
+      if (tky > 0) by = T(by + (tfy - tapply(sy)) / tdy);
+    };
+    __syncthreads();
+    for (int it = 0; it < A.ns; it++) {
+      thalf(std::integral_constant<int, 1>{}, 0);
+      thalf(std::integral_constant<int, 2>{}, 0);
+    }
+    T bx, by;
+    resolve(bx, by);
+    __syncthreads();
+    if (tok) {
+      lx[tli] = bx;
+      ly[tli] = by;
+    }
+    __syncthreads();
+    if (tok) {
+      trx[tm] = tkx == 0 ? T(tfx - tapply(sx)) : T(0);
+      try_[tm] = tky == 0 ? T(tfy - tapply(sy)) : T(0);
+    }
+    __syncthreads();
+    const int Nc = A.Xc * A.Yc, n = 2 * Nc;
+    for (int q = t; q < Nc; q += VC_B) {
+      const int ic = q / A.Yc, jc = q % A.Yc;
+      tfc[q] = (A.ckx[q] == 0 && jc <= A.Yc - 2) ? restrict_rw((const T *)trx, Y, A.xn, A.ym, ic, jc) : T(0);
+      tfc[Nc + q] = (A.cky[q] == 0 && ic <= A.Xc - 2) ? restrict_rw((const T *)try_, Y, A.xm, A.yn, ic, jc) : T(0);
+    }
+    __syncthreads();
+    for (int r = t; r < n; r += VC_B) {
+      double sum = 0;
+#pragma unroll 8
+      for (int c = 0; c < n; c++) sum += A.ainvT[(size_t)c * n + r] * (double)tfc[c];
+      tuc[r] = T(sum);
+    }
+    __syncthreads();
+    if (tok) {
+      T vx = lx[tli], vy = ly[tli];
+      if (tkx != 2 && tj <= Y - 2) vx += T(prolong_at((const T *)tuc, A.Yc, A.xn, A.ym, ti, tj));
+      if (tky != 2 && ti <= X - 2) vy += T(prolong_at((const T *)(tuc + Nc), A.Yc, A.xm, A.yn, ti, tj));
+      lx[tli] = vx;
+      ly[tli] = vy;
+    }
+    __syncthreads();
+    for (int it = 0; it < A.ns; it++) {
+      thalf(std::integral_constant<int, 2>{}, 1);
+      thalf(std::integral_constant<int, 1>{}, 1);
+    }
+    resolve(bx, by);
+    if (tok) {
+      A.ox[tm] = bx;
+      A.oy[tm] = by;
+    }
+  }
+  grid.sync();
+  // upwards: post-smoothing with the coarse correction added on load
+  for (int k = V.nl - 1; k >= 0; k--) {
+    if (lev == k) pass(V.post[k], false);
+    if (k > 0) grid.sync();
+  }
+}
+
 // ---------------------------------------------------------------- VelMG host
 template <class T>
 VLev<T> VelMG<T>::lev(int l) const {
@@ -1264,6 +1533,7 @@ int VelMG<T>::setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &k
                     const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,
                     const std::vector<double> &kky0) {
   shape.clear();
+  vgen++;
   geo = &g;
   const int nl = (int)g.size();
   L.clear();
@@ -1375,6 +1645,7 @@ template <class T>
 int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, const double *gp, double mu_, double dt_,
                       hipStream_t s, hipStream_t side_) {
   ST_OK(join(s));  // a previous inversion still running (refresh without a join)
+  vgen++;
   mu = mu_;
   dt = dt_;
   const int nl = (int)L.size();
@@ -1428,6 +1699,7 @@ int VelMG<T>::refresh(const double *nu, const double *nd, const double *ro, cons
 
 template <class T>
 int VelMG<T>::refresh_from(const VelMG<double> &d, hipStream_t s) {
+  vgen++;
   mu = d.mu;
   dt = d.dt;
   const int nl = (int)L.size();
@@ -1522,8 +1794,10 @@ int VelMG<T>::tune(hipStream_t s, int reps) {
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
+  // levels of the cooperative launch keep its 32x16 tiles
+  const int lcoop = coop_level();
   for (int l = 1; l < nl - 1; l++) {
-    if (tail_at(l)) continue;
+    if (tail_at(l) || (lcoop > 0 && l >= lcoop)) continue;
     VelLevel<T> &v = L[l];
     const int ns = nsmooth_c <= 0 ? nsmooth : nsmooth_c;
     const auto key = std::make_tuple((int)sizeof(T), v.X, v.Y, ns);
@@ -1583,6 +1857,7 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
     return I2E_OK;
   }
   const int ns = (l == 0 || nsmooth_c <= 0) ? nsmooth : nsmooth_c;
+  if (l > 0 && l == coop_level()) return vcoarse(l, s);
   if (tail_at(l)) {
     const VelLevel<T> &c = L[nl - 1];
     const VTail<T> a{v.X,      v.Y,      c.X,        c.Y,        ns,         v.kx.p,     v.ky.p,   c.kx.p,
@@ -1701,6 +1976,159 @@ int VelMG<T>::vcycle(int l, hipStream_t s) {
   return I2E_OK;
 }
 
+
+// ---------------------------------------------------------------- cooperative coarse V-cycle (host)
+template <class T>
+int VelMG<T>::coop_level() const {
+  if (sizeof(T) != 4 || !coop) return -1;
+  const int nl = (int)L.size();
+  int lt = -1;
+  for (int l = 1; l < nl; l++)
+    if (tail_at(l)) lt = l;
+  if (lt < 0) return -1;
+  // blocks the device holds at once for this kernel (one per CU at 512
+  // threads and its registers; queried once)
+  static int resident = -1;
+  if (resident < 0) {
+    int per = 0, dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_vcoarse<float>, VC_B, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      resident = 0;
+    } else {
+      resident = per * cus;
+    }
+  }
+  auto tiles = [&](int l) { return ((L[l].Y + 31) / 32) * ((L[l].X + 15) / 16); };
+  int lc = -1, blocks = 1;
+  for (int l = lt - 1; l >= 1 && lt - l <= VC_MAXL; l--) {
+    if (L[l].W != 2 || tiles(l) > 32) break;
+    if (blocks + tiles(l) > resident) break;
+    blocks += tiles(l);
+    lc = l;
+  }
+  return lc;
+}
+
+template <class T>
+int VelMG<T>::vcoarse(int lc, hipStream_t s) {
+  if constexpr (sizeof(T) != 4) {
+    return I2E_ERR_ARG;
+  } else {
+    const int nl = (int)L.size();
+    int lt = -1;
+    for (int l = 1; l < nl; l++)
+      if (tail_at(l)) lt = l;
+    VCoarse<T> V{};
+    V.nl = lt - lc;
+    const MapView none{nullptr, nullptr, nullptr, nullptr};
+    int b = 0;
+    for (int k = 0; k < V.nl; k++) {
+      const int l = lc + k;
+      VelLevel<T> &v = L[l];
+      const int ns = nsmooth_c <= 0 ? nsmooth : nsmooth_c;
+      V.b0[k] = b;
+      V.ntj[k] = (v.Y + 31) / 32;
+      b += V.ntj[k] * ((v.X + 15) / 16);
+      TileArgs<T> A{v.X, v.Y, v.kx.p, v.ky.p, v.fx.p, v.fy.p, v.ux.p, v.uy.p, v.ux2.p, v.uy2.p, nullptr, nullptr,
+                    0, none, none, none, none, 1, ns, 0, v.dx.p, v.dy.p};
+      A.omega = omega;
+      if (k > 0) {
+        const VelLevel<T> &f = L[l - 1];
+        A.rfx = f.rx.p;
+        A.rfy = f.ry.p;
+        A.fxo = v.fx.p;
+        A.fyo = v.fy.p;
+        A.Yf = f.Y;
+        A.fxn = view(f.mxn);
+        A.fxm = view(f.mxm);
+        A.fyn = view(f.myn);
+        A.fym = view(f.mym);
+      }
+      V.pre[k] = A;
+      TileArgs<T> Rg = A;
+      Rg.ux = v.ux2.p;
+      Rg.uy = v.uy2.p;
+      Rg.zero = 0;
+      V.res[k] = Rg;
+      const VelLevel<T> &c = L[l + 1];
+      TileArgs<T> P = A;
+      P.rfx = P.rfy = nullptr;
+      P.fxo = P.fyo = nullptr;
+      P.zero = 0;
+      P.ux = v.ux2.p;
+      P.uy = v.uy2.p;
+      P.ox = v.ux.p;
+      P.oy = v.uy.p;
+      P.ex = c.ux.p;
+      P.ey = c.uy.p;
+      P.Yc = c.Y;
+      P.xn = view(v.mxn);
+      P.xm = view(v.mxm);
+      P.yn = view(v.myn);
+      P.ym = view(v.mym);
+      P.rev = 1;
+      V.post[k] = P;
+      V.S[k] = v.S.p;
+      V.rx[k] = v.rx.p;
+      V.ry[k] = v.ry.p;
+    }
+    V.b0[V.nl] = b;
+    VelLevel<T> &t = L[lt];
+    const VelLevel<T> &c = L[nl - 1];
+    const VelLevel<T> &f = L[lt - 1];
+    const int nst = nsmooth_c <= 0 ? nsmooth : nsmooth_c;
+    V.tail = VTail<T>{t.X,    t.Y,    c.X,    c.Y,    nst,    t.kx.p,      t.ky.p,      c.kx.p,      c.ky.p,
+                      t.S.p,  t.fx.p, t.fy.p, view(t.mxn), view(t.mxm), view(t.myn), view(t.mym), ainvT.p,
+                      t.ux.p, t.uy.p, omega};
+    V.tfx = t.fx.p;
+    V.tfy = t.fy.p;
+    V.Yf = f.Y;
+    V.xn = view(f.mxn);
+    V.xm = view(f.mxm);
+    V.yn = view(f.myn);
+    V.ym = view(f.mym);
+    // the arguments change only with the hierarchy (set-up and refresh may
+    // reallocate level buffers) and the smoothing options: copied to the
+    // device when one of those changed since the last launch, through a
+    // pinned host buffer whose previous copy has completed
+    const int nsx = nsmooth_c <= 0 ? nsmooth : nsmooth_c;
+    if (!vch || vk_gen != vgen || vk_l != lc || vk_ns != nsx || vk_om != omega) {
+      if (!vch) {
+        ST_OK(vcd.alloc(sizeof(V)));
+        HIP_OK(hipHostMalloc((void **)&vch, sizeof(V), hipHostMallocDefault));
+        HIP_OK(hipEventCreateWithFlags(&vce, hipEventDisableTiming));
+      } else {
+        HIP_OK(hipEventSynchronize(vce));
+      }
+      memcpy(vch, &V, sizeof(V));
+      HIP_OK(hipMemcpyAsync(vcd.p, vch, sizeof(V), hipMemcpyHostToDevice, s));
+      HIP_OK(hipEventRecord(vce, s));
+      vk_gen = vgen;
+      vk_l = lc;
+      vk_ns = nsx;
+      vk_om = omega;
+    }
+    const VCoarse<T> *Vd = (const VCoarse<T> *)vcd.p;
+    void *args[] = {&Vd};
+    if (kt) kt->begin(K_VCOARSE, s);
+    HIP_OK(hipLaunchCooperativeKernel((const void *)k_vcoarse<T>, dim3(b + 1), dim3(VC_B), args, 0, s));
+    if (kt) {
+      // algorithmic bytes: each level's pre- and post-smoothing sweeps (10
+      // sizeof(T) per point and sweep) and residual; representation: the
+      // stencils once, the iterates, right-hand sides and residuals
+      double alg = 0, rep = 0;
+      for (int l = lc; l <= lt; l++) {
+        alg += 2.0 * 10.0 * sizeof(T) * L[l].N * nst;
+        rep += (100.0 * sizeof(T) + 10.0 * sizeof(T)) * L[l].N;
+      }
+      kt->end(K_VCOARSE, s, alg, rep);
+    }
+    return I2E_OK;
+  }
+}
+
 template <class T>
 void VelMG<T>::release() {
   for (auto &v : L) {
@@ -1715,6 +2143,12 @@ void VelMG<T>::release() {
   dense.release();
   ainv.release();
   ainvT.release();
+  vcd.release();
+  if (vch) (void)hipHostFree(vch);
+  if (vce) (void)hipEventDestroy(vce);
+  vch = nullptr;
+  vce = nullptr;
+  vk_gen = -1;
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   ev_fork = ev_join = nullptr;

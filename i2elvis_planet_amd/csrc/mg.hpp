@@ -127,6 +127,7 @@ struct VelMG {
   int stencil0 = 0;         // finest level smoothed with its probed 3x3 stencil (else matrix-free rows)
   const double *S0d = nullptr;  // that stencil in FP64 (from the FP64 hierarchy) for FP32 iterates
   int diff0 = 0;            // FP32 hierarchy: finest stencil stored in difference form (refresh_from)
+  int coop = 0;             // FP32: the levels from coop_level() down in one cooperative launch
   KTimer *kt = nullptr;     // optional per-launch timing of the smoothers
   int setup(const std::vector<GeoLevel> &g, const std::vector<int8_t> &kx0, const std::vector<int8_t> &ky0,
             const std::vector<int> &nbx0, const std::vector<int> &nby0, const std::vector<double> &kkx0,
@@ -142,6 +143,17 @@ struct VelMG {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int join(hipStream_t s);
   bool tail_at(int l) const;
+  // first level of the cooperative coarse V-cycle (-1: none): FP32, every
+  // level from it to the tail a 5x5 Galerkin level of at most 32 default
+  // tiles, at most VC_MAXL levels, all blocks co-resident
+  int coop_level() const;
+  int vcoarse(int l, hipStream_t s);
+  DBuf<char> vcd;           // its arguments on the device, their pinned host copy
+  char *vch = nullptr;      // and the event of its last upload
+  hipEvent_t vce = nullptr;
+  long vgen = 0, vk_gen = -1;  // hierarchy generation (set-up, refresh), and that of the upload
+  int vk_l = -1, vk_ns = -1;
+  double vk_om = 0;
   // the same hierarchy rounded from an FP64 one on the same geometry: the
   // Galerkin products and the coarsest inverse are formed in FP64 (products
   // of rounded fine-level entries lose the near-cancellation that carries

@@ -215,7 +215,16 @@ typedef struct i2e_solver_opts {
                               tiles change the block Gauss-Seidel partition,
                               so iteration counts can differ slightly);
                               0 (default): the fixed shapes                 */
-  int32_t pad_opts;
+  int32_t coarse_coop;     /* mixed precision: 1: the FP32 velocity levels of
+                              at most 32 tiles of 32x16 points, down to the
+                              two-level tail, in one cooperative launch per
+                              V-cycle (the same arithmetic, bit for bit, as
+                              one launch per pass); 0 (default): one launch
+                              per pass.  On MI355X a cooperative launch
+                              costs ~20 us before its first instruction and
+                              each grid-wide barrier ~5 us, against ~4 us
+                              per ordinary launch: 501^2 measured 63.1 ms
+                              per step with it, 55.8 without              */
 } i2e_solver_opts;
 
 typedef struct i2e_stats {
