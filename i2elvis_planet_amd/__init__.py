@@ -26,7 +26,7 @@ _EXPORTS = [
     "i2e_stokes_apply_dev", "i2e_gravity_apply_dev", "i2e_heat_apply_dev", "i2e_device_ptr", "i2e_time_kernels",
     "i2e_stokes_rhs_dev", "i2e_stokes_precond_dev", "i2e_vcycle_dev",
     "i2e_dev_alloc", "i2e_dev_free", "i2e_copy_to_dev", "i2e_copy_to_host",
-    "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_step_batch",
+    "i2e_kernel_timer", "i2e_kernel_timer_read", "i2e_stokes_residuals", "i2e_band_info", "i2e_step_batch",
     "i2e_dev_copy", "i2e_prn_write", "i2e_prn_read_header", "i2e_prn_read", "i2e_set_params", "i2e_record", "i2e_replay", "i2e_peak_t", "i2e_adjoint_peak_t", "i2e_adjoint_peak_t_coupled", "i2e_adjoint_marker_positions", "i2e_visual", "i2e_visual_write", "i2e_markersave", "i2e_grain", "i2e_heat_apply_t_dev", "i2e_sink_rate", "i2e_adjoint_sink_rate", "i2e_stokes_apply_t_dev",
     "i2e_type_counts", "i2e_maxvelstep_noalloc", "i2e_kernel_timer_repr", "i2e_set_marker_sort",
     "i2e_sort_markers", "i2e_init_read", "i2e_init_generate", "i2e_mode_read", "i2e_tuned_shapes", "i2e_hydration",
@@ -100,6 +100,7 @@ def load_library(path=LIB_PATH):
         "i2e_step_batch": ([C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "i2e_kernel_timer_read": ([vp, C.POINTER(C.c_int64), dp, dp], C.c_int),
         "i2e_kernel_timer_repr": ([vp, dp], C.c_int),
+        "i2e_band_info": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp], C.c_int),
         "i2e_set_marker_sort": ([vp, C.c_int], C.c_int),
         "i2e_sort_markers": ([vp], C.c_int),
         "i2e_tuned_shapes": ([vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -295,6 +296,12 @@ class Context:
 
     def vcycle(self, f_dev, u_dev):
         _chk(_lib.i2e_vcycle_dev(self._h, C.c_void_p(f_dev), C.c_void_p(u_dev)), "vcycle")
+
+    def band_info(self):
+        """(kl, ku, bytes) of the last banded Stokes factorisation (stokes_direct = 1)."""
+        kl, ku, b = C.c_int32(), C.c_int32(), C.c_double()
+        _chk(_lib.i2e_band_info(self._h, C.byref(kl), C.byref(ku), C.byref(b)), "band_info")
+        return kl.value, ku.value, b.value
 
     def stokes_residuals(self):
         """viterate's residual diagnostics (movemart.c:1049-1176) as a dict."""

@@ -90,7 +90,7 @@ class SolverOpts(C.Structure):
         ("schur_scale", C.c_double), ("omega", C.c_double),
         ("coarse_max", C.c_int32), ("schur", C.c_int32),
         ("graphs", C.c_int32), ("nsmooth_coarse", C.c_int32), ("orth", C.c_int32), ("fine_stencil", C.c_int32),
-        ("stokes_guess", C.c_int32), ("nsmooth_cell", C.c_int32), ("autotune", C.c_int32), ("coarse_coop", C.c_int32),
+        ("stokes_guess", C.c_int32), ("nsmooth_cell", C.c_int32), ("autotune", C.c_int32), ("coarse_coop", C.c_int32), ("stokes_direct", C.c_int32), ("pad_opts", C.c_int32),
     ]
 
 

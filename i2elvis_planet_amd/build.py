@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libi2e.so")
 OBJ = os.path.join(HERE, "_obj")
-SOURCES = ["markers.hip", "ops.hip", "mg.hip", "krylov.hip", "ctx.hip", "adjoint.hip", "flow_adjoint.hip", "noalloc.hip", "sort.hip", "postproc.hip", "prn.cpp", "init_t3c.cpp"]
+SOURCES = ["markers.hip", "ops.hip", "mg.hip", "krylov.hip", "ctx.hip", "adjoint.hip", "flow_adjoint.hip", "noalloc.hip", "sort.hip", "postproc.hip", "band.hip", "prn.cpp", "init_t3c.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]

@@ -486,6 +486,7 @@ int i2e_default_solver(i2e_solver_opts *o) {
   o->stokes_guess = I2E_GUESS_EXTRAP;
   o->autotune = 0;
   o->coarse_coop = 0;
+  o->stokes_direct = 0;
   o->stokes_rtol = 1e-8;
   o->heat_rtol = 1e-10;
   o->grav_rtol = 1e-10;
@@ -574,6 +575,8 @@ int i2e_destroy(i2e_ctx *c) {
   c->kt.ev.clear();
   adj_release(c->adj);
   c->adj = nullptr;
+  band_release(c->band);
+  c->band = nullptr;
   c->v32.release();
   c->v64.release();
   c->l32.release();
@@ -631,7 +634,8 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
       o->fine_stencil > I2E_FINE_DIFF32 ||
       (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT) ||
       (o->precision != I2E_PREC_FP64 && o->precision != I2E_PREC_MIXED) || (o->graphs != 0 && o->graphs != 1) ||
-      (o->autotune != 0 && o->autotune != 1) || (o->coarse_coop != 0 && o->coarse_coop != 1)) {
+      (o->autotune != 0 && o->autotune != 1) || (o->coarse_coop != 0 && o->coarse_coop != 1) ||
+      (o->stokes_direct != 0 && o->stokes_direct != 1)) {
     set_error("set_solver: option out of range");
     return I2E_ERR_ARG;
   }
@@ -1314,15 +1318,21 @@ static int stokes_solve(i2e_ctx *c) {
   // each projection pass writes the velocity components of the pending
   // vector into the finest velocity level's right-hand side, saving the
   // preconditioner's split pass
-  c->fs.sx = mixed(c) ? (void *)c->v32.L[0].fx.p : (void *)c->v64.L[0].fx.p;
-  c->fs.sy = mixed(c) ? (void *)c->v32.L[0].fy.p : (void *)c->v64.L[0].fy.p;
+  const bool direct = c->o.stokes_direct != 0;
+  if (direct) ST_OK(band_factor(c));
+  c->fs.sx = direct ? nullptr : mixed(c) ? (void *)c->v32.L[0].fx.p : (void *)c->v64.L[0].fx.p;
+  c->fs.sy = direct ? nullptr : mixed(c) ? (void *)c->v32.L[0].fy.p : (void *)c->v64.L[0].fy.p;
   c->fs.split_f32 = mixed(c);
   c->fs.split_of = nullptr;
   auto Mf = [&](const double *r, double *z) -> int {
     c->kt.begin(K_PRECOND, c->s);
-    const bool presplit = r == c->fs.split_of;
-    c->fs.split_of = nullptr;
-    ST_OK(stokes_precond(c, r, z, presplit));
+    if (direct) {
+      ST_OK(band_apply(c, r, z));
+    } else {
+      const bool presplit = r == c->fs.split_of;
+      c->fs.split_of = nullptr;
+      ST_OK(stokes_precond(c, r, z, presplit));
+    }
     c->kt.end(K_PRECOND, c->s, 0.0, 0.0);
     return I2E_OK;
   };

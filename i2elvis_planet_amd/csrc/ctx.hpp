@@ -6,6 +6,8 @@
 namespace i2e {
 struct AdjHist;               // recorded trajectory and thermal history (adjoint.hip)
 void adj_release(AdjHist *h);
+struct BandLU;                // banded LU of the Stokes rows (band.hip)
+void band_release(BandLU *b);
 __global__ void k_copy(int n, const double *a, double *b);
 __global__ void k_heat_fields(int N, const double *ro, const double *cp, double dt, const int *bondm, double *c,
                               int8_t *mask, double *dm);
@@ -80,6 +82,7 @@ struct i2e_ctx {
   int stokes_valid = 0;
   double stokes_dt = 0;
   DBuf<double> fadj;            // flow adjoint scratch (flow_adjoint.hip)
+  BandLU *band = nullptr;       // stokes_direct: the factored Stokes rows (band.hip)
   DBuf<int> fflag;
   DBuf<int8_t> fmask;
 
@@ -95,3 +98,8 @@ int stokes_precond(i2e_ctx *c, const double *r, double *z, bool presplit = false
 int stokes_prepare(i2e_ctx *c);
 // G x = b for the gravity rows, from x = 0 (b overwritten; ctx.hip)
 int gravity_solve_rhs(i2e_ctx *c, double *b, double *x, int *its, double *rr);
+// stokes_direct: assemble, equilibrate and factor the Stokes rows of the
+// current fields; z = A^-1 r by the factors (band.hip)
+int band_factor(i2e_ctx *c);
+int band_apply(i2e_ctx *c, const double *r, double *z);
+int band_dims(const i2e_ctx *c, int *kl, int *ku, double *bytes);

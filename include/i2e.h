@@ -225,6 +225,16 @@ typedef struct i2e_solver_opts {
                               each grid-wide barrier ~5 us, against ~4 us
                               per ordinary launch: 501^2 measured 63.1 ms
                               per step with it, 55.8 without              */
+  int32_t stokes_direct;   /* 1: the Stokes FGMRES is preconditioned by an
+                              LU factorisation of the equilibrated rows
+                              (band storage, partial pivoting inside the
+                              band, FP64 whatever the precision), refactored
+                              every solve: for viscosity contrasts the
+                              multigrid cannot precondition (melt against
+                              the clip).  Memory ~(3 kl + ku) 8 B per
+                              unknown, kl ~ 3 ynumy (27 GB at 501^2);
+                              0 (default): multigrid                      */
+  int32_t pad_opts;
 } i2e_solver_opts;
 
 typedef struct i2e_stats {
@@ -358,6 +368,11 @@ typedef struct i2e_resid {
   double vxmin, vxmax, vymin, vymax, prmin, prmax, numin, numax;
 } i2e_resid;
 int i2e_stokes_residuals(i2e_ctx *c, i2e_resid *out);
+/* Bandwidths of the last banded factorisation (stokes_direct = 1): kl
+ * subdiagonals and ku superdiagonals of the assembled Stokes rows in the
+ * reference's unknown order (gausmat4's system, gausmart.c:174-404), and
+ * the device bytes of its band storage.                                     */
+int i2e_band_info(i2e_ctx *c, int32_t *kl, int32_t *ku, double *bytes);
 
 /* Per-launch HIP-event timing of one kernel on the context's stream (no
  * counterpart in the reference), for roofline figures taken inside a normal

@@ -1,0 +1,161 @@
+"""GPU tests of the banded LU Stokes preconditioner (stokes_direct = 1,
+csrc/band.hip) through the C-ABI.
+
+Pins: on cold states (51^2, 201^2) FGMRES with the factors converges in a
+few iterations to 1e-12 and matches the oracle's direct solve; on the
+molten 501^2 geometry at 101^2 (melt at 1e2 Pa s against the 1e23 Pa s
+clip, a 1e21 contrast, where the multigrid preconditioner diverges) the
+solve meets the metric's ||b - Ax|| / ||b|| <= 1e-8 and a scaled residual
+of the oracle's assembled rows (built from the GPU's own fields) near
+machine precision, and matches the oracle's rows solved by scipy's LU after
+Ruiz equilibration: the pressure to 1e-8, the velocities to 1e-2.  The
+velocity bound is what FP64 determines there: two backward-stable direct
+solves of the same equilibrated system (scipy's LU with COLAMD ordering and
+with the natural, banded ordering) both reach a scaled residual of 1e-14
+and differ by 2e-3 in the velocities (tests/test_hot_regime.py pins it on
+the CPU).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from i2elvis_planet_amd import Context, configs, default_solver
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def direct_opts(rtol):
+    o = default_solver()
+    o.precision = 0
+    o.stokes_direct = 1
+    o.stokes_rtol = rtol
+    o.grav_rtol = 1e-13
+    o.maxit = 20
+    o.restart = 20
+    return o
+
+
+def equilibrated(A, sweeps=10):
+    import scipy.sparse as sp
+    B = A.tocsr().copy()
+    r = np.ones(A.shape[0])
+    c = np.ones(A.shape[1])
+    for _ in range(sweeps):
+        rs = np.sqrt(abs(B).max(axis=1).toarray().ravel())
+        cs = np.sqrt(abs(B).max(axis=0).toarray().ravel())
+        rs[rs == 0] = 1
+        cs[cs == 0] = 1
+        B = sp.diags(1 / rs) @ B @ sp.diags(1 / cs)
+        r /= rs
+        c /= cs
+    return B, r, c
+
+
+def interleave(hs):
+    x = np.zeros(3 * hs.N)
+    x[0::3], x[1::3], x[2::3] = hs.pr, hs.vx, hs.vy
+    return x
+
+
+# the 201^2 rows stagnate near 5e-12 in FP64 whatever the preconditioner
+@pytest.mark.parametrize("name,rtol", [("c1", 1e-12), ("c2", 3e-11)])
+def test_direct_cold_matches_oracle(orc, name, rtol):
+    say("case direct cold", name)
+    p, hs, _ = configs.make(name)
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    ctx = Context(p, hs.copy(), solver=direct_opts(rtol))
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.viterate()
+    st = ctx.stats()
+    kl, ku, nbytes = ctx.band_info()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o.viterate()
+    say("%s: kl %d ku %d band %.2f GB, FGMRES iterations %d relres %.2e" % (name, kl, ku, nbytes / 1e9,
+                                                                          st.stokes_iters, st.stokes_relres))
+    assert kl >= 3 * hs.ynumy and ku >= 3 * hs.ynumy
+    assert st.stokes_relres <= rtol and st.stokes_iters <= 3
+    # the GPU's solution in the oracle's rows assembled from the GPU's fields
+    got.timestep = hs.timestep
+    M = orc.Oracle(p, got).stokes_matrix()
+    res = np.linalg.norm(M.b - M.scipy() @ interleave(got)) / np.linalg.norm(M.b)
+    say("residual in the oracle's rows %.2e" % res)
+    assert res <= 3 * rtol
+    # against the oracle's own step (its gravity potential from its solve)
+    for f in ("vx", "vy", "pr"):
+        e = rel(getattr(got, f), getattr(hs, f))
+        say(f, e)
+        assert e < 1e-8, f
+
+
+RTOL_MOLTEN = 3e-11
+
+
+def test_direct_molten_first_solve(orc):
+    import scipy.sparse.linalg as spla
+    say("case direct molten 101^2")
+    p, hs, cfg = configs.make_hot("c3", n=101, size=120e3, radius=30e3, nblob=16)
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    assert hs.nu.min() == pytest.approx(1e2) and hs.nu.max() == pytest.approx(1e23)
+    hs.timestep = p.maxtmstep
+    ctx = Context(p, hs.copy(), solver=direct_opts(RTOL_MOLTEN))
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.viterate()
+    st = ctx.stats()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    say("FGMRES iterations %d relres %.2e" % (st.stokes_iters, st.stokes_relres))
+    assert st.stokes_relres <= RTOL_MOLTEN
+    # the rows the GPU solved (its own gravity potential), assembled by the oracle
+    got.timestep = hs.timestep
+    M = orc.Oracle(p, got).stokes_matrix()
+    A = M.scipy().tocsr()
+    x = interleave(got)
+    B, r, c = equilibrated(A)
+    res = M.b - A @ x
+    unscaled = np.linalg.norm(res) / np.linalg.norm(M.b)
+    scaled = np.linalg.norm(r * res) / np.linalg.norm(r * M.b)
+    def scipy_solve(**kw):
+        lu = spla.splu(B.tocsc(), **kw)
+        y = lu.solve(r * M.b)
+        for _ in range(2):
+            y = y + lu.solve(r * M.b - B @ y)
+        return c * y
+
+    xe = scipy_solve()
+    ev = rel(np.r_[x[1::3], x[2::3]], np.r_[xe[1::3], xe[2::3]])
+    ep = rel(x[0::3], xe[0::3])
+    # the band LU's own pivoting order: natural columns, partial pivoting
+    xn = scipy_solve(permc_spec="NATURAL", diag_pivot_thresh=1.0)
+    evn = rel(np.r_[x[1::3], x[2::3]], np.r_[xn[1::3], xn[2::3]])
+    evs = rel(np.r_[xn[1::3], xn[2::3]], np.r_[xe[1::3], xe[2::3]])
+    say("oracle rows: unscaled %.2e scaled %.2e; vs scipy (COLAMD): velocities %.2e P %.2e; vs scipy (natural): "
+        "velocities %.2e; the two scipy orderings differ by %.2e" % (unscaled, scaled, ev, ep, evn, evs))
+    assert unscaled <= 1e-8
+    assert scaled <= 1e-12
+    assert ep <= 1e-8
+    assert ev <= 1e-2
+
+
+def test_direct_refuses_option_out_of_range():
+    from i2elvis_planet_amd import I2EError
+    p, hs, _ = configs.make("c1")
+    ctx = Context(p, hs, solver=direct_opts(1e-10))
+    o = direct_opts(1e-10)
+    o.stokes_direct = 2
+    with pytest.raises(I2EError):
+        ctx.set_solver(o)
+    ctx.close()
