@@ -35,7 +35,7 @@ namespace i2e {
 constexpr int BNE = 40;    // assembled entries per row at most
 constexpr int BNB = 16;    // panel width
 constexpr int BPT = 1024;  // panel and solve workgroup
-constexpr int BROWS = 2;   // panel rows per thread at most: kl + BNB <= BROWS * BPT
+constexpr int BPMAX = 2 * 768;  // panel rows at most: kl + BNB <= BPMAX
 constexpr int BW = 4096;   // LDS window of the solves (doubles): kl + BNB, KV + BNB <= BW
 constexpr int BSWEEPS = 10;
 
@@ -43,9 +43,9 @@ struct BandLU {
   int64_t n = 0;  // unknowns (3 N)
   int kl = 0, ku = 0, KV = 0;
   int64_t LD = 0;
-  DBuf<double> ab, rs, cs, rmx, w, eval;
+  DBuf<double> ab, rs, cs, rmx, w, eval, dinv;
   DBuf<unsigned long long> cmx;
-  DBuf<int> ipiv, ecol, ecnt, dims;
+  DBuf<int> ipiv, ecol, ecnt, dims, mv;
 };
 
 void band_release(BandLU *b) {
@@ -58,6 +58,8 @@ void band_release(BandLU *b) {
   b->eval.release();
   b->cmx.release();
   b->ipiv.release();
+  b->dinv.release();
+  b->mv.release();
   b->ecol.release();
   b->ecnt.release();
   b->dims.release();
@@ -217,172 +219,256 @@ __global__ void k_band_scatter(int64_t n, const int *ecnt, const int *ecol, cons
 }
 
 // ---------------------------------------------------------------- factorisation
+// Pivot search key of a candidate row: |v|'s bits with the low 12 mantissa
+// bits replaced by 4095 - row, so that the largest key is the largest
+// magnitude (to 2^-40 relative) and, among those, the first row; one
+// 64-bit max then finds both.
+__device__ __forceinline__ unsigned long long piv_key(double v, int row) {
+  return ((unsigned long long)__double_as_longlong(fabs(v)) & ~0xFFFull) | (unsigned long long)(0xFFF - row);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long dpp_max_u64(unsigned long long k) {
+  const unsigned long long o = (unsigned long long)__double_as_longlong(dpp_get<CTRL, ROW_MASK>(__longlong_as_double((long long)k)));
+  return o > k ? o : k;
+}
+// maximum over the wavefront's 64 lanes (all active), to every lane
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
+  k = dpp_max_u64<0xB1, 0xF>(k);
+  k = dpp_max_u64<0x4E, 0xF>(k);
+  k = dpp_max_u64<0x141, 0xF>(k);
+  k = dpp_max_u64<0x140, 0xF>(k);
+  k = dpp_max_u64<0x142, 0xA>(k);
+  k = dpp_max_u64<0x143, 0xC>(k);
+  const int lo = __builtin_amdgcn_readlane((int)k, 63), hi = __builtin_amdgcn_readlane((int)(k >> 32), 63);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
 // Panel columns j0 .. j0+jb-1, rows j0 .. j0+jb+kl-1 (below that the panel
-// columns are zero): thread t holds panel rows t and t + BPT in registers.
-// Partial pivoting picks the largest magnitude in the column (the first
-// such row on ties), rows are interchanged across the whole panel, the
-// multipliers divide by the pivot.  Then each thread takes trailing columns
-// q in [j0+jb, j0+jb+KV): the panel's interchanges in order (rows c and
-// ipiv[c], where column q still lies in row c's band), then the unit-lower
-// triangle on the panel rows.
-template <int ROWS>
-__global__ __launch_bounds__(BPT) void k_band_panel(double *ab, int64_t LD, int KV, int kl, int64_t n, int64_t j0,
-                                                    int *ipiv, int *info) {
-  __shared__ double rowP[BNB], rowK[BNB];
-  __shared__ double wv[BPT / 64];
-  __shared__ int wi[BPT / 64], sp;
+// columns are zero).  Thread t holds panel rows t + s NT (s < ROWS) in
+// registers and keeps them: a pivot choice only renames rows (each row
+// keeps its position in a register; the chosen row and the row at the
+// pivot's position exchange positions, as the interchanges of partial
+// pivoting would move them).  Each column is one
+// pass of a rolled loop, so the code stays in the instruction cache: the
+// column in hand is a[0] of every row, the rows shift left after each
+// column, and the value a row leaves for the column (its multiplier, or its
+// U entry once it has been chosen as pivot) shifts into out[BNB-1].  The
+// pivot is the largest magnitude among rows not yet chosen (piv_key; per
+// wavefront by DPP, then across wavefronts through LDS); the chosen row's
+// values pass through LDS to the rank-1 update of the others.  At the end
+// every row is stored at its final position.
+// Besides the factors it leaves, per panel, the pivots' reciprocals
+// (dinv, for the backward solve) and the row moves its interchanges make,
+// for the forward solve: mv[64 panel + 2 s] = position, mv[.. + 1] = the
+// position whose value lands there (relative to j0; -1: unused), over the
+// panel rows and the rows below named by an interchange.
+template <int ROWS, int NT>
+__global__ __launch_bounds__(NT) void k_band_panel(double *ab, int64_t LD, int KV, int kl, int64_t n, int64_t j0,
+                                                   int *ipiv, int *info, double *dinv, int *mv) {
+  static_assert(BPMAX < 4095, "pivot keys hold 12 bits of row");
+  static_assert(ROWS <= 2, "two rows per thread at most");
+  __shared__ double rowP[BNB + 1];  // the pivot row and, last, 1 / pivot
+  __shared__ unsigned long long wk[NT / 64];
+  __shared__ int ppos, nmv, mpos[2 * BNB], msrc[2 * BNB];
+  // the launch sizes the block to the panel's rows (whole wavefronts)
+  const int nt = blockDim.x, NW = nt / 64;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int jb = (int)(n - j0 < BNB ? n - j0 : BNB);
   const int PR = (int)((n < j0 + jb + kl ? n : j0 + jb + kl) - j0);
-  double a[ROWS][BNB];
+  // one array per held row and value (a 2-D array is not kept in registers)
+  double a0[BNB], a1[BNB], o0[BNB], o1[BNB];
+  int d0 = 0, d1 = 0;             // chosen as pivot already
+  int p0 = t, p1 = t + nt;        // current positions
+  auto rows = [&](auto f) {
+    f(a0, o0, d0, t);
+    if constexpr (ROWS > 1) f(a1, o1, d1, t + nt);
+  };
+  auto posof = [&](int ii) -> int & { return ii == t ? p0 : p1; };
+  if (t == 0) nmv = 0;
+  rows([&](double(&ar)[BNB], double(&orow)[BNB], int &, int ii) {
 #pragma unroll
-  for (int s = 0; s < ROWS; s++) {
-    const int ii = t + s * BPT;
-#pragma unroll
-    for (int k = 0; k < BNB; k++) a[s][k] = (ii < PR && k < jb) ? ab[(j0 + k) * LD + KV + ii - k] : 0.0;
-  }
-#pragma unroll
+    for (int c = 0; c < BNB; c++) {
+      ar[c] = (ii < PR && c < jb) ? ab[(j0 + c) * LD + KV + ii - c] : 0.0;
+      orow[c] = 0.0;
+    }
+  });
+  __syncthreads();
+#pragma unroll 1
   for (int k = 0; k < BNB; k++) {
     if (k < jb) {
-      const int hi = PR < k + kl + 1 ? PR : k + kl + 1;
-      double bv = -1.0;
-      int bi = INT_MAX;
-#pragma unroll
-      for (int s = 0; s < ROWS; s++) {
-        const int ii = t + s * BPT;
-        if (ii >= k && ii < hi) {
-          const double v = fabs(a[s][k]);
-          if (v > bv) {
-            bv = v;
-            bi = ii;
-          }
+      unsigned long long key = 0;
+      rows([&](double(&ar)[BNB], double(&)[BNB], int &done, int ii) {
+        if (!done && ii < PR) {
+          const unsigned long long kk = piv_key(ar[0], ii);
+          key = kk > key ? kk : key;
         }
-      }
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double ov = __shfl_xor(bv, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ov > bv || (ov == bv && oi < bi)) {
-          bv = ov;
-          bi = oi;
-        }
-      }
-      if (lane == 0) {
-        wv[wave] = bv;
-        wi[wave] = bi;
-      }
+      });
+      key = wave_max_u64(key);
+      if (lane == 0) wk[wave] = key;
       __syncthreads();
+      for (int w = 0; w < NW; w++) key = wk[w] > key ? wk[w] : key;
+      const int r = 0xFFF - (int)(key & 0xFFF);  // the chosen row
+      rows([&](double(&ar)[BNB], double(&)[BNB], int &, int ii) {
+        if (ii == r) {
+#pragma unroll
+          for (int c = 0; c < BNB; c++) rowP[c] = ar[c];
+          rowP[BNB] = ar[0] != 0.0 ? 1.0 / ar[0] : 0.0;
+          ppos = posof(ii);
+          ipiv[j0 + k] = (int)(j0 + posof(ii));
+        }
+      });
+      __syncthreads();
+      {
+        // the chosen row moves to position k, the row there to its place
+        const int pk = ppos;
+        rows([&](double(&)[BNB], double(&)[BNB], int &, int ii) {
+          int &ps = posof(ii);
+          if (ii == r)
+            ps = k;
+          else if (ps == k)
+            ps = pk;
+        });
+      }
+      // multipliers scaled by the pivot's reciprocal, as LAPACK's dgetf2 does
+      const double rpiv = rowP[BNB];
       if (t == 0) {
-        double v = wv[0];
-        int p = wi[0];
-        for (int q = 1; q < BPT / 64; q++)
-          if (wv[q] > v || (wv[q] == v && wi[q] < p)) {
-            v = wv[q];
-            p = wi[q];
-          }
-        sp = p;
-        ipiv[j0 + k] = (int)(j0 + p);
+        if (rpiv == 0.0) atomicMin(info, (int)(j0 + k));
+        dinv[j0 + k] = rpiv;
       }
-      __syncthreads();
-      const int p = sp;
+      rows([&](double(&ar)[BNB], double(&orow)[BNB], int &done, int ii) {
+        double enter = ar[0];
+        if (ii == r) {
+          done = 1;
+        } else if (!done) {
+          const double l = ar[0] * rpiv;
+          enter = l;
 #pragma unroll
-      for (int s = 0; s < ROWS; s++) {
-        const int ii = t + s * BPT;
-        if (ii == p)
-#pragma unroll
-          for (int q = 0; q < BNB; q++) rowP[q] = a[s][q];
-        if (ii == k)
-#pragma unroll
-          for (int q = 0; q < BNB; q++) rowK[q] = a[s][q];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < ROWS; s++) {
-        const int ii = t + s * BPT;
-        if (ii == k) {
-#pragma unroll
-          for (int q = 0; q < BNB; q++) a[s][q] = rowP[q];
-        } else if (ii == p) {
-#pragma unroll
-          for (int q = 0; q < BNB; q++) a[s][q] = rowK[q];
+          for (int c = 1; c < BNB; c++) ar[c] = ar[c] - l * rowP[c];
         }
-      }
-      const double piv = rowP[k];
-      if (piv == 0.0) {
-        if (t == 0) atomicMin(info, (int)(j0 + k));
-      } else {
 #pragma unroll
-        for (int s = 0; s < ROWS; s++) {
-          const int ii = t + s * BPT;
-          if (ii > k && ii < hi) {
-            const double l = a[s][k] / piv;
-            a[s][k] = l;
-#pragma unroll
-            for (int q = k + 1; q < BNB; q++) a[s][q] = a[s][q] - l * rowP[q];
-          }
+        for (int c = 0; c + 1 < BNB; c++) {
+          ar[c] = ar[c + 1];
+          orow[c] = orow[c + 1];
         }
-      }
-      __syncthreads();
+        ar[BNB - 1] = 0.0;
+        orow[BNB - 1] = enter;
+      });
+    } else {
+      // a last panel narrower than BNB: shift on so that out[c] is column c
+      rows([&](double(&)[BNB], double(&orow)[BNB], int &, int) {
+#pragma unroll
+        for (int c = 0; c + 1 < BNB; c++) orow[c] = orow[c + 1];
+        orow[BNB - 1] = 0.0;
+      });
     }
   }
-#pragma unroll
-  for (int s = 0; s < ROWS; s++) {
-    const int ii = t + s * BPT;
-    if (ii < PR)
-#pragma unroll
-      for (int k = 0; k < BNB; k++)
-        if (k < jb) ab[(j0 + k) * LD + KV + ii - k] = a[s][k];
+  // row moves (at most two per interchange): position, and the position
+  // the row there started from
+  rows([&](double(&)[BNB], double(&)[BNB], int &, int ii) {
+    const int pos = posof(ii);
+    if (ii < PR && pos != ii) {
+      const int e = atomicAdd(&nmv, 1);
+      mpos[e] = pos;
+      msrc[e] = ii;
+    }
+  });
+  __syncthreads();
+  int *m = mv + (j0 / BNB) * 4 * BNB;
+  if (t < 2 * BNB) {
+    m[2 * t] = t < nmv ? mpos[t] : -1;
+    m[2 * t + 1] = t < nmv ? msrc[t] : -1;
   }
+  rows([&](double(&)[BNB], double(&orow)[BNB], int &, int ii) {
+    if (ii < PR) {
+      const int pos = posof(ii);
+#pragma unroll
+      for (int c = 0; c < BNB; c++)
+        if (c < jb) ab[(j0 + c) * LD + KV + pos - c] = orow[c];
+    }
+  });
+}
+
+// Where the panel's interchanges put rows: slot k < BNB is panel row j0+k,
+// slot BNB + e the row below the panel that interchange e names first
+// (xr[e]; -1 where interchange e names a panel row or a row named before);
+// the row of interchange k sits in slot sl[k] (>= k).  Thread k < jb
+// fills entry k from the panel's pivots pv[] (in LDS).
+__device__ __forceinline__ void panel_slot(const int *pv, int64_t j0, int jb, int k, int *sl, int64_t *xr) {
+  const int64_t p = pv[k];
+  if (p < j0 + jb) {
+    sl[k] = (int)(p - j0);
+    xr[k] = -1;
+    return;
+  }
+  int e = 0;
+  while (pv[e] != p) e++;  // the first interchange naming p (e <= k)
+  sl[k] = BNB + e;
+  xr[k] = e == k ? p : -1;
 }
 
 // The panel's interchanges and unit-lower triangle on the trailing columns
-// q in [j0+jb, j0+jb+KV), one thread per column: rows c and ipiv[c] in
-// order where column q still lies in row c's band, then the triangle on the
-// panel rows.
+// q in [j0+jb, j0+jb+KV): 16 lanes per column, lane k loading panel row
+// j0+k (16 consecutive doubles per column) and the row below the panel that
+// interchange k names first; the interchanges, composed once per block
+// into one permutation of the 32 slots, move the values through LDS, and
+// the triangle runs across the 16 lanes by shuffles.  (Where column q lies
+// beyond a panel row's band, that row and the row it exchanges with both
+// hold zero there.)
 __global__ __launch_bounds__(256) void k_band_trsm(double *ab, int64_t LD, int KV, int64_t n, int64_t j0,
                                                    const int *ipiv) {
   __shared__ double L11[BNB][BNB + 1];
-  __shared__ int pvl[BNB];
-  const int t = threadIdx.x;
+  __shared__ double vs[256 / BNB][2 * BNB + 1];
+  __shared__ int sl[BNB], pv[BNB], perm[2 * BNB];
+  __shared__ int64_t xr[BNB];
+  const int t = threadIdx.x, cq = t / BNB, k = t % BNB;
   const int jb = (int)(n - j0 < BNB ? n - j0 : BNB);
-  for (int e = t; e < BNB * BNB; e += 256) {
-    const int r = e % BNB, k = e / BNB;
-    L11[r][k] = (r > k && r < jb) ? ab[(j0 + k) * LD + KV + r - k] : 0.0;
+  {
+    const int r = t % BNB, c = t / BNB;
+    L11[r][c] = (r > c && r < jb) ? ab[(j0 + c) * LD + KV + r - c] : 0.0;
   }
-  if (t < BNB) pvl[t] = t < jb ? (int)(ipiv[j0 + t] - j0) : t;
+  if (t < jb) pv[t] = ipiv[j0 + t];
   __syncthreads();
-  const int64_t q0 = j0 + jb, q1 = n < j0 + jb + KV ? n : j0 + jb + KV;
-  const int64_t q = q0 + (int64_t)blockIdx.x * 256 + t;
-  if (q < q1) {
-    double *col = ab + q * LD + KV;  // col[i - q] = A(i, q)
-    for (int k = 0; k < jb; k++) {
-      const int64_t c = j0 + k, p = j0 + pvl[k];
-      if (p != c && q - c <= KV) {
-        const double x = col[c - q];
-        col[c - q] = col[p - q];
-        col[p - q] = x;
-      }
+  if (t < jb) panel_slot(pv, j0, jb, t, sl, xr);
+  if (t >= jb && t < BNB) xr[t] = -1;
+  __syncthreads();
+  if (t == 0) {
+    // slot s ends up holding the value first in slot perm[s]
+    for (int e = 0; e < 2 * BNB; e++) perm[e] = e;
+    for (int c = 0; c < jb; c++) {
+      const int s2 = sl[c];
+      const int x = perm[c];
+      perm[c] = perm[s2];
+      perm[s2] = x;
     }
-    double uu[BNB];
-#pragma unroll
-    for (int k = 0; k < BNB; k++) uu[k] = (k < jb && q - (j0 + k) <= KV) ? col[j0 + k - q] : 0.0;
-#pragma unroll
-    for (int k = 0; k < BNB; k++)
-#pragma unroll
-      for (int k2 = k + 1; k2 < BNB; k2++) uu[k2] = uu[k2] - L11[k2][k] * uu[k];
-#pragma unroll
-    for (int k = 0; k < BNB; k++)
-      if (k < jb && q - (j0 + k) <= KV) col[j0 + k - q] = uu[k];
   }
+  const int64_t q1 = n < j0 + jb + KV ? n : j0 + jb + KV;
+  const int64_t q = j0 + jb + (int64_t)blockIdx.x * (256 / BNB) + cq;
+  const bool act = q < q1;
+  double *col = ab + (act ? q : 0) * LD + KV;  // col[i - q] = A(i, q)
+  const bool inb = act && k < jb && q - (j0 + k) <= KV;
+  const int64_t xk = xr[k];
+  vs[cq][k] = inb ? col[j0 + k - q] : 0.0;
+  vs[cq][BNB + k] = (act && xk >= 0) ? col[xk - q] : 0.0;
+  __syncthreads();
+  double u = vs[cq][perm[k]];
+  const double ue = vs[cq][perm[BNB + k]];
+#pragma unroll
+  for (int c = 0; c < BNB; c++) {
+    const double uc = __shfl(u, c, BNB);
+    if (k > c) u = u - L11[k][c] * uc;
+  }
+  if (inb) col[j0 + k - q] = u;
+  if (act && xk >= 0) col[xk - q] = ue;
 }
 
 // A(i, q) -= sum_k L(i, j0+k) U(j0+k, q) for i in [j0+jb, j0+jb+kl),
 // q in [j0+jb, j0+jb+KV): the window lies inside the band; U(j0+k, q) is
 // zero where q lies beyond row j0+k's band.  64 x 64 tiles, each thread one
-// row and 16 columns (coalesced along the rows of a column).
+// row and 16 columns (coalesced along the rows of a column), the 16 loads
+// issued together.
 __global__ __launch_bounds__(256) void k_band_gemm(double *ab, int64_t LD, int KV, int kl, int64_t n, int64_t j0,
                                                    int jb) {
-  __shared__ double U[BNB][65];
+  __shared__ double U[64][BNB + 1];  // U[cq][k] = U(j0+k, tq+cq)
   const int64_t r0 = j0 + jb, r1 = n < j0 + jb + kl ? n : j0 + jb + kl;
   const int64_t q1 = n < j0 + jb + KV ? n : j0 + jb + KV;
   const int64_t tq = j0 + jb + (int64_t)blockIdx.y * 64, ti = r0 + (int64_t)blockIdx.x * 64;
@@ -392,63 +478,74 @@ __global__ __launch_bounds__(256) void k_band_gemm(double *ab, int64_t LD, int K
     const int64_t q = tq + cq;
     double v = 0;
     if (k < jb && q < q1 && q - (j0 + k) <= KV) v = ab[q * LD + KV + (j0 + k) - q];
-    U[k][cq] = v;
+    U[cq][k] = v;
   }
   __syncthreads();
   const int lane = t & 63, grp = t >> 6;
   const int64_t i = ti + lane;
   if (i >= r1) return;
+  // A(i, q) for consecutive q lies LD - 1 apart
+  const int64_t st = LD - 1, qa = tq + grp * 16;
+  const int nc = (int)(q1 - qa < 16 ? q1 - qa : 16);
+  double *base = ab + qa * st + KV + i;
+  const double *lb = ab + j0 * st + KV + i;
   double l[BNB];
 #pragma unroll
-  for (int k = 0; k < BNB; k++) l[k] = k < jb ? ab[(j0 + k) * LD + KV + i - (j0 + k)] : 0.0;
-  for (int cq = grp * 16; cq < grp * 16 + 16; cq++) {
-    const int64_t q = tq + cq;
-    if (q >= q1) break;
-    double *cp = ab + q * LD + KV + i - q;
-    double acc = *cp;
+  for (int k = 0; k < BNB; k++) l[k] = k < jb ? lb[k * st] : 0.0;
+  double cv[16];
 #pragma unroll
-    for (int k = 0; k < BNB; k++) acc = fma(-l[k], U[k][cq], acc);
-    *cp = acc;
+  for (int c = 0; c < 16; c++) cv[c] = c < nc ? base[c * st] : 0.0;
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    double acc = cv[c];
+#pragma unroll
+    for (int k = 0; k < BNB; k++) acc = fma(-l[k], U[grp * 16 + c][k], acc);
+    if (c < nc) base[c * st] = acc;
   }
 }
 
 // ---------------------------------------------------------------- solves
-// L y = P b, panel by panel: the panel's interchanges, its unit-lower
-// triangle (one wavefront), then the rows below it; b[i] lives at
-// win[i % BW] while it is active.
+// L y = P b, panel by panel: b[i] lives at win[i % BW] while it is active.
+// Per panel: the new entries and the panel's unit-lower triangle come into
+// LDS together; wavefront 0 makes the panel's row moves (all reads before
+// all writes), then solves the triangle by lane shuffles; then every
+// thread updates rows below the panel, its 16 loads issued together.
 __global__ __launch_bounds__(BPT) void k_band_fwd(const double *ab, int64_t LD, int KV, int kl, int64_t n,
-                                                  const int *ipiv, double *b) {
-  __shared__ double win[BW];
+                                                  const int *mv, double *b) {
+  __shared__ double win[BW], tri[BNB][BNB + 1];
   const int t = threadIdx.x, lane = t & 63;
   int64_t loaded = 0;
   for (int64_t j0 = 0; j0 < n; j0 += BNB) {
     const int jb = (int)(n - j0 < BNB ? n - j0 : BNB);
     const int64_t rmax = n < j0 + jb + kl ? n : j0 + jb + kl;
+    const int *m = mv + (j0 / BNB) * 4 * BNB;
+    const int pos = t < 2 * BNB ? m[2 * t] : -1, src = t < 2 * BNB ? m[2 * t + 1] : -1;
     for (int64_t i = loaded + t; i < rmax; i += BPT) win[i % BW] = b[i];
     loaded = rmax > loaded ? rmax : loaded;
-    __syncthreads();
-    if (t == 0)
-      for (int k = 0; k < jb; k++) {
-        const int64_t c = j0 + k, p = ipiv[c];
-        if (p != c) {
-          const double x = win[c % BW];
-          win[c % BW] = win[p % BW];
-          win[p % BW] = x;
-        }
-      }
+    if (t < BNB * BNB) {
+      const int r = t % BNB, k = t / BNB;
+      tri[r][k] = (r > k && r < jb) ? ab[(j0 + k) * LD + KV + r - k] : 0.0;
+    }
     __syncthreads();
     if (t < 64) {
+      const double v = pos >= 0 ? win[(j0 + src) % BW] : 0.0;
+      if (pos >= 0) win[(j0 + pos) % BW] = v;
       double yr = lane < jb ? win[(j0 + lane) % BW] : 0.0;
       for (int k = 0; k < jb; k++) {
         const double yk = __shfl(yr, k);
-        if (lane > k && lane < jb) yr -= ab[(j0 + k) * LD + KV + lane - k] * yk;
+        if (lane > k && lane < jb) yr -= tri[lane][k] * yk;
       }
       if (lane < jb) win[(j0 + lane) % BW] = yr;
     }
     __syncthreads();
     for (int64_t i = j0 + jb + t; i < rmax; i += BPT) {
+      double lv[BNB];
+#pragma unroll
+      for (int k = 0; k < BNB; k++) lv[k] = k < jb ? ab[(j0 + k) * LD + KV + i - (j0 + k)] : 0.0;
       double acc = win[i % BW];
-      for (int k = 0; k < jb; k++) acc -= ab[(j0 + k) * LD + KV + i - (j0 + k)] * win[(j0 + k) % BW];
+#pragma unroll
+      for (int k = 0; k < BNB; k++)
+        if (k < jb) acc -= lv[k] * win[(j0 + k) % BW];
       win[i % BW] = acc;
     }
     if (t < jb) b[j0 + t] = win[(j0 + t) % BW];
@@ -456,10 +553,13 @@ __global__ __launch_bounds__(BPT) void k_band_fwd(const double *ab, int64_t LD, 
   }
 }
 
-// U x = y, panels from the last: the panel's upper triangle (one
-// wavefront), then the rows above it within KV.
-__global__ __launch_bounds__(BPT) void k_band_bwd(const double *ab, int64_t LD, int KV, int64_t n, double *b) {
-  __shared__ double win[BW];
+// U x = y, panels from the last: the panel's upper triangle comes into LDS
+// with the new entries, wavefront 0 solves it (multiplying by the pivots'
+// reciprocals), then the rows above within KV take the panel's columns,
+// 16 loads a row issued together.
+__global__ __launch_bounds__(BPT) void k_band_bwd(const double *ab, int64_t LD, int KV, int64_t n,
+                                                  const double *dinv, double *b) {
+  __shared__ double win[BW], tri[BNB][BNB + 1];
   const int t = threadIdx.x, lane = t & 63;
   int64_t lo = n;  // b[lo .. n) are in the window
   for (int64_t j0 = ((n - 1) / BNB) * BNB; j0 >= 0; j0 -= BNB) {
@@ -467,23 +567,33 @@ __global__ __launch_bounds__(BPT) void k_band_bwd(const double *ab, int64_t LD, 
     const int64_t rlo = j0 - KV > 0 ? j0 - KV : 0;
     for (int64_t i = rlo + t; i < lo; i += BPT) win[i % BW] = b[i];
     lo = rlo < lo ? rlo : lo;
+    if (t < BNB * BNB) {
+      const int r = t % BNB, k = t / BNB;
+      tri[r][k] = (r < k && k < jb) ? ab[(j0 + k) * LD + KV + r - k] : 0.0;
+    }
+    const double di = (t < 64 && lane < jb) ? dinv[j0 + lane] : 0.0;
     __syncthreads();
     if (t < 64) {
       double yr = lane < jb ? win[(j0 + lane) % BW] : 0.0;
       for (int k = jb - 1; k >= 0; k--) {
-        if (lane == k) yr = yr / ab[(j0 + k) * LD + KV];
+        if (lane == k) yr = yr * di;
         const double xk = __shfl(yr, k);
-        if (lane < k) yr -= ab[(j0 + k) * LD + KV + lane - k] * xk;
+        if (lane < k) yr -= tri[lane][k] * xk;
       }
       if (lane < jb) win[(j0 + lane) % BW] = yr;
     }
     __syncthreads();
     for (int64_t r = rlo + t; r < j0; r += BPT) {
-      double acc = win[r % BW];
-      for (int k = 0; k < jb; k++) {
+      double uv[BNB];
+#pragma unroll
+      for (int k = 0; k < BNB; k++) {
         const int64_t q = j0 + k;
-        if (q - r <= KV) acc -= ab[q * LD + KV + r - q] * win[q % BW];
+        uv[k] = (k < jb && q - r <= KV) ? ab[q * LD + KV + r - q] : 0.0;
       }
+      double acc = win[r % BW];
+#pragma unroll
+      for (int k = 0; k < BNB; k++)
+        if (k < jb) acc -= uv[k] * win[(j0 + k) % BW];
       win[r % BW] = acc;
     }
     if (t < jb) b[j0 + t] = win[(j0 + t) % BW];
@@ -518,6 +628,8 @@ int band_factor(i2e_ctx *c) {
   ST_OK(B.cmx.alloc(n));
   ST_OK(B.w.alloc(n));
   ST_OK(B.ipiv.alloc(n));
+  ST_OK(B.dinv.alloc(n));
+  ST_OK(B.mv.alloc((size_t)((n + BNB - 1) / BNB) * 4 * BNB));
   HIP_OK(hipMemsetAsync(B.dims.p, 0, 4 * sizeof(int), c->s));
   hipLaunchKernelGGL(k_band_rows, dim3((Y + 255) / 256, X), dim3(256), 0, c->s, c->sop(), c->bc, B.ecol.p, B.eval.p, B.ecnt.p,
                      B.dims.p);
@@ -534,7 +646,7 @@ int band_factor(i2e_ctx *c) {
   B.KV = B.kl + B.ku;
   B.LD = (int64_t)B.KV + B.kl + BNB;
   const double bytes = (double)n * (double)B.LD * sizeof(double);
-  if (B.kl + BNB > BROWS * BPT || B.KV + BNB > BW || B.KV < BNB || bytes > BAND_MAX_BYTES) {
+  if (B.kl + BNB > BPMAX || B.KV + BNB > BW || B.KV < BNB || bytes > BAND_MAX_BYTES) {
     set_error("band LU: bandwidths kl %d ku %d (%.1f GB) beyond what the band kernels take (kl + ku <= %d)", B.kl,
               B.ku, bytes / 1e9, BW - BNB);
     return I2E_ERR_UNSUPPORTED;
@@ -558,15 +670,18 @@ int band_factor(i2e_ctx *c) {
   HIP_OK(hipMemcpyAsync(B.dims.p + 3, &big, sizeof(int), hipMemcpyHostToDevice, c->s));
   for (int64_t j0 = 0; j0 < n; j0 += BNB) {
     const int jb = (int)(n - j0 < BNB ? n - j0 : BNB);
-    if (B.kl + BNB <= BPT)
-      hipLaunchKernelGGL(k_band_panel<1>, dim3(1), dim3(BPT), 0, c->s, B.ab.p, B.LD, B.KV, B.kl, n, j0, B.ipiv.p,
-                         B.dims.p + 3);
+    // panel rows per thread: 1 at 1024 threads, else 2 at 768 (the
+    // registers 768 threads leave hold 64 doubles a thread without spills)
+    // (the block: the panel's rows rounded up to whole wavefronts)
+    if (B.kl + BNB <= 1024)
+      hipLaunchKernelGGL((k_band_panel<1, 1024>), dim3(1), dim3(((B.kl + BNB + 63) / 64) * 64), 0, c->s, B.ab.p,
+                         B.LD, B.KV, B.kl, n, j0, B.ipiv.p, B.dims.p + 3, B.dinv.p, B.mv.p);
     else
-      hipLaunchKernelGGL(k_band_panel<2>, dim3(1), dim3(BPT), 0, c->s, B.ab.p, B.LD, B.KV, B.kl, n, j0, B.ipiv.p,
-                         B.dims.p + 3);
+      hipLaunchKernelGGL((k_band_panel<2, 768>), dim3(1), dim3(((B.kl + BNB + 127) / 128) * 64), 0, c->s, B.ab.p,
+                         B.LD, B.KV, B.kl, n, j0, B.ipiv.p, B.dims.p + 3, B.dinv.p, B.mv.p);
     if (j0 + jb < n) {
       const int64_t nq0 = (n < j0 + jb + B.KV ? n : j0 + jb + B.KV) - (j0 + jb);
-      hipLaunchKernelGGL(k_band_trsm, dim3(nblk(nq0, 256)), dim3(256), 0, c->s, B.ab.p, B.LD, B.KV, n, j0,
+      hipLaunchKernelGGL(k_band_trsm, dim3(nblk(nq0, 256 / BNB)), dim3(256), 0, c->s, B.ab.p, B.LD, B.KV, n, j0,
                          (const int *)B.ipiv.p);
       const int64_t nr = (n < j0 + jb + B.kl ? n : j0 + jb + B.kl) - (j0 + jb);
       const int64_t nq = (n < j0 + jb + B.KV ? n : j0 + jb + B.KV) - (j0 + jb);
@@ -592,8 +707,10 @@ int band_apply(i2e_ctx *c, const double *r, double *z) {
   const int64_t n = B.n;
   const unsigned g = nblk(n, 256);
   hipLaunchKernelGGL(k_band_mul, dim3(g), dim3(256), 0, c->s, n, B.rs.p, r, B.w.p);
-  hipLaunchKernelGGL(k_band_fwd, dim3(1), dim3(BPT), 0, c->s, B.ab.p, B.LD, B.KV, B.kl, n, B.ipiv.p, B.w.p);
-  hipLaunchKernelGGL(k_band_bwd, dim3(1), dim3(BPT), 0, c->s, B.ab.p, B.LD, B.KV, n, B.w.p);
+  hipLaunchKernelGGL(k_band_fwd, dim3(1), dim3(BPT), 0, c->s, B.ab.p, B.LD, B.KV, B.kl, n, (const int *)B.mv.p,
+                     B.w.p);
+  hipLaunchKernelGGL(k_band_bwd, dim3(1), dim3(BPT), 0, c->s, B.ab.p, B.LD, B.KV, n, (const double *)B.dinv.p,
+                     B.w.p);
   hipLaunchKernelGGL(k_band_mul, dim3(g), dim3(256), 0, c->s, n, B.cs.p, B.w.p, z);
   HIP_OK(hipGetLastError());
   return I2E_OK;

@@ -144,10 +144,11 @@ def test_direct_molten_first_solve(orc):
     evs = rel(np.r_[xn[1::3], xn[2::3]], np.r_[xe[1::3], xe[2::3]])
     say("oracle rows: unscaled %.2e scaled %.2e; vs scipy (COLAMD): velocities %.2e P %.2e; vs scipy (natural): "
         "velocities %.2e; the two scipy orderings differ by %.2e" % (unscaled, scaled, ev, ep, evn, evs))
-    assert unscaled <= 1e-8
-    assert scaled <= 1e-12
-    assert ep <= 1e-8
-    assert ev <= 1e-2
+    assert unscaled <= RTOL_MOLTEN * 3
+    assert scaled <= 1e-13
+    assert ep <= 1e-9
+    # no farther from scipy's COLAMD solve than twice scipy's own two orderings are apart
+    assert ev <= 2 * evs
 
 
 def test_direct_refuses_option_out_of_range():
