@@ -43,7 +43,7 @@ struct BandLU {
   int64_t n = 0;  // unknowns (3 N)
   int kl = 0, ku = 0, KV = 0;
   int64_t LD = 0;
-  DBuf<double> ab, rs, cs, rmx, w, eval, dinv;
+  DBuf<double> ab, rs, cs, rmx, w, w2, eval, dinv;
   DBuf<unsigned long long> cmx;
   DBuf<int> ipiv, ecol, ecnt, dims, mv;
 };
@@ -55,6 +55,7 @@ void band_release(BandLU *b) {
   b->cs.release();
   b->rmx.release();
   b->w.release();
+  b->w2.release();
   b->eval.release();
   b->cmx.release();
   b->ipiv.release();
@@ -605,6 +606,10 @@ __global__ void k_band_mul(int64_t n, const double *s, const double *x, double *
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k < n) y[k] = s[k] * x[k];
 }
+__global__ void k_band_div(int64_t n, const double *s, double *y) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) y[k] = y[k] / s[k];
+}
 
 }  // namespace i2e
 
@@ -627,6 +632,7 @@ int band_factor(i2e_ctx *c) {
   ST_OK(B.rmx.alloc(n));
   ST_OK(B.cmx.alloc(n));
   ST_OK(B.w.alloc(n));
+  ST_OK(B.w2.alloc(n));
   ST_OK(B.ipiv.alloc(n));
   ST_OK(B.dinv.alloc(n));
   ST_OK(B.mv.alloc((size_t)((n + BNB - 1) / BNB) * 4 * BNB));
@@ -701,17 +707,52 @@ int band_factor(i2e_ctx *c) {
   return I2E_OK;
 }
 
-// z = C (LU)^-1 R r: the equilibrated factors applied to the unscaled rows
-int band_apply(i2e_ctx *c, const double *r, double *z) {
+// unscaled rows: z = C (LU)^-1 R r, the equilibrated factors applied to the
+// assembled system; equilibrated rows (scaled): z = (LU)^-1 r
+int band_apply(i2e_ctx *c, const double *r, double *z, bool scaled) {
   BandLU &B = *c->band;
   const int64_t n = B.n;
   const unsigned g = nblk(n, 256);
-  hipLaunchKernelGGL(k_band_mul, dim3(g), dim3(256), 0, c->s, n, B.rs.p, r, B.w.p);
+  if (scaled)
+    HIP_OK(hipMemcpyAsync(B.w.p, r, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  else
+    hipLaunchKernelGGL(k_band_mul, dim3(g), dim3(256), 0, c->s, n, B.rs.p, r, B.w.p);
   hipLaunchKernelGGL(k_band_fwd, dim3(1), dim3(BPT), 0, c->s, B.ab.p, B.LD, B.KV, B.kl, n, (const int *)B.mv.p,
                      B.w.p);
   hipLaunchKernelGGL(k_band_bwd, dim3(1), dim3(BPT), 0, c->s, B.ab.p, B.LD, B.KV, n, (const double *)B.dinv.p,
                      B.w.p);
-  hipLaunchKernelGGL(k_band_mul, dim3(g), dim3(256), 0, c->s, n, B.cs.p, B.w.p, z);
+  if (scaled)
+    HIP_OK(hipMemcpyAsync(z, B.w.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  else
+    hipLaunchKernelGGL(k_band_mul, dim3(g), dim3(256), 0, c->s, n, B.cs.p, B.w.p, z);
+  HIP_OK(hipGetLastError());
+  return I2E_OK;
+}
+
+// The equilibrated system R A C y = R b (x = C y): b <- R b, x <- x / C
+// before the solve, x <- C y after it; y -> R A C y through the
+// matrix-free rows
+int band_scale_system(i2e_ctx *c, double *b, double *x) {
+  BandLU &B = *c->band;
+  const unsigned g = nblk(B.n, 256);
+  hipLaunchKernelGGL(k_band_mul, dim3(g), dim3(256), 0, c->s, B.n, B.rs.p, b, b);
+  hipLaunchKernelGGL(k_band_div, dim3(g), dim3(256), 0, c->s, B.n, B.cs.p, x);
+  HIP_OK(hipGetLastError());
+  return I2E_OK;
+}
+int band_unscale(i2e_ctx *c, double *x) {
+  BandLU &B = *c->band;
+  hipLaunchKernelGGL(k_band_mul, dim3(nblk(B.n, 256)), dim3(256), 0, c->s, B.n, B.cs.p, x, x);
+  HIP_OK(hipGetLastError());
+  return I2E_OK;
+}
+int band_op(i2e_ctx *c, const double *y, double *out) {
+  BandLU &B = *c->band;
+  const unsigned g = nblk(B.n, 256);
+  hipLaunchKernelGGL(k_band_mul, dim3(g), dim3(256), 0, c->s, B.n, B.cs.p, y, B.w2.p);
+  hipLaunchKernelGGL(k_stokes_apply, dim3((c->Y + 255) / 256, c->X), dim3(256), 0, c->s, c->sop(), c->bc,
+                     (const double *)B.w2.p, out);
+  hipLaunchKernelGGL(k_band_mul, dim3(g), dim3(256), 0, c->s, B.n, B.rs.p, out, out);
   HIP_OK(hipGetLastError());
   return I2E_OK;
 }

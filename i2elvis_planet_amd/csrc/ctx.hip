@@ -635,7 +635,7 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
       (o->schur != I2E_SCHUR_BFBT && o->schur != I2E_SCHUR_MASS && o->schur != I2E_SCHUR_WBFBT) ||
       (o->precision != I2E_PREC_FP64 && o->precision != I2E_PREC_MIXED) || (o->graphs != 0 && o->graphs != 1) ||
       (o->autotune != 0 && o->autotune != 1) || (o->coarse_coop != 0 && o->coarse_coop != 1) ||
-      (o->stokes_direct != 0 && o->stokes_direct != 1)) {
+      o->stokes_direct < 0 || o->stokes_direct > 2) {
     set_error("set_solver: option out of range");
     return I2E_ERR_ARG;
   }
@@ -1311,15 +1311,20 @@ static int stokes_solve(i2e_ctx *c) {
   // write the three residual components, 8 B each per node; this
   // implementation also reads the 4 BC row numbers (int32) per node
   const double alg = 80.0 * (double)c->N, bytes = alg + 16.0 * (double)c->N;
+  const int direct = c->o.stokes_direct;
+  if (direct) ST_OK(band_factor(c));
+  // direct = 2: FGMRES on the equilibrated rows R A C y = R b, so that its
+  // residual, and the tolerance, are the equilibrated ones
+  const bool eq = direct == 2;
+  if (eq) ST_OK(band_scale_system(c, c->bs.p, c->xs.p));
   auto A = [&](const double *x, double *y) -> int {
+    if (eq) return band_op(c, x, y);
     KT_LAUNCH(&c->kt, K_STOKES_APPLY, alg, bytes, k_stokes_apply, nb2(X, Y), dim3(256), 0, c->s, o, c->bc, x, y);
     return I2E_OK;
   };
   // each projection pass writes the velocity components of the pending
   // vector into the finest velocity level's right-hand side, saving the
   // preconditioner's split pass
-  const bool direct = c->o.stokes_direct != 0;
-  if (direct) ST_OK(band_factor(c));
   c->fs.sx = direct ? nullptr : mixed(c) ? (void *)c->v32.L[0].fx.p : (void *)c->v64.L[0].fx.p;
   c->fs.sy = direct ? nullptr : mixed(c) ? (void *)c->v32.L[0].fy.p : (void *)c->v64.L[0].fy.p;
   c->fs.split_f32 = mixed(c);
@@ -1327,7 +1332,7 @@ static int stokes_solve(i2e_ctx *c) {
   auto Mf = [&](const double *r, double *z) -> int {
     c->kt.begin(K_PRECOND, c->s);
     if (direct) {
-      ST_OK(band_apply(c, r, z));
+      ST_OK(band_apply(c, r, z, eq));
     } else {
       const bool presplit = r == c->fs.split_of;
       c->fs.split_of = nullptr;
@@ -1343,8 +1348,10 @@ static int stokes_solve(i2e_ctx *c) {
   c->fs.split_of = nullptr;
   c->st.stokes_iters = it;
   c->st.stokes_relres = rr;
+  if (eq) ST_OK(band_unscale(c, c->xs.p));
   if (st) {
-    set_error("Stokes solve: %d iterations, relative residual %.3e > %.1e", it, rr, c->o.stokes_rtol);
+    set_error("Stokes solve: %d iterations, relative residual %.3e > %.1e%s", it, rr, c->o.stokes_rtol,
+              eq ? " (equilibrated rows)" : "");
     return st;
   }
   hipLaunchKernelGGL(k_stokes_unpack, nb2(X, Y), dim3(256), 0, c->s, X, Y, c->xs.p, c->F.pr, c->F.vx, c->F.vy);

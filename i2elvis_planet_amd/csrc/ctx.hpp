@@ -101,5 +101,9 @@ int gravity_solve_rhs(i2e_ctx *c, double *b, double *x, int *its, double *rr);
 // stokes_direct: assemble, equilibrate and factor the Stokes rows of the
 // current fields; z = A^-1 r by the factors (band.hip)
 int band_factor(i2e_ctx *c);
-int band_apply(i2e_ctx *c, const double *r, double *z);
+int band_apply(i2e_ctx *c, const double *r, double *z, bool scaled = false);
+// stokes_direct = 2: FGMRES on the equilibrated rows R A C y = R b
+int band_scale_system(i2e_ctx *c, double *b, double *x);
+int band_unscale(i2e_ctx *c, double *x);
+int band_op(i2e_ctx *c, const double *y, double *out);
 int band_dims(const i2e_ctx *c, int *kl, int *ku, double *bytes);

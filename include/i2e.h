@@ -226,14 +226,19 @@ typedef struct i2e_solver_opts {
                               per ordinary launch: 501^2 measured 63.1 ms
                               per step with it, 55.8 without              */
   int32_t stokes_direct;   /* 1: the Stokes FGMRES is preconditioned by an
-                              LU factorisation of the equilibrated rows
-                              (band storage, partial pivoting inside the
-                              band, FP64 whatever the precision), refactored
-                              every solve: for viscosity contrasts the
-                              multigrid cannot precondition (melt against
-                              the clip).  Memory ~(3 kl + ku) 8 B per
-                              unknown, kl ~ 3 ynumy (27 GB at 501^2);
-                              0 (default): multigrid                      */
+                              LU factorisation of the Ruiz-equilibrated
+                              rows R A C (band storage, partial pivoting
+                              inside the band, FP64 whatever the
+                              precision), refactored every solve: for
+                              viscosity contrasts the multigrid cannot
+                              precondition (melt against the clip).
+                              Memory ~(3 kl + ku) 8 B per unknown, kl ~
+                              3 ynumy (27 GB at 501^2).  2: the same, with
+                              FGMRES on R A C y = R b, so that stokes_rtol
+                              and stokes_relres refer to the equilibrated
+                              residual ||R (b - A x)|| / ||R b|| (where the
+                              melt makes ||b - A x|| / ||b|| <= 1e-8
+                              unreachable in FP64); 0 (default): multigrid */
   int32_t pad_opts;
 } i2e_solver_opts;
 
