@@ -248,10 +248,15 @@ def kernel_table(ctx, o, snap):
     return out
 
 
-def hot_attempt(ctx, o, hs_cold, cfg):
+def hot_attempt(ctx, o, hs_cold, cfg, direct_steps=2):
     """The molten regime (core at 1800 K, melt above 0.4 at 1e2 Pa s against
-    the 1e23 clip): one Stokes solve attempted per precision, capped at 300
-    Krylov iterations; reports what the solver reached."""
+    the 1e23 clip): one Stokes solve attempted per precision with the
+    multigrid preconditioner, capped at 300 Krylov iterations (reports what
+    it reached); then the banded LU of the equilibrated rows
+    (stokes_direct = 2): the first molten solve to an equilibrated residual
+    of 3e-12 (one FGMRES iteration at 501^2; its floor lies near 1e-12), and
+    direct_steps full steps at 1e-9 (the floor near the settled step),
+    timed."""
     from i2elvis_planet_amd import I2EError, configs
     hot = configs.heat_state(hs_cold.copy(), cfg, t_core=1800.0)
     out = {}
@@ -277,6 +282,40 @@ def hot_attempt(ctx, o, hs_cold, cfg):
         out[mode] = {"converged": conv, "stokes_iters": st.stokes_iters, "stokes_relres": rr,
                      "ms": round((time.perf_counter() - t0) * 1e3, 1), "eta_min": st0.numin, "eta_max": st0.numax}
         log("hot %s: converged %s, %d its, relres %r" % (mode, conv, st.stokes_iters, rr))
+    o2 = type(o).from_buffer_copy(o)
+    o2.precision, o2.stokes_direct, o2.stokes_rtol, o2.maxit, o2.restart = 0, 2, 3e-12, 20, 20
+    ctx.set_solver(o2)
+    ctx.upload(hot)
+    ctx.ronurecalc()
+    ctx.set_scalars(hot.timestep or 7.8894e10, hot.timesum)
+    ctx.maxvelstep()
+    d = {}
+    try:
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.viterate()
+        ctx.sync()
+        st = ctx.stats()
+        kl, ku, nbytes = ctx.band_info()
+        d["first_solve"] = {"ms": round((time.perf_counter() - t0) * 1e3, 1), "stokes_iters": st.stokes_iters,
+                            "equilibrated_relres": st.stokes_relres, "band_kl": kl, "band_ku": ku,
+                            "band_GB": round(nbytes / 1e9, 2)}
+        o2.stokes_rtol = 1e-9
+        ctx.set_solver(o2)
+        steps = []
+        for k in range(direct_steps):
+            t0 = time.perf_counter()
+            ctx.step()
+            ctx.sync()
+            st = ctx.stats()
+            steps.append({"ms": round((time.perf_counter() - t0) * 1e3, 1), "ms_stokes": round(st.ms_stokes, 1),
+                          "stokes_iters": st.stokes_iters, "equilibrated_relres": st.stokes_relres,
+                          "dt_s": ctx.scalars()[0]})
+        d["steps"] = steps
+    except I2EError as e:
+        d["error"] = str(e)
+    out["direct_equilibrated"] = d
+    log("hot direct: %s" % d)
     ctx.set_solver(o)
     return out
 
