@@ -46,6 +46,7 @@ struct BandLU {
   DBuf<double> ab, rs, cs, rmx, w, w2, eval, dinv;
   DBuf<unsigned long long> cmx;
   DBuf<int> ipiv, ecol, ecnt, dims, mv;
+  hipEvent_t ev_t = nullptr, ev_b = nullptr;  // trailing interchanges done; the rest of an update done
 };
 
 void band_release(BandLU *b) {
@@ -61,6 +62,8 @@ void band_release(BandLU *b) {
   b->ipiv.release();
   b->dinv.release();
   b->mv.release();
+  if (b->ev_t) (void)hipEventDestroy(b->ev_t);
+  if (b->ev_b) (void)hipEventDestroy(b->ev_b);
   b->ecol.release();
   b->ecnt.release();
   b->dims.release();
@@ -463,16 +466,18 @@ __global__ __launch_bounds__(256) void k_band_trsm(double *ab, int64_t LD, int K
 }
 
 // A(i, q) -= sum_k L(i, j0+k) U(j0+k, q) for i in [j0+jb, j0+jb+kl),
-// q in [j0+jb, j0+jb+KV): the window lies inside the band; U(j0+k, q) is
+// q in [qa, qb) within [j0+jb, j0+jb+KV) (the factorisation runs the next
+// panel's columns first, the rest beside the next panel on a second
+// stream): the window lies inside the band; U(j0+k, q) is
 // zero where q lies beyond row j0+k's band.  64 x 64 tiles, each thread one
 // row and 16 columns (coalesced along the rows of a column), the 16 loads
 // issued together.
 __global__ __launch_bounds__(256) void k_band_gemm(double *ab, int64_t LD, int KV, int kl, int64_t n, int64_t j0,
-                                                   int jb) {
+                                                   int jb, int64_t qa, int64_t qb) {
   __shared__ double U[64][BNB + 1];  // U[cq][k] = U(j0+k, tq+cq)
   const int64_t r0 = j0 + jb, r1 = n < j0 + jb + kl ? n : j0 + jb + kl;
-  const int64_t q1 = n < j0 + jb + KV ? n : j0 + jb + KV;
-  const int64_t tq = j0 + jb + (int64_t)blockIdx.y * 64, ti = r0 + (int64_t)blockIdx.x * 64;
+  const int64_t q1 = qb;
+  const int64_t tq = qa + (int64_t)blockIdx.y * 64, ti = r0 + (int64_t)blockIdx.x * 64;
   const int t = threadIdx.x;
   for (int e = t; e < BNB * 64; e += 256) {
     const int k = e % BNB, cq = e / BNB;
@@ -486,9 +491,9 @@ __global__ __launch_bounds__(256) void k_band_gemm(double *ab, int64_t LD, int K
   const int64_t i = ti + lane;
   if (i >= r1) return;
   // A(i, q) for consecutive q lies LD - 1 apart
-  const int64_t st = LD - 1, qa = tq + grp * 16;
-  const int nc = (int)(q1 - qa < 16 ? q1 - qa : 16);
-  double *base = ab + qa * st + KV + i;
+  const int64_t st = LD - 1, qc = tq + grp * 16;
+  const int nc = (int)(q1 - qc < 16 ? q1 - qc : 16);
+  double *base = ab + qc * st + KV + i;
   const double *lb = ab + j0 * st + KV + i;
   double l[BNB];
 #pragma unroll
@@ -674,6 +679,18 @@ int band_factor(i2e_ctx *c) {
                      B.KV);
   const int big = INT_MAX;
   HIP_OK(hipMemcpyAsync(B.dims.p + 3, &big, sizeof(int), hipMemcpyHostToDevice, c->s));
+  // Look-ahead: each panel's update runs the next panel's 16 columns on the
+  // main stream, which then factors the next panel, while the rest of the
+  // window is updated on the side stream; the next panel's trailing
+  // interchanges wait for that rest (they touch the same columns).  Only
+  // for wide windows: at 501^2 (KV = 3010) it took the solve from 4.67 to
+  // 4.37 s, at 201^2 (KV = 1210) the two events per panel cost more than
+  // the overlap gave (0.476 -> 0.503 s).
+  const bool ahead = B.KV > 2048 && c->s2;
+  if (!B.ev_t) HIP_OK(hipEventCreateWithFlags(&B.ev_t, hipEventDisableTiming));
+  if (!B.ev_b) HIP_OK(hipEventCreateWithFlags(&B.ev_b, hipEventDisableTiming));
+  hipStream_t side = c->s2 ? c->s2 : c->s;
+  bool pending = false;
   for (int64_t j0 = 0; j0 < n; j0 += BNB) {
     const int jb = (int)(n - j0 < BNB ? n - j0 : BNB);
     // panel rows per thread: 1 at 1024 threads, else 2 at 768 (the
@@ -686,16 +703,28 @@ int band_factor(i2e_ctx *c) {
       hipLaunchKernelGGL((k_band_panel<2, 768>), dim3(1), dim3(((B.kl + BNB + 127) / 128) * 64), 0, c->s, B.ab.p,
                          B.LD, B.KV, B.kl, n, j0, B.ipiv.p, B.dims.p + 3, B.dinv.p, B.mv.p);
     if (j0 + jb < n) {
-      const int64_t nq0 = (n < j0 + jb + B.KV ? n : j0 + jb + B.KV) - (j0 + jb);
-      hipLaunchKernelGGL(k_band_trsm, dim3(nblk(nq0, 256 / BNB)), dim3(256), 0, c->s, B.ab.p, B.LD, B.KV, n, j0,
-                         (const int *)B.ipiv.p);
-      const int64_t nr = (n < j0 + jb + B.kl ? n : j0 + jb + B.kl) - (j0 + jb);
-      const int64_t nq = (n < j0 + jb + B.KV ? n : j0 + jb + B.KV) - (j0 + jb);
-      if (nr > 0 && nq > 0)
-        hipLaunchKernelGGL(k_band_gemm, dim3(nblk(nr, 64), nblk(nq, 64)), dim3(256), 0, c->s, B.ab.p, B.LD, B.KV,
-                           B.kl, n, j0, jb);
+      if (pending) HIP_OK(hipStreamWaitEvent(c->s, B.ev_b, 0));
+      pending = false;
+      const int64_t q0 = j0 + jb, q1 = n < j0 + jb + B.KV ? n : j0 + jb + B.KV;
+      hipLaunchKernelGGL(k_band_trsm, dim3(nblk(q1 - q0, 256 / BNB)), dim3(256), 0, c->s, B.ab.p, B.LD, B.KV, n,
+                         j0, (const int *)B.ipiv.p);
+      const int64_t nr = (n < j0 + jb + B.kl ? n : j0 + jb + B.kl) - q0;
+      const int64_t qs = !ahead ? q1 : q1 < q0 + BNB ? q1 : q0 + BNB;
+      if (nr > 0) {
+        hipLaunchKernelGGL(k_band_gemm, dim3(nblk(nr, 64), nblk(qs - q0, 64)), dim3(256), 0, c->s, B.ab.p, B.LD,
+                           B.KV, B.kl, n, j0, jb, q0, qs);
+        if (qs < q1) {
+          HIP_OK(hipEventRecord(B.ev_t, c->s));
+          HIP_OK(hipStreamWaitEvent(side, B.ev_t, 0));
+          hipLaunchKernelGGL(k_band_gemm, dim3(nblk(nr, 64), nblk(q1 - qs, 64)), dim3(256), 0, side, B.ab.p, B.LD,
+                             B.KV, B.kl, n, j0, jb, qs, q1);
+          HIP_OK(hipEventRecord(B.ev_b, side));
+          pending = true;
+        }
+      }
     }
   }
+  if (pending) HIP_OK(hipStreamWaitEvent(c->s, B.ev_b, 0));
   HIP_OK(hipGetLastError());
   int info = 0;
   HIP_OK(hipMemcpyAsync(&info, B.dims.p + 3, sizeof(int), hipMemcpyDeviceToHost, c->s));
