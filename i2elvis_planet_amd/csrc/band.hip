@@ -526,6 +526,15 @@ __global__ __launch_bounds__(BPT) void k_band_fwd(const double *ab, int64_t LD, 
     const int64_t rmax = n < j0 + jb + kl ? n : j0 + jb + kl;
     const int *m = mv + (j0 / BNB) * 4 * BNB;
     const int pos = t < 2 * BNB ? m[2 * t] : -1, src = t < 2 * BNB ? m[2 * t + 1] : -1;
+    // this thread's rows below the panel (kl <= 2 BPT), their multipliers
+    // loaded now so that the loads overlap the triangle
+    const int64_t i0 = j0 + jb + t, i1 = i0 + BPT;
+    double l0[BNB], l1[BNB];
+#pragma unroll
+    for (int k = 0; k < BNB; k++) {
+      l0[k] = (k < jb && i0 < rmax) ? ab[(j0 + k) * LD + KV + i0 - (j0 + k)] : 0.0;
+      l1[k] = (k < jb && i1 < rmax) ? ab[(j0 + k) * LD + KV + i1 - (j0 + k)] : 0.0;
+    }
     for (int64_t i = loaded + t; i < rmax; i += BPT) win[i % BW] = b[i];
     loaded = rmax > loaded ? rmax : loaded;
     if (t < BNB * BNB) {
@@ -544,15 +553,19 @@ __global__ __launch_bounds__(BPT) void k_band_fwd(const double *ab, int64_t LD, 
       if (lane < jb) win[(j0 + lane) % BW] = yr;
     }
     __syncthreads();
-    for (int64_t i = j0 + jb + t; i < rmax; i += BPT) {
-      double lv[BNB];
-#pragma unroll
-      for (int k = 0; k < BNB; k++) lv[k] = k < jb ? ab[(j0 + k) * LD + KV + i - (j0 + k)] : 0.0;
-      double acc = win[i % BW];
+    if (i0 < rmax) {
+      double acc = win[i0 % BW];
 #pragma unroll
       for (int k = 0; k < BNB; k++)
-        if (k < jb) acc -= lv[k] * win[(j0 + k) % BW];
-      win[i % BW] = acc;
+        if (k < jb) acc -= l0[k] * win[(j0 + k) % BW];
+      win[i0 % BW] = acc;
+    }
+    if (i1 < rmax) {
+      double acc = win[i1 % BW];
+#pragma unroll
+      for (int k = 0; k < BNB; k++)
+        if (k < jb) acc -= l1[k] * win[(j0 + k) % BW];
+      win[i1 % BW] = acc;
     }
     if (t < jb) b[j0 + t] = win[(j0 + t) % BW];
     __syncthreads();
@@ -578,6 +591,16 @@ __global__ __launch_bounds__(BPT) void k_band_bwd(const double *ab, int64_t LD, 
       tri[r][k] = (r < k && k < jb) ? ab[(j0 + k) * LD + KV + r - k] : 0.0;
     }
     const double di = (t < 64 && lane < jb) ? dinv[j0 + lane] : 0.0;
+    // this thread's first two rows above the panel, their U entries loaded
+    // now so that the loads overlap the triangle
+    const int64_t r0 = rlo + t, r1 = r0 + BPT;
+    double u0[BNB], u1[BNB];
+#pragma unroll
+    for (int k = 0; k < BNB; k++) {
+      const int64_t q = j0 + k;
+      u0[k] = (k < jb && r0 < j0 && q - r0 <= KV) ? ab[q * LD + KV + r0 - q] : 0.0;
+      u1[k] = (k < jb && r1 < j0 && q - r1 <= KV) ? ab[q * LD + KV + r1 - q] : 0.0;
+    }
     __syncthreads();
     if (t < 64) {
       double yr = lane < jb ? win[(j0 + lane) % BW] : 0.0;
@@ -589,7 +612,21 @@ __global__ __launch_bounds__(BPT) void k_band_bwd(const double *ab, int64_t LD, 
       if (lane < jb) win[(j0 + lane) % BW] = yr;
     }
     __syncthreads();
-    for (int64_t r = rlo + t; r < j0; r += BPT) {
+    if (r0 < j0) {
+      double acc = win[r0 % BW];
+#pragma unroll
+      for (int k = 0; k < BNB; k++)
+        if (k < jb) acc -= u0[k] * win[(j0 + k) % BW];
+      win[r0 % BW] = acc;
+    }
+    if (r1 < j0) {
+      double acc = win[r1 % BW];
+#pragma unroll
+      for (int k = 0; k < BNB; k++)
+        if (k < jb) acc -= u1[k] * win[(j0 + k) % BW];
+      win[r1 % BW] = acc;
+    }
+    for (int64_t r = rlo + t + 2 * BPT; r < j0; r += BPT) {
       double uv[BNB];
 #pragma unroll
       for (int k = 0; k < BNB; k++) {
