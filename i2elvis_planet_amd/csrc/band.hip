@@ -1,6 +1,6 @@
 // band.hip -- the Stokes rows assembled into band storage and factorised by
 // LU with partial pivoting inside the band: the FGMRES preconditioner of
-// stokes_direct = 1.  The reference solves every Stokes system directly
+// stokes_direct = 1 and 2.  The reference solves every Stokes system directly
 // (gausmat4 -> PARDISO, gausmart.c:174-404); this is the direct solve a
 // banded matrix allows on one GPU, for the systems the multigrid cannot
 // precondition (melt at 1e2 Pa s against the 1e23 Pa s clip).
@@ -18,12 +18,15 @@
 //      fill of row interchanges) plus BNB rows for a panel's multipliers:
 //      A(i, c) at ab[c LD + KV + i - c], KV = kl + ku, LD = KV + kl + BNB;
 //   4. per panel of BNB columns: one workgroup factors the panel with
-//      partial pivoting (rows interchanged across the whole panel), applies
-//      the interchanges and the unit-lower triangle to the panel rows of the
-//      trailing columns; then the trailing band window (kl rows x KV
-//      columns) takes the rank-BNB update;
+//      partial pivoting (the panel's multipliers end in the permuted order
+//      of its interchanges, as in LAPACK's dgetrf; band storage keeps them
+//      in BNB extra rows), a second kernel applies the interchanges and the
+//      unit-lower triangle to the panel rows of the trailing columns, and
+//      the trailing band window (kl rows x KV columns) takes the rank-BNB
+//      update, the next panel's columns first when the window is wide;
 //   5. the solves: one workgroup each, the active part of the vector in an
-//      LDS window, panel by panel.
+//      LDS window, panel by panel, with the row moves and pivot reciprocals
+//      the factorisation recorded.
 #include <climits>
 
 #include "ctx.hpp"
