@@ -100,6 +100,38 @@ def test_direct_cold_matches_oracle(orc, name, rtol):
         assert e < 1e-8, f
 
 
+@pytest.mark.parametrize("kind", ["general", "continuity", "both"])
+def test_direct_general_bc_rows(orc, kind):
+    """Velocity BC rows beyond free slip at 51^2 (several neighbours, one of
+    the other component or two nodes in, and continuity-type rows of an open
+    wall, movemart.c:863-879) assembled into the band from their tables:
+    one or two iterations, the oracle's direct solve matched to 1e-8."""
+    from harness.bc import continuity_rows, general_rows
+    say("case direct bc", kind)
+    p, hs, _ = configs.make("c1")
+    if kind in ("general", "both"):
+        general_rows(hs)
+    if kind in ("continuity", "both"):
+        continuity_rows(hs)
+    o = orc.Oracle(p, hs)
+    o.ronurecalc()
+    hs.timestep = p.maxtmstep
+    o.maxvelstep()
+    ctx = Context(p, hs.copy(), solver=direct_opts(1e-12))
+    ctx.set_scalars(hs.timestep, hs.timesum)
+    ctx.viterate()
+    st = ctx.stats()
+    got = ctx.download(hs.copy())
+    ctx.close()
+    o.viterate()
+    say("%s: iterations %d relres %.2e" % (kind, st.stokes_iters, st.stokes_relres))
+    assert st.stokes_relres <= 1e-12 and st.stokes_iters <= 3
+    for f in ("vx", "vy", "pr"):
+        e = rel(getattr(got, f), getattr(hs, f))
+        say(f, e)
+        assert e < 1e-8, f
+
+
 RTOL_MOLTEN = 3e-11
 
 
