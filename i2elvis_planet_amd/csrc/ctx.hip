@@ -657,6 +657,13 @@ int i2e_set_solver(i2e_ctx *c, const i2e_solver_opts *o) {
     ST_OK(c->fh.init(c->N, o->restart));
     ST_OK(c->fgv.init(c->N, o->restart));
   }
+  if (!o->stokes_direct && c->band) {
+    // the band storage (27 GB at 501^2) goes with the direct solve
+    HIP_OK(hipStreamSynchronize(c->s));
+    if (c->s2) HIP_OK(hipStreamSynchronize(c->s2));
+    band_release(c->band);
+    c->band = nullptr;
+  }
   if (!o->autotune) {  // back to the default tile shapes
     c->v32.shape.clear();
     c->v64.shape.clear();

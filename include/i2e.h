@@ -238,7 +238,8 @@ typedef struct i2e_solver_opts {
                               and stokes_relres refer to the equilibrated
                               residual ||R (b - A x)|| / ||R b|| (where the
                               melt makes ||b - A x|| / ||b|| <= 1e-8
-                              unreachable in FP64); 0 (default): multigrid */
+                              unreachable in FP64); 0 (default): multigrid;
+                              setting 0 frees the band storage            */
   int32_t pad_opts;
 } i2e_solver_opts;
 

@@ -233,6 +233,32 @@ def test_direct_equilibrated_molten_steps(orc):
     ctx.close()
 
 
+def test_direct_off_frees_the_band():
+    """Switching stokes_direct off frees the band storage; the multigrid
+    solve then runs on the same context, and switching back refactors."""
+    from i2elvis_planet_amd import I2EError
+    say("case direct off")
+    p, hs, _ = configs.make("c1")
+    ctx = Context(p, hs, solver=direct_opts(1e-10))
+    ctx.ronurecalc()
+    ctx.viterate()
+    assert ctx.band_info()[0] > 0
+    o = direct_opts(1e-10)
+    o.stokes_direct = 0
+    o.maxit = 600
+    o.restart = 200
+    ctx.set_solver(o)
+    with pytest.raises(I2EError, match="band"):
+        ctx.band_info()
+    ctx.viterate()
+    assert ctx.stats().stokes_relres <= 1e-10
+    ctx.set_solver(direct_opts(1e-10))
+    ctx.viterate()
+    st = ctx.stats()
+    assert st.stokes_iters <= 3 and st.stokes_relres <= 1e-10 and ctx.band_info()[0] > 0
+    ctx.close()
+
+
 def test_direct_refuses_option_out_of_range():
     from i2elvis_planet_amd import I2EError
     p, hs, _ = configs.make("c1")
